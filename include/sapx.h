@@ -1,0 +1,242 @@
+/*
+ * sapx — MI355X (gfx950) SAP compliant-contact step, C ABI.
+ *
+ * Plain pointers and sizes only.  Model descriptions are HOST memory (the
+ * library keeps its own device copy); state / control / contact views are
+ * caller-allocated DEVICE memory.  Every entry point returns an int status
+ * (SAPX_OK = 0, negative = error class) and never throws; the message of the
+ * last error on the calling thread is available from sapx_last_error().
+ * All work is enqueued asynchronously on the stream passed in (a hipStream_t,
+ * NULL = default stream); nothing synchronises unless stated.
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef SAPX_H
+#define SAPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SAPX_OK 0
+#define SAPX_ERR_INVALID (-1)     /* bad argument / inconsistent model (ValueError)      */
+#define SAPX_ERR_UNSUPPORTED (-2) /* feature outside the native path (NotImplementedError) */
+#define SAPX_ERR_HIP (-3)         /* HIP runtime failure (RuntimeError)                   */
+#define SAPX_ERR_CAPACITY (-4)    /* sizes exceed the compiled kernel limits              */
+
+#define SAPX_LS_ARMIJO_DECAY 0
+#define SAPX_LS_EXACT_ROOT 1
+
+#define SAPX_WEIGHT_BODY_INERTIA 0
+#define SAPX_WEIGHT_DIAG_DELASSUS 1
+
+#define SAPX_POINT_WITNESS 0
+#define SAPX_POINT_MIDPOINT 1
+
+#define SAPX_INTEGRATE_MIDPOINT 0
+#define SAPX_INTEGRATE_SAP_EULER 1
+
+typedef struct SapxModel SapxModel;
+typedef struct SapxSolver SapxSolver;
+typedef struct SapxPipeline SapxPipeline;
+
+/*
+ * Model arrays, field-for-field the reference's SapModel
+ * (sim/sap_runtime.py:47-137).  Host pointers; transforms are 7 floats
+ * [p, q(xyzw)], inertia 9 floats row-major.  Worlds must be equally shaped
+ * and world-major (joints, dofs, coords, bodies, shapes of world w are the
+ * w-th contiguous block); global shapes (world -1, e.g. the ground plane)
+ * follow all per-world shapes.
+ */
+typedef struct SapxModelDesc {
+    int32_t world_count;
+    int32_t joint_count;
+    int32_t dof_count;
+    int32_t coord_count;
+    int32_t body_count;
+    int32_t shape_count;
+    int32_t articulation_count;
+    int32_t pair_count;
+
+    const int32_t* joint_type;
+    const int32_t* joint_parent;
+    const int32_t* joint_child;
+    const int32_t* joint_q_start;   /* joint_count + 1 */
+    const int32_t* joint_qd_start;  /* joint_count + 1 */
+    const int32_t* joint_dof_dim;   /* joint_count x 2 */
+    const int32_t* joint_ancestor;
+    const int32_t* articulation_start; /* articulation_count + 1 */
+    const float* joint_axis;        /* dof_count x 3 */
+    const float* joint_X_p;         /* joint_count x 7 */
+    const float* joint_X_c;         /* joint_count x 7 */
+    const float* joint_armature;    /* dof_count */
+
+    const float* body_mass;
+    const float* body_inertia;      /* body_count x 9 */
+    const float* body_com;          /* body_count x 3 */
+    const float* gravity;           /* world_count x 3 */
+
+    const int32_t* joint_target_mode;
+    const float* joint_target_ke;
+    const float* joint_target_kd;
+    const float* joint_effort_limit;
+    const float* joint_limit_lower;
+    const float* joint_limit_upper;
+    const float* joint_limit_ke;
+    const float* joint_limit_kd;
+
+    const int32_t* shape_body;
+    const int32_t* shape_type;
+    const int32_t* shape_world;
+    const float* shape_transform;   /* shape_count x 7 */
+    const float* shape_scale;       /* shape_count x 3 */
+    const float* shape_margin;
+    const float* shape_gap;
+    const float* shape_collision_radius;
+    const float* shape_material_mu;
+    const float* shape_material_ke;
+    const float* shape_material_tau;
+    const int32_t* shape_pairs;     /* pair_count x 2, explicit candidate pairs */
+} SapxModelDesc;
+
+/*
+ * Solver knobs: SolverSAP.__init__ keyword arguments after preset expansion
+ * (sim/solver_sap.py:749-968, presets 75-126).
+ */
+typedef struct SapxSolverOpts {
+    int32_t max_rigid_contact;      /* per-world contact capacity C */
+    double fallback_mu;
+    double fallback_stiffness;
+    double fallback_tau_d;          /* per-shape dissipation fallback */
+    double contact_beta;
+    double contact_sigma;
+    double diag_shift;
+    int32_t max_iterations;
+    double optimality_abs_tol;
+    double optimality_rel_tol;
+    double cost_abs_tol;
+    double cost_rel_tol;
+    int32_t line_search;            /* SAPX_LS_* */
+    int32_t line_search_max_iterations;
+    double armijo_c;
+    double rho;
+    double line_search_relative_slop;
+    int32_t contact_weight_mode;    /* SAPX_WEIGHT_* */
+    int32_t contact_point_mode;     /* SAPX_POINT_* */
+    int32_t position_integration;   /* SAPX_INTEGRATE_* */
+} SapxSolverOpts;
+
+/* Public-order state (SapState, sim/sap_runtime.py:177-205), device memory. */
+typedef struct SapxStateView {
+    float* joint_q;     /* coord_count */
+    float* joint_qd;    /* dof_count, public order */
+    float* body_q;      /* body_count x 7 */
+    float* body_qd;     /* body_count x 6, may be NULL */
+    float* body_f;      /* body_count x 6 public [f, tau] at COM, may be NULL */
+} SapxStateView;
+
+/* Public-order control (SapControl, sim/sap_runtime.py:208-227), device memory; NULL = zeros. */
+typedef struct SapxControlView {
+    const float* joint_f;
+    const float* joint_target_pos;
+    const float* joint_target_vel;
+    const float* joint_act;
+} SapxControlView;
+
+/*
+ * Flat rigid contacts (SapContacts / collision Contacts,
+ * sim/sap_runtime.py:230-264, sim/collision/contacts.py:14-114).
+ * count: 1 int (live count).  world_start: world_count+1 ints, CSR of a
+ * world-grouped list (written by sapx_collide); when NULL the step groups
+ * the flat list itself (stable, flat order within a world).
+ */
+typedef struct SapxContactsView {
+    int32_t* count;
+    int32_t* world_start;
+    int32_t* shape0;
+    int32_t* shape1;
+    float* point0;      /* x 3, body frame of shape0's body */
+    float* point1;      /* x 3, body frame of shape1's body */
+    float* normal;      /* x 3, world, from shape0 to shape1 */
+    float* margin0;
+    float* margin1;
+    int32_t capacity;
+} SapxContactsView;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int32_t sapx_version(void);
+
+/* Message of the last error raised on this thread (never NULL). */
+const char* sapx_last_error(void);
+
+/*
+ * Build the device-resident model: uploads arrays, validates equal world
+ * shapes, builds the per-world topology template and level schedule.
+ * Replaces model upload + host schedules in SapFreeMotion.__init__
+ * (sim/free_motion.py:1427-1626), SapContactJacobian.__init__
+ * (sim/contact_jacobian.py:1233-1540) and SapContactSolve.__init__
+ * (sim/contact_solve.py:3291-3651).
+ */
+int32_t sapx_model_create(const SapxModelDesc* desc, SapxModel** out);
+void sapx_model_destroy(SapxModel* model);
+
+/* Solver instance (SolverSAP.__init__, sim/solver_sap.py:749-968). */
+int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, SapxSolver** out);
+void sapx_solver_destroy(SapxSolver* solver);
+
+/* Clear the warm-start guess (SolverSAP.reset_runtime_state, sim/solver_sap.py:1004-1017). */
+int32_t sapx_solver_reset(SapxSolver* solver, void* stream);
+
+/*
+ * One SAP timestep for every world: SolverSAP.step(state_in, state_out,
+ * control, contacts, dt) (sim/solver_sap.py:1388-1494) — boundary conversion,
+ * free motion, contact Jacobian, Newton solve with line search, integration
+ * and public forward kinematics, fused per world.  dt_world (device, f64,
+ * world_count) overrides dt when non-NULL (sim/solver_sap.py:975-992).
+ * state_in and state_out may alias.
+ */
+int32_t sapx_step(SapxSolver* solver, const SapxStateView* state_in, const SapxStateView* state_out,
+                  const SapxControlView* control, const SapxContactsView* contacts, double dt,
+                  const double* dt_world, void* stream);
+
+/*
+ * Per-world statistics of the last step, copied to HOST arrays of
+ * world_count entries (SapContactSolveResult.newton_iterations_env etc.,
+ * sim/contact_solve.py:188-232).  Any pointer may be NULL.  Synchronises the
+ * stream.  flags bit0 = converged, bit1 = optimality condition reached,
+ * bit2 = cost criterion reached, bit3 = iteration cap hit, bits 8..15 = last
+ * line-search status (two's complement, 0 = ok).
+ */
+int32_t sapx_get_stats(SapxSolver* solver, int32_t* newton_iterations, int32_t* line_search_iterations,
+                       int32_t* contact_count, int32_t* flags, void* stream);
+
+/*
+ * Copy an internal per-world buffer of the last step into caller DEVICE
+ * memory (parity/debug; SapContactJacobian.capture_local_snapshots,
+ * sim/contact_jacobian.py:1611-1626).  which: 0 = v (f64 dof_count, SAP
+ * order), 1 = v_star (f64 dof_count), 2 = v0 (f64 dof_count), 3 = contact
+ * impulses gamma (f64 world_count x C x 3), 4 = contact phi0 (f64 world_count
+ * x C), 5 = contact weight (f64 world_count x C), 6 = dynamics matrix A
+ * diagonal (f64 dof_count).  bytes must equal the buffer size.
+ */
+int32_t sapx_solver_read(SapxSolver* solver, int32_t which, void* dst, int64_t bytes, void* stream);
+
+/* Collision pipeline (SapCollisionPipeline.__init__, sim/collision/pipeline.py:322-508). */
+int32_t sapx_pipeline_create(const SapxModel* model, int32_t max_contacts_per_world, SapxPipeline** out);
+void sapx_pipeline_destroy(SapxPipeline* pipeline);
+
+/*
+ * Rigid contact generation: SapCollisionPipeline.collide(state, contacts)
+ * (sim/collision/pipeline.py:546-709) — shape AABBs, explicit-pair AABB test
+ * and primitive narrowphase, written world-grouped in candidate-pair order
+ * (deterministic).  body_q: device, body_count x 7.
+ */
+int32_t sapx_collide(SapxPipeline* pipeline, const float* body_q, const SapxContactsView* contacts, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SAPX_H */

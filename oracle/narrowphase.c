@@ -1,0 +1,416 @@
+/*
+ * CPU oracle for rigid contact generation — TEST INFRASTRUCTURE ONLY.
+ *
+ * Float32 restatement (compiled with -ffp-contract=off, IEEE sqrt/div) of the
+ * reference collision path for primitive shapes:
+ *   shape AABBs             sim/collision/pipeline.py:121-179
+ *   explicit-pair AABB test sim/collision/broad_phase_common.py:18-35,
+ *                           sim/collision/broad_phase_nxn.py:21-59
+ *   primitive narrowphase   sim/collision/narrow_phase.py:141-577
+ *   primitives              sim/collision/primitive.py:37-90 (segment helpers),
+ *                           89-129 (plane-sphere, sphere-sphere), 132-166
+ *                           (sphere-capsule), 169-264 (capsule-capsule),
+ *                           267-319 (plane-capsule), 354-428 (plane-box),
+ *                           1150-1205 (sphere-box)
+ *   gap check + writer      sim/collision/contact_data.py:36-56,
+ *                           sim/collision/pipeline.py:51-118
+ * Contacts are appended per world in candidate-pair order; a pair whose whole
+ * batch does not fit into the world's capacity is skipped, like the
+ * reference's overflow rule (narrow_phase.py:534-539).
+ *
+ * Only tests/, the smoke check and the CPU-baseline leg of bench.py may load
+ * this library.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#define GEO_PLANE 1
+#define GEO_SPHERE 3
+#define GEO_CAPSULE 4
+#define GEO_BOX 7
+#define MAXVAL 1e10f
+#define MINVAL 1e-15f
+
+typedef struct { float x, y, z; } v3;
+typedef struct { float x, y, z, w; } q4;
+
+static v3 mk(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static v3 add(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+static v3 sub(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+static v3 scl(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+static float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static v3 cross(v3 a, v3 b) {
+    return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static float length(v3 a) { return sqrtf(dot(a, a)); }
+static v3 normalize(v3 a) {
+    float l = length(a);
+    if (l > 0.0f) return mk(a.x / l, a.y / l, a.z / l);
+    return mk(0.0f, 0.0f, 0.0f);
+}
+static float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+static v3 qrot(q4 q, v3 v) {
+    v3 qv = mk(q.x, q.y, q.z);
+    v3 t = scl(cross(qv, v), 2.0f);
+    return add(add(v, scl(t, q.w)), cross(qv, t));
+}
+static q4 qmul(q4 a, q4 b) {
+    q4 r;
+    r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+    r.y = a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x;
+    r.z = a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w;
+    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    return r;
+}
+typedef struct { v3 p; q4 q; } tf;
+static tf load_tf(const float* a) {
+    tf t;
+    t.p = mk(a[0], a[1], a[2]);
+    t.q.x = a[3]; t.q.y = a[4]; t.q.z = a[5]; t.q.w = a[6];
+    return t;
+}
+static tf tf_mul(tf a, tf b) {
+    tf r;
+    r.p = add(a.p, qrot(a.q, b.p));
+    r.q = qmul(a.q, b.q);
+    return r;
+}
+static v3 tf_inv_point(tf a, v3 x) {
+    q4 qi = a.q; qi.x = -qi.x; qi.y = -qi.y; qi.z = -qi.z;
+    return qrot(qi, sub(x, a.p));
+}
+static v3 tf_inv_vector(tf a, v3 x) {
+    q4 qi = a.q; qi.x = -qi.x; qi.y = -qi.y; qi.z = -qi.z;
+    return qrot(qi, x);
+}
+/* R columns = rotated basis vectors; matvec R @ v and R^T @ v */
+static void rot_cols(q4 q, v3 c[3]) {
+    c[0] = qrot(q, mk(1.0f, 0.0f, 0.0f));
+    c[1] = qrot(q, mk(0.0f, 1.0f, 0.0f));
+    c[2] = qrot(q, mk(0.0f, 0.0f, 1.0f));
+}
+static v3 rmul(const v3 c[3], v3 v) { return add(add(scl(c[0], v.x), scl(c[1], v.y)), scl(c[2], v.z)); }
+static v3 rtmul(const v3 c[3], v3 v) { return mk(dot(c[0], v), dot(c[1], v), dot(c[2], v)); }
+
+/* ------------------------------------------------------------------ AABB */
+static void shape_aabb(int type, v3 pos, q4 q, v3 scale, float radius, float eff_gap, v3* lo, v3* hi) {
+    v3 h;
+    if (type == GEO_PLANE && scale.x == 0.0f && scale.y == 0.0f) {
+        h = mk(radius, radius, radius);
+    } else if (type == GEO_SPHERE) {
+        h = mk(scale.x, scale.x, scale.x);
+    } else if (type == GEO_CAPSULE) {
+        v3 a = qrot(q, mk(0.0f, 0.0f, 1.0f));
+        h = mk(fabsf(a.x) * scale.y + scale.x, fabsf(a.y) * scale.y + scale.x, fabsf(a.z) * scale.y + scale.x);
+    } else if (type == GEO_BOX) {
+        v3 c[3];
+        rot_cols(q, c);
+        h = mk(fabsf(c[0].x) * scale.x + fabsf(c[1].x) * scale.y + fabsf(c[2].x) * scale.z,
+               fabsf(c[0].y) * scale.x + fabsf(c[1].y) * scale.y + fabsf(c[2].y) * scale.z,
+               fabsf(c[0].z) * scale.x + fabsf(c[1].z) * scale.y + fabsf(c[2].z) * scale.z);
+    } else {
+        h = mk(radius, radius, radius);
+    }
+    *lo = sub(sub(pos, h), mk(eff_gap, eff_gap, eff_gap));
+    *hi = add(add(pos, h), mk(eff_gap, eff_gap, eff_gap));
+}
+
+static int aabb_overlap(v3 l1, v3 u1, v3 l2, v3 u2) {
+    return l1.x <= u2.x && u1.x >= l2.x && l1.y <= u2.y && u1.y >= l2.y && l1.z <= u2.z && u1.z >= l2.z;
+}
+
+/* -------------------------------------------------------------- primitives */
+static v3 closest_segment_point(v3 a, v3 b, v3 pt) {
+    v3 ab = sub(b, a);
+    float t = dot(sub(pt, a), ab) / (dot(ab, ab) + 1e-6f);
+    return add(a, scl(ab, clampf(t, 0.0f, 1.0f)));
+}
+
+static void plane_sphere(v3 n, v3 ppos, v3 spos, float r, float* dist, v3* pos) {
+    float d = dot(sub(spos, ppos), n) - r;
+    *dist = d;
+    *pos = sub(spos, scl(n, r + 0.5f * d));
+}
+
+static void sphere_sphere(v3 p1, float r1, v3 p2, float r2, float* dist, v3* pos, v3* n) {
+    v3 dir = sub(p2, p1);
+    float d = length(dir);
+    v3 nn;
+    if (d == 0.0f) nn = mk(1.0f, 0.0f, 0.0f);
+    else nn = mk(dir.x / d, dir.y / d, dir.z / d);
+    d = d - (r1 + r2);
+    *pos = add(p1, scl(nn, r1 + 0.5f * d));
+    *dist = d;
+    *n = nn;
+}
+
+static void sphere_capsule(v3 sp, float sr, v3 cp, v3 axis, float cr, float hh, float* dist, v3* pos, v3* n) {
+    v3 seg = scl(axis, hh);
+    v3 pt = closest_segment_point(sub(cp, seg), add(cp, seg), sp);
+    sphere_sphere(sp, sr, pt, cr, dist, pos, n);
+}
+
+static void capsule_capsule(v3 p1, v3 a1, float r1, float h1, v3 p2, v3 a2, float r2, float h2,
+                            float dist[2], v3 pos[2], v3* normal) {
+    dist[0] = MAXVAL; dist[1] = MAXVAL;
+    v3 axis1 = scl(a1, h1);
+    v3 axis2 = scl(a2, h2);
+    v3 dif = sub(p1, p2);
+    float ma = dot(axis1, axis1);
+    float mb = -dot(axis1, axis2);
+    float mc = dot(axis2, axis2);
+    float u = -dot(axis1, dif);
+    float v = dot(axis2, dif);
+    float det = ma * mc - mb * mb;
+    if (fabsf(det) >= MINVAL) {
+        float inv = 1.0f / det;
+        float x1 = (mc * u - mb * v) * inv;
+        float x2 = (ma * v - mb * u) * inv;
+        if (x1 > 1.0f) { x1 = 1.0f; x2 = (v - mb) / mc; }
+        else if (x1 < -1.0f) { x1 = -1.0f; x2 = (v + mb) / mc; }
+        if (x2 > 1.0f) { x2 = 1.0f; x1 = clampf((u - mb) / ma, -1.0f, 1.0f); }
+        else if (x2 < -1.0f) { x2 = -1.0f; x1 = clampf((u + mb) / ma, -1.0f, 1.0f); }
+        v3 v1 = add(p1, scl(axis1, x1));
+        v3 v2 = add(p2, scl(axis2, x2));
+        sphere_sphere(v1, r1, v2, r2, &dist[0], &pos[0], normal);
+    } else {
+        v3 v1 = add(p1, axis1);
+        float x2 = clampf((v - mb) / mc, -1.0f, 1.0f);
+        v3 v2 = add(p2, scl(axis2, x2));
+        sphere_sphere(v1, r1, v2, r2, &dist[0], &pos[0], normal);
+        v3 n2;
+        v1 = sub(p1, axis1);
+        x2 = clampf((v + mb) / mc, -1.0f, 1.0f);
+        v2 = add(p2, scl(axis2, x2));
+        sphere_sphere(v1, r1, v2, r2, &dist[1], &pos[1], &n2);
+    }
+}
+
+static void plane_capsule(v3 n, v3 ppos, v3 cp, v3 axis, float r, float hh, float dist[2], v3 pos[2]) {
+    v3 seg = scl(axis, hh);
+    plane_sphere(n, ppos, add(cp, seg), r, &dist[0], &pos[0]);
+    plane_sphere(n, ppos, sub(cp, seg), r, &dist[1], &pos[1]);
+}
+
+static void plane_box(v3 n, v3 ppos, v3 bpos, q4 bq, v3 size, float margin, float dist[4], v3 pos[4]) {
+    v3 c[3];
+    rot_cols(bq, c);
+    float center_dist = dot(sub(bpos, ppos), n);
+    for (int i = 0; i < 4; ++i) { dist[i] = MAXVAL; pos[i] = mk(0.0f, 0.0f, 0.0f); }
+    int ncontact = 0, worst = 0;
+    for (int i = 0; i < 8; ++i) {
+        v3 corner = mk((i & 1) ? size.x : -size.x, (i & 2) ? size.y : -size.y, (i & 4) ? size.z : -size.z);
+        corner = rmul(c, corner);
+        float ldist = dot(n, corner);
+        float cdist = center_dist + ldist;
+        if (cdist > margin) continue;
+        v3 cpos = sub(add(corner, bpos), scl(n, 0.5f * cdist));
+        if (ncontact < 4) {
+            dist[ncontact] = cdist;
+            pos[ncontact] = cpos;
+            if (ncontact == 0 || cdist > dist[worst]) worst = ncontact;
+            ncontact++;
+        } else if (cdist < dist[worst]) {
+            dist[worst] = cdist;
+            pos[worst] = cpos;
+            worst = 0;
+            if (dist[1] > dist[worst]) worst = 1;
+            if (dist[2] > dist[worst]) worst = 2;
+            if (dist[3] > dist[worst]) worst = 3;
+        }
+    }
+}
+
+static float vget(v3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+static void vset(v3* v, int i, float s) { if (i == 0) v->x = s; else if (i == 1) v->y = s; else v->z = s; }
+
+static void sphere_box(v3 sp, float r, v3 bpos, q4 bq, v3 size, float* dist, v3* pos, v3* normal) {
+    v3 c[3];
+    rot_cols(bq, c);
+    v3 center = rtmul(c, sub(sp, bpos));
+    v3 clamped = mk(fmaxf(-size.x, fminf(size.x, center.x)), fmaxf(-size.y, fminf(size.y, center.y)),
+                    fmaxf(-size.z, fminf(size.z, center.z)));
+    v3 diff = sub(clamped, center);
+    float d = length(diff);
+    v3 cdir = diff;
+    if (d != 0.0f) cdir = mk(diff.x / d, diff.y / d, diff.z / d);
+    v3 lp;
+    if (d <= 1e-6f) {
+        float closest = 2.0f * (size.x + size.y + size.z);
+        int k = 0;
+        for (int i = 0; i < 6; ++i) {
+            float sgn = (i % 2) ? 1.0f : -1.0f;
+            float fd = fabsf(sgn * vget(size, i / 2) - vget(center, i / 2));
+            if (closest > fd) { closest = fd; k = i; }
+        }
+        v3 nearest = mk(0.0f, 0.0f, 0.0f);
+        vset(&nearest, k / 2, (k % 2) ? -1.0f : 1.0f);
+        lp = add(center, scl(nearest, (r - closest) / 2.0f));
+        *normal = rmul(c, nearest);
+        *dist = -closest - r;
+    } else {
+        v3 deepest = add(center, scl(cdir, r));
+        lp = scl(add(clamped, deepest), 0.5f);
+        *normal = rmul(c, cdir);
+        *dist = d - r;
+    }
+    *pos = add(bpos, rmul(c, lp));
+}
+
+/* ------------------------------------------------------------ gap check */
+static void contact_points(v3 center, v3 n_in, float d, float ra, float rb, v3* xa, v3* xb, v3* n) {
+    *n = normalize(n_in);
+    *xa = sub(center, scl(*n, 0.5f * d + ra));
+    *xb = add(center, scl(*n, 0.5f * d + rb));
+}
+static int passes_gap(v3 center, v3 n_in, float d, float ra, float rb, float ma, float mb, float gap_sum) {
+    v3 xa, xb, n;
+    contact_points(center, n_in, d, ra, rb, &xa, &xb, &n);
+    float total = ra + rb + ma + mb;
+    float dd = dot(sub(xb, xa), n) - total;
+    return dd <= gap_sum;
+}
+
+/*
+ * Pair-level narrowphase.  Returns the number of candidate contacts (0..4)
+ * and fills dist/pos/normal/radii/shape order.  Unsupported pair types set
+ * *unsupported = 1.
+ */
+static int narrow_pair(int sa, int sb, const int* shape_type, const float* shape_scale, const tf* X,
+                       float dist[4], v3 pos[4], v3* normal, float* ra_out, float* rb_out,
+                       int* shape_a_out, int* shape_b_out, float gap_sum, int* unsupported) {
+    int ta = shape_type[sa], tb = shape_type[sb];
+    if (ta > tb) { int t = sa; sa = sb; sb = t; t = ta; ta = tb; tb = t; }
+    *shape_a_out = sa; *shape_b_out = sb;
+    v3 scale_a = mk(shape_scale[3 * sa], shape_scale[3 * sa + 1], shape_scale[3 * sa + 2]);
+    v3 scale_b = mk(shape_scale[3 * sb], shape_scale[3 * sb + 1], shape_scale[3 * sb + 2]);
+    tf Xa = X[sa], Xb = X[sb];
+    float ra = 0.0f, rb = 0.0f;
+    if (ta == GEO_SPHERE || ta == GEO_CAPSULE) ra = scale_a.x;
+    if (tb == GEO_SPHERE || tb == GEO_CAPSULE) rb = scale_b.x;
+    for (int i = 0; i < 4; ++i) { dist[i] = MAXVAL; pos[i] = mk(0.0f, 0.0f, 0.0f); }
+    *normal = mk(0.0f, 0.0f, 0.0f);
+    *unsupported = 0;
+    v3 ez = mk(0.0f, 0.0f, 1.0f);
+    if (ta == GEO_PLANE && tb == GEO_SPHERE) {
+        v3 n = qrot(Xa.q, ez);
+        plane_sphere(n, Xa.p, Xb.p, scale_b.x, &dist[0], &pos[0]);
+        *normal = n;
+    } else if (ta == GEO_PLANE && tb == GEO_BOX) {
+        v3 n = qrot(Xa.q, ez);
+        plane_box(n, Xa.p, Xb.p, Xb.q, scale_b, gap_sum, dist, pos);
+        *normal = n;
+    } else if (ta == GEO_SPHERE && tb == GEO_SPHERE) {
+        sphere_sphere(Xa.p, scale_a.x, Xb.p, scale_b.x, &dist[0], &pos[0], normal);
+    } else if (ta == GEO_PLANE && tb == GEO_CAPSULE) {
+        v3 n = qrot(Xa.q, ez);
+        v3 ax = qrot(Xb.q, ez);
+        plane_capsule(n, Xa.p, Xb.p, ax, scale_b.x, scale_b.y, dist, pos);
+        *normal = n;
+    } else if (ta == GEO_SPHERE && tb == GEO_CAPSULE) {
+        v3 ax = qrot(Xb.q, ez);
+        sphere_capsule(Xa.p, scale_a.x, Xb.p, ax, scale_b.x, scale_b.y, &dist[0], &pos[0], normal);
+    } else if (ta == GEO_CAPSULE && tb == GEO_CAPSULE) {
+        v3 a1 = qrot(Xa.q, ez), a2 = qrot(Xb.q, ez);
+        capsule_capsule(Xa.p, a1, scale_a.x, scale_a.y, Xb.p, a2, scale_b.x, scale_b.y, dist, pos, normal);
+    } else if (ta == GEO_SPHERE && tb == GEO_BOX) {
+        sphere_box(Xa.p, scale_a.x, Xb.p, Xb.q, scale_b, &dist[0], &pos[0], normal);
+    } else {
+        *unsupported = 1;
+        return 0;
+    }
+    *ra_out = ra; *rb_out = rb;
+    int n = 0;
+    for (int i = 0; i < 4; ++i) n += dist[i] < MAXVAL;
+    return n;
+}
+
+/*
+ * Full collide for all worlds.  pairs: world-major candidate pair list,
+ * pair_world[p] = world of pair p, pairs grouped by world in ascending order.
+ * Outputs are written compact (flat) in world-major order; world_start has
+ * world_count+1 entries.  Returns the number of unsupported pair encounters.
+ */
+int sapx_oracle_collide(int world_count, int shape_count, const float* body_q, const float* shape_transform,
+                        const int* shape_body, const int* shape_type, const float* shape_scale,
+                        const float* shape_margin, const float* shape_gap, const float* shape_radius,
+                        const int* pairs, const int* pair_world, int pair_count, int max_contacts_per_world,
+                        int out_capacity, int* out_count, int* world_start, int* out_shape0, int* out_shape1,
+                        float* out_point0, float* out_point1, float* out_normal, float* out_margin0,
+                        float* out_margin1, int* out_pair, tf* scratch_tf, float* scratch_aabb) {
+    int unsupported_total = 0;
+    for (int s = 0; s < shape_count; ++s) {
+        tf xs = load_tf(shape_transform + 7 * s);
+        int b = shape_body[s];
+        if (b >= 0) xs = tf_mul(load_tf(body_q + 7 * b), xs);
+        scratch_tf[s] = xs;
+        v3 lo, hi;
+        v3 sc = mk(shape_scale[3 * s], shape_scale[3 * s + 1], shape_scale[3 * s + 2]);
+        shape_aabb(shape_type[s], xs.p, xs.q, sc, shape_radius[s], shape_margin[s] + shape_gap[s], &lo, &hi);
+        scratch_aabb[6 * s + 0] = lo.x; scratch_aabb[6 * s + 1] = lo.y; scratch_aabb[6 * s + 2] = lo.z;
+        scratch_aabb[6 * s + 3] = hi.x; scratch_aabb[6 * s + 4] = hi.y; scratch_aabb[6 * s + 5] = hi.z;
+    }
+    int total = 0;
+    int p = 0;
+    for (int w = 0; w < world_count; ++w) {
+        world_start[w] = total;
+        int wcount = 0, claimed = 0;
+        for (; p < pair_count && pair_world[p] == w; ++p) {
+            int s1 = pairs[2 * p], s2 = pairs[2 * p + 1];
+            if (s1 == s2 || s1 < 0 || s2 < 0) continue;
+            const float* A = scratch_aabb + 6 * s1;
+            const float* B = scratch_aabb + 6 * s2;
+            if (!aabb_overlap(mk(A[0], A[1], A[2]), mk(A[3], A[4], A[5]), mk(B[0], B[1], B[2]), mk(B[3], B[4], B[5])))
+                continue;
+            float dist[4];
+            v3 pos[4], normal;
+            float ra, rb;
+            int sa, sb, unsup;
+            float gap_sum = shape_gap[s1] + shape_gap[s2];
+            int nc = narrow_pair(s1, s2, shape_type, shape_scale, scratch_tf, dist, pos, &normal, &ra, &rb, &sa, &sb,
+                                 gap_sum, &unsup);
+            unsupported_total += unsup;
+            if (nc == 0) continue;
+            float ma = shape_margin[sa], mb = shape_margin[sb];
+            int valid[4], nv = 0;
+            for (int i = 0; i < 4; ++i) {
+                valid[i] = dist[i] < MAXVAL && passes_gap(pos[i], normal, dist[i], ra, rb, ma, mb, gap_sum);
+                nv += valid[i];
+            }
+            if (nv == 0) continue;
+            /* batch append with the reference's overflow rule: the claim always
+               advances, a batch is written only if it fits entirely */
+            int fits = claimed + nv <= max_contacts_per_world;
+            claimed += nv;
+            if (!fits) continue;
+            int ba = shape_body[sa], bb = shape_body[sb];
+            for (int i = 0; i < 4; ++i) {
+                if (!valid[i]) continue;
+                int idx = total + wcount;
+                if (idx < out_capacity) {
+                    v3 xa, xb, n;
+                    contact_points(pos[i], normal, dist[i], ra, rb, &xa, &xb, &n);
+                    v3 la = xa, lb = xb;
+                    if (ba >= 0) la = tf_inv_point(load_tf(body_q + 7 * ba), xa);
+                    if (bb >= 0) lb = tf_inv_point(load_tf(body_q + 7 * bb), xb);
+                    out_shape0[idx] = sa;
+                    out_shape1[idx] = sb;
+                    out_point0[3 * idx] = la.x; out_point0[3 * idx + 1] = la.y; out_point0[3 * idx + 2] = la.z;
+                    out_point1[3 * idx] = lb.x; out_point1[3 * idx + 1] = lb.y; out_point1[3 * idx + 2] = lb.z;
+                    out_normal[3 * idx] = n.x; out_normal[3 * idx + 1] = n.y; out_normal[3 * idx + 2] = n.z;
+                    out_margin0[idx] = ra + ma;
+                    out_margin1[idx] = rb + mb;
+                    out_pair[idx] = p;
+                }
+                wcount++;
+            }
+        }
+        total += wcount;
+    }
+    world_start[world_count] = total;
+    *out_count = total;
+    (void)tf_inv_vector;
+    return unsupported_total;
+}

@@ -1,0 +1,638 @@
+// C ABI of libsapx (declared in include/sapx.h): model upload + validation,
+// per-world topology template, solver / pipeline objects and launches.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/sapx.h"
+#include "sapx_device.h"
+
+#include "sapx_params.h"
+
+namespace sapx {
+hipError_t launch_step(const StepParams& P, hipStream_t stream);
+hipError_t launch_collide(const CollideParams& P, const CompactParams& Q, hipStream_t stream);
+hipError_t launch_group_contacts(const GroupParams& G, hipStream_t stream);
+}  // namespace sapx
+
+using namespace sapx;
+
+namespace {
+
+thread_local std::string g_last_error = "";
+
+int32_t fail(int32_t code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) return fail(SAPX_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+template <typename T>
+int32_t upload(std::vector<void*>& owned, const T* host, size_t count, const T** dev_out) {
+    if (count == 0) {
+        *dev_out = nullptr;
+        return SAPX_OK;
+    }
+    void* d = nullptr;
+    HIP_TRY(hipMalloc(&d, count * sizeof(T)));
+    owned.push_back(d);
+    HIP_TRY(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+    *dev_out = reinterpret_cast<const T*>(d);
+    return SAPX_OK;
+}
+
+size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+}  // namespace
+
+struct SapxModel {
+    DevModel dm;
+    std::vector<void*> owned;
+    int32_t world_count = 0, joint_count = 0, dof_count = 0, coord_count = 0, body_count = 0, shape_count = 0;
+    ~SapxModel() {
+        for (void* p : owned) (void)hipFree(p);
+    }
+};
+
+struct SapxSolver {
+    const SapxModel* model = nullptr;
+    SapxSolverOpts opts{};
+    double* v_guess = nullptr;
+    int use_guess = 0;
+    int* st = nullptr;  // 4 x W
+    double* dbg = nullptr;
+    size_t dbg_off[7] = {0};
+    size_t dbg_size[7] = {0};
+    int npad = 0, js = 0;
+    int lds[15] = {0};
+    int smem_bytes = 0;
+    // grouping buffers for ungrouped contact lists
+    int group_capacity = 0;
+    int* g_keys = nullptr;
+    int* g_keys_out = nullptr;
+    int* g_vals = nullptr;
+    int* g_vals_out = nullptr;
+    int* g_world_start = nullptr;
+    int* g_shape0 = nullptr;
+    int* g_shape1 = nullptr;
+    float* g_p0 = nullptr;
+    float* g_p1 = nullptr;
+    float* g_n = nullptr;
+    float* g_m0 = nullptr;
+    float* g_m1 = nullptr;
+    void* cub_tmp = nullptr;
+    size_t cub_tmp_bytes = 0;
+    ~SapxSolver() {
+        void* ptrs[] = {v_guess, st, dbg, g_keys, g_keys_out, g_vals, g_vals_out, g_world_start, g_shape0, g_shape1,
+                        g_p0, g_p1, g_n, g_m0, g_m1, cub_tmp};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+};
+
+struct SapxPipeline {
+    const SapxModel* model = nullptr;
+    int C = 0;
+    int* st_shape0 = nullptr;
+    int* st_shape1 = nullptr;
+    float* st_p0 = nullptr;
+    float* st_p1 = nullptr;
+    float* st_n = nullptr;
+    float* st_m0 = nullptr;
+    float* st_m1 = nullptr;
+    int* wcount = nullptr;
+    int* unsupported = nullptr;
+    ~SapxPipeline() {
+        void* ptrs[] = {st_shape0, st_shape1, st_p0, st_p1, st_n, st_m0, st_m1, wcount, unsupported};
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+};
+
+extern "C" {
+
+int32_t sapx_version(void) { return 100; }
+
+const char* sapx_last_error(void) { return g_last_error.c_str(); }
+
+int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
+    if (!d || !out) return fail(SAPX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const int W = d->world_count;
+    if (W <= 0) return fail(SAPX_ERR_INVALID, "world_count must be positive");
+    if (d->joint_count <= 0 || d->dof_count <= 0)
+        return fail(SAPX_ERR_INVALID, "SolverSAP requires a model with articulated joint DOFs.");
+    if (d->joint_count % W || d->dof_count % W || d->coord_count % W || d->body_count % W)
+        return fail(SAPX_ERR_INVALID, "joints/dofs/coords/bodies must divide evenly across worlds");
+    const int J = d->joint_count / W, N = d->dof_count / W, NQ = d->coord_count / W, NB = d->body_count / W;
+    if (J > kMaxTemplate || NB > kMaxTemplate)
+        return fail(SAPX_ERR_CAPACITY, "more than 64 joints or bodies per world");
+    if (N > kMaxDof) return fail(SAPX_ERR_CAPACITY, "more than 64 dofs per world");
+    // ---- topology template + equal-world validation
+    std::vector<int> t_jtype(J), t_jparent(J), t_jchild(J), t_jqs(J + 1), t_jds(J + 1);
+    for (int w = 0; w < W; ++w) {
+        for (int j = 0; j < J; ++j) {
+            const int g = w * J + j;
+            const int jt = d->joint_type[g];
+            const int par = d->joint_parent[g] >= 0 ? d->joint_parent[g] - w * NB : -1;
+            const int chi = d->joint_child[g] - w * NB;
+            const int qs = d->joint_q_start[g] - w * NQ;
+            const int ds = d->joint_qd_start[g] - w * N;
+            if (w == 0) {
+                t_jtype[j] = jt; t_jparent[j] = par; t_jchild[j] = chi; t_jqs[j] = qs; t_jds[j] = ds;
+                if (jt == kJointBall || jt == kJointD6)
+                    return fail(SAPX_ERR_UNSUPPORTED, "ball and D6 joints are not in the native step yet");
+                if (chi < 0 || chi >= NB) return fail(SAPX_ERR_INVALID, "joint child outside its world");
+                if (par >= NB) return fail(SAPX_ERR_INVALID, "joint parent outside its world");
+            } else if (jt != t_jtype[j] || par != t_jparent[j] || chi != t_jchild[j] || qs != t_jqs[j] || ds != t_jds[j]) {
+                return fail(SAPX_ERR_UNSUPPORTED, "worlds must share one topology (world " + std::to_string(w) + ")");
+            }
+        }
+    }
+    t_jqs[J] = NQ;
+    t_jds[J] = N;
+    // body -> joint, depth / levels, ancestor + subtree masks
+    std::vector<int> body_joint(NB, -1);
+    for (int j = 0; j < J; ++j) {
+        if (body_joint[t_jchild[j]] != -1) return fail(SAPX_ERR_INVALID, "body is the child of two joints");
+        body_joint[t_jchild[j]] = j;
+    }
+    std::vector<int> depth(J, 0);
+    for (int j = 0; j < J; ++j) {
+        const int p = t_jparent[j];
+        if (p >= 0) {
+            const int pj = body_joint[p];
+            if (pj < 0) return fail(SAPX_ERR_UNSUPPORTED, "parent body without a joint");
+            if (pj >= j) return fail(SAPX_ERR_INVALID, "joints must be ordered parents-first");
+            depth[j] = depth[pj] + 1;
+        }
+    }
+    int nlev = 0;
+    for (int j = 0; j < J; ++j) nlev = std::max(nlev, depth[j] + 1);
+    std::vector<int> level_start(nlev + 1, 0), level_joint;
+    for (int L = 0; L < nlev; ++L) {
+        level_start[L] = (int)level_joint.size();
+        for (int j = 0; j < J; ++j)
+            if (depth[j] == L) level_joint.push_back(j);
+    }
+    level_start[nlev] = (int)level_joint.size();
+    std::vector<unsigned long long> anc(NB, 0ull), sub(NB, 0ull);
+    for (int b = 0; b < NB; ++b) {
+        int j = body_joint[b];
+        while (j >= 0) {
+            anc[b] |= 1ull << j;
+            const int p = t_jparent[j];
+            j = p >= 0 ? body_joint[p] : -1;
+        }
+        // subtree: walk up from b marking b in every ancestor body's subtree
+        int cur = b;
+        while (cur >= 0) {
+            sub[cur] |= 1ull << b;
+            const int jj = body_joint[cur];
+            cur = jj >= 0 ? t_jparent[jj] : -1;
+        }
+    }
+    // dof maps (contact_solve.py:3681-3720) and articulation dof ranges (3722-3760)
+    std::vector<int> dof_joint(N), dof_coord(N, -1), dof_target(N), dof_limit(N, 0);
+    for (int j = 0; j < J; ++j) {
+        const int nd = t_jds[j + 1] - t_jds[j];
+        for (int a = 0; a < nd; ++a) {
+            const int dd = t_jds[j] + a;
+            dof_joint[dd] = j;
+            dof_target[dd] = dd;
+        }
+        const int jt = t_jtype[j];
+        if (jt == kJointFree || jt == kJointDistance) {
+            for (int a = 0; a < 3; ++a) {
+                dof_target[t_jds[j] + a] = t_jds[j] + a + 3;
+                dof_target[t_jds[j] + a + 3] = t_jds[j] + a;
+            }
+        }
+        if (jt == kJointPrismatic || jt == kJointRevolute) {
+            for (int a = 0; a < nd; ++a) {
+                dof_coord[t_jds[j] + a] = t_jqs[j] + a;
+                if (nd == 1) dof_limit[t_jds[j] + a] = 1;
+            }
+        }
+    }
+    std::vector<int> art_start(NB, -1), art_count(NB, 0);
+    for (int a = 0; a < d->articulation_count; ++a) {
+        const int js0 = d->articulation_start[a], js1 = d->articulation_start[a + 1];
+        if (js0 >= J) break;  // articulations of world 0 only
+        int st = 1 << 30, cnt = 0;
+        for (int j = js0; j < js1 && j < J; ++j) {
+            const int nd = t_jds[j + 1] - t_jds[j];
+            if (nd > 0) { st = std::min(st, t_jds[j]); cnt += nd; }
+        }
+        if (cnt <= 0) continue;
+        for (int j = js0; j < js1 && j < J; ++j) {
+            art_start[t_jchild[j]] = st;
+            art_count[t_jchild[j]] = cnt;
+        }
+    }
+    // shapes: world blocks then globals
+    const int Stot = d->shape_count;
+    int S = 0;
+    int first_global = Stot;
+    for (int s = 0; s < Stot; ++s) {
+        if (d->shape_world[s] < 0) { first_global = std::min(first_global, s); }
+        else if (d->shape_world[s] == 0) ++S;
+    }
+    for (int s = first_global; s < Stot; ++s)
+        if (d->shape_world[s] >= 0) return fail(SAPX_ERR_UNSUPPORTED, "global shapes must follow all world shapes");
+    if (first_global != W * S) return fail(SAPX_ERR_UNSUPPORTED, "every world must own the same number of shapes");
+    for (int s = 0; s < W * S; ++s)
+        if (d->shape_world[s] != s / S) return fail(SAPX_ERR_UNSUPPORTED, "shapes must be grouped by world");
+    for (int s = 0; s < Stot; ++s) {
+        const int t = d->shape_type[s];
+        if (t != kGeoPlane && t != kGeoSphere && t != kGeoCapsule && t != kGeoBox)
+            return fail(SAPX_ERR_UNSUPPORTED, "shape type " + std::to_string(t) + " is not in the native narrowphase");
+    }
+    // pair template: pairs grouped by world (stable), encoded local / global
+    std::vector<std::vector<int>> per_world(W);
+    for (int p = 0; p < d->pair_count; ++p) {
+        const int a = d->shape_pairs[2 * p], b = d->shape_pairs[2 * p + 1];
+        if (a < 0 || b < 0 || a >= Stot || b >= Stot) return fail(SAPX_ERR_INVALID, "pair references a bad shape");
+        const int wa = d->shape_world[a], wb = d->shape_world[b];
+        const int wp = wa >= 0 ? wa : wb;
+        if (wp < 0) continue;  // global-global pairs never collide in this path
+        if (wa >= 0 && wb >= 0 && wa != wb) continue;
+        const int ea = wa >= 0 ? a - wa * S : -a - 1;
+        const int eb = wb >= 0 ? b - wb * S : -b - 1;
+        per_world[wp].push_back(ea);
+        per_world[wp].push_back(eb);
+    }
+    for (int w = 1; w < W; ++w)
+        if (per_world[w] != per_world[0])
+            return fail(SAPX_ERR_UNSUPPORTED, "candidate pairs must repeat identically in every world");
+    // X_c identity flags
+    std::vector<int> xcid(d->joint_count);
+    for (int g = 0; g < d->joint_count; ++g) {
+        const float* x = d->joint_X_c + 7 * g;
+        xcid[g] = (x[0] == 0.0f && x[1] == 0.0f && x[2] == 0.0f && x[3] == 0.0f && x[4] == 0.0f && x[5] == 0.0f &&
+                   x[6] == 1.0f);
+    }
+    SapxModel* m = new SapxModel();
+    DevModel& dm = m->dm;
+    dm.W = W; dm.J = J; dm.N = N; dm.NQ = NQ; dm.NB = NB; dm.S = S;
+    dm.P = (int)per_world[0].size() / 2;
+    dm.num_levels = nlev;
+    dm.total_shapes = Stot;
+    dm.global_shape_base = W * S;
+    int32_t rc = SAPX_OK;
+    auto& o = m->owned;
+#define UP(dst, src, cnt)                                   \
+    if (rc == SAPX_OK) rc = upload(o, src, (size_t)(cnt), &dm.dst);
+    UP(t_jtype, t_jtype.data(), J);
+    UP(t_jparent, t_jparent.data(), J);
+    UP(t_jchild, t_jchild.data(), J);
+    UP(t_jqs, t_jqs.data(), J + 1);
+    UP(t_jds, t_jds.data(), J + 1);
+    UP(t_level_start, level_start.data(), nlev + 1);
+    UP(t_level_joint, level_joint.data(), J);
+    UP(t_dof_joint, dof_joint.data(), N);
+    UP(t_dof_coord, dof_coord.data(), N);
+    UP(t_dof_target, dof_target.data(), N);
+    UP(t_dof_limit_ok, dof_limit.data(), N);
+    UP(t_body_joint, body_joint.data(), NB);
+    UP(t_body_anc, anc.data(), NB);
+    UP(t_body_sub, sub.data(), NB);
+    UP(t_body_art_start, art_start.data(), NB);
+    UP(t_body_art_count, art_count.data(), NB);
+    UP(t_pairs, per_world[0].data(), per_world[0].size());
+    UP(j_xc_identity, xcid.data(), d->joint_count);
+    UP(joint_axis, d->joint_axis, (size_t)d->dof_count * 3);
+    UP(joint_X_p, d->joint_X_p, (size_t)d->joint_count * 7);
+    UP(joint_X_c, d->joint_X_c, (size_t)d->joint_count * 7);
+    UP(armature, d->joint_armature, d->dof_count);
+    UP(body_mass, d->body_mass, d->body_count);
+    UP(body_inertia, d->body_inertia, (size_t)d->body_count * 9);
+    UP(body_com, d->body_com, (size_t)d->body_count * 3);
+    UP(gravity, d->gravity, (size_t)W * 3);
+    UP(target_mode, d->joint_target_mode, d->dof_count);
+    UP(target_ke, d->joint_target_ke, d->dof_count);
+    UP(target_kd, d->joint_target_kd, d->dof_count);
+    UP(effort_limit, d->joint_effort_limit, d->dof_count);
+    UP(limit_lower, d->joint_limit_lower, d->dof_count);
+    UP(limit_upper, d->joint_limit_upper, d->dof_count);
+    UP(limit_ke, d->joint_limit_ke, d->dof_count);
+    UP(limit_kd, d->joint_limit_kd, d->dof_count);
+    UP(shape_body, d->shape_body, Stot);
+    UP(shape_type, d->shape_type, Stot);
+    UP(shape_transform, d->shape_transform, (size_t)Stot * 7);
+    UP(shape_scale, d->shape_scale, (size_t)Stot * 3);
+    UP(shape_margin, d->shape_margin, Stot);
+    UP(shape_gap, d->shape_gap, Stot);
+    UP(shape_radius, d->shape_collision_radius, Stot);
+    UP(shape_mu, d->shape_material_mu, Stot);
+    UP(shape_ke, d->shape_material_ke, Stot);
+    UP(shape_tau, d->shape_material_tau, Stot);
+#undef UP
+    if (rc != SAPX_OK) {
+        delete m;
+        return rc;
+    }
+    m->world_count = W;
+    m->joint_count = d->joint_count;
+    m->dof_count = d->dof_count;
+    m->coord_count = d->coord_count;
+    m->body_count = d->body_count;
+    m->shape_count = Stot;
+    *out = m;
+    return SAPX_OK;
+}
+
+void sapx_model_destroy(SapxModel* model) { delete model; }
+
+int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, SapxSolver** out) {
+    if (!model || !opts || !out) return fail(SAPX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const DevModel& dm = model->dm;
+    const int C = opts->max_rigid_contact;
+    if (C <= 0) return fail(SAPX_ERR_INVALID, "max_rigid_contact must be positive");
+    if (opts->line_search != SAPX_LS_ARMIJO_DECAY && opts->line_search != SAPX_LS_EXACT_ROOT)
+        return fail(SAPX_ERR_UNSUPPORTED, "line search variant not in the native step");
+    if (opts->line_search == SAPX_LS_ARMIJO_DECAY && !(opts->rho > 0.0 && opts->rho < 1.0))
+        return fail(SAPX_ERR_INVALID, "rho must lie in (0, 1)");
+    SapxSolver* s = new SapxSolver();
+    s->model = model;
+    s->opts = *opts;
+    const int n = dm.N, nb = dm.NB;
+    s->npad = (int)round_up(n, 16);
+    s->js = (int)round_up(n, 4);
+    const int npad = s->npad, hs = npad + 1, js = s->js;
+    // LDS layout (see Smem in sapx_step.hip)
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = round_up(off + bytes, 16);
+        return (int)o;
+    };
+    s->lds[0] = take((size_t)n * npad * 4);                                              // A
+    s->lds[1] = take(std::max((size_t)npad * hs * 4, (size_t)C * 104));                  // H / contact scratch
+    s->lds[2] = take(std::max((size_t)3 * C * js * 4, (size_t)(50 * nb + 13 * n) * 4));  // J / kinematics
+    s->lds[3] = take((size_t)nb * 8 * 4);                                                // bq
+    s->lds[4] = take((size_t)npad * 6 * 4);                                              // S
+    s->lds[5] = take((size_t)C * 4 * 8);                                                 // ck
+    s->lds[6] = take((size_t)C * 9 * 8);                                                 // cv
+    s->lds[7] = take((size_t)C * 9 * 4);                                                 // cG
+    s->lds[8] = take((size_t)C * 2 * 4);                                                 // cb
+    s->lds[9] = take((size_t)10 * npad * 8);                                             // d64
+    s->lds[10] = take((size_t)12 * npad * 4);                                            // d32
+    s->lds[11] = take((size_t)4 * npad * 4);                                             // dflag
+    s->lds[12] = take((size_t)8 * 8 * 8);                                                // red
+    s->lds[13] = take((size_t)2 * dm.NQ * 4);                                            // q
+    s->lds[14] = take(64);                                                               // misc
+    s->smem_bytes = (int)off;
+    if (s->smem_bytes > 160 * 1024) {
+        delete s;
+        return fail(SAPX_ERR_CAPACITY, "per-world LDS footprint exceeds 160 KiB; lower max_rigid_contact");
+    }
+    const size_t W = dm.W, D = (size_t)dm.W * n;
+    if (hipMalloc(&s->v_guess, D * sizeof(double)) != hipSuccess || hipMalloc(&s->st, 4 * W * sizeof(int)) != hipSuccess) {
+        delete s;
+        return fail(SAPX_ERR_HIP, "hipMalloc failed for solver buffers");
+    }
+    (void)hipMemset(s->v_guess, 0, D * sizeof(double));
+    (void)hipMemset(s->st, 0, 4 * W * sizeof(int));
+    const size_t sizes[7] = {D, D, D, W * C * 3, W * C, W * C, D};
+    size_t tot = 0;
+    for (int i = 0; i < 7; ++i) {
+        s->dbg_off[i] = tot;
+        s->dbg_size[i] = sizes[i] * sizeof(double);
+        tot += sizes[i];
+    }
+    if (hipMalloc(&s->dbg, tot * sizeof(double)) != hipSuccess) {
+        delete s;
+        return fail(SAPX_ERR_HIP, "hipMalloc failed for solver debug buffers");
+    }
+    (void)hipMemset(s->dbg, 0, tot * sizeof(double));
+    s->use_guess = 0;
+    *out = s;
+    return SAPX_OK;
+}
+
+void sapx_solver_destroy(SapxSolver* solver) { delete solver; }
+
+int32_t sapx_solver_reset(SapxSolver* solver, void* stream) {
+    if (!solver) return fail(SAPX_ERR_INVALID, "null solver");
+    solver->use_guess = 0;
+    (void)stream;
+    return SAPX_OK;
+}
+
+static int32_t ensure_group_buffers(SapxSolver* s, int capacity) {
+    if (capacity <= s->group_capacity) return SAPX_OK;
+    void* ptrs[] = {s->g_keys, s->g_keys_out, s->g_vals, s->g_vals_out, s->g_shape0, s->g_shape1, s->g_p0, s->g_p1,
+                    s->g_n, s->g_m0, s->g_m1, s->cub_tmp};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    const size_t c = (size_t)capacity;
+    HIP_TRY(hipMalloc(&s->g_keys, c * 4));
+    HIP_TRY(hipMalloc(&s->g_keys_out, c * 4));
+    HIP_TRY(hipMalloc(&s->g_vals, c * 4));
+    HIP_TRY(hipMalloc(&s->g_vals_out, c * 4));
+    HIP_TRY(hipMalloc(&s->g_shape0, c * 4));
+    HIP_TRY(hipMalloc(&s->g_shape1, c * 4));
+    HIP_TRY(hipMalloc(&s->g_p0, c * 12));
+    HIP_TRY(hipMalloc(&s->g_p1, c * 12));
+    HIP_TRY(hipMalloc(&s->g_n, c * 12));
+    HIP_TRY(hipMalloc(&s->g_m0, c * 4));
+    HIP_TRY(hipMalloc(&s->g_m1, c * 4));
+    if (!s->g_world_start) HIP_TRY(hipMalloc(&s->g_world_start, ((size_t)s->model->dm.W + 1) * 4));
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, s->g_keys, s->g_keys_out, s->g_vals, s->g_vals_out,
+                                               capacity));
+    size_t tmp2 = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, s->g_keys, s->g_world_start, s->model->dm.W + 1));
+    s->cub_tmp_bytes = std::max(tmp, tmp2);
+    HIP_TRY(hipMalloc(&s->cub_tmp, s->cub_tmp_bytes));
+    s->group_capacity = capacity;
+    return SAPX_OK;
+}
+
+int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* out, const SapxControlView* ctl,
+                  const SapxContactsView* c, double dt, const double* dt_world, void* stream) {
+    if (!s || !in || !out) return fail(SAPX_ERR_INVALID, "null argument");
+    if (!in->joint_q || !in->joint_qd || !in->body_q || !out->joint_q || !out->joint_qd || !out->body_q)
+        return fail(SAPX_ERR_INVALID, "state views need joint_q, joint_qd and body_q");
+    if (!(dt > 0.0) && !dt_world) return fail(SAPX_ERR_INVALID, "dt must be positive");
+    const DevModel& dm = s->model->dm;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    StepParams P;
+    std::memset(&P, 0, sizeof(P));
+    P.m = dm;
+    P.jq_in = in->joint_q;
+    P.jqd_in = in->joint_qd;
+    P.bq_in = in->body_q;
+    P.bf_in = in->body_f;
+    if (ctl) {
+        P.c_jf = ctl->joint_f;
+        P.c_tpos = ctl->joint_target_pos;
+        P.c_tvel = ctl->joint_target_vel;
+        P.c_act = ctl->joint_act;
+    }
+    if (c && c->count) {
+        if (c->world_start) {
+            P.cworld_start = c->world_start;
+            P.cshape0 = c->shape0; P.cshape1 = c->shape1;
+            P.cpoint0 = c->point0; P.cpoint1 = c->point1; P.cnormal = c->normal;
+            P.cmargin0 = c->margin0; P.cmargin1 = c->margin1;
+        } else {
+            // group an arbitrary flat list by world (stable radix sort on the world id)
+            const int cap = c->capacity;
+            if (cap > 0) {
+                int32_t rc = ensure_group_buffers(s, cap);
+                if (rc != SAPX_OK) return rc;
+                GroupParams G;
+                G.W = dm.W; G.NB = dm.NB; G.capacity = cap; G.count = c->count;
+                G.shape_body = dm.shape_body;
+                G.in_shape0 = c->shape0; G.in_shape1 = c->shape1;
+                G.in_p0 = c->point0; G.in_p1 = c->point1; G.in_n = c->normal; G.in_m0 = c->margin0; G.in_m1 = c->margin1;
+                G.keys = s->g_keys; G.vals = s->g_vals; G.keys_out = s->g_keys_out; G.vals_out = s->g_vals_out;
+                G.world_start = s->g_world_start;
+                G.out_shape0 = s->g_shape0; G.out_shape1 = s->g_shape1;
+                G.out_p0 = s->g_p0; G.out_p1 = s->g_p1; G.out_n = s->g_n; G.out_m0 = s->g_m0; G.out_m1 = s->g_m1;
+                G.cub_tmp = s->cub_tmp; G.cub_tmp_bytes = s->cub_tmp_bytes;
+                HIP_TRY(launch_group_contacts(G, st));
+                P.cworld_start = s->g_world_start;
+                P.cshape0 = s->g_shape0; P.cshape1 = s->g_shape1;
+                P.cpoint0 = s->g_p0; P.cpoint1 = s->g_p1; P.cnormal = s->g_n;
+                P.cmargin0 = s->g_m0; P.cmargin1 = s->g_m1;
+            }
+        }
+    }
+    P.jq_out = out->joint_q;
+    P.jqd_out = out->joint_qd;
+    P.bq_out = out->body_q;
+    P.bqd_out = out->body_qd;
+    P.bf_out = out->body_f;
+    P.v_guess = s->v_guess;
+    P.use_guess = s->use_guess;
+    const size_t W = dm.W;
+    P.st_iters = s->st;
+    P.st_ls = s->st + W;
+    P.st_nc = s->st + 2 * W;
+    P.st_flags = s->st + 3 * W;
+    P.dbg_v = s->dbg + s->dbg_off[0];
+    P.dbg_vstar = s->dbg + s->dbg_off[1];
+    P.dbg_v0 = s->dbg + s->dbg_off[2];
+    P.dbg_gamma = s->dbg + s->dbg_off[3];
+    P.dbg_phi0 = s->dbg + s->dbg_off[4];
+    P.dbg_w = s->dbg + s->dbg_off[5];
+    P.dbg_adiag = s->dbg + s->dbg_off[6];
+    P.dt = dt;
+    P.dt_world = dt_world;
+    P.C = s->opts.max_rigid_contact;
+    const SapxSolverOpts& o = s->opts;
+    P.fallback_mu = o.fallback_mu;
+    P.fallback_k = o.fallback_stiffness;
+    P.fallback_tau = o.fallback_tau_d;
+    P.beta = o.contact_beta;
+    P.sigma = o.contact_sigma;
+    P.diag_shift = o.diag_shift;
+    P.max_iter = o.max_iterations;
+    P.opt_abs = o.optimality_abs_tol;
+    P.opt_rel = o.optimality_rel_tol;
+    P.cost_abs = o.cost_abs_tol;
+    P.cost_rel = o.cost_rel_tol;
+    P.ls_mode = o.line_search;
+    P.ls_max_iter = o.line_search_max_iterations;
+    P.armijo_c = o.armijo_c;
+    P.rho = o.rho;
+    P.rel_slop = o.line_search_relative_slop;
+    P.weight_mode = o.contact_weight_mode;
+    P.point_mode = o.contact_point_mode;
+    P.integ_mode = o.position_integration;
+    P.npad = s->npad;
+    P.js = s->js;
+    P.o_A = s->lds[0]; P.o_H = s->lds[1]; P.o_J = s->lds[2]; P.o_bq = s->lds[3]; P.o_S = s->lds[4];
+    P.o_ck = s->lds[5]; P.o_cv = s->lds[6]; P.o_cG = s->lds[7]; P.o_cb = s->lds[8]; P.o_dv64 = s->lds[9];
+    P.o_dv32 = s->lds[10]; P.o_dflag = s->lds[11]; P.o_red = s->lds[12]; P.o_q = s->lds[13]; P.o_misc = s->lds[14];
+    P.smem_bytes = s->smem_bytes;
+    HIP_TRY(launch_step(P, st));
+    s->use_guess = 1;
+    return SAPX_OK;
+}
+
+int32_t sapx_get_stats(SapxSolver* s, int32_t* newton_iterations, int32_t* line_search_iterations,
+                       int32_t* contact_count, int32_t* flags, void* stream) {
+    if (!s) return fail(SAPX_ERR_INVALID, "null solver");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const size_t W = s->model->dm.W;
+    int32_t* dst[4] = {newton_iterations, line_search_iterations, contact_count, flags};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k]) HIP_TRY(hipMemcpyAsync(dst[k], s->st + k * W, W * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return SAPX_OK;
+}
+
+int32_t sapx_solver_read(SapxSolver* s, int32_t which, void* dst, int64_t bytes, void* stream) {
+    if (!s || !dst) return fail(SAPX_ERR_INVALID, "null argument");
+    if (which < 0 || which > 6) return fail(SAPX_ERR_INVALID, "unknown buffer id");
+    if ((size_t)bytes != s->dbg_size[which]) return fail(SAPX_ERR_INVALID, "byte count does not match the buffer");
+    HIP_TRY(hipMemcpyAsync(dst, s->dbg + s->dbg_off[which], (size_t)bytes, hipMemcpyDeviceToDevice,
+                           reinterpret_cast<hipStream_t>(stream)));
+    return SAPX_OK;
+}
+
+int32_t sapx_pipeline_create(const SapxModel* model, int32_t max_contacts_per_world, SapxPipeline** out) {
+    if (!model || !out) return fail(SAPX_ERR_INVALID, "null argument");
+    if (max_contacts_per_world <= 0) return fail(SAPX_ERR_INVALID, "max_contacts_per_world must be positive");
+    *out = nullptr;
+    SapxPipeline* p = new SapxPipeline();
+    p->model = model;
+    p->C = max_contacts_per_world;
+    const size_t n = (size_t)model->dm.W * p->C;
+    bool ok = hipMalloc(&p->st_shape0, n * 4) == hipSuccess && hipMalloc(&p->st_shape1, n * 4) == hipSuccess &&
+              hipMalloc(&p->st_p0, n * 12) == hipSuccess && hipMalloc(&p->st_p1, n * 12) == hipSuccess &&
+              hipMalloc(&p->st_n, n * 12) == hipSuccess && hipMalloc(&p->st_m0, n * 4) == hipSuccess &&
+              hipMalloc(&p->st_m1, n * 4) == hipSuccess && hipMalloc(&p->wcount, (size_t)model->dm.W * 4) == hipSuccess &&
+              hipMalloc(&p->unsupported, 4) == hipSuccess;
+    if (!ok) {
+        delete p;
+        return fail(SAPX_ERR_HIP, "hipMalloc failed for pipeline buffers");
+    }
+    (void)hipMemset(p->unsupported, 0, 4);
+    *out = p;
+    return SAPX_OK;
+}
+
+void sapx_pipeline_destroy(SapxPipeline* pipeline) { delete pipeline; }
+
+int32_t sapx_collide(SapxPipeline* p, const float* body_q, const SapxContactsView* c, void* stream) {
+    if (!p || !body_q || !c) return fail(SAPX_ERR_INVALID, "null argument");
+    if (!c->count || !c->world_start || !c->shape0 || !c->shape1 || !c->point0 || !c->point1 || !c->normal ||
+        !c->margin0 || !c->margin1)
+        return fail(SAPX_ERR_INVALID, "contact view is missing arrays");
+    const DevModel& dm = p->model->dm;
+    CollideParams P;
+    P.m = dm;
+    P.body_q = body_q;
+    P.C = p->C;
+    P.st_shape0 = p->st_shape0; P.st_shape1 = p->st_shape1;
+    P.st_p0 = p->st_p0; P.st_p1 = p->st_p1; P.st_n = p->st_n; P.st_m0 = p->st_m0; P.st_m1 = p->st_m1;
+    P.wcount = p->wcount;
+    P.unsupported = p->unsupported;
+    CompactParams Q;
+    Q.W = dm.W; Q.C = p->C; Q.wcount = p->wcount; Q.world_start = c->world_start; Q.count = c->count;
+    Q.st_shape0 = p->st_shape0; Q.st_shape1 = p->st_shape1; Q.st_p0 = p->st_p0; Q.st_p1 = p->st_p1; Q.st_n = p->st_n;
+    Q.st_m0 = p->st_m0; Q.st_m1 = p->st_m1;
+    Q.shape0 = c->shape0; Q.shape1 = c->shape1; Q.p0 = c->point0; Q.p1 = c->point1; Q.nrm = c->normal;
+    Q.m0 = c->margin0; Q.m1 = c->margin1; Q.capacity = c->capacity;
+    HIP_TRY(launch_collide(P, Q, reinterpret_cast<hipStream_t>(stream)));
+    return SAPX_OK;
+}
+
+}  // extern "C"

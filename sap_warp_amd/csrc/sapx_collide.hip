@@ -1,0 +1,410 @@
+// Rigid contact generation on gfx950: shape AABBs, explicit-pair AABB test and
+// primitive narrowphase, one workgroup per world, deterministic output order.
+//
+// Compiled with -ffp-contract=off and IEEE sqrt/div so contact geometry is
+// bit-identical to the float32 CPU restatement (same operation order).
+//
+// Reference: sim/collision/pipeline.py:51-179 (writer, AABBs),
+// broad_phase_common.py:18-35 / broad_phase_nxn.py:21-59 (pair test),
+// narrow_phase.py:141-577 (primitive kernel), primitive.py:37-428, 1150-1205,
+// contact_data.py:36-56 (gap check).
+//
+// Output: contacts of world w are appended in candidate-pair order; a pair's
+// contacts are written only if the whole batch fits into the world's capacity
+// (the reference's batch-overflow rule, narrow_phase.py:534-539).  A second
+// pass scans per-world counts and compacts into the flat SapContacts arrays.
+#include "sapx_device.h"
+#include "sapx_params.h"
+
+#pragma clang fp contract(off)
+
+namespace sapx {
+
+constexpr float kMaxVal = 1e10f;
+constexpr float kMinVal = 1e-15f;
+
+
+
+__device__ __forceinline__ V3 nsafe(V3 a) {
+    float l = length(a);
+    if (l > 0.0f) return v3(a.x / l, a.y / l, a.z / l);
+    return v3(0.0f, 0.0f, 0.0f);
+}
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+__device__ __forceinline__ void rot_cols(Q4 q, V3 c[3]) {
+    c[0] = qrot(q, v3(1.0f, 0.0f, 0.0f));
+    c[1] = qrot(q, v3(0.0f, 1.0f, 0.0f));
+    c[2] = qrot(q, v3(0.0f, 0.0f, 1.0f));
+}
+__device__ __forceinline__ V3 rmul(const V3 c[3], V3 v) { return (c[0] * v.x + c[1] * v.y) + c[2] * v.z; }
+__device__ __forceinline__ V3 rtmul(const V3 c[3], V3 v) { return v3(dot(c[0], v), dot(c[1], v), dot(c[2], v)); }
+
+__device__ void shape_aabb(int type, V3 pos, Q4 q, V3 scale, float radius, float eff, V3* lo, V3* hi) {
+    V3 h;
+    if (type == kGeoPlane && scale.x == 0.0f && scale.y == 0.0f) {
+        h = v3(radius, radius, radius);
+    } else if (type == kGeoSphere) {
+        h = v3(scale.x, scale.x, scale.x);
+    } else if (type == kGeoCapsule) {
+        V3 a = qrot(q, v3(0.0f, 0.0f, 1.0f));
+        h = v3(fabsf(a.x) * scale.y + scale.x, fabsf(a.y) * scale.y + scale.x, fabsf(a.z) * scale.y + scale.x);
+    } else if (type == kGeoBox) {
+        V3 c[3];
+        rot_cols(q, c);
+        h = v3(fabsf(c[0].x) * scale.x + fabsf(c[1].x) * scale.y + fabsf(c[2].x) * scale.z,
+               fabsf(c[0].y) * scale.x + fabsf(c[1].y) * scale.y + fabsf(c[2].y) * scale.z,
+               fabsf(c[0].z) * scale.x + fabsf(c[1].z) * scale.y + fabsf(c[2].z) * scale.z);
+    } else {
+        h = v3(radius, radius, radius);
+    }
+    *lo = (pos - h) - v3(eff, eff, eff);
+    *hi = (pos + h) + v3(eff, eff, eff);
+}
+
+__device__ __forceinline__ V3 closest_segment_point(V3 a, V3 b, V3 pt) {
+    V3 ab = b - a;
+    float t = dot(pt - a, ab) / (dot(ab, ab) + 1e-6f);
+    return a + ab * clampf(t, 0.0f, 1.0f);
+}
+__device__ __forceinline__ void plane_sphere(V3 n, V3 pp, V3 sp, float r, float* dist, V3* pos) {
+    float d = dot(sp - pp, n) - r;
+    *dist = d;
+    *pos = sp - n * (r + 0.5f * d);
+}
+__device__ __forceinline__ void sphere_sphere(V3 p1, float r1, V3 p2, float r2, float* dist, V3* pos, V3* nout) {
+    V3 dir = p2 - p1;
+    float d = length(dir);
+    V3 nn;
+    if (d == 0.0f) nn = v3(1.0f, 0.0f, 0.0f);
+    else nn = v3(dir.x / d, dir.y / d, dir.z / d);
+    d = d - (r1 + r2);
+    *pos = p1 + nn * (r1 + 0.5f * d);
+    *dist = d;
+    *nout = nn;
+}
+__device__ void capsule_capsule(V3 p1, V3 a1, float r1, float h1, V3 p2, V3 a2, float r2, float h2, float dist[4],
+                                V3 pos[4], V3* normal) {
+    V3 axis1 = a1 * h1;
+    V3 axis2 = a2 * h2;
+    V3 dif = p1 - p2;
+    float ma = dot(axis1, axis1);
+    float mb = -dot(axis1, axis2);
+    float mc = dot(axis2, axis2);
+    float u = -dot(axis1, dif);
+    float v = dot(axis2, dif);
+    float det = ma * mc - mb * mb;
+    if (fabsf(det) >= kMinVal) {
+        float inv = 1.0f / det;
+        float x1 = (mc * u - mb * v) * inv;
+        float x2 = (ma * v - mb * u) * inv;
+        if (x1 > 1.0f) { x1 = 1.0f; x2 = (v - mb) / mc; }
+        else if (x1 < -1.0f) { x1 = -1.0f; x2 = (v + mb) / mc; }
+        if (x2 > 1.0f) { x2 = 1.0f; x1 = clampf((u - mb) / ma, -1.0f, 1.0f); }
+        else if (x2 < -1.0f) { x2 = -1.0f; x1 = clampf((u + mb) / ma, -1.0f, 1.0f); }
+        V3 v1 = p1 + axis1 * x1;
+        V3 v2 = p2 + axis2 * x2;
+        sphere_sphere(v1, r1, v2, r2, &dist[0], &pos[0], normal);
+    } else {
+        V3 v1 = p1 + axis1;
+        float x2 = clampf((v - mb) / mc, -1.0f, 1.0f);
+        V3 v2 = p2 + axis2 * x2;
+        sphere_sphere(v1, r1, v2, r2, &dist[0], &pos[0], normal);
+        V3 n2;
+        v1 = p1 - axis1;
+        x2 = clampf((v + mb) / mc, -1.0f, 1.0f);
+        v2 = p2 + axis2 * x2;
+        sphere_sphere(v1, r1, v2, r2, &dist[1], &pos[1], &n2);
+    }
+}
+__device__ void plane_box(V3 n, V3 pp, V3 bp, Q4 bq, V3 size, float margin, float dist[4], V3 pos[4]) {
+    V3 c[3];
+    rot_cols(bq, c);
+    float center_dist = dot(bp - pp, n);
+    int ncontact = 0, worst = 0;
+    for (int i = 0; i < 8; ++i) {
+        V3 corner = v3((i & 1) ? size.x : -size.x, (i & 2) ? size.y : -size.y, (i & 4) ? size.z : -size.z);
+        corner = rmul(c, corner);
+        float ldist = dot(n, corner);
+        float cdist = center_dist + ldist;
+        if (cdist > margin) continue;
+        V3 cpos = (corner + bp) - n * (0.5f * cdist);
+        if (ncontact < 4) {
+            dist[ncontact] = cdist;
+            pos[ncontact] = cpos;
+            if (ncontact == 0 || cdist > dist[worst]) worst = ncontact;
+            ncontact++;
+        } else if (cdist < dist[worst]) {
+            dist[worst] = cdist;
+            pos[worst] = cpos;
+            worst = 0;
+            if (dist[1] > dist[worst]) worst = 1;
+            if (dist[2] > dist[worst]) worst = 2;
+            if (dist[3] > dist[worst]) worst = 3;
+        }
+    }
+}
+__device__ __forceinline__ float vget(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+__device__ void sphere_box(V3 sp, float r, V3 bp, Q4 bq, V3 size, float* dist, V3* pos, V3* normal) {
+    V3 c[3];
+    rot_cols(bq, c);
+    V3 center = rtmul(c, sp - bp);
+    V3 clamped = v3(fmaxf(-size.x, fminf(size.x, center.x)), fmaxf(-size.y, fminf(size.y, center.y)),
+                    fmaxf(-size.z, fminf(size.z, center.z)));
+    V3 diff = clamped - center;
+    float d = length(diff);
+    V3 cdir = diff;
+    if (d != 0.0f) cdir = v3(diff.x / d, diff.y / d, diff.z / d);
+    V3 lp;
+    if (d <= 1e-6f) {
+        float closest = 2.0f * (size.x + size.y + size.z);
+        int k = 0;
+        for (int i = 0; i < 6; ++i) {
+            float sgn = (i % 2) ? 1.0f : -1.0f;
+            float fd = fabsf(sgn * vget(size, i / 2) - vget(center, i / 2));
+            if (closest > fd) { closest = fd; k = i; }
+        }
+        V3 nearest = v3(0.0f, 0.0f, 0.0f);
+        float sv = (k % 2) ? -1.0f : 1.0f;
+        if (k / 2 == 0) nearest.x = sv; else if (k / 2 == 1) nearest.y = sv; else nearest.z = sv;
+        lp = center + nearest * ((r - closest) / 2.0f);
+        *normal = rmul(c, nearest);
+        *dist = -closest - r;
+    } else {
+        V3 deepest = center + cdir * r;
+        lp = (clamped + deepest) * 0.5f;
+        *normal = rmul(c, cdir);
+        *dist = d - r;
+    }
+    *pos = bp + rmul(c, lp);
+}
+__device__ __forceinline__ void contact_points(V3 center, V3 n_in, float d, float ra, float rb, V3* xa, V3* xb, V3* n) {
+    *n = nsafe(n_in);
+    *xa = center - (*n) * (0.5f * d + ra);
+    *xb = center + (*n) * (0.5f * d + rb);
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
+    extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+    const DevModel& M = P.m;
+    const int w = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nglob = M.total_shapes - M.global_shape_base;
+    const int ns = M.S + nglob;
+    float* X = reinterpret_cast<float*>(smem_raw);  // ns x 8 world transforms
+    float* AB = X + ns * 8;                          // ns x 6 aabbs
+    int* scan = reinterpret_cast<int*>(AB + ns * 6); // NT/64 wave totals + 2
+    for (int s = tid; s < ns; s += NT) {
+        const int g = (s < M.S) ? w * M.S + s : M.global_shape_base + (s - M.S);
+        TF xs = load_tf(M.shape_transform + (size_t)g * 7);
+        const int b = M.shape_body[g];
+        if (b >= 0) xs = tf_mul(load_tf(P.body_q + (size_t)b * 7), xs);
+        float* o = X + s * 8;
+        o[0] = xs.p.x; o[1] = xs.p.y; o[2] = xs.p.z; o[3] = xs.q.x; o[4] = xs.q.y; o[5] = xs.q.z; o[6] = xs.q.w;
+        V3 lo, hi;
+        const float* sc = M.shape_scale + (size_t)g * 3;
+        shape_aabb(M.shape_type[g], xs.p, xs.q, v3(sc[0], sc[1], sc[2]), M.shape_radius[g],
+                   M.shape_margin[g] + M.shape_gap[g], &lo, &hi);
+        float* a = AB + s * 6;
+        a[0] = lo.x; a[1] = lo.y; a[2] = lo.z; a[3] = hi.x; a[4] = hi.y; a[5] = hi.z;
+    }
+    if (tid == 0) { scan[NT / 64] = 0; scan[NT / 64 + 1] = 0; }
+    __syncthreads();
+    int claim_base = 0;  // uniform running claim of earlier chunks
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int p0 = 0; p0 < M.P; p0 += NT) {
+        const int p = p0 + tid;
+        int nv = 0;
+        float dist[4] = {kMaxVal, kMaxVal, kMaxVal, kMaxVal};
+        V3 pos[4] = {v3(0, 0, 0), v3(0, 0, 0), v3(0, 0, 0), v3(0, 0, 0)};
+        V3 normal = v3(0, 0, 0);
+        float ra = 0.0f, rb = 0.0f;
+        int la = -1, lb = -1, ga = -1, gb = -1;
+        bool valid[4] = {false, false, false, false};
+        if (p < M.P) {
+            int e0 = M.t_pairs[p * 2], e1 = M.t_pairs[p * 2 + 1];
+            int l1 = e0 >= 0 ? e0 : M.S + (-e0 - 1 - M.global_shape_base);
+            int l2 = e1 >= 0 ? e1 : M.S + (-e1 - 1 - M.global_shape_base);
+            int g1 = e0 >= 0 ? w * M.S + e0 : -e0 - 1;
+            int g2 = e1 >= 0 ? w * M.S + e1 : -e1 - 1;
+            const float* A = AB + l1 * 6;
+            const float* B = AB + l2 * 6;
+            bool ov = A[0] <= B[3] && A[3] >= B[0] && A[1] <= B[4] && A[4] >= B[1] && A[2] <= B[5] && A[5] >= B[2];
+            if (ov && g1 != g2) {
+                int ta = M.shape_type[g1], tb = M.shape_type[g2];
+                la = l1; lb = l2; ga = g1; gb = g2;
+                if (ta > tb) {
+                    int t = la; la = lb; lb = t;
+                    t = ga; ga = gb; gb = t;
+                    t = ta; ta = tb; tb = t;
+                }
+                const float* sca = M.shape_scale + (size_t)ga * 3;
+                const float* scb = M.shape_scale + (size_t)gb * 3;
+                V3 scale_a = v3(sca[0], sca[1], sca[2]);
+                V3 scale_b = v3(scb[0], scb[1], scb[2]);
+                TF Xa = load_tf(X + la * 8), Xb = load_tf(X + lb * 8);
+                if (ta == kGeoSphere || ta == kGeoCapsule) ra = scale_a.x;
+                if (tb == kGeoSphere || tb == kGeoCapsule) rb = scale_b.x;
+                const float gap_sum = M.shape_gap[g1] + M.shape_gap[g2];
+                const V3 ez = v3(0.0f, 0.0f, 1.0f);
+                if (ta == kGeoPlane && tb == kGeoSphere) {
+                    V3 nn = qrot(Xa.q, ez);
+                    plane_sphere(nn, Xa.p, Xb.p, scale_b.x, &dist[0], &pos[0]);
+                    normal = nn;
+                } else if (ta == kGeoPlane && tb == kGeoBox) {
+                    V3 nn = qrot(Xa.q, ez);
+                    plane_box(nn, Xa.p, Xb.p, Xb.q, scale_b, gap_sum, dist, pos);
+                    normal = nn;
+                } else if (ta == kGeoSphere && tb == kGeoSphere) {
+                    sphere_sphere(Xa.p, scale_a.x, Xb.p, scale_b.x, &dist[0], &pos[0], &normal);
+                } else if (ta == kGeoPlane && tb == kGeoCapsule) {
+                    V3 nn = qrot(Xa.q, ez);
+                    V3 ax = qrot(Xb.q, ez);
+                    V3 seg = ax * scale_b.y;
+                    plane_sphere(nn, Xa.p, Xb.p + seg, scale_b.x, &dist[0], &pos[0]);
+                    plane_sphere(nn, Xa.p, Xb.p - seg, scale_b.x, &dist[1], &pos[1]);
+                    normal = nn;
+                } else if (ta == kGeoSphere && tb == kGeoCapsule) {
+                    V3 ax = qrot(Xb.q, ez);
+                    V3 seg = ax * scale_b.y;
+                    V3 pt = closest_segment_point(Xb.p - seg, Xb.p + seg, Xa.p);
+                    sphere_sphere(Xa.p, scale_a.x, pt, scale_b.x, &dist[0], &pos[0], &normal);
+                } else if (ta == kGeoCapsule && tb == kGeoCapsule) {
+                    V3 a1 = qrot(Xa.q, ez), a2 = qrot(Xb.q, ez);
+                    capsule_capsule(Xa.p, a1, scale_a.x, scale_a.y, Xb.p, a2, scale_b.x, scale_b.y, dist, pos, &normal);
+                } else if (ta == kGeoSphere && tb == kGeoBox) {
+                    sphere_box(Xa.p, scale_a.x, Xb.p, Xb.q, scale_b, &dist[0], &pos[0], &normal);
+                } else {
+                    atomicAdd(P.unsupported, 1);
+                }
+                const float ma = M.shape_margin[ga], mbm = M.shape_margin[gb];
+                for (int i = 0; i < 4; ++i) {
+                    if (dist[i] < kMaxVal) {
+                        V3 xa, xb, nn;
+                        contact_points(pos[i], normal, dist[i], ra, rb, &xa, &xb, &nn);
+                        float total = ((ra + rb) + ma) + mbm;
+                        float dd = dot(xb - xa, nn) - total;
+                        valid[i] = dd <= gap_sum;
+                        nv += valid[i] ? 1 : 0;
+                    }
+                }
+            }
+        }
+        // block exclusive scan of nv in pair order
+        int incl = nv;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        if (lane == 63) scan[wave] = incl;
+        __syncthreads();
+        int wave_off = 0, chunk_total = 0;
+        for (int wv = 0; wv < NT / 64; ++wv) {
+            if (wv < wave) wave_off += scan[wv];
+            chunk_total += scan[wv];
+        }
+        const int prefix = claim_base + wave_off + incl - nv;
+        if (nv > 0 && prefix + nv <= P.C) {
+            int idx = prefix;
+            const size_t base = (size_t)w * P.C;
+            const int bA = M.shape_body[ga], bB = M.shape_body[gb];
+            const float ma = M.shape_margin[ga], mbm = M.shape_margin[gb];
+            for (int i = 0; i < 4; ++i) {
+                if (!valid[i]) continue;
+                V3 xa, xb, nn;
+                contact_points(pos[i], normal, dist[i], ra, rb, &xa, &xb, &nn);
+                V3 pa = xa, pb = xb;
+                if (bA >= 0) {
+                    TF t = load_tf(P.body_q + (size_t)bA * 7);
+                    pa = qrot(qconj(t.q), xa - t.p);
+                }
+                if (bB >= 0) {
+                    TF t = load_tf(P.body_q + (size_t)bB * 7);
+                    pb = qrot(qconj(t.q), xb - t.p);
+                }
+                const size_t o = base + idx;
+                P.st_shape0[o] = ga;
+                P.st_shape1[o] = gb;
+                P.st_p0[o * 3] = pa.x; P.st_p0[o * 3 + 1] = pa.y; P.st_p0[o * 3 + 2] = pa.z;
+                P.st_p1[o * 3] = pb.x; P.st_p1[o * 3 + 1] = pb.y; P.st_p1[o * 3 + 2] = pb.z;
+                P.st_n[o * 3] = nn.x; P.st_n[o * 3 + 1] = nn.y; P.st_n[o * 3 + 2] = nn.z;
+                P.st_m0[o] = ra + ma;
+                P.st_m1[o] = rb + mbm;
+                ++idx;
+            }
+            atomicMax(&scan[NT / 64], prefix + nv);
+        }
+        claim_base += chunk_total;
+        __syncthreads();
+    }
+    if (tid == 0) P.wcount[w] = scan[NT / 64];
+}
+
+// exclusive scan of per-world counts (single workgroup)
+template <int NT>
+__global__ __launch_bounds__(NT) void sapx_world_scan_kernel(CompactParams P) {
+    __shared__ int part[NT / 64];
+    __shared__ int carry;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < P.W; b0 += NT) {
+        const int wi = b0 + tid;
+        const int x = wi < P.W ? P.wcount[wi] : 0;
+        int incl = x;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += y;
+        }
+        if (lane == 63) part[wave] = incl;
+        __syncthreads();
+        int off = carry, tot = 0;
+        for (int wv = 0; wv < NT / 64; ++wv) {
+            if (wv < wave) off += part[wv];
+            tot += part[wv];
+        }
+        if (wi < P.W) P.world_start[wi] = off + incl - x;
+        __syncthreads();
+        if (tid == 0) carry += tot;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        P.world_start[P.W] = carry;
+        P.count[0] = carry;
+    }
+}
+
+__global__ void sapx_compact_kernel(CompactParams P) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int w = (int)(t / P.C), k = (int)(t % P.C);
+    if (w >= P.W) return;
+    if (k >= P.wcount[w]) return;
+    const int dst = P.world_start[w] + k;
+    if (dst >= P.capacity) return;
+    P.shape0[dst] = P.st_shape0[t];
+    P.shape1[dst] = P.st_shape1[t];
+    for (int e = 0; e < 3; ++e) {
+        P.p0[(size_t)dst * 3 + e] = P.st_p0[t * 3 + e];
+        P.p1[(size_t)dst * 3 + e] = P.st_p1[t * 3 + e];
+        P.nrm[(size_t)dst * 3 + e] = P.st_n[t * 3 + e];
+    }
+    P.m0[dst] = P.st_m0[t];
+    P.m1[dst] = P.st_m1[t];
+}
+
+hipError_t launch_collide(const CollideParams& P, const CompactParams& Q, hipStream_t stream) {
+    constexpr int NT = 128;
+    const int nglob = P.m.total_shapes - P.m.global_shape_base;
+    const int ns = P.m.S + nglob;
+    const size_t smem = (size_t)ns * 14 * sizeof(float) + (NT / 64 + 2) * sizeof(int) + 16;
+    hipLaunchKernelGGL(sapx_collide_kernel<NT>, dim3(P.m.W), dim3(NT), smem, stream, P);
+    hipLaunchKernelGGL(sapx_world_scan_kernel<1024>, dim3(1), dim3(1024), 0, stream, Q);
+    const size_t total = (size_t)Q.W * Q.C;
+    const int blk = 256;
+    hipLaunchKernelGGL(sapx_compact_kernel, dim3((unsigned)((total + blk - 1) / blk)), dim3(blk), 0, stream, Q);
+    return hipGetLastError();
+}
+
+}  // namespace sapx

@@ -1,0 +1,170 @@
+// Device-side types and small math helpers shared by the sapx kernels.
+//
+// Conventions: transforms are [p(3), q(4) xyzw]; spatial vectors are
+// angular-first [w, v] inside the solver (the reference's SAP buffers,
+// sim/sap_helpers.py:771-785) and linear-first in the public body_qd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sapx {
+
+constexpr int kJointPrismatic = 0;
+constexpr int kJointRevolute = 1;
+constexpr int kJointBall = 2;
+constexpr int kJointFixed = 3;
+constexpr int kJointFree = 4;
+constexpr int kJointDistance = 5;
+constexpr int kJointD6 = 6;
+
+constexpr int kGeoPlane = 1;
+constexpr int kGeoSphere = 3;
+constexpr int kGeoCapsule = 4;
+constexpr int kGeoBox = 7;
+
+constexpr int kMaxTemplate = 64;  // joints / bodies per world (64-bit masks)
+constexpr int kMaxDof = 64;       // dofs per world (one wavefront per row set)
+
+// --------------------------------------------------------------------------
+// Device model: per-world topology template (local indices, shared by all
+// worlds) + global per-item parameter arrays (world-major).
+// --------------------------------------------------------------------------
+struct DevModel {
+    int W;   // worlds
+    int J;   // joints per world
+    int N;   // dofs per world
+    int NQ;  // coords per world
+    int NB;  // bodies per world
+    int S;   // shapes per world
+    int P;   // candidate pairs per world
+    int num_levels;
+    int total_shapes;
+    int global_shape_base;  // first global (world -1) shape index = W*S
+
+    // template, local indices
+    const int* t_jtype;
+    const int* t_jparent;   // local body or -1
+    const int* t_jchild;    // local body
+    const int* t_jqs;       // J+1 local coord starts
+    const int* t_jds;       // J+1 local dof starts
+    const int* t_level_start;  // num_levels+1
+    const int* t_level_joint;  // J
+    const int* t_dof_joint;    // N
+    const int* t_dof_coord;    // N: local coord or -1
+    const int* t_dof_target;   // N: local dof of the drive target (free joints swap blocks)
+    const int* t_dof_limit_ok; // N
+    const int* t_body_joint;   // NB: joint with this child body, -1 if none
+    const unsigned long long* t_body_anc;  // NB: bit j = joint j on the root path (incl. own)
+    const unsigned long long* t_body_sub;  // NB: bit b = body b in subtree (incl. self)
+    const int* t_body_art_start;  // NB: first local dof of the body's articulation (-1 none)
+    const int* t_body_art_count;  // NB
+    const int* t_pairs;           // P x 2: >=0 local shape, <0 => global shape (-v-1)
+
+    // global per-item arrays
+    const int* j_xc_identity;  // W*J
+    const float* joint_axis;   // W*N x 3
+    const float* joint_X_p;    // W*J x 7
+    const float* joint_X_c;    // W*J x 7
+    const float* armature;     // W*N
+    const float* body_mass;    // W*NB
+    const float* body_inertia; // W*NB x 9
+    const float* body_com;     // W*NB x 3
+    const float* gravity;      // W x 3
+    const int* target_mode;    // W*N
+    const float* target_ke;
+    const float* target_kd;
+    const float* effort_limit;
+    const float* limit_lower;
+    const float* limit_upper;
+    const float* limit_ke;
+    const float* limit_kd;
+    const int* shape_body;     // total_shapes
+    const int* shape_type;
+    const float* shape_transform;  // x7
+    const float* shape_scale;      // x3
+    const float* shape_margin;
+    const float* shape_gap;
+    const float* shape_radius;
+    const float* shape_mu;
+    const float* shape_ke;
+    const float* shape_tau;
+};
+
+// --------------------------------------------------------------------------
+// float3 / quaternion helpers (float32)
+// --------------------------------------------------------------------------
+struct V3 {
+    float x, y, z;
+};
+struct Q4 {
+    float x, y, z, w;
+};
+
+__device__ __forceinline__ V3 v3(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return v3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float length(V3 a) { return sqrtf(dot(a, a)); }
+
+__device__ __forceinline__ V3 qrot(Q4 q, V3 v) {
+    V3 qv = v3(q.x, q.y, q.z);
+    V3 t = cross(qv, v) * 2.0f;
+    return v + t * q.w + cross(qv, t);
+}
+__device__ __forceinline__ Q4 qconj(Q4 q) { return Q4{-q.x, -q.y, -q.z, q.w}; }
+__device__ __forceinline__ Q4 qmul(Q4 a, Q4 b) {
+    return Q4{a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+              a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+__device__ __forceinline__ Q4 qnormalize(Q4 q) {
+    float n = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    if (n <= 0.0f) return Q4{0.0f, 0.0f, 0.0f, 1.0f};
+    float inv = 1.0f / n;
+    return Q4{q.x * inv, q.y * inv, q.z * inv, q.w * inv};
+}
+__device__ __forceinline__ Q4 q_axis_angle(V3 a, float angle) {
+    float s = sinf(0.5f * angle);
+    float c = cosf(0.5f * angle);
+    return Q4{a.x * s, a.y * s, a.z * s, c};
+}
+
+struct TF {
+    V3 p;
+    Q4 q;
+};
+__device__ __forceinline__ TF load_tf(const float* a) {
+    return TF{v3(a[0], a[1], a[2]), Q4{a[3], a[4], a[5], a[6]}};
+}
+__device__ __forceinline__ void store_tf(float* a, TF t) {
+    a[0] = t.p.x; a[1] = t.p.y; a[2] = t.p.z;
+    a[3] = t.q.x; a[4] = t.q.y; a[5] = t.q.z; a[6] = t.q.w;
+}
+// composition with normalisation (free-motion kinematics, sap_helpers.py:670-676)
+__device__ __forceinline__ TF tf_compose_n(TF a, TF b) {
+    Q4 aq = qnormalize(a.q);
+    Q4 bq = qnormalize(b.q);
+    return TF{a.p + qrot(aq, b.p), qnormalize(qmul(aq, bq))};
+}
+__device__ __forceinline__ TF tf_inverse_n(TF x) {
+    Q4 qi = qconj(qnormalize(x.q));
+    return TF{-qrot(qi, x.p), qi};
+}
+// plain composition (public FK / collision, no normalisation)
+__device__ __forceinline__ TF tf_mul(TF a, TF b) { return TF{a.p + qrot(a.q, b.p), qmul(a.q, b.q)}; }
+__device__ __forceinline__ TF tf_inverse(TF x) {
+    Q4 qi = qconj(x.q);
+    return TF{-qrot(qi, x.p), qi};
+}
+__device__ __forceinline__ bool tf_is_identity(const float* a) {
+    return a[0] == 0.0f && a[1] == 0.0f && a[2] == 0.0f && a[3] == 0.0f && a[4] == 0.0f && a[5] == 0.0f &&
+           a[6] == 1.0f;
+}
+
+}  // namespace sapx

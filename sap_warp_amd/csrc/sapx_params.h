@@ -1,0 +1,124 @@
+// Kernel parameter blocks shared by the host API and the kernels.
+#pragma once
+
+#include "sapx_device.h"
+
+namespace sapx {
+
+struct StepParams {
+    DevModel m;
+    const float* jq_in;
+    const float* jqd_in;
+    const float* bq_in;
+    const float* bf_in;
+    const float* c_jf;
+    const float* c_tpos;
+    const float* c_tvel;
+    const float* c_act;
+    const int* cworld_start;
+    const int* cshape0;
+    const int* cshape1;
+    const float* cpoint0;
+    const float* cpoint1;
+    const float* cnormal;
+    const float* cmargin0;
+    const float* cmargin1;
+    float* jq_out;
+    float* jqd_out;
+    float* bq_out;
+    float* bqd_out;
+    float* bf_out;
+    double* v_guess;
+    int use_guess;
+    int* st_iters;
+    int* st_ls;
+    int* st_nc;
+    int* st_flags;
+    double* dbg_v;
+    double* dbg_vstar;
+    double* dbg_v0;
+    double* dbg_gamma;
+    double* dbg_phi0;
+    double* dbg_w;
+    double* dbg_adiag;
+    double dt;
+    const double* dt_world;
+    int C;
+    double fallback_mu, fallback_k, fallback_tau, beta, sigma, diag_shift;
+    int max_iter;
+    double opt_abs, opt_rel, cost_abs, cost_rel;
+    int ls_mode, ls_max_iter;
+    double armijo_c, rho, rel_slop;
+    int weight_mode, point_mode, integ_mode;
+    int npad, js;
+    // LDS byte offsets
+    int o_A, o_H, o_J, o_bq, o_S, o_ck, o_cv, o_cG, o_cb, o_dv64, o_dv32, o_dflag, o_red, o_q, o_misc;
+    int smem_bytes;
+};
+
+struct CollideParams {
+    DevModel m;
+    const float* body_q;
+    int C;
+    int* st_shape0;
+    int* st_shape1;
+    float* st_p0;
+    float* st_p1;
+    float* st_n;
+    float* st_m0;
+    float* st_m1;
+    int* wcount;
+    int* unsupported;
+};
+
+struct CompactParams {
+    int W, C;
+    const int* wcount;
+    int* world_start;
+    int* count;
+    const int* st_shape0;
+    const int* st_shape1;
+    const float* st_p0;
+    const float* st_p1;
+    const float* st_n;
+    const float* st_m0;
+    const float* st_m1;
+    int* shape0;
+    int* shape1;
+    float* p0;
+    float* p1;
+    float* nrm;
+    float* m0;
+    float* m1;
+    int capacity;
+};
+
+// grouping of an arbitrary flat contact list by world (stable radix sort)
+struct GroupParams {
+    int W, NB, capacity;
+    const int* count;
+    const int* shape_body;
+    const int* in_shape0;
+    const int* in_shape1;
+    const float* in_p0;
+    const float* in_p1;
+    const float* in_n;
+    const float* in_m0;
+    const float* in_m1;
+    int* keys;
+    int* vals;
+    int* keys_out;
+    int* vals_out;
+    int* world_start;
+    int* out_shape0;
+    int* out_shape1;
+    float* out_p0;
+    float* out_p1;
+    float* out_n;
+    float* out_m0;
+    float* out_m1;
+    void* cub_tmp;
+    size_t cub_tmp_bytes;
+};
+
+}  // namespace sapx

@@ -1,0 +1,1464 @@
+// Fused SAP timestep, one workgroup per world (gfx950).
+//
+// A world's whole step runs inside one workgroup with every intermediate in
+// LDS: public->SAP boundary conversion, forward kinematics, bias forces,
+// composite-rigid-body mass matrix, free motion, contact frames / Jacobian /
+// weights, PD and limit terms, the SAP Newton loop (projection onto the
+// regularised friction cone, gradient, Hessian, blocked Cholesky with MFMA
+// Schur updates, exact-root or Armijo line search, convergence test),
+// position integration and public forward kinematics.  HBM traffic is the
+// compulsory state/control/contact reads and state writes only.
+//
+// Precision: kinematics, dynamics, the contact Jacobian, the Hessian and its
+// Cholesky factor are float32 (the reference's approx32 preset solves its
+// linear systems in float32, sim/solver_sap.py:78-87); velocities, the SAP
+// objective, gradient, impulses and the line search are float64 accumulated
+// over the float32 data, as in the preset's fp64 contact solve.
+//
+// Reference algorithm (file:line) for each phase is cited inline.
+#include "sapx_device.h"
+#include "sapx_params.h"
+
+namespace sapx {
+
+constexpr double kPi = 3.141592653589793;
+constexpr double kSoftNormTol = 1.0e-7;   // contact_solve.py:47
+constexpr double kPdBeta = 0.1;           // contact_solve.py:48
+constexpr double kLimitBeta = 0.1;        // contact_solve.py:49
+constexpr double kLimitWindow = 2.0;      // contact_solve.py:51
+constexpr double kExactAlphaMax = 1.5;    // contact_solve.py:52
+constexpr double kExactFTol = 1.0e-8;     // contact_solve.py:54
+
+
+// ---------------------------------------------------------------------------
+// block utilities
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ __forceinline__ void block_sum(double* vals, int K, double* red) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    for (int k = 0; k < K; ++k) {
+        double x = vals[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) red[wave * 8 + k] = x;
+    }
+    __syncthreads();
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int wv = 0; wv < NW; ++wv) s += red[wv * 8 + k];
+        vals[k] = s;
+    }
+    __syncthreads();
+}
+
+template <int NT>
+__device__ __forceinline__ int block_or(int x, double* red) {
+    double v[1] = {x ? 1.0 : 0.0};
+    block_sum<NT>(v, 1, red);
+    return v[0] > 0.0;
+}
+
+__device__ __forceinline__ bool bit(unsigned long long mask, int i) { return (mask >> i) & 1ull; }
+
+// SAP spatial helpers (angular-first), float32
+struct SV {
+    V3 w, v;
+};
+__device__ __forceinline__ SV sv(V3 w, V3 v) { return SV{w, v}; }
+__device__ __forceinline__ SV load_sv(const float* a) { return SV{v3(a[0], a[1], a[2]), v3(a[3], a[4], a[5])}; }
+__device__ __forceinline__ void store_sv(float* a, SV s) {
+    a[0] = s.w.x; a[1] = s.w.y; a[2] = s.w.z; a[3] = s.v.x; a[4] = s.v.y; a[5] = s.v.z;
+}
+__device__ __forceinline__ SV sv_scale(SV a, float s) { return SV{a.w * s, a.v * s}; }
+__device__ __forceinline__ SV sv_add(SV a, SV b) { return SV{a.w + b.w, a.v + b.v}; }
+__device__ __forceinline__ float sv_dot(SV a, SV b) { return dot(a.w, b.w) + dot(a.v, b.v); }
+// sap_helpers.py:948-994
+__device__ __forceinline__ SV shift_velocity(SV V, V3 p) { return SV{V.w, V.v + cross(V.w, p)}; }
+__device__ __forceinline__ SV compose_velocity(SV Vp, V3 p, SV Vpb) {
+    return SV{Vp.w + Vpb.w, Vp.v + cross(Vp.w, p) + Vpb.v};
+}
+__device__ __forceinline__ SV compose_acceleration(SV Ap, SV Vp, V3 p, SV Vpb, SV Apb) {
+    V3 w = Vp.w;
+    V3 a_shift = Ap.v + cross(Ap.w, p) + cross(w, cross(w, p));
+    return SV{Ap.w + Apb.w + cross(w, Vpb.w), a_shift + Apb.v + cross(w, Vpb.v) * 2.0f};
+}
+// mobilizer velocity -> body origin (sap_helpers.py:997-1024)
+__device__ __forceinline__ SV mob_to_body(Q4 R, V3 p, SV Vf) {
+    return SV{qrot(R, Vf.w), qrot(R, Vf.v + cross(Vf.w, p))};
+}
+__device__ __forceinline__ SV mob_bias_to_body(Q4 R, V3 p, SV Vf) {
+    V3 w = Vf.w;
+    return SV{v3(0.0f, 0.0f, 0.0f), qrot(R, cross(w, cross(w, p)))};
+}
+
+// ---------------------------------------------------------------------------
+// LDS view
+// ---------------------------------------------------------------------------
+struct Smem {
+    float* A;       // n x npad  dynamics matrix
+    float* H;       // npad x (npad+1)  Hessian / Cholesky
+    float* J;       // 3C x js  contact Jacobian (kinematics scratch before it)
+    float* bq;      // NB x 8  body poses (free-motion FK)
+    float* S;       // N x 6  motion subspace (body origin, world)
+    double* ck;     // C x 4  rt, rn, vhat, mu
+    double* cv;     // C x 9  vc, dvc, gamma
+    float* cG;      // C x 9
+    int* cb;        // C x 2 local bodies
+    double* d64;    // 10 x npad: v, v0, vstar, dv, grad, jimp, dp, av, tmp, tmp2
+    float* d32;     // 12 x npad
+    int* dflag;     // 4 x npad
+    double* red;    // 8 x 8
+    float* q;       // 2 x NQ coords (in, out)
+    float* misc;
+};
+
+enum { V_V = 0, V_V0, V_VS, V_DV, V_G, V_JI, V_DP, V_AV, V_T1, V_T2, V_NUM };
+enum { F_HD = 0, F_AINV, F_DSC, F_PDA, F_PDG, F_PDL, F_LOV, F_UPV, F_LOR, F_UPR, F_TAU, F_JF, F_NUM };
+enum { I_PART = 0, I_PD, I_LO, I_UP, I_NUM };
+
+__device__ __forceinline__ Smem smem_view(char* base, const StepParams& P) {
+    Smem s;
+    s.A = reinterpret_cast<float*>(base + P.o_A);
+    s.H = reinterpret_cast<float*>(base + P.o_H);
+    s.J = reinterpret_cast<float*>(base + P.o_J);
+    s.bq = reinterpret_cast<float*>(base + P.o_bq);
+    s.S = reinterpret_cast<float*>(base + P.o_S);
+    s.ck = reinterpret_cast<double*>(base + P.o_ck);
+    s.cv = reinterpret_cast<double*>(base + P.o_cv);
+    s.cG = reinterpret_cast<float*>(base + P.o_cG);
+    s.cb = reinterpret_cast<int*>(base + P.o_cb);
+    s.d64 = reinterpret_cast<double*>(base + P.o_dv64);
+    s.d32 = reinterpret_cast<float*>(base + P.o_dv32);
+    s.dflag = reinterpret_cast<int*>(base + P.o_dflag);
+    s.red = reinterpret_cast<double*>(base + P.o_red);
+    s.q = reinterpret_cast<float*>(base + P.o_q);
+    s.misc = reinterpret_cast<float*>(base + P.o_misc);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Cholesky of H (npad x npad, row stride hs), in place, lower.
+// Diagonal 16x16 blocks: wave-synchronous right-looking factor in registers;
+// panel TRSM: one lane per row; trailing Schur updates: MFMA f32 16x16x4.
+// Rows/cols >= n hold the identity (identity padding, blocked_cholesky.py:725-901).
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ void cholesky_blocked(float* H, int npad, int hs) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nb = npad >> 4;
+    for (int kb = 0; kb < nb; ++kb) {
+        const int k0 = kb * 16;
+        // 1. diagonal block factor (wave 0; lanes 0..15 own one row each)
+        if (wave == 0) {
+            float row[16];
+            const int r = lane & 15;
+#pragma unroll
+            for (int c = 0; c < 16; ++c) row[c] = H[(k0 + r) * hs + k0 + c];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                float djj = __shfl(row[j], j, 64);
+                djj = sqrtf(fmaxf(djj, 1.0e-30f));
+                float lij = row[j] / djj;
+                if (r == j) lij = djj;
+                if (r < j) lij = row[j];
+                row[j] = lij;
+                // rank-1 update of the trailing columns of rows below j
+#pragma unroll
+                for (int c = j + 1; c < 16; ++c) {
+                    float lcj = __shfl(row[j], c, 64);  // L[c][j]
+                    if (r > j && c <= r) row[c] -= lij * lcj;
+                }
+            }
+            if (lane < 16) {
+#pragma unroll
+                for (int c = 0; c < 16; ++c) H[(k0 + r) * hs + k0 + c] = (c <= r) ? row[c] : 0.0f;
+            }
+        }
+        __syncthreads();
+        // 2. panel TRSM: rows below the diagonal block
+        const int rows_below = npad - k0 - 16;
+        for (int t = threadIdx.x; t < rows_below; t += NT) {
+            const int i = k0 + 16 + t;
+            float x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x[c] = H[i * hs + k0 + c];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                float s = x[c];
+#pragma unroll
+                for (int k = 0; k < c; ++k) s -= x[k] * H[(k0 + c) * hs + k0 + k];
+                x[c] = s / H[(k0 + c) * hs + k0 + c];
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) H[i * hs + k0 + c] = x[c];
+        }
+        __syncthreads();
+        // 3. trailing update with MFMA: tiles (ib, jb), kb < jb <= ib
+        const int rem = nb - kb - 1;
+        const int ntiles = rem * (rem + 1) / 2;
+        for (int t = wave; t < ntiles; t += NW) {
+            int ib = 0, acc_t = t;
+            while (acc_t > ib) { acc_t -= ib + 1; ++ib; }
+            const int jb = acc_t;
+            const int i0 = (kb + 1 + ib) * 16;
+            const int j0 = (kb + 1 + jb) * 16;
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                const int kcol = k0 + kk * 4 + (lane >> 4);
+                float a = H[(i0 + (lane & 15)) * hs + kcol];
+                float b = H[(j0 + (lane & 15)) * hs + kcol];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int i = i0 + 4 * (lane >> 4) + e;
+                const int j = j0 + (lane & 15);
+                H[i * hs + j] -= acc[e];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Solve L L^T x = b for one right-hand side held in LDS (double), wave 0 only.
+// x overwrites b.  Rows >= n are identity.
+__device__ void chol_solve_wave0(const float* H, int hs, int n, double* b) {
+    const int lane = threadIdx.x & 63;
+    if ((threadIdx.x >> 6) != 0) return;
+    double y = (lane < n) ? b[lane] : 0.0;
+    // forward: L y = b
+    for (int j = 0; j < n; ++j) {
+        double yj = __shfl(y, j, 64);
+        double ljj = (double)H[j * hs + j];
+        yj = yj / ljj;
+        if (lane == j) y = yj;
+        if (lane > j && lane < n) y -= (double)H[lane * hs + j] * yj;
+    }
+    // backward: L^T x = y
+    for (int j = n - 1; j >= 0; --j) {
+        double xj = __shfl(y, j, 64);
+        xj = xj / (double)H[j * hs + j];
+        if (lane == j) y = xj;
+        if (lane < j) y -= (double)H[j * hs + lane] * xj;
+    }
+    if (lane < n) b[lane] = y;
+}
+
+// ---------------------------------------------------------------------------
+// SAP projection (contact_solve.py:743-921); returns cost, writes gamma and G
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double project_contact(const double* k, const double vc[3], double g[3], float* G,
+                                                  bool want_G) {
+    const double rt = k[0], rn = k[1], vhat = k[2], mu = k[3];
+    const double rt_inv = 1.0 / rt, rn_inv = 1.0 / rn;
+    const double y0 = -rt_inv * vc[0], y1 = -rt_inv * vc[1], y2 = rn_inv * (vhat - vc[2]);
+    const double yr = sqrt(y0 * y0 + y1 * y1 + kSoftNormTol * kSoftNormTol);
+    double tx = 0.0, ty = 0.0;
+    if (yr > 0.0) { tx = y0 / yr; ty = y1 / yr; }
+    g[0] = g[1] = g[2] = 0.0;
+    float Gm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (mu <= 1.0e-12) {
+        if (y2 > 0.0) {
+            g[2] = y2;
+            Gm[8] = (float)rn_inv;
+        }
+    } else {
+        const double mu_tilde = mu * sqrt(rt / rn);
+        const double mu_hat = mu * rt / rn;
+        const double f = 1.0 / (1.0 + mu_tilde * mu_tilde);
+        if (yr <= mu * y2) {
+            g[0] = y0; g[1] = y1; g[2] = y2;
+            Gm[0] = (float)rt_inv; Gm[4] = (float)rt_inv; Gm[8] = (float)rn_inv;
+        } else if ((-mu_hat * yr < y2) && (y2 < yr / mu)) {
+            const double gn = (y2 + mu_hat * yr) * f;
+            g[0] = mu * gn * tx; g[1] = mu * gn * ty; g[2] = gn;
+            if (want_G) {
+                const double p00 = tx * tx, p01 = tx * ty, p11 = ty * ty;
+                const double gy = (yr > 0.0) ? gn / yr : 0.0;
+                const double mf = mu_hat * f;
+                Gm[0] = (float)(mu * (gy * (1.0 - p00) + mf * p00) * rt_inv);
+                Gm[1] = (float)(mu * (gy * (-p01) + mf * p01) * rt_inv);
+                Gm[2] = (float)(mu * f * tx * rn_inv);
+                Gm[3] = (float)(mu * (gy * (-p01) + mf * p01) * rt_inv);
+                Gm[4] = (float)(mu * (gy * (1.0 - p11) + mf * p11) * rt_inv);
+                Gm[5] = (float)(mu * f * ty * rn_inv);
+                Gm[6] = (float)(mf * tx * rt_inv);
+                Gm[7] = (float)(mf * ty * rt_inv);
+                Gm[8] = (float)(f * rn_inv);
+            }
+        }
+    }
+    if (want_G) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) G[e] = Gm[e];
+    }
+    return 0.5 * (rt * (g[0] * g[0] + g[1] * g[1]) + rn * g[2] * g[2]);
+}
+
+// contact-space curvature dvc^T G dvc with G recomputed at vc (exact line search)
+__device__ __forceinline__ double project_curvature(const double* k, const double vc[3], const double dvc[3],
+                                                    double g[3], double* cost) {
+    float G[9];
+    *cost = project_contact(k, vc, g, G, true);
+    double Gd0 = (double)G[0] * dvc[0] + (double)G[1] * dvc[1] + (double)G[2] * dvc[2];
+    double Gd1 = (double)G[3] * dvc[0] + (double)G[4] * dvc[1] + (double)G[5] * dvc[2];
+    double Gd2 = (double)G[6] * dvc[0] + (double)G[7] * dvc[1] + (double)G[8] * dvc[2];
+    return dvc[0] * Gd0 + dvc[1] * Gd1 + dvc[2] * Gd2;
+}
+
+// PD / limit helpers (sap_helpers.py:2251-2340)
+__device__ __forceinline__ double clamp_s(double x, double lim) {
+    if (lim <= 0.0 || !isfinite(lim)) return x;
+    return fmax(-lim, fmin(lim, x));
+}
+__device__ __forceinline__ double clamp_d(double x, double lim) {
+    if (lim <= 0.0 || !isfinite(lim)) return 1.0;
+    return (-lim <= x && x <= lim) ? 1.0 : 0.0;
+}
+__device__ __forceinline__ double clamp_a(double x, double lim) {
+    if (lim <= 0.0 || !isfinite(lim)) return 0.5 * x * x;
+    if (x < -lim) return -lim * (x + 0.5 * lim);
+    if (x <= lim) return 0.5 * x * x;
+    return lim * (x - 0.5 * lim);
+}
+
+// model (PD + limit) terms of dof i at velocity vi: returns cost, adds impulse and hdiag
+__device__ __forceinline__ double model_terms(const Smem& s, int npad, int i, double vi, double dt, double* imp,
+                                              double* hd) {
+    double cost = 0.0;
+    *imp = 0.0;
+    *hd = 0.0;
+    const float* d32 = s.d32;
+    if (s.dflag[I_PD * npad + i]) {
+        const double gain = d32[F_PDG * npad + i];
+        const double lim = d32[F_PDL * npad + i];
+        const double y = (double)d32[F_PDA * npad + i] - gain * vi;
+        *imp += dt * clamp_s(y, lim);
+        *hd += dt * gain * clamp_d(y, lim);
+        if (gain > 0.0) cost += (dt / gain) * clamp_a(y, lim);
+    }
+    if (s.dflag[I_LO * npad + i]) {
+        const double r = d32[F_LOR * npad + i];
+        const double g = (1.0 / r) * ((double)d32[F_LOV * npad + i] - vi);
+        if (g > 0.0) { *imp += g; *hd += 1.0 / r; cost += 0.5 * r * g * g; }
+    }
+    if (s.dflag[I_UP * npad + i]) {
+        const double r = d32[F_UPR * npad + i];
+        const double g = (1.0 / r) * ((double)d32[F_UPV * npad + i] + vi);
+        if (g > 0.0) { *imp -= g; *hd += 1.0 / r; cost += 0.5 * r * g * g; }
+    }
+    return cost;
+}
+
+// ---------------------------------------------------------------------------
+// The fused step kernel
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
+    extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+    const DevModel& M = P.m;
+    const int w = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int n = M.N;
+    const int nb = M.NB;
+    const int nj = M.J;
+    const int npad = P.npad;
+    const int hs = npad + 1;
+    const int js = P.js;
+    const int C = P.C;
+    Smem s = smem_view(smem_raw, P);
+    double* d64 = s.d64;
+    float* d32 = s.d32;
+    const double dt = P.dt_world ? P.dt_world[w] : P.dt;
+    const float h = (float)dt;
+
+    // scratch aliases inside the J region (valid until the Jacobian is built)
+    float* kV = s.J;                 // nb x 6 body velocity
+    float* kA = kV + nb * 6;          // nb x 6 bias acceleration
+    float* kF = kA + nb * 6;          // nb x 6 bias force (incl. external)
+    float* kI = kF + nb * 6;          // nb x 10 inertia about O: I(6 sym), m c(3), m
+    float* kIc = kI + nb * 10;        // nb x 10 composite
+    float* kFc = kIc + nb * 10;       // nb x 6 composite force at body origin
+    float* kSO = kFc + nb * 6;        // n x 6 motion subspace about O
+    float* kFi = kSO + n * 6;         // n x 6 composite inertia * SO
+    float* kExt = kFi + n * 6;        // nb x 6 external wrench (SAP, negated)
+    float* kQdPub = kExt + nb * 6;    // n public velocity
+    float* q_in = s.q;
+    float* q_out = s.q + M.NQ;
+
+    // ------------------------------------------------------------------ 0/1
+    // loads + public -> SAP boundary (sap_helpers.py:1482-1522, 1811-1848, 2058-2072)
+    for (int k = tid; k < M.NQ; k += NT) q_in[k] = P.jq_in[(size_t)w * M.NQ + k];
+    for (int d = tid; d < n; d += NT) {
+        const size_t g = (size_t)w * n + d;
+        kQdPub[d] = P.jqd_in[g];
+        d64[V_V0 * npad + d] = (double)P.jqd_in[g];
+        d32[F_JF * npad + d] = P.c_jf ? P.c_jf[g] : 0.0f;
+    }
+    for (int b = tid; b < nb; b += NT) {
+        float* e = kExt + b * 6;
+        for (int k = 0; k < 6; ++k) e[k] = 0.0f;
+        if (P.bf_in) {
+            const size_t gb = (size_t)w * nb + b;
+            const float* f = P.bf_in + gb * 6;
+            V3 fc = v3(f[0], f[1], f[2]);
+            V3 tc = v3(f[3], f[4], f[5]);
+            TF x = load_tf(P.bq_in + gb * 7);
+            const float* com = M.body_com + gb * 3;
+            V3 c = qrot(qnormalize(x.q), v3(com[0], com[1], com[2]));
+            V3 to = tc + cross(c, fc);
+            e[0] = -to.x; e[1] = -to.y; e[2] = -to.z;
+            e[3] = -fc.x; e[4] = -fc.y; e[5] = -fc.z;
+        }
+    }
+    __syncthreads();
+    for (int j = tid; j < nj; j += NT) {
+        const int jt = M.t_jtype[j];
+        if (jt == kJointFree || jt == kJointDistance) {
+            const int ds = M.t_jds[j];
+            const int child = M.t_jchild[j];
+            const size_t gb = (size_t)w * nb + child;
+            TF x = load_tf(P.bq_in + gb * 7);
+            const float* com = M.body_com + gb * 3;
+            V3 r = qrot(qnormalize(x.q), v3(com[0], com[1], com[2]));
+            const size_t g = (size_t)w * n + ds;
+            double vc0 = P.jqd_in[g + 0], vc1 = P.jqd_in[g + 1], vc2 = P.jqd_in[g + 2];
+            double w0 = P.jqd_in[g + 3], w1 = P.jqd_in[g + 4], w2 = P.jqd_in[g + 5];
+            double rx = r.x, ry = r.y, rz = r.z;
+            d64[V_V0 * npad + ds + 0] = w0;
+            d64[V_V0 * npad + ds + 1] = w1;
+            d64[V_V0 * npad + ds + 2] = w2;
+            d64[V_V0 * npad + ds + 3] = vc0 - (w1 * rz - w2 * ry);
+            d64[V_V0 * npad + ds + 4] = vc1 - (w2 * rx - w0 * rz);
+            d64[V_V0 * npad + ds + 5] = vc2 - (w0 * ry - w1 * rx);
+            if (P.c_jf) {
+                V3 fc = v3(P.c_jf[g], P.c_jf[g + 1], P.c_jf[g + 2]);
+                V3 tc = v3(P.c_jf[g + 3], P.c_jf[g + 4], P.c_jf[g + 5]);
+                V3 to = tc + cross(r, fc);
+                d32[F_JF * npad + ds + 0] = to.x;
+                d32[F_JF * npad + ds + 1] = to.y;
+                d32[F_JF * npad + ds + 2] = to.z;
+                d32[F_JF * npad + ds + 3] = fc.x;
+                d32[F_JF * npad + ds + 4] = fc.y;
+                d32[F_JF * npad + ds + 5] = fc.z;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ 2
+    // forward kinematics + body velocity / bias acceleration by tree level
+    // (free_motion.py:412-530, sap_helpers.py:704-1433)
+    for (int L = 0; L < M.num_levels; ++L) {
+        const int l0 = M.t_level_start[L], l1 = M.t_level_start[L + 1];
+        for (int t = l0 + tid; t < l1; t += NT) {
+            const int j = M.t_level_joint[t];
+            const size_t gj = (size_t)w * nj + j;
+            const int jt = M.t_jtype[j];
+            const int parent = M.t_jparent[j];
+            const int child = M.t_jchild[j];
+            const int qs = M.t_jqs[j];
+            const int ds = M.t_jds[j];
+            TF Xp = load_tf(M.joint_X_p + gj * 7);
+            TF Xc = load_tf(M.joint_X_c + gj * 7);
+            const bool xcid = M.j_xc_identity[gj] != 0;
+            TF Xwpj = Xp;
+            if (parent >= 0) Xwpj = tf_compose_n(load_tf(s.bq + parent * 8), Xp);
+            TF Xwc;
+            SV vPB = sv(v3(0, 0, 0), v3(0, 0, 0));
+            SV aPB = vPB;
+            const float* axp = M.joint_axis + ((size_t)w * n + ds) * 3;
+            if (jt == kJointRevolute && xcid) {
+                V3 axis = v3(axp[0], axp[1], axp[2]);
+                Q4 qfm = q_axis_angle(axis, q_in[qs]);
+                Xwc = TF{Xwpj.p, qmul(Xwpj.q, qfm)};
+                SV S = sv(qrot(Xwpj.q, axis), v3(0, 0, 0));
+                store_sv(s.S + ds * 6, S);
+                vPB = sv_scale(S, (float)d64[V_V0 * npad + ds]);
+            } else {
+                TF Xj = TF{v3(0, 0, 0), Q4{0, 0, 0, 1}};
+                V3 axis = v3(axp[0], axp[1], axp[2]);
+                if (jt == kJointPrismatic) {
+                    Xj.p = axis * q_in[qs];
+                } else if (jt == kJointRevolute) {
+                    Xj.q = q_axis_angle(axis, q_in[qs]);
+                } else if (jt == kJointFree || jt == kJointDistance) {
+                    Xj = TF{v3(q_in[qs], q_in[qs + 1], q_in[qs + 2]),
+                            Q4{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]}};
+                }
+                TF Xwm = tf_compose_n(Xwpj, Xj);
+                Xwc = xcid ? Xwm : tf_compose_n(Xwm, tf_inverse_n(Xc));
+                TF Xmb = tf_inverse_n(Xc);
+                V3 pmb = qrot(Xj.q, Xmb.p);
+                Q4 R = Xwpj.q;
+                if (jt == kJointPrismatic) {
+                    SV S = mob_to_body(R, pmb, sv(v3(0, 0, 0), axis));
+                    store_sv(s.S + ds * 6, S);
+                    float qd = (float)d64[V_V0 * npad + ds];
+                    vPB = sv_scale(S, qd);
+                    aPB = mob_bias_to_body(R, pmb, sv(v3(0, 0, 0), axis * qd));
+                } else if (jt == kJointRevolute) {
+                    SV S = sv(qrot(R, axis), qrot(R, cross(axis, pmb)));
+                    store_sv(s.S + ds * 6, S);
+                    float qd = (float)d64[V_V0 * npad + ds];
+                    V3 wf = axis * qd;
+                    aPB = sv(v3(0, 0, 0), qrot(R, cross(wf, cross(wf, pmb))));
+                    vPB = sv_scale(S, qd);
+                } else if (jt == kJointFree || jt == kJointDistance) {
+                    const double* v0 = d64 + V_V0 * npad + ds;
+                    SV Vf = sv(v3((float)v0[0], (float)v0[1], (float)v0[2]), v3((float)v0[3], (float)v0[4], (float)v0[5]));
+                    vPB = mob_to_body(R, pmb, Vf);
+                    for (int k = 0; k < 3; ++k) {
+                        V3 e = v3(k == 0, k == 1, k == 2);
+                        store_sv(s.S + (ds + k) * 6, mob_to_body(R, pmb, sv(e, v3(0, 0, 0))));
+                        store_sv(s.S + (ds + 3 + k) * 6, mob_to_body(R, pmb, sv(v3(0, 0, 0), e)));
+                    }
+                    aPB = mob_bias_to_body(R, pmb, Vf);
+                }
+            }
+            float* bqc = s.bq + child * 8;
+            bqc[0] = Xwc.p.x; bqc[1] = Xwc.p.y; bqc[2] = Xwc.p.z;
+            bqc[3] = Xwc.q.x; bqc[4] = Xwc.q.y; bqc[5] = Xwc.q.z; bqc[6] = Xwc.q.w;
+            SV V = vPB, A = aPB;
+            if (parent >= 0) {
+                V3 pPB = Xwc.p - v3(s.bq[parent * 8], s.bq[parent * 8 + 1], s.bq[parent * 8 + 2]);
+                SV Vp = load_sv(kV + parent * 6);
+                SV Ap = load_sv(kA + parent * 6);
+                V = compose_velocity(Vp, pPB, vPB);
+                A = compose_acceleration(Ap, Vp, pPB, vPB, aPB);
+            }
+            store_sv(kV + child * 6, V);
+            store_sv(kA + child * 6, A);
+        }
+        __syncthreads();
+    }
+
+    // ------------------------------------------------------------------ 3
+    // body dynamics (free_motion.py:680-719) + inertia about O = root origin
+    const V3 O = v3(s.bq[0], s.bq[1], s.bq[2]);
+    for (int b = tid; b < nb; b += NT) {
+        const size_t gb = (size_t)w * nb + b;
+        const float m = M.body_mass[gb];
+        const float* Ip = M.body_inertia + gb * 9;
+        const float* cp = M.body_com + gb * 3;
+        const float* g = M.gravity + (size_t)w * 3;
+        const float* bqb = s.bq + b * 8;
+        Q4 q = Q4{bqb[3], bqb[4], bqb[5], bqb[6]};
+        V3 c0 = qrot(q, v3(1, 0, 0)), c1 = qrot(q, v3(0, 1, 0)), c2 = qrot(q, v3(0, 0, 1));
+        // I_W = R I R^T
+        float Rm[3][3] = {{c0.x, c1.x, c2.x}, {c0.y, c1.y, c2.y}, {c0.z, c1.z, c2.z}};
+        float T[3][3], IW[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 3; ++cc)
+                T[r][cc] = Rm[r][0] * Ip[0 * 3 + cc] + Rm[r][1] * Ip[1 * 3 + cc] + Rm[r][2] * Ip[2 * 3 + cc];
+        for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 3; ++cc)
+                IW[r][cc] = T[r][0] * Rm[cc][0] + T[r][1] * Rm[cc][1] + T[r][2] * Rm[cc][2];
+        V3 c = qrot(q, v3(cp[0], cp[1], cp[2]));
+        float cc2 = dot(c, c);
+        float IB[3][3];
+        float cv_[3] = {c.x, c.y, c.z};
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) IB[r][k] = IW[r][k] + m * ((r == k ? cc2 : 0.0f) - cv_[r] * cv_[k]);
+        SV V = load_sv(kV + b * 6);
+        SV A = load_sv(kA + b * 6);
+        V3 Iw = v3(IB[0][0] * V.w.x + IB[0][1] * V.w.y + IB[0][2] * V.w.z,
+                   IB[1][0] * V.w.x + IB[1][1] * V.w.y + IB[1][2] * V.w.z,
+                   IB[2][0] * V.w.x + IB[2][1] * V.w.y + IB[2][2] * V.w.z);
+        V3 Ia = v3(IB[0][0] * A.w.x + IB[0][1] * A.w.y + IB[0][2] * A.w.z,
+                   IB[1][0] * A.w.x + IB[1][1] * A.w.y + IB[1][2] * A.w.z,
+                   IB[2][0] * A.w.x + IB[2][1] * A.w.y + IB[2][2] * A.w.z);
+        // I_s * A = [I_B alpha + m c x a, -m c x alpha + m a]
+        V3 fw = Ia + cross(c, A.v) * m;
+        V3 fv = (A.v - cross(c, A.w)) * m;
+        // bias [w x I_B w, m w x (w x c)]
+        fw = fw + cross(V.w, Iw);
+        fv = fv + cross(V.w, cross(V.w, c)) * m;
+        // gravity [c x m g, m g]
+        V3 mg = v3(g[0], g[1], g[2]) * m;
+        fw = fw - cross(c, mg);
+        fv = fv - mg;
+        const float* e = kExt + b * 6;
+        float* F = kF + b * 6;
+        F[0] = fw.x + e[0]; F[1] = fw.y + e[1]; F[2] = fw.z + e[2];
+        F[3] = fv.x + e[3]; F[4] = fv.y + e[4]; F[5] = fv.z + e[5];
+        // inertia about O: c_O = (p_b - O) + c
+        V3 co = (v3(bqb[0], bqb[1], bqb[2]) - O) + c;
+        float co2 = dot(co, co);
+        float co_[3] = {co.x, co.y, co.z};
+        float* I = kI + b * 10;
+        int e6 = 0;
+        for (int r = 0; r < 3; ++r)
+            for (int k = r; k < 3; ++k) I[e6++] = IW[r][k] + m * ((r == k ? co2 : 0.0f) - co_[r] * co_[k]);
+        I[6] = m * co.x; I[7] = m * co.y; I[8] = m * co.z; I[9] = m;
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ 4
+    // composites over subtrees: inertia about O, force at body origin
+    for (int t = tid; t < nb * 10; t += NT) {
+        const int c = t / 10, k = t % 10;
+        const unsigned long long sub = M.t_body_sub[c];
+        float acc = 0.0f;
+        for (int b = 0; b < nb; ++b)
+            if (bit(sub, b)) acc += kI[b * 10 + k];
+        kIc[c * 10 + k] = acc;
+    }
+    for (int c = tid; c < nb; c += NT) {
+        const unsigned long long sub = M.t_body_sub[c];
+        V3 pc = v3(s.bq[c * 8], s.bq[c * 8 + 1], s.bq[c * 8 + 2]);
+        V3 tw = v3(0, 0, 0), fv = v3(0, 0, 0);
+        for (int b = 0; b < nb; ++b) {
+            if (!bit(sub, b)) continue;
+            const float* F = kF + b * 6;
+            V3 f = v3(F[3], F[4], F[5]);
+            V3 pb = v3(s.bq[b * 8], s.bq[b * 8 + 1], s.bq[b * 8 + 2]);
+            // shift_force(F_b, p_c - p_b) = [tau - (p_c - p_b) x f, f] (sap_helpers.py:962-966)
+            tw = tw + v3(F[0], F[1], F[2]) - cross(pc - pb, f);
+            fv = fv + f;
+        }
+        float* Fc = kFc + c * 6;
+        Fc[0] = tw.x; Fc[1] = tw.y; Fc[2] = tw.z; Fc[3] = fv.x; Fc[4] = fv.y; Fc[5] = fv.z;
+    }
+    __syncthreads();
+    for (int d = tid; d < n; d += NT) {
+        const int j = M.t_dof_joint[d];
+        const int c = M.t_jchild[j];
+        SV S = load_sv(s.S + d * 6);
+        // tau = joint_f - S . f_s  (free_motion.py:93-147, sap_helpers.py:2090-2118)
+        d32[F_TAU * npad + d] = d32[F_JF * npad + d] - sv_dot(S, load_sv(kFc + c * 6));
+        // S about O
+        V3 pc = v3(s.bq[c * 8], s.bq[c * 8 + 1], s.bq[c * 8 + 2]);
+        SV SO = shift_velocity(S, O - pc);
+        store_sv(kSO + d * 6, SO);
+        // composite inertia of the joint's child subtree times SO
+        const float* I = kIc + c * 10;
+        V3 hv = v3(I[6], I[7], I[8]);
+        V3 Iw = v3(I[0] * SO.w.x + I[1] * SO.w.y + I[2] * SO.w.z, I[1] * SO.w.x + I[3] * SO.w.y + I[4] * SO.w.z,
+                   I[2] * SO.w.x + I[4] * SO.w.y + I[5] * SO.w.z);
+        SV Fi = sv(Iw + cross(hv, SO.v), SO.v * I[9] - cross(hv, SO.w));
+        store_sv(kFi + d * 6, Fi);
+    }
+    __syncthreads();
+    // mass matrix by composite rigid bodies: H_ij = (Ic S_i) . S_j for joint(j) on
+    // joint(i)'s root path (equals J^T M J of free_motion.py:843-1179)
+    for (int t = tid; t < npad * npad; t += NT) {
+        const int i = t / npad, j = t % npad;
+        float val;
+        if (i < n && j < n) {
+            const int ji = M.t_dof_joint[i], jj = M.t_dof_joint[j];
+            const unsigned long long anc_i = M.t_body_anc[M.t_jchild[ji]];
+            const unsigned long long anc_j = M.t_body_anc[M.t_jchild[jj]];
+            val = 0.0f;
+            if (bit(anc_i, jj)) val = sv_dot(load_sv(kFi + i * 6), load_sv(kSO + j * 6));
+            else if (bit(anc_j, ji)) val = sv_dot(load_sv(kFi + j * 6), load_sv(kSO + i * 6));
+            if (i == j) val += M.armature[(size_t)w * n + i];
+            s.A[i * npad + j] = val;
+        } else {
+            val = (i == j) ? 1.0f : 0.0f;
+        }
+        s.H[i * hs + j] = val;
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ 5
+    // free motion v* = v0 + dt A^-1 tau with one refinement step (free_motion.py:1970-2178)
+    cholesky_blocked<NT>(s.H, npad, hs);
+    for (int d = tid; d < npad; d += NT) d64[V_T1 * npad + d] = (d < n) ? (double)d32[F_TAU * npad + d] : 0.0;
+    __syncthreads();
+    chol_solve_wave0(s.H, hs, n, d64 + V_T1 * npad);
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) {
+        double r = (double)d32[F_TAU * npad + i];
+        for (int j = 0; j < n; ++j) r -= (double)s.A[i * npad + j] * d64[V_T1 * npad + j];
+        d64[V_T2 * npad + i] = r;
+    }
+    __syncthreads();
+    chol_solve_wave0(s.H, hs, n, d64 + V_T2 * npad);
+    __syncthreads();
+    for (int d = tid; d < npad; d += NT) {
+        double qdd = (d < n) ? d64[V_T1 * npad + d] + d64[V_T2 * npad + d] : 0.0;
+        double v0 = (d < n) ? d64[V_V0 * npad + d] : 0.0;
+        d64[V_VS * npad + d] = v0 + dt * qdd;
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ 6
+    // contacts of this world (contact_jacobian.py:404-562, sap_helpers.py:148-503)
+    int nc = 0;
+    int c_begin = 0;
+    if (P.cworld_start) {
+        c_begin = P.cworld_start[w];
+        nc = P.cworld_start[w + 1] - c_begin;
+    }
+    if (nc > C) nc = C;
+    if (nc < 0) nc = 0;
+    float* cx0 = s.H;            // C x 3 witness 0 (world)
+    float* cx1 = cx0 + C * 3;    // witness 1
+    float* cpc = cx1 + C * 3;    // contact point
+    float* cR = cpc + C * 3;     // C x 9 frame columns cx, cy, n
+    double* cphi = reinterpret_cast<double*>(cR + C * 9);  // C x 4: phi0, k, tau, mu (raw)
+    for (int c = tid; c < nc; c += NT) {
+        const int idx = c_begin + c;
+        const int sh0 = P.cshape0[idx], sh1 = P.cshape1[idx];
+        const int b0g = sh0 >= 0 ? M.shape_body[sh0] : -1;
+        const int b1g = sh1 >= 0 ? M.shape_body[sh1] : -1;
+        const int lb0 = b0g >= 0 ? b0g - w * nb : -1;
+        const int lb1 = b1g >= 0 ? b1g - w * nb : -1;
+        s.cb[c * 2] = lb0;
+        s.cb[c * 2 + 1] = lb1;
+        V3 x0 = v3(P.cpoint0[idx * 3], P.cpoint0[idx * 3 + 1], P.cpoint0[idx * 3 + 2]);
+        V3 x1 = v3(P.cpoint1[idx * 3], P.cpoint1[idx * 3 + 1], P.cpoint1[idx * 3 + 2]);
+        if (lb0 >= 0) {
+            const float* bb = s.bq + lb0 * 8;
+            x0 = v3(bb[0], bb[1], bb[2]) + qrot(qnormalize(Q4{bb[3], bb[4], bb[5], bb[6]}), x0);
+        }
+        if (lb1 >= 0) {
+            const float* bb = s.bq + lb1 * 8;
+            x1 = v3(bb[0], bb[1], bb[2]) + qrot(qnormalize(Q4{bb[3], bb[4], bb[5], bb[6]}), x1);
+        }
+        V3 nrm = v3(P.cnormal[idx * 3], P.cnormal[idx * 3 + 1], P.cnormal[idx * 3 + 2]);
+        float n2 = dot(nrm, nrm);
+        nrm = (n2 <= 1.0e-32f) ? v3(0, 0, 0) : nrm * (1.0f / sqrtf(n2));
+        // frame: MakeFromOneUnitVector (sap_helpers.py:467-503)
+        float ax = fabsf(nrm.x), ay = fabsf(nrm.y), az = fabsf(nrm.z);
+        int im = 0;
+        float mn = ax;
+        if (ay < mn) { im = 1; mn = ay; }
+        if (az < mn) { im = 2; mn = az; }
+        float mag = sqrtf(1.0f - mn * mn);
+        float rr = 1.0f / mag;
+        float u = im == 0 ? nrm.x : (im == 1 ? nrm.y : nrm.z);
+        float sg = -rr * u;
+        V3 tx, ty;
+        if (im == 0) { tx = v3(0.0f, -rr * nrm.z, rr * nrm.y); ty = v3(mag, sg * nrm.y, sg * nrm.z); }
+        else if (im == 1) { tx = v3(rr * nrm.z, 0.0f, -rr * nrm.x); ty = v3(sg * nrm.x, mag, sg * nrm.z); }
+        else { tx = v3(-rr * nrm.y, rr * nrm.x, 0.0f); ty = v3(sg * nrm.x, sg * nrm.y, mag); }
+        // materials (sap_helpers.py:148-237)
+        double k0 = P.fallback_k, k1 = P.fallback_k;
+        if (sh0 >= 0) k0 = M.shape_ke[sh0];
+        if (!isfinite(k0) || k0 <= 0.0) k0 = P.fallback_k;
+        if (!isfinite(k0) || k0 <= 0.0) k0 = 1.0;
+        if (sh1 >= 0) k1 = M.shape_ke[sh1];
+        if (!isfinite(k1) || k1 <= 0.0) k1 = P.fallback_k;
+        if (!isfinite(k1) || k1 <= 0.0) k1 = 1.0;
+        double kden = k0 + k1;
+        double w0 = 0.5, w1 = 0.5;
+        if (kden != 0.0) { w0 = k0 / kden; w1 = k1 / kden; }
+        V3 pc = x0 * (float)w0 + x1 * (float)w1;
+        double kc;
+        if (!isfinite(k0)) kc = k1;
+        else if (!isfinite(k1)) kc = k0;
+        else if (kden <= 1.0e-12) kc = fmax(k0, k1);
+        else kc = k0 * k1 / kden;
+        double mu0 = P.fallback_mu, mu1 = P.fallback_mu;
+        if (sh0 >= 0) mu0 = M.shape_mu[sh0];
+        if (sh1 >= 0) mu1 = M.shape_mu[sh1];
+        mu0 = fmax(mu0, 0.0);
+        mu1 = fmax(mu1, 0.0);
+        double mud = mu0 + mu1;
+        double muc = (mud <= 1.0e-12) ? 0.0 : 2.0 * mu0 * mu1 / mud;
+        double t0 = P.fallback_tau, t1 = P.fallback_tau;
+        if (sh0 >= 0) t0 = M.shape_tau[sh0];
+        if (!isfinite(t0) || t0 < 0.0) t0 = P.fallback_tau;
+        if (!isfinite(t0) || t0 < 0.0) t0 = 0.0;
+        if (sh1 >= 0) t1 = M.shape_tau[sh1];
+        if (!isfinite(t1) || t1 < 0.0) t1 = P.fallback_tau;
+        if (!isfinite(t1) || t1 < 0.0) t1 = 0.0;
+        double phi = ((double)x1.x - x0.x) * nrm.x + ((double)x1.y - x0.y) * nrm.y + ((double)x1.z - x0.z) * nrm.z -
+                     (double)P.cmargin0[idx] - (double)P.cmargin1[idx];
+        cx0[c * 3] = x0.x; cx0[c * 3 + 1] = x0.y; cx0[c * 3 + 2] = x0.z;
+        cx1[c * 3] = x1.x; cx1[c * 3 + 1] = x1.y; cx1[c * 3 + 2] = x1.z;
+        cpc[c * 3] = pc.x; cpc[c * 3 + 1] = pc.y; cpc[c * 3 + 2] = pc.z;
+        float* R = cR + c * 9;
+        R[0] = tx.x; R[1] = tx.y; R[2] = tx.z; R[3] = ty.x; R[4] = ty.y; R[5] = ty.z;
+        R[6] = nrm.x; R[7] = nrm.y; R[8] = nrm.z;
+        cphi[c * 4] = phi;
+        cphi[c * 4 + 1] = kc;
+        cphi[c * 4 + 2] = t0 + t1;
+        cphi[c * 4 + 3] = muc;
+    }
+    __syncthreads();
+    // contact Jacobian: J_c[:, d] = R^T (u1 - u0),  u_s = S_v + S_w x (x_s - p_child(d))
+    // for d on body s's root path (contact_jacobian.py:519-562)
+    for (int t = tid; t < nc * js; t += NT) {
+        const int c = t / js, d = t % js;
+        float j0 = 0.0f, j1 = 0.0f, j2 = 0.0f;
+        if (d < n) {
+            const int jd = M.t_dof_joint[d];
+            const int cd = M.t_jchild[jd];
+            SV S = load_sv(s.S + d * 6);
+            V3 pcd = v3(s.bq[cd * 8], s.bq[cd * 8 + 1], s.bq[cd * 8 + 2]);
+            V3 rel = v3(0, 0, 0);
+            const int lb0 = s.cb[c * 2], lb1 = s.cb[c * 2 + 1];
+            const bool wit = P.point_mode == 0;
+            if (lb1 >= 0 && bit(M.t_body_anc[lb1], jd)) {
+                V3 xs = wit ? v3(cx1[c * 3], cx1[c * 3 + 1], cx1[c * 3 + 2]) : v3(cpc[c * 3], cpc[c * 3 + 1], cpc[c * 3 + 2]);
+                rel = rel + S.v + cross(S.w, xs - pcd);
+            }
+            if (lb0 >= 0 && bit(M.t_body_anc[lb0], jd)) {
+                V3 xs = wit ? v3(cx0[c * 3], cx0[c * 3 + 1], cx0[c * 3 + 2]) : v3(cpc[c * 3], cpc[c * 3 + 1], cpc[c * 3 + 2]);
+                rel = rel - (S.v + cross(S.w, xs - pcd));
+            }
+            const float* R = cR + c * 9;
+            j0 = R[0] * rel.x + R[1] * rel.y + R[2] * rel.z;
+            j1 = R[3] * rel.x + R[4] * rel.y + R[5] * rel.z;
+            j2 = R[6] * rel.x + R[7] * rel.y + R[8] * rel.z;
+        }
+        s.J[(c * 3 + 0) * js + d] = j0;
+        s.J[(c * 3 + 1) * js + d] = j1;
+        s.J[(c * 3 + 2) * js + d] = j2;
+    }
+    // diag(A) data (contact_solve.py:405-419)
+    for (int d = tid; d < npad; d += NT) {
+        double a = (d < n) ? (double)s.A[d * npad + d] : 1.0;
+        if (a < 1.0e-12 || !isfinite(a)) a = 1.0e-12;
+        d32[F_AINV * npad + d] = (float)(1.0 / a);
+        d32[F_DSC * npad + d] = (float)(1.0 / sqrt(a));
+    }
+    __syncthreads();
+    // effective contact weights + regularisation constants (contact_jacobian.py:153-251,
+    // contact_solve.py:791-822)
+    const double bf = P.beta * P.beta / (4.0 * kPi * kPi);
+    for (int c = tid; c < nc; c += NT) {
+        double wv;
+        if (P.weight_mode == 0) {
+            const float* R = cR + c * 9;
+            V3 dirs[3] = {v3(R[0], R[1], R[2]), v3(R[3], R[4], R[5]), v3(R[6], R[7], R[8])};
+            double acc = 0.0;
+            for (int side = 0; side < 2; ++side) {
+                const int lb = s.cb[c * 2 + side];
+                if (lb < 0) continue;
+                const size_t gb = (size_t)w * nb + lb;
+                const double m = M.body_mass[gb];
+                if (m <= 1.0e-12) continue;
+                const float* bb = s.bq + lb * 8;
+                Q4 q = qnormalize(Q4{bb[3], bb[4], bb[5], bb[6]});
+                const float* com = M.body_com + gb * 3;
+                V3 comw = v3(bb[0], bb[1], bb[2]) + qrot(q, v3(com[0], com[1], com[2]));
+                const float* xs = side == 0 ? cx0 + c * 3 : cx1 + c * 3;
+                V3 r = v3(xs[0], xs[1], xs[2]) - comw;
+                const float* I = M.body_inertia + gb * 9;
+                double ix = fmax((double)I[0], 1.0e-12), iy = fmax((double)I[4], 1.0e-12), iz = fmax((double)I[8], 1.0e-12);
+                for (int k = 0; k < 3; ++k) {
+                    V3 dn = dirs[k];
+                    float dl = sqrtf(dot(dn, dn));
+                    dn = (dl * dl <= 1.0e-32f) ? v3(0, 0, 0) : dn * (1.0f / dl);
+                    V3 cb = qrot(qconj(q), cross(r, dn));
+                    acc += 1.0 / m + (double)cb.x * cb.x / ix + (double)cb.y * cb.y / iy + (double)cb.z * cb.z / iz;
+                }
+            }
+            wv = acc / 3.0;
+        } else {
+            double W00 = 0, W01 = 0, W02 = 0, W11 = 0, W12 = 0, W22 = 0;
+            for (int d = 0; d < n; ++d) {
+                const double ai = d32[F_AINV * npad + d];
+                const double a0 = s.J[(c * 3) * js + d], a1 = s.J[(c * 3 + 1) * js + d], a2 = s.J[(c * 3 + 2) * js + d];
+                W00 += ai * a0 * a0; W01 += ai * a0 * a1; W02 += ai * a0 * a2;
+                W11 += ai * a1 * a1; W12 += ai * a1 * a2; W22 += ai * a2 * a2;
+            }
+            wv = sqrt(W00 * W00 + W11 * W11 + W22 * W22 + 2.0 * (W01 * W01 + W02 * W02 + W12 * W12)) / 3.0;
+        }
+        if (wv < 1.0e-12 || !isfinite(wv)) wv = 1.0e-12;
+        double kc = cphi[c * 4 + 1];
+        if (kc <= 0.0 || !isfinite(kc)) kc = 1.0;
+        const double tau_c = fmax(cphi[c * 4 + 2], 0.0);
+        const double rn_soft = 1.0 / (dt * kc * (dt + tau_c));
+        const double rn = fmax(fmax(bf * wv, rn_soft), 1.0e-30);
+        const double rt = fmax(P.sigma * wv, 1.0e-30);
+        double mu = cphi[c * 4 + 3];
+        if (mu < 0.0 || !isfinite(mu)) mu = 0.0;
+        s.ck[c * 4 + 0] = rt;
+        s.ck[c * 4 + 1] = rn;
+        s.ck[c * 4 + 2] = -cphi[c * 4] / (dt + tau_c);
+        s.ck[c * 4 + 3] = mu;
+        if (P.dbg_phi0) P.dbg_phi0[(size_t)w * C + c] = cphi[c * 4];
+        if (P.dbg_w) P.dbg_w[(size_t)w * C + c] = wv;
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ 7
+    // PD + limit terms (contact_solve.py:490-683) and participating dofs (422-487)
+    for (int i = tid; i < npad; i += NT) {
+        int pd = 0, lo = 0, up = 0, part = 0;
+        float pda = 0, pdg = 0, pdl = 0, lov = 0, upv = 0, lor = 1, upr = 1;
+        if (i < n) {
+            const int t = M.t_dof_target[i];
+            const size_t gt = (size_t)w * n + t;
+            const size_t gi = (size_t)w * n + i;
+            const int coord = M.t_dof_coord[i];
+            const double ainv = d32[F_AINV * npad + i];
+            if (M.target_mode[gt] != 0) {
+                const double kp = M.target_ke[gt], kd = M.target_kd[gt];
+                if (!(kp <= 0.0 && kd <= 0.0) && !(coord < 0 && kp > 0.0)) {
+                    // effective gains (sap_helpers.py:2305-2340)
+                    const double pbf = kPdBeta * kPdBeta / (4.0 * kPi * kPi);
+                    const double r_nr = pbf * fmax(ainv, 1.0e-12);
+                    const double common = dt * kp + kd;
+                    double kpe = 0.0, kde = 0.0;
+                    if (common > 0.0) {
+                        const double r = 1.0 / (dt * common);
+                        if (r >= r_nr) { kpe = kp; kde = kd; }
+                        else if (dt * kp > kd && kp > 0.0) {
+                            const double tau = kd / kp;
+                            kpe = 1.0 / (dt * (dt + tau) * r_nr);
+                            kde = tau * kpe;
+                        } else if (kd > 0.0) {
+                            const double tinv = kp / kd;
+                            kde = 1.0 / (dt * (1.0 + dt * tinv) * r_nr);
+                            kpe = tinv * kde;
+                        } else {
+                            kpe = 1.0 / (dt * dt * r_nr);
+                            kde = 0.0;
+                        }
+                    }
+                    const double gain = dt * kpe + kde;
+                    if (gain > 0.0) {
+                        double q0 = 0.0, qt = 0.0;
+                        if (coord >= 0) {
+                            q0 = q_in[coord];
+                            qt = P.c_tpos ? P.c_tpos[gt] : 0.0f;
+                        }
+                        const double vd = P.c_tvel ? P.c_tvel[gt] : 0.0f;
+                        const double u0 = P.c_act ? P.c_act[gt] : 0.0f;
+                        pd = 1;
+                        pda = (float)(kpe * (qt - q0) + kde * vd + u0);
+                        pdg = (float)gain;
+                        pdl = M.effort_limit[gt];
+                    }
+                }
+            }
+            if (coord >= 0 && M.t_dof_limit_ok[i]) {
+                const double ke = M.limit_ke[gi];
+                if (ke > 0.0) {
+                    const double q0 = q_in[coord];
+                    const double lower = M.limit_lower[gi], upper = M.limit_upper[gi];
+                    const double window = kLimitWindow * dt *
+                                          fmax(fabs(d64[V_V0 * npad + i]), fabs(d64[V_VS * npad + i]));
+                    const double kd = M.limit_kd[gi];
+                    const double tau_d = kd > 0.0 ? kd / ke : 0.0;
+                    const double lbf = kLimitBeta * kLimitBeta / (4.0 * kPi * kPi);
+                    const double r = fmax(lbf * fmax(ainv, 1.0e-12), 1.0 / (dt * ke * (dt + tau_d)));
+                    if (isfinite(lower) && q0 - lower <= window) {
+                        lo = 1; lov = (float)(-(q0 - lower) / (dt + tau_d)); lor = (float)r;
+                    }
+                    if (isfinite(upper) && upper - q0 <= window) {
+                        up = 1; upv = (float)(-(upper - q0) / (dt + tau_d)); upr = (float)r;
+                    }
+                }
+            }
+            part = pd | lo | up;
+            for (int c = 0; c < nc && !part; ++c) {
+                for (int side = 0; side < 2; ++side) {
+                    const int lb = s.cb[c * 2 + side];
+                    if (lb >= 0) {
+                        const int st = M.t_body_art_start[lb], ct = M.t_body_art_count[lb];
+                        if (st >= 0 && ct > 0 && i >= st && i < st + ct) part = 1;
+                    }
+                }
+                if (s.J[(c * 3) * js + i] != 0.0f || s.J[(c * 3 + 1) * js + i] != 0.0f ||
+                    s.J[(c * 3 + 2) * js + i] != 0.0f)
+                    part = 1;
+            }
+        }
+        s.dflag[I_PART * npad + i] = part;
+        s.dflag[I_PD * npad + i] = pd;
+        s.dflag[I_LO * npad + i] = lo;
+        s.dflag[I_UP * npad + i] = up;
+        d32[F_PDA * npad + i] = pda;
+        d32[F_PDG * npad + i] = pdg;
+        d32[F_PDL * npad + i] = pdl;
+        d32[F_LOV * npad + i] = lov;
+        d32[F_UPV * npad + i] = upv;
+        d32[F_LOR * npad + i] = lor;
+        d32[F_UPR * npad + i] = upr;
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ 8
+    // SAP Newton solve (contact_solve.py:6129-6285, 5474-5655)
+    int any_model = 0;
+    for (int i = tid; i < n; i += NT)
+        any_model |= s.dflag[I_PD * npad + i] | s.dflag[I_LO * npad + i] | s.dflag[I_UP * npad + i];
+    any_model = block_or<NT>(any_model, s.red);
+    double* v = d64 + V_V * npad;
+    double* vs = d64 + V_VS * npad;
+    double* dvv = d64 + V_DV * npad;
+    double* grad = d64 + V_G * npad;
+    double* jimp = d64 + V_JI * npad;
+    double* dp = d64 + V_DP * npad;
+    double* av = d64 + V_AV * npad;
+    const bool unconstrained = (nc == 0) && !any_model;
+    for (int i = tid; i < npad; i += NT) {
+        double vi = 0.0;
+        if (i < n) {
+            vi = P.use_guess ? P.v_guess[(size_t)w * n + i] : d64[V_V0 * npad + i];
+            if (unconstrained || !s.dflag[I_PART * npad + i]) vi = vs[i];
+        }
+        v[i] = vi;
+        dvv[i] = 0.0;
+    }
+    __syncthreads();
+    int iteration = 0, ls_total = 0, flags = 0, ls_status = 0;
+    if (unconstrained) {
+        flags = 1;
+    } else {
+        double alpha = 1.0, prev_cost = 0.0;
+        const double cost_min_alpha = 0.5;
+        while (true) {
+            // ---- evaluate at v: vc = J v, projection, model terms, gradient
+            for (int r = tid; r < 3 * nc; r += NT) {
+                double acc = 0.0;
+                const float* Jr = s.J + r * js;
+                for (int d = 0; d < n; ++d) acc += (double)Jr[d] * v[d];
+                s.cv[(r / 3) * 9 + (r % 3)] = acc;
+            }
+            __syncthreads();
+            double part3[4] = {0.0, 0.0, 0.0, 0.0};  // contact cost + model cost
+            for (int c = tid; c < nc; c += NT) {
+                double vc[3] = {s.cv[c * 9], s.cv[c * 9 + 1], s.cv[c * 9 + 2]};
+                double g[3];
+                part3[0] += project_contact(s.ck + c * 4, vc, g, s.cG + c * 9, true);
+                s.cv[c * 9 + 6] = g[0]; s.cv[c * 9 + 7] = g[1]; s.cv[c * 9 + 8] = g[2];
+            }
+            for (int i = tid; i < n; i += NT) {
+                double imp, hd;
+                part3[0] += model_terms(s, npad, i, v[i], dt, &imp, &hd);
+                d32[F_HD * npad + i] = (float)hd;
+                dp[i] = imp;  // temporarily holds the model impulse
+            }
+            __syncthreads();
+            for (int i = tid; i < n; i += NT) {
+                double jc = dp[i];
+                for (int c = 0; c < nc; ++c) {
+                    jc += (double)s.J[(c * 3) * js + i] * s.cv[c * 9 + 6] + (double)s.J[(c * 3 + 1) * js + i] * s.cv[c * 9 + 7] +
+                          (double)s.J[(c * 3 + 2) * js + i] * s.cv[c * 9 + 8];
+                }
+                double ar = 0.0, avv = 0.0;
+                const float* Ai = s.A + i * npad;
+                for (int j = 0; j < n; ++j) {
+                    const double a = Ai[j];
+                    ar += a * (v[j] - vs[j]);
+                    avv += a * v[j];
+                }
+                jimp[i] = jc;
+                grad[i] = ar - jc;
+                av[i] = avv;
+                part3[0] += 0.5 * (v[i] - vs[i]) * ar;
+                if (s.dflag[I_PART * npad + i]) {
+                    const double sc = d32[F_DSC * npad + i];
+                    part3[1] += (sc * grad[i]) * (sc * grad[i]);
+                    part3[2] += (sc * avv) * (sc * avv);
+                    part3[3] += (sc * jc) * (sc * jc);
+                }
+            }
+            block_sum<NT>(part3, 4, s.red);
+            const double cost = part3[0];
+            const double gnorm = sqrt(fmax(part3[1], 0.0));
+            const double pnorm = sqrt(fmax(part3[2], 0.0));
+            const double jnorm = sqrt(fmax(part3[3], 0.0));
+            // ---- convergence (contact_solve.py:1970-2087)
+            const double opt_tol = P.opt_abs + P.opt_rel * fmax(pnorm, jnorm);
+            bool opt = gnorm <= opt_tol;
+            bool costok = false;
+            if (iteration > 0 && alpha > cost_min_alpha) {
+                const double scale = 0.5 * (fabs(cost) + fabs(prev_cost));
+                if (fabs(cost - prev_cost) < P.cost_abs + P.cost_rel * scale) costok = true;
+            }
+            if (opt || costok) {
+                flags = 1 | (opt ? 2 : 0) | (costok ? 4 : 0);
+                break;
+            }
+            if (iteration >= P.max_iter) {
+                flags = 8;
+                break;
+            }
+            // ---- Hessian H = A + J^T G J + diag(hd) with MFMA (contact_solve.py:1334-1654)
+            {
+                constexpr int NWV = NT / 64;
+                const int lane = tid & 63, wave = tid >> 6;
+                const int tb = npad >> 4;
+                const int ntile = tb * (tb + 1) / 2;
+                const int K = 3 * nc;
+                for (int tt = wave; tt < ntile; tt += NWV) {
+                    int ib = 0, rem = tt;
+                    while (rem > ib) { rem -= ib + 1; ++ib; }
+                    const int jb = rem;
+                    const int i0 = ib * 16, j0 = jb * 16;
+                    typedef float f4 __attribute__((ext_vector_type(4)));
+                    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                    const int ia = i0 + (lane & 15);
+                    const int jbb = j0 + (lane & 15);
+                    for (int k0 = 0; k0 < K; k0 += 4) {
+                        const int k = k0 + (lane >> 4);
+                        float a = 0.0f, b = 0.0f;
+                        if (k < K) {
+                            const int c = k / 3, r = k % 3;
+                            if (ia < n) a = s.J[k * js + ia];
+                            if (jbb < n) {
+                                const float* G = s.cG + c * 9 + r * 3;
+                                const float* Jc = s.J + (c * 3) * js + jbb;
+                                b = G[0] * Jc[0] + G[1] * Jc[js] + G[2] * Jc[2 * js];
+                            }
+                        }
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int i = i0 + 4 * (lane >> 4) + e;
+                        const int j = j0 + (lane & 15);
+                        float val;
+                        if (i < n && j < n) {
+                            val = s.A[i * npad + j] + acc[e];
+                            if (i == j) val += d32[F_HD * npad + i] + (float)P.diag_shift;
+                        } else {
+                            val = (i == j) ? 1.0f : 0.0f;
+                        }
+                        s.H[i * hs + j] = val;
+                    }
+                }
+            }
+            __syncthreads();
+            cholesky_blocked<NT>(s.H, npad, hs);
+            for (int i = tid; i < npad; i += NT) dvv[i] = (i < n) ? -grad[i] : 0.0;
+            __syncthreads();
+            chol_solve_wave0(s.H, hs, n, dvv);
+            __syncthreads();
+            // ---- line search data (contact_solve.py:2234-2263)
+            for (int r = tid; r < 3 * nc; r += NT) {
+                double acc = 0.0;
+                const float* Jr = s.J + r * js;
+                for (int d = 0; d < n; ++d) acc += (double)Jr[d] * dvv[d];
+                s.cv[(r / 3) * 9 + 3 + (r % 3)] = acc;
+            }
+            double ld[4] = {0, 0, 0, 0};  // dell0, dell_a0, d2ell_a, m0
+            for (int i = tid; i < n; i += NT) {
+                double acc = 0.0;
+                const float* Ai = s.A + i * npad;
+                for (int j = 0; j < n; ++j) acc += (double)Ai[j] * dvv[j];
+                dp[i] = acc;
+                const double ri = v[i] - vs[i];
+                ld[0] += grad[i] * dvv[i];
+                ld[1] += acc * ri;
+                ld[2] += dvv[i] * acc;
+                double ar = 0.0;
+                for (int j = 0; j < n; ++j) ar += (double)Ai[j] * (v[j] - vs[j]);
+                ld[3] += 0.5 * ri * ar;
+            }
+            block_sum<NT>(ld, 4, s.red);
+            const double dell0 = ld[0], dell_a0 = ld[1], d2ell_a = ld[2], m0 = ld[3];
+            // trial evaluation of phi(a), phi'(a), phi''(a)
+            auto trial = [&](double a, bool second, double* d1, double* d2) -> double {
+                double tv[3] = {0.0, 0.0, 0.0};
+                for (int c = tid; c < nc; c += NT) {
+                    double vc[3], dvc[3], g[3], cc;
+                    for (int r = 0; r < 3; ++r) {
+                        dvc[r] = s.cv[c * 9 + 3 + r];
+                        vc[r] = s.cv[c * 9 + r] + a * dvc[r];
+                    }
+                    if (second) {
+                        tv[2] += project_curvature(s.ck + c * 4, vc, dvc, g, &cc);
+                    } else {
+                        float Gd[9];
+                        cc = project_contact(s.ck + c * 4, vc, g, Gd, false);
+                    }
+                    tv[0] += cc;
+                    tv[1] -= dvc[0] * g[0] + dvc[1] * g[1] + dvc[2] * g[2];
+                }
+                for (int i = tid; i < n; i += NT) {
+                    double imp, hd;
+                    tv[0] += model_terms(s, npad, i, v[i] + a * dvv[i], dt, &imp, &hd);
+                    tv[1] -= dvv[i] * imp;
+                    tv[2] += hd * dvv[i] * dvv[i];
+                }
+                block_sum<NT>(tv, 3, s.red);
+                *d1 = dell_a0 + a * d2ell_a + tv[1];
+                *d2 = d2ell_a + tv[2];
+                return m0 + a * dell_a0 + 0.5 * a * a * d2ell_a + tv[0];
+            };
+            bool accepted = false;
+            double a_acc = 1.0;
+            int ls_it = 0;
+            ls_status = 0;
+            if (!(dell0 < 0.0) || !isfinite(dell0)) {
+                ls_status = -1;
+            } else if (P.ls_mode == 0) {
+                // Armijo backtracking (contact_solve.py:2670-2905, 5108-5305)
+                const double amax = 1.0 / P.rho;
+                double a = amax, d1, d2;
+                double ell = trial(a, false, &d1, &d2);
+                const double slop = (P.rel_slop / 10.0) * fmax(1.0, 0.5 * (fabs(ell) + fabs(cost)));
+                if (d1 < 0.0 || d1 < slop) {
+                    accepted = true;
+                    a_acc = a;
+                } else if (P.ls_max_iter > 1) {
+                    double a_prev = a, e_prev = ell;
+                    int it = 1;
+                    while (true) {
+                        a = a * P.rho;
+                        ell = trial(a, false, &d1, &d2);
+                        const double den = a - a_prev;
+                        const double dapprox = fabs(den) > 0.0 ? (ell - e_prev) / den : 0.0;
+                        if (fabs(dapprox) < slop) { accepted = true; a_acc = a; ls_it = it; break; }
+                        if (ell > e_prev && ell < cost + P.armijo_c * a * dell0) {
+                            accepted = true;
+                            ls_it = it;
+                            a_acc = (e_prev < cost + P.armijo_c * a_prev * dell0) ? a_prev : a;
+                            break;
+                        }
+                        if (it >= P.ls_max_iter - 1) {
+                            if (ell < cost + P.armijo_c * a * dell0) { accepted = true; a_acc = a; ls_it = P.ls_max_iter; }
+                            else ls_status = -2;
+                            break;
+                        }
+                        a_prev = a;
+                        e_prev = ell;
+                        ++it;
+                    }
+                } else {
+                    ls_status = -2;
+                }
+            } else {
+                // exact root (contact_solve.py:2913-3140, 5700-5908), cubic Hermite seed
+                const double amax = kExactAlphaMax;
+                double d1, d2;
+                double ell = trial(amax, true, &d1, &d2);
+                if (d1 <= 0.0) {
+                    accepted = true; a_acc = amax;
+                } else if (-dell0 < P.cost_abs + P.cost_rel * ell) {
+                    accepted = true; a_acc = 1.0;
+                } else {
+                    const double scale = -dell0;
+                    const double f_upper0 = d1 / scale;
+                    if (!isfinite(f_upper0) || f_upper0 <= 0.0) {
+                        ls_status = -3;
+                    } else {
+                        double guess = -dell0 / d2;
+                        const double hh = amax;
+                        const double theta = dell0 + d1 - 3.0 * (ell - cost) / hh;
+                        const double disc = theta * theta - dell0 * d1;
+                        if (isfinite(disc) && disc >= 0.0) {
+                            const double gm = sqrt(disc);
+                            const double den = d1 - dell0 + 2.0 * gm;
+                            if (den != 0.0) {
+                                const double cubic = hh - hh * (d1 + gm - theta) / den;
+                                if (isfinite(cubic) && cubic > 0.0 && cubic <= hh) guess = cubic;
+                            }
+                        }
+                        if (guess > amax) guess = amax;
+                        if (!isfinite(guess) || guess < 0.0 || guess > amax || kExactFTol * guess <= 0.0) {
+                            ls_status = -4;
+                        } else {
+                            double x_lo = 0.0, x_hi = amax, f_hi = f_upper0, root = guess;
+                            double mdx = -amax, mdx_prev = -amax;
+                            const double x_tol = kExactFTol * guess;
+                            int it = 1;
+                            while (true) {
+                                trial(root, true, &d1, &d2);
+                                const double f = d1 / scale, df = d2 / scale;
+                                if ((f < 0.0 && f_hi >= 0.0) || (f >= 0.0 && f_hi < 0.0)) {
+                                    x_lo = root;
+                                } else {
+                                    x_hi = root;
+                                    f_hi = f;
+                                }
+                                if (fabs(f) < kExactFTol) { accepted = true; a_acc = root; ls_it = it; break; }
+                                if (it >= P.ls_max_iter) { ls_status = -5; break; }
+                                const bool slow = 2.0 * fabs(f) > fabs(mdx_prev * df);
+                                mdx_prev = mdx;
+                                const double bdx = 0.5 * (x_lo - x_hi);
+                                const double broot = x_lo - bdx;
+                                double ndx = 0.0, nroot = root;
+                                if (fabs(df) > 0.0) { ndx = f / df; nroot = root - ndx; }
+                                const bool bis = slow || nroot < x_lo || nroot > x_hi || !isfinite(nroot) || !isfinite(ndx);
+                                const double nxt = bis ? broot : nroot;
+                                const double nd = bis ? bdx : ndx;
+                                if (fabs(nd) < x_tol) { accepted = true; a_acc = nxt; ls_it = it; break; }
+                                root = nxt;
+                                mdx = nd;
+                                ++it;
+                            }
+                        }
+                    }
+                }
+            }
+            // ---- commit (contact_solve.py:3154-3176)
+            if (accepted) {
+                alpha = a_acc;
+                prev_cost = cost;
+                ls_total += ls_it;
+                for (int i = tid; i < n; i += NT) v[i] += alpha * dvv[i];
+            }
+            __syncthreads();
+            ++iteration;
+        }
+    }
+    __syncthreads();
+
+    // ------------------------------------------------------------------ 9
+    // integration (solver_sap.py:288-519) + SAP -> public velocity
+    float* qd_pub = kQdPub;  // reuse: public output velocity
+    for (int k = tid; k < M.NQ; k += NT) q_out[k] = q_in[k];
+    __syncthreads();
+    for (int j = tid; j < nj; j += NT) {
+        const int jt = M.t_jtype[j];
+        const int qs = M.t_jqs[j], ds = M.t_jds[j], nd = M.t_jds[j + 1] - M.t_jds[j];
+        if (jt == kJointFree || jt == kJointDistance) {
+            Q4 q = Q4{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]};
+            V3 wn = v3((float)v[ds], (float)v[ds + 1], (float)v[ds + 2]);
+            V3 von = v3((float)v[ds + 3], (float)v[ds + 4], (float)v[ds + 5]);
+            const double* v0 = d64 + V_V0 * npad + ds;
+            V3 wo = v3((float)v0[0], (float)v0[1], (float)v0[2]);
+            V3 voo = v3((float)v0[3], (float)v0[4], (float)v0[5]);
+            Q4 qn;
+            V3 vmid;
+            if (P.integ_mode == 0) {
+                V3 wm = (wo + wn) * 0.5f;
+                float ang = length(wm) * h;
+                qn = q;
+                if (ang > 1.0e-12f) {
+                    V3 ax = wm * (1.0f / length(wm));
+                    float sa = sinf(0.5f * ang), ca = cosf(0.5f * ang);
+                    Q4 r = Q4{ax.x * sa, ax.y * sa, ax.z * sa, ca};
+                    Q4 o = qmul(r, q);
+                    float nn = sqrtf(o.x * o.x + o.y * o.y + o.z * o.z + o.w * o.w);
+                    if (nn > 0.0f) { float inv = 1.0f / nn; o = Q4{o.x * inv, o.y * inv, o.z * inv, o.w * inv}; }
+                    qn = o;
+                }
+                vmid = (voo + von) * 0.5f;
+            } else {
+                float hd = 0.5f * h;
+                float x = q.x + hd * (q.w * wn.x + wn.y * q.z - wn.z * q.y);
+                float y = q.y + hd * (q.w * wn.y + wn.z * q.x - wn.x * q.z);
+                float z = q.z + hd * (q.w * wn.z + wn.x * q.y - wn.y * q.x);
+                float sw = q.w - hd * (wn.x * q.x + wn.y * q.y + wn.z * q.z);
+                float nn = sqrtf(x * x + y * y + z * z + sw * sw);
+                if (nn > 0.0f) { float inv = 1.0f / nn; x *= inv; y *= inv; z *= inv; sw *= inv; }
+                qn = Q4{x, y, z, sw};
+                vmid = von;
+            }
+            q_out[qs] = q_in[qs] + h * vmid.x;
+            q_out[qs + 1] = q_in[qs + 1] + h * vmid.y;
+            q_out[qs + 2] = q_in[qs + 2] + h * vmid.z;
+            q_out[qs + 3] = qn.x; q_out[qs + 4] = qn.y; q_out[qs + 5] = qn.z; q_out[qs + 6] = qn.w;
+            const size_t gb = (size_t)w * nb + M.t_jchild[j];
+            const float* com = M.body_com + gb * 3;
+            V3 rc = qrot(qnormalize(qn), v3(com[0], com[1], com[2]));
+            V3 vcom = von + cross(wn, rc);
+            qd_pub[ds] = vcom.x; qd_pub[ds + 1] = vcom.y; qd_pub[ds + 2] = vcom.z;
+            qd_pub[ds + 3] = wn.x; qd_pub[ds + 4] = wn.y; qd_pub[ds + 5] = wn.z;
+        } else if (jt == kJointFixed || jt == kJointBall) {
+            for (int a = 0; a < nd; ++a) qd_pub[ds + a] = (float)v[ds + a];
+        } else {
+            for (int a = 0; a < nd; ++a) {
+                const float qd = (float)v[ds + a];
+                const float qm = P.integ_mode == 0 ? 0.5f * ((float)d64[V_V0 * npad + ds + a] + qd) : qd;
+                q_out[qs + a] = q_in[qs + a] + h * qm;
+                qd_pub[ds + a] = qd;
+            }
+        }
+    }
+    __syncthreads();
+    // public forward kinematics (sap_kinematics.py:84-229)
+    float* pbq = s.bq;     // reuse: public poses
+    float* pbqd = kV;      // reuse: public velocities (nb x 6, [v, w])
+    for (int L = 0; L < M.num_levels; ++L) {
+        const int l0 = M.t_level_start[L], l1 = M.t_level_start[L + 1];
+        for (int t = l0 + tid; t < l1; t += NT) {
+            const int j = M.t_level_joint[t];
+            const size_t gj = (size_t)w * nj + j;
+            const int jt = M.t_jtype[j];
+            const int parent = M.t_jparent[j], child = M.t_jchild[j];
+            const int qs = M.t_jqs[j], ds = M.t_jds[j];
+            TF Xp = load_tf(M.joint_X_p + gj * 7);
+            TF Xc = load_tf(M.joint_X_c + gj * 7);
+            TF Xwpj = Xp;
+            V3 vlin = v3(0, 0, 0), vang = v3(0, 0, 0);
+            if (parent >= 0) {
+                TF Xwp = load_tf(pbq + parent * 8);
+                Xwpj = tf_mul(Xwp, Xp);
+                const float* com = M.body_com + ((size_t)w * nb + parent) * 3;
+                V3 rp = Xwpj.p - (Xwp.p + qrot(Xwp.q, v3(com[0], com[1], com[2])));
+                const float* vp = pbqd + parent * 6;
+                vang = v3(vp[3], vp[4], vp[5]);
+                vlin = v3(vp[0], vp[1], vp[2]) + cross(vang, rp);
+            }
+            TF Xj = TF{v3(0, 0, 0), Q4{0, 0, 0, 1}};
+            V3 jl = v3(0, 0, 0), ja = v3(0, 0, 0);
+            const float* axp = M.joint_axis + ((size_t)w * n + ds) * 3;
+            V3 axis = v3(axp[0], axp[1], axp[2]);
+            if (jt == kJointPrismatic) {
+                Xj.p = axis * q_out[qs];
+                jl = axis * qd_pub[ds];
+            } else if (jt == kJointRevolute) {
+                Xj.q = q_axis_angle(axis, q_out[qs]);
+                ja = axis * qd_pub[ds];
+            } else if (jt == kJointFree || jt == kJointDistance) {
+                Xj = TF{v3(q_out[qs], q_out[qs + 1], q_out[qs + 2]), Q4{q_out[qs + 3], q_out[qs + 4], q_out[qs + 5], q_out[qs + 6]}};
+                jl = v3(qd_pub[ds], qd_pub[ds + 1], qd_pub[ds + 2]);
+                ja = v3(qd_pub[ds + 3], qd_pub[ds + 4], qd_pub[ds + 5]);
+            }
+            TF Xwc = tf_mul(tf_mul(Xwpj, Xj), tf_inverse(Xc));
+            float* o = pbq + child * 8;
+            o[0] = Xwc.p.x; o[1] = Xwc.p.y; o[2] = Xwc.p.z; o[3] = Xwc.q.x; o[4] = Xwc.q.y; o[5] = Xwc.q.z; o[6] = Xwc.q.w;
+            V3 lv = vlin + qrot(Xwpj.q, jl);
+            V3 av3 = vang + qrot(Xwpj.q, ja);
+            float* ov = pbqd + child * 6;
+            ov[0] = lv.x; ov[1] = lv.y; ov[2] = lv.z; ov[3] = av3.x; ov[4] = av3.y; ov[5] = av3.z;
+        }
+        __syncthreads();
+    }
+    // ------------------------------------------------------------------ outputs
+    for (int k = tid; k < M.NQ; k += NT) P.jq_out[(size_t)w * M.NQ + k] = q_out[k];
+    for (int d = tid; d < n; d += NT) {
+        const size_t g = (size_t)w * n + d;
+        P.jqd_out[g] = qd_pub[d];
+        P.v_guess[g] = v[d];
+        if (P.dbg_v) P.dbg_v[g] = v[d];
+        if (P.dbg_vstar) P.dbg_vstar[g] = vs[d];
+        if (P.dbg_v0) P.dbg_v0[g] = d64[V_V0 * npad + d];
+        if (P.dbg_adiag) P.dbg_adiag[g] = s.A[d * npad + d];
+    }
+    for (int t = tid; t < nb * 7; t += NT) {
+        const int b = t / 7, k = t % 7;
+        P.bq_out[((size_t)w * nb + b) * 7 + k] = pbq[b * 8 + k];
+    }
+    if (P.bqd_out)
+        for (int t = tid; t < nb * 6; t += NT) P.bqd_out[(size_t)w * nb * 6 + t] = pbqd[t];
+    if (P.bf_out && P.bf_in && P.bf_out != P.bf_in)
+        for (int t = tid; t < nb * 6; t += NT) P.bf_out[(size_t)w * nb * 6 + t] = P.bf_in[(size_t)w * nb * 6 + t];
+    if (P.dbg_gamma)
+        for (int t = tid; t < C * 3; t += NT) {
+            const int c = t / 3, r = t % 3;
+            P.dbg_gamma[(size_t)w * C * 3 + t] = (c < nc) ? s.cv[c * 9 + 6 + r] : 0.0;
+        }
+    if (tid == 0) {
+        P.st_iters[w] = iteration;
+        P.st_ls[w] = ls_total;
+        P.st_nc[w] = nc;
+        P.st_flags[w] = flags | ((ls_status & 0xff) << 8);
+    }
+}
+
+}  // namespace sapx
+
+// host-side launcher (C++ linkage inside the library)
+namespace sapx {
+hipError_t launch_step(const StepParams& P, hipStream_t stream) {
+    constexpr int NT = 256;
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)sapx_step_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(sapx_step_kernel<NT>, dim3(P.m.W), dim3(NT), P.smem_bytes, stream, P);
+    return hipGetLastError();
+}
+}  // namespace sapx

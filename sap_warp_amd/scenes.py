@@ -1,0 +1,752 @@
+"""Synthetic scene builders producing ``SapModel`` instances.
+
+The builder reproduces the model-array conventions of the reference scene
+loader (``sim/loader/scene.py``): body/shape/joint bookkeeping
+(``_SapSceneBuilder.add_body/add_shape/add_joint``, ``scene.py:282-495``),
+density-based inertia accumulation with the parallel-axis theorem
+(``_update_body_mass`` ``scene.py:389-411``, inertia formulas
+``scene.py:5847-5918``), same-body / parent-child collision filters
+(``scene.py:319-373, 477-484``), collision-group pair tests
+(``scene.py:5788-5801``) and the explicit candidate pair list
+(``_find_shape_contact_pairs`` ``scene.py:1188-1212``).
+
+Scenes:
+
+* :func:`build_ant_scene` — the 14-DOF ant batch (free torso sphere, four
+  two-hinge capsule legs) used as the CPU-runnable case.
+* :func:`build_humanoid_scene` — the 35-DOF humanoid batch (free pelvis + 29
+  revolute joints, 30 bodies, capsule/sphere/box links) used as the GPU
+  workload.
+
+Worlds are independent (no cross-world pairs), so by default every world sits
+at the origin (``spacing=0``): positions stay O(1 m) and float32 geometry keeps
+full precision.  A non-zero spacing reproduces the reference's grid layout
+(``DEFAULT_REPLICATE_SPACING``, ``scene.py:42``).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, replace
+
+import numpy as np
+
+from .runtime import (
+    GEO_BOX,
+    GEO_CAPSULE,
+    GEO_PLANE,
+    GEO_SPHERE,
+    SAP_BODY_FLAG_DYNAMIC,
+    SAP_JOINT_BALL,
+    SAP_JOINT_DISTANCE,
+    SAP_JOINT_FIXED,
+    SAP_JOINT_FREE,
+    SAP_JOINT_PRISMATIC,
+    SAP_JOINT_REVOLUTE,
+    SHAPE_FLAG_COLLIDE_PARTICLES,
+    SHAPE_FLAG_COLLIDE_SHAPES,
+    SHAPE_FLAG_VISIBLE,
+    SapModel,
+)
+
+IDENTITY_TF = np.array([0, 0, 0, 0, 0, 0, 1], dtype=np.float64)
+
+
+# ----------------------------------------------------------------------------
+# small math helpers (float64, quaternions xyzw)
+# ----------------------------------------------------------------------------
+def quat_from_axis_angle(axis, angle) -> np.ndarray:
+    axis = np.asarray(axis, dtype=np.float64)
+    axis = axis / np.linalg.norm(axis)
+    s = math.sin(0.5 * angle)
+    return np.array([axis[0] * s, axis[1] * s, axis[2] * s, math.cos(0.5 * angle)])
+
+
+def quat_mul(a, b) -> np.ndarray:
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array(
+        [
+            aw * bx + ax * bw + ay * bz - az * by,
+            aw * by - ax * bz + ay * bw + az * bx,
+            aw * bz + ax * by - ay * bx + az * bw,
+            aw * bw - ax * bx - ay * by - az * bz,
+        ]
+    )
+
+
+def quat_to_matrix(q) -> np.ndarray:
+    x, y, z, w = q
+    return np.array(
+        [
+            [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+            [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+            [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+        ]
+    )
+
+
+def quat_rotate(q, v) -> np.ndarray:
+    return quat_to_matrix(q) @ np.asarray(v, dtype=np.float64)
+
+
+def tf(p=(0.0, 0.0, 0.0), q=(0.0, 0.0, 0.0, 1.0)) -> np.ndarray:
+    return np.array([p[0], p[1], p[2], q[0], q[1], q[2], q[3]], dtype=np.float64)
+
+
+def tf_mul(a, b) -> np.ndarray:
+    p = a[:3] + quat_rotate(a[3:], b[:3])
+    return np.concatenate([p, quat_mul(a[3:], b[3:])])
+
+
+def tf_point(a, p) -> np.ndarray:
+    return a[:3] + quat_rotate(a[3:], p)
+
+
+def quat_z_to(direction) -> np.ndarray:
+    """Quaternion rotating local +Z onto ``direction`` (capsule axis convention)."""
+    d = np.asarray(direction, dtype=np.float64)
+    d = d / np.linalg.norm(d)
+    z = np.array([0.0, 0.0, 1.0])
+    c = float(np.dot(z, d))
+    if c > 1.0 - 1e-12:
+        return np.array([0.0, 0.0, 0.0, 1.0])
+    if c < -1.0 + 1e-12:
+        return np.array([1.0, 0.0, 0.0, 0.0])
+    axis = np.cross(z, d)
+    return quat_from_axis_angle(axis, math.acos(max(-1.0, min(1.0, c))))
+
+
+# ----------------------------------------------------------------------------
+# inertia formulas (scene.py:5847-5918)
+# ----------------------------------------------------------------------------
+def inertia_sphere(density, r):
+    mass = density * 4.0 / 3.0 * math.pi * r ** 3
+    i = 2.0 / 5.0 * mass * r * r
+    return mass, np.diag([i, i, i])
+
+
+def inertia_capsule(density, r, half_height):
+    h = 2.0 * half_height
+    ms = density * (4.0 / 3.0) * math.pi * r ** 3
+    mc = density * math.pi * r * r * h
+    ixy = mc * (0.25 * r * r + h * h / 12.0) + ms * (0.4 * r * r + 0.375 * r * h + 0.25 * h * h)
+    iz = (mc * 0.5 + ms * 0.4) * r * r
+    return ms + mc, np.diag([ixy, ixy, iz])
+
+
+def inertia_box(density, hx, hy, hz):
+    mass = density * 8.0 * hx * hy * hz
+    return mass, np.diag(
+        [mass / 3.0 * (hy * hy + hz * hz), mass / 3.0 * (hx * hx + hz * hz), mass / 3.0 * (hx * hx + hy * hy)]
+    )
+
+
+def shape_volume(shape_type, scale) -> float:
+    if shape_type == GEO_SPHERE:
+        return 4.0 / 3.0 * math.pi * scale[0] ** 3
+    if shape_type == GEO_CAPSULE:
+        r, hh = scale[0], scale[1]
+        return 4.0 / 3.0 * math.pi * r ** 3 + math.pi * r * r * 2.0 * hh
+    if shape_type == GEO_BOX:
+        return 8.0 * scale[0] * scale[1] * scale[2]
+    return 0.0
+
+
+def transform_inertia(mass, inertia, offset, q) -> np.ndarray:
+    """Parallel-axis + rotation (``_sap_transform_inertia`` scene.py:5913-5918)."""
+    rot = quat_to_matrix(q)
+    o = np.asarray(offset, dtype=np.float64)
+    return rot @ inertia @ rot.T + mass * (np.dot(o, o) * np.eye(3) - np.outer(o, o))
+
+
+def shape_radius(shape_type, scale) -> float:
+    """``_compute_shape_radius`` (scene.py:5804-5820)."""
+    if shape_type == GEO_SPHERE:
+        return float(scale[0])
+    if shape_type == GEO_BOX:
+        return float(np.linalg.norm(scale))
+    if shape_type == GEO_CAPSULE:
+        return float(scale[0] + scale[1])
+    if shape_type == GEO_PLANE:
+        if scale[0] > 0.0 and scale[1] > 0.0:
+            return float(np.linalg.norm(scale))
+        return 1.0e6
+    return 10.0
+
+
+# ----------------------------------------------------------------------------
+# builder
+# ----------------------------------------------------------------------------
+@dataclass
+class ShapeCfg:
+    """Per-shape material / collision config (defaults: ``_SapShapeConfig`` scene.py:86-110)."""
+
+    density: float = 1000.0
+    ke: float = 2.5e3
+    tau: float = 0.01
+    mu: float = 1.0
+    margin: float = 0.0
+    gap: float | None = None
+    collision_group: int = 1
+    collision_filter_parent: bool = True
+    has_shape_collision: bool = True
+
+    def flags(self) -> int:
+        f = SHAPE_FLAG_VISIBLE | SHAPE_FLAG_COLLIDE_PARTICLES
+        if self.has_shape_collision:
+            f |= SHAPE_FLAG_COLLIDE_SHAPES
+        return f
+
+
+@dataclass
+class DofCfg:
+    """Per-axis joint config (defaults: ``_SapJointDofConfig`` scene.py:169-183)."""
+
+    axis: tuple = (1.0, 0.0, 0.0)
+    target_pos: float = 0.0
+    target_vel: float = 0.0
+    target_ke: float = 0.0
+    target_kd: float = 0.0
+    limit_lower: float = -1.0e10
+    limit_upper: float = 1.0e10
+    limit_ke: float = 1.0e4
+    limit_kd: float = 10.0
+    armature: float = 0.0
+    effort_limit: float = 1.0e6
+    actuator_mode: int = 0
+
+
+def _free_axes():
+    return [DofCfg(axis=a, limit_ke=0.0, limit_kd=0.0) for a in ((1, 0, 0), (0, 1, 0), (0, 0, 1))]
+
+
+def _joint_dof_count(joint_type: int, num_axes: int) -> tuple[int, int]:
+    """(dof_count, coord_count) per joint type (scene.py:1215-1222)."""
+    if joint_type == SAP_JOINT_BALL:
+        return 3, 4
+    if joint_type in (SAP_JOINT_FREE, SAP_JOINT_DISTANCE):
+        return 6, 7
+    if joint_type == SAP_JOINT_FIXED:
+        return 0, 0
+    return num_axes, num_axes
+
+
+def _test_group_pair(a: int, b: int) -> bool:
+    """scene.py:5788-5795."""
+    if a == 0 or b == 0:
+        return False
+    if a > 0:
+        return a == b or b < 0
+    if a < 0:
+        return a != b
+    return False
+
+
+class SceneBuilder:
+    """Host-side model builder (``_SapSceneBuilder`` semantics)."""
+
+    def __init__(self, *, rigid_gap: float = 0.1, gravity: float = -9.81):
+        self.rigid_gap = float(rigid_gap)
+        self.gravity = float(gravity)
+        self.current_world = -1
+        self.world_count = 1
+        # bodies
+        self.body_q: list = []
+        self.body_mass: list = []
+        self.body_inertia: list = []
+        self.body_com: list = []
+        self.body_world: list = []
+        self.body_lock_inertia: list = []
+        self.body_shapes: dict = {-1: []}
+        # joints
+        self.joint_type: list = []
+        self.joint_parent: list = []
+        self.joint_child: list = []
+        self.joint_parents: dict = {}
+        self.joint_children: dict = {}
+        self.joint_X_p: list = []
+        self.joint_X_c: list = []
+        self.joint_q: list = []
+        self.joint_qd: list = []
+        self.joint_q_start: list = []
+        self.joint_qd_start: list = []
+        self.joint_dof_dim: list = []
+        self.joint_articulation: list = []
+        self.dofs: list = []
+        self.articulation_start: list = []
+        # shapes
+        self.shape_transform: list = []
+        self.shape_body: list = []
+        self.shape_type: list = []
+        self.shape_scale: list = []
+        self.shape_cfg: list = []
+        self.shape_world: list = []
+        self.filter_pairs: set = set()
+
+    # ------------------------------------------------------------------ bodies
+    def add_body(self, xform, *, mass=0.0, com=None, inertia=None, lock_inertia=False) -> int:
+        bid = len(self.body_q)
+        self.body_q.append(np.asarray(xform, dtype=np.float64).copy())
+        self.body_mass.append(float(mass))
+        self.body_com.append(np.zeros(3) if com is None else np.asarray(com, dtype=np.float64))
+        self.body_inertia.append(np.zeros((3, 3)) if inertia is None else np.asarray(inertia, dtype=np.float64))
+        self.body_world.append(self.current_world)
+        self.body_lock_inertia.append(bool(lock_inertia))
+        self.body_shapes[bid] = []
+        return bid
+
+    def _update_body_mass(self, body, mass, inertia, p, q):
+        """scene.py:389-411."""
+        if body == -1:
+            return
+        new_mass = self.body_mass[body] + mass
+        if new_mass == 0.0:
+            return
+        old_mass = self.body_mass[body]
+        new_com = (self.body_com[body] * old_mass + p * mass) / new_mass
+        com_offset = new_com - self.body_com[body]
+        shape_offset = new_com - p
+        new_inertia = transform_inertia(old_mass, self.body_inertia[body], com_offset, (0, 0, 0, 1)) + transform_inertia(
+            mass, inertia, shape_offset, q
+        )
+        self.body_mass[body] = new_mass
+        self.body_inertia[body] = new_inertia
+        self.body_com[body] = new_com
+
+    # ------------------------------------------------------------------ shapes
+    def add_shape(self, body: int, shape_type: int, xform, scale, cfg: ShapeCfg) -> int:
+        sid = len(self.shape_type)
+        self.body_shapes.setdefault(body, [])
+        if cfg.has_shape_collision:
+            for other in self.body_shapes[body]:
+                self._filter(other, sid)
+        self.shape_body.append(body)
+        self.body_shapes[body].append(sid)
+        self.shape_transform.append(np.asarray(xform, dtype=np.float64).copy())
+        self.shape_type.append(int(shape_type))
+        self.shape_scale.append(tuple(float(s) for s in scale))
+        self.shape_cfg.append(replace(cfg))
+        self.shape_world.append(self.current_world)
+        if cfg.has_shape_collision and cfg.collision_filter_parent and body > -1:
+            for parent_body in self.joint_parents.get(body, []):
+                if parent_body > -1:
+                    for ps in self.body_shapes[parent_body]:
+                        self._filter(ps, sid)
+            for child_body in self.joint_children.get(body, []):
+                for cs in self.body_shapes[child_body]:
+                    self._filter(sid, cs)
+        if cfg.density > 0.0 and body >= 0 and not self.body_lock_inertia[body]:
+            if shape_type == GEO_SPHERE:
+                m, inertia = inertia_sphere(cfg.density, scale[0])
+            elif shape_type == GEO_CAPSULE:
+                m, inertia = inertia_capsule(cfg.density, scale[0], scale[1])
+            elif shape_type == GEO_BOX:
+                m, inertia = inertia_box(cfg.density, *scale)
+            else:
+                m, inertia = 0.0, np.zeros((3, 3))
+            if m > 0.0:
+                x = self.shape_transform[sid]
+                self._update_body_mass(body, m, inertia, x[:3].copy(), x[3:])
+        return sid
+
+    def add_ground_plane(self, cfg: ShapeCfg, height: float = 0.0) -> int:
+        world = self.current_world
+        self.current_world = -1
+        sid = self.add_shape(-1, GEO_PLANE, tf((0.0, 0.0, height)), (0.0, 0.0, 0.0), replace(cfg, density=0.0))
+        self.current_world = world
+        return sid
+
+    def _filter(self, a: int, b: int):
+        self.filter_pairs.add((min(a, b), max(a, b)))
+
+    # ------------------------------------------------------------------ joints
+    def add_joint(
+        self,
+        parent: int,
+        child: int,
+        joint_type: int,
+        *,
+        linear_axes=(),
+        angular_axes=(),
+        parent_xform=None,
+        child_xform=None,
+        collision_filter_parent=True,
+    ) -> int:
+        jid = len(self.joint_type)
+        lin = [a if isinstance(a, DofCfg) else DofCfg(axis=tuple(a)) for a in linear_axes]
+        ang = [a if isinstance(a, DofCfg) else DofCfg(axis=tuple(a)) for a in angular_axes]
+        self.joint_type.append(int(joint_type))
+        self.joint_parent.append(int(parent))
+        self.joint_child.append(int(child))
+        self.joint_X_p.append(IDENTITY_TF.copy() if parent_xform is None else np.asarray(parent_xform, np.float64))
+        self.joint_X_c.append(IDENTITY_TF.copy() if child_xform is None else np.asarray(child_xform, np.float64))
+        self.joint_q_start.append(len(self.joint_q))
+        self.joint_qd_start.append(len(self.joint_qd))
+        self.joint_dof_dim.append((len(lin), len(ang)))
+        self.joint_articulation.append(-1)
+        self.dofs.extend(lin + ang)
+        dof_count, coord_count = _joint_dof_count(joint_type, len(lin) + len(ang))
+        self.joint_q.extend([0.0] * coord_count)
+        self.joint_qd.extend([0.0] * dof_count)
+        if joint_type in (SAP_JOINT_FREE, SAP_JOINT_DISTANCE, SAP_JOINT_BALL) and coord_count > 0:
+            self.joint_q[-1] = 1.0
+        if joint_type == SAP_JOINT_FREE:
+            s = self.joint_q_start[jid]
+            self.joint_q[s : s + 7] = list(self.body_q[child])
+        self.joint_parents.setdefault(child, []).append(parent)
+        ch = self.joint_children.setdefault(parent, [])
+        if child not in ch:
+            ch.append(child)
+        if collision_filter_parent and parent > -1:
+            for cs in self.body_shapes.get(child, []):
+                if not self.shape_cfg[cs].has_shape_collision:
+                    continue
+                for ps in self.body_shapes.get(parent, []):
+                    if not self.shape_cfg[ps].has_shape_collision:
+                        continue
+                    self._filter(ps, cs)
+        return jid
+
+    def add_free_joint(self, child: int, **kw) -> int:
+        return self.add_joint(-1, child, SAP_JOINT_FREE, linear_axes=_free_axes(), angular_axes=_free_axes(), **kw)
+
+    def add_articulation(self, joints) -> None:
+        joints = [int(j) for j in joints]
+        art = len(self.articulation_start)
+        self.articulation_start.append(joints[0])
+        for j in joints:
+            self.joint_articulation[j] = art
+
+    # --------------------------------------------------------------- finalize
+    def shape_pairs(self) -> np.ndarray:
+        """Explicit candidate pairs (``_find_shape_contact_pairs`` scene.py:1188-1212)."""
+        colliding = [i for i, c in enumerate(self.shape_cfg) if c.has_shape_collision]
+        order = sorted(colliding, key=lambda i: self.shape_world[i])
+        pairs = []
+        for i1, s1 in enumerate(order):
+            w1 = self.shape_world[s1]
+            g1 = self.shape_cfg[s1].collision_group
+            for s2 in order[i1 + 1 :]:
+                w2 = self.shape_world[s2]
+                g2 = self.shape_cfg[s2].collision_group
+                if w1 != -1 and w2 != -1 and w1 != w2:
+                    break
+                if w1 != -1 and w2 != -1 and w1 != w2:
+                    continue
+                if not _test_group_pair(g1, g2):
+                    continue
+                a, b = (s2, s1) if s1 > s2 else (s1, s2)
+                if (a, b) not in self.filter_pairs:
+                    pairs.append((a, b))
+        return np.asarray(pairs, dtype=np.int32).reshape(-1, 2)
+
+    def finalize(self, device: str = "cpu") -> SapModel:
+        f32 = np.float32
+        i32 = np.int32
+        joint_count = len(self.joint_type)
+        articulation_start = list(self.articulation_start) + [joint_count]
+        child_to_joint = {c: j for j, c in enumerate(self.joint_child)}
+        joint_ancestor = [child_to_joint.get(p, -1) for p in self.joint_parent]
+        world_count = max(self.world_count, 1)
+        gravity = np.tile(np.array([0.0, 0.0, self.gravity], dtype=f32), (world_count, 1))
+
+        def dof_field(name):
+            return np.asarray([getattr(d, name) for d in self.dofs], dtype=f32)
+
+        shape_gap = [c.gap if c.gap is not None else self.rigid_gap for c in self.shape_cfg]
+        # body_qd public convention [v_com, w]; zero at construction
+        body_q = np.asarray(self.body_q, dtype=f32).reshape(-1, 7)
+        model = SapModel(
+            device=device,
+            joint_count=joint_count,
+            joint_dof_count=len(self.joint_qd),
+            joint_coord_count=len(self.joint_q),
+            body_count=len(self.body_q),
+            articulation_count=len(self.articulation_start),
+            world_count=world_count,
+            shape_count=len(self.shape_type),
+            joint_type=np.asarray(self.joint_type, dtype=i32),
+            joint_articulation=np.asarray(self.joint_articulation, dtype=i32),
+            joint_parent=np.asarray(self.joint_parent, dtype=i32),
+            joint_child=np.asarray(self.joint_child, dtype=i32),
+            joint_q_start=np.asarray(self.joint_q_start + [len(self.joint_q)], dtype=i32),
+            joint_qd_start=np.asarray(self.joint_qd_start + [len(self.joint_qd)], dtype=i32),
+            joint_dof_dim=np.asarray(self.joint_dof_dim, dtype=i32).reshape(-1, 2),
+            joint_axis=np.asarray([d.axis for d in self.dofs], dtype=f32).reshape(-1, 3),
+            joint_X_p=np.asarray(self.joint_X_p, dtype=f32).reshape(-1, 7),
+            joint_X_c=np.asarray(self.joint_X_c, dtype=f32).reshape(-1, 7),
+            joint_ancestor=np.asarray(joint_ancestor, dtype=i32),
+            articulation_start=np.asarray(articulation_start, dtype=i32),
+            joint_q=np.asarray(self.joint_q, dtype=f32),
+            joint_qd=np.asarray(self.joint_qd, dtype=f32),
+            body_q=body_q,
+            body_qd=np.zeros((len(self.body_q), 6), dtype=f32),
+            body_mass=np.asarray(self.body_mass, dtype=f32),
+            body_inertia=np.asarray(self.body_inertia, dtype=f32).reshape(-1, 3, 3),
+            body_com=np.asarray(self.body_com, dtype=f32).reshape(-1, 3),
+            body_flags=np.full(len(self.body_q), SAP_BODY_FLAG_DYNAMIC, dtype=i32),
+            body_world=np.asarray(self.body_world, dtype=i32),
+            gravity=gravity,
+            joint_armature=dof_field("armature"),
+            shape_body=np.asarray(self.shape_body, dtype=i32),
+            shape_material_mu=np.asarray([c.mu for c in self.shape_cfg], dtype=f32),
+            shape_material_ke=np.asarray([c.ke for c in self.shape_cfg], dtype=f32),
+            shape_material_tau=np.asarray([c.tau for c in self.shape_cfg], dtype=f32),
+            joint_target_mode=np.asarray([d.actuator_mode for d in self.dofs], dtype=i32),
+            joint_target_ke=dof_field("target_ke"),
+            joint_target_kd=dof_field("target_kd"),
+            joint_target_pos=dof_field("target_pos"),
+            joint_target_vel=dof_field("target_vel"),
+            joint_f=np.zeros(len(self.dofs), dtype=f32),
+            joint_act=np.zeros(len(self.dofs), dtype=f32),
+            joint_effort_limit=dof_field("effort_limit"),
+            joint_limit_lower=dof_field("limit_lower"),
+            joint_limit_upper=dof_field("limit_upper"),
+            joint_limit_ke=dof_field("limit_ke"),
+            joint_limit_kd=dof_field("limit_kd"),
+            shape_transform=np.asarray(self.shape_transform, dtype=f32).reshape(-1, 7),
+            shape_type=np.asarray(self.shape_type, dtype=i32),
+            shape_scale=np.asarray(self.shape_scale, dtype=f32).reshape(-1, 3),
+            shape_flags=np.asarray([c.flags() for c in self.shape_cfg], dtype=i32),
+            shape_margin=np.asarray([c.margin for c in self.shape_cfg], dtype=f32),
+            shape_gap=np.asarray(shape_gap, dtype=f32),
+            shape_collision_radius=np.asarray(
+                [shape_radius(t, s) for t, s in zip(self.shape_type, self.shape_scale)], dtype=f32
+            ),
+            shape_collision_group=np.asarray([c.collision_group for c in self.shape_cfg], dtype=i32),
+            shape_world=np.asarray(self.shape_world, dtype=i32),
+            shape_pairs=self.shape_pairs(),
+        )
+        return model
+
+
+# ----------------------------------------------------------------------------
+# scenes
+# ----------------------------------------------------------------------------
+def _grid_offset(world: int, world_count: int, spacing: float) -> np.ndarray:
+    if spacing == 0.0:
+        return np.zeros(3)
+    side = int(math.ceil(math.sqrt(world_count)))
+    return np.array([(world % side) * spacing, (world // side) * spacing, 0.0])
+
+
+def _set_free_joint_state(b: SceneBuilder, joint: int, pose: np.ndarray, vel_public: np.ndarray):
+    qs = b.joint_q_start[joint]
+    b.joint_q[qs : qs + 7] = list(pose)
+    ds = b.joint_qd_start[joint]
+    b.joint_qd[ds : ds + 6] = list(vel_public)
+
+
+def build_ant_scene(
+    num_worlds: int = 64,
+    seed: int = 1,
+    *,
+    spacing: float = 0.0,
+    device: str = "cpu",
+) -> SapModel:
+    """64-world ant batch: torso sphere r=0.25 on a free joint + 4 legs x 2 hinges.
+
+    Capsule links r=0.04, segment lengths U(0.2, 0.4); mu=1.0; per-shape
+    stiffness 2e5 N/m and dissipation 5 s so each contact pair combines to
+    k = 1e5 N/m (series, ``sap_helpers.py:198-207``) and tau = 10 s (sum,
+    ``sap_helpers.py:226-237``).  Initial hinge angles U(-0.3, 0.3), base height
+    U(0.4, 0.8), all velocities N(0, 0.5).  No drives or limits.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = SceneBuilder(rigid_gap=0.02)
+    shape = ShapeCfg(density=1000.0, ke=2.0e5, tau=5.0, mu=1.0, gap=0.02)
+    b.world_count = num_worlds
+    for w in range(num_worlds):
+        b.current_world = w
+        off = _grid_offset(w, num_worlds, spacing)
+        height = rng.uniform(0.4, 0.8)
+        yaw = rng.uniform(-math.pi, math.pi)
+        root_q = quat_from_axis_angle((0, 0, 1), yaw)
+        root_pose = tf(off + np.array([0.0, 0.0, height]), root_q)
+        torso = b.add_body(root_pose)
+        b.add_shape(torso, GEO_SPHERE, tf(), (0.25, 0.0, 0.0), shape)
+        joints = [b.add_free_joint(torso)]
+        for leg in range(4):
+            ang = math.pi / 4 + leg * math.pi / 2
+            radial = np.array([math.cos(ang), math.sin(ang), 0.0])
+            tangent = np.array([-math.sin(ang), math.cos(ang), 0.0])
+            l1 = rng.uniform(0.2, 0.4)
+            l2 = rng.uniform(0.2, 0.4)
+            hip_local = 0.25 * radial
+            thigh = b.add_body(tf_mul(root_pose, tf(hip_local)))
+            b.add_shape(thigh, GEO_CAPSULE, tf(0.5 * l1 * radial, quat_z_to(radial)), (0.04, 0.5 * l1, 0.0), shape)
+            joints.append(
+                b.add_joint(torso, thigh, SAP_JOINT_REVOLUTE, angular_axes=[DofCfg(axis=(0, 0, 1), limit_ke=0.0)],
+                            parent_xform=tf(hip_local))
+            )
+            down = 0.5 * radial - 0.866 * np.array([0.0, 0.0, 1.0])
+            down /= np.linalg.norm(down)
+            shin = b.add_body(tf_mul(root_pose, tf(hip_local + l1 * radial)))
+            b.add_shape(shin, GEO_CAPSULE, tf(0.5 * l2 * down, quat_z_to(down)), (0.04, 0.5 * l2, 0.0), shape)
+            joints.append(
+                b.add_joint(thigh, shin, SAP_JOINT_REVOLUTE, angular_axes=[DofCfg(axis=tuple(tangent), limit_ke=0.0)],
+                            parent_xform=tf(l1 * radial))
+            )
+        b.add_articulation(joints)
+        # initial state
+        lin = rng.normal(0.0, 0.5, size=3)
+        angv = rng.normal(0.0, 0.5, size=3)
+        _set_free_joint_state(b, joints[0], root_pose, np.concatenate([lin, angv]))
+        for j in joints[1:]:
+            b.joint_q[b.joint_q_start[j]] = rng.uniform(-0.3, 0.3)
+            b.joint_qd[b.joint_qd_start[j]] = rng.normal(0.0, 0.5)
+    b.current_world = -1
+    b.add_ground_plane(ShapeCfg(density=0.0, ke=2.0e5, tau=5.0, mu=1.0, gap=0.02, collision_group=-1))
+    return _with_consistent_body_q(b.finalize(device))
+
+
+# Humanoid kinematic template: (name, parent, axis, parent offset, shape)
+# shape: None | ("capsule", center, direction, r, half_height) | ("sphere", center, r) | ("box", center, half)
+def _humanoid_template():
+    t = []
+    t.append(("pelvis", None, None, (0.0, 0.0, 0.0), ("capsule", (0, 0, 0.02), (0, 1, 0), 0.07, 0.05)))
+    for side, sy in (("l", 1.0), ("r", -1.0)):
+        t.append((f"{side}_hip_p", "pelvis", (0, 1, 0), (0.0, 0.09 * sy, -0.05), None))
+        t.append((f"{side}_hip_r", f"{side}_hip_p", (1, 0, 0), (0.0, 0.0, 0.0), None))
+        t.append((f"{side}_thigh", f"{side}_hip_r", (0, 0, 1), (0.0, 0.0, 0.0),
+                  ("capsule", (0, 0, -0.22), (0, 0, 1), 0.06, 0.13)))
+        t.append((f"{side}_shin", f"{side}_thigh", (0, 1, 0), (0.0, 0.0, -0.4),
+                  ("capsule", (0, 0, -0.2), (0, 0, 1), 0.05, 0.15)))
+        t.append((f"{side}_ankle", f"{side}_shin", (0, 1, 0), (0.0, 0.0, -0.4), None))
+        t.append((f"{side}_foot", f"{side}_ankle", (1, 0, 0), (0.0, 0.0, 0.0),
+                  ("box", (0.04, 0.0, -0.05), (0.10, 0.05, 0.025))))
+    t.append(("waist_y", "pelvis", (0, 0, 1), (0.0, 0.0, 0.10), None))
+    t.append(("waist_r", "waist_y", (1, 0, 0), (0.0, 0.0, 0.0), None))
+    t.append(("torso", "waist_r", (0, 1, 0), (0.0, 0.0, 0.0),
+              ("capsule", (0, 0, 0.25), (0, 0, 1), 0.12, 0.10), ("sphere", (0, 0, 0.60), 0.10)))
+    for side, sy in (("l", 1.0), ("r", -1.0)):
+        t.append((f"{side}_sh_p", "torso", (0, 1, 0), (0.0, 0.2 * sy, 0.35), None))
+        t.append((f"{side}_sh_r", f"{side}_sh_p", (1, 0, 0), (0.0, 0.0, 0.0), None))
+        t.append((f"{side}_uarm", f"{side}_sh_r", (0, 0, 1), (0.0, 0.0, 0.0),
+                  ("capsule", (0, 0, -0.13), (0, 0, 1), 0.045, 0.10)))
+        t.append((f"{side}_farm", f"{side}_uarm", (0, 1, 0), (0.0, 0.0, -0.26),
+                  ("capsule", (0, 0, -0.12), (0, 0, 1), 0.04, 0.09)))
+        t.append((f"{side}_wr_r", f"{side}_farm", (0, 0, 1), (0.0, 0.0, -0.24), None))
+        t.append((f"{side}_wr_p", f"{side}_wr_r", (0, 1, 0), (0.0, 0.0, 0.0), None))
+        t.append((f"{side}_hand", f"{side}_wr_p", (1, 0, 0), (0.0, 0.0, 0.0), ("sphere", (0, 0, -0.05), 0.045)))
+    return t
+
+
+HUMANOID_STAND_HEIGHT = 0.05 + 0.8 + 0.075
+
+
+def build_humanoid_scene(
+    num_worlds: int = 4096,
+    seed: int = 2,
+    *,
+    spacing: float = 0.0,
+    contact_gap: float = 0.01,
+    device: str = "cpu",
+) -> SapModel:
+    """35-DOF humanoid batch: free pelvis + 29 revolute joints, 30 bodies.
+
+    Tree: legs 2 x 6 (hip pitch/roll/yaw, knee, ankle pitch/roll), waist 3,
+    arms 2 x 7 (shoulder pitch/roll/yaw, elbow, wrist roll/pitch/yaw); the
+    head is a sphere rigidly attached to the torso.  Every body mass is
+    U(0.5, 5) kg (shape density is set to hit it; shapeless joint links get a
+    0.05 m sphere-like inertia).  Per-shape friction mu ~ U(0.6, 1.2), stiffness
+    2e5 N/m, dissipation 5 ms (pair: 1e5 N/m, 10 ms).  Links are capsules and
+    spheres; the feet are boxes in their own collision groups (2 and 3) so they
+    touch only the ground (group -1).  PD position drives on every revolute
+    DOF (ke=200, kd=10, effort limit 300), limits +-2 rad, armature 0.01.
+    Half of the worlds start standing on the ground, half start falling from
+    U(1.0, 1.6) m with tilted bases and random joint angles and velocities.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = SceneBuilder(rigid_gap=contact_gap)
+    template = _humanoid_template()
+    b.world_count = num_worlds
+    for w in range(num_worlds):
+        b.current_world = w
+        off = _grid_offset(w, num_worlds, spacing)
+        falling = bool(rng.random() < 0.5)
+        if falling:
+            height = rng.uniform(1.0, 1.6)
+            tilt_axis = np.array([rng.normal(), rng.normal(), 0.0])
+            tilt = rng.uniform(0.0, 0.6)
+            root_q = quat_mul(quat_from_axis_angle((0, 0, 1), rng.uniform(-math.pi, math.pi)),
+                              quat_from_axis_angle(tilt_axis, tilt))
+            qscale, vscale = 0.3, 0.5
+        else:
+            height = HUMANOID_STAND_HEIGHT + rng.uniform(0.0, 0.01)
+            root_q = quat_from_axis_angle((0, 0, 1), rng.uniform(-math.pi, math.pi))
+            qscale, vscale = 0.03, 0.05
+        root_pose = tf(off + np.array([0.0, 0.0, height]), root_q)
+        names: dict = {}
+        joints = []
+        world_pose: dict = {}
+        for entry in template:
+            name, parent, axis, p_off = entry[0], entry[1], entry[2], entry[3]
+            shapes = entry[4:]
+            if parent is None:
+                pose = root_pose
+            else:
+                pose = tf_mul(world_pose[parent], tf(p_off))
+            world_pose[name] = pose
+            mass = rng.uniform(0.5, 5.0)
+            real_shapes = [s for s in shapes if s is not None]
+            if real_shapes:
+                body = b.add_body(pose)
+                vol = sum(
+                    shape_volume(GEO_CAPSULE if s[0] == "capsule" else GEO_SPHERE if s[0] == "sphere" else GEO_BOX,
+                                 (s[3], s[4], 0.0) if s[0] == "capsule" else (s[2], 0, 0) if s[0] == "sphere" else s[2])
+                    for s in real_shapes
+                )
+                density = mass / vol
+            else:
+                r = 0.05
+                body = b.add_body(pose, mass=mass, inertia=np.eye(3) * (0.4 * mass * r * r))
+                density = 0.0
+            names[name] = body
+            # joint
+            if parent is None:
+                joints.append(b.add_free_joint(body))
+            else:
+                lim = 2.0
+                dof = DofCfg(axis=axis, target_ke=200.0, target_kd=10.0, actuator_mode=1, limit_lower=-lim,
+                             limit_upper=lim, limit_ke=1.0e4, limit_kd=10.0, armature=0.01, effort_limit=300.0)
+                joints.append(b.add_joint(names[parent], body, SAP_JOINT_REVOLUTE, angular_axes=[dof],
+                                          parent_xform=tf(p_off)))
+            mu = rng.uniform(0.6, 1.2)
+            group = 1
+            if name == "l_foot":
+                group = 2
+            elif name == "r_foot":
+                group = 3
+            cfg = ShapeCfg(density=density, ke=2.0e5, tau=0.005, mu=mu, gap=contact_gap, collision_group=group)
+            for s in real_shapes:
+                if s[0] == "capsule":
+                    b.add_shape(body, GEO_CAPSULE, tf(s[1], quat_z_to(s[2])), (s[3], s[4], 0.0), cfg)
+                elif s[0] == "sphere":
+                    b.add_shape(body, GEO_SPHERE, tf(s[1]), (s[2], 0.0, 0.0), cfg)
+                else:
+                    b.add_shape(body, GEO_BOX, tf(s[1]), tuple(s[2]), cfg)
+        b.add_articulation(joints)
+        # initial joint state (public order)
+        lin = rng.normal(0.0, vscale, size=3)
+        angv = rng.normal(0.0, vscale, size=3)
+        _set_free_joint_state(b, joints[0], root_pose, np.concatenate([lin, angv]))
+        for j in joints[1:]:
+            qv = rng.uniform(-qscale, qscale)
+            b.joint_q[b.joint_q_start[j]] = qv
+            b.joint_qd[b.joint_qd_start[j]] = rng.normal(0.0, vscale)
+            b.dofs[b.joint_qd_start[j]].target_pos = qv
+    b.current_world = -1
+    b.add_ground_plane(ShapeCfg(density=0.0, ke=2.0e5, tau=0.005, mu=1.0, gap=contact_gap, collision_group=-1))
+    return _with_consistent_body_q(b.finalize(device))
+
+
+def _with_consistent_body_q(model: SapModel) -> SapModel:
+    """Recompute body_q/body_qd from joint_q/joint_qd with the public FK."""
+    from .kinematics_host import eval_fk_public
+
+    body_q, body_qd = eval_fk_public(model, model.joint_q, model.joint_qd)
+    return replace(model, body_q=body_q.astype(np.float32), body_qd=body_qd.astype(np.float32))
+
+
+__all__ = ["SceneBuilder", "ShapeCfg", "DofCfg", "build_ant_scene", "build_humanoid_scene"]

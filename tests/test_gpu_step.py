@@ -1,0 +1,101 @@
+"""GPU step parity: one SolverSAP.step on the MI355X vs the float64 oracle from the same input.
+
+Tolerances (north star): post-step generalized velocities within 1e-5
+relative (max-norm, relative to max(|v|, 1 m/s)), positions within 1e-5, and
+every world converged by the SAP optimality / cost criterion.
+"""
+import numpy as np
+import pytest
+
+from oracle import sap_oracle as so
+from tests.helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+VEL_TOL = 1.0e-5
+
+
+def _scene(name, worlds, seed):
+    from sap_warp_amd.scenes import build_ant_scene, build_humanoid_scene
+
+    if name == "ant":
+        return build_ant_scene(worlds, seed=seed, device="cuda:0"), 1.0e-3, 16
+    return build_humanoid_scene(worlds, seed=seed, device="cuda:0"), 2.0e-3, 64
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _contacts_np(c):
+    count = int(_np(c.rigid_contact_count)[0])
+    return dict(count=count, world_start=_np(c.rigid_contact_world_start), shape0=_np(c.rigid_contact_shape0),
+                shape1=_np(c.rigid_contact_shape1), point0=_np(c.rigid_contact_point0),
+                point1=_np(c.rigid_contact_point1), normal=_np(c.rigid_contact_normal),
+                margin0=_np(c.rigid_contact_margin0), margin1=_np(c.rigid_contact_margin1))
+
+
+def _state_np(s):
+    return dict(joint_q=_np(s.joint_q), joint_qd=_np(s.joint_qd), body_q=_np(s.body_q), body_qd=_np(s.body_qd),
+                body_f=_np(s.body_f))
+
+
+def _control_np(c):
+    return dict(joint_f=_np(c.joint_f), joint_target_pos=_np(c.joint_target_pos),
+                joint_target_vel=_np(c.joint_target_vel), joint_act=_np(c.joint_act))
+
+
+def _run_and_compare(scene, preset, ls, worlds, steps, seed=0):
+    import torch
+
+    from sap_warp_amd.collision import SapCollisionPipeline
+    from sap_warp_amd.solver_sap import SolverSAP
+
+    model, dt, C = _scene(scene, worlds, seed)
+    solver = SolverSAP(model, max_rigid_contact=C, line_search_variant=ls, contact_preset_variant=preset,
+                       collect_iteration_stats=True)
+    pipe = SapCollisionPipeline(model, max_contacts_per_world=C)
+    contacts = pipe.contacts()
+    s0, s1 = model.state(), model.state()
+    ctl = model.control()
+    opts = so.StepOptions(preset=preset, line_search=ls, max_rigid_contact=C)
+    v_guess = None
+    worst = dict(v=0.0, vstar=0.0, q=0.0, qd=0.0, bq=0.0)
+    for k in range(steps):
+        pipe.collide(s0, contacts)
+        st_in = _state_np(s0)
+        c_np = _contacts_np(contacts)
+        solver.step(s0, s1, ctl, contacts, dt)
+        torch.cuda.synchronize()
+        stats = solver.last_contact_solve_result
+        ref = so.step(model, st_in, _control_np(ctl), c_np, dt, opts, v_guess=v_guess)
+        v_gpu = _np(solver.read_buffer(0))
+        vs_gpu = _np(solver.read_buffer(1))
+        for w in range(model.world_count):
+            r = ref["per_world"][w]
+            d0, d1 = r["free_motion"].v_star.shape[0] * w, r["free_motion"].v_star.shape[0] * (w + 1)
+            worst["vstar"] = max(worst["vstar"], rel_err(vs_gpu[d0:d1], r["v_star"]))
+            worst["v"] = max(worst["v"], rel_err(v_gpu[d0:d1], r["v_sap"]))
+            assert stats.converged_env[w] == 1, f"world {w} did not converge at step {k}"
+            assert r["result"].converged, f"oracle world {w} did not converge at step {k}"
+        worst["q"] = max(worst["q"], rel_err(_np(s1.joint_q), ref["joint_q"]))
+        worst["qd"] = max(worst["qd"], rel_err(_np(s1.joint_qd), ref["joint_qd"]))
+        worst["bq"] = max(worst["bq"], rel_err(_np(s1.body_q), ref["body_q"]))
+        v_guess = v_gpu.astype(np.float64)
+        s0, s1 = s1, s0
+    print(scene, preset, ls, {k: f"{v:.2e}" for k, v in worst.items()})
+    assert worst["vstar"] <= VEL_TOL, worst
+    assert worst["v"] <= VEL_TOL, worst
+    assert worst["qd"] <= VEL_TOL, worst
+    assert worst["q"] <= VEL_TOL, worst
+    assert worst["bq"] <= VEL_TOL, worst
+
+
+@pytest.mark.parametrize("preset,ls", [("approx32", "armijo_decay"), ("approx32", "exact_root"),
+                                       ("drake", "exact_root"), ("drake", "armijo_decay")])
+def test_humanoid_step_parity(gpu_device, preset, ls):
+    _run_and_compare("humanoid", preset, ls, worlds=6, steps=5)
+
+
+def test_ant_step_parity(gpu_device):
+    _run_and_compare("ant", "approx32", "armijo_decay", worlds=8, steps=5)
