@@ -1,0 +1,240 @@
+#!/usr/bin/env python
+"""World-steps/s of the SAP contact step on one MI355X.
+
+Workload: the batched plain-SAP humanoid configuration — 4096 humanoid
+worlds (floating base + 29 revolute joints, 35 DOF, 30 bodies, capsule /
+sphere / box links), float32 state, dt = 2e-3, PD position targets driven by
+a random walk (sigma = 0.05 rad per step), exact-root line search, approx32
+preset, 64 contacts per world capacity.  One step = collide (AABB + explicit
+pairs + primitive narrowphase) + SolverSAP.step (free motion, contact
+Jacobian, Newton solve, integration, public FK) for every world, with all
+inputs resident in HBM.
+
+Prints one JSON line (metric / value / roofline / cpu_baseline ...).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3   # MI355X dense FP32 (vector and matrix)
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+
+
+def step_flops(n, nb, nc, k, t):
+    """Algorithmic FP operations of one world-step (free motion + Jacobian + k Newton iterations).
+
+    Per-unit figures: free motion 2*6nb*6n + 2*6nb*n(n+1)/2 + n^3/3 + 2n^2 + 600nb;
+    contact Jacobian 12*nc*n; per Newton iteration 3*(2*3nc*n) + 2*3nc*n(n+1)/2
+    + 18nc*n + 4n^2 + n^3/3 + 2n^2 + 150nc; per line-search trial 150nc + 10n.
+    """
+    fm = 2 * 6 * nb * 6 * n + 2 * 6 * nb * n * (n + 1) / 2 + n ** 3 / 3 + 2 * n * n + 600 * nb
+    jac = 12 * nc * n
+    newton = 3 * (2 * 3 * nc * n) + 2 * 3 * nc * n * (n + 1) / 2 + 18 * nc * n + 4 * n * n + n ** 3 / 3 + 2 * n * n + 150 * nc
+    return fm + jac + k * newton + t * (150 * nc + 10 * n)
+
+
+def collide_bytes(model, contacts_written, S, P):
+    """Compulsory HBM bytes of one collide launch (body poses, shape data, pair template, contacts out)."""
+    W = model.world_count
+    body = model.body_count * 28
+    shapes = W * S * (28 + 12 + 4 * 5 + 4)
+    pairs = P * 8
+    out = contacts_written * (4 * 2 + 12 * 3 + 8) * 2 + (W + 1) * 4  # staging write + compact read/write
+    return body + shapes + pairs + out
+
+
+def read_traffic():
+    """HBM bytes per step-kernel launch from the committed rocprofv3 PMC summary (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "sapx_step_humanoid4096_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch"):
+                return float(line.split("=")[1])
+    return None
+
+
+def cpu_baseline(worlds: int, steps: int, seed: int, dt: float, line_search: str):
+    """Time the float64 oracle (CPU restatement of the reference step) on a bounded sample."""
+    import numpy as np
+
+    try:
+        from threadpoolctl import threadpool_limits
+    except Exception:  # pragma: no cover
+        threadpool_limits = None
+    from oracle import collide as oc
+    from oracle import sap_oracle as so
+    from sap_warp_amd.scenes import build_humanoid_scene
+
+    model = build_humanoid_scene(worlds, seed=seed)
+    opts = so.StepOptions(preset="approx32", line_search=line_search, max_rigid_contact=64)
+    state = dict(joint_q=model.joint_q, joint_qd=model.joint_qd, body_q=model.body_q, body_qd=model.body_qd,
+                 body_f=np.zeros((model.body_count, 6), np.float32))
+    control = dict(joint_f=model.joint_f, joint_target_pos=model.joint_target_pos.copy(),
+                   joint_target_vel=model.joint_target_vel, joint_act=model.joint_act)
+    import contextlib
+
+    guess = None
+    with (threadpool_limits(1) if threadpool_limits else contextlib.nullcontext()):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            c = oc.collide(model, state["body_q"], 64)
+            out = so.step(model, state, control, c, dt, opts, v_guess=guess)
+            guess = out["v_sap"]
+            state = dict(joint_q=out["joint_q"].astype(np.float32), joint_qd=out["joint_qd"].astype(np.float32),
+                         body_q=out["body_q"].astype(np.float32), body_qd=out["body_qd"].astype(np.float32),
+                         body_f=state["body_f"])
+        el = time.perf_counter() - t0
+    return worlds * steps / el, el
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--worlds", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--line-search", default="exact_root", choices=["exact_root", "armijo_decay"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-worlds", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; worlds shard trivially across GPUs "
+                                   "(see DESIGN.md) but no multi-GPU path is built", "n_gpus": args.gpus}))
+        return 2
+
+    import numpy as np
+    import torch
+
+    from sap_warp_amd.collision import SapCollisionPipeline
+    from sap_warp_amd.scenes import build_humanoid_scene
+    from sap_warp_amd.solver_sap import SolverSAP
+
+    dev = "cuda:0"
+    torch.cuda.set_device(0)
+    dt = 2.0e-3
+    C = 64
+    W = args.worlds
+    model = build_humanoid_scene(W, seed=args.seed, device=dev)
+    solver = SolverSAP(model, max_rigid_contact=C, line_search_variant=args.line_search)
+    pipe = SapCollisionPipeline(model, max_contacts_per_world=C)
+    contacts = pipe.contacts()
+    s0, s1 = model.state(), model.state()
+    ctl = model.control()
+    n = model.dof_per_env
+    total = args.warmup + args.steps
+    # PD target random walk, sigma = 0.05 rad/step on the revolute dofs, precomputed on the GPU
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1234 + args.seed)
+    mask = torch.zeros(n, device=dev)
+    mask[6:] = 1.0
+    incr = torch.randn((total, W, n), generator=gen, device=dev) * 0.05 * mask
+    incr = incr.reshape(total, -1).contiguous()
+    contact_acc = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def one_step(k, a, b):
+        ctl.joint_target_pos.add_(incr[k])
+        pipe.collide(a, contacts)
+        solver.step(a, b, ctl, contacts, dt)
+
+    for k in range(args.warmup):
+        one_step(k, s0, s1)
+        s0, s1 = s1, s0
+    torch.cuda.synchronize()
+    solver.iteration_histogram(reset=True)
+    ev_start = torch.cuda.Event(enable_timing=True)
+    ev_end = torch.cuda.Event(enable_timing=True)
+    ev_s = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ev_e = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ev_c = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    ev_start.record()
+    for i in range(args.steps):
+        k = args.warmup + i
+        ctl.joint_target_pos.add_(incr[k])
+        ev_c[i].record()
+        pipe.collide(s0, contacts)
+        ev_s[i].record()
+        solver.step(s0, s1, ctl, contacts, dt)
+        ev_e[i].record()
+        contact_acc.add_(contacts.rigid_contact_count)
+        s0, s1 = s1, s0
+    ev_end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall0
+    ms_total = ev_start.elapsed_time(ev_end)
+    step_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_s, ev_e)]))
+    collide_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_c, ev_s)]))
+    hist = solver.iteration_histogram(reset=True)
+    ws = int(hist[257])
+    iters = hist[:256].astype(np.float64)
+    mean_it = float((iters * np.arange(256)).sum() / max(ws, 1))
+    cdf = np.cumsum(iters) / max(ws, 1)
+    p99_it = int(np.searchsorted(cdf, 0.99))
+    ls_per_ws = float(hist[256]) / max(ws, 1)
+    mean_nc = float(contact_acc.item()) / (args.steps * W)
+    stats = solver.iteration_stats()
+    converged_frac = float(np.mean(stats.converged_env == 1))
+    value = W * args.steps / (ms_total / 1000.0)
+    # roofline of the dominant kernel (the fused step): algorithmic FP32-class operations per launch
+    nb = model.bodies_per_env
+    flops_launch = W * step_flops(n, nb, mean_nc, mean_it, ls_per_ws)
+    achieved_tflops = flops_launch / (step_ms / 1000.0) / 1.0e12
+    S_world = (model.shape_count - 1) // W
+    P_world = model.shape_pairs.shape[0] // W
+    cbytes = collide_bytes(model, mean_nc * W, S_world, P_world)
+    collide_gbs = cbytes / (collide_ms / 1000.0) / 1.0e9
+    out = {
+        "metric": "world-steps/s at 4096 humanoid worlds; mean Newton iters, learned vs plain",
+        "value": round(value, 1),
+        "unit": "world-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_total / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32/f64",
+        "data": "synthetic (seeded humanoid batch, random-walk PD targets)",
+        "config": {"workload": "batched plain SAP, 4096 humanoid worlds (35 DOF), dt=2e-3, approx32 preset, "
+                               f"{args.line_search} line search, C=64 contacts/world",
+                   "worlds": W, "dof_per_world": n, "bodies_per_world": nb, "dt": dt, "steps_timed": args.steps},
+        "newton_iterations": {"mean": round(mean_it, 3), "p99": p99_it, "line_search_trials_per_world_step":
+                              round(ls_per_ws, 3), "world_steps": ws, "warm_start": "previous step (plain)"},
+        "mean_contacts_per_world": round(mean_nc, 3),
+        "converged_fraction_last_step": converged_frac,
+        "kernel_ms": {"sapx_step_kernel": round(step_ms, 4), "collide_total": round(collide_ms, 4)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 4), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 6),
+                     "traffic": read_traffic(), "kernel": "sapx_step_kernel",
+                     "flops_per_launch": round(flops_launch)},
+        "roofline_collide": {"bound": "hbm", "achieved": round(collide_gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                             "frac": round(collide_gbs / PEAK_HBM_GBS, 6), "bytes_per_launch": round(cbytes)},
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu_baseline:
+        v_cpu, el = cpu_baseline(args.cpu_worlds, args.cpu_steps, args.seed, dt, args.line_search)
+        out["cpu_baseline"] = {"value": round(v_cpu, 3), "unit": "world-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"float64 numpy oracle, first {args.cpu_worlds} humanoid worlds x "
+                                         f"{args.cpu_steps} steps (collide + step), {el:.1f} s"}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

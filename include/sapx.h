@@ -213,6 +213,18 @@ int32_t sapx_get_stats(SapxSolver* solver, int32_t* newton_iterations, int32_t* 
                        int32_t* contact_count, int32_t* flags, void* stream);
 
 /*
+ * Accumulated Newton-iteration histogram over every world-step since the last
+ * reset, copied to HOST: hist[0..255] = number of world-steps that took that
+ * many Newton iterations (255 = 255 or more), hist[256] = total line-search
+ * iterations, hist[257] = world-steps counted.  The reference has no
+ * counterpart (its per-world counters, contact_solve.py:224-229, are
+ * overwritten every step); it serves the iteration statistics of the
+ * benchmark without a per-step synchronisation.  Synchronises the stream;
+ * reset != 0 clears the accumulators afterwards.
+ */
+int32_t sapx_get_histogram(SapxSolver* solver, int64_t* hist, int32_t reset, void* stream);
+
+/*
  * Copy an internal per-world buffer of the last step into caller DEVICE
  * memory (parity/debug; SapContactJacobian.capture_local_snapshots,
  * sim/contact_jacobian.py:1611-1626).  which: 0 = v (f64 dof_count, SAP

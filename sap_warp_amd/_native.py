@@ -99,6 +99,7 @@ EXPORTS = {
     "sapx_step": (_I, [_P, ctypes.POINTER(SapxStateView), ctypes.POINTER(SapxStateView),
                        ctypes.POINTER(SapxControlView), ctypes.POINTER(SapxContactsView), _D, _P, _P]),
     "sapx_get_stats": (_I, [_P, _P, _P, _P, _P, _P]),
+    "sapx_get_histogram": (_I, [_P, _P, _I, _P]),
     "sapx_solver_read": (_I, [_P, _I, _P, ctypes.c_int64, _P]),
     "sapx_pipeline_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
     "sapx_pipeline_destroy": (None, [_P]),

@@ -71,6 +71,7 @@ struct SapxSolver {
     double* v_guess = nullptr;
     int use_guess = 0;
     int* st = nullptr;  // 4 x W
+    unsigned long long* hist = nullptr;  // 258 accumulators
     double* dbg = nullptr;
     size_t dbg_off[7] = {0};
     size_t dbg_size[7] = {0};
@@ -94,7 +95,7 @@ struct SapxSolver {
     void* cub_tmp = nullptr;
     size_t cub_tmp_bytes = 0;
     ~SapxSolver() {
-        void* ptrs[] = {v_guess, st, dbg, g_keys, g_keys_out, g_vals, g_vals_out, g_world_start, g_shape0, g_shape1,
+        void* ptrs[] = {v_guess, st, hist, dbg, g_keys, g_keys_out, g_vals, g_vals_out, g_world_start, g_shape0, g_shape1,
                         g_p0, g_p1, g_n, g_m0, g_m1, cub_tmp};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
@@ -406,6 +407,11 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
     }
     (void)hipMemset(s->v_guess, 0, D * sizeof(double));
     (void)hipMemset(s->st, 0, 4 * W * sizeof(int));
+    if (hipMalloc(&s->hist, 258 * sizeof(unsigned long long)) != hipSuccess) {
+        delete s;
+        return fail(SAPX_ERR_HIP, "hipMalloc failed for the iteration histogram");
+    }
+    (void)hipMemset(s->hist, 0, 258 * sizeof(unsigned long long));
     const size_t sizes[7] = {D, D, D, W * C * 3, W * C, W * C, D};
     size_t tot = 0;
     for (int i = 0; i < 7; ++i) {
@@ -525,6 +531,7 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
     P.st_ls = s->st + W;
     P.st_nc = s->st + 2 * W;
     P.st_flags = s->st + 3 * W;
+    P.st_hist = s->hist;
     P.dbg_v = s->dbg + s->dbg_off[0];
     P.dbg_vstar = s->dbg + s->dbg_off[1];
     P.dbg_v0 = s->dbg + s->dbg_off[2];
@@ -575,6 +582,15 @@ int32_t sapx_get_stats(SapxSolver* s, int32_t* newton_iterations, int32_t* line_
     for (int k = 0; k < 4; ++k)
         if (dst[k]) HIP_TRY(hipMemcpyAsync(dst[k], s->st + k * W, W * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    return SAPX_OK;
+}
+
+int32_t sapx_get_histogram(SapxSolver* s, int64_t* hist, int32_t reset, void* stream) {
+    if (!s || !hist) return fail(SAPX_ERR_INVALID, "null argument");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpyAsync(hist, s->hist, 258 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (reset) HIP_TRY(hipMemsetAsync(s->hist, 0, 258 * sizeof(unsigned long long), st));
     return SAPX_OK;
 }
 

@@ -34,6 +34,7 @@ struct StepParams {
     int* st_ls;
     int* st_nc;
     int* st_flags;
+    unsigned long long* st_hist;  // 256 iteration bins + [256] line-search total + [257] world-steps
     double* dbg_v;
     double* dbg_vstar;
     double* dbg_v0;

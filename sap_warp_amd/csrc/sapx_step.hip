@@ -1444,6 +1444,11 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         P.st_ls[w] = ls_total;
         P.st_nc[w] = nc;
         P.st_flags[w] = flags | ((ls_status & 0xff) << 8);
+        if (P.st_hist) {
+            atomicAdd(&P.st_hist[iteration < 255 ? iteration : 255], 1ull);
+            atomicAdd(&P.st_hist[256], (unsigned long long)ls_total);
+            atomicAdd(&P.st_hist[257], 1ull);
+        }
     }
 }
 

@@ -648,97 +648,193 @@ def build_humanoid_scene(
     Tree: legs 2 x 6 (hip pitch/roll/yaw, knee, ankle pitch/roll), waist 3,
     arms 2 x 7 (shoulder pitch/roll/yaw, elbow, wrist roll/pitch/yaw); the
     head is a sphere rigidly attached to the torso.  Every body mass is
-    U(0.5, 5) kg (shape density is set to hit it; shapeless joint links get a
-    0.05 m sphere-like inertia).  Per-shape friction mu ~ U(0.6, 1.2), stiffness
-    2e5 N/m, dissipation 5 ms (pair: 1e5 N/m, 10 ms).  Links are capsules and
-    spheres; the feet are boxes in their own collision groups (2 and 3) so they
-    touch only the ground (group -1).  PD position drives on every revolute
-    DOF (ke=200, kd=10, effort limit 300), limits +-2 rad, armature 0.01.
-    Half of the worlds start standing on the ground, half start falling from
-    U(1.0, 1.6) m with tilted bases and random joint angles and velocities.
+    U(0.5, 5) kg (the shapes' density is scaled to it; shapeless joint links
+    get a 0.05 m sphere-like inertia).  Per-body friction mu ~ U(0.6, 1.2),
+    per-shape stiffness 2e5 N/m and dissipation 5 ms (pair: 1e5 N/m, 10 ms).
+    Links are capsules and spheres; the feet are boxes in their own collision
+    groups (2 and 3) so they touch only the ground (group -1).  PD position
+    drives on every revolute DOF (ke=200, kd=10, effort limit 300, target =
+    initial angle), limits +-2 rad, armature 0.01.  Half of the worlds start
+    standing on the ground, half start falling from U(1.0, 1.6) m with tilted
+    bases and random joint angles and velocities.
+
+    One template world is built with the builder (unit masses) and replicated
+    with vectorised per-world parameters; inertia scales linearly with mass
+    for a fixed geometry, so this equals building every world separately.
     """
     rng = np.random.Generator(np.random.PCG64(seed))
-    b = SceneBuilder(rigid_gap=contact_gap)
+    W = int(num_worlds)
+    tb = SceneBuilder(rigid_gap=contact_gap)
+    tb.world_count = 1
+    tb.current_world = 0
     template = _humanoid_template()
-    b.world_count = num_worlds
-    for w in range(num_worlds):
-        b.current_world = w
-        off = _grid_offset(w, num_worlds, spacing)
-        falling = bool(rng.random() < 0.5)
-        if falling:
-            height = rng.uniform(1.0, 1.6)
-            tilt_axis = np.array([rng.normal(), rng.normal(), 0.0])
-            tilt = rng.uniform(0.0, 0.6)
-            root_q = quat_mul(quat_from_axis_angle((0, 0, 1), rng.uniform(-math.pi, math.pi)),
-                              quat_from_axis_angle(tilt_axis, tilt))
-            qscale, vscale = 0.3, 0.5
-        else:
-            height = HUMANOID_STAND_HEIGHT + rng.uniform(0.0, 0.01)
-            root_q = quat_from_axis_angle((0, 0, 1), rng.uniform(-math.pi, math.pi))
-            qscale, vscale = 0.03, 0.05
-        root_pose = tf(off + np.array([0.0, 0.0, height]), root_q)
-        names: dict = {}
-        joints = []
-        world_pose: dict = {}
-        for entry in template:
-            name, parent, axis, p_off = entry[0], entry[1], entry[2], entry[3]
-            shapes = entry[4:]
-            if parent is None:
-                pose = root_pose
-            else:
-                pose = tf_mul(world_pose[parent], tf(p_off))
-            world_pose[name] = pose
-            mass = rng.uniform(0.5, 5.0)
-            real_shapes = [s for s in shapes if s is not None]
-            if real_shapes:
-                body = b.add_body(pose)
-                vol = sum(
-                    shape_volume(GEO_CAPSULE if s[0] == "capsule" else GEO_SPHERE if s[0] == "sphere" else GEO_BOX,
-                                 (s[3], s[4], 0.0) if s[0] == "capsule" else (s[2], 0, 0) if s[0] == "sphere" else s[2])
-                    for s in real_shapes
-                )
-                density = mass / vol
-            else:
-                r = 0.05
-                body = b.add_body(pose, mass=mass, inertia=np.eye(3) * (0.4 * mass * r * r))
-                density = 0.0
-            names[name] = body
-            # joint
-            if parent is None:
-                joints.append(b.add_free_joint(body))
-            else:
-                lim = 2.0
-                dof = DofCfg(axis=axis, target_ke=200.0, target_kd=10.0, actuator_mode=1, limit_lower=-lim,
-                             limit_upper=lim, limit_ke=1.0e4, limit_kd=10.0, armature=0.01, effort_limit=300.0)
-                joints.append(b.add_joint(names[parent], body, SAP_JOINT_REVOLUTE, angular_axes=[dof],
-                                          parent_xform=tf(p_off)))
-            mu = rng.uniform(0.6, 1.2)
-            group = 1
-            if name == "l_foot":
-                group = 2
-            elif name == "r_foot":
-                group = 3
-            cfg = ShapeCfg(density=density, ke=2.0e5, tau=0.005, mu=mu, gap=contact_gap, collision_group=group)
-            for s in real_shapes:
-                if s[0] == "capsule":
-                    b.add_shape(body, GEO_CAPSULE, tf(s[1], quat_z_to(s[2])), (s[3], s[4], 0.0), cfg)
-                elif s[0] == "sphere":
-                    b.add_shape(body, GEO_SPHERE, tf(s[1]), (s[2], 0.0, 0.0), cfg)
+    names: dict = {}
+    joints = []
+    world_pose: dict = {}
+    root_pose = tf((0.0, 0.0, HUMANOID_STAND_HEIGHT))
+    body_group = []
+    for entry in template:
+        name, parent, axis, p_off = entry[0], entry[1], entry[2], entry[3]
+        shapes = [x for x in entry[4:] if x is not None]
+        pose = root_pose if parent is None else tf_mul(world_pose[parent], tf(p_off))
+        world_pose[name] = pose
+        if shapes:
+            body = tb.add_body(pose)
+            vol = 0.0
+            for x in shapes:
+                if x[0] == "capsule":
+                    vol += shape_volume(GEO_CAPSULE, (x[3], x[4], 0.0))
+                elif x[0] == "sphere":
+                    vol += shape_volume(GEO_SPHERE, (x[2], 0.0, 0.0))
                 else:
-                    b.add_shape(body, GEO_BOX, tf(s[1]), tuple(s[2]), cfg)
-        b.add_articulation(joints)
-        # initial joint state (public order)
-        lin = rng.normal(0.0, vscale, size=3)
-        angv = rng.normal(0.0, vscale, size=3)
-        _set_free_joint_state(b, joints[0], root_pose, np.concatenate([lin, angv]))
-        for j in joints[1:]:
-            qv = rng.uniform(-qscale, qscale)
-            b.joint_q[b.joint_q_start[j]] = qv
-            b.joint_qd[b.joint_qd_start[j]] = rng.normal(0.0, vscale)
-            b.dofs[b.joint_qd_start[j]].target_pos = qv
-    b.current_world = -1
-    b.add_ground_plane(ShapeCfg(density=0.0, ke=2.0e5, tau=0.005, mu=1.0, gap=contact_gap, collision_group=-1))
-    return _with_consistent_body_q(b.finalize(device))
+                    vol += shape_volume(GEO_BOX, x[2])
+            density = 1.0 / vol
+        else:
+            r = 0.05
+            body = tb.add_body(pose, mass=1.0, inertia=np.eye(3) * (0.4 * r * r))
+            density = 0.0
+        names[name] = body
+        if parent is None:
+            joints.append(tb.add_free_joint(body))
+        else:
+            dof = DofCfg(axis=axis, target_ke=200.0, target_kd=10.0, actuator_mode=1, limit_lower=-2.0,
+                         limit_upper=2.0, limit_ke=1.0e4, limit_kd=10.0, armature=0.01, effort_limit=300.0)
+            joints.append(tb.add_joint(names[parent], body, SAP_JOINT_REVOLUTE, angular_axes=[dof],
+                                       parent_xform=tf(p_off)))
+        group = 2 if name == "l_foot" else (3 if name == "r_foot" else 1)
+        cfg = ShapeCfg(density=density, ke=2.0e5, tau=0.005, mu=1.0, gap=contact_gap, collision_group=group)
+        for x in shapes:
+            if x[0] == "capsule":
+                tb.add_shape(body, GEO_CAPSULE, tf(x[1], quat_z_to(x[2])), (x[3], x[4], 0.0), cfg)
+            elif x[0] == "sphere":
+                tb.add_shape(body, GEO_SPHERE, tf(x[1]), (x[2], 0.0, 0.0), cfg)
+            else:
+                tb.add_shape(body, GEO_BOX, tf(x[1]), tuple(x[2]), cfg)
+    tb.add_articulation(joints)
+    tb.current_world = -1
+    tb.add_ground_plane(ShapeCfg(density=0.0, ke=2.0e5, tau=0.005, mu=1.0, gap=contact_gap, collision_group=-1))
+    t = tb.finalize(device)
+    J, NB, N, NQ = t.joint_count, t.body_count, t.joint_dof_count, t.joint_coord_count
+    S = t.shape_count - 1  # world shapes; the ground plane is the last shape
+
+    # ---- per-world random parameters (vectorised, fixed draw order)
+    mass = rng.uniform(0.5, 5.0, size=(W, NB))
+    mu_body = rng.uniform(0.6, 1.2, size=(W, NB))
+    falling = rng.random(W) < 0.5
+    height = np.where(falling, rng.uniform(1.0, 1.6, size=W), HUMANOID_STAND_HEIGHT + rng.uniform(0.0, 0.01, size=W))
+    yaw = rng.uniform(-math.pi, math.pi, size=W)
+    tilt_axis = rng.normal(size=(W, 2))
+    tilt = np.where(falling, rng.uniform(0.0, 0.6, size=W), 0.0)
+    qscale = np.where(falling, 0.3, 0.03)
+    vscale = np.where(falling, 0.5, 0.05)
+    root_lin = rng.normal(size=(W, 3)) * vscale[:, None]
+    root_ang = rng.normal(size=(W, 3)) * vscale[:, None]
+    rev_q = rng.uniform(-1.0, 1.0, size=(W, J - 1)) * qscale[:, None]
+    rev_qd = rng.normal(size=(W, J - 1)) * vscale[:, None]
+
+    def tile(a, offset=None):
+        a = np.asarray(a)
+        reps = np.concatenate([a] * W, axis=0) if W > 1 else a.copy()
+        if offset is not None:
+            off = np.repeat(np.arange(W) * offset, a.shape[0]).reshape((-1,) + (1,) * (a.ndim - 1))
+            reps = np.where(reps >= 0, reps + off, reps)
+        return reps
+
+    shape_w = np.asarray(t.shape_transform[:S])
+    # root orientation: yaw * tilt
+    qr = np.zeros((W, 4))
+    for w in range(W):
+        ax = np.array([tilt_axis[w, 0], tilt_axis[w, 1], 0.0])
+        qr[w] = quat_mul(quat_from_axis_angle((0, 0, 1), yaw[w]), quat_from_axis_angle(ax, tilt[w]))
+    off = np.stack([_grid_offset(w, W, spacing) for w in range(W)]) if spacing else np.zeros((W, 3))
+    joint_q = np.tile(t.joint_q.astype(np.float64), (W, 1))
+    joint_qd = np.tile(t.joint_qd.astype(np.float64), (W, 1))
+    joint_q[:, 0:3] = off + np.stack([np.zeros(W), np.zeros(W), height], axis=1)
+    joint_q[:, 3:7] = qr
+    joint_qd[:, 0:3] = root_lin
+    joint_qd[:, 3:6] = root_ang
+    joint_q[:, 7:] = rev_q
+    joint_qd[:, 6:] = rev_qd
+    target_pos = np.tile(t.joint_target_pos.astype(np.float64), (W, 1))
+    target_pos[:, 6:] = rev_q
+    body_mass = mass.reshape(-1)
+    body_inertia = (np.tile(t.body_inertia, (W, 1, 1)) * body_mass[:, None, None]).astype(np.float32)
+    mu_shape = mu_body[:, t.shape_body[:S]].reshape(-1)
+    ground = S
+    shape_pairs_t = np.asarray(t.shape_pairs, dtype=np.int64)
+    pairs = []
+    for w in range(W):
+        pw = shape_pairs_t.copy()
+        pw = np.where(pw == ground, W * S, pw + w * S)
+        pairs.append(pw)
+    # the builder lists all ground pairs first (world -1 sorts first), then per-world pairs
+    pairs = np.concatenate(pairs, axis=0)
+    is_ground = np.any(pairs == W * S, axis=1)
+    pairs = np.concatenate([pairs[is_ground], pairs[~is_ground]], axis=0).astype(np.int32)
+
+    def cat_shapes(a_world, a_ground):
+        return np.concatenate([np.tile(a_world, (W,) + (1,) * (a_world.ndim - 1)), a_ground], axis=0)
+
+    model = SapModel(
+        device=device,
+        joint_count=J * W,
+        joint_dof_count=N * W,
+        joint_coord_count=NQ * W,
+        body_count=NB * W,
+        articulation_count=W,
+        world_count=W,
+        shape_count=S * W + 1,
+        joint_type=tile(t.joint_type).astype(np.int32),
+        joint_articulation=np.repeat(np.arange(W), J).astype(np.int32),
+        joint_parent=tile(t.joint_parent, NB).astype(np.int32),
+        joint_child=tile(t.joint_child, NB).astype(np.int32),
+        joint_q_start=np.concatenate([(np.arange(W)[:, None] * NQ + t.joint_q_start[:-1]).reshape(-1), [NQ * W]]).astype(np.int32),
+        joint_qd_start=np.concatenate([(np.arange(W)[:, None] * N + t.joint_qd_start[:-1]).reshape(-1), [N * W]]).astype(np.int32),
+        joint_dof_dim=tile(t.joint_dof_dim).astype(np.int32),
+        joint_axis=tile(t.joint_axis).astype(np.float32),
+        joint_X_p=tile(t.joint_X_p).astype(np.float32),
+        joint_X_c=tile(t.joint_X_c).astype(np.float32),
+        joint_ancestor=tile(t.joint_ancestor, J).astype(np.int32),
+        articulation_start=(np.arange(W + 1) * J).astype(np.int32),
+        joint_q=joint_q.reshape(-1).astype(np.float32),
+        joint_qd=joint_qd.reshape(-1).astype(np.float32),
+        body_q=np.zeros((NB * W, 7), np.float32),
+        body_qd=np.zeros((NB * W, 6), np.float32),
+        body_mass=body_mass.astype(np.float32),
+        body_inertia=body_inertia,
+        body_com=tile(t.body_com).astype(np.float32),
+        body_flags=tile(t.body_flags).astype(np.int32),
+        body_world=np.repeat(np.arange(W), NB).astype(np.int32),
+        gravity=np.tile(t.gravity[:1], (W, 1)).astype(np.float32),
+        joint_armature=tile(t.joint_armature).astype(np.float32),
+        shape_body=np.concatenate([tile(t.shape_body[:S], NB), [-1]]).astype(np.int32),
+        shape_material_mu=np.concatenate([mu_shape, t.shape_material_mu[S:]]).astype(np.float32),
+        shape_material_ke=cat_shapes(t.shape_material_ke[:S], t.shape_material_ke[S:]).astype(np.float32),
+        shape_material_tau=cat_shapes(t.shape_material_tau[:S], t.shape_material_tau[S:]).astype(np.float32),
+        joint_target_mode=tile(t.joint_target_mode).astype(np.int32),
+        joint_target_ke=tile(t.joint_target_ke).astype(np.float32),
+        joint_target_kd=tile(t.joint_target_kd).astype(np.float32),
+        joint_target_pos=target_pos.reshape(-1).astype(np.float32),
+        joint_target_vel=tile(t.joint_target_vel).astype(np.float32),
+        joint_f=np.zeros(N * W, np.float32),
+        joint_act=np.zeros(N * W, np.float32),
+        joint_effort_limit=tile(t.joint_effort_limit).astype(np.float32),
+        joint_limit_lower=tile(t.joint_limit_lower).astype(np.float32),
+        joint_limit_upper=tile(t.joint_limit_upper).astype(np.float32),
+        joint_limit_ke=tile(t.joint_limit_ke).astype(np.float32),
+        joint_limit_kd=tile(t.joint_limit_kd).astype(np.float32),
+        shape_transform=cat_shapes(shape_w, t.shape_transform[S:]).astype(np.float32),
+        shape_type=cat_shapes(t.shape_type[:S], t.shape_type[S:]).astype(np.int32),
+        shape_scale=cat_shapes(t.shape_scale[:S], t.shape_scale[S:]).astype(np.float32),
+        shape_flags=cat_shapes(t.shape_flags[:S], t.shape_flags[S:]).astype(np.int32),
+        shape_margin=cat_shapes(t.shape_margin[:S], t.shape_margin[S:]).astype(np.float32),
+        shape_gap=cat_shapes(t.shape_gap[:S], t.shape_gap[S:]).astype(np.float32),
+        shape_collision_radius=cat_shapes(t.shape_collision_radius[:S], t.shape_collision_radius[S:]).astype(np.float32),
+        shape_collision_group=cat_shapes(t.shape_collision_group[:S], t.shape_collision_group[S:]).astype(np.int32),
+        shape_world=np.concatenate([np.repeat(np.arange(W), S), [-1]]).astype(np.int32),
+        shape_pairs=pairs,
+    )
+    return _with_consistent_body_q(model)
 
 
 def _with_consistent_body_q(model: SapModel) -> SapModel:

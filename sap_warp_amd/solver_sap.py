@@ -398,6 +398,13 @@ class SolverSAP:
                   "sapx_get_stats")
         return SapContactSolveResult(*arrs)
 
+    def iteration_histogram(self, reset: bool = True) -> np.ndarray:
+        """Accumulated Newton-iteration histogram (``sapx_get_histogram``): 258 int64 entries."""
+        h = np.zeros(258, dtype=np.int64)
+        nat.check(self._lib.sapx_get_histogram(self._handle, h.ctypes.data, int(bool(reset)), nat.stream_ptr()),
+                  "sapx_get_histogram")
+        return h
+
     def read_buffer(self, which: int):
         """Copy an internal buffer of the last step (``sapx_solver_read``) into a new float64 tensor."""
         import torch
