@@ -1,0 +1,136 @@
+"""C ABI checks that need no GPU: exports, struct layouts, validation paths.
+
+* every function declared in ``include/sapx.h`` is exported by ``libsapx.so``;
+* the ctypes mirrors in ``sap_warp_amd/_native.py`` have the C layout (sizes
+  and field offsets printed by a tiny C program compiled against the header);
+* ``sapx_model_create`` rejects bad models before touching the device, with
+  the error classes the Python layer maps to ValueError / NotImplementedError.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from sap_warp_amd import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "sapx.h")
+
+
+def _declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sapx_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_function():
+    lib = nat.load()
+    names = _declared_functions()
+    assert len(names) >= 14
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(nat.EXPORTS), set(names) - set(nat.EXPORTS)
+
+
+def test_version_and_last_error():
+    lib = nat.load()
+    assert lib.sapx_version() >= 100
+    assert isinstance(lib.sapx_last_error(), bytes)
+
+
+STRUCTS = {
+    "SapxModelDesc": nat.SapxModelDesc,
+    "SapxSolverOpts": nat.SapxSolverOpts,
+    "SapxStateView": nat.SapxStateView,
+    "SapxControlView": nat.SapxControlView,
+    "SapxContactsView": nat.SapxContactsView,
+}
+
+
+def test_ctypes_layout_matches_header(tmp_path):
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sapx.h"', "int main(void) {"]
+    for cname, py in STRUCTS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in out:
+        if line:
+            k, v = line.rsplit(" ", 1)
+            got[k] = int(v)
+    for cname, py in STRUCTS.items():
+        assert got[f"{cname} size"] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[f"{cname}.{fname}"] == getattr(py, fname).offset, (cname, fname)
+
+
+def _desc_from(model):
+    keep = nat.HostArrays()
+    return nat.model_desc(model, keep), keep
+
+
+def test_model_create_rejects_ball_joints():
+    from sap_warp_amd.runtime import SAP_JOINT_BALL
+    from sap_warp_amd.scenes import DofCfg, SceneBuilder, ShapeCfg, tf
+    from sap_warp_amd.runtime import GEO_SPHERE
+
+    b = SceneBuilder()
+    b.current_world = 0
+    root = b.add_body(tf((0, 0, 1)))
+    b.add_shape(root, GEO_SPHERE, tf(), (0.1, 0, 0), ShapeCfg())
+    j0 = b.add_free_joint(root)
+    child = b.add_body(tf((0, 0, 0.5)))
+    b.add_shape(child, GEO_SPHERE, tf(), (0.1, 0, 0), ShapeCfg())
+    j1 = b.add_joint(root, child, SAP_JOINT_BALL, angular_axes=[DofCfg(axis=a) for a in ((1, 0, 0), (0, 1, 0), (0, 0, 1))],
+                     parent_xform=tf((0, 0, -0.5)))
+    b.add_articulation([j0, j1])
+    model = b.finalize()
+    lib = nat.load()
+    desc, _keep = _desc_from(model)
+    h = ctypes.c_void_p()
+    with pytest.raises(NotImplementedError, match="ball"):
+        nat.check(lib.sapx_model_create(ctypes.byref(desc), ctypes.byref(h)), "sapx_model_create")
+
+
+def test_model_create_rejects_unequal_worlds():
+    from sap_warp_amd.scenes import build_ant_scene
+
+    model = build_ant_scene(num_worlds=2, seed=1)
+    jt = model.joint_type.copy()
+    jt[model.joint_count // 2 + 1] = 0  # world 1's first hinge becomes prismatic
+    from dataclasses import replace
+
+    bad = replace(model, joint_type=jt)
+    lib = nat.load()
+    desc, _keep = _desc_from(bad)
+    h = ctypes.c_void_p()
+    with pytest.raises(NotImplementedError, match="topology"):
+        nat.check(lib.sapx_model_create(ctypes.byref(desc), ctypes.byref(h)), "sapx_model_create")
+
+
+def test_null_arguments_are_invalid():
+    lib = nat.load()
+    with pytest.raises(ValueError):
+        nat.check(lib.sapx_model_create(None, None), "sapx_model_create")
+    with pytest.raises(ValueError):
+        nat.check(lib.sapx_solver_read(None, 0, None, 0, None), "sapx_solver_read")
+    with pytest.raises(ValueError):
+        nat.check(lib.sapx_collide(None, None, None, None), "sapx_collide")
+
+
+def test_library_has_no_cpu_fallback_symbols():
+    """The shipped library is the HIP path only: no oracle symbols linked in."""
+    lib = nat.load()
+    assert not hasattr(lib, "sapx_oracle_collide")
+    assert np.dtype(np.int32).itemsize == 4
