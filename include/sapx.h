@@ -126,6 +126,8 @@ typedef struct SapxSolverOpts {
     int32_t contact_weight_mode;    /* SAPX_WEIGHT_* */
     int32_t contact_point_mode;     /* SAPX_POINT_* */
     int32_t position_integration;   /* SAPX_INTEGRATE_* */
+    int32_t capture_snapshots;      /* keep per-step internal buffers for sapx_solver_read
+                                       (capture_contact_jacobian_snapshots, sim/solver_sap.py:778) */
 } SapxSolverOpts;
 
 /* Public-order state (SapState, sim/sap_runtime.py:177-205), device memory. */
@@ -231,7 +233,8 @@ int32_t sapx_get_histogram(SapxSolver* solver, int64_t* hist, int32_t reset, voi
  * order), 1 = v_star (f64 dof_count), 2 = v0 (f64 dof_count), 3 = contact
  * impulses gamma (f64 world_count x C x 3), 4 = contact phi0 (f64 world_count
  * x C), 5 = contact weight (f64 world_count x C), 6 = dynamics matrix A
- * diagonal (f64 dof_count).  bytes must equal the buffer size.
+ * diagonal (f64 dof_count).  bytes must equal the buffer size.  Requires a
+ * solver created with capture_snapshots != 0 (SAPX_ERR_INVALID otherwise).
  */
 int32_t sapx_solver_read(SapxSolver* solver, int32_t which, void* dst, int64_t bytes, void* stream);
 

@@ -69,6 +69,7 @@ class SapxSolverOpts(ctypes.Structure):
         ("line_search", _I), ("line_search_max_iterations", _I),
         ("armijo_c", _D), ("rho", _D), ("line_search_relative_slop", _D),
         ("contact_weight_mode", _I), ("contact_point_mode", _I), ("position_integration", _I),
+        ("capture_snapshots", _I),
     ]
 
 

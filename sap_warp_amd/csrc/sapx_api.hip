@@ -76,7 +76,7 @@ struct SapxSolver {
     size_t dbg_off[7] = {0};
     size_t dbg_size[7] = {0};
     int npad = 0, js = 0;
-    int lds[15] = {0};
+    int lds[16] = {0};
     int smem_bytes = 0;
     // grouping buffers for ungrouped contact lists
     int group_capacity = 0;
@@ -389,12 +389,13 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
     s->lds[6] = take((size_t)C * 9 * 8);                                                 // cv
     s->lds[7] = take((size_t)C * 9 * 4);                                                 // cG
     s->lds[8] = take((size_t)C * 2 * 4);                                                 // cb
-    s->lds[9] = take((size_t)10 * npad * 8);                                             // d64
-    s->lds[10] = take((size_t)12 * npad * 4);                                            // d32
+    s->lds[9] = take((size_t)12 * npad * 8);                                             // d64 (V_NUM rows)
+    s->lds[10] = take((size_t)16 * npad * 4);                                            // d32 (>= F_NUM rows)
     s->lds[11] = take((size_t)4 * npad * 4);                                             // dflag
     s->lds[12] = take((size_t)8 * 8 * 8);                                                // red
     s->lds[13] = take((size_t)2 * dm.NQ * 4);                                            // q
-    s->lds[14] = take(64);                                                               // misc
+    s->lds[14] = take((size_t)(dm.J * 16 + n * 4 + nb * 4 + nb * 9 + nb) * 4);             // per-world model params
+    s->lds[15] = take((size_t)nb * 8 * 8);                                               // bqd (f64 poses)
     s->smem_bytes = (int)off;
     if (s->smem_bytes > 160 * 1024) {
         delete s;
@@ -419,11 +420,13 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
         s->dbg_size[i] = sizes[i] * sizeof(double);
         tot += sizes[i];
     }
-    if (hipMalloc(&s->dbg, tot * sizeof(double)) != hipSuccess) {
-        delete s;
-        return fail(SAPX_ERR_HIP, "hipMalloc failed for solver debug buffers");
+    if (opts->capture_snapshots) {
+        if (hipMalloc(&s->dbg, tot * sizeof(double)) != hipSuccess) {
+            delete s;
+            return fail(SAPX_ERR_HIP, "hipMalloc failed for solver snapshot buffers");
+        }
+        (void)hipMemset(s->dbg, 0, tot * sizeof(double));
     }
-    (void)hipMemset(s->dbg, 0, tot * sizeof(double));
     s->use_guess = 0;
     *out = s;
     return SAPX_OK;
@@ -532,13 +535,15 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
     P.st_nc = s->st + 2 * W;
     P.st_flags = s->st + 3 * W;
     P.st_hist = s->hist;
-    P.dbg_v = s->dbg + s->dbg_off[0];
-    P.dbg_vstar = s->dbg + s->dbg_off[1];
-    P.dbg_v0 = s->dbg + s->dbg_off[2];
-    P.dbg_gamma = s->dbg + s->dbg_off[3];
-    P.dbg_phi0 = s->dbg + s->dbg_off[4];
-    P.dbg_w = s->dbg + s->dbg_off[5];
-    P.dbg_adiag = s->dbg + s->dbg_off[6];
+    if (s->dbg) {
+        P.dbg_v = s->dbg + s->dbg_off[0];
+        P.dbg_vstar = s->dbg + s->dbg_off[1];
+        P.dbg_v0 = s->dbg + s->dbg_off[2];
+        P.dbg_gamma = s->dbg + s->dbg_off[3];
+        P.dbg_phi0 = s->dbg + s->dbg_off[4];
+        P.dbg_w = s->dbg + s->dbg_off[5];
+        P.dbg_adiag = s->dbg + s->dbg_off[6];
+    }
     P.dt = dt;
     P.dt_world = dt_world;
     P.C = s->opts.max_rigid_contact;
@@ -567,6 +572,7 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
     P.o_A = s->lds[0]; P.o_H = s->lds[1]; P.o_J = s->lds[2]; P.o_bq = s->lds[3]; P.o_S = s->lds[4];
     P.o_ck = s->lds[5]; P.o_cv = s->lds[6]; P.o_cG = s->lds[7]; P.o_cb = s->lds[8]; P.o_dv64 = s->lds[9];
     P.o_dv32 = s->lds[10]; P.o_dflag = s->lds[11]; P.o_red = s->lds[12]; P.o_q = s->lds[13]; P.o_misc = s->lds[14];
+    P.o_bqd = s->lds[15];
     P.smem_bytes = s->smem_bytes;
     HIP_TRY(launch_step(P, st));
     s->use_guess = 1;
@@ -597,6 +603,7 @@ int32_t sapx_get_histogram(SapxSolver* s, int64_t* hist, int32_t reset, void* st
 int32_t sapx_solver_read(SapxSolver* s, int32_t which, void* dst, int64_t bytes, void* stream) {
     if (!s || !dst) return fail(SAPX_ERR_INVALID, "null argument");
     if (which < 0 || which > 6) return fail(SAPX_ERR_INVALID, "unknown buffer id");
+    if (!s->dbg) return fail(SAPX_ERR_INVALID, "solver was created without capture_snapshots");
     if ((size_t)bytes != s->dbg_size[which]) return fail(SAPX_ERR_INVALID, "byte count does not match the buffer");
     HIP_TRY(hipMemcpyAsync(dst, s->dbg + s->dbg_off[which], (size_t)bytes, hipMemcpyDeviceToDevice,
                            reinterpret_cast<hipStream_t>(stream)));

@@ -167,4 +167,69 @@ __device__ __forceinline__ bool tf_is_identity(const float* a) {
            a[6] == 1.0f;
 }
 
+// ---------------------------------------------------------------------------
+// float64 transforms for the free-motion forward kinematics (the reference
+// evaluates it with transformd / quatd, sap_helpers.py:575-683)
+// ---------------------------------------------------------------------------
+struct D3 {
+    double x, y, z;
+};
+struct DQ {
+    double x, y, z, w;
+};
+struct DTF {
+    D3 p;
+    DQ q;
+};
+__device__ __forceinline__ D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+__device__ __forceinline__ D3 dadd(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ D3 dsub(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ D3 dscale(D3 a, double s) { return d3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ D3 dcross(D3 a, D3 b) {
+    return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ D3 dqrot_unit(DQ q, D3 v) {
+    D3 qv = d3(q.x, q.y, q.z);
+    D3 t = dscale(dcross(qv, v), 2.0);
+    return dadd(dadd(v, dscale(t, q.w)), dcross(qv, t));
+}
+__device__ __forceinline__ DQ dqmul(DQ a, DQ b) {
+    return DQ{a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
+              a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+}
+__device__ __forceinline__ DQ dqnormalize(DQ q) {
+    double n = sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    if (n <= 0.0) return DQ{0.0, 0.0, 0.0, 1.0};
+    return DQ{q.x / n, q.y / n, q.z / n, q.w / n};
+}
+__device__ __forceinline__ DQ dq_axis_angle(D3 a, double angle) {
+    double sn, cs;
+    sincos(0.5 * angle, &sn, &cs);
+    return DQ{a.x * sn, a.y * sn, a.z * sn, cs};
+}
+__device__ __forceinline__ DTF dtf_from(TF t) {
+    return DTF{d3(t.p.x, t.p.y, t.p.z), DQ{t.q.x, t.q.y, t.q.z, t.q.w}};
+}
+__device__ __forceinline__ TF tf_from(DTF t) {
+    return TF{v3((float)t.p.x, (float)t.p.y, (float)t.p.z), Q4{(float)t.q.x, (float)t.q.y, (float)t.q.z, (float)t.q.w}};
+}
+__device__ __forceinline__ DTF load_dtf(const double* a) { return DTF{d3(a[0], a[1], a[2]), DQ{a[3], a[4], a[5], a[6]}}; }
+__device__ __forceinline__ void store_dtf(double* a, DTF t) {
+    a[0] = t.p.x; a[1] = t.p.y; a[2] = t.p.z;
+    a[3] = t.q.x; a[4] = t.q.y; a[5] = t.q.z; a[6] = t.q.w;
+}
+__device__ __forceinline__ DTF dtf_compose_n(DTF a, DTF b) {
+    DQ aq = dqnormalize(a.q);
+    DQ bq = dqnormalize(b.q);
+    return DTF{dadd(a.p, dqrot_unit(aq, b.p)), dqnormalize(dqmul(aq, bq))};
+}
+__device__ __forceinline__ DTF dtf_inverse_n(DTF x) {
+    DQ n = dqnormalize(x.q);
+    DQ qi = DQ{-n.x, -n.y, -n.z, n.w};
+    D3 p = dqrot_unit(qi, x.p);
+    return DTF{d3(-p.x, -p.y, -p.z), qi};
+}
+// x.p + rotate(normalize(x.q), p)  (_transform_point_d, sap_helpers.py:240-266)
+__device__ __forceinline__ D3 dtf_point(DTF x, D3 p) { return dadd(x.p, dqrot_unit(dqnormalize(x.q), p)); }
+
 }  // namespace sapx

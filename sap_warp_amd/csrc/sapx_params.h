@@ -53,7 +53,7 @@ struct StepParams {
     int weight_mode, point_mode, integ_mode;
     int npad, js;
     // LDS byte offsets
-    int o_A, o_H, o_J, o_bq, o_S, o_ck, o_cv, o_cG, o_cb, o_dv64, o_dv32, o_dflag, o_red, o_q, o_misc;
+    int o_A, o_H, o_J, o_bq, o_S, o_ck, o_cv, o_cG, o_cb, o_dv64, o_dv32, o_dflag, o_red, o_q, o_misc, o_bqd;
     int smem_bytes;
 };
 

@@ -28,6 +28,7 @@ constexpr double kLimitBeta = 0.1;        // contact_solve.py:49
 constexpr double kLimitWindow = 2.0;      // contact_solve.py:51
 constexpr double kExactAlphaMax = 1.5;    // contact_solve.py:52
 constexpr double kExactFTol = 1.0e-8;     // contact_solve.py:54
+constexpr int kNewtonRefine = 1;          // float64-residual refinement steps per Newton direction
 
 
 // ---------------------------------------------------------------------------
@@ -107,16 +108,17 @@ struct Smem {
     double* cv;     // C x 9  vc, dvc, gamma
     float* cG;      // C x 9
     int* cb;        // C x 2 local bodies
-    double* d64;    // 10 x npad: v, v0, vstar, dv, grad, jimp, dp, av, tmp, tmp2
-    float* d32;     // 12 x npad
+    double* d64;    // V_NUM x npad f64 vectors
+    float* d32;     // F_NUM x npad f32 vectors
     int* dflag;     // 4 x npad
     double* red;    // 8 x 8
     float* q;       // 2 x NQ coords (in, out)
-    float* misc;
+    float* mp;      // per-world model parameters (prefetched)
+    double* bqd;    // NB x 8  body poses in float64 (free-motion FK)
 };
 
-enum { V_V = 0, V_V0, V_VS, V_DV, V_G, V_JI, V_DP, V_AV, V_T1, V_T2, V_NUM };
-enum { F_HD = 0, F_AINV, F_DSC, F_PDA, F_PDG, F_PDL, F_LOV, F_UPV, F_LOR, F_UPR, F_TAU, F_JF, F_NUM };
+enum { V_V = 0, V_V0, V_VS, V_DV, V_G, V_JI, V_DP, V_AV, V_T1, V_T2, V_AVS, V_AR, V_NUM };
+enum { F_HD = 0, F_AINV, F_DSC, F_PDA, F_PDG, F_PDL, F_LOV, F_UPV, F_LOR, F_UPR, F_TAU, F_JF, F_DINV, F_NUM };
 enum { I_PART = 0, I_PD, I_LO, I_UP, I_NUM };
 
 __device__ __forceinline__ Smem smem_view(char* base, const StepParams& P) {
@@ -135,53 +137,69 @@ __device__ __forceinline__ Smem smem_view(char* base, const StepParams& P) {
     s.dflag = reinterpret_cast<int*>(base + P.o_dflag);
     s.red = reinterpret_cast<double*>(base + P.o_red);
     s.q = reinterpret_cast<float*>(base + P.o_q);
-    s.misc = reinterpret_cast<float*>(base + P.o_misc);
+    s.mp = reinterpret_cast<float*>(base + P.o_misc);
+    s.bqd = reinterpret_cast<double*>(base + P.o_bqd);
     return s;
 }
 
 // ---------------------------------------------------------------------------
+// wave-level broadcast of lane l's value (v_readlane: VALU -> SGPR, no LDS trip)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float rl(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// ---------------------------------------------------------------------------
 // Blocked Cholesky of H (npad x npad, row stride hs), in place, lower.
-// Diagonal 16x16 blocks: wave-synchronous right-looking factor in registers;
-// panel TRSM: one lane per row; trailing Schur updates: MFMA f32 16x16x4.
-// Rows/cols >= n hold the identity (identity padding, blocked_cholesky.py:725-901).
+// Diagonal 16x16 blocks: wave-synchronous right-looking factor held in
+// registers (one row per lane, readlane broadcasts), stopping at row n;
+// panel TRSM: one lane per row using the stored inverse diagonal; trailing
+// Schur updates: MFMA f32 16x16x4.  dinv[i] = 1 / L_ii.  Rows/cols >= n hold
+// the identity (identity padding as in blocked_cholesky.py:725-901).
 // ---------------------------------------------------------------------------
 template <int NT>
-__device__ void cholesky_blocked(float* H, int npad, int hs) {
+__device__ void cholesky_blocked(float* H, int npad, int hs, int n, float* dinv) {
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int nb = npad >> 4;
     for (int kb = 0; kb < nb; ++kb) {
         const int k0 = kb * 16;
+        const int jmax = min(16, n - k0);
         // 1. diagonal block factor (wave 0; lanes 0..15 own one row each)
         if (wave == 0) {
             float row[16];
             const int r = lane & 15;
 #pragma unroll
             for (int c = 0; c < 16; ++c) row[c] = H[(k0 + r) * hs + k0 + c];
+            float my_dinv = 1.0f;
 #pragma unroll
             for (int j = 0; j < 16; ++j) {
-                float djj = __shfl(row[j], j, 64);
-                djj = sqrtf(fmaxf(djj, 1.0e-30f));
-                float lij = row[j] / djj;
-                if (r == j) lij = djj;
-                if (r < j) lij = row[j];
-                row[j] = lij;
-                // rank-1 update of the trailing columns of rows below j
+                if (j < jmax) {
+                    const float djj = fmaxf(rl(row[j], j), 1.0e-30f);
+                    const float di = __builtin_amdgcn_rsqf(djj);
+                    const float d = djj * di;
+                    float lij = row[j];
+                    if (r == j) { lij = d; my_dinv = di; }
+                    else if (r > j) lij = row[j] * di;
+                    row[j] = lij;
 #pragma unroll
-                for (int c = j + 1; c < 16; ++c) {
-                    float lcj = __shfl(row[j], c, 64);  // L[c][j]
-                    if (r > j && c <= r) row[c] -= lij * lcj;
+                    for (int c = j + 1; c < 16; ++c) {
+                        const float lcj = rl(row[j], c);  // L[c][j]
+                        if (r > j && c <= r) row[c] -= lij * lcj;
+                    }
                 }
             }
             if (lane < 16) {
 #pragma unroll
                 for (int c = 0; c < 16; ++c) H[(k0 + r) * hs + k0 + c] = (c <= r) ? row[c] : 0.0f;
+                dinv[k0 + r] = my_dinv;
             }
         }
         __syncthreads();
-        // 2. panel TRSM: rows below the diagonal block
-        const int rows_below = npad - k0 - 16;
+        if (kb == nb - 1) break;
+        // 2. panel TRSM: rows below the diagonal block that hold real dofs
+        const int rows_below = min(npad, n) - k0 - 16;
         for (int t = threadIdx.x; t < rows_below; t += NT) {
             const int i = k0 + 16 + t;
             float x[16];
@@ -189,10 +207,10 @@ __device__ void cholesky_blocked(float* H, int npad, int hs) {
             for (int c = 0; c < 16; ++c) x[c] = H[i * hs + k0 + c];
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
-                float s = x[c];
+                float sacc = x[c];
 #pragma unroll
-                for (int k = 0; k < c; ++k) s -= x[k] * H[(k0 + c) * hs + k0 + k];
-                x[c] = s / H[(k0 + c) * hs + k0 + c];
+                for (int k = 0; k < c; ++k) sacc -= x[k] * H[(k0 + c) * hs + k0 + k];
+                x[c] = sacc * dinv[k0 + c];
             }
 #pragma unroll
             for (int c = 0; c < 16; ++c) H[i * hs + k0 + c] = x[c];
@@ -212,9 +230,9 @@ __device__ void cholesky_blocked(float* H, int npad, int hs) {
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) {
                 const int kcol = k0 + kk * 4 + (lane >> 4);
-                float a = H[(i0 + (lane & 15)) * hs + kcol];
-                float b = H[(j0 + (lane & 15)) * hs + kcol];
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+                float av = H[(i0 + (lane & 15)) * hs + kcol];
+                float bv = H[(j0 + (lane & 15)) * hs + kcol];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -227,28 +245,30 @@ __device__ void cholesky_blocked(float* H, int npad, int hs) {
     }
 }
 
-// Solve L L^T x = b for one right-hand side held in LDS (double), wave 0 only.
-// x overwrites b.  Rows >= n are identity.
-__device__ void chol_solve_wave0(const float* H, int hs, int n, double* b) {
+// Solve L L^T x = b (float32 factor), wave 0 only; b/x in LDS as double.
+// Serial substitution with readlane broadcasts; rows >= n are identity.
+__device__ void chol_solve_wave0(const float* H, int hs, int n, const float* dinv, double* b) {
     const int lane = threadIdx.x & 63;
     if ((threadIdx.x >> 6) != 0) return;
-    double y = (lane < n) ? b[lane] : 0.0;
+    float y = (lane < n) ? (float)b[lane] : 0.0f;
+    const float di = (lane < n) ? dinv[lane] : 1.0f;
     // forward: L y = b
+#pragma unroll 4
     for (int j = 0; j < n; ++j) {
-        double yj = __shfl(y, j, 64);
-        double ljj = (double)H[j * hs + j];
-        yj = yj / ljj;
+        const float lij = (lane > j && lane < n) ? H[lane * hs + j] : 0.0f;
+        const float yj = rl(y, j) * rl(di, j);
         if (lane == j) y = yj;
-        if (lane > j && lane < n) y -= (double)H[lane * hs + j] * yj;
+        y -= lij * yj;
     }
     // backward: L^T x = y
+#pragma unroll 4
     for (int j = n - 1; j >= 0; --j) {
-        double xj = __shfl(y, j, 64);
-        xj = xj / (double)H[j * hs + j];
+        const float lji = (lane < j) ? H[j * hs + lane] : 0.0f;
+        const float xj = rl(y, j) * rl(di, j);
         if (lane == j) y = xj;
-        if (lane < j) y -= (double)H[j * hs + lane] * xj;
+        y -= lji * xj;
     }
-    if (lane < n) b[lane] = y;
+    if (lane < n) b[lane] = (double)y;
 }
 
 // ---------------------------------------------------------------------------
@@ -311,6 +331,20 @@ __device__ __forceinline__ double project_curvature(const double* k, const doubl
     double Gd1 = (double)G[3] * dvc[0] + (double)G[4] * dvc[1] + (double)G[5] * dvc[2];
     double Gd2 = (double)G[6] * dvc[0] + (double)G[7] * dvc[1] + (double)G[8] * dvc[2];
     return dvc[0] * Gd0 + dvc[1] * Gd1 + dvc[2] * Gd2;
+}
+
+// sum_d a[d] * x[d] for float32 row a and float64 vector x (4 independent accumulators)
+__device__ __forceinline__ double dot_fd(const float* a, const double* x, int n) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int d = 0;
+    for (; d + 4 <= n; d += 4) {
+        s0 += (double)a[d] * x[d];
+        s1 += (double)a[d + 1] * x[d + 1];
+        s2 += (double)a[d + 2] * x[d + 2];
+        s3 += (double)a[d + 3] * x[d + 3];
+    }
+    for (; d < n; ++d) s0 += (double)a[d] * x[d];
+    return (s0 + s1) + (s2 + s3);
 }
 
 // PD / limit helpers (sap_helpers.py:2251-2340)
@@ -395,7 +429,24 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
 
     // ------------------------------------------------------------------ 0/1
     // loads + public -> SAP boundary (sap_helpers.py:1482-1522, 1811-1848, 2058-2072)
+    // per-world model parameters -> LDS (one coalesced pass; reused by every phase)
+    float* mXp = s.mp;                 // nj x 8
+    float* mXc = mXp + nj * 8;         // nj x 8
+    float* mAx = mXc + nj * 8;         // n x 4
+    float* mCom = mAx + n * 4;         // nb x 4
+    float* mI = mCom + nb * 4;         // nb x 9
+    float* mM = mI + nb * 9;           // nb
+    for (int t = tid; t < nj * 7; t += NT) {
+        const int j = t / 7, k = t % 7;
+        mXp[j * 8 + k] = M.joint_X_p[(size_t)w * nj * 7 + t];
+        mXc[j * 8 + k] = M.joint_X_c[(size_t)w * nj * 7 + t];
+    }
+    for (int t = tid; t < n * 3; t += NT) mAx[(t / 3) * 4 + t % 3] = M.joint_axis[(size_t)w * n * 3 + t];
+    for (int t = tid; t < nb * 3; t += NT) mCom[(t / 3) * 4 + t % 3] = M.body_com[(size_t)w * nb * 3 + t];
+    for (int t = tid; t < nb * 9; t += NT) mI[t] = M.body_inertia[(size_t)w * nb * 9 + t];
+    for (int t = tid; t < nb; t += NT) mM[t] = M.body_mass[(size_t)w * nb + t];
     for (int k = tid; k < M.NQ; k += NT) q_in[k] = P.jq_in[(size_t)w * M.NQ + k];
+    __syncthreads();
     for (int d = tid; d < n; d += NT) {
         const size_t g = (size_t)w * n + d;
         kQdPub[d] = P.jqd_in[g];
@@ -411,7 +462,7 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
             V3 fc = v3(f[0], f[1], f[2]);
             V3 tc = v3(f[3], f[4], f[5]);
             TF x = load_tf(P.bq_in + gb * 7);
-            const float* com = M.body_com + gb * 3;
+            const float* com = mCom + b * 4;
             V3 c = qrot(qnormalize(x.q), v3(com[0], com[1], com[2]));
             V3 to = tc + cross(c, fc);
             e[0] = -to.x; e[1] = -to.y; e[2] = -to.z;
@@ -426,7 +477,7 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
             const int child = M.t_jchild[j];
             const size_t gb = (size_t)w * nb + child;
             TF x = load_tf(P.bq_in + gb * 7);
-            const float* com = M.body_com + gb * 3;
+            const float* com = mCom + child * 4;
             V3 r = qrot(qnormalize(x.q), v3(com[0], com[1], com[2]));
             const size_t g = (size_t)w * n + ds;
             double vc0 = P.jqd_in[g + 0], vc1 = P.jqd_in[g + 1], vc2 = P.jqd_in[g + 2];
@@ -466,19 +517,22 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
             const int child = M.t_jchild[j];
             const int qs = M.t_jqs[j];
             const int ds = M.t_jds[j];
-            TF Xp = load_tf(M.joint_X_p + gj * 7);
-            TF Xc = load_tf(M.joint_X_c + gj * 7);
-            const bool xcid = M.j_xc_identity[gj] != 0;
-            TF Xwpj = Xp;
-            if (parent >= 0) Xwpj = tf_compose_n(load_tf(s.bq + parent * 8), Xp);
+            TF Xp = load_tf(mXp + j * 8);
+            TF Xc = load_tf(mXc + j * 8);
+            const bool xcid = tf_is_identity(mXc + j * 8);
+            // poses in float64 (free_motion.py:412-530 evaluates them with transformd)
+            const DTF Xp_d = dtf_from(Xp);
+            const DTF Xwpj_d = parent >= 0 ? dtf_compose_n(load_dtf(s.bqd + parent * 8), Xp_d) : Xp_d;
+            const TF Xwpj = tf_from(Xwpj_d);
+            DTF Xwc_d;
             TF Xwc;
             SV vPB = sv(v3(0, 0, 0), v3(0, 0, 0));
             SV aPB = vPB;
-            const float* axp = M.joint_axis + ((size_t)w * n + ds) * 3;
+            const float* axp = mAx + ds * 4;
             if (jt == kJointRevolute && xcid) {
                 V3 axis = v3(axp[0], axp[1], axp[2]);
-                Q4 qfm = q_axis_angle(axis, q_in[qs]);
-                Xwc = TF{Xwpj.p, qmul(Xwpj.q, qfm)};
+                const DQ qfm = dq_axis_angle(d3(axis.x, axis.y, axis.z), (double)q_in[qs]);
+                Xwc_d = DTF{Xwpj_d.p, dqmul(Xwpj_d.q, qfm)};
                 SV S = sv(qrot(Xwpj.q, axis), v3(0, 0, 0));
                 store_sv(s.S + ds * 6, S);
                 vPB = sv_scale(S, (float)d64[V_V0 * npad + ds]);
@@ -493,8 +547,14 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
                     Xj = TF{v3(q_in[qs], q_in[qs + 1], q_in[qs + 2]),
                             Q4{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]}};
                 }
-                TF Xwm = tf_compose_n(Xwpj, Xj);
-                Xwc = xcid ? Xwm : tf_compose_n(Xwm, tf_inverse_n(Xc));
+                DTF Xj_d = dtf_from(Xj);
+                if (jt == kJointRevolute) Xj_d.q = dq_axis_angle(d3(axis.x, axis.y, axis.z), (double)q_in[qs]);
+                if (jt == kJointPrismatic) {
+                    const double qv = q_in[qs];
+                    Xj_d.p = d3(axis.x * qv, axis.y * qv, axis.z * qv);
+                }
+                const DTF Xwm_d = dtf_compose_n(Xwpj_d, Xj_d);
+                Xwc_d = xcid ? Xwm_d : dtf_compose_n(Xwm_d, dtf_inverse_n(dtf_from(Xc)));
                 TF Xmb = tf_inverse_n(Xc);
                 V3 pmb = qrot(Xj.q, Xmb.p);
                 Q4 R = Xwpj.q;
@@ -523,6 +583,8 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
                     aPB = mob_bias_to_body(R, pmb, Vf);
                 }
             }
+            store_dtf(s.bqd + child * 8, Xwc_d);
+            Xwc = tf_from(Xwc_d);
             float* bqc = s.bq + child * 8;
             bqc[0] = Xwc.p.x; bqc[1] = Xwc.p.y; bqc[2] = Xwc.p.z;
             bqc[3] = Xwc.q.x; bqc[4] = Xwc.q.y; bqc[5] = Xwc.q.z; bqc[6] = Xwc.q.w;
@@ -545,9 +607,9 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     const V3 O = v3(s.bq[0], s.bq[1], s.bq[2]);
     for (int b = tid; b < nb; b += NT) {
         const size_t gb = (size_t)w * nb + b;
-        const float m = M.body_mass[gb];
-        const float* Ip = M.body_inertia + gb * 9;
-        const float* cp = M.body_com + gb * 3;
+        const float m = mM[b];
+        const float* Ip = mI + b * 9;
+        const float* cp = mCom + b * 4;
         const float* g = M.gravity + (size_t)w * 3;
         const float* bqb = s.bq + b * 8;
         Q4 q = Q4{bqb[3], bqb[4], bqb[5], bqb[6]};
@@ -670,10 +732,10 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
 
     // ------------------------------------------------------------------ 5
     // free motion v* = v0 + dt A^-1 tau with one refinement step (free_motion.py:1970-2178)
-    cholesky_blocked<NT>(s.H, npad, hs);
+    cholesky_blocked<NT>(s.H, npad, hs, n, d32 + F_DINV * npad);
     for (int d = tid; d < npad; d += NT) d64[V_T1 * npad + d] = (d < n) ? (double)d32[F_TAU * npad + d] : 0.0;
     __syncthreads();
-    chol_solve_wave0(s.H, hs, n, d64 + V_T1 * npad);
+    chol_solve_wave0(s.H, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
     __syncthreads();
     for (int i = tid; i < n; i += NT) {
         double r = (double)d32[F_TAU * npad + i];
@@ -681,7 +743,7 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         d64[V_T2 * npad + i] = r;
     }
     __syncthreads();
-    chol_solve_wave0(s.H, hs, n, d64 + V_T2 * npad);
+    chol_solve_wave0(s.H, hs, n, d32 + F_DINV * npad, d64 + V_T2 * npad);
     __syncthreads();
     for (int d = tid; d < npad; d += NT) {
         double qdd = (d < n) ? d64[V_T1 * npad + d] + d64[V_T2 * npad + d] : 0.0;
@@ -714,19 +776,17 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         const int lb1 = b1g >= 0 ? b1g - w * nb : -1;
         s.cb[c * 2] = lb0;
         s.cb[c * 2 + 1] = lb1;
-        V3 x0 = v3(P.cpoint0[idx * 3], P.cpoint0[idx * 3 + 1], P.cpoint0[idx * 3 + 2]);
-        V3 x1 = v3(P.cpoint1[idx * 3], P.cpoint1[idx * 3 + 1], P.cpoint1[idx * 3 + 2]);
-        if (lb0 >= 0) {
-            const float* bb = s.bq + lb0 * 8;
-            x0 = v3(bb[0], bb[1], bb[2]) + qrot(qnormalize(Q4{bb[3], bb[4], bb[5], bb[6]}), x0);
-        }
-        if (lb1 >= 0) {
-            const float* bb = s.bq + lb1 * 8;
-            x1 = v3(bb[0], bb[1], bb[2]) + qrot(qnormalize(Q4{bb[3], bb[4], bb[5], bb[6]}), x1);
-        }
-        V3 nrm = v3(P.cnormal[idx * 3], P.cnormal[idx * 3 + 1], P.cnormal[idx * 3 + 2]);
-        float n2 = dot(nrm, nrm);
-        nrm = (n2 <= 1.0e-32f) ? v3(0, 0, 0) : nrm * (1.0f / sqrtf(n2));
+        // world witness points and the signed distance in float64 from the float64 poses
+        D3 x0d = d3(P.cpoint0[idx * 3], P.cpoint0[idx * 3 + 1], P.cpoint0[idx * 3 + 2]);
+        D3 x1d = d3(P.cpoint1[idx * 3], P.cpoint1[idx * 3 + 1], P.cpoint1[idx * 3 + 2]);
+        if (lb0 >= 0) x0d = dtf_point(load_dtf(s.bqd + lb0 * 8), x0d);
+        if (lb1 >= 0) x1d = dtf_point(load_dtf(s.bqd + lb1 * 8), x1d);
+        D3 nd = d3(P.cnormal[idx * 3], P.cnormal[idx * 3 + 1], P.cnormal[idx * 3 + 2]);
+        const double nd2 = nd.x * nd.x + nd.y * nd.y + nd.z * nd.z;
+        nd = (nd2 <= 1.0e-32) ? d3(0.0, 0.0, 0.0) : dscale(nd, 1.0 / sqrt(nd2));
+        const V3 x0 = v3((float)x0d.x, (float)x0d.y, (float)x0d.z);
+        const V3 x1 = v3((float)x1d.x, (float)x1d.y, (float)x1d.z);
+        V3 nrm = v3((float)nd.x, (float)nd.y, (float)nd.z);
         // frame: MakeFromOneUnitVector (sap_helpers.py:467-503)
         float ax = fabsf(nrm.x), ay = fabsf(nrm.y), az = fabsf(nrm.z);
         int im = 0;
@@ -752,7 +812,8 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         double kden = k0 + k1;
         double w0 = 0.5, w1 = 0.5;
         if (kden != 0.0) { w0 = k0 / kden; w1 = k1 / kden; }
-        V3 pc = x0 * (float)w0 + x1 * (float)w1;
+        const D3 pcd = dadd(dscale(x0d, w0), dscale(x1d, w1));
+        V3 pc = v3((float)pcd.x, (float)pcd.y, (float)pcd.z);
         double kc;
         if (!isfinite(k0)) kc = k1;
         else if (!isfinite(k1)) kc = k0;
@@ -772,8 +833,8 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         if (sh1 >= 0) t1 = M.shape_tau[sh1];
         if (!isfinite(t1) || t1 < 0.0) t1 = P.fallback_tau;
         if (!isfinite(t1) || t1 < 0.0) t1 = 0.0;
-        double phi = ((double)x1.x - x0.x) * nrm.x + ((double)x1.y - x0.y) * nrm.y + ((double)x1.z - x0.z) * nrm.z -
-                     (double)P.cmargin0[idx] - (double)P.cmargin1[idx];
+        const D3 gapv = dsub(x1d, x0d);
+        double phi = gapv.x * nd.x + gapv.y * nd.y + gapv.z * nd.z - (double)P.cmargin0[idx] - (double)P.cmargin1[idx];
         cx0[c * 3] = x0.x; cx0[c * 3 + 1] = x0.y; cx0[c * 3 + 2] = x0.z;
         cx1[c * 3] = x1.x; cx1[c * 3 + 1] = x1.y; cx1[c * 3 + 2] = x1.z;
         cpc[c * 3] = pc.x; cpc[c * 3 + 1] = pc.y; cpc[c * 3 + 2] = pc.z;
@@ -837,15 +898,15 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
                 const int lb = s.cb[c * 2 + side];
                 if (lb < 0) continue;
                 const size_t gb = (size_t)w * nb + lb;
-                const double m = M.body_mass[gb];
+                const double m = mM[lb];
                 if (m <= 1.0e-12) continue;
                 const float* bb = s.bq + lb * 8;
                 Q4 q = qnormalize(Q4{bb[3], bb[4], bb[5], bb[6]});
-                const float* com = M.body_com + gb * 3;
+                const float* com = mCom + lb * 4;
                 V3 comw = v3(bb[0], bb[1], bb[2]) + qrot(q, v3(com[0], com[1], com[2]));
                 const float* xs = side == 0 ? cx0 + c * 3 : cx1 + c * 3;
                 V3 r = v3(xs[0], xs[1], xs[2]) - comw;
-                const float* I = M.body_inertia + gb * 9;
+                const float* I = mI + lb * 9;
                 double ix = fmax((double)I[0], 1.0e-12), iy = fmax((double)I[4], 1.0e-12), iz = fmax((double)I[8], 1.0e-12);
                 for (int k = 0; k < 3; ++k) {
                     V3 dn = dirs[k];
@@ -1004,6 +1065,7 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         }
         v[i] = vi;
         dvv[i] = 0.0;
+        d64[V_AVS * npad + i] = (i < n) ? dot_fd(s.A + i * npad, vs, n) : 0.0;
     }
     __syncthreads();
     int iteration = 0, ls_total = 0, flags = 0, ls_status = 0;
@@ -1013,44 +1075,48 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         double alpha = 1.0, prev_cost = 0.0;
         const double cost_min_alpha = 0.5;
         while (true) {
-            // ---- evaluate at v: vc = J v, projection, model terms, gradient
-            for (int r = tid; r < 3 * nc; r += NT) {
-                double acc = 0.0;
-                const float* Jr = s.J + r * js;
-                for (int d = 0; d < n; ++d) acc += (double)Jr[d] * v[d];
-                s.cv[(r / 3) * 9 + (r % 3)] = acc;
+            // ---- evaluate at v: vc = J v and A v (one stage), projection + model terms,
+            //      then impulses / gradient / norms (contact_solve.py:743-921, 1493-1627)
+            for (int t = tid; t < 3 * nc + n; t += NT) {
+                if (t < 3 * nc) {
+                    s.cv[(t / 3) * 9 + (t % 3)] = dot_fd(s.J + t * js, v, n);
+                } else {
+                    const int i = t - 3 * nc;
+                    const double avi = dot_fd(s.A + i * npad, v, n);
+                    av[i] = avi;
+                    d64[V_AR * npad + i] = avi - d64[V_AVS * npad + i];
+                }
             }
             __syncthreads();
-            double part3[4] = {0.0, 0.0, 0.0, 0.0};  // contact cost + model cost
-            for (int c = tid; c < nc; c += NT) {
-                double vc[3] = {s.cv[c * 9], s.cv[c * 9 + 1], s.cv[c * 9 + 2]};
-                double g[3];
-                part3[0] += project_contact(s.ck + c * 4, vc, g, s.cG + c * 9, true);
-                s.cv[c * 9 + 6] = g[0]; s.cv[c * 9 + 7] = g[1]; s.cv[c * 9 + 8] = g[2];
-            }
-            for (int i = tid; i < n; i += NT) {
-                double imp, hd;
-                part3[0] += model_terms(s, npad, i, v[i], dt, &imp, &hd);
-                d32[F_HD * npad + i] = (float)hd;
-                dp[i] = imp;  // temporarily holds the model impulse
+            double part3[4] = {0.0, 0.0, 0.0, 0.0};  // cost, |D g|^2, |D A v|^2, |D j|^2
+            for (int t = tid; t < nc + n; t += NT) {
+                if (t < nc) {
+                    const int c = t;
+                    double vc[3] = {s.cv[c * 9], s.cv[c * 9 + 1], s.cv[c * 9 + 2]};
+                    double g[3];
+                    part3[0] += project_contact(s.ck + c * 4, vc, g, s.cG + c * 9, true);
+                    s.cv[c * 9 + 6] = g[0]; s.cv[c * 9 + 7] = g[1]; s.cv[c * 9 + 8] = g[2];
+                } else {
+                    const int i = t - nc;
+                    double imp, hd;
+                    part3[0] += model_terms(s, npad, i, v[i], dt, &imp, &hd);
+                    d32[F_HD * npad + i] = (float)hd;
+                    dp[i] = imp;  // temporarily holds the model impulse
+                }
             }
             __syncthreads();
             for (int i = tid; i < n; i += NT) {
-                double jc = dp[i];
+                double j0a = 0.0, j1a = 0.0, j2a = 0.0;
                 for (int c = 0; c < nc; ++c) {
-                    jc += (double)s.J[(c * 3) * js + i] * s.cv[c * 9 + 6] + (double)s.J[(c * 3 + 1) * js + i] * s.cv[c * 9 + 7] +
-                          (double)s.J[(c * 3 + 2) * js + i] * s.cv[c * 9 + 8];
+                    j0a += (double)s.J[(c * 3) * js + i] * s.cv[c * 9 + 6];
+                    j1a += (double)s.J[(c * 3 + 1) * js + i] * s.cv[c * 9 + 7];
+                    j2a += (double)s.J[(c * 3 + 2) * js + i] * s.cv[c * 9 + 8];
                 }
-                double ar = 0.0, avv = 0.0;
-                const float* Ai = s.A + i * npad;
-                for (int j = 0; j < n; ++j) {
-                    const double a = Ai[j];
-                    ar += a * (v[j] - vs[j]);
-                    avv += a * v[j];
-                }
+                const double jc = dp[i] + ((j0a + j1a) + j2a);
+                const double ar = d64[V_AR * npad + i];
+                const double avv = av[i];
                 jimp[i] = jc;
                 grad[i] = ar - jc;
-                av[i] = avv;
                 part3[0] += 0.5 * (v[i] - vs[i]) * ar;
                 if (s.dflag[I_PART * npad + i]) {
                     const double sc = d32[F_DSC * npad + i];
@@ -1126,31 +1192,62 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
                 }
             }
             __syncthreads();
-            cholesky_blocked<NT>(s.H, npad, hs);
+            cholesky_blocked<NT>(s.H, npad, hs, n, d32 + F_DINV * npad);
             for (int i = tid; i < npad; i += NT) dvv[i] = (i < n) ? -grad[i] : 0.0;
             __syncthreads();
-            chol_solve_wave0(s.H, hs, n, dvv);
+            chol_solve_wave0(s.H, hs, n, d32 + F_DINV * npad, dvv);
             __syncthreads();
-            // ---- line search data (contact_solve.py:2234-2263)
-            for (int r = tid; r < 3 * nc; r += NT) {
-                double acc = 0.0;
-                const float* Jr = s.J + r * js;
-                for (int d = 0; d < n; ++d) acc += (double)Jr[d] * dvv[d];
-                s.cv[(r / 3) * 9 + 3 + (r % 3)] = acc;
+            // ---- mixed-precision refinement of the Newton direction: residual
+            //      r = -grad - (A + J^T G J + diag(hd + shift)) dv in float64 from the stored
+            //      float32 operands, correction solved with the float32 factor
+            for (int rep = 0; rep < kNewtonRefine; ++rep) {
+                for (int t = tid; t < 3 * nc + n; t += NT) {
+                    if (t < 3 * nc) {
+                        s.cv[(t / 3) * 9 + 3 + (t % 3)] = dot_fd(s.J + t * js, dvv, n);
+                    } else {
+                        const int i = t - 3 * nc;
+                        dp[i] = dot_fd(s.A + i * npad, dvv, n);
+                    }
+                }
+                __syncthreads();
+                for (int i = tid; i < npad; i += NT) {
+                    double r = 0.0;
+                    if (i < n) {
+                        double acc = dp[i] + ((double)d32[F_HD * npad + i] + P.diag_shift) * dvv[i];
+                        for (int c = 0; c < nc; ++c) {
+                            const float* G = s.cG + c * 9;
+                            const double t0 = s.cv[c * 9 + 3], t1 = s.cv[c * 9 + 4], t2 = s.cv[c * 9 + 5];
+                            const double u0 = (double)G[0] * t0 + (double)G[1] * t1 + (double)G[2] * t2;
+                            const double u1 = (double)G[3] * t0 + (double)G[4] * t1 + (double)G[5] * t2;
+                            const double u2 = (double)G[6] * t0 + (double)G[7] * t1 + (double)G[8] * t2;
+                            acc += (double)s.J[(c * 3) * js + i] * u0 + (double)s.J[(c * 3 + 1) * js + i] * u1 +
+                                   (double)s.J[(c * 3 + 2) * js + i] * u2;
+                        }
+                        r = -grad[i] - acc;
+                    }
+                    d64[V_T1 * npad + i] = r;
+                }
+                __syncthreads();
+                chol_solve_wave0(s.H, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
+                __syncthreads();
+                for (int i = tid; i < n; i += NT) dvv[i] += d64[V_T1 * npad + i];
+                __syncthreads();
             }
+            // ---- line search data: dvc = J dv and dp = A dv in one stage (contact_solve.py:2234-2263)
             double ld[4] = {0, 0, 0, 0};  // dell0, dell_a0, d2ell_a, m0
-            for (int i = tid; i < n; i += NT) {
-                double acc = 0.0;
-                const float* Ai = s.A + i * npad;
-                for (int j = 0; j < n; ++j) acc += (double)Ai[j] * dvv[j];
-                dp[i] = acc;
-                const double ri = v[i] - vs[i];
-                ld[0] += grad[i] * dvv[i];
-                ld[1] += acc * ri;
-                ld[2] += dvv[i] * acc;
-                double ar = 0.0;
-                for (int j = 0; j < n; ++j) ar += (double)Ai[j] * (v[j] - vs[j]);
-                ld[3] += 0.5 * ri * ar;
+            for (int t = tid; t < 3 * nc + n; t += NT) {
+                if (t < 3 * nc) {
+                    s.cv[(t / 3) * 9 + 3 + (t % 3)] = dot_fd(s.J + t * js, dvv, n);
+                } else {
+                    const int i = t - 3 * nc;
+                    const double acc = dot_fd(s.A + i * npad, dvv, n);
+                    dp[i] = acc;
+                    const double ri = v[i] - vs[i];
+                    ld[0] += grad[i] * dvv[i];
+                    ld[1] += acc * ri;
+                    ld[2] += dvv[i] * acc;
+                    ld[3] += 0.5 * ri * d64[V_AR * npad + i];
+                }
             }
             block_sum<NT>(ld, 4, s.red);
             const double dell0 = ld[0], dell_a0 = ld[1], d2ell_a = ld[2], m0 = ld[3];
@@ -1349,7 +1446,7 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
             q_out[qs + 2] = q_in[qs + 2] + h * vmid.z;
             q_out[qs + 3] = qn.x; q_out[qs + 4] = qn.y; q_out[qs + 5] = qn.z; q_out[qs + 6] = qn.w;
             const size_t gb = (size_t)w * nb + M.t_jchild[j];
-            const float* com = M.body_com + gb * 3;
+            const float* com = mCom + M.t_jchild[j] * 4;
             V3 rc = qrot(qnormalize(qn), v3(com[0], com[1], com[2]));
             V3 vcom = von + cross(wn, rc);
             qd_pub[ds] = vcom.x; qd_pub[ds + 1] = vcom.y; qd_pub[ds + 2] = vcom.z;
@@ -1377,14 +1474,14 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
             const int jt = M.t_jtype[j];
             const int parent = M.t_jparent[j], child = M.t_jchild[j];
             const int qs = M.t_jqs[j], ds = M.t_jds[j];
-            TF Xp = load_tf(M.joint_X_p + gj * 7);
-            TF Xc = load_tf(M.joint_X_c + gj * 7);
+            TF Xp = load_tf(mXp + j * 8);
+            TF Xc = load_tf(mXc + j * 8);
             TF Xwpj = Xp;
             V3 vlin = v3(0, 0, 0), vang = v3(0, 0, 0);
             if (parent >= 0) {
                 TF Xwp = load_tf(pbq + parent * 8);
                 Xwpj = tf_mul(Xwp, Xp);
-                const float* com = M.body_com + ((size_t)w * nb + parent) * 3;
+                const float* com = mCom + parent * 4;
                 V3 rp = Xwpj.p - (Xwp.p + qrot(Xwp.q, v3(com[0], com[1], com[2])));
                 const float* vp = pbqd + parent * 6;
                 vang = v3(vp[3], vp[4], vp[5]);
@@ -1392,7 +1489,7 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
             }
             TF Xj = TF{v3(0, 0, 0), Q4{0, 0, 0, 1}};
             V3 jl = v3(0, 0, 0), ja = v3(0, 0, 0);
-            const float* axp = M.joint_axis + ((size_t)w * n + ds) * 3;
+            const float* axp = mAx + ds * 4;
             V3 axis = v3(axp[0], axp[1], axp[2]);
             if (jt == kJointPrismatic) {
                 Xj.p = axis * q_out[qs];

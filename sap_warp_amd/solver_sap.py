@@ -257,6 +257,7 @@ class SolverSAP:
         opts.contact_point_mode = nat.POINT_MIDPOINT if self.contact_point_mode == "contact_midpoint" else nat.POINT_WITNESS
         opts.position_integration = (nat.INTEGRATE_SAP_EULER if self.position_integration == "sap_euler"
                                      else nat.INTEGRATE_MIDPOINT)
+        opts.capture_snapshots = int(self.capture_contact_jacobian_snapshots)
         self._opts = opts
         h = ctypes.c_void_p()
         nat.check(self._lib.sapx_solver_create(self._native_model.handle, ctypes.byref(opts), ctypes.byref(h)),

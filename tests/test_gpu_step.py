@@ -53,6 +53,7 @@ def _run_and_compare(scene, preset, ls, worlds, steps, seed=0):
 
     model, dt, C = _scene(scene, worlds, seed)
     solver = SolverSAP(model, max_rigid_contact=C, line_search_variant=ls, contact_preset_variant=preset,
+                       capture_contact_jacobian_snapshots=True,
                        collect_iteration_stats=True)
     pipe = SapCollisionPipeline(model, max_contacts_per_world=C)
     contacts = pipe.contacts()
