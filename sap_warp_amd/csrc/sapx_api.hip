@@ -75,8 +75,8 @@ struct SapxSolver {
     double* dbg = nullptr;
     size_t dbg_off[7] = {0};
     size_t dbg_size[7] = {0};
-    int npad = 0, js = 0;
-    int lds[16] = {0};
+    int npad = 0, js = 0, ast = 0;
+    int lds[17] = {0};
     int smem_bytes = 0;
     // grouping buffers for ungrouped contact lists
     int group_capacity = 0;
@@ -204,6 +204,15 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
             cur = jj >= 0 ? t_jparent[jj] : -1;
         }
     }
+    // child-body lists (CSR) for the deterministic reverse-level composite sweep
+    std::vector<int> child_start(NB + 1, 0), child_list;
+    for (int c = 0; c < NB; ++c) {
+        child_start[c] = (int)child_list.size();
+        for (int j = 0; j < J; ++j)
+            if (t_jparent[j] == c) child_list.push_back(t_jchild[j]);
+    }
+    child_start[NB] = (int)child_list.size();
+    if (child_list.empty()) child_list.push_back(0);
     // dof maps (contact_solve.py:3681-3720) and articulation dof ranges (3722-3760)
     std::vector<int> dof_joint(N), dof_coord(N, -1), dof_target(N), dof_limit(N, 0);
     for (int j = 0; j < J; ++j) {
@@ -310,6 +319,39 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     UP(t_body_anc, anc.data(), NB);
     UP(t_body_sub, sub.data(), NB);
     UP(t_body_art_start, art_start.data(), NB);
+    UP(t_child_start, child_start.data(), NB + 1);
+    UP(t_child, child_list.data(), child_list.size());
+    {
+        // packed topology template (one coalesced copy into LDS per world)
+        std::vector<int> topo;
+        auto add = [&](int field, const int* src, size_t cnt) {
+            dm.topo_off[field] = (int)topo.size();
+            topo.insert(topo.end(), src, src + cnt);
+        };
+        add(TP_JTYPE, t_jtype.data(), J);
+        add(TP_JPARENT, t_jparent.data(), J);
+        add(TP_JCHILD, t_jchild.data(), J);
+        add(TP_JQS, t_jqs.data(), J + 1);
+        add(TP_JDS, t_jds.data(), J + 1);
+        add(TP_LEVEL_START, level_start.data(), nlev + 1);
+        add(TP_LEVEL_JOINT, level_joint.data(), J);
+        add(TP_DOF_JOINT, dof_joint.data(), N);
+        add(TP_DOF_COORD, dof_coord.data(), N);
+        add(TP_DOF_TARGET, dof_target.data(), N);
+        add(TP_DOF_LIMIT_OK, dof_limit.data(), N);
+        add(TP_BODY_ART_START, art_start.data(), NB);
+        add(TP_BODY_ART_COUNT, art_count.data(), NB);
+        add(TP_CHILD_START, child_start.data(), NB + 1);
+        add(TP_CHILD, child_list.data(), child_list.size());
+        if (topo.size() % 2) topo.push_back(0);  // 8-byte alignment of the u64 masks
+        dm.topo_off[TP_BODY_ANC] = (int)topo.size();
+        for (int b = 0; b < NB; ++b) {
+            topo.push_back((int)(anc[b] & 0xffffffffull));
+            topo.push_back((int)(anc[b] >> 32));
+        }
+        dm.topo_ints = (int)topo.size();
+        UP(topo, topo.data(), topo.size());
+    }
     UP(t_body_art_count, art_count.data(), NB);
     UP(t_pairs, per_world[0].data(), per_world[0].size());
     UP(j_xc_identity, xcid.data(), d->joint_count);
@@ -371,7 +413,9 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
     s->opts = *opts;
     const int n = dm.N, nb = dm.NB;
     s->npad = (int)round_up(n, 16);
-    s->js = (int)round_up(n, 4);
+    // odd row strides keep row-per-thread LDS access free of bank conflicts
+    s->js = (n % 2) ? n : n + 1;
+    s->ast = s->npad + 1;
     const int npad = s->npad, hs = npad + 1, js = s->js;
     // LDS layout (see Smem in sapx_step.hip)
     size_t off = 0;
@@ -380,12 +424,12 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
         off = round_up(off + bytes, 16);
         return (int)o;
     };
-    s->lds[0] = take((size_t)n * npad * 4);                                              // A
+    s->lds[0] = take((size_t)n * s->ast * 4);                                            // A
     s->lds[1] = take(std::max((size_t)npad * hs * 4, (size_t)C * 104));                  // H / contact scratch
-    s->lds[2] = take(std::max((size_t)3 * C * js * 4, (size_t)(50 * nb + 13 * n) * 4));  // J / kinematics
+    s->lds[2] = take(std::max((size_t)3 * C * js * 4, (size_t)(50 * nb + 14 * n + 2) * 4 + (size_t)dm.J * 64));  // J / kinematics
     s->lds[3] = take((size_t)nb * 8 * 4);                                                // bq
     s->lds[4] = take((size_t)npad * 6 * 4);                                              // S
-    s->lds[5] = take((size_t)C * 4 * 8);                                                 // ck
+    s->lds[5] = take((size_t)C * 8 * 8);                                                 // ck (kCk = 8)
     s->lds[6] = take((size_t)C * 9 * 8);                                                 // cv
     s->lds[7] = take((size_t)C * 9 * 4);                                                 // cG
     s->lds[8] = take((size_t)C * 2 * 4);                                                 // cb
@@ -396,6 +440,7 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
     s->lds[13] = take((size_t)2 * dm.NQ * 4);                                            // q
     s->lds[14] = take((size_t)(dm.J * 16 + n * 4 + nb * 4 + nb * 9 + nb) * 4);             // per-world model params
     s->lds[15] = take((size_t)nb * 8 * 8);                                               // bqd (f64 poses)
+    s->lds[16] = take((size_t)dm.topo_ints * 4);                                         // topology template
     s->smem_bytes = (int)off;
     if (s->smem_bytes > 160 * 1024) {
         delete s;
@@ -569,10 +614,12 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
     P.integ_mode = o.position_integration;
     P.npad = s->npad;
     P.js = s->js;
+    P.ast = s->ast;
     P.o_A = s->lds[0]; P.o_H = s->lds[1]; P.o_J = s->lds[2]; P.o_bq = s->lds[3]; P.o_S = s->lds[4];
     P.o_ck = s->lds[5]; P.o_cv = s->lds[6]; P.o_cG = s->lds[7]; P.o_cb = s->lds[8]; P.o_dv64 = s->lds[9];
     P.o_dv32 = s->lds[10]; P.o_dflag = s->lds[11]; P.o_red = s->lds[12]; P.o_q = s->lds[13]; P.o_misc = s->lds[14];
     P.o_bqd = s->lds[15];
+    P.o_topo = s->lds[16];
     P.smem_bytes = s->smem_bytes;
     HIP_TRY(launch_step(P, st));
     s->use_guess = 1;

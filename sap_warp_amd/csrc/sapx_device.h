@@ -30,6 +30,12 @@ constexpr int kMaxDof = 64;       // dofs per world (one wavefront per row set)
 // Device model: per-world topology template (local indices, shared by all
 // worlds) + global per-item parameter arrays (world-major).
 // --------------------------------------------------------------------------
+enum TopoField {
+    TP_JTYPE = 0, TP_JPARENT, TP_JCHILD, TP_JQS, TP_JDS, TP_LEVEL_START, TP_LEVEL_JOINT, TP_DOF_JOINT,
+    TP_DOF_COORD, TP_DOF_TARGET, TP_DOF_LIMIT_OK, TP_BODY_ART_START, TP_BODY_ART_COUNT, TP_CHILD_START,
+    TP_CHILD, TP_BODY_ANC, TP_COUNT
+};
+
 struct DevModel {
     int W;   // worlds
     int J;   // joints per world
@@ -41,6 +47,12 @@ struct DevModel {
     int num_levels;
     int total_shapes;
     int global_shape_base;  // first global (world -1) shape index = W*S
+
+    // the per-world topology template packed into one int array (copied into LDS by
+    // the step kernel); topo_off[k] = first int of field k (see enum TopoField)
+    const int* topo;
+    int topo_ints;
+    int topo_off[16];
 
     // template, local indices
     const int* t_jtype;
@@ -59,6 +71,8 @@ struct DevModel {
     const unsigned long long* t_body_sub;  // NB: bit b = body b in subtree (incl. self)
     const int* t_body_art_start;  // NB: first local dof of the body's articulation (-1 none)
     const int* t_body_art_count;  // NB
+    const int* t_child_start;     // NB+1: CSR of child bodies
+    const int* t_child;           // child bodies, grouped by parent
     const int* t_pairs;           // P x 2: >=0 local shape, <0 => global shape (-v-1)
 
     // global per-item arrays

@@ -51,9 +51,9 @@ struct StepParams {
     int ls_mode, ls_max_iter;
     double armijo_c, rho, rel_slop;
     int weight_mode, point_mode, integ_mode;
-    int npad, js;
+    int npad, js, ast;  // padded dof count, J row stride, A row stride (both odd)
     // LDS byte offsets
-    int o_A, o_H, o_J, o_bq, o_S, o_ck, o_cv, o_cG, o_cb, o_dv64, o_dv32, o_dflag, o_red, o_q, o_misc, o_bqd;
+    int o_A, o_H, o_J, o_bq, o_S, o_ck, o_cv, o_cG, o_cb, o_dv64, o_dv32, o_dflag, o_red, o_q, o_misc, o_bqd, o_topo;
     int smem_bytes;
 };
 

@@ -29,36 +29,73 @@ constexpr double kLimitWindow = 2.0;      // contact_solve.py:51
 constexpr double kExactAlphaMax = 1.5;    // contact_solve.py:52
 constexpr double kExactFTol = 1.0e-8;     // contact_solve.py:54
 constexpr int kNewtonRefine = 1;          // float64-residual refinement steps per Newton direction
+constexpr int kCk = 8;                    // doubles of per-contact projection constants
 
 
 // ---------------------------------------------------------------------------
 // block utilities
 // ---------------------------------------------------------------------------
+// double-precision DPP move (two 32-bit halves); lanes outside ROW_MASK keep `old`
+template <int CTRL, int ROW_MASK = 0xF>
+__device__ __forceinline__ double dpp_d(double old, double x) {
+    const long long xi = __double_as_longlong(x), oi = __double_as_longlong(old);
+    const int lo = __builtin_amdgcn_update_dpp((int)oi, (int)xi, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(oi >> 32), (int)(xi >> 32), CTRL, ROW_MASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double readlane_d(double x, int l) {
+    const long long xi = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)xi, l);
+    const int hi = __builtin_amdgcn_readlane((int)(xi >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// sum over the 64 lanes of a fully active wave, returned uniformly: quad / half-row /
+// row butterflies, then row_bcast:15 and row_bcast:31 carry row sums up to lane 63
+__device__ __forceinline__ double wave_sum(double x) {
+    x += dpp_d<0xB1>(0.0, x);        // quad_perm [1,0,3,2]
+    x += dpp_d<0x4E>(0.0, x);        // quad_perm [2,3,0,1]
+    x += dpp_d<0x141>(0.0, x);       // row_half_mirror
+    x += dpp_d<0x140>(0.0, x);       // row_mirror
+    x += dpp_d<0x142, 0xA>(0.0, x);  // row_bcast:15 -> rows 1, 3
+    x += dpp_d<0x143, 0xC>(0.0, x);  // row_bcast:31 -> rows 2, 3
+    return readlane_d(x, 63);
+}
+
+// order this wave's LDS writes before its later LDS reads from other lanes
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Block-wide sums of K <= 8 values.  Wave partials go to one half of `red`
+// (alternating with `parity`), so a single barrier suffices: the half written
+// by call k is rewritten only at call k + 2, after every thread has passed the
+// barrier of call k + 1 and therefore finished reading call k's partials.
 template <int NT>
-__device__ __forceinline__ void block_sum(double* vals, int K, double* red) {
+__device__ __forceinline__ void block_sum(double* vals, int K, double* red, int& parity) {
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    double* half = red + parity * (NW * 8);
     for (int k = 0; k < K; ++k) {
-        double x = vals[k];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
-        if (lane == 0) red[wave * 8 + k] = x;
+        const double x = wave_sum(vals[k]);
+        if (lane == 0) half[wave * 8 + k] = x;
     }
     __syncthreads();
     for (int k = 0; k < K; ++k) {
-        double s = 0.0;
+        double sum = 0.0;
 #pragma unroll
-        for (int wv = 0; wv < NW; ++wv) s += red[wv * 8 + k];
-        vals[k] = s;
+        for (int wv = 0; wv < NW; ++wv) sum += half[wv * 8 + k];
+        vals[k] = sum;
     }
-    __syncthreads();
+    parity ^= 1;
 }
 
 template <int NT>
-__device__ __forceinline__ int block_or(int x, double* red) {
+__device__ __forceinline__ int block_or(int x, double* red, int& parity) {
     double v[1] = {x ? 1.0 : 0.0};
-    block_sum<NT>(v, 1, red);
+    block_sum<NT>(v, 1, red, parity);
     return v[0] > 0.0;
 }
 
@@ -99,12 +136,12 @@ __device__ __forceinline__ SV mob_bias_to_body(Q4 R, V3 p, SV Vf) {
 // LDS view
 // ---------------------------------------------------------------------------
 struct Smem {
-    float* A;       // n x npad  dynamics matrix
+    float* A;       // n x ast  dynamics matrix
     float* H;       // npad x (npad+1)  Hessian / Cholesky
     float* J;       // 3C x js  contact Jacobian (kinematics scratch before it)
     float* bq;      // NB x 8  body poses (free-motion FK)
     float* S;       // N x 6  motion subspace (body origin, world)
-    double* ck;     // C x 4  rt, rn, vhat, mu
+    double* ck;     // C x kCk  rt, rn, vhat, mu, 1/rt, 1/rn, mu_hat, f
     double* cv;     // C x 9  vc, dvc, gamma
     float* cG;      // C x 9
     int* cb;        // C x 2 local bodies
@@ -116,6 +153,47 @@ struct Smem {
     float* mp;      // per-world model parameters (prefetched)
     double* bqd;    // NB x 8  body poses in float64 (free-motion FK)
 };
+
+// LDS copy of the per-world topology template (identical for every world)
+struct Topo {
+    const int* jtype;
+    const int* jparent;
+    const int* jchild;
+    const int* jqs;
+    const int* jds;
+    const int* level_start;
+    const int* level_joint;
+    const int* dof_joint;
+    const int* dof_coord;
+    const int* dof_target;
+    const int* dof_limit_ok;
+    const int* body_art_start;
+    const int* body_art_count;
+    const int* child_start;
+    const int* child;
+    const unsigned long long* body_anc;
+};
+
+__device__ __forceinline__ Topo topo_view(const int* t, const DevModel& M) {
+    Topo T;
+    T.jtype = t + M.topo_off[TP_JTYPE];
+    T.jparent = t + M.topo_off[TP_JPARENT];
+    T.jchild = t + M.topo_off[TP_JCHILD];
+    T.jqs = t + M.topo_off[TP_JQS];
+    T.jds = t + M.topo_off[TP_JDS];
+    T.level_start = t + M.topo_off[TP_LEVEL_START];
+    T.level_joint = t + M.topo_off[TP_LEVEL_JOINT];
+    T.dof_joint = t + M.topo_off[TP_DOF_JOINT];
+    T.dof_coord = t + M.topo_off[TP_DOF_COORD];
+    T.dof_target = t + M.topo_off[TP_DOF_TARGET];
+    T.dof_limit_ok = t + M.topo_off[TP_DOF_LIMIT_OK];
+    T.body_art_start = t + M.topo_off[TP_BODY_ART_START];
+    T.body_art_count = t + M.topo_off[TP_BODY_ART_COUNT];
+    T.child_start = t + M.topo_off[TP_CHILD_START];
+    T.child = t + M.topo_off[TP_CHILD];
+    T.body_anc = reinterpret_cast<const unsigned long long*>(t + M.topo_off[TP_BODY_ANC]);
+    return T;
+}
 
 enum { V_V = 0, V_V0, V_VS, V_DV, V_G, V_JI, V_DP, V_AV, V_T1, V_T2, V_AVS, V_AR, V_NUM };
 enum { F_HD = 0, F_AINV, F_DSC, F_PDA, F_PDG, F_PDL, F_LOV, F_UPV, F_LOR, F_UPR, F_TAU, F_JF, F_DINV, F_NUM };
@@ -158,7 +236,7 @@ __device__ __forceinline__ float rl(float x, int l) {
 // the identity (identity padding as in blocked_cholesky.py:725-901).
 // ---------------------------------------------------------------------------
 template <int NT>
-__device__ void cholesky_blocked(float* H, int npad, int hs, int n, float* dinv) {
+__device__ __forceinline__ void cholesky_blocked(float* H, int npad, int hs, int n, float* dinv) {
     constexpr int NW = NT / 64;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -246,29 +324,146 @@ __device__ void cholesky_blocked(float* H, int npad, int hs, int n, float* dinv)
 }
 
 // Solve L L^T x = b (float32 factor), wave 0 only; b/x in LDS as double.
-// Serial substitution with readlane broadcasts; rows >= n are identity.
-__device__ void chol_solve_wave0(const float* H, int hs, int n, const float* dinv, double* b) {
+// Lane i owns x_i.  Serial substitution in 16-column blocks: each block's L
+// entries are loaded into registers before its 16 dependent steps, so the
+// chain is readlane -> multiply -> FMA without LDS latency.  Rows >= n are
+// identity.  Requires n <= 64.
+__device__ __forceinline__ void chol_solve_wave0(const float* H, int hs, int n, const float* dinv, double* b) {
     const int lane = threadIdx.x & 63;
     if ((threadIdx.x >> 6) != 0) return;
     float y = (lane < n) ? (float)b[lane] : 0.0f;
     const float di = (lane < n) ? dinv[lane] : 1.0f;
-    // forward: L y = b
-#pragma unroll 4
-    for (int j = 0; j < n; ++j) {
-        const float lij = (lane > j && lane < n) ? H[lane * hs + j] : 0.0f;
-        const float yj = rl(y, j) * rl(di, j);
-        if (lane == j) y = yj;
-        y -= lij * yj;
+    const int nblk = (n + 15) >> 4;
+    // forward: L y = b; lane i uses row i of L
+    for (int jb = 0; jb < nblk; ++jb) {
+        const int j0 = jb * 16;
+        float L[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int j = j0 + k;
+            L[k] = (lane > j && lane < n && j < n) ? H[lane * hs + j] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int j = j0 + k;
+            if (j < n) {
+                const float yj = rl(y, j) * rl(di, j);
+                if (lane == j) y = yj;
+                y -= L[k] * yj;
+            }
+        }
     }
-    // backward: L^T x = y
-#pragma unroll 4
-    for (int j = n - 1; j >= 0; --j) {
-        const float lji = (lane < j) ? H[j * hs + lane] : 0.0f;
-        const float xj = rl(y, j) * rl(di, j);
-        if (lane == j) y = xj;
-        y -= lji * xj;
+    // backward: L^T x = y; lane i uses column i of L
+    for (int jb = nblk - 1; jb >= 0; --jb) {
+        const int j0 = jb * 16;
+        float L[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int j = j0 + k;
+            L[k] = (lane < j && j < n) ? H[j * hs + lane] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 15; k >= 0; --k) {
+            const int j = j0 + k;
+            if (j < n) {
+                const float xj = rl(y, j) * rl(di, j);
+                if (lane == j) y = xj;
+                y -= L[k] * xj;
+            }
+        }
     }
     if (lane < n) b[lane] = (double)y;
+}
+
+// Inverses of the diagonal 16x16 blocks of the Cholesky factor, one block per
+// wave: lane c of the block's wave forward-substitutes column c of inv(L_kk)
+// (all L reads are LDS broadcasts, no cross-lane chain).  The strictly lower
+// part of inv(L_kk) is stored transposed in the block's strictly upper half,
+// which the factorization leaves unused; its diagonal is dinv.
+template <int NT>
+__device__ __forceinline__ void chol_block_inverses(float* H, int npad, int hs, const float* dinv) {
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nblk = npad >> 4;
+    for (int kb = wave; kb < nblk; kb += NW) {
+        const int k0 = kb * 16;
+        const int c = lane & 15;
+        float x[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int m = 0; m < i; ++m) acc += H[(k0 + i) * hs + k0 + m] * x[m];
+            const float di = dinv[k0 + i];
+            x[i] = (i == c) ? di : ((i > c) ? -di * acc : 0.0f);
+        }
+        if (lane < 16) {
+#pragma unroll
+            for (int i = 1; i < 16; ++i)
+                if (i > c) H[(k0 + c) * hs + k0 + i] = x[i];
+        }
+    }
+    __syncthreads();
+}
+
+// Solve L L^T x = b with the blocked factor and its inverted diagonal blocks
+// (chol_block_inverses), wave 0 only.  Lane i owns x_i.  Per 16-row block: the
+// off-block contribution is a dot product over finished blocks (independent
+// readlanes), then the block is one 16x16 triangular mat-vec.  Rows >= n are
+// identity.  Requires npad <= 64.
+__device__ __forceinline__ void chol_solve_blocks(const float* H, int npad, int hs, int n, const float* dinv,
+                                                  double* b) {
+    const int lane = threadIdx.x & 63;
+    if ((threadIdx.x >> 6) != 0) return;
+    const int nblk = npad >> 4;
+    const int row = lane < npad ? lane : npad - 1;
+    const float di = lane < n ? dinv[lane] : 1.0f;
+    float y = (lane < n) ? (float)b[lane] : 0.0f;
+    // forward: L y = b
+    for (int kb = 0; kb < nblk; ++kb) {
+        const int k0 = kb * 16;
+        float r = y;
+        for (int mb = 0; mb < kb; ++mb) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int m = mb * 16 + t;
+                r -= H[row * hs + m] * rl(y, m);
+            }
+        }
+        const int li = lane - k0;  // position inside the block (valid for 0..15)
+        float yk = di * r;
+#pragma unroll
+        for (int t = 0; t < 15; ++t) {
+            const float rt = rl(r, k0 + t);
+            const float dk = (li > t && li < 16) ? H[(k0 + t) * hs + k0 + (li & 15)] : 0.0f;
+            yk += dk * rt;
+        }
+        if (li >= 0 && li < 16) y = yk;
+    }
+    // backward: L^T x = y
+    float x = y;
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+        const int k0 = kb * 16;
+        float r = x;
+        for (int mb = kb + 1; mb < nblk; ++mb) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const int m = mb * 16 + t;
+                r -= H[m * hs + row] * rl(x, m);
+            }
+        }
+        const int li = lane - k0;
+        float xk = di * r;
+#pragma unroll
+        for (int t = 1; t < 16; ++t) {
+            const float rt = rl(r, k0 + t);
+            const float dk = (li >= 0 && li < t) ? H[(k0 + (li & 15)) * hs + k0 + t] : 0.0f;
+            xk += dk * rt;
+        }
+        if (li >= 0 && li < 16) x = xk;
+    }
+    if (lane < n) b[lane] = (double)x;
 }
 
 // ---------------------------------------------------------------------------
@@ -277,11 +472,12 @@ __device__ void chol_solve_wave0(const float* H, int hs, int n, const float* din
 __device__ __forceinline__ double project_contact(const double* k, const double vc[3], double g[3], float* G,
                                                   bool want_G) {
     const double rt = k[0], rn = k[1], vhat = k[2], mu = k[3];
-    const double rt_inv = 1.0 / rt, rn_inv = 1.0 / rn;
+    const double rt_inv = k[4], rn_inv = k[5];
     const double y0 = -rt_inv * vc[0], y1 = -rt_inv * vc[1], y2 = rn_inv * (vhat - vc[2]);
+    // the soft norm keeps yr >= 1e-7 > 0
     const double yr = sqrt(y0 * y0 + y1 * y1 + kSoftNormTol * kSoftNormTol);
-    double tx = 0.0, ty = 0.0;
-    if (yr > 0.0) { tx = y0 / yr; ty = y1 / yr; }
+    const double yr_inv = 1.0 / yr;
+    const double tx = y0 * yr_inv, ty = y1 * yr_inv;
     g[0] = g[1] = g[2] = 0.0;
     float Gm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (mu <= 1.0e-12) {
@@ -290,23 +486,21 @@ __device__ __forceinline__ double project_contact(const double* k, const double 
             Gm[8] = (float)rn_inv;
         }
     } else {
-        const double mu_tilde = mu * sqrt(rt / rn);
-        const double mu_hat = mu * rt / rn;
-        const double f = 1.0 / (1.0 + mu_tilde * mu_tilde);
+        const double mu_hat = k[6], f = k[7];
         if (yr <= mu * y2) {
             g[0] = y0; g[1] = y1; g[2] = y2;
             Gm[0] = (float)rt_inv; Gm[4] = (float)rt_inv; Gm[8] = (float)rn_inv;
-        } else if ((-mu_hat * yr < y2) && (y2 < yr / mu)) {
+        } else if ((-mu_hat * yr < y2) && (y2 * mu < yr)) {
             const double gn = (y2 + mu_hat * yr) * f;
             g[0] = mu * gn * tx; g[1] = mu * gn * ty; g[2] = gn;
             if (want_G) {
                 const double p00 = tx * tx, p01 = tx * ty, p11 = ty * ty;
-                const double gy = (yr > 0.0) ? gn / yr : 0.0;
+                const double gy = gn * yr_inv;
                 const double mf = mu_hat * f;
                 Gm[0] = (float)(mu * (gy * (1.0 - p00) + mf * p00) * rt_inv);
                 Gm[1] = (float)(mu * (gy * (-p01) + mf * p01) * rt_inv);
                 Gm[2] = (float)(mu * f * tx * rn_inv);
-                Gm[3] = (float)(mu * (gy * (-p01) + mf * p01) * rt_inv);
+                Gm[3] = Gm[1];
                 Gm[4] = (float)(mu * (gy * (1.0 - p11) + mf * p11) * rt_inv);
                 Gm[5] = (float)(mu * f * ty * rn_inv);
                 Gm[6] = (float)(mf * tx * rt_inv);
@@ -395,7 +589,7 @@ __device__ __forceinline__ double model_terms(const Smem& s, int npad, int i, do
 // The fused step kernel
 // ---------------------------------------------------------------------------
 template <int NT>
-__global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
+__global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     const DevModel& M = P.m;
     const int w = blockIdx.x;
@@ -406,8 +600,10 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     const int npad = P.npad;
     const int hs = npad + 1;
     const int js = P.js;
+    const int ast = P.ast;  // row stride of A (odd: conflict-free row-per-thread access)
     const int C = P.C;
     Smem s = smem_view(smem_raw, P);
+    const Topo T = topo_view(reinterpret_cast<const int*>(smem_raw + P.o_topo), M);
     double* d64 = s.d64;
     float* d32 = s.d32;
     const double dt = P.dt_world ? P.dt_world[w] : P.dt;
@@ -445,6 +641,8 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     for (int t = tid; t < nb * 3; t += NT) mCom[(t / 3) * 4 + t % 3] = M.body_com[(size_t)w * nb * 3 + t];
     for (int t = tid; t < nb * 9; t += NT) mI[t] = M.body_inertia[(size_t)w * nb * 9 + t];
     for (int t = tid; t < nb; t += NT) mM[t] = M.body_mass[(size_t)w * nb + t];
+    int* topo_lds = reinterpret_cast<int*>(smem_raw + P.o_topo);
+    for (int k = tid; k < M.topo_ints; k += NT) topo_lds[k] = M.topo[k];
     for (int k = tid; k < M.NQ; k += NT) q_in[k] = P.jq_in[(size_t)w * M.NQ + k];
     __syncthreads();
     for (int d = tid; d < n; d += NT) {
@@ -471,10 +669,10 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     }
     __syncthreads();
     for (int j = tid; j < nj; j += NT) {
-        const int jt = M.t_jtype[j];
+        const int jt = T.jtype[j];
         if (jt == kJointFree || jt == kJointDistance) {
-            const int ds = M.t_jds[j];
-            const int child = M.t_jchild[j];
+            const int ds = T.jds[j];
+            const int child = T.jchild[j];
             const size_t gb = (size_t)w * nb + child;
             TF x = load_tf(P.bq_in + gb * 7);
             const float* com = mCom + child * 4;
@@ -505,102 +703,125 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     __syncthreads();
 
     // ------------------------------------------------------------------ 2
-    // forward kinematics + body velocity / bias acceleration by tree level
-    // (free_motion.py:412-530, sap_helpers.py:704-1433)
-    for (int L = 0; L < M.num_levels; ++L) {
-        const int l0 = M.t_level_start[L], l1 = M.t_level_start[L + 1];
-        for (int t = l0 + tid; t < l1; t += NT) {
-            const int j = M.t_level_joint[t];
-            const size_t gj = (size_t)w * nj + j;
-            const int jt = M.t_jtype[j];
-            const int parent = M.t_jparent[j];
-            const int child = M.t_jchild[j];
-            const int qs = M.t_jqs[j];
-            const int ds = M.t_jds[j];
-            TF Xp = load_tf(mXp + j * 8);
-            TF Xc = load_tf(mXc + j * 8);
-            const bool xcid = tf_is_identity(mXc + j * 8);
-            // poses in float64 (free_motion.py:412-530 evaluates them with transformd)
-            const DTF Xp_d = dtf_from(Xp);
-            const DTF Xwpj_d = parent >= 0 ? dtf_compose_n(load_dtf(s.bqd + parent * 8), Xp_d) : Xp_d;
-            const TF Xwpj = tf_from(Xwpj_d);
-            DTF Xwc_d;
-            TF Xwc;
-            SV vPB = sv(v3(0, 0, 0), v3(0, 0, 0));
-            SV aPB = vPB;
-            const float* axp = mAx + ds * 4;
-            if (jt == kJointRevolute && xcid) {
-                V3 axis = v3(axp[0], axp[1], axp[2]);
-                const DQ qfm = dq_axis_angle(d3(axis.x, axis.y, axis.z), (double)q_in[qs]);
-                Xwc_d = DTF{Xwpj_d.p, dqmul(Xwpj_d.q, qfm)};
-                SV S = sv(qrot(Xwpj.q, axis), v3(0, 0, 0));
+    // forward kinematics + body velocity / bias acceleration (free_motion.py:412-530,
+    // sap_helpers.py:704-1433).  (a) every joint's local part in parallel: mobilizer
+    // transform (float64) composed with X_c^-1, motion subspace / velocity / bias in the
+    // parent joint frame; (b) wave 0 walks the tree level by level and only composes with
+    // the parent and rotates the local parts, with wave-level ordering instead of block
+    // barriers between levels.
+    double* kXl = reinterpret_cast<double*>(kQdPub + ((n + 1) & ~1));  // nj x 8 local transforms
+    for (int j = tid; j < nj; j += NT) {
+        const int jt = T.jtype[j];
+        const int child = T.jchild[j];
+        const int qs = T.jqs[j];
+        const int ds = T.jds[j];
+        const TF Xc = load_tf(mXc + j * 8);
+        const bool xcid = tf_is_identity(mXc + j * 8);
+        const float* axp = mAx + ds * 4;
+        const V3 axis = v3(axp[0], axp[1], axp[2]);
+        const D3 axd = d3(axis.x, axis.y, axis.z);
+        DTF Xl = DTF{d3(0.0, 0.0, 0.0), DQ{0.0, 0.0, 0.0, 1.0}};
+        double plain = 0.0;  // 1: X_wc = [X_wpj.p, X_wpj.q * q_FM] without normalisation
+        SV vl = sv(v3(0, 0, 0), v3(0, 0, 0));
+        SV al = vl;
+        if (jt == kJointRevolute && xcid) {
+            // _sap_revolute_identity_child_kinematics (sap_helpers.py:704-720)
+            Xl.q = dq_axis_angle(axd, (double)q_in[qs]);
+            plain = 1.0;
+            const SV S = sv(axis, v3(0, 0, 0));
+            store_sv(s.S + ds * 6, S);
+            vl = sv_scale(S, (float)d64[V_V0 * npad + ds]);
+        } else {
+            DTF Xj = Xl;
+            if (jt == kJointPrismatic) {
+                const double qv = q_in[qs];
+                Xj.p = d3(axis.x * qv, axis.y * qv, axis.z * qv);
+            } else if (jt == kJointRevolute) {
+                Xj.q = dq_axis_angle(axd, (double)q_in[qs]);
+            } else if (jt == kJointFree || jt == kJointDistance) {
+                Xj = DTF{d3(q_in[qs], q_in[qs + 1], q_in[qs + 2]), DQ{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]}};
+            }
+            Xl = xcid ? Xj : dtf_compose_n(Xj, dtf_inverse_n(dtf_from(Xc)));
+            // mobilizer velocity / bias in the parent joint frame (sap_helpers.py:1289-1433)
+            const TF Xmb = tf_inverse_n(Xc);
+            const Q4 qj = Q4{(float)Xj.q.x, (float)Xj.q.y, (float)Xj.q.z, (float)Xj.q.w};
+            const V3 pmb = qrot(qj, Xmb.p);
+            if (jt == kJointPrismatic) {
+                const SV S = sv(v3(0, 0, 0), axis);
                 store_sv(s.S + ds * 6, S);
-                vPB = sv_scale(S, (float)d64[V_V0 * npad + ds]);
-            } else {
-                TF Xj = TF{v3(0, 0, 0), Q4{0, 0, 0, 1}};
-                V3 axis = v3(axp[0], axp[1], axp[2]);
-                if (jt == kJointPrismatic) {
-                    Xj.p = axis * q_in[qs];
-                } else if (jt == kJointRevolute) {
-                    Xj.q = q_axis_angle(axis, q_in[qs]);
-                } else if (jt == kJointFree || jt == kJointDistance) {
-                    Xj = TF{v3(q_in[qs], q_in[qs + 1], q_in[qs + 2]),
-                            Q4{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]}};
-                }
-                DTF Xj_d = dtf_from(Xj);
-                if (jt == kJointRevolute) Xj_d.q = dq_axis_angle(d3(axis.x, axis.y, axis.z), (double)q_in[qs]);
-                if (jt == kJointPrismatic) {
-                    const double qv = q_in[qs];
-                    Xj_d.p = d3(axis.x * qv, axis.y * qv, axis.z * qv);
-                }
-                const DTF Xwm_d = dtf_compose_n(Xwpj_d, Xj_d);
-                Xwc_d = xcid ? Xwm_d : dtf_compose_n(Xwm_d, dtf_inverse_n(dtf_from(Xc)));
-                TF Xmb = tf_inverse_n(Xc);
-                V3 pmb = qrot(Xj.q, Xmb.p);
-                Q4 R = Xwpj.q;
-                if (jt == kJointPrismatic) {
-                    SV S = mob_to_body(R, pmb, sv(v3(0, 0, 0), axis));
-                    store_sv(s.S + ds * 6, S);
-                    float qd = (float)d64[V_V0 * npad + ds];
-                    vPB = sv_scale(S, qd);
-                    aPB = mob_bias_to_body(R, pmb, sv(v3(0, 0, 0), axis * qd));
-                } else if (jt == kJointRevolute) {
-                    SV S = sv(qrot(R, axis), qrot(R, cross(axis, pmb)));
-                    store_sv(s.S + ds * 6, S);
-                    float qd = (float)d64[V_V0 * npad + ds];
-                    V3 wf = axis * qd;
-                    aPB = sv(v3(0, 0, 0), qrot(R, cross(wf, cross(wf, pmb))));
-                    vPB = sv_scale(S, qd);
-                } else if (jt == kJointFree || jt == kJointDistance) {
-                    const double* v0 = d64 + V_V0 * npad + ds;
-                    SV Vf = sv(v3((float)v0[0], (float)v0[1], (float)v0[2]), v3((float)v0[3], (float)v0[4], (float)v0[5]));
-                    vPB = mob_to_body(R, pmb, Vf);
-                    for (int k = 0; k < 3; ++k) {
-                        V3 e = v3(k == 0, k == 1, k == 2);
-                        store_sv(s.S + (ds + k) * 6, mob_to_body(R, pmb, sv(e, v3(0, 0, 0))));
-                        store_sv(s.S + (ds + 3 + k) * 6, mob_to_body(R, pmb, sv(v3(0, 0, 0), e)));
-                    }
-                    aPB = mob_bias_to_body(R, pmb, Vf);
+                vl = sv_scale(S, (float)d64[V_V0 * npad + ds]);
+            } else if (jt == kJointRevolute) {
+                const SV S = sv(axis, cross(axis, pmb));
+                store_sv(s.S + ds * 6, S);
+                const float qd = (float)d64[V_V0 * npad + ds];
+                const V3 wf = axis * qd;
+                al = sv(v3(0, 0, 0), cross(wf, cross(wf, pmb)));
+                vl = sv_scale(S, qd);
+            } else if (jt == kJointFree || jt == kJointDistance) {
+                const double* v0 = d64 + V_V0 * npad + ds;
+                const V3 wv = v3((float)v0[0], (float)v0[1], (float)v0[2]);
+                const V3 vv = v3((float)v0[3], (float)v0[4], (float)v0[5]);
+                vl = sv(wv, vv + cross(wv, pmb));
+                al = sv(v3(0, 0, 0), cross(wv, cross(wv, pmb)));
+                for (int k = 0; k < 3; ++k) {
+                    const V3 e = v3(k == 0, k == 1, k == 2);
+                    store_sv(s.S + (ds + k) * 6, sv(e, cross(e, pmb)));
+                    store_sv(s.S + (ds + 3 + k) * 6, sv(v3(0, 0, 0), e));
                 }
             }
-            store_dtf(s.bqd + child * 8, Xwc_d);
-            Xwc = tf_from(Xwc_d);
-            float* bqc = s.bq + child * 8;
-            bqc[0] = Xwc.p.x; bqc[1] = Xwc.p.y; bqc[2] = Xwc.p.z;
-            bqc[3] = Xwc.q.x; bqc[4] = Xwc.q.y; bqc[5] = Xwc.q.z; bqc[6] = Xwc.q.w;
-            SV V = vPB, A = aPB;
-            if (parent >= 0) {
-                V3 pPB = Xwc.p - v3(s.bq[parent * 8], s.bq[parent * 8 + 1], s.bq[parent * 8 + 2]);
-                SV Vp = load_sv(kV + parent * 6);
-                SV Ap = load_sv(kA + parent * 6);
-                V = compose_velocity(Vp, pPB, vPB);
-                A = compose_acceleration(Ap, Vp, pPB, vPB, aPB);
-            }
-            store_sv(kV + child * 6, V);
-            store_sv(kA + child * 6, A);
         }
-        __syncthreads();
+        store_dtf(kXl + j * 8, Xl);
+        kXl[j * 8 + 7] = plain;
+        store_sv(kV + child * 6, vl);
+        store_sv(kA + child * 6, al);
     }
+    __syncthreads();
+    if (tid < 64) {
+        const int lane = tid;
+        for (int L = 0; L < M.num_levels; ++L) {
+            const int l0 = T.level_start[L], l1 = T.level_start[L + 1];
+            for (int t = l0 + lane; t < l1; t += 64) {
+                const int j = T.level_joint[t];
+                const int parent = T.jparent[j];
+                const int child = T.jchild[j];
+                const int ds = T.jds[j], nd = T.jds[j + 1] - ds;
+                const DTF Xp_d = dtf_from(load_tf(mXp + j * 8));
+                const DTF Xwpj_d = parent >= 0 ? dtf_compose_n(load_dtf(s.bqd + parent * 8), Xp_d) : Xp_d;
+                const DTF Xl = load_dtf(kXl + j * 8);
+                const DTF Xwc_d = (kXl[j * 8 + 7] != 0.0) ? DTF{Xwpj_d.p, dqmul(Xwpj_d.q, Xl.q)}
+                                                          : dtf_compose_n(Xwpj_d, Xl);
+                const Q4 R = tf_from(Xwpj_d).q;
+                for (int k = 0; k < nd; ++k) {
+                    const SV Sl = load_sv(s.S + (ds + k) * 6);
+                    store_sv(s.S + (ds + k) * 6, sv(qrot(R, Sl.w), qrot(R, Sl.v)));
+                }
+                const SV vl = load_sv(kV + child * 6);
+                const SV al = load_sv(kA + child * 6);
+                const SV vPB = sv(qrot(R, vl.w), qrot(R, vl.v));
+                const SV aPB = sv(qrot(R, al.w), qrot(R, al.v));
+                store_dtf(s.bqd + child * 8, Xwc_d);
+                const TF Xwc = tf_from(Xwc_d);
+                float* bqc = s.bq + child * 8;
+                bqc[0] = Xwc.p.x; bqc[1] = Xwc.p.y; bqc[2] = Xwc.p.z;
+                bqc[3] = Xwc.q.x; bqc[4] = Xwc.q.y; bqc[5] = Xwc.q.z; bqc[6] = Xwc.q.w;
+                SV V = vPB, A = aPB;
+                if (parent >= 0) {
+                    const V3 pPB = Xwc.p - v3(s.bq[parent * 8], s.bq[parent * 8 + 1], s.bq[parent * 8 + 2]);
+                    const SV Vp = load_sv(kV + parent * 6);
+                    const SV Ap = load_sv(kA + parent * 6);
+                    V = compose_velocity(Vp, pPB, vPB);
+                    A = compose_acceleration(Ap, Vp, pPB, vPB, aPB);
+                }
+                store_sv(kV + child * 6, V);
+                store_sv(kA + child * 6, A);
+            }
+            // the next level reads this level's LDS writes from other lanes of this wave
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    __syncthreads();
 
     // ------------------------------------------------------------------ 3
     // body dynamics (free_motion.py:680-719) + inertia about O = root origin
@@ -664,35 +885,46 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     __syncthreads();
 
     // ------------------------------------------------------------------ 4
-    // composites over subtrees: inertia about O, force at body origin
-    for (int t = tid; t < nb * 10; t += NT) {
-        const int c = t / 10, k = t % 10;
-        const unsigned long long sub = M.t_body_sub[c];
-        float acc = 0.0f;
-        for (int b = 0; b < nb; ++b)
-            if (bit(sub, b)) acc += kI[b * 10 + k];
-        kIc[c * 10 + k] = acc;
-    }
-    for (int c = tid; c < nb; c += NT) {
-        const unsigned long long sub = M.t_body_sub[c];
-        V3 pc = v3(s.bq[c * 8], s.bq[c * 8 + 1], s.bq[c * 8 + 2]);
-        V3 tw = v3(0, 0, 0), fv = v3(0, 0, 0);
-        for (int b = 0; b < nb; ++b) {
-            if (!bit(sub, b)) continue;
-            const float* F = kF + b * 6;
-            V3 f = v3(F[3], F[4], F[5]);
-            V3 pb = v3(s.bq[b * 8], s.bq[b * 8 + 1], s.bq[b * 8 + 2]);
-            // shift_force(F_b, p_c - p_b) = [tau - (p_c - p_b) x f, f] (sap_helpers.py:962-966)
-            tw = tw + v3(F[0], F[1], F[2]) - cross(pc - pb, f);
-            fv = fv + f;
+    // composites over subtrees (inertia about O, force at the body origin): reverse
+    // level sweep on wave 0, each body pulls its children's finished composites
+    // (deterministic order; wave-level ordering between levels)
+    if (tid < 64) {
+        const int lane = tid;
+        for (int L = M.num_levels - 1; L >= 0; --L) {
+            const int l0 = T.level_start[L], l1 = T.level_start[L + 1];
+            for (int t = l0 + lane; t < l1; t += 64) {
+                const int c = T.jchild[T.level_joint[t]];
+                float Ic[10];
+#pragma unroll
+                for (int k = 0; k < 10; ++k) Ic[k] = kI[c * 10 + k];
+                const float* F = kF + c * 6;
+                V3 tw = v3(F[0], F[1], F[2]), fv = v3(F[3], F[4], F[5]);
+                const V3 pc = v3(s.bq[c * 8], s.bq[c * 8 + 1], s.bq[c * 8 + 2]);
+                for (int e = T.child_start[c]; e < T.child_start[c + 1]; ++e) {
+                    const int ch = T.child[e];
+#pragma unroll
+                    for (int k = 0; k < 10; ++k) Ic[k] += kIc[ch * 10 + k];
+                    const float* Fch = kFc + ch * 6;
+                    const V3 f = v3(Fch[3], Fch[4], Fch[5]);
+                    const V3 pch = v3(s.bq[ch * 8], s.bq[ch * 8 + 1], s.bq[ch * 8 + 2]);
+                    // shift_force(F_ch, p_c - p_ch) = [tau - (p_c - p_ch) x f, f] (sap_helpers.py:962-966)
+                    tw = tw + v3(Fch[0], Fch[1], Fch[2]) - cross(pc - pch, f);
+                    fv = fv + f;
+                }
+#pragma unroll
+                for (int k = 0; k < 10; ++k) kIc[c * 10 + k] = Ic[k];
+                float* Fc = kFc + c * 6;
+                Fc[0] = tw.x; Fc[1] = tw.y; Fc[2] = tw.z; Fc[3] = fv.x; Fc[4] = fv.y; Fc[5] = fv.z;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        float* Fc = kFc + c * 6;
-        Fc[0] = tw.x; Fc[1] = tw.y; Fc[2] = tw.z; Fc[3] = fv.x; Fc[4] = fv.y; Fc[5] = fv.z;
     }
     __syncthreads();
     for (int d = tid; d < n; d += NT) {
-        const int j = M.t_dof_joint[d];
-        const int c = M.t_jchild[j];
+        const int j = T.dof_joint[d];
+        const int c = T.jchild[j];
         SV S = load_sv(s.S + d * 6);
         // tau = joint_f - S . f_s  (free_motion.py:93-147, sap_helpers.py:2090-2118)
         d32[F_TAU * npad + d] = d32[F_JF * npad + d] - sv_dot(S, load_sv(kFc + c * 6));
@@ -715,14 +947,14 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         const int i = t / npad, j = t % npad;
         float val;
         if (i < n && j < n) {
-            const int ji = M.t_dof_joint[i], jj = M.t_dof_joint[j];
-            const unsigned long long anc_i = M.t_body_anc[M.t_jchild[ji]];
-            const unsigned long long anc_j = M.t_body_anc[M.t_jchild[jj]];
+            const int ji = T.dof_joint[i], jj = T.dof_joint[j];
+            const unsigned long long anc_i = T.body_anc[T.jchild[ji]];
+            const unsigned long long anc_j = T.body_anc[T.jchild[jj]];
             val = 0.0f;
             if (bit(anc_i, jj)) val = sv_dot(load_sv(kFi + i * 6), load_sv(kSO + j * 6));
             else if (bit(anc_j, ji)) val = sv_dot(load_sv(kFi + j * 6), load_sv(kSO + i * 6));
             if (i == j) val += M.armature[(size_t)w * n + i];
-            s.A[i * npad + j] = val;
+            s.A[i * ast + j] = val;
         } else {
             val = (i == j) ? 1.0f : 0.0f;
         }
@@ -733,17 +965,16 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     // ------------------------------------------------------------------ 5
     // free motion v* = v0 + dt A^-1 tau with one refinement step (free_motion.py:1970-2178)
     cholesky_blocked<NT>(s.H, npad, hs, n, d32 + F_DINV * npad);
+    chol_block_inverses<NT>(s.H, npad, hs, d32 + F_DINV * npad);
     for (int d = tid; d < npad; d += NT) d64[V_T1 * npad + d] = (d < n) ? (double)d32[F_TAU * npad + d] : 0.0;
     __syncthreads();
-    chol_solve_wave0(s.H, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
+    chol_solve_blocks(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
     __syncthreads();
     for (int i = tid; i < n; i += NT) {
-        double r = (double)d32[F_TAU * npad + i];
-        for (int j = 0; j < n; ++j) r -= (double)s.A[i * npad + j] * d64[V_T1 * npad + j];
-        d64[V_T2 * npad + i] = r;
+        d64[V_T2 * npad + i] = (double)d32[F_TAU * npad + i] - dot_fd(s.A + i * ast, d64 + V_T1 * npad, n);
     }
     __syncthreads();
-    chol_solve_wave0(s.H, hs, n, d32 + F_DINV * npad, d64 + V_T2 * npad);
+    chol_solve_blocks(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T2 * npad);
     __syncthreads();
     for (int d = tid; d < npad; d += NT) {
         double qdd = (d < n) ? d64[V_T1 * npad + d] + d64[V_T2 * npad + d] : 0.0;
@@ -853,18 +1084,18 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         const int c = t / js, d = t % js;
         float j0 = 0.0f, j1 = 0.0f, j2 = 0.0f;
         if (d < n) {
-            const int jd = M.t_dof_joint[d];
-            const int cd = M.t_jchild[jd];
+            const int jd = T.dof_joint[d];
+            const int cd = T.jchild[jd];
             SV S = load_sv(s.S + d * 6);
             V3 pcd = v3(s.bq[cd * 8], s.bq[cd * 8 + 1], s.bq[cd * 8 + 2]);
             V3 rel = v3(0, 0, 0);
             const int lb0 = s.cb[c * 2], lb1 = s.cb[c * 2 + 1];
             const bool wit = P.point_mode == 0;
-            if (lb1 >= 0 && bit(M.t_body_anc[lb1], jd)) {
+            if (lb1 >= 0 && bit(T.body_anc[lb1], jd)) {
                 V3 xs = wit ? v3(cx1[c * 3], cx1[c * 3 + 1], cx1[c * 3 + 2]) : v3(cpc[c * 3], cpc[c * 3 + 1], cpc[c * 3 + 2]);
                 rel = rel + S.v + cross(S.w, xs - pcd);
             }
-            if (lb0 >= 0 && bit(M.t_body_anc[lb0], jd)) {
+            if (lb0 >= 0 && bit(T.body_anc[lb0], jd)) {
                 V3 xs = wit ? v3(cx0[c * 3], cx0[c * 3 + 1], cx0[c * 3 + 2]) : v3(cpc[c * 3], cpc[c * 3 + 1], cpc[c * 3 + 2]);
                 rel = rel - (S.v + cross(S.w, xs - pcd));
             }
@@ -879,7 +1110,7 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     }
     // diag(A) data (contact_solve.py:405-419)
     for (int d = tid; d < npad; d += NT) {
-        double a = (d < n) ? (double)s.A[d * npad + d] : 1.0;
+        double a = (d < n) ? (double)s.A[d * ast + d] : 1.0;
         if (a < 1.0e-12 || !isfinite(a)) a = 1.0e-12;
         d32[F_AINV * npad + d] = (float)(1.0 / a);
         d32[F_DSC * npad + d] = (float)(1.0 / sqrt(a));
@@ -936,10 +1167,17 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         const double rt = fmax(P.sigma * wv, 1.0e-30);
         double mu = cphi[c * 4 + 3];
         if (mu < 0.0 || !isfinite(mu)) mu = 0.0;
-        s.ck[c * 4 + 0] = rt;
-        s.ck[c * 4 + 1] = rn;
-        s.ck[c * 4 + 2] = -cphi[c * 4] / (dt + tau_c);
-        s.ck[c * 4 + 3] = mu;
+        // per-contact constants of the projection, hoisted out of every evaluation
+        double* kc8 = s.ck + c * kCk;
+        kc8[0] = rt;
+        kc8[1] = rn;
+        kc8[2] = -cphi[c * 4] / (dt + tau_c);
+        kc8[3] = mu;
+        kc8[4] = 1.0 / rt;
+        kc8[5] = 1.0 / rn;
+        const double mu_tilde = mu * sqrt(rt / rn);
+        kc8[6] = mu * rt / rn;                          // mu_hat
+        kc8[7] = 1.0 / (1.0 + mu_tilde * mu_tilde);     // f
         if (P.dbg_phi0) P.dbg_phi0[(size_t)w * C + c] = cphi[c * 4];
         if (P.dbg_w) P.dbg_w[(size_t)w * C + c] = wv;
     }
@@ -951,10 +1189,10 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         int pd = 0, lo = 0, up = 0, part = 0;
         float pda = 0, pdg = 0, pdl = 0, lov = 0, upv = 0, lor = 1, upr = 1;
         if (i < n) {
-            const int t = M.t_dof_target[i];
+            const int t = T.dof_target[i];
             const size_t gt = (size_t)w * n + t;
             const size_t gi = (size_t)w * n + i;
-            const int coord = M.t_dof_coord[i];
+            const int coord = T.dof_coord[i];
             const double ainv = d32[F_AINV * npad + i];
             if (M.target_mode[gt] != 0) {
                 const double kp = M.target_ke[gt], kd = M.target_kd[gt];
@@ -996,7 +1234,7 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
                     }
                 }
             }
-            if (coord >= 0 && M.t_dof_limit_ok[i]) {
+            if (coord >= 0 && T.dof_limit_ok[i]) {
                 const double ke = M.limit_ke[gi];
                 if (ke > 0.0) {
                     const double q0 = q_in[coord];
@@ -1020,7 +1258,7 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
                 for (int side = 0; side < 2; ++side) {
                     const int lb = s.cb[c * 2 + side];
                     if (lb >= 0) {
-                        const int st = M.t_body_art_start[lb], ct = M.t_body_art_count[lb];
+                        const int st = T.body_art_start[lb], ct = T.body_art_count[lb];
                         if (st >= 0 && ct > 0 && i >= st && i < st + ct) part = 1;
                     }
                 }
@@ -1044,11 +1282,12 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     __syncthreads();
 
     // ------------------------------------------------------------------ 8
-    // SAP Newton solve (contact_solve.py:6129-6285, 5474-5655)
-    int any_model = 0;
-    for (int i = tid; i < n; i += NT)
-        any_model |= s.dflag[I_PD * npad + i] | s.dflag[I_LO * npad + i] | s.dflag[I_UP * npad + i];
-    any_model = block_or<NT>(any_model, s.red);
+    // SAP Newton solve (contact_solve.py:6129-6285, 5474-5655).  Every stage that needs at
+    // most one row / contact / dof per lane runs on wave 0 alone with wave-level ordering
+    // (no block barriers); the Hessian assembly and its factorization use the whole block.
+    // ctl[0] carries wave 0's continue / stop decision to the other waves.
+    const int lane = tid & 63, wave = tid >> 6;
+    int* ctl = reinterpret_cast<int*>(s.red);
     double* v = d64 + V_V * npad;
     double* vs = d64 + V_VS * npad;
     double* dvv = d64 + V_DV * npad;
@@ -1056,161 +1295,178 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     double* jimp = d64 + V_JI * npad;
     double* dp = d64 + V_DP * npad;
     double* av = d64 + V_AV * npad;
-    const bool unconstrained = (nc == 0) && !any_model;
-    for (int i = tid; i < npad; i += NT) {
-        double vi = 0.0;
-        if (i < n) {
-            vi = P.use_guess ? P.v_guess[(size_t)w * n + i] : d64[V_V0 * npad + i];
-            if (unconstrained || !s.dflag[I_PART * npad + i]) vi = vs[i];
+    int iteration = 0, ls_total = 0, flags = 0, ls_status = 0;  // meaningful on wave 0
+    bool unconstrained = false;
+    if (wave == 0) {
+        int am = 0;
+        for (int i = lane; i < n; i += 64)
+            am |= s.dflag[I_PD * npad + i] | s.dflag[I_LO * npad + i] | s.dflag[I_UP * npad + i];
+        unconstrained = (nc == 0) && !__any(am);
+        for (int i = lane; i < npad; i += 64) {
+            double vi = 0.0;
+            if (i < n) {
+                vi = P.use_guess ? P.v_guess[(size_t)w * n + i] : d64[V_V0 * npad + i];
+                if (unconstrained || !s.dflag[I_PART * npad + i]) vi = vs[i];
+            }
+            v[i] = vi;
+            dvv[i] = 0.0;
+            d64[V_AVS * npad + i] = (i < n) ? dot_fd(s.A + i * ast, vs, n) : 0.0;
         }
-        v[i] = vi;
-        dvv[i] = 0.0;
-        d64[V_AVS * npad + i] = (i < n) ? dot_fd(s.A + i * npad, vs, n) : 0.0;
+        wave_sync();
     }
-    __syncthreads();
-    int iteration = 0, ls_total = 0, flags = 0, ls_status = 0;
-    if (unconstrained) {
-        flags = 1;
-    } else {
-        double alpha = 1.0, prev_cost = 0.0;
-        const double cost_min_alpha = 0.5;
-        while (true) {
-            // ---- evaluate at v: vc = J v and A v (one stage), projection + model terms,
-            //      then impulses / gradient / norms (contact_solve.py:743-921, 1493-1627)
-            for (int t = tid; t < 3 * nc + n; t += NT) {
-                if (t < 3 * nc) {
-                    s.cv[(t / 3) * 9 + (t % 3)] = dot_fd(s.J + t * js, v, n);
-                } else {
-                    const int i = t - 3 * nc;
-                    const double avi = dot_fd(s.A + i * npad, v, n);
-                    av[i] = avi;
-                    d64[V_AR * npad + i] = avi - d64[V_AVS * npad + i];
-                }
-            }
-            __syncthreads();
-            double part3[4] = {0.0, 0.0, 0.0, 0.0};  // cost, |D g|^2, |D A v|^2, |D j|^2
-            for (int t = tid; t < nc + n; t += NT) {
-                if (t < nc) {
-                    const int c = t;
-                    double vc[3] = {s.cv[c * 9], s.cv[c * 9 + 1], s.cv[c * 9 + 2]};
-                    double g[3];
-                    part3[0] += project_contact(s.ck + c * 4, vc, g, s.cG + c * 9, true);
-                    s.cv[c * 9 + 6] = g[0]; s.cv[c * 9 + 7] = g[1]; s.cv[c * 9 + 8] = g[2];
-                } else {
-                    const int i = t - nc;
-                    double imp, hd;
-                    part3[0] += model_terms(s, npad, i, v[i], dt, &imp, &hd);
-                    d32[F_HD * npad + i] = (float)hd;
-                    dp[i] = imp;  // temporarily holds the model impulse
-                }
-            }
-            __syncthreads();
-            for (int i = tid; i < n; i += NT) {
-                double j0a = 0.0, j1a = 0.0, j2a = 0.0;
-                for (int c = 0; c < nc; ++c) {
-                    j0a += (double)s.J[(c * 3) * js + i] * s.cv[c * 9 + 6];
-                    j1a += (double)s.J[(c * 3 + 1) * js + i] * s.cv[c * 9 + 7];
-                    j2a += (double)s.J[(c * 3 + 2) * js + i] * s.cv[c * 9 + 8];
-                }
-                const double jc = dp[i] + ((j0a + j1a) + j2a);
-                const double ar = d64[V_AR * npad + i];
-                const double avv = av[i];
-                jimp[i] = jc;
-                grad[i] = ar - jc;
-                part3[0] += 0.5 * (v[i] - vs[i]) * ar;
-                if (s.dflag[I_PART * npad + i]) {
-                    const double sc = d32[F_DSC * npad + i];
-                    part3[1] += (sc * grad[i]) * (sc * grad[i]);
-                    part3[2] += (sc * avv) * (sc * avv);
-                    part3[3] += (sc * jc) * (sc * jc);
-                }
-            }
-            block_sum<NT>(part3, 4, s.red);
-            const double cost = part3[0];
-            const double gnorm = sqrt(fmax(part3[1], 0.0));
-            const double pnorm = sqrt(fmax(part3[2], 0.0));
-            const double jnorm = sqrt(fmax(part3[3], 0.0));
-            // ---- convergence (contact_solve.py:1970-2087)
-            const double opt_tol = P.opt_abs + P.opt_rel * fmax(pnorm, jnorm);
-            bool opt = gnorm <= opt_tol;
-            bool costok = false;
-            if (iteration > 0 && alpha > cost_min_alpha) {
-                const double scale = 0.5 * (fabs(cost) + fabs(prev_cost));
-                if (fabs(cost - prev_cost) < P.cost_abs + P.cost_rel * scale) costok = true;
-            }
-            if (opt || costok) {
-                flags = 1 | (opt ? 2 : 0) | (costok ? 4 : 0);
-                break;
-            }
-            if (iteration >= P.max_iter) {
-                flags = 8;
-                break;
-            }
-            // ---- Hessian H = A + J^T G J + diag(hd) with MFMA (contact_solve.py:1334-1654)
-            {
-                constexpr int NWV = NT / 64;
-                const int lane = tid & 63, wave = tid >> 6;
-                const int tb = npad >> 4;
-                const int ntile = tb * (tb + 1) / 2;
-                const int K = 3 * nc;
-                for (int tt = wave; tt < ntile; tt += NWV) {
-                    int ib = 0, rem = tt;
-                    while (rem > ib) { rem -= ib + 1; ++ib; }
-                    const int jb = rem;
-                    const int i0 = ib * 16, j0 = jb * 16;
-                    typedef float f4 __attribute__((ext_vector_type(4)));
-                    f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-                    const int ia = i0 + (lane & 15);
-                    const int jbb = j0 + (lane & 15);
-                    for (int k0 = 0; k0 < K; k0 += 4) {
-                        const int k = k0 + (lane >> 4);
-                        float a = 0.0f, b = 0.0f;
-                        if (k < K) {
-                            const int c = k / 3, r = k % 3;
-                            if (ia < n) a = s.J[k * js + ia];
-                            if (jbb < n) {
-                                const float* G = s.cG + c * 9 + r * 3;
-                                const float* Jc = s.J + (c * 3) * js + jbb;
-                                b = G[0] * Jc[0] + G[1] * Jc[js] + G[2] * Jc[2 * js];
-                            }
-                        }
-                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    double alpha = 1.0, prev_cost = 0.0, cost = 0.0;
+    const double cost_min_alpha = 0.5;
+    while (true) {
+        if (wave == 0) {
+            bool stop = false;
+            if (unconstrained) {
+                flags = 1;
+                stop = true;
+            } else {
+                // ---- evaluate at v: vc = J v and A v, projection + model terms, then
+                //      impulses / gradient / norms (contact_solve.py:743-921, 1493-1627)
+                for (int t = lane; t < 3 * nc + n; t += 64) {
+                    if (t < 3 * nc) {
+                        s.cv[(t / 3) * 9 + (t % 3)] = dot_fd(s.J + t * js, v, n);
+                    } else {
+                        const int i = t - 3 * nc;
+                        const double avi = dot_fd(s.A + i * ast, v, n);
+                        av[i] = avi;
+                        d64[V_AR * npad + i] = avi - d64[V_AVS * npad + i];
                     }
+                }
+                wave_sync();
+                double part3[4] = {0.0, 0.0, 0.0, 0.0};  // cost, |D g|^2, |D A v|^2, |D j|^2
+                for (int t = lane; t < nc + n; t += 64) {
+                    if (t < nc) {
+                        const int c = t;
+                        double vc[3] = {s.cv[c * 9], s.cv[c * 9 + 1], s.cv[c * 9 + 2]};
+                        double g[3];
+                        part3[0] += project_contact(s.ck + c * kCk, vc, g, s.cG + c * 9, true);
+                        s.cv[c * 9 + 6] = g[0]; s.cv[c * 9 + 7] = g[1]; s.cv[c * 9 + 8] = g[2];
+                    } else {
+                        const int i = t - nc;
+                        double imp, hd;
+                        part3[0] += model_terms(s, npad, i, v[i], dt, &imp, &hd);
+                        d32[F_HD * npad + i] = (float)hd;
+                        dp[i] = imp;  // temporarily holds the model impulse
+                    }
+                }
+                wave_sync();
+                for (int i = lane; i < n; i += 64) {
+                    double j0a = 0.0, j1a = 0.0, j2a = 0.0;
+                    for (int c = 0; c < nc; ++c) {
+                        j0a += (double)s.J[(c * 3) * js + i] * s.cv[c * 9 + 6];
+                        j1a += (double)s.J[(c * 3 + 1) * js + i] * s.cv[c * 9 + 7];
+                        j2a += (double)s.J[(c * 3 + 2) * js + i] * s.cv[c * 9 + 8];
+                    }
+                    const double jc = dp[i] + ((j0a + j1a) + j2a);
+                    const double ar = d64[V_AR * npad + i];
+                    const double avv = av[i];
+                    jimp[i] = jc;
+                    grad[i] = ar - jc;
+                    part3[0] += 0.5 * (v[i] - vs[i]) * ar;
+                    if (s.dflag[I_PART * npad + i]) {
+                        const double sc = d32[F_DSC * npad + i];
+                        part3[1] += (sc * grad[i]) * (sc * grad[i]);
+                        part3[2] += (sc * avv) * (sc * avv);
+                        part3[3] += (sc * jc) * (sc * jc);
+                    }
+                }
+                wave_sync();
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int i = i0 + 4 * (lane >> 4) + e;
-                        const int j = j0 + (lane & 15);
-                        float val;
-                        if (i < n && j < n) {
-                            val = s.A[i * npad + j] + acc[e];
-                            if (i == j) val += d32[F_HD * npad + i] + (float)P.diag_shift;
-                        } else {
-                            val = (i == j) ? 1.0f : 0.0f;
-                        }
-                        s.H[i * hs + j] = val;
-                    }
+                for (int k = 0; k < 4; ++k) part3[k] = wave_sum(part3[k]);
+                cost = part3[0];
+                const double gnorm = sqrt(fmax(part3[1], 0.0));
+                const double pnorm = sqrt(fmax(part3[2], 0.0));
+                const double jnorm = sqrt(fmax(part3[3], 0.0));
+                // ---- convergence (contact_solve.py:1970-2087)
+                const double opt_tol = P.opt_abs + P.opt_rel * fmax(pnorm, jnorm);
+                const bool opt = gnorm <= opt_tol;
+                bool costok = false;
+                if (iteration > 0 && alpha > cost_min_alpha) {
+                    const double scale = 0.5 * (fabs(cost) + fabs(prev_cost));
+                    if (fabs(cost - prev_cost) < P.cost_abs + P.cost_rel * scale) costok = true;
+                }
+                if (opt || costok) {
+                    flags = 1 | (opt ? 2 : 0) | (costok ? 4 : 0);
+                    stop = true;
+                } else if (iteration >= P.max_iter) {
+                    flags = 8;
+                    stop = true;
                 }
             }
-            __syncthreads();
-            cholesky_blocked<NT>(s.H, npad, hs, n, d32 + F_DINV * npad);
-            for (int i = tid; i < npad; i += NT) dvv[i] = (i < n) ? -grad[i] : 0.0;
-            __syncthreads();
-            chol_solve_wave0(s.H, hs, n, d32 + F_DINV * npad, dvv);
-            __syncthreads();
+            if (lane == 0) ctl[0] = stop ? 0 : 1;
+        }
+        __syncthreads();
+        if (ctl[0] == 0) break;
+        // ---- Hessian H = A + J^T G J + diag(hd) with MFMA (contact_solve.py:1334-1654)
+        {
+            constexpr int NWV = NT / 64;
+            const int hl = tid & 63, hw = tid >> 6;
+            const int tb = npad >> 4;
+            const int ntile = tb * (tb + 1) / 2;
+            const int K = 3 * nc;
+            for (int tt = hw; tt < ntile; tt += NWV) {
+                int ib = 0, rem = tt;
+                while (rem > ib) { rem -= ib + 1; ++ib; }
+                const int jb = rem;
+                const int i0 = ib * 16, j0 = jb * 16;
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                const int ia = i0 + (hl & 15);
+                const int jbb = j0 + (hl & 15);
+                for (int k0 = 0; k0 < K; k0 += 4) {
+                    const int k = k0 + (hl >> 4);
+                    float a = 0.0f, b = 0.0f;
+                    if (k < K) {
+                        const int c = k / 3, r = k % 3;
+                        if (ia < n) a = s.J[k * js + ia];
+                        if (jbb < n) {
+                            const float* G = s.cG + c * 9 + r * 3;
+                            const float* Jc = s.J + (c * 3) * js + jbb;
+                            b = G[0] * Jc[0] + G[1] * Jc[js] + G[2] * Jc[2 * js];
+                        }
+                    }
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int i = i0 + 4 * (hl >> 4) + e;
+                    const int j = j0 + (hl & 15);
+                    float val;
+                    if (i < n && j < n) {
+                        val = s.A[i * ast + j] + acc[e];
+                        if (i == j) val += d32[F_HD * npad + i] + (float)P.diag_shift;
+                    } else {
+                        val = (i == j) ? 1.0f : 0.0f;
+                    }
+                    s.H[i * hs + j] = val;
+                }
+            }
+        }
+        __syncthreads();
+        cholesky_blocked<NT>(s.H, npad, hs, n, d32 + F_DINV * npad);
+        chol_block_inverses<NT>(s.H, npad, hs, d32 + F_DINV * npad);
+        if (wave == 0) {
+            for (int i = lane; i < npad; i += 64) dvv[i] = (i < n) ? -grad[i] : 0.0;
+            wave_sync();
+            chol_solve_blocks(s.H, npad, hs, n, d32 + F_DINV * npad, dvv);
+            wave_sync();
             // ---- mixed-precision refinement of the Newton direction: residual
             //      r = -grad - (A + J^T G J + diag(hd + shift)) dv in float64 from the stored
             //      float32 operands, correction solved with the float32 factor
             for (int rep = 0; rep < kNewtonRefine; ++rep) {
-                for (int t = tid; t < 3 * nc + n; t += NT) {
+                for (int t = lane; t < 3 * nc + n; t += 64) {
                     if (t < 3 * nc) {
                         s.cv[(t / 3) * 9 + 3 + (t % 3)] = dot_fd(s.J + t * js, dvv, n);
                     } else {
                         const int i = t - 3 * nc;
-                        dp[i] = dot_fd(s.A + i * npad, dvv, n);
+                        dp[i] = dot_fd(s.A + i * ast, dvv, n);
                     }
                 }
-                __syncthreads();
-                for (int i = tid; i < npad; i += NT) {
+                wave_sync();
+                for (int i = lane; i < npad; i += 64) {
                     double r = 0.0;
                     if (i < n) {
                         double acc = dp[i] + ((double)d32[F_HD * npad + i] + P.diag_shift) * dvv[i];
@@ -1227,20 +1483,20 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
                     }
                     d64[V_T1 * npad + i] = r;
                 }
-                __syncthreads();
-                chol_solve_wave0(s.H, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
-                __syncthreads();
-                for (int i = tid; i < n; i += NT) dvv[i] += d64[V_T1 * npad + i];
-                __syncthreads();
+                wave_sync();
+                chol_solve_blocks(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
+                wave_sync();
+                for (int i = lane; i < n; i += 64) dvv[i] += d64[V_T1 * npad + i];
+                wave_sync();
             }
             // ---- line search data: dvc = J dv and dp = A dv in one stage (contact_solve.py:2234-2263)
             double ld[4] = {0, 0, 0, 0};  // dell0, dell_a0, d2ell_a, m0
-            for (int t = tid; t < 3 * nc + n; t += NT) {
+            for (int t = lane; t < 3 * nc + n; t += 64) {
                 if (t < 3 * nc) {
                     s.cv[(t / 3) * 9 + 3 + (t % 3)] = dot_fd(s.J + t * js, dvv, n);
                 } else {
                     const int i = t - 3 * nc;
-                    const double acc = dot_fd(s.A + i * npad, dvv, n);
+                    const double acc = dot_fd(s.A + i * ast, dvv, n);
                     dp[i] = acc;
                     const double ri = v[i] - vs[i];
                     ld[0] += grad[i] * dvv[i];
@@ -1249,33 +1505,36 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
                     ld[3] += 0.5 * ri * d64[V_AR * npad + i];
                 }
             }
-            block_sum<NT>(ld, 4, s.red);
+            wave_sync();
+#pragma unroll
+            for (int k = 0; k < 4; ++k) ld[k] = wave_sum(ld[k]);
             const double dell0 = ld[0], dell_a0 = ld[1], d2ell_a = ld[2], m0 = ld[3];
             // trial evaluation of phi(a), phi'(a), phi''(a)
             auto trial = [&](double a, bool second, double* d1, double* d2) -> double {
                 double tv[3] = {0.0, 0.0, 0.0};
-                for (int c = tid; c < nc; c += NT) {
+                for (int c = lane; c < nc; c += 64) {
                     double vc[3], dvc[3], g[3], cc;
                     for (int r = 0; r < 3; ++r) {
                         dvc[r] = s.cv[c * 9 + 3 + r];
                         vc[r] = s.cv[c * 9 + r] + a * dvc[r];
                     }
                     if (second) {
-                        tv[2] += project_curvature(s.ck + c * 4, vc, dvc, g, &cc);
+                        tv[2] += project_curvature(s.ck + c * kCk, vc, dvc, g, &cc);
                     } else {
                         float Gd[9];
-                        cc = project_contact(s.ck + c * 4, vc, g, Gd, false);
+                        cc = project_contact(s.ck + c * kCk, vc, g, Gd, false);
                     }
                     tv[0] += cc;
                     tv[1] -= dvc[0] * g[0] + dvc[1] * g[1] + dvc[2] * g[2];
                 }
-                for (int i = tid; i < n; i += NT) {
+                for (int i = lane; i < n; i += 64) {
                     double imp, hd;
                     tv[0] += model_terms(s, npad, i, v[i] + a * dvv[i], dt, &imp, &hd);
                     tv[1] -= dvv[i] * imp;
                     tv[2] += hd * dvv[i] * dvv[i];
                 }
-                block_sum<NT>(tv, 3, s.red);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) tv[k] = wave_sum(tv[k]);
                 *d1 = dell_a0 + a * d2ell_a + tv[1];
                 *d2 = d2ell_a + tv[2];
                 return m0 + a * dell_a0 + 0.5 * a * a * d2ell_a + tv[0];
@@ -1391,9 +1650,9 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
                 alpha = a_acc;
                 prev_cost = cost;
                 ls_total += ls_it;
-                for (int i = tid; i < n; i += NT) v[i] += alpha * dvv[i];
+                for (int i = lane; i < n; i += 64) v[i] += alpha * dvv[i];
             }
-            __syncthreads();
+            wave_sync();
             ++iteration;
         }
     }
@@ -1405,8 +1664,8 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
     for (int k = tid; k < M.NQ; k += NT) q_out[k] = q_in[k];
     __syncthreads();
     for (int j = tid; j < nj; j += NT) {
-        const int jt = M.t_jtype[j];
-        const int qs = M.t_jqs[j], ds = M.t_jds[j], nd = M.t_jds[j + 1] - M.t_jds[j];
+        const int jt = T.jtype[j];
+        const int qs = T.jqs[j], ds = T.jds[j], nd = T.jds[j + 1] - T.jds[j];
         if (jt == kJointFree || jt == kJointDistance) {
             Q4 q = Q4{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]};
             V3 wn = v3((float)v[ds], (float)v[ds + 1], (float)v[ds + 2]);
@@ -1445,8 +1704,8 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
             q_out[qs + 1] = q_in[qs + 1] + h * vmid.y;
             q_out[qs + 2] = q_in[qs + 2] + h * vmid.z;
             q_out[qs + 3] = qn.x; q_out[qs + 4] = qn.y; q_out[qs + 5] = qn.z; q_out[qs + 6] = qn.w;
-            const size_t gb = (size_t)w * nb + M.t_jchild[j];
-            const float* com = mCom + M.t_jchild[j] * 4;
+            const size_t gb = (size_t)w * nb + T.jchild[j];
+            const float* com = mCom + T.jchild[j] * 4;
             V3 rc = qrot(qnormalize(qn), v3(com[0], com[1], com[2]));
             V3 vcom = von + cross(wn, rc);
             qd_pub[ds] = vcom.x; qd_pub[ds + 1] = vcom.y; qd_pub[ds + 2] = vcom.z;
@@ -1463,55 +1722,69 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         }
     }
     __syncthreads();
-    // public forward kinematics (sap_kinematics.py:84-229)
+    // public forward kinematics (sap_kinematics.py:84-229): joint-local transform
+    // X_j X_c^-1 and joint velocity in parallel, then the wave-0 tree walk
     float* pbq = s.bq;     // reuse: public poses
     float* pbqd = kV;      // reuse: public velocities (nb x 6, [v, w])
-    for (int L = 0; L < M.num_levels; ++L) {
-        const int l0 = M.t_level_start[L], l1 = M.t_level_start[L + 1];
-        for (int t = l0 + tid; t < l1; t += NT) {
-            const int j = M.t_level_joint[t];
-            const size_t gj = (size_t)w * nj + j;
-            const int jt = M.t_jtype[j];
-            const int parent = M.t_jparent[j], child = M.t_jchild[j];
-            const int qs = M.t_jqs[j], ds = M.t_jds[j];
-            TF Xp = load_tf(mXp + j * 8);
-            TF Xc = load_tf(mXc + j * 8);
-            TF Xwpj = Xp;
-            V3 vlin = v3(0, 0, 0), vang = v3(0, 0, 0);
-            if (parent >= 0) {
-                TF Xwp = load_tf(pbq + parent * 8);
-                Xwpj = tf_mul(Xwp, Xp);
-                const float* com = mCom + parent * 4;
-                V3 rp = Xwpj.p - (Xwp.p + qrot(Xwp.q, v3(com[0], com[1], com[2])));
-                const float* vp = pbqd + parent * 6;
-                vang = v3(vp[3], vp[4], vp[5]);
-                vlin = v3(vp[0], vp[1], vp[2]) + cross(vang, rp);
-            }
-            TF Xj = TF{v3(0, 0, 0), Q4{0, 0, 0, 1}};
-            V3 jl = v3(0, 0, 0), ja = v3(0, 0, 0);
-            const float* axp = mAx + ds * 4;
-            V3 axis = v3(axp[0], axp[1], axp[2]);
-            if (jt == kJointPrismatic) {
-                Xj.p = axis * q_out[qs];
-                jl = axis * qd_pub[ds];
-            } else if (jt == kJointRevolute) {
-                Xj.q = q_axis_angle(axis, q_out[qs]);
-                ja = axis * qd_pub[ds];
-            } else if (jt == kJointFree || jt == kJointDistance) {
-                Xj = TF{v3(q_out[qs], q_out[qs + 1], q_out[qs + 2]), Q4{q_out[qs + 3], q_out[qs + 4], q_out[qs + 5], q_out[qs + 6]}};
-                jl = v3(qd_pub[ds], qd_pub[ds + 1], qd_pub[ds + 2]);
-                ja = v3(qd_pub[ds + 3], qd_pub[ds + 4], qd_pub[ds + 5]);
-            }
-            TF Xwc = tf_mul(tf_mul(Xwpj, Xj), tf_inverse(Xc));
-            float* o = pbq + child * 8;
-            o[0] = Xwc.p.x; o[1] = Xwc.p.y; o[2] = Xwc.p.z; o[3] = Xwc.q.x; o[4] = Xwc.q.y; o[5] = Xwc.q.z; o[6] = Xwc.q.w;
-            V3 lv = vlin + qrot(Xwpj.q, jl);
-            V3 av3 = vang + qrot(Xwpj.q, ja);
-            float* ov = pbqd + child * 6;
-            ov[0] = lv.x; ov[1] = lv.y; ov[2] = lv.z; ov[3] = av3.x; ov[4] = av3.y; ov[5] = av3.z;
+    float* kPl = reinterpret_cast<float*>(kXl);  // nj x 16: X_j X_c^-1 (7), joint lin (3), ang (3)
+    for (int j = tid; j < nj; j += NT) {
+        const int jt = T.jtype[j];
+        const int qs = T.jqs[j], ds = T.jds[j];
+        TF Xj = TF{v3(0, 0, 0), Q4{0, 0, 0, 1}};
+        V3 jl = v3(0, 0, 0), ja = v3(0, 0, 0);
+        const float* axp = mAx + ds * 4;
+        const V3 axis = v3(axp[0], axp[1], axp[2]);
+        if (jt == kJointPrismatic) {
+            Xj.p = axis * q_out[qs];
+            jl = axis * qd_pub[ds];
+        } else if (jt == kJointRevolute) {
+            Xj.q = q_axis_angle(axis, q_out[qs]);
+            ja = axis * qd_pub[ds];
+        } else if (jt == kJointFree || jt == kJointDistance) {
+            Xj = TF{v3(q_out[qs], q_out[qs + 1], q_out[qs + 2]), Q4{q_out[qs + 3], q_out[qs + 4], q_out[qs + 5], q_out[qs + 6]}};
+            jl = v3(qd_pub[ds], qd_pub[ds + 1], qd_pub[ds + 2]);
+            ja = v3(qd_pub[ds + 3], qd_pub[ds + 4], qd_pub[ds + 5]);
         }
-        __syncthreads();
+        const TF Xjc = tf_mul(Xj, tf_inverse(load_tf(mXc + j * 8)));
+        float* o = kPl + j * 16;
+        store_tf(o, Xjc);
+        o[7] = jl.x; o[8] = jl.y; o[9] = jl.z; o[10] = ja.x; o[11] = ja.y; o[12] = ja.z;
     }
+    __syncthreads();
+    if (tid < 64) {
+        const int lane = tid;
+        for (int L = 0; L < M.num_levels; ++L) {
+            const int l0 = T.level_start[L], l1 = T.level_start[L + 1];
+            for (int t = l0 + lane; t < l1; t += 64) {
+                const int j = T.level_joint[t];
+                const int parent = T.jparent[j], child = T.jchild[j];
+                const TF Xp = load_tf(mXp + j * 8);
+                TF Xwpj = Xp;
+                V3 vlin = v3(0, 0, 0), vang = v3(0, 0, 0);
+                if (parent >= 0) {
+                    const TF Xwp = load_tf(pbq + parent * 8);
+                    Xwpj = tf_mul(Xwp, Xp);
+                    const float* com = mCom + parent * 4;
+                    const V3 rp = Xwpj.p - (Xwp.p + qrot(Xwp.q, v3(com[0], com[1], com[2])));
+                    const float* vp = pbqd + parent * 6;
+                    vang = v3(vp[3], vp[4], vp[5]);
+                    vlin = v3(vp[0], vp[1], vp[2]) + cross(vang, rp);
+                }
+                const float* pl = kPl + j * 16;
+                const TF Xwc = tf_mul(Xwpj, load_tf(pl));
+                float* o = pbq + child * 8;
+                o[0] = Xwc.p.x; o[1] = Xwc.p.y; o[2] = Xwc.p.z; o[3] = Xwc.q.x; o[4] = Xwc.q.y; o[5] = Xwc.q.z; o[6] = Xwc.q.w;
+                const V3 lv = vlin + qrot(Xwpj.q, v3(pl[7], pl[8], pl[9]));
+                const V3 av3 = vang + qrot(Xwpj.q, v3(pl[10], pl[11], pl[12]));
+                float* ov = pbqd + child * 6;
+                ov[0] = lv.x; ov[1] = lv.y; ov[2] = lv.z; ov[3] = av3.x; ov[4] = av3.y; ov[5] = av3.z;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    __syncthreads();
     // ------------------------------------------------------------------ outputs
     for (int k = tid; k < M.NQ; k += NT) P.jq_out[(size_t)w * M.NQ + k] = q_out[k];
     for (int d = tid; d < n; d += NT) {
@@ -1521,7 +1794,7 @@ __global__ __launch_bounds__(NT) void sapx_step_kernel(StepParams P) {
         if (P.dbg_v) P.dbg_v[g] = v[d];
         if (P.dbg_vstar) P.dbg_vstar[g] = vs[d];
         if (P.dbg_v0) P.dbg_v0[g] = d64[V_V0 * npad + d];
-        if (P.dbg_adiag) P.dbg_adiag[g] = s.A[d * npad + d];
+        if (P.dbg_adiag) P.dbg_adiag[g] = s.A[d * ast + d];
     }
     for (int t = tid; t < nb * 7; t += NT) {
         const int b = t / 7, k = t % 7;
