@@ -33,6 +33,10 @@ UNITS = {
     "sapx_collide.hip": ["-ffp-contract=off"],
     "sapx_group.hip": [],
     "sapx_api.hip": [],
+    "sapx_wave16.hip": [],
+    "sapx_wave32.hip": [],
+    "sapx_wave48.hip": [],
+    "sapx_wave64.hip": [],
 }
 
 

@@ -17,6 +17,10 @@
 
 namespace sapx {
 hipError_t launch_step(const StepParams& P, hipStream_t stream);
+hipError_t launch_wave_np16(const StepParams& P, int cb, hipStream_t stream);
+hipError_t launch_wave_np32(const StepParams& P, int cb, hipStream_t stream);
+hipError_t launch_wave_np48(const StepParams& P, int cb, hipStream_t stream);
+hipError_t launch_wave_np64(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_collide(const CollideParams& P, const CompactParams& Q, hipStream_t stream);
 hipError_t launch_group_contacts(const GroupParams& G, hipStream_t stream);
 }  // namespace sapx
@@ -78,6 +82,10 @@ struct SapxSolver {
     int npad = 0, js = 0, ast = 0;
     int lds[17] = {0};
     int smem_bytes = 0;
+    // one-wave-per-world kernel: contact variants (capacity 16, and 64 for the rest)
+    bool wave = false;
+    WaveLayout wl_small{}, wl_big{};
+    int cb_small = 0, cb_big = 0;
     // grouping buffers for ungrouped contact lists
     int group_capacity = 0;
     int* g_keys = nullptr;
@@ -446,6 +454,49 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
         delete s;
         return fail(SAPX_ERR_CAPACITY, "per-world LDS footprint exceeds 160 KiB; lower max_rigid_contact");
     }
+    {
+        // one wave per world: LDS layout per contact variant (see sapx_wave_impl.h)
+        const int NP = npad;
+        auto layout = [&](int CB) {
+            WaveLayout L{};
+            size_t o = 0;
+            auto tk = [&](size_t bytes) {
+                size_t r = o;
+                o = round_up(o + bytes, 16);
+                return (int)r;
+            };
+            L.js = (n % 2) ? n : n + 1;
+            L.ls = NP + 1;
+            L.o_topo = tk((size_t)dm.topo_ints * 4);
+            L.o_mp = tk((size_t)(dm.J * 16 + n * 4 + nb * 4 + nb * 9 + nb) * 4);
+            L.o_q = tk((size_t)2 * dm.NQ * 4);
+            L.o_bq = tk((size_t)nb * 8 * 4);
+            L.o_bqd = tk((size_t)nb * 8 * 8);
+            L.o_S = tk((size_t)n * 6 * 4);
+            L.o_vec = tk((size_t)128 * 8);
+            L.o_ck = tk((size_t)CB * 8 * 8);
+            L.o_cG = tk((size_t)CB * 9 * 4);
+            L.o_A = tk((size_t)n * L.ls * 4);
+            // J rows then the factor rows; the kinematics scratch overlays both
+            const size_t jbytes = round_up((size_t)3 * CB * L.js * 4, 16);
+            const size_t ltbytes = ((size_t)NP * L.ls + 64) * 4;
+            const size_t kin = (size_t)(50 * nb + 14 * n + 2) * 4 + (size_t)dm.J * 64;
+            L.o_J = (int)o;
+            L.o_Lt = (int)(o + jbytes);
+            o = round_up(o + std::max(jbytes + ltbytes, kin), 16);
+            L.bytes = (int)o;
+            return L;
+        };
+        const char* env = getenv("SAPX_STEP_KERNEL");
+        const bool want_wave = !(env && std::string(env) == "fused");
+        if (want_wave && NP <= 64 && C <= 64) {
+            s->cb_small = 16;
+            s->wl_small = layout(16);
+            s->cb_big = C > 16 ? 64 : 0;
+            if (s->cb_big) s->wl_big = layout(64);
+            s->wave = s->wl_small.bytes <= 160 * 1024 && (!s->cb_big || s->wl_big.bytes <= 160 * 1024);
+        }
+    }
     const size_t W = dm.W, D = (size_t)dm.W * n;
     if (hipMalloc(&s->v_guess, D * sizeof(double)) != hipSuccess || hipMalloc(&s->st, 4 * W * sizeof(int)) != hipSuccess) {
         delete s;
@@ -621,7 +672,26 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
     P.o_bqd = s->lds[15];
     P.o_topo = s->lds[16];
     P.smem_bytes = s->smem_bytes;
-    HIP_TRY(launch_step(P, st));
+    if (s->wave) {
+        auto launch = [&](int cb) {
+            switch (s->npad) {
+                case 16: return launch_wave_np16(P, cb, st);
+                case 32: return launch_wave_np32(P, cb, st);
+                case 48: return launch_wave_np48(P, cb, st);
+                default: return launch_wave_np64(P, cb, st);
+            }
+        };
+        P.wl = s->wl_small;
+        P.wave_min_nc = 0;
+        HIP_TRY(launch(s->cb_small));
+        if (s->cb_big) {
+            P.wl = s->wl_big;
+            P.wave_min_nc = s->cb_small + 1;
+            HIP_TRY(launch(s->cb_big));
+        }
+    } else {
+        HIP_TRY(launch_step(P, st));
+    }
     s->use_guess = 1;
     return SAPX_OK;
 }

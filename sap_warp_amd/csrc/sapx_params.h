@@ -5,6 +5,14 @@
 
 namespace sapx {
 
+// LDS layout of the one-wave-per-world step kernel (byte offsets); js / ls are
+// the odd row strides of the contact Jacobian and of the factor rows.
+struct WaveLayout {
+    int o_topo, o_mp, o_q, o_bq, o_bqd, o_S, o_vec, o_ck, o_cG, o_A, o_J, o_Lt;
+    int bytes;
+    int js, ls;
+};
+
 struct StepParams {
     DevModel m;
     const float* jq_in;
@@ -55,6 +63,8 @@ struct StepParams {
     // LDS byte offsets
     int o_A, o_H, o_J, o_bq, o_S, o_ck, o_cv, o_cG, o_cb, o_dv64, o_dv32, o_dflag, o_red, o_q, o_misc, o_bqd, o_topo;
     int smem_bytes;
+    WaveLayout wl;       // one-wave kernel layout for this launch's contact variant
+    int wave_min_nc;     // the variant processes worlds with wave_min_nc <= nc <= its capacity
 };
 
 struct CollideParams {

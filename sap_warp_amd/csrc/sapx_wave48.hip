@@ -1,0 +1,10 @@
+// One-wave-per-world step kernel instantiated for 48 padded dofs (see sapx_wave_impl.h).
+#include "sapx_wave_impl.h"
+
+namespace sapx {
+hipError_t launch_wave_np48(const StepParams& P, int cb, hipStream_t stream) {
+    if (cb == 16) return wv::launch_variant<48, 16>(P, stream);
+    if (cb == 64) return wv::launch_variant<48, 64>(P, stream);
+    return hipErrorInvalidValue;
+}
+}  // namespace sapx

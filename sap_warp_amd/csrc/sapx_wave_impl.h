@@ -1,0 +1,1285 @@
+// One-wave-per-world SAP timestep (gfx950).
+//
+// Each world is one 64-lane wavefront (one workgroup): lane i owns dof i (its
+// row of the dynamics matrix A and of the Hessian / Cholesky factor live in
+// VGPRs), lane c owns contact c.  Cross-lane data moves by v_readlane
+// broadcasts and DPP reductions, so no phase needs a workgroup barrier and
+// LDS only holds the contact Jacobian, the kinematics scratch, poses and the
+// transposed factor (≈ 30 KB at 35 dofs / 16 contacts: five worlds per CU,
+// every wave busy).  The algorithm and the reference citations are those of
+// the fused step kernel (sapx_step.hip); this kernel re-maps the same phases
+// onto a single wave:
+//   0/1 loads + public -> SAP boundary      sap_helpers.py:1482-1522, 1811-1848, 2058-2072
+//   2   forward kinematics (float64 poses)  free_motion.py:412-530
+//   3   body dynamics                       free_motion.py:680-719
+//   4   composites, tau, A (CRBA rows)      free_motion.py:93-147, 843-1179
+//   5   v* = v0 + dt A^-1 tau               free_motion.py:1970-2178
+//   6   contacts, J, weights, constants     contact_jacobian.py:153-562, contact_solve.py:791-822
+//   7   PD / limits / participation         contact_solve.py:405-741
+//   8   Newton loop with line search        contact_solve.py:743-3176, 6129-6285
+//   9   integration + public FK             solver_sap.py:288-519, sap_kinematics.py:84-229
+// NP = dofs rounded up to 16 (<= 64), CB = contact slots of this variant.
+#ifndef SAPX_WAVE_IMPL_H
+#define SAPX_WAVE_IMPL_H
+
+#include "sapx_kcommon.h"
+
+namespace sapx {
+namespace wv {
+
+// ---------------------------------------------------------------------------
+// register-row dense kernels (lane i owns row i; rows / columns >= n are the
+// identity in H and zero in A; lanes >= NP hold zero rows)
+// ---------------------------------------------------------------------------
+
+// sum_j A[row][j] x_j for the LDS row of this lane, x_j broadcast from lane j
+__device__ __forceinline__ double lds_row_dot(const float* arow, double x, int n) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+        s0 += (double)arow[j] * readlane_d(x, j);
+        s1 += (double)arow[j + 1] * readlane_d(x, j + 1);
+        s2 += (double)arow[j + 2] * readlane_d(x, j + 2);
+        s3 += (double)arow[j + 3] * readlane_d(x, j + 3);
+    }
+    for (; j < n; ++j) s0 += (double)arow[j] * readlane_d(x, j);
+    return (s0 + s1) + (s2 + s3);
+}
+
+// Right-looking Cholesky of the register rows: h becomes L (row `lane`, lower,
+// strictly upper part zero), dinv = 1 / L_ii.
+template <int NP>
+__device__ __forceinline__ void chol_rows(float (&h)[NP], float& dinv, int lane) {
+    dinv = 1.0f;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const float pj = fmaxf(rl(h[j], j), 1.0e-30f);
+        const float dj = __builtin_amdgcn_rsqf(pj);
+        float lij = h[j] * dj;
+        if (lane < j) lij = 0.0f;
+        h[j] = lij;
+        if (lane == j) dinv = dj;
+#pragma unroll
+        for (int k = j + 1; k < NP; ++k) h[k] = fmaf(-lij, rl(lij, k), h[k]);
+    }
+}
+
+// forward substitution L y = b with the register rows
+template <int NP>
+__device__ __forceinline__ float fwd_rows(const float (&h)[NP], float dinv, float b, int lane) {
+    float y = b;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+        const float yj = rl(y, j) * rl(dinv, j);
+        y = (lane == j) ? yj : fmaf(-h[j], yj, y);
+    }
+    return y;
+}
+
+// back substitution L^T x = y with the factor's rows stored in LDS (row j at
+// Lt + j * ls); lane i reads column i in 16-row blocks ahead of the chain
+template <int NP>
+__device__ __forceinline__ float bwd_lds(const float* Lt, int ls, float dinv, float y, int lane) {
+#pragma unroll
+    for (int jb = NP - 16; jb >= 0; jb -= 16) {
+        float Lc[16];
+#pragma unroll
+        for (int t = 0; t < 16; ++t) Lc[t] = Lt[(jb + t) * ls + lane];
+#pragma unroll
+        for (int t = 15; t >= 0; --t) {
+            const int j = jb + t;
+            const float xj = rl(y, j) * rl(dinv, j);
+            y = (lane == j) ? xj : fmaf(-Lc[t], xj, y);
+        }
+    }
+    return y;
+}
+
+template <int NP>
+__device__ __forceinline__ void store_rows(const float (&h)[NP], float* Lt, int ls, int lane) {
+    if (lane < NP) {
+#pragma unroll
+        for (int k = 0; k < NP; ++k) Lt[lane * ls + k] = h[k];
+    }
+}
+
+// L L^T x = b (float32 factor), returns x on each lane
+template <int NP>
+__device__ __forceinline__ float chol_solve(const float (&h)[NP], const float* Lt, int ls, float dinv, float b, int lane) {
+    const float y = fwd_rows<NP>(h, dinv, b, lane);
+    wave_sync();
+    return bwd_lds<NP>(Lt, ls, dinv, y, lane);
+}
+
+// ---------------------------------------------------------------------------
+// per-dof PD / limit terms held in registers (contact_solve.py:1569-1612)
+// ---------------------------------------------------------------------------
+struct DofTerms {
+    int pd, lo, up;
+    float pda, pdg, pdl, lov, upv, lor, upr;
+};
+
+__device__ __forceinline__ double dof_terms(const DofTerms& m, double vi, double dt, double* imp, double* hd) {
+    double cost = 0.0;
+    *imp = 0.0;
+    *hd = 0.0;
+    if (m.pd) {
+        const double gain = m.pdg;
+        const double lim = m.pdl;
+        const double y = (double)m.pda - gain * vi;
+        *imp += dt * clamp_s(y, lim);
+        *hd += dt * gain * clamp_d(y, lim);
+        if (gain > 0.0) cost += (dt / gain) * clamp_a(y, lim);
+    }
+    if (m.lo) {
+        const double r = m.lor;
+        const double g = (1.0 / r) * ((double)m.lov - vi);
+        if (g > 0.0) { *imp += g; *hd += 1.0 / r; cost += 0.5 * r * g * g; }
+    }
+    if (m.up) {
+        const double r = m.upr;
+        const double g = (1.0 / r) * ((double)m.upv + vi);
+        if (g > 0.0) { *imp -= g; *hd += 1.0 / r; cost += 0.5 * r * g * g; }
+    }
+    return cost;
+}
+
+// ---------------------------------------------------------------------------
+// the kernel
+// ---------------------------------------------------------------------------
+template <int NP, int CB>
+__global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
+    extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+    const DevModel& M = P.m;
+    const WaveLayout& L = P.wl;
+    const int w = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int n = M.N, nb = M.NB, nj = M.J;
+    const int js = L.js, ls = L.ls;
+    const int C = P.C;
+
+    // contact range of this world; the variant for fewer contacts owns the world
+    int nc = 0, c_begin = 0;
+    if (P.cworld_start) {
+        c_begin = P.cworld_start[w];
+        nc = P.cworld_start[w + 1] - c_begin;
+    }
+    if (nc > C) nc = C;
+    if (nc < 0) nc = 0;
+    if (nc > CB || nc < P.wave_min_nc) return;
+
+    const double dt = P.dt_world ? P.dt_world[w] : P.dt;
+    const float h = (float)dt;
+
+    int* topo = reinterpret_cast<int*>(smem_raw + L.o_topo);
+    float* mp = reinterpret_cast<float*>(smem_raw + L.o_mp);
+    float* q_in = reinterpret_cast<float*>(smem_raw + L.o_q);
+    float* q_out = q_in + M.NQ;
+    float* bq = reinterpret_cast<float*>(smem_raw + L.o_bq);
+    double* bqd = reinterpret_cast<double*>(smem_raw + L.o_bqd);
+    float* Sg = reinterpret_cast<float*>(smem_raw + L.o_S);
+    double* vec = reinterpret_cast<double*>(smem_raw + L.o_vec);  // 2 x 64 doubles of cross-lane data
+    double* ck = reinterpret_cast<double*>(smem_raw + L.o_ck);    // CB x kCk
+    float* cG = reinterpret_cast<float*>(smem_raw + L.o_cG);      // CB x 9
+    float* Jm = reinterpret_cast<float*>(smem_raw + L.o_J);       // 3CB x js
+    float* Lt = reinterpret_cast<float*>(smem_raw + L.o_Lt);      // NP x ls (+64)
+    float* Am = reinterpret_cast<float*>(smem_raw + L.o_A);       // n x ls dynamics matrix rows
+    const float* arow = Am + (lane < n ? lane : 0) * ls;
+    // kinematics scratch overlays the J / Lt region (phases 2-4 and 9)
+    float* kV = Jm;
+    float* kA = kV + nb * 6;
+    float* kF = kA + nb * 6;
+    float* kI = kF + nb * 6;
+    float* kIc = kI + nb * 10;
+    float* kFc = kIc + nb * 10;
+    float* kSO = kFc + nb * 6;
+    float* kFi = kSO + n * 6;
+    float* kExt = kFi + n * 6;
+    float* kQdPub = kExt + nb * 6;
+    double* kXl = reinterpret_cast<double*>(kQdPub + ((n + 1) & ~1));
+    float* mXp = mp;
+    float* mXc = mXp + nj * 8;
+    float* mAx = mXc + nj * 8;
+    float* mCom = mAx + n * 4;
+    float* mI = mCom + nb * 4;
+    float* mM = mI + nb * 9;
+
+    // ------------------------------------------------------------------ 0/1
+    for (int k = lane; k < M.topo_ints; k += 64) topo[k] = M.topo[k];
+    for (int t = lane; t < nj * 7; t += 64) {
+        const int j = t / 7, k = t % 7;
+        mXp[j * 8 + k] = M.joint_X_p[(size_t)w * nj * 7 + t];
+        mXc[j * 8 + k] = M.joint_X_c[(size_t)w * nj * 7 + t];
+    }
+    for (int t = lane; t < n * 3; t += 64) mAx[(t / 3) * 4 + t % 3] = M.joint_axis[(size_t)w * n * 3 + t];
+    for (int t = lane; t < nb * 3; t += 64) mCom[(t / 3) * 4 + t % 3] = M.body_com[(size_t)w * nb * 3 + t];
+    for (int t = lane; t < nb * 9; t += 64) mI[t] = M.body_inertia[(size_t)w * nb * 9 + t];
+    for (int t = lane; t < nb; t += 64) mM[t] = M.body_mass[(size_t)w * nb + t];
+    for (int k = lane; k < M.NQ; k += 64) q_in[k] = P.jq_in[(size_t)w * M.NQ + k];
+    wave_sync();
+    const Topo T = topo_view(topo, M);
+
+    // boundary: SAP-order v0 and joint force of dof `lane` (free joints: [w, v_O], [tau_O, f])
+    double v0 = 0.0;
+    float jf = 0.0f;
+    if (lane < n) {
+        const int i = lane;
+        const size_t g = (size_t)w * n + i;
+        const int j = T.dof_joint[i];
+        const int jt = T.jtype[j];
+        v0 = (double)P.jqd_in[g];
+        jf = P.c_jf ? P.c_jf[g] : 0.0f;
+        if (jt == kJointFree || jt == kJointDistance) {
+            const int ds = T.jds[j];
+            const int a = i - ds;
+            const int child = T.jchild[j];
+            const TF x = load_tf(P.bq_in + ((size_t)w * nb + child) * 7);
+            const float* com = mCom + child * 4;
+            const V3 r = qrot(qnormalize(x.q), v3(com[0], com[1], com[2]));
+            const size_t gd = (size_t)w * n + ds;
+            const double w0 = P.jqd_in[gd + 3], w1 = P.jqd_in[gd + 4], w2 = P.jqd_in[gd + 5];
+            const double rx = r.x, ry = r.y, rz = r.z;
+            if (a < 3) {
+                v0 = a == 0 ? w0 : (a == 1 ? w1 : w2);
+            } else {
+                const double vc = P.jqd_in[gd + a - 3];
+                const double wxr = a == 3 ? (w1 * rz - w2 * ry) : (a == 4 ? (w2 * rx - w0 * rz) : (w0 * ry - w1 * rx));
+                v0 = vc - wxr;
+            }
+            if (P.c_jf) {
+                const V3 fc = v3(P.c_jf[gd], P.c_jf[gd + 1], P.c_jf[gd + 2]);
+                const V3 tc = v3(P.c_jf[gd + 3], P.c_jf[gd + 4], P.c_jf[gd + 5]);
+                const V3 to = tc + cross(r, fc);
+                jf = a == 0 ? to.x : a == 1 ? to.y : a == 2 ? to.z : a == 3 ? fc.x : a == 4 ? fc.y : fc.z;
+            }
+        }
+    }
+    vec[lane] = v0;  // v0 by dof for the joint-parallel phases
+    for (int b = lane; b < nb; b += 64) {
+        float* e = kExt + b * 6;
+        for (int k = 0; k < 6; ++k) e[k] = 0.0f;
+        if (P.bf_in) {
+            const size_t gb = (size_t)w * nb + b;
+            const float* f = P.bf_in + gb * 6;
+            const V3 fc = v3(f[0], f[1], f[2]);
+            const V3 tc = v3(f[3], f[4], f[5]);
+            const TF x = load_tf(P.bq_in + gb * 7);
+            const float* com = mCom + b * 4;
+            const V3 c = qrot(qnormalize(x.q), v3(com[0], com[1], com[2]));
+            const V3 to = tc + cross(c, fc);
+            e[0] = -to.x; e[1] = -to.y; e[2] = -to.z;
+            e[3] = -fc.x; e[4] = -fc.y; e[5] = -fc.z;
+        }
+    }
+    wave_sync();
+
+    // ------------------------------------------------------------------ 2
+    // forward kinematics: joint-local parts in parallel, then the level walk
+    for (int j = lane; j < nj; j += 64) {
+        const int jt = T.jtype[j];
+        const int child = T.jchild[j];
+        const int qs = T.jqs[j];
+        const int ds = T.jds[j];
+        const TF Xc = load_tf(mXc + j * 8);
+        const bool xcid = tf_is_identity(mXc + j * 8);
+        const float* axp = mAx + ds * 4;
+        const V3 axis = v3(axp[0], axp[1], axp[2]);
+        const D3 axd = d3(axis.x, axis.y, axis.z);
+        DTF Xl = DTF{d3(0.0, 0.0, 0.0), DQ{0.0, 0.0, 0.0, 1.0}};
+        double plain = 0.0;
+        SV vl = sv(v3(0, 0, 0), v3(0, 0, 0));
+        SV al = vl;
+        if (jt == kJointRevolute && xcid) {
+            Xl.q = dq_axis_angle(axd, (double)q_in[qs]);
+            plain = 1.0;
+            const SV S = sv(axis, v3(0, 0, 0));
+            store_sv(Sg + ds * 6, S);
+            vl = sv_scale(S, (float)vec[ds]);
+        } else {
+            DTF Xj = Xl;
+            if (jt == kJointPrismatic) {
+                const double qv = q_in[qs];
+                Xj.p = d3(axis.x * qv, axis.y * qv, axis.z * qv);
+            } else if (jt == kJointRevolute) {
+                Xj.q = dq_axis_angle(axd, (double)q_in[qs]);
+            } else if (jt == kJointFree || jt == kJointDistance) {
+                Xj = DTF{d3(q_in[qs], q_in[qs + 1], q_in[qs + 2]), DQ{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]}};
+            }
+            Xl = xcid ? Xj : dtf_compose_n(Xj, dtf_inverse_n(dtf_from(Xc)));
+            const TF Xmb = tf_inverse_n(Xc);
+            const Q4 qj = Q4{(float)Xj.q.x, (float)Xj.q.y, (float)Xj.q.z, (float)Xj.q.w};
+            const V3 pmb = qrot(qj, Xmb.p);
+            if (jt == kJointPrismatic) {
+                const SV S = sv(v3(0, 0, 0), axis);
+                store_sv(Sg + ds * 6, S);
+                vl = sv_scale(S, (float)vec[ds]);
+            } else if (jt == kJointRevolute) {
+                const SV S = sv(axis, cross(axis, pmb));
+                store_sv(Sg + ds * 6, S);
+                const float qd = (float)vec[ds];
+                const V3 wf = axis * qd;
+                al = sv(v3(0, 0, 0), cross(wf, cross(wf, pmb)));
+                vl = sv_scale(S, qd);
+            } else if (jt == kJointFree || jt == kJointDistance) {
+                const V3 wv = v3((float)vec[ds], (float)vec[ds + 1], (float)vec[ds + 2]);
+                const V3 vv = v3((float)vec[ds + 3], (float)vec[ds + 4], (float)vec[ds + 5]);
+                vl = sv(wv, vv + cross(wv, pmb));
+                al = sv(v3(0, 0, 0), cross(wv, cross(wv, pmb)));
+                for (int k = 0; k < 3; ++k) {
+                    const V3 e = v3(k == 0, k == 1, k == 2);
+                    store_sv(Sg + (ds + k) * 6, sv(e, cross(e, pmb)));
+                    store_sv(Sg + (ds + 3 + k) * 6, sv(v3(0, 0, 0), e));
+                }
+            }
+        }
+        store_dtf(kXl + j * 8, Xl);
+        kXl[j * 8 + 7] = plain;
+        store_sv(kV + child * 6, vl);
+        store_sv(kA + child * 6, al);
+    }
+    wave_sync();
+    for (int Lv = 0; Lv < M.num_levels; ++Lv) {
+        const int l0 = T.level_start[Lv], l1 = T.level_start[Lv + 1];
+        for (int t = l0 + lane; t < l1; t += 64) {
+            const int j = T.level_joint[t];
+            const int parent = T.jparent[j];
+            const int child = T.jchild[j];
+            const int ds = T.jds[j], nd = T.jds[j + 1] - ds;
+            const DTF Xp_d = dtf_from(load_tf(mXp + j * 8));
+            const DTF Xwpj_d = parent >= 0 ? dtf_compose_n(load_dtf(bqd + parent * 8), Xp_d) : Xp_d;
+            const DTF Xl = load_dtf(kXl + j * 8);
+            const DTF Xwc_d = (kXl[j * 8 + 7] != 0.0) ? DTF{Xwpj_d.p, dqmul(Xwpj_d.q, Xl.q)} : dtf_compose_n(Xwpj_d, Xl);
+            const Q4 R = tf_from(Xwpj_d).q;
+            for (int k = 0; k < nd; ++k) {
+                const SV Sl = load_sv(Sg + (ds + k) * 6);
+                store_sv(Sg + (ds + k) * 6, sv(qrot(R, Sl.w), qrot(R, Sl.v)));
+            }
+            const SV vl = load_sv(kV + child * 6);
+            const SV al = load_sv(kA + child * 6);
+            const SV vPB = sv(qrot(R, vl.w), qrot(R, vl.v));
+            const SV aPB = sv(qrot(R, al.w), qrot(R, al.v));
+            store_dtf(bqd + child * 8, Xwc_d);
+            const TF Xwc = tf_from(Xwc_d);
+            float* bqc = bq + child * 8;
+            bqc[0] = Xwc.p.x; bqc[1] = Xwc.p.y; bqc[2] = Xwc.p.z;
+            bqc[3] = Xwc.q.x; bqc[4] = Xwc.q.y; bqc[5] = Xwc.q.z; bqc[6] = Xwc.q.w;
+            SV V = vPB, A = aPB;
+            if (parent >= 0) {
+                const V3 pPB = Xwc.p - v3(bq[parent * 8], bq[parent * 8 + 1], bq[parent * 8 + 2]);
+                const SV Vp = load_sv(kV + parent * 6);
+                const SV Ap = load_sv(kA + parent * 6);
+                V = compose_velocity(Vp, pPB, vPB);
+                A = compose_acceleration(Ap, Vp, pPB, vPB, aPB);
+            }
+            store_sv(kV + child * 6, V);
+            store_sv(kA + child * 6, A);
+        }
+        wave_sync();
+    }
+
+    // ------------------------------------------------------------------ 3
+    // body dynamics about each body origin, inertia about O = root origin
+    const V3 O = v3(bq[0], bq[1], bq[2]);
+    for (int b = lane; b < nb; b += 64) {
+        const float m = mM[b];
+        const float* Ip = mI + b * 9;
+        const float* cp = mCom + b * 4;
+        const float* g = M.gravity + (size_t)w * 3;
+        const float* bqb = bq + b * 8;
+        const Q4 q = Q4{bqb[3], bqb[4], bqb[5], bqb[6]};
+        const V3 c0 = qrot(q, v3(1, 0, 0)), c1 = qrot(q, v3(0, 1, 0)), c2 = qrot(q, v3(0, 0, 1));
+        const float Rm[3][3] = {{c0.x, c1.x, c2.x}, {c0.y, c1.y, c2.y}, {c0.z, c1.z, c2.z}};
+        float Tm[3][3], IW[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 3; ++cc)
+                Tm[r][cc] = Rm[r][0] * Ip[0 * 3 + cc] + Rm[r][1] * Ip[1 * 3 + cc] + Rm[r][2] * Ip[2 * 3 + cc];
+        for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 3; ++cc)
+                IW[r][cc] = Tm[r][0] * Rm[cc][0] + Tm[r][1] * Rm[cc][1] + Tm[r][2] * Rm[cc][2];
+        const V3 c = qrot(q, v3(cp[0], cp[1], cp[2]));
+        const float cc2 = dot(c, c);
+        float IB[3][3];
+        const float cv_[3] = {c.x, c.y, c.z};
+        for (int r = 0; r < 3; ++r)
+            for (int k = 0; k < 3; ++k) IB[r][k] = IW[r][k] + m * ((r == k ? cc2 : 0.0f) - cv_[r] * cv_[k]);
+        const SV V = load_sv(kV + b * 6);
+        const SV A = load_sv(kA + b * 6);
+        const V3 Iw = v3(IB[0][0] * V.w.x + IB[0][1] * V.w.y + IB[0][2] * V.w.z,
+                         IB[1][0] * V.w.x + IB[1][1] * V.w.y + IB[1][2] * V.w.z,
+                         IB[2][0] * V.w.x + IB[2][1] * V.w.y + IB[2][2] * V.w.z);
+        const V3 Ia = v3(IB[0][0] * A.w.x + IB[0][1] * A.w.y + IB[0][2] * A.w.z,
+                         IB[1][0] * A.w.x + IB[1][1] * A.w.y + IB[1][2] * A.w.z,
+                         IB[2][0] * A.w.x + IB[2][1] * A.w.y + IB[2][2] * A.w.z);
+        V3 fw = Ia + cross(c, A.v) * m;
+        V3 fv = (A.v - cross(c, A.w)) * m;
+        fw = fw + cross(V.w, Iw);
+        fv = fv + cross(V.w, cross(V.w, c)) * m;
+        const V3 mg = v3(g[0], g[1], g[2]) * m;
+        fw = fw - cross(c, mg);
+        fv = fv - mg;
+        const float* e = kExt + b * 6;
+        float* F = kF + b * 6;
+        F[0] = fw.x + e[0]; F[1] = fw.y + e[1]; F[2] = fw.z + e[2];
+        F[3] = fv.x + e[3]; F[4] = fv.y + e[4]; F[5] = fv.z + e[5];
+        const V3 co = (v3(bqb[0], bqb[1], bqb[2]) - O) + c;
+        const float co2 = dot(co, co);
+        const float co_[3] = {co.x, co.y, co.z};
+        float* I = kI + b * 10;
+        int e6 = 0;
+        for (int r = 0; r < 3; ++r)
+            for (int k = r; k < 3; ++k) I[e6++] = IW[r][k] + m * ((r == k ? co2 : 0.0f) - co_[r] * co_[k]);
+        I[6] = m * co.x; I[7] = m * co.y; I[8] = m * co.z; I[9] = m;
+    }
+    wave_sync();
+
+    // ------------------------------------------------------------------ 4
+    // composites (reverse level sweep, each body pulls its children), tau, A rows
+    for (int Lv = M.num_levels - 1; Lv >= 0; --Lv) {
+        const int l0 = T.level_start[Lv], l1 = T.level_start[Lv + 1];
+        for (int t = l0 + lane; t < l1; t += 64) {
+            const int c = T.jchild[T.level_joint[t]];
+            float Ic[10];
+#pragma unroll
+            for (int k = 0; k < 10; ++k) Ic[k] = kI[c * 10 + k];
+            const float* F = kF + c * 6;
+            V3 tw = v3(F[0], F[1], F[2]), fv = v3(F[3], F[4], F[5]);
+            const V3 pc = v3(bq[c * 8], bq[c * 8 + 1], bq[c * 8 + 2]);
+            for (int e = T.child_start[c]; e < T.child_start[c + 1]; ++e) {
+                const int ch = T.child[e];
+#pragma unroll
+                for (int k = 0; k < 10; ++k) Ic[k] += kIc[ch * 10 + k];
+                const float* Fch = kFc + ch * 6;
+                const V3 f = v3(Fch[3], Fch[4], Fch[5]);
+                const V3 pch = v3(bq[ch * 8], bq[ch * 8 + 1], bq[ch * 8 + 2]);
+                tw = tw + v3(Fch[0], Fch[1], Fch[2]) - cross(pc - pch, f);
+                fv = fv + f;
+            }
+#pragma unroll
+            for (int k = 0; k < 10; ++k) kIc[c * 10 + k] = Ic[k];
+            float* Fc = kFc + c * 6;
+            Fc[0] = tw.x; Fc[1] = tw.y; Fc[2] = tw.z; Fc[3] = fv.x; Fc[4] = fv.y; Fc[5] = fv.z;
+        }
+        wave_sync();
+    }
+    float tau = 0.0f;
+    SV SOi = sv(v3(0, 0, 0), v3(0, 0, 0)), Fii = SOi;
+    int ji = 0;
+    if (lane < n) {
+        ji = T.dof_joint[lane];
+        const int c = T.jchild[ji];
+        const SV S = load_sv(Sg + lane * 6);
+        tau = jf - sv_dot(S, load_sv(kFc + c * 6));
+        const V3 pc = v3(bq[c * 8], bq[c * 8 + 1], bq[c * 8 + 2]);
+        SOi = shift_velocity(S, O - pc);
+        const float* I = kIc + c * 10;
+        const V3 hv = v3(I[6], I[7], I[8]);
+        const V3 Iw = v3(I[0] * SOi.w.x + I[1] * SOi.w.y + I[2] * SOi.w.z, I[1] * SOi.w.x + I[3] * SOi.w.y + I[4] * SOi.w.z,
+                         I[2] * SOi.w.x + I[4] * SOi.w.y + I[5] * SOi.w.z);
+        Fii = sv(Iw + cross(hv, SOi.v), SOi.v * I[9] - cross(hv, SOi.w));
+        store_sv(kSO + lane * 6, SOi);
+        store_sv(kFi + lane * 6, Fii);
+    }
+    wave_sync();
+    // dof-ancestry mask of this lane: bit j set when joint(j) lies on joint(lane)'s root path
+    unsigned long long dmask = 0ull;
+    if (lane < n) {
+        const unsigned long long anc = T.body_anc[T.jchild[ji]];
+        for (int j = 0; j < n; ++j)
+            if (bit(anc, T.dof_joint[j])) dmask |= 1ull << j;
+    }
+    const float arm_i = (lane < n) ? M.armature[(size_t)w * n + lane] : 0.0f;
+    float adiag = 1.0f;
+    for (int j = 0; j < n; ++j) {
+        const unsigned long long mj =
+            ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(dmask >> 32), j) << 32) |
+            (unsigned)__builtin_amdgcn_readlane((int)dmask, j);
+        const SV SOj = load_sv(kSO + j * 6);
+        const SV Fij = load_sv(kFi + j * 6);
+        float val = 0.0f;
+        if ((dmask >> j) & 1ull) val = sv_dot(Fii, SOj);
+        else if ((mj >> lane) & 1ull) val = sv_dot(Fij, SOi);
+        if (lane == j) { val += arm_i; adiag = val; }
+        if (lane < n) Am[lane * ls + j] = val;
+    }
+    wave_sync();
+
+    // ------------------------------------------------------------------ 5
+    // free motion v* = v0 + dt A^-1 tau with one float64-residual refinement
+    float hrow[NP];
+    float dinv;
+#pragma unroll
+    for (int j = 0; j < NP; ++j) hrow[j] = (lane < n) ? (j < n ? arow[j] : 0.0f) : ((lane == j) ? 1.0f : 0.0f);
+    chol_rows<NP>(hrow, dinv, lane);
+    store_rows<NP>(hrow, Lt, ls, lane);
+    wave_sync();
+    const float x1 = chol_solve<NP>(hrow, Lt, ls, dinv, lane < n ? tau : 0.0f, lane);
+    const double x1d = lane < n ? (double)x1 : 0.0;
+    const double r1 = lane < n ? (double)tau - lds_row_dot(arow, x1d, n) : 0.0;
+    const float x2 = chol_solve<NP>(hrow, Lt, ls, dinv, (float)r1, lane);
+    const double vs = lane < n ? v0 + dt * (x1d + (double)x2) : 0.0;
+    double ainv = 1.0, dsc = 1.0;
+    {
+        double ad = (lane < n) ? (double)adiag : 1.0;
+        if (ad < 1.0e-12 || !isfinite(ad)) ad = 1.0e-12;
+        ainv = (float)(1.0 / ad);
+        dsc = (float)(1.0 / sqrt(ad));
+    }
+
+    // ------------------------------------------------------------------ 6
+    // contacts: lane c builds contact c (witnesses / phi0 in float64)
+    int cb0 = -1, cb1 = -1;
+    V3 cx0 = v3(0, 0, 0), cx1 = cx0, cpc = cx0, tx = cx0, ty = cx0, nrm = cx0;
+    double phi = 0.0, kcomb = 1.0, taus = 0.0, muc = 0.0;
+    if (lane < nc) {
+        const int idx = c_begin + lane;
+        const int sh0 = P.cshape0[idx], sh1 = P.cshape1[idx];
+        const int b0g = sh0 >= 0 ? M.shape_body[sh0] : -1;
+        const int b1g = sh1 >= 0 ? M.shape_body[sh1] : -1;
+        cb0 = b0g >= 0 ? b0g - w * nb : -1;
+        cb1 = b1g >= 0 ? b1g - w * nb : -1;
+        D3 x0d = d3(P.cpoint0[idx * 3], P.cpoint0[idx * 3 + 1], P.cpoint0[idx * 3 + 2]);
+        D3 x1d = d3(P.cpoint1[idx * 3], P.cpoint1[idx * 3 + 1], P.cpoint1[idx * 3 + 2]);
+        if (cb0 >= 0) x0d = dtf_point(load_dtf(bqd + cb0 * 8), x0d);
+        if (cb1 >= 0) x1d = dtf_point(load_dtf(bqd + cb1 * 8), x1d);
+        D3 nd = d3(P.cnormal[idx * 3], P.cnormal[idx * 3 + 1], P.cnormal[idx * 3 + 2]);
+        const double nd2 = nd.x * nd.x + nd.y * nd.y + nd.z * nd.z;
+        nd = (nd2 <= 1.0e-32) ? d3(0.0, 0.0, 0.0) : dscale(nd, 1.0 / sqrt(nd2));
+        cx0 = v3((float)x0d.x, (float)x0d.y, (float)x0d.z);
+        cx1 = v3((float)x1d.x, (float)x1d.y, (float)x1d.z);
+        nrm = v3((float)nd.x, (float)nd.y, (float)nd.z);
+        // frame: MakeFromOneUnitVector (sap_helpers.py:467-503)
+        const float ax = fabsf(nrm.x), ay = fabsf(nrm.y), az = fabsf(nrm.z);
+        int im = 0;
+        float mn = ax;
+        if (ay < mn) { im = 1; mn = ay; }
+        if (az < mn) { im = 2; mn = az; }
+        const float mag = sqrtf(1.0f - mn * mn);
+        const float rr = 1.0f / mag;
+        const float u = im == 0 ? nrm.x : (im == 1 ? nrm.y : nrm.z);
+        const float sg = -rr * u;
+        if (im == 0) { tx = v3(0.0f, -rr * nrm.z, rr * nrm.y); ty = v3(mag, sg * nrm.y, sg * nrm.z); }
+        else if (im == 1) { tx = v3(rr * nrm.z, 0.0f, -rr * nrm.x); ty = v3(sg * nrm.x, mag, sg * nrm.z); }
+        else { tx = v3(-rr * nrm.y, rr * nrm.x, 0.0f); ty = v3(sg * nrm.x, sg * nrm.y, mag); }
+        // materials (sap_helpers.py:148-237)
+        double k0 = P.fallback_k, k1 = P.fallback_k;
+        if (sh0 >= 0) k0 = M.shape_ke[sh0];
+        if (!isfinite(k0) || k0 <= 0.0) k0 = P.fallback_k;
+        if (!isfinite(k0) || k0 <= 0.0) k0 = 1.0;
+        if (sh1 >= 0) k1 = M.shape_ke[sh1];
+        if (!isfinite(k1) || k1 <= 0.0) k1 = P.fallback_k;
+        if (!isfinite(k1) || k1 <= 0.0) k1 = 1.0;
+        const double kden = k0 + k1;
+        double w0 = 0.5, w1 = 0.5;
+        if (kden != 0.0) { w0 = k0 / kden; w1 = k1 / kden; }
+        const D3 pcd = dadd(dscale(x0d, w0), dscale(x1d, w1));
+        cpc = v3((float)pcd.x, (float)pcd.y, (float)pcd.z);
+        if (!isfinite(k0)) kcomb = k1;
+        else if (!isfinite(k1)) kcomb = k0;
+        else if (kden <= 1.0e-12) kcomb = fmax(k0, k1);
+        else kcomb = k0 * k1 / kden;
+        double mu0 = P.fallback_mu, mu1 = P.fallback_mu;
+        if (sh0 >= 0) mu0 = M.shape_mu[sh0];
+        if (sh1 >= 0) mu1 = M.shape_mu[sh1];
+        mu0 = fmax(mu0, 0.0);
+        mu1 = fmax(mu1, 0.0);
+        const double mud = mu0 + mu1;
+        muc = (mud <= 1.0e-12) ? 0.0 : 2.0 * mu0 * mu1 / mud;
+        double t0 = P.fallback_tau, t1 = P.fallback_tau;
+        if (sh0 >= 0) t0 = M.shape_tau[sh0];
+        if (!isfinite(t0) || t0 < 0.0) t0 = P.fallback_tau;
+        if (!isfinite(t0) || t0 < 0.0) t0 = 0.0;
+        if (sh1 >= 0) t1 = M.shape_tau[sh1];
+        if (!isfinite(t1) || t1 < 0.0) t1 = P.fallback_tau;
+        if (!isfinite(t1) || t1 < 0.0) t1 = 0.0;
+        taus = t0 + t1;
+        const D3 gapv = dsub(x1d, x0d);
+        phi = gapv.x * nd.x + gapv.y * nd.y + gapv.z * nd.z - (double)P.cmargin0[idx] - (double)P.cmargin1[idx];
+    }
+    // contact Jacobian rows of contact `lane`: J_c[:, d] = R^T (u1 - u0)
+    vec[64 + lane] = (lane < n) ? ainv : 0.0;   // diag(A)^-1 by dof for the Delassus weights
+    if (lane < nc) {
+        const bool wit = P.point_mode == 0;
+        const V3 xs1 = wit ? cx1 : cpc;
+        const V3 xs0 = wit ? cx0 : cpc;
+        const unsigned long long anc1 = cb1 >= 0 ? T.body_anc[cb1] : 0ull;
+        const unsigned long long anc0 = cb0 >= 0 ? T.body_anc[cb0] : 0ull;
+        float* J0 = Jm + (lane * 3) * js;
+        for (int d = 0; d < n; ++d) {
+            const int jd = T.dof_joint[d];
+            const int cd = T.jchild[jd];
+            const SV S = load_sv(Sg + d * 6);
+            const V3 pcdv = v3(bq[cd * 8], bq[cd * 8 + 1], bq[cd * 8 + 2]);
+            V3 rel = v3(0, 0, 0);
+            if (bit(anc1, jd)) rel = rel + S.v + cross(S.w, xs1 - pcdv);
+            if (bit(anc0, jd)) rel = rel - (S.v + cross(S.w, xs0 - pcdv));
+            J0[d] = dot(tx, rel);
+            J0[js + d] = dot(ty, rel);
+            J0[2 * js + d] = dot(nrm, rel);
+        }
+    }
+    wave_sync();
+    // effective weights + regularisation constants (contact_jacobian.py:153-251, contact_solve.py:791-822)
+    if (lane < nc) {
+        double wvv;
+        if (P.weight_mode == 0) {
+            const V3 dirs[3] = {tx, ty, nrm};
+            double acc = 0.0;
+            for (int side = 0; side < 2; ++side) {
+                const int lb = side == 0 ? cb0 : cb1;
+                if (lb < 0) continue;
+                const double m = mM[lb];
+                if (m <= 1.0e-12) continue;
+                const float* bb = bq + lb * 8;
+                const Q4 q = qnormalize(Q4{bb[3], bb[4], bb[5], bb[6]});
+                const float* com = mCom + lb * 4;
+                const V3 comw = v3(bb[0], bb[1], bb[2]) + qrot(q, v3(com[0], com[1], com[2]));
+                const V3 r = (side == 0 ? cx0 : cx1) - comw;
+                const float* I = mI + lb * 9;
+                const double ix = fmax((double)I[0], 1.0e-12), iy = fmax((double)I[4], 1.0e-12), iz = fmax((double)I[8], 1.0e-12);
+                for (int k = 0; k < 3; ++k) {
+                    V3 dn = dirs[k];
+                    const float dl = sqrtf(dot(dn, dn));
+                    dn = (dl * dl <= 1.0e-32f) ? v3(0, 0, 0) : dn * (1.0f / dl);
+                    const V3 cbv = qrot(qconj(q), cross(r, dn));
+                    acc += 1.0 / m + (double)cbv.x * cbv.x / ix + (double)cbv.y * cbv.y / iy + (double)cbv.z * cbv.z / iz;
+                }
+            }
+            wvv = acc / 3.0;
+        } else {
+            double W00 = 0, W01 = 0, W02 = 0, W11 = 0, W12 = 0, W22 = 0;
+            const float* J0 = Jm + (lane * 3) * js;
+            for (int d = 0; d < n; ++d) {
+                const double ai = vec[64 + d];
+                const double a0 = J0[d], a1 = J0[js + d], a2 = J0[2 * js + d];
+                W00 += ai * a0 * a0; W01 += ai * a0 * a1; W02 += ai * a0 * a2;
+                W11 += ai * a1 * a1; W12 += ai * a1 * a2; W22 += ai * a2 * a2;
+            }
+            wvv = sqrt(W00 * W00 + W11 * W11 + W22 * W22 + 2.0 * (W01 * W01 + W02 * W02 + W12 * W12)) / 3.0;
+        }
+        if (wvv < 1.0e-12 || !isfinite(wvv)) wvv = 1.0e-12;
+        double kc = kcomb;
+        if (kc <= 0.0 || !isfinite(kc)) kc = 1.0;
+        const double tau_c = fmax(taus, 0.0);
+        const double bfac = P.beta * P.beta / (4.0 * kPi * kPi);
+        const double rn_soft = 1.0 / (dt * kc * (dt + tau_c));
+        const double rn = fmax(fmax(bfac * wvv, rn_soft), 1.0e-30);
+        const double rt = fmax(P.sigma * wvv, 1.0e-30);
+        double mu = muc;
+        if (mu < 0.0 || !isfinite(mu)) mu = 0.0;
+        double* k8 = ck + lane * kCk;
+        k8[0] = rt;
+        k8[1] = rn;
+        k8[2] = -phi / (dt + tau_c);
+        k8[3] = mu;
+        k8[4] = 1.0 / rt;
+        k8[5] = 1.0 / rn;
+        const double mu_tilde = mu * sqrt(rt / rn);
+        k8[6] = mu * rt / rn;
+        k8[7] = 1.0 / (1.0 + mu_tilde * mu_tilde);
+        if (P.dbg_phi0) P.dbg_phi0[(size_t)w * C + lane] = phi;
+        if (P.dbg_w) P.dbg_w[(size_t)w * C + lane] = wvv;
+    }
+    wave_sync();
+
+    // ------------------------------------------------------------------ 7
+    // PD + limit terms (contact_solve.py:490-683) and participating dofs (422-487)
+    DofTerms mt = {0, 0, 0, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f};
+    int part = 0;
+    if (lane < n) {
+        const int i = lane;
+        const int t = T.dof_target[i];
+        const size_t gt = (size_t)w * n + t;
+        const size_t gi = (size_t)w * n + i;
+        const int coord = T.dof_coord[i];
+        if (M.target_mode[gt] != 0) {
+            const double kp = M.target_ke[gt], kd = M.target_kd[gt];
+            if (!(kp <= 0.0 && kd <= 0.0) && !(coord < 0 && kp > 0.0)) {
+                const double pbf = kPdBeta * kPdBeta / (4.0 * kPi * kPi);
+                const double r_nr = pbf * fmax(ainv, 1.0e-12);
+                const double common = dt * kp + kd;
+                double kpe = 0.0, kde = 0.0;
+                if (common > 0.0) {
+                    const double r = 1.0 / (dt * common);
+                    if (r >= r_nr) { kpe = kp; kde = kd; }
+                    else if (dt * kp > kd && kp > 0.0) {
+                        const double tc = kd / kp;
+                        kpe = 1.0 / (dt * (dt + tc) * r_nr);
+                        kde = tc * kpe;
+                    } else if (kd > 0.0) {
+                        const double tinv = kp / kd;
+                        kde = 1.0 / (dt * (1.0 + dt * tinv) * r_nr);
+                        kpe = tinv * kde;
+                    } else {
+                        kpe = 1.0 / (dt * dt * r_nr);
+                        kde = 0.0;
+                    }
+                }
+                const double gain = dt * kpe + kde;
+                if (gain > 0.0) {
+                    double q0 = 0.0, qt = 0.0;
+                    if (coord >= 0) {
+                        q0 = q_in[coord];
+                        qt = P.c_tpos ? P.c_tpos[gt] : 0.0f;
+                    }
+                    const double vd = P.c_tvel ? P.c_tvel[gt] : 0.0f;
+                    const double u0 = P.c_act ? P.c_act[gt] : 0.0f;
+                    mt.pd = 1;
+                    mt.pda = (float)(kpe * (qt - q0) + kde * vd + u0);
+                    mt.pdg = (float)gain;
+                    mt.pdl = M.effort_limit[gt];
+                }
+            }
+        }
+        if (coord >= 0 && T.dof_limit_ok[i]) {
+            const double ke = M.limit_ke[gi];
+            if (ke > 0.0) {
+                const double q0 = q_in[coord];
+                const double lower = M.limit_lower[gi], upper = M.limit_upper[gi];
+                const double window = kLimitWindow * dt * fmax(fabs(v0), fabs(vs));
+                const double kd = M.limit_kd[gi];
+                const double tau_d = kd > 0.0 ? kd / ke : 0.0;
+                const double lbf = kLimitBeta * kLimitBeta / (4.0 * kPi * kPi);
+                const double r = fmax(lbf * fmax(ainv, 1.0e-12), 1.0 / (dt * ke * (dt + tau_d)));
+                if (isfinite(lower) && q0 - lower <= window) {
+                    mt.lo = 1; mt.lov = (float)(-(q0 - lower) / (dt + tau_d)); mt.lor = (float)r;
+                }
+                if (isfinite(upper) && upper - q0 <= window) {
+                    mt.up = 1; mt.upv = (float)(-(upper - q0) / (dt + tau_d)); mt.upr = (float)r;
+                }
+            }
+        }
+        part = mt.pd | mt.lo | mt.up;
+    }
+    for (int c = 0; c < nc; ++c) {
+        const int b0 = __builtin_amdgcn_readlane(cb0, c), b1 = __builtin_amdgcn_readlane(cb1, c);
+        if (lane < n && !part) {
+            for (int side = 0; side < 2; ++side) {
+                const int lb = side == 0 ? b0 : b1;
+                if (lb >= 0) {
+                    const int st = T.body_art_start[lb], ct = T.body_art_count[lb];
+                    if (st >= 0 && ct > 0 && lane >= st && lane < st + ct) part = 1;
+                }
+            }
+            const float* J0 = Jm + (c * 3) * js;
+            if (J0[lane] != 0.0f || J0[js + lane] != 0.0f || J0[2 * js + lane] != 0.0f) part = 1;
+        }
+    }
+
+    // ------------------------------------------------------------------ 8
+    // SAP Newton solve (contact_solve.py:6129-6285, 5474-5655)
+    const bool unconstrained = (nc == 0) && !__any(mt.pd | mt.lo | mt.up);
+    double v = 0.0;
+    if (lane < n) {
+        v = P.use_guess ? P.v_guess[(size_t)w * n + lane] : v0;
+        if (unconstrained || !part) v = vs;
+    }
+    const double avs = lane < n ? lds_row_dot(arow, vs, n) : 0.0;
+    int iteration = 0, ls_total = 0, flags = 0, ls_status = 0;
+    // lane-c contact state: vc, dvc, gamma (float64), G in LDS (cG)
+    double vc0 = 0, vc1 = 0, vc2 = 0, dvc0 = 0, dvc1 = 0, dvc2 = 0, g0 = 0, g1 = 0, g2 = 0;
+    // lane-i dof state
+    double grad = 0.0, ar = 0.0, dvv = 0.0;
+    float hd = 0.0f;
+    double alpha = 1.0, prev_cost = 0.0;
+    const double cost_min_alpha = 0.5;
+    const double* k8 = ck + (lane < CB ? lane : 0) * kCk;
+    if (unconstrained) {
+        flags = 1;
+    } else {
+        while (true) {
+            // ---- evaluate at v: vc = J v, A v; projection + model terms; impulses / gradient / norms
+            double av;
+            {
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+                double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+                const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
+                for (int j = 0; j < n; ++j) {
+                    const double xj = readlane_d(v, j);
+                    const double aij = (double)arow[j] * xj;
+                    if ((j & 3) == 0) s0 += aij;
+                    else if ((j & 3) == 1) s1 += aij;
+                    else if ((j & 3) == 2) s2 += aij;
+                    else s3 += aij;
+                    if (lane < nc) {
+                        c0 += (double)J0[j] * xj;
+                        c1 += (double)J0[js + j] * xj;
+                        c2 += (double)J0[2 * js + j] * xj;
+                    }
+                }
+                av = lane < n ? (s0 + s1) + (s2 + s3) : 0.0;
+                vc0 = c0; vc1 = c1; vc2 = c2;
+            }
+            ar = av - avs;
+            double part4[4] = {0.0, 0.0, 0.0, 0.0};  // cost, |D g|^2, |D A v|^2, |D j|^2
+            if (lane < nc) {
+                const double vcv[3] = {vc0, vc1, vc2};
+                double gg[3];
+                part4[0] += project_contact(k8, vcv, gg, cG + lane * 9, true);
+                g0 = gg[0]; g1 = gg[1]; g2 = gg[2];
+            }
+            double imp = 0.0, hdd = 0.0;
+            if (lane < n) part4[0] += dof_terms(mt, v, dt, &imp, &hdd);
+            hd = (float)hdd;
+            double jc = imp;
+            for (int c = 0; c < nc; ++c) {
+                const float* Jc = Jm + (c * 3) * js;
+                const double gc0 = readlane_d(g0, c), gc1 = readlane_d(g1, c), gc2 = readlane_d(g2, c);
+                if (lane < n) jc += (double)Jc[lane] * gc0 + (double)Jc[js + lane] * gc1 + (double)Jc[2 * js + lane] * gc2;
+            }
+            grad = ar - jc;
+            if (lane < n) {
+                part4[0] += 0.5 * (v - vs) * ar;
+                if (part) {
+                    part4[1] += (dsc * grad) * (dsc * grad);
+                    part4[2] += (dsc * av) * (dsc * av);
+                    part4[3] += (dsc * jc) * (dsc * jc);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) part4[k] = wave_sum(part4[k]);
+            const double cost = part4[0];
+            const double gnorm = sqrt(fmax(part4[1], 0.0));
+            const double pnorm = sqrt(fmax(part4[2], 0.0));
+            const double jnorm = sqrt(fmax(part4[3], 0.0));
+            // ---- convergence (contact_solve.py:1970-2087)
+            const double opt_tol = P.opt_abs + P.opt_rel * fmax(pnorm, jnorm);
+            const bool opt = gnorm <= opt_tol;
+            bool costok = false;
+            if (iteration > 0 && alpha > cost_min_alpha) {
+                const double scale = 0.5 * (fabs(cost) + fabs(prev_cost));
+                if (fabs(cost - prev_cost) < P.cost_abs + P.cost_rel * scale) costok = true;
+            }
+            if (opt || costok) {
+                flags = 1 | (opt ? 2 : 0) | (costok ? 4 : 0);
+                break;
+            }
+            if (iteration >= P.max_iter) {
+                flags = 8;
+                break;
+            }
+            wave_sync();
+            // ---- Hessian rows H = A + J^T G J + diag(hd + shift); J^T G J by MFMA into LDS
+            {
+                constexpr int TB = NP / 16;
+                const int K = 3 * nc;
+                for (int tt = 0; tt < TB * TB; ++tt) {
+                    const int ib = tt / TB, jb = tt % TB;
+                    const int i0 = ib * 16, j0 = jb * 16;
+                    typedef float f4 __attribute__((ext_vector_type(4)));
+                    f4 accm = {0.0f, 0.0f, 0.0f, 0.0f};
+                    const int ia = i0 + (lane & 15);
+                    const int jbb = j0 + (lane & 15);
+                    for (int kk0 = 0; kk0 < K; kk0 += 4) {
+                        const int k = kk0 + (lane >> 4);
+                        float av_ = 0.0f, bv_ = 0.0f;
+                        if (k < K) {
+                            const int c = k / 3, r = k % 3;
+                            if (ia < n) av_ = Jm[k * js + ia];
+                            if (jbb < n) {
+                                const float* G = cG + c * 9 + r * 3;
+                                const float* Jc = Jm + (c * 3) * js + jbb;
+                                bv_ = G[0] * Jc[0] + G[1] * Jc[js] + G[2] * Jc[2 * js];
+                            }
+                        }
+                        accm = __builtin_amdgcn_mfma_f32_16x16x4f32(av_, bv_, accm, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) Lt[(i0 + 4 * (lane >> 4) + e) * ls + j0 + (lane & 15)] = accm[e];
+                }
+            }
+            wave_sync();
+            const float hdiag = hd + (float)P.diag_shift;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const float hc = Lt[(lane < NP ? lane : 0) * ls + j];
+                hrow[j] = (lane < n) ? (j < n ? arow[j] + hc + ((lane == j) ? hdiag : 0.0f) : 0.0f) : ((lane == j) ? 1.0f : 0.0f);
+            }
+            wave_sync();
+            chol_rows<NP>(hrow, dinv, lane);
+            store_rows<NP>(hrow, Lt, ls, lane);
+            wave_sync();
+            float dvf = chol_solve<NP>(hrow, Lt, ls, dinv, lane < n ? (float)(-grad) : 0.0f, lane);
+            dvv = lane < n ? (double)dvf : 0.0;
+            // ---- mixed-precision refinement: r = -grad - (A + J^T G J + diag) dv in float64
+            for (int rep = 0; rep < kNewtonRefine; ++rep) {
+                double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+                const double adv = lane < n ? lds_row_dot(arow, dvv, n) : 0.0;
+                {
+                    const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
+                    for (int j = 0; j < n; ++j) {
+                        const double xj = readlane_d(dvv, j);
+                        if (lane < nc) {
+                            t0 += (double)J0[j] * xj;
+                            t1 += (double)J0[js + j] * xj;
+                            t2 += (double)J0[2 * js + j] * xj;
+                        }
+                    }
+                }
+                double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+                if (lane < nc) {
+                    const float* G = cG + lane * 9;
+                    u0 = (double)G[0] * t0 + (double)G[1] * t1 + (double)G[2] * t2;
+                    u1 = (double)G[3] * t0 + (double)G[4] * t1 + (double)G[5] * t2;
+                    u2 = (double)G[6] * t0 + (double)G[7] * t1 + (double)G[8] * t2;
+                }
+                double acc = adv + ((double)hd + P.diag_shift) * dvv;
+                for (int c = 0; c < nc; ++c) {
+                    const float* Jc = Jm + (c * 3) * js;
+                    const double uc0 = readlane_d(u0, c), uc1 = readlane_d(u1, c), uc2 = readlane_d(u2, c);
+                    if (lane < n) acc += (double)Jc[lane] * uc0 + (double)Jc[js + lane] * uc1 + (double)Jc[2 * js + lane] * uc2;
+                }
+                const double res = lane < n ? -grad - acc : 0.0;
+                const float corr = chol_solve<NP>(hrow, Lt, ls, dinv, (float)res, lane);
+                if (lane < n) dvv += (double)corr;
+            }
+            // ---- line search data (contact_solve.py:2234-2263)
+            double ld[4] = {0, 0, 0, 0};  // dell0, dell_a0, d2ell_a, m0
+            {
+                const double adv = lane < n ? lds_row_dot(arow, dvv, n) : 0.0;
+                double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+                {
+                    const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
+                    for (int j = 0; j < n; ++j) {
+                        const double xj = readlane_d(dvv, j);
+                        if (lane < nc) {
+                            d0 += (double)J0[j] * xj;
+                            d1 += (double)J0[js + j] * xj;
+                            d2 += (double)J0[2 * js + j] * xj;
+                        }
+                    }
+                }
+                dvc0 = d0; dvc1 = d1; dvc2 = d2;
+                if (lane < n) {
+                    const double ri = v - vs;
+                    ld[0] = grad * dvv;
+                    ld[1] = adv * ri;
+                    ld[2] = dvv * adv;
+                    ld[3] = 0.5 * ri * ar;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; ++k) ld[k] = wave_sum(ld[k]);
+            }
+            const double dell0 = ld[0], dell_a0 = ld[1], d2ell_a = ld[2], m0 = ld[3];
+            // trial evaluation of phi(a), phi'(a), phi''(a)
+            auto trial = [&](double al, bool second, double* dd1, double* dd2) -> double {
+                double tv[3] = {0.0, 0.0, 0.0};
+                if (lane < nc) {
+                    const double dvc[3] = {dvc0, dvc1, dvc2};
+                    const double vcv[3] = {vc0 + al * dvc0, vc1 + al * dvc1, vc2 + al * dvc2};
+                    double gg[3], cc;
+                    if (second) {
+                        tv[2] += project_curvature(k8, vcv, dvc, gg, &cc);
+                    } else {
+                        float Gd[9];
+                        cc = project_contact(k8, vcv, gg, Gd, false);
+                    }
+                    tv[0] += cc;
+                    tv[1] -= dvc[0] * gg[0] + dvc[1] * gg[1] + dvc[2] * gg[2];
+                }
+                if (lane < n) {
+                    double impt, hdt;
+                    tv[0] += dof_terms(mt, v + al * dvv, dt, &impt, &hdt);
+                    tv[1] -= dvv * impt;
+                    tv[2] += hdt * dvv * dvv;
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) tv[k] = wave_sum(tv[k]);
+                *dd1 = dell_a0 + al * d2ell_a + tv[1];
+                *dd2 = d2ell_a + tv[2];
+                return m0 + al * dell_a0 + 0.5 * al * al * d2ell_a + tv[0];
+            };
+            bool accepted = false;
+            double a_acc = 1.0;
+            int ls_it = 0;
+            ls_status = 0;
+            if (!(dell0 < 0.0) || !isfinite(dell0)) {
+                ls_status = -1;
+            } else if (P.ls_mode == 0) {
+                // Armijo backtracking (contact_solve.py:2670-2905, 5108-5305)
+                const double amax = 1.0 / P.rho;
+                double al = amax, dd1, dd2;
+                double ell = trial(al, false, &dd1, &dd2);
+                const double slop = (P.rel_slop / 10.0) * fmax(1.0, 0.5 * (fabs(ell) + fabs(cost)));
+                if (dd1 < 0.0 || dd1 < slop) {
+                    accepted = true;
+                    a_acc = al;
+                } else if (P.ls_max_iter > 1) {
+                    double a_prev = al, e_prev = ell;
+                    int it = 1;
+                    while (true) {
+                        al = al * P.rho;
+                        ell = trial(al, false, &dd1, &dd2);
+                        const double den = al - a_prev;
+                        const double dapprox = fabs(den) > 0.0 ? (ell - e_prev) / den : 0.0;
+                        if (fabs(dapprox) < slop) { accepted = true; a_acc = al; ls_it = it; break; }
+                        if (ell > e_prev && ell < cost + P.armijo_c * al * dell0) {
+                            accepted = true;
+                            ls_it = it;
+                            a_acc = (e_prev < cost + P.armijo_c * a_prev * dell0) ? a_prev : al;
+                            break;
+                        }
+                        if (it >= P.ls_max_iter - 1) {
+                            if (ell < cost + P.armijo_c * al * dell0) { accepted = true; a_acc = al; ls_it = P.ls_max_iter; }
+                            else ls_status = -2;
+                            break;
+                        }
+                        a_prev = al;
+                        e_prev = ell;
+                        ++it;
+                    }
+                } else {
+                    ls_status = -2;
+                }
+            } else {
+                // exact root (contact_solve.py:2913-3140, 5700-5908), cubic Hermite seed
+                const double amax = kExactAlphaMax;
+                double dd1, dd2;
+                const double ell = trial(amax, true, &dd1, &dd2);
+                if (dd1 <= 0.0) {
+                    accepted = true; a_acc = amax;
+                } else if (-dell0 < P.cost_abs + P.cost_rel * ell) {
+                    accepted = true; a_acc = 1.0;
+                } else {
+                    const double scale = -dell0;
+                    const double f_upper0 = dd1 / scale;
+                    if (!isfinite(f_upper0) || f_upper0 <= 0.0) {
+                        ls_status = -3;
+                    } else {
+                        double guess = -dell0 / dd2;
+                        const double hh = amax;
+                        const double theta = dell0 + dd1 - 3.0 * (ell - cost) / hh;
+                        const double disc = theta * theta - dell0 * dd1;
+                        if (isfinite(disc) && disc >= 0.0) {
+                            const double gm = sqrt(disc);
+                            const double den = dd1 - dell0 + 2.0 * gm;
+                            if (den != 0.0) {
+                                const double cubic = hh - hh * (dd1 + gm - theta) / den;
+                                if (isfinite(cubic) && cubic > 0.0 && cubic <= hh) guess = cubic;
+                            }
+                        }
+                        if (guess > amax) guess = amax;
+                        if (!isfinite(guess) || guess < 0.0 || guess > amax || kExactFTol * guess <= 0.0) {
+                            ls_status = -4;
+                        } else {
+                            double x_lo = 0.0, x_hi = amax, f_hi = f_upper0, root = guess;
+                            double mdx = -amax, mdx_prev = -amax;
+                            const double x_tol = kExactFTol * guess;
+                            int it = 1;
+                            while (true) {
+                                trial(root, true, &dd1, &dd2);
+                                const double f = dd1 / scale, df = dd2 / scale;
+                                if ((f < 0.0 && f_hi >= 0.0) || (f >= 0.0 && f_hi < 0.0)) {
+                                    x_lo = root;
+                                } else {
+                                    x_hi = root;
+                                    f_hi = f;
+                                }
+                                if (fabs(f) < kExactFTol) { accepted = true; a_acc = root; ls_it = it; break; }
+                                if (it >= P.ls_max_iter) { ls_status = -5; break; }
+                                const bool slow = 2.0 * fabs(f) > fabs(mdx_prev * df);
+                                mdx_prev = mdx;
+                                const double bdx = 0.5 * (x_lo - x_hi);
+                                const double broot = x_lo - bdx;
+                                double ndx = 0.0, nroot = root;
+                                if (fabs(df) > 0.0) { ndx = f / df; nroot = root - ndx; }
+                                const bool bis = slow || nroot < x_lo || nroot > x_hi || !isfinite(nroot) || !isfinite(ndx);
+                                const double nxt = bis ? broot : nroot;
+                                const double nd = bis ? bdx : ndx;
+                                if (fabs(nd) < x_tol) { accepted = true; a_acc = nxt; ls_it = it; break; }
+                                root = nxt;
+                                mdx = nd;
+                                ++it;
+                            }
+                        }
+                    }
+                }
+            }
+            // ---- commit (contact_solve.py:3154-3176)
+            if (accepted) {
+                alpha = a_acc;
+                prev_cost = cost;
+                ls_total += ls_it;
+                if (lane < n) v += alpha * dvv;
+            }
+            ++iteration;
+        }
+    }
+
+    // ------------------------------------------------------------------ 9
+    // integration (solver_sap.py:288-519) + SAP -> public velocity
+    vec[lane] = v0;
+    vec[64 + lane] = v;
+    for (int k = lane; k < M.NQ; k += 64) q_out[k] = q_in[k];
+    wave_sync();
+    const double* v0v = vec;
+    const double* vv = vec + 64;
+    float* qd_pub = kQdPub;
+    for (int j = lane; j < nj; j += 64) {
+        const int jt = T.jtype[j];
+        const int qs = T.jqs[j], ds = T.jds[j], nd = T.jds[j + 1] - T.jds[j];
+        if (jt == kJointFree || jt == kJointDistance) {
+            const Q4 q = Q4{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]};
+            const V3 wn = v3((float)vv[ds], (float)vv[ds + 1], (float)vv[ds + 2]);
+            const V3 von = v3((float)vv[ds + 3], (float)vv[ds + 4], (float)vv[ds + 5]);
+            const V3 wo = v3((float)v0v[ds], (float)v0v[ds + 1], (float)v0v[ds + 2]);
+            const V3 voo = v3((float)v0v[ds + 3], (float)v0v[ds + 4], (float)v0v[ds + 5]);
+            Q4 qn;
+            V3 vmid;
+            if (P.integ_mode == 0) {
+                const V3 wm = (wo + wn) * 0.5f;
+                const float ang = length(wm) * h;
+                qn = q;
+                if (ang > 1.0e-12f) {
+                    const V3 ax = wm * (1.0f / length(wm));
+                    const float sa = sinf(0.5f * ang), ca = cosf(0.5f * ang);
+                    const Q4 r = Q4{ax.x * sa, ax.y * sa, ax.z * sa, ca};
+                    Q4 o = qmul(r, q);
+                    const float nn = sqrtf(o.x * o.x + o.y * o.y + o.z * o.z + o.w * o.w);
+                    if (nn > 0.0f) { const float inv = 1.0f / nn; o = Q4{o.x * inv, o.y * inv, o.z * inv, o.w * inv}; }
+                    qn = o;
+                }
+                vmid = (voo + von) * 0.5f;
+            } else {
+                const float hdl = 0.5f * h;
+                float x = q.x + hdl * (q.w * wn.x + wn.y * q.z - wn.z * q.y);
+                float y = q.y + hdl * (q.w * wn.y + wn.z * q.x - wn.x * q.z);
+                float z = q.z + hdl * (q.w * wn.z + wn.x * q.y - wn.y * q.x);
+                float sw = q.w - hdl * (wn.x * q.x + wn.y * q.y + wn.z * q.z);
+                const float nn = sqrtf(x * x + y * y + z * z + sw * sw);
+                if (nn > 0.0f) { const float inv = 1.0f / nn; x *= inv; y *= inv; z *= inv; sw *= inv; }
+                qn = Q4{x, y, z, sw};
+                vmid = von;
+            }
+            q_out[qs] = q_in[qs] + h * vmid.x;
+            q_out[qs + 1] = q_in[qs + 1] + h * vmid.y;
+            q_out[qs + 2] = q_in[qs + 2] + h * vmid.z;
+            q_out[qs + 3] = qn.x; q_out[qs + 4] = qn.y; q_out[qs + 5] = qn.z; q_out[qs + 6] = qn.w;
+            const float* com = mCom + T.jchild[j] * 4;
+            const V3 rc = qrot(qnormalize(qn), v3(com[0], com[1], com[2]));
+            const V3 vcom = von + cross(wn, rc);
+            qd_pub[ds] = vcom.x; qd_pub[ds + 1] = vcom.y; qd_pub[ds + 2] = vcom.z;
+            qd_pub[ds + 3] = wn.x; qd_pub[ds + 4] = wn.y; qd_pub[ds + 5] = wn.z;
+        } else if (jt == kJointFixed || jt == kJointBall) {
+            for (int a2 = 0; a2 < nd; ++a2) qd_pub[ds + a2] = (float)vv[ds + a2];
+        } else {
+            for (int a2 = 0; a2 < nd; ++a2) {
+                const float qd = (float)vv[ds + a2];
+                const float qm = P.integ_mode == 0 ? 0.5f * ((float)v0v[ds + a2] + qd) : qd;
+                q_out[qs + a2] = q_in[qs + a2] + h * qm;
+                qd_pub[ds + a2] = qd;
+            }
+        }
+    }
+    wave_sync();
+    // public forward kinematics (sap_kinematics.py:84-229)
+    float* pbq = bq;
+    float* pbqd = kV;
+    float* kPl = reinterpret_cast<float*>(kXl);
+    for (int j = lane; j < nj; j += 64) {
+        const int jt = T.jtype[j];
+        const int qs = T.jqs[j], ds = T.jds[j];
+        TF Xj = TF{v3(0, 0, 0), Q4{0, 0, 0, 1}};
+        V3 jl = v3(0, 0, 0), ja = v3(0, 0, 0);
+        const float* axp = mAx + ds * 4;
+        const V3 axis = v3(axp[0], axp[1], axp[2]);
+        if (jt == kJointPrismatic) {
+            Xj.p = axis * q_out[qs];
+            jl = axis * qd_pub[ds];
+        } else if (jt == kJointRevolute) {
+            Xj.q = q_axis_angle(axis, q_out[qs]);
+            ja = axis * qd_pub[ds];
+        } else if (jt == kJointFree || jt == kJointDistance) {
+            Xj = TF{v3(q_out[qs], q_out[qs + 1], q_out[qs + 2]), Q4{q_out[qs + 3], q_out[qs + 4], q_out[qs + 5], q_out[qs + 6]}};
+            jl = v3(qd_pub[ds], qd_pub[ds + 1], qd_pub[ds + 2]);
+            ja = v3(qd_pub[ds + 3], qd_pub[ds + 4], qd_pub[ds + 5]);
+        }
+        const TF Xjc = tf_mul(Xj, tf_inverse(load_tf(mXc + j * 8)));
+        float* o = kPl + j * 16;
+        store_tf(o, Xjc);
+        o[7] = jl.x; o[8] = jl.y; o[9] = jl.z; o[10] = ja.x; o[11] = ja.y; o[12] = ja.z;
+    }
+    wave_sync();
+    for (int Lv = 0; Lv < M.num_levels; ++Lv) {
+        const int l0 = T.level_start[Lv], l1 = T.level_start[Lv + 1];
+        for (int t = l0 + lane; t < l1; t += 64) {
+            const int j = T.level_joint[t];
+            const int parent = T.jparent[j], child = T.jchild[j];
+            const TF Xp = load_tf(mXp + j * 8);
+            TF Xwpj = Xp;
+            V3 vlin = v3(0, 0, 0), vang = v3(0, 0, 0);
+            if (parent >= 0) {
+                const TF Xwp = load_tf(pbq + parent * 8);
+                Xwpj = tf_mul(Xwp, Xp);
+                const float* com = mCom + parent * 4;
+                const V3 rp = Xwpj.p - (Xwp.p + qrot(Xwp.q, v3(com[0], com[1], com[2])));
+                const float* vp = pbqd + parent * 6;
+                vang = v3(vp[3], vp[4], vp[5]);
+                vlin = v3(vp[0], vp[1], vp[2]) + cross(vang, rp);
+            }
+            const float* pl = kPl + j * 16;
+            const TF Xwc = tf_mul(Xwpj, load_tf(pl));
+            float* o = pbq + child * 8;
+            o[0] = Xwc.p.x; o[1] = Xwc.p.y; o[2] = Xwc.p.z; o[3] = Xwc.q.x; o[4] = Xwc.q.y; o[5] = Xwc.q.z; o[6] = Xwc.q.w;
+            const V3 lv = vlin + qrot(Xwpj.q, v3(pl[7], pl[8], pl[9]));
+            const V3 av3 = vang + qrot(Xwpj.q, v3(pl[10], pl[11], pl[12]));
+            float* ov = pbqd + child * 6;
+            ov[0] = lv.x; ov[1] = lv.y; ov[2] = lv.z; ov[3] = av3.x; ov[4] = av3.y; ov[5] = av3.z;
+        }
+        wave_sync();
+    }
+    // ------------------------------------------------------------------ outputs
+    for (int k = lane; k < M.NQ; k += 64) P.jq_out[(size_t)w * M.NQ + k] = q_out[k];
+    if (lane < n) {
+        const size_t g = (size_t)w * n + lane;
+        P.jqd_out[g] = qd_pub[lane];
+        P.v_guess[g] = v;
+        if (P.dbg_v) P.dbg_v[g] = v;
+        if (P.dbg_vstar) P.dbg_vstar[g] = vs;
+        if (P.dbg_v0) P.dbg_v0[g] = v0;
+        if (P.dbg_adiag) P.dbg_adiag[g] = adiag;
+    }
+    for (int t = lane; t < nb * 7; t += 64) {
+        const int b = t / 7, k = t % 7;
+        P.bq_out[((size_t)w * nb + b) * 7 + k] = pbq[b * 8 + k];
+    }
+    if (P.bqd_out)
+        for (int t = lane; t < nb * 6; t += 64) P.bqd_out[(size_t)w * nb * 6 + t] = pbqd[t];
+    if (P.bf_out && P.bf_in && P.bf_out != P.bf_in)
+        for (int t = lane; t < nb * 6; t += 64) P.bf_out[(size_t)w * nb * 6 + t] = P.bf_in[(size_t)w * nb * 6 + t];
+    if (P.dbg_gamma) {
+        for (int t = nc * 3 + lane; t < C * 3; t += 64) P.dbg_gamma[(size_t)w * C * 3 + t] = 0.0;
+        if (lane < nc) {
+            P.dbg_gamma[(size_t)w * C * 3 + lane * 3] = g0;
+            P.dbg_gamma[(size_t)w * C * 3 + lane * 3 + 1] = g1;
+            P.dbg_gamma[(size_t)w * C * 3 + lane * 3 + 2] = g2;
+        }
+    }
+    if (lane == 0) {
+        P.st_iters[w] = iteration;
+        P.st_ls[w] = ls_total;
+        P.st_nc[w] = nc;
+        P.st_flags[w] = flags | ((ls_status & 0xff) << 8);
+        if (P.st_hist) {
+            atomicAdd(&P.st_hist[iteration < 255 ? iteration : 255], 1ull);
+            atomicAdd(&P.st_hist[256], (unsigned long long)ls_total);
+            atomicAdd(&P.st_hist[257], 1ull);
+        }
+    }
+}
+
+template <int NP, int CB>
+hipError_t launch_variant(const StepParams& P, hipStream_t stream) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)sapx_wave_kernel<NP, CB>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((sapx_wave_kernel<NP, CB>), dim3(P.m.W), dim3(64), P.wl.bytes, stream, P);
+    return hipGetLastError();
+}
+
+}  // namespace wv
+}  // namespace sapx
+
+#endif  // SAPX_WAVE_IMPL_H
