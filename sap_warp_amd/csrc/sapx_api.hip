@@ -465,7 +465,7 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
                 o = round_up(o + bytes, 16);
                 return (int)r;
             };
-            L.js = (n % 2) ? n : n + 1;
+            L.js = NP + 1;  // zero-padded rows: the mat-vecs run over NP columns unguarded
             L.ls = NP + 1;
             L.o_topo = tk((size_t)dm.topo_ints * 4);
             L.o_mp = tk((size_t)(dm.J * 16 + n * 4 + nb * 4 + nb * 9 + nb) * 4);
@@ -480,7 +480,7 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
             // J rows then the factor rows; the kinematics scratch overlays both
             const size_t jbytes = round_up((size_t)3 * CB * L.js * 4, 16);
             const size_t ltbytes = ((size_t)NP * L.ls + 64) * 4;
-            const size_t kin = (size_t)(50 * nb + 14 * n + 2) * 4 + (size_t)dm.J * 64;
+            const size_t kin = (size_t)(50 * nb + 14 * n + 2) * 4 + (size_t)dm.J * 128;
             L.o_J = (int)o;
             L.o_Lt = (int)(o + jbytes);
             o = round_up(o + std::max(jbytes + ltbytes, kin), 16);

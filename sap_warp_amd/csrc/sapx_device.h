@@ -211,10 +211,12 @@ __device__ __forceinline__ DQ dqmul(DQ a, DQ b) {
     return DQ{a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y, a.w * b.y - a.x * b.z + a.y * b.w + a.z * b.x,
               a.w * b.z + a.x * b.y - a.y * b.x + a.z * b.w, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
 }
+// q / |q| through one reciprocal square root (within two ulps of the four divisions)
 __device__ __forceinline__ DQ dqnormalize(DQ q) {
-    double n = sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
-    if (n <= 0.0) return DQ{0.0, 0.0, 0.0, 1.0};
-    return DQ{q.x / n, q.y / n, q.z / n, q.w / n};
+    const double n2 = q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w;
+    if (n2 <= 0.0) return DQ{0.0, 0.0, 0.0, 1.0};
+    const double inv = rsqrt(n2);
+    return DQ{q.x * inv, q.y * inv, q.z * inv, q.w * inv};
 }
 __device__ __forceinline__ DQ dq_axis_angle(D3 a, double angle) {
     double sn, cs;
@@ -236,6 +238,11 @@ __device__ __forceinline__ DTF dtf_compose_n(DTF a, DTF b) {
     DQ aq = dqnormalize(a.q);
     DQ bq = dqnormalize(b.q);
     return DTF{dadd(a.p, dqrot_unit(aq, b.p)), dqnormalize(dqmul(aq, bq))};
+}
+// composition of transforms whose rotations are already unit (only the product is
+// renormalised): normalising a unit float64 quaternion again changes it by at most an ulp
+__device__ __forceinline__ DTF dtf_compose_u(DTF a, DTF b) {
+    return DTF{dadd(a.p, dqrot_unit(a.q, b.p)), dqnormalize(dqmul(a.q, b.q))};
 }
 __device__ __forceinline__ DTF dtf_inverse_n(DTF x) {
     DQ n = dqnormalize(x.q);

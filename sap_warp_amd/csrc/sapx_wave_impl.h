@@ -46,21 +46,72 @@ __device__ __forceinline__ double lds_row_dot(const float* arow, double x, int n
     return (s0 + s1) + (s2 + s3);
 }
 
+// One pass over the dofs: lane i gets (A x)_i from its LDS row, lane c gets the three
+// contact-Jacobian rows of contact c times x.  x_j is broadcast from lane j and must be
+// zero on lanes >= n; A and J rows are zero-padded to NP columns, so the loop is fully
+// unrolled without guards (every lane's loads are independent of the chain).
+template <int NP>
+__device__ __forceinline__ void matvec_AJ(const float* arow, const float* J0, int js, double x, double& ax,
+                                          double& jx0, double& jx1, double& jx2) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < NP; j += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double xj = readlane_d(x, j + u);
+            const double t = (double)arow[j + u] * xj;
+            if (u == 0) s0 += t; else if (u == 1) s1 += t; else if (u == 2) s2 += t; else s3 += t;
+            c0 += (double)J0[j + u] * xj;
+            c1 += (double)J0[js + j + u] * xj;
+            c2 += (double)J0[2 * js + j + u] * xj;
+        }
+    }
+    ax = (s0 + s1) + (s2 + s3);
+    jx0 = c0; jx1 = c1; jx2 = c2;
+}
+
+// (J^T y)_lane = sum_c sum_r J[3c+r][lane] y_{c,r}, y broadcast from lane c (zero for
+// c >= nc; J rows of unused slots are zero).  Contacts in groups of four.
+__device__ __forceinline__ double jt_times(const float* Jm, int js, int nc, int lane, double y0, double y1, double y2) {
+    double acc0 = 0.0, acc1 = 0.0;
+    const int col = lane < js ? lane : 0;
+    for (int c0 = 0; c0 < nc; c0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = c0 + u;
+            const float* Jc = Jm + (c * 3) * js + col;
+            const double t = (double)Jc[0] * readlane_d(y0, c) + (double)Jc[js] * readlane_d(y1, c) +
+                             (double)Jc[2 * js] * readlane_d(y2, c);
+            if (u & 1) acc1 += t; else acc0 += t;
+        }
+    }
+    return acc0 + acc1;
+}
+
 // Right-looking Cholesky of the register rows: h becomes L (row `lane`, lower,
 // strictly upper part zero), dinv = 1 / L_ii.
+// Steps and update columns beyond the dof count n are skipped in groups of eight (the
+// padding rows / columns are the identity and need no work).
 template <int NP>
-__device__ __forceinline__ void chol_rows(float (&h)[NP], float& dinv, int lane) {
+__device__ __forceinline__ void chol_rows(float (&h)[NP], float& dinv, int lane, int n) {
     dinv = 1.0f;
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
-        const float pj = fmaxf(rl(h[j], j), 1.0e-30f);
-        const float dj = __builtin_amdgcn_rsqf(pj);
-        float lij = h[j] * dj;
-        if (lane < j) lij = 0.0f;
-        h[j] = lij;
-        if (lane == j) dinv = dj;
+        if (j < n) {
+            const float pj = fmaxf(rl(h[j], j), 1.0e-30f);
+            const float dj = __builtin_amdgcn_rsqf(pj);
+            float lij = h[j] * dj;
+            if (lane < j) lij = 0.0f;
+            h[j] = lij;
+            if (lane == j) dinv = dj;
 #pragma unroll
-        for (int k = j + 1; k < NP; ++k) h[k] = fmaf(-lij, rl(lij, k), h[k]);
+            for (int k8 = ((j + 1) / 8) * 8; k8 < NP; k8 += 8) {
+                if (k8 < n) {
+#pragma unroll
+                    for (int k = (k8 > j + 1 ? k8 : j + 1); k < k8 + 8; ++k) h[k] = fmaf(-lij, rl(lij, k), h[k]);
+                }
+            }
+        }
     }
 }
 
@@ -197,6 +248,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     float* kExt = kFi + n * 6;
     float* kQdPub = kExt + nb * 6;
     double* kXl = reinterpret_cast<double*>(kQdPub + ((n + 1) & ~1));
+    double* kXpn = kXl + nj * 8;  // nj x 8: X_p with its rotation normalised (float64)
     float* mXp = mp;
     float* mXc = mXp + nj * 8;
     float* mAx = mXc + nj * 8;
@@ -334,6 +386,11 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         }
         store_dtf(kXl + j * 8, Xl);
         kXl[j * 8 + 7] = plain;
+        {
+            DTF Xpn = dtf_from(load_tf(mXp + j * 8));
+            Xpn.q = dqnormalize(Xpn.q);
+            store_dtf(kXpn + j * 8, Xpn);
+        }
         store_sv(kV + child * 6, vl);
         store_sv(kA + child * 6, al);
     }
@@ -345,10 +402,14 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
             const int parent = T.jparent[j];
             const int child = T.jchild[j];
             const int ds = T.jds[j], nd = T.jds[j + 1] - ds;
-            const DTF Xp_d = dtf_from(load_tf(mXp + j * 8));
-            const DTF Xwpj_d = parent >= 0 ? dtf_compose_n(load_dtf(bqd + parent * 8), Xp_d) : Xp_d;
+            // parent poses, normalised X_p and the local transforms are unit: only products are
+            // renormalised (the root's X_wpj is X_p as given, like the reference)
+            const DTF Xwpj_d = parent >= 0 ? dtf_compose_u(load_dtf(bqd + parent * 8), load_dtf(kXpn + j * 8))
+                                           : dtf_from(load_tf(mXp + j * 8));
             const DTF Xl = load_dtf(kXl + j * 8);
-            const DTF Xwc_d = (kXl[j * 8 + 7] != 0.0) ? DTF{Xwpj_d.p, dqmul(Xwpj_d.q, Xl.q)} : dtf_compose_n(Xwpj_d, Xl);
+            const DQ qwpj = parent >= 0 ? Xwpj_d.q : dqnormalize(Xwpj_d.q);
+            const DTF Xwc_d = (kXl[j * 8 + 7] != 0.0) ? DTF{Xwpj_d.p, dqmul(qwpj, Xl.q)}
+                                                      : dtf_compose_u(DTF{Xwpj_d.p, qwpj}, Xl);
             const Q4 R = tf_from(Xwpj_d).q;
             for (int k = 0; k < nd; ++k) {
                 const SV Sl = load_sv(Sg + (ds + k) * 6);
@@ -501,6 +562,8 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         if (lane == j) { val += arm_i; adiag = val; }
         if (lane < n) Am[lane * ls + j] = val;
     }
+    if (lane < n)
+        for (int j = n; j < NP; ++j) Am[lane * ls + j] = 0.0f;
     wave_sync();
 
     // ------------------------------------------------------------------ 5
@@ -509,7 +572,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     float dinv;
 #pragma unroll
     for (int j = 0; j < NP; ++j) hrow[j] = (lane < n) ? (j < n ? arow[j] : 0.0f) : ((lane == j) ? 1.0f : 0.0f);
-    chol_rows<NP>(hrow, dinv, lane);
+    chol_rows<NP>(hrow, dinv, lane, n);
     store_rows<NP>(hrow, Lt, ls, lane);
     wave_sync();
     const float x1 = chol_solve<NP>(hrow, Lt, ls, dinv, lane < n ? tau : 0.0f, lane);
@@ -539,8 +602,9 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         cb1 = b1g >= 0 ? b1g - w * nb : -1;
         D3 x0d = d3(P.cpoint0[idx * 3], P.cpoint0[idx * 3 + 1], P.cpoint0[idx * 3 + 2]);
         D3 x1d = d3(P.cpoint1[idx * 3], P.cpoint1[idx * 3 + 1], P.cpoint1[idx * 3 + 2]);
-        if (cb0 >= 0) x0d = dtf_point(load_dtf(bqd + cb0 * 8), x0d);
-        if (cb1 >= 0) x1d = dtf_point(load_dtf(bqd + cb1 * 8), x1d);
+        // body poses in bqd carry unit rotations (normalised in the level walk)
+        if (cb0 >= 0) { const DTF X = load_dtf(bqd + cb0 * 8); x0d = dadd(X.p, dqrot_unit(X.q, x0d)); }
+        if (cb1 >= 0) { const DTF X = load_dtf(bqd + cb1 * 8); x1d = dadd(X.p, dqrot_unit(X.q, x1d)); }
         D3 nd = d3(P.cnormal[idx * 3], P.cnormal[idx * 3 + 1], P.cnormal[idx * 3 + 2]);
         const double nd2 = nd.x * nd.x + nd.y * nd.y + nd.z * nd.z;
         nd = (nd2 <= 1.0e-32) ? d3(0.0, 0.0, 0.0) : dscale(nd, 1.0 / sqrt(nd2));
@@ -595,7 +659,11 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         const D3 gapv = dsub(x1d, x0d);
         phi = gapv.x * nd.x + gapv.y * nd.y + gapv.z * nd.z - (double)P.cmargin0[idx] - (double)P.cmargin1[idx];
     }
-    // contact Jacobian rows of contact `lane`: J_c[:, d] = R^T (u1 - u0)
+    // contact Jacobian rows of contact `lane`: J_c[:, d] = R^T (u1 - u0); every row and
+    // padding column of the region starts at zero, and G of unused slots is zero
+    for (int t = lane; t < 3 * CB * js; t += 64) Jm[t] = 0.0f;
+    for (int t = lane; t < CB * 9; t += 64) cG[t] = 0.0f;
+    wave_sync();
     vec[64 + lane] = (lane < n) ? ainv : 0.0;   // diag(A)^-1 by dof for the Delassus weights
     if (lane < nc) {
         const bool wit = P.point_mode == 0;
@@ -790,23 +858,10 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
             // ---- evaluate at v: vc = J v, A v; projection + model terms; impulses / gradient / norms
             double av;
             {
-                double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-                double c0 = 0.0, c1 = 0.0, c2 = 0.0;
                 const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
-                for (int j = 0; j < n; ++j) {
-                    const double xj = readlane_d(v, j);
-                    const double aij = (double)arow[j] * xj;
-                    if ((j & 3) == 0) s0 += aij;
-                    else if ((j & 3) == 1) s1 += aij;
-                    else if ((j & 3) == 2) s2 += aij;
-                    else s3 += aij;
-                    if (lane < nc) {
-                        c0 += (double)J0[j] * xj;
-                        c1 += (double)J0[js + j] * xj;
-                        c2 += (double)J0[2 * js + j] * xj;
-                    }
-                }
-                av = lane < n ? (s0 + s1) + (s2 + s3) : 0.0;
+                double c0, c1, c2;
+                matvec_AJ<NP>(arow, J0, js, v, av, c0, c1, c2);
+                if (lane >= n) av = 0.0;
                 vc0 = c0; vc1 = c1; vc2 = c2;
             }
             ar = av - avs;
@@ -820,12 +875,9 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
             double imp = 0.0, hdd = 0.0;
             if (lane < n) part4[0] += dof_terms(mt, v, dt, &imp, &hdd);
             hd = (float)hdd;
-            double jc = imp;
-            for (int c = 0; c < nc; ++c) {
-                const float* Jc = Jm + (c * 3) * js;
-                const double gc0 = readlane_d(g0, c), gc1 = readlane_d(g1, c), gc2 = readlane_d(g2, c);
-                if (lane < n) jc += (double)Jc[lane] * gc0 + (double)Jc[js + lane] * gc1 + (double)Jc[2 * js + lane] * gc2;
-            }
+            if (lane >= nc) { g0 = 0.0; g1 = 0.0; g2 = 0.0; }
+            const double jtg = jt_times(Jm, js, nc, lane, g0, g1, g2);
+            double jc = imp + (lane < n ? jtg : 0.0);
             grad = ar - jc;
             if (lane < n) {
                 part4[0] += 0.5 * (v - vs) * ar;
@@ -858,34 +910,44 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 break;
             }
             wave_sync();
-            // ---- Hessian rows H = A + J^T G J + diag(hd + shift); J^T G J by MFMA into LDS
+            // ---- Hessian rows H = A + J^T G J + diag(hd + shift); J^T G J by MFMA f32 16x16x4
+            //      into LDS: k-steps outer, all TB x TB tiles accumulate from one set of operands
             {
                 constexpr int TB = NP / 16;
+                typedef float f4 __attribute__((ext_vector_type(4)));
+                f4 acc[TB][TB];
+#pragma unroll
+                for (int ib = 0; ib < TB; ++ib)
+#pragma unroll
+                    for (int jb = 0; jb < TB; ++jb) acc[ib][jb] = f4{0.0f, 0.0f, 0.0f, 0.0f};
                 const int K = 3 * nc;
-                for (int tt = 0; tt < TB * TB; ++tt) {
-                    const int ib = tt / TB, jb = tt % TB;
-                    const int i0 = ib * 16, j0 = jb * 16;
-                    typedef float f4 __attribute__((ext_vector_type(4)));
-                    f4 accm = {0.0f, 0.0f, 0.0f, 0.0f};
-                    const int ia = i0 + (lane & 15);
-                    const int jbb = j0 + (lane & 15);
-                    for (int kk0 = 0; kk0 < K; kk0 += 4) {
-                        const int k = kk0 + (lane >> 4);
-                        float av_ = 0.0f, bv_ = 0.0f;
-                        if (k < K) {
-                            const int c = k / 3, r = k % 3;
-                            if (ia < n) av_ = Jm[k * js + ia];
-                            if (jbb < n) {
-                                const float* G = cG + c * 9 + r * 3;
-                                const float* Jc = Jm + (c * 3) * js + jbb;
-                                bv_ = G[0] * Jc[0] + G[1] * Jc[js] + G[2] * Jc[2 * js];
-                            }
-                        }
-                        accm = __builtin_amdgcn_mfma_f32_16x16x4f32(av_, bv_, accm, 0, 0, 0);
+                const int col = lane & 15;
+                for (int kk0 = 0; kk0 < K; kk0 += 4) {
+                    const int k = kk0 + (lane >> 4);  // k < 3 CB: padded rows are zero
+                    const int c = k / 3, r = k - 3 * (k / 3);
+                    const float* G = cG + c * 9 + r * 3;
+                    const float g0f = G[0], g1f = G[1], g2f = G[2];
+                    const float* Jk = Jm + k * js;
+                    const float* Jc = Jm + (c * 3) * js;
+                    float av_[TB], bv_[TB];
+#pragma unroll
+                    for (int t = 0; t < TB; ++t) {
+                        av_[t] = Jk[t * 16 + col];
+                        bv_[t] = g0f * Jc[t * 16 + col] + g1f * Jc[js + t * 16 + col] + g2f * Jc[2 * js + t * 16 + col];
                     }
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) Lt[(i0 + 4 * (lane >> 4) + e) * ls + j0 + (lane & 15)] = accm[e];
+                    for (int ib = 0; ib < TB; ++ib)
+#pragma unroll
+                        for (int jb = 0; jb < TB; ++jb)
+                            acc[ib][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av_[ib], bv_[jb], acc[ib][jb], 0, 0, 0);
                 }
+#pragma unroll
+                for (int ib = 0; ib < TB; ++ib)
+#pragma unroll
+                    for (int jb = 0; jb < TB; ++jb)
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            Lt[(ib * 16 + 4 * (lane >> 4) + e) * ls + jb * 16 + col] = acc[ib][jb][e];
             }
             wave_sync();
             const float hdiag = hd + (float)P.diag_shift;
@@ -895,25 +957,17 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 hrow[j] = (lane < n) ? (j < n ? arow[j] + hc + ((lane == j) ? hdiag : 0.0f) : 0.0f) : ((lane == j) ? 1.0f : 0.0f);
             }
             wave_sync();
-            chol_rows<NP>(hrow, dinv, lane);
+            chol_rows<NP>(hrow, dinv, lane, n);
             store_rows<NP>(hrow, Lt, ls, lane);
             wave_sync();
             float dvf = chol_solve<NP>(hrow, Lt, ls, dinv, lane < n ? (float)(-grad) : 0.0f, lane);
             dvv = lane < n ? (double)dvf : 0.0;
             // ---- mixed-precision refinement: r = -grad - (A + J^T G J + diag) dv in float64
             for (int rep = 0; rep < kNewtonRefine; ++rep) {
-                double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-                const double adv = lane < n ? lds_row_dot(arow, dvv, n) : 0.0;
+                double t0, t1, t2, adv;
                 {
                     const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
-                    for (int j = 0; j < n; ++j) {
-                        const double xj = readlane_d(dvv, j);
-                        if (lane < nc) {
-                            t0 += (double)J0[j] * xj;
-                            t1 += (double)J0[js + j] * xj;
-                            t2 += (double)J0[2 * js + j] * xj;
-                        }
-                    }
+                    matvec_AJ<NP>(arow, J0, js, dvv, adv, t0, t1, t2);
                 }
                 double u0 = 0.0, u1 = 0.0, u2 = 0.0;
                 if (lane < nc) {
@@ -922,12 +976,8 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                     u1 = (double)G[3] * t0 + (double)G[4] * t1 + (double)G[5] * t2;
                     u2 = (double)G[6] * t0 + (double)G[7] * t1 + (double)G[8] * t2;
                 }
-                double acc = adv + ((double)hd + P.diag_shift) * dvv;
-                for (int c = 0; c < nc; ++c) {
-                    const float* Jc = Jm + (c * 3) * js;
-                    const double uc0 = readlane_d(u0, c), uc1 = readlane_d(u1, c), uc2 = readlane_d(u2, c);
-                    if (lane < n) acc += (double)Jc[lane] * uc0 + (double)Jc[js + lane] * uc1 + (double)Jc[2 * js + lane] * uc2;
-                }
+                const double jtu = jt_times(Jm, js, nc, lane, u0, u1, u2);
+                const double acc = adv + ((double)hd + P.diag_shift) * dvv + jtu;
                 const double res = lane < n ? -grad - acc : 0.0;
                 const float corr = chol_solve<NP>(hrow, Lt, ls, dinv, (float)res, lane);
                 if (lane < n) dvv += (double)corr;
@@ -935,18 +985,10 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
             // ---- line search data (contact_solve.py:2234-2263)
             double ld[4] = {0, 0, 0, 0};  // dell0, dell_a0, d2ell_a, m0
             {
-                const double adv = lane < n ? lds_row_dot(arow, dvv, n) : 0.0;
-                double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+                double adv, d0, d1, d2;
                 {
                     const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
-                    for (int j = 0; j < n; ++j) {
-                        const double xj = readlane_d(dvv, j);
-                        if (lane < nc) {
-                            d0 += (double)J0[j] * xj;
-                            d1 += (double)J0[js + j] * xj;
-                            d2 += (double)J0[2 * js + j] * xj;
-                        }
-                    }
+                    matvec_AJ<NP>(arow, J0, js, dvv, adv, d0, d1, d2);
                 }
                 dvc0 = d0; dvc1 = d1; dvc2 = d2;
                 if (lane < n) {
