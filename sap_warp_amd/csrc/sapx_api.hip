@@ -86,6 +86,10 @@ struct SapxSolver {
     bool wave = false;
     WaveLayout wl_small{}, wl_big{};
     int cb_small = 0, cb_big = 0;
+    // the 64-slot variant runs on a second stream (default priority), forked from and joined
+    // back into the caller's stream, so its few long worlds overlap the 16-slot launch
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // grouping buffers for ungrouped contact lists
     int group_capacity = 0;
     int* g_keys = nullptr;
@@ -107,6 +111,9 @@ struct SapxSolver {
                         g_p0, g_p1, g_n, g_m0, g_m1, cub_tmp};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (aux) (void)hipStreamDestroy(aux);
     }
 };
 
@@ -473,7 +480,7 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
             L.o_bq = tk((size_t)nb * 8 * 4);
             L.o_bqd = tk((size_t)nb * 8 * 8);
             L.o_S = tk((size_t)n * 6 * 4);
-            L.o_vec = tk((size_t)128 * 8);
+            L.o_vec = tk((size_t)128 * 8);  // also the Cholesky panel broadcast (64 x float4)
             L.o_ck = tk((size_t)CB * 8 * 8);
             L.o_cG = tk((size_t)CB * 9 * 4);
             L.o_A = tk((size_t)n * L.ls * 4);
@@ -487,14 +494,25 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
             L.bytes = (int)o;
             return L;
         };
+        // SAPX_STEP_KERNEL: "fused" selects the 256-thread kernel, "wave64" sends every world
+        // through the 64-slot wave variant (both for tests and comparisons)
         const char* env = getenv("SAPX_STEP_KERNEL");
-        const bool want_wave = !(env && std::string(env) == "fused");
-        if (want_wave && NP <= 64 && C <= 64) {
-            s->cb_small = 16;
-            s->wl_small = layout(16);
-            s->cb_big = C > 16 ? 64 : 0;
+        const std::string kenv = env ? std::string(env) : std::string();
+        if (kenv != "fused" && NP <= 64 && C <= 64) {
+            const bool only_big = kenv == "wave64";
+            s->cb_small = only_big ? 0 : 16;
+            if (s->cb_small) s->wl_small = layout(16);
+            s->cb_big = (C > 16 || only_big) ? 64 : 0;
             if (s->cb_big) s->wl_big = layout(64);
-            s->wave = s->wl_small.bytes <= 160 * 1024 && (!s->cb_big || s->wl_big.bytes <= 160 * 1024);
+            s->wave = (!s->cb_small || s->wl_small.bytes <= 160 * 1024) && (!s->cb_big || s->wl_big.bytes <= 160 * 1024);
+        }
+        if (s->wave && s->cb_small && s->cb_big) {
+            if (hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) {
+                delete s;
+                return fail(SAPX_ERR_HIP, "stream / event creation failed for the solver");
+            }
         }
     }
     const size_t W = dm.W, D = (size_t)dm.W * n;
@@ -673,21 +691,33 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
     P.o_topo = s->lds[16];
     P.smem_bytes = s->smem_bytes;
     if (s->wave) {
-        auto launch = [&](int cb) {
+        auto launch_on = [&](const StepParams& Q, int cb, hipStream_t sq) {
             switch (s->npad) {
-                case 16: return launch_wave_np16(P, cb, st);
-                case 32: return launch_wave_np32(P, cb, st);
-                case 48: return launch_wave_np48(P, cb, st);
-                default: return launch_wave_np64(P, cb, st);
+                case 16: return launch_wave_np16(Q, cb, sq);
+                case 32: return launch_wave_np32(Q, cb, sq);
+                case 48: return launch_wave_np48(Q, cb, sq);
+                default: return launch_wave_np64(Q, cb, sq);
             }
         };
-        P.wl = s->wl_small;
-        P.wave_min_nc = 0;
-        HIP_TRY(launch(s->cb_small));
-        if (s->cb_big) {
-            P.wl = s->wl_big;
-            P.wave_min_nc = s->cb_small + 1;
-            HIP_TRY(launch(s->cb_big));
+        if (s->cb_small && s->cb_big) {
+            // fork: the worlds above 16 contacts start first on the second stream, the rest
+            // fill the remaining wave slots from the caller's stream; join before returning
+            StepParams Pb = P;
+            Pb.wl = s->wl_big;
+            Pb.wave_min_nc = s->cb_small + 1;
+            HIP_TRY(hipEventRecord(s->ev_fork, st));
+            HIP_TRY(hipStreamWaitEvent(s->aux, s->ev_fork, 0));
+            HIP_TRY(launch_on(Pb, s->cb_big, s->aux));
+            HIP_TRY(hipEventRecord(s->ev_join, s->aux));
+            P.wl = s->wl_small;
+            P.wave_min_nc = 0;
+            HIP_TRY(launch_on(P, s->cb_small, st));
+            HIP_TRY(hipStreamWaitEvent(st, s->ev_join, 0));
+        } else {
+            const int cb = s->cb_small ? s->cb_small : s->cb_big;
+            P.wl = s->cb_small ? s->wl_small : s->wl_big;
+            P.wave_min_nc = 0;
+            HIP_TRY(launch_on(P, cb, st));
         }
     } else {
         HIP_TRY(launch_step(P, st));

@@ -29,20 +29,21 @@ namespace wv {
 
 // ---------------------------------------------------------------------------
 // register-row dense kernels (lane i owns row i; rows / columns >= n are the
-// identity in H and zero in A; lanes >= NP hold zero rows)
+// identity in H and zero in A; lanes >= NP hold zero rows).
 // ---------------------------------------------------------------------------
 
-// sum_j A[row][j] x_j for the LDS row of this lane, x_j broadcast from lane j
-__device__ __forceinline__ double lds_row_dot(const float* arow, double x, int n) {
+// sum_j A[row][j] x_j for the LDS row of this lane (A zero-padded to NP columns);
+// x is lane j's value (zero on lanes >= n), broadcast by v_readlane
+template <int NP>
+__device__ __forceinline__ double lds_row_dot(const float* arow, double x) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int j = 0;
-    for (; j + 4 <= n; j += 4) {
+#pragma unroll
+    for (int j = 0; j < NP; j += 4) {
         s0 += (double)arow[j] * readlane_d(x, j);
         s1 += (double)arow[j + 1] * readlane_d(x, j + 1);
         s2 += (double)arow[j + 2] * readlane_d(x, j + 2);
         s3 += (double)arow[j + 3] * readlane_d(x, j + 3);
     }
-    for (; j < n; ++j) s0 += (double)arow[j] * readlane_d(x, j);
     return (s0 + s1) + (s2 + s3);
 }
 
@@ -88,28 +89,47 @@ __device__ __forceinline__ double jt_times(const float* Jm, int js, int nc, int 
     return acc0 + acc1;
 }
 
-// Right-looking Cholesky of the register rows: h becomes L (row `lane`, lower,
-// strictly upper part zero), dinv = 1 / L_ii.
-// Steps and update columns beyond the dof count n are skipped in groups of eight (the
-// padding rows / columns are the identity and need no work).
+// Right-looking Cholesky of the register rows in panels of four columns: h becomes L
+// (row `lane`, lower, strictly upper part zero), dinv = 1 / L_ii.  Inside a panel the
+// pivots and the panel's own columns use v_readlane; the panel is then broadcast
+// through `pan` (one float4 per row) and every trailing column k takes its four
+// updates from one broadcast read.  Panels and trailing columns beyond the dof count
+// n are skipped (the padding is the identity and needs no work).
 template <int NP>
-__device__ __forceinline__ void chol_rows(float (&h)[NP], float& dinv, int lane, int n) {
+__device__ __forceinline__ void chol_rows(float (&h)[NP], float& dinv, int lane, int n, float4* pan) {
     dinv = 1.0f;
 #pragma unroll
-    for (int j = 0; j < NP; ++j) {
-        if (j < n) {
-            const float pj = fmaxf(rl(h[j], j), 1.0e-30f);
-            const float dj = __builtin_amdgcn_rsqf(pj);
-            float lij = h[j] * dj;
-            if (lane < j) lij = 0.0f;
-            h[j] = lij;
-            if (lane == j) dinv = dj;
+    for (int j0 = 0; j0 < NP; j0 += 4) {
+        if (j0 < n) {
 #pragma unroll
-            for (int k8 = ((j + 1) / 8) * 8; k8 < NP; k8 += 8) {
-                if (k8 < n) {
+            for (int t = 0; t < 4; ++t) {
+                const int j = j0 + t;
+                const float pj = fmaxf(rl(h[j], j), 1.0e-30f);
+                const float dj = __builtin_amdgcn_rsqf(pj);
+                float lij = h[j] * dj;
+                if (lane < j) lij = 0.0f;
+                h[j] = lij;
+                if (lane == j) dinv = dj;
 #pragma unroll
-                    for (int k = (k8 > j + 1 ? k8 : j + 1); k < k8 + 8; ++k) h[k] = fmaf(-lij, rl(lij, k), h[k]);
+                for (int u = t + 1; u < 4; ++u) h[j0 + u] = fmaf(-lij, rl(lij, j0 + u), h[j0 + u]);
+            }
+            if (j0 + 4 < NP && j0 + 4 < n) {
+                pan[lane] = make_float4(h[j0], h[j0 + 1], h[j0 + 2], h[j0 + 3]);
+                wave_sync();
+#pragma unroll
+                for (int k4 = j0 + 4; k4 < NP; k4 += 4) {
+                    if (k4 < n) {
+#pragma unroll
+                        for (int k = k4; k < k4 + 4; ++k) {
+                            const float4 pk = pan[k];
+                            float a = fmaf(-h[j0], pk.x, h[k]);
+                            a = fmaf(-h[j0 + 1], pk.y, a);
+                            a = fmaf(-h[j0 + 2], pk.z, a);
+                            h[k] = fmaf(-h[j0 + 3], pk.w, a);
+                        }
+                    }
                 }
+                wave_sync();
             }
         }
     }
@@ -121,7 +141,7 @@ __device__ __forceinline__ float fwd_rows(const float (&h)[NP], float dinv, floa
     float y = b;
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
-        const float yj = rl(y, j) * rl(dinv, j);
+        const float yj = rl(y * dinv, j);
         y = (lane == j) ? yj : fmaf(-h[j], yj, y);
     }
     return y;
@@ -139,7 +159,7 @@ __device__ __forceinline__ float bwd_lds(const float* Lt, int ls, float dinv, fl
 #pragma unroll
         for (int t = 15; t >= 0; --t) {
             const int j = jb + t;
-            const float xj = rl(y, j) * rl(dinv, j);
+            const float xj = rl(y * dinv, j);
             y = (lane == j) ? xj : fmaf(-Lc[t], xj, y);
         }
     }
@@ -572,12 +592,13 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     float dinv;
 #pragma unroll
     for (int j = 0; j < NP; ++j) hrow[j] = (lane < n) ? (j < n ? arow[j] : 0.0f) : ((lane == j) ? 1.0f : 0.0f);
-    chol_rows<NP>(hrow, dinv, lane, n);
+    chol_rows<NP>(hrow, dinv, lane, n, reinterpret_cast<float4*>(vec));
     store_rows<NP>(hrow, Lt, ls, lane);
     wave_sync();
     const float x1 = chol_solve<NP>(hrow, Lt, ls, dinv, lane < n ? tau : 0.0f, lane);
     const double x1d = lane < n ? (double)x1 : 0.0;
-    const double r1 = lane < n ? (double)tau - lds_row_dot(arow, x1d, n) : 0.0;
+    const double ax1 = lds_row_dot<NP>(arow, x1d);
+    const double r1 = lane < n ? (double)tau - ax1 : 0.0;
     const float x2 = chol_solve<NP>(hrow, Lt, ls, dinv, (float)r1, lane);
     const double vs = lane < n ? v0 + dt * (x1d + (double)x2) : 0.0;
     double ainv = 1.0, dsc = 1.0;
@@ -841,7 +862,8 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         v = P.use_guess ? P.v_guess[(size_t)w * n + lane] : v0;
         if (unconstrained || !part) v = vs;
     }
-    const double avs = lane < n ? lds_row_dot(arow, vs, n) : 0.0;
+    double avs = lds_row_dot<NP>(arow, vs);
+    if (lane >= n) avs = 0.0;
     int iteration = 0, ls_total = 0, flags = 0, ls_status = 0;
     // lane-c contact state: vc, dvc, gamma (float64), G in LDS (cG)
     double vc0 = 0, vc1 = 0, vc2 = 0, dvc0 = 0, dvc1 = 0, dvc2 = 0, g0 = 0, g1 = 0, g2 = 0;
@@ -957,7 +979,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 hrow[j] = (lane < n) ? (j < n ? arow[j] + hc + ((lane == j) ? hdiag : 0.0f) : 0.0f) : ((lane == j) ? 1.0f : 0.0f);
             }
             wave_sync();
-            chol_rows<NP>(hrow, dinv, lane, n);
+            chol_rows<NP>(hrow, dinv, lane, n, reinterpret_cast<float4*>(vec));
             store_rows<NP>(hrow, Lt, ls, lane);
             wave_sync();
             float dvf = chol_solve<NP>(hrow, Lt, ls, dinv, lane < n ? (float)(-grad) : 0.0f, lane);

@@ -98,5 +98,12 @@ def test_humanoid_step_parity(gpu_device, preset, ls):
     _run_and_compare("humanoid", preset, ls, worlds=6, steps=5)
 
 
+@pytest.mark.parametrize("kernel", ["wave64", "fused"])
+def test_humanoid_step_parity_other_kernels(gpu_device, monkeypatch, kernel):
+    """The 64-slot wave variant alone (every world) and the 256-thread kernel, same bar."""
+    monkeypatch.setenv("SAPX_STEP_KERNEL", kernel)
+    _run_and_compare("humanoid", "approx32", "exact_root", worlds=6, steps=5)
+
+
 def test_ant_step_parity(gpu_device):
     _run_and_compare("ant", "approx32", "armijo_decay", worlds=8, steps=5)
