@@ -37,6 +37,7 @@ UNITS = {
     "sapx_wave32.hip": [],
     "sapx_wave48.hip": [],
     "sapx_wave64.hip": [],
+    "sapx_order.hip": [],
 }
 
 

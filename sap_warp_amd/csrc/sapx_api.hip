@@ -22,6 +22,7 @@ hipError_t launch_wave_np32(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_wave_np48(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_wave_np64(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_collide(const CollideParams& P, const CompactParams& Q, hipStream_t stream);
+hipError_t launch_world_order(const int* cworld_start, const int* iters, int W, int C, int* order, hipStream_t stream);
 hipError_t launch_group_contacts(const GroupParams& G, hipStream_t stream);
 }  // namespace sapx
 
@@ -90,6 +91,8 @@ struct SapxSolver {
     // back into the caller's stream, so its few long worlds overlap the 16-slot launch
     hipStream_t aux = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // most-expensive-first dispatch order of the worlds (wave kernels), rebuilt every step
+    int* order = nullptr;
     // grouping buffers for ungrouped contact lists
     int group_capacity = 0;
     int* g_keys = nullptr;
@@ -108,7 +111,7 @@ struct SapxSolver {
     size_t cub_tmp_bytes = 0;
     ~SapxSolver() {
         void* ptrs[] = {v_guess, st, hist, dbg, g_keys, g_keys_out, g_vals, g_vals_out, g_world_start, g_shape0, g_shape1,
-                        g_p0, g_p1, g_n, g_m0, g_m1, cub_tmp};
+                        g_p0, g_p1, g_n, g_m0, g_m1, cub_tmp, order};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -506,6 +509,12 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
             if (s->cb_big) s->wl_big = layout(64);
             s->wave = (!s->cb_small || s->wl_small.bytes <= 160 * 1024) && (!s->cb_big || s->wl_big.bytes <= 160 * 1024);
         }
+        // SAPX_WORLD_ORDER=0 keeps the identity order (comparisons)
+        const char* oenv = getenv("SAPX_WORLD_ORDER");
+        if (s->wave && !(oenv && std::string(oenv) == "0") && hipMalloc(&s->order, (size_t)dm.W * sizeof(int)) != hipSuccess) {
+            delete s;
+            return fail(SAPX_ERR_HIP, "hipMalloc failed for the world order");
+        }
         if (s->wave && s->cb_small && s->cb_big) {
             if (hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
@@ -699,6 +708,11 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
                 default: return launch_wave_np64(Q, cb, sq);
             }
         };
+        P.world_order = nullptr;
+        if (s->order) {
+            HIP_TRY(launch_world_order(P.cworld_start, P.st_iters, (int)dm.W, P.C, s->order, st));
+            P.world_order = s->order;
+        }
         if (s->cb_small && s->cb_big) {
             // fork: the worlds above 16 contacts start first on the second stream, the rest
             // fill the remaining wave slots from the caller's stream; join before returning

@@ -65,6 +65,7 @@ struct StepParams {
     int smem_bytes;
     WaveLayout wl;       // one-wave kernel layout for this launch's contact variant
     int wave_min_nc;     // the variant processes worlds with wave_min_nc <= nc <= its capacity
+    const int* world_order;  // blockIdx -> world (a permutation of 0..W-1), or null for the identity
 };
 
 struct CollideParams {

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     const DevModel& M = P.m;
     const WaveLayout& L = P.wl;
-    const int w = blockIdx.x;
+    const int w = P.world_order ? P.world_order[blockIdx.x] : (int)blockIdx.x;
     const int lane = threadIdx.x;
     const int n = M.N, nb = M.NB, nj = M.J;
     const int js = L.js, ls = L.ls;
