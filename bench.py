@@ -98,6 +98,152 @@ def cpu_baseline(worlds: int, steps: int, seed: int, dt: float, line_search: str
     return worlds * steps / el, el
 
 
+def _scene_and_solver(W, C, seed, dev, line_search):
+    from sap_warp_amd.collision import SapCollisionPipeline
+    from sap_warp_amd.scenes import build_humanoid_scene
+    from sap_warp_amd.solver_sap import SolverSAP
+
+    model = build_humanoid_scene(W, seed=seed, device=dev)
+    solver = SolverSAP(model, max_rigid_contact=C, line_search_variant=line_search)
+    pipe = SapCollisionPipeline(model, max_contacts_per_world=C)
+    return model, solver, pipe
+
+
+def _target_walk(model, W, steps, seed, dev):
+    """PD-target random walk, sigma = 0.05 rad per step on the revolute dofs, precomputed on the GPU."""
+    import torch
+
+    n = model.dof_per_env
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1234 + seed)
+    mask = torch.zeros(n, device=dev)
+    mask[6:] = 1.0
+    incr = torch.randn((steps, W, n), generator=gen, device=dev) * 0.05 * mask
+    return incr.reshape(steps, -1).contiguous()
+
+
+def train_learned(args, W, C, dt, dev):
+    """Training set from plain rollouts of held-in seeds, then GPU training of the warm-start MLP."""
+    import torch
+
+    from sap_warp_amd.learned import LearnedWarmStart
+
+    t0 = time.perf_counter()
+    seeds = [args.seed + 100 + k for k in range(args.train_seeds)]
+    lws = None
+    xs, ys = [], []
+    for sd in seeds:
+        model, solver, pipe = _scene_and_solver(W, C, sd, dev, args.line_search)
+        contacts = pipe.contacts()
+        ctl = model.control()
+        s0, s1 = model.state(), model.state()
+        incr = _target_walk(model, W, args.train_steps, sd, dev)
+        obj = LearnedWarmStart(solver, seed=0, mlp=None if lws is None else lws.mlp)
+        lws = obj if lws is None else lws
+
+        def ctl_fn(k):
+            ctl.joint_target_pos.add_(incr[k])
+
+        x, y, _, _ = obj.collect(pipe, s0, s1, ctl, contacts, args.train_steps, dt, control_fn=ctl_fn)
+        xs.append(x)
+        ys.append(y)
+        del obj
+    x = torch.cat(xs)
+    y = torch.cat(ys)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    lws.fit(x, y, epochs=args.train_epochs, batch=4096, lr=args.train_lr, lr_final=args.train_lr * 0.1)
+    info = {"samples": int(x.shape[0]), "seeds": seeds, "steps_per_seed": args.train_steps,
+            "epochs": args.train_epochs, "adam_steps": lws.train_steps, "final_loss": round(lws.final_loss, 5),
+            "data_seconds": round(t_data, 2), "train_seconds": round(lws.train_seconds, 2),
+            "held_out_seed": args.seed, "network": {"dims": lws.mlp.dims, "features": lws.F}}
+    del x, y, xs, ys
+    torch.cuda.empty_cache()
+    return lws, info
+
+
+def timed_run(args, W, C, dt, dev, lws_trained):
+    """Warmup + timed steps on the benchmark seed, plain (lws_trained None) or from the learned start."""
+    import numpy as np
+    import torch
+
+    from sap_warp_amd.learned import LearnedWarmStart
+
+    model, solver, pipe = _scene_and_solver(W, C, args.seed, dev, args.line_search)
+    lws = None
+    if lws_trained is not None:
+        lws = LearnedWarmStart(solver, mlp=lws_trained.mlp)
+    contacts = pipe.contacts()
+    s0, s1 = model.state(), model.state()
+    ctl = model.control()
+    total = args.warmup + args.steps
+    incr = _target_walk(model, W, total, args.seed, dev)
+    contact_acc = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def solve(a, b):
+        if lws is None:
+            solver.step(a, b, ctl, contacts, dt)
+        else:
+            lws.prepare(a, ctl, contacts)
+            solver.step(a, b, ctl, contacts, dt)
+            lws.finish()
+
+    for k in range(args.warmup):
+        ctl.joint_target_pos.add_(incr[k])
+        pipe.collide(s0, contacts)
+        solve(s0, s1)
+        s0, s1 = s1, s0
+    torch.cuda.synchronize()
+    solver.iteration_histogram(reset=True)
+    ev_start = torch.cuda.Event(enable_timing=True)
+    ev_end = torch.cuda.Event(enable_timing=True)
+    ev_c = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ev_p = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ev_s = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ev_e = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    ev_start.record()
+    for i in range(args.steps):
+        k = args.warmup + i
+        ctl.joint_target_pos.add_(incr[k])
+        ev_c[i].record()
+        pipe.collide(s0, contacts)
+        ev_p[i].record()
+        if lws is not None:
+            lws.prepare(s0, ctl, contacts)
+        ev_s[i].record()
+        solver.step(s0, s1, ctl, contacts, dt)
+        ev_e[i].record()
+        if lws is not None:
+            lws.finish()
+        contact_acc.add_(contacts.rigid_contact_count)
+        s0, s1 = s1, s0
+    ev_end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall0
+    ms_total = ev_start.elapsed_time(ev_end)
+    step_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_s, ev_e)]))
+    pre_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_p, ev_s)]))
+    collide_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_c, ev_p)]))
+    hist = solver.iteration_histogram(reset=True)
+    ws = int(hist[257])
+    iters = hist[:256].astype(np.float64)
+    mean_it = float((iters * np.arange(256)).sum() / max(ws, 1))
+    cdf = np.cumsum(iters) / max(ws, 1)
+    stats = solver.iteration_stats()
+    return {
+        "model": model, "n": model.dof_per_env, "nb": model.bodies_per_env,
+        "value": W * args.steps / (ms_total / 1000.0), "ms_per_step": ms_total / args.steps,
+        "step_ms": step_ms, "pre_ms": pre_ms, "collide_ms": collide_ms, "wall": wall,
+        "mean_it": mean_it, "p99_it": int(np.searchsorted(cdf, 0.99)), "ls_per_ws": float(hist[256]) / max(ws, 1),
+        "world_steps": ws, "taken": int(hist[258]), "rejected": int(hist[259]),
+        "mean_nc": float(contact_acc.item()) / (args.steps * W),
+        "converged_frac": float(np.mean(stats.converged_env == 1)),
+        "S_world": (model.shape_count - 1) // W, "P_world": model.shape_pairs.shape[0] // W,
+    }
+
+
 def copy_calibration() -> int:
     """Three copies of a 512 MiB buffer: the HBM counters of a kernel of known traffic."""
     import torch
@@ -122,6 +268,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-worlds", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=24)
+    ap.add_argument("--learned", type=int, default=1, help="train and time the learned warm start (1) or not (0)")
+    ap.add_argument("--train-seeds", type=int, default=2)
+    ap.add_argument("--train-steps", type=int, default=250)
+    ap.add_argument("--train-epochs", type=int, default=12)
+    ap.add_argument("--train-lr", type=float, default=1e-3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 512 MiB device copies (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -135,116 +286,75 @@ def main() -> int:
     import numpy as np
     import torch
 
-    from sap_warp_amd.collision import SapCollisionPipeline
-    from sap_warp_amd.scenes import build_humanoid_scene
-    from sap_warp_amd.solver_sap import SolverSAP
-
     dev = "cuda:0"
     torch.cuda.set_device(0)
     dt = 2.0e-3
     C = 64
     W = args.worlds
-    model = build_humanoid_scene(W, seed=args.seed, device=dev)
-    solver = SolverSAP(model, max_rigid_contact=C, line_search_variant=args.line_search)
-    pipe = SapCollisionPipeline(model, max_contacts_per_world=C)
-    contacts = pipe.contacts()
-    s0, s1 = model.state(), model.state()
-    ctl = model.control()
-    n = model.dof_per_env
-    total = args.warmup + args.steps
-    # PD target random walk, sigma = 0.05 rad/step on the revolute dofs, precomputed on the GPU
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(1234 + args.seed)
-    mask = torch.zeros(n, device=dev)
-    mask[6:] = 1.0
-    incr = torch.randn((total, W, n), generator=gen, device=dev) * 0.05 * mask
-    incr = incr.reshape(total, -1).contiguous()
-    contact_acc = torch.zeros(1, dtype=torch.int64, device=dev)
 
-    def one_step(k, a, b):
-        ctl.joint_target_pos.add_(incr[k])
-        pipe.collide(a, contacts)
-        solver.step(a, b, ctl, contacts, dt)
-
-    for k in range(args.warmup):
-        one_step(k, s0, s1)
-        s0, s1 = s1, s0
-    torch.cuda.synchronize()
-    solver.iteration_histogram(reset=True)
-    ev_start = torch.cuda.Event(enable_timing=True)
-    ev_end = torch.cuda.Event(enable_timing=True)
-    ev_s = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ev_e = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ev_c = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t_wall0 = time.perf_counter()
-    ev_start.record()
-    for i in range(args.steps):
-        k = args.warmup + i
-        ctl.joint_target_pos.add_(incr[k])
-        ev_c[i].record()
-        pipe.collide(s0, contacts)
-        ev_s[i].record()
-        solver.step(s0, s1, ctl, contacts, dt)
-        ev_e[i].record()
-        contact_acc.add_(contacts.rigid_contact_count)
-        s0, s1 = s1, s0
-    ev_end.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_wall0
-    ms_total = ev_start.elapsed_time(ev_end)
-    step_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_s, ev_e)]))
-    collide_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(ev_c, ev_s)]))
-    hist = solver.iteration_histogram(reset=True)
-    ws = int(hist[257])
-    iters = hist[:256].astype(np.float64)
-    mean_it = float((iters * np.arange(256)).sum() / max(ws, 1))
-    cdf = np.cumsum(iters) / max(ws, 1)
-    p99_it = int(np.searchsorted(cdf, 0.99))
-    ls_per_ws = float(hist[256]) / max(ws, 1)
-    mean_nc = float(contact_acc.item()) / (args.steps * W)
-    stats = solver.iteration_stats()
-    converged_frac = float(np.mean(stats.converged_env == 1))
-    value = W * args.steps / (ms_total / 1000.0)
-    # roofline of the dominant kernel, the step (the wave kernel pair sapx_step launches back to back on
-    # the solver's stream: the 16-contact variant, then the 64-contact variant for the worlds above 16):
-    # algorithmic FP32-class operations per step over the HIP-event time of the pair
-    nb = model.bodies_per_env
-    flops_launch = W * step_flops(n, nb, mean_nc, mean_it, ls_per_ws)
-    achieved_tflops = flops_launch / (step_ms / 1000.0) / 1.0e12
-    S_world = (model.shape_count - 1) // W
-    P_world = model.shape_pairs.shape[0] // W
-    cbytes = collide_bytes(model, mean_nc * W, S_world, P_world)
-    collide_gbs = cbytes / (collide_ms / 1000.0) / 1.0e9
+    lws = None
+    train_info = None
+    if args.learned:
+        lws, train_info = train_learned(args, W, C, dt, dev)
+    plain = timed_run(args, W, C, dt, dev, None)
+    learned = timed_run(args, W, C, dt, dev, lws) if lws is not None else None
+    head = plain  # the benchmark workload is plain SAP; the learned start is reported beside it
+    n, nb = head["n"], head["nb"]
+    # roofline of the dominant kernel, the step (the wave kernel pair sapx_step launches on the
+    # solver's stream and a forked second stream: the 16-contact variant, and the 64-contact
+    # variant for the worlds above 16): algorithmic FP32-class operations per step over the
+    # HIP-event time of the pair
+    flops_launch = W * step_flops(n, nb, head["mean_nc"], head["mean_it"], head["ls_per_ws"])
+    achieved_tflops = flops_launch / (head["step_ms"] / 1000.0) / 1.0e12
+    cbytes = collide_bytes(head["model"], head["mean_nc"] * W, head["S_world"], head["P_world"])
+    collide_gbs = cbytes / (head["collide_ms"] / 1000.0) / 1.0e9
+    kernel_ms = {"sapx_step": round(head["step_ms"], 4), "collide_total": round(head["collide_ms"], 4)}
     out = {
         "metric": "world-steps/s at 4096 humanoid worlds; mean Newton iters, learned vs plain",
-        "value": round(value, 1),
+        "value": round(head["value"], 1),
         "unit": "world-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_total / args.steps, 4),
+        "ms_per_step": round(head["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32/f64",
         "data": "synthetic (seeded humanoid batch, random-walk PD targets)",
-        "config": {"workload": "batched plain SAP, 4096 humanoid worlds (35 DOF), dt=2e-3, approx32 preset, "
-                               f"{args.line_search} line search, C=64 contacts/world",
+        "config": {"workload": "batched SAP, 4096 humanoid worlds (35 DOF), dt=2e-3, approx32 preset, "
+                               f"{args.line_search} line search, C=64 contacts/world, plain warm start "
+                               "(previous step's solution)",
                    "worlds": W, "dof_per_world": n, "bodies_per_world": nb, "dt": dt, "steps_timed": args.steps},
-        "newton_iterations": {"mean": round(mean_it, 3), "p99": p99_it, "line_search_trials_per_world_step":
-                              round(ls_per_ws, 3), "world_steps": ws, "warm_start": "previous step (plain)"},
-        "mean_contacts_per_world": round(mean_nc, 3),
-        "converged_fraction_last_step": converged_frac,
-        "kernel_ms": {"sapx_step": round(step_ms, 4), "collide_total": round(collide_ms, 4)},
+        "newton_iterations": {"mean": round(head["mean_it"], 3), "p99": head["p99_it"],
+                              "line_search_trials_per_world_step": round(head["ls_per_ws"], 3),
+                              "world_steps": head["world_steps"],
+                              "warm_start": "previous step (plain)"},
+        "mean_contacts_per_world": round(head["mean_nc"], 3),
+        "converged_fraction_last_step": head["converged_frac"],
+        "kernel_ms": kernel_ms,
         "roofline": {"bound": "mfma", "achieved": round(achieved_tflops, 4), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 6),
                      "traffic": read_traffic(), "kernel": STEP_KERNELS,
                      "flops_per_launch": round(flops_launch)},
         "roofline_collide": {"bound": "hbm", "achieved": round(collide_gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": round(collide_gbs / PEAK_HBM_GBS, 6), "bytes_per_launch": round(cbytes)},
-        "wall_s": round(wall, 3),
+        "wall_s": round(head["wall"], 3),
     }
+    if learned is not None:
+        taken, rejected = learned["taken"], learned["rejected"]
+        out["learned"] = {
+            "value": round(learned["value"], 1),
+            "ms_per_step": round(learned["ms_per_step"], 4),
+            "step_ms": round(learned["step_ms"], 4),
+            "features_mlp_ms": round(learned["pre_ms"], 4),
+            "speedup_vs_plain": round(learned["value"] / plain["value"], 4),
+            "mean_newton_iterations": round(learned["mean_it"], 3),
+            "p99_newton_iterations": learned["p99_it"],
+            "iteration_reduction": round(1.0 - learned["mean_it"] / max(plain["mean_it"], 1e-12), 4),
+            "fallback_rate": round(rejected / max(taken + rejected, 1), 4),
+            "train": train_info,
+        }
     if not args.no_cpu_baseline:
         v_cpu, el = cpu_baseline(args.cpu_worlds, args.cpu_steps, args.seed, dt, args.line_search)
         out["cpu_baseline"] = {"value": round(v_cpu, 3), "unit": "world-steps/s", "cores": 1, "kind": "port",

@@ -218,7 +218,8 @@ int32_t sapx_get_stats(SapxSolver* solver, int32_t* newton_iterations, int32_t* 
  * Accumulated Newton-iteration histogram over every world-step since the last
  * reset, copied to HOST: hist[0..255] = number of world-steps that took that
  * many Newton iterations (255 = 255 or more), hist[256] = total line-search
- * iterations, hist[257] = world-steps counted.  The reference has no
+ * iterations, hist[257] = world-steps counted, hist[258] / hist[259] =
+ * learned warm starts taken / rejected (260 entries).  The reference has no
  * counterpart (its per-world counters, contact_solve.py:224-229, are
  * overwritten every step); it serves the iteration statistics of the
  * benchmark without a per-step synchronisation.  Synchronises the stream;
@@ -249,6 +250,78 @@ void sapx_pipeline_destroy(SapxPipeline* pipeline);
  * (deterministic).  body_q: device, body_count x 7.
  */
 int32_t sapx_collide(SapxPipeline* pipeline, const float* body_q, const SapxContactsView* contacts, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Learned warm start.  No counterpart in the reference: an MLP predicts a
+ * correction of SolverSAP's plain initial velocity (the previous step's
+ * solution, or v0) from state and contact features; the step starts from the
+ * prediction only where its SAP cost is lower than the plain start's, and the
+ * Newton solve runs to the unchanged tolerance either way.  The network is
+ * trained on the GPU by the library's own forward / backward / Adam kernels.
+ * ------------------------------------------------------------------------- */
+
+/* Input features per world (the layout is fixed by the model's topology). */
+int32_t sapx_solver_feature_count(const SapxSolver* solver);
+
+/*
+ * Features of every world into features (device, world_count x feature_count
+ * float32): free joints (height, gravity direction and velocities in the body
+ * frame), per-dof q / qd / target errors, per-body contact count, minimum
+ * signed distance and summed normal, then the previous step's correction
+ * prev_target (device, world_count x dof_per_world, as target_out of
+ * sapx_solver_set_warm_start_io wrote it; NULL = zeros).  contacts must be
+ * world-grouped (world_start set), as sapx_collide writes them.
+ */
+int32_t sapx_features(SapxSolver* solver, const SapxStateView* state_in, const SapxControlView* control,
+                      const SapxContactsView* contacts, const float* prev_target, float* features, void* stream);
+
+/*
+ * Buffers used by the following sapx_step calls (device, world_count x
+ * dof_per_world float32, or NULL): start_correction is added to the plain
+ * start (free-joint rows in the joint's body frame) and taken where it lowers
+ * the SAP cost; target_out receives converged v minus the plain start in the
+ * same convention (training data).  Requires the one-wave step kernel.
+ * The histogram of sapx_get_histogram counts learned starts taken (hist[258])
+ * and rejected (hist[259]).
+ */
+int32_t sapx_solver_set_warm_start_io(SapxSolver* solver, const float* start_correction, float* target_out);
+
+typedef struct SapxMlp SapxMlp;
+
+/* buffer ids of sapx_mlp_write / sapx_mlp_read */
+#define SAPX_MLP_PARAMS 0 /* flat float32 parameters: per layer W (out x in, row-major) then b */
+#define SAPX_MLP_X_MEAN 1 /* input normalisation: x_n = (x - mean) / std, in_features each   */
+#define SAPX_MLP_X_STD 2
+#define SAPX_MLP_Y_MEAN 3 /* output normalisation: y = y_n * std + mean, out_features each  */
+#define SAPX_MLP_Y_STD 4
+#define SAPX_MLP_GRADS 5  /* gradients of the last sapx_mlp_gradients call (read only)      */
+
+/*
+ * ReLU MLP with `layers` linear layers of padded widths dims[0..layers]
+ * (multiples of 64); in_features / out_features are the real widths.  Weights
+ * start at zero: write them with sapx_mlp_write(SAPX_MLP_PARAMS).
+ */
+int32_t sapx_mlp_create(int32_t layers, const int32_t* dims, int32_t in_features, int32_t out_features,
+                        SapxMlp** out);
+void sapx_mlp_destroy(SapxMlp* mlp);
+int64_t sapx_mlp_param_count(const SapxMlp* mlp);
+/* Host -> device copy of a buffer; writing the parameters also resets the Adam moments. */
+int32_t sapx_mlp_write(SapxMlp* mlp, int32_t which, const float* host, int64_t count);
+/* Device -> host copy (synchronises the device). */
+int32_t sapx_mlp_read(const SapxMlp* mlp, int32_t which, float* host, int64_t count);
+/* Pre-size the workspace for batches up to `batch` rows (otherwise grown on first use). */
+int32_t sapx_mlp_reserve(SapxMlp* mlp, int32_t batch);
+/* y (device, batch x out_features) = network(x (device, batch x in_features)), de-normalised. */
+int32_t sapx_mlp_forward(SapxMlp* mlp, const float* x, int32_t batch, float* y, void* stream);
+/*
+ * Gradients of the mean squared error (normalised output units, real outputs
+ * only) against target (device, batch x out_features); loss (device float, or
+ * NULL) receives the loss.
+ */
+int32_t sapx_mlp_gradients(SapxMlp* mlp, const float* x, const float* target, int32_t batch, float* loss,
+                           void* stream);
+/* One bias-corrected Adam step with the last gradients. */
+int32_t sapx_mlp_adam(SapxMlp* mlp, float lr, float beta1, float beta2, float eps, void* stream);
 
 #ifdef __cplusplus
 }

@@ -105,7 +105,28 @@ EXPORTS = {
     "sapx_pipeline_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
     "sapx_pipeline_destroy": (None, [_P]),
     "sapx_collide": (_I, [_P, _P, ctypes.POINTER(SapxContactsView), _P]),
+    "sapx_solver_feature_count": (_I, [_P]),
+    "sapx_features": (_I, [_P, ctypes.POINTER(SapxStateView), ctypes.POINTER(SapxControlView),
+                           ctypes.POINTER(SapxContactsView), _P, _P, _P]),
+    "sapx_solver_set_warm_start_io": (_I, [_P, _P, _P]),
+    "sapx_mlp_create": (_I, [_I, _P, _I, _I, ctypes.POINTER(_P)]),
+    "sapx_mlp_destroy": (None, [_P]),
+    "sapx_mlp_param_count": (ctypes.c_int64, [_P]),
+    "sapx_mlp_write": (_I, [_P, _I, _P, ctypes.c_int64]),
+    "sapx_mlp_read": (_I, [_P, _I, _P, ctypes.c_int64]),
+    "sapx_mlp_reserve": (_I, [_P, _I]),
+    "sapx_mlp_forward": (_I, [_P, _P, _I, _P, _P]),
+    "sapx_mlp_gradients": (_I, [_P, _P, _P, _I, _P, _P]),
+    "sapx_mlp_adam": (_I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
 }
+
+HIST_SIZE = 260  # sapx_get_histogram entries
+MLP_PARAMS = 0
+MLP_X_MEAN = 1
+MLP_X_STD = 2
+MLP_Y_MEAN = 3
+MLP_Y_STD = 4
+MLP_GRADS = 5
 
 _lib = None
 

@@ -38,6 +38,8 @@ UNITS = {
     "sapx_wave48.hip": [],
     "sapx_wave64.hip": [],
     "sapx_order.hip": [],
+    "sapx_learned.hip": [],
+    "sapx_mlp.hip": [],
 }
 
 

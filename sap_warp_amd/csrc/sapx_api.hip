@@ -23,7 +23,9 @@ hipError_t launch_wave_np48(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_wave_np64(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_collide(const CollideParams& P, const CompactParams& Q, hipStream_t stream);
 hipError_t launch_world_order(const int* cworld_start, const int* iters, int W, int C, int* order, hipStream_t stream);
+hipError_t launch_features(const FeatureParams& P, hipStream_t stream);
 hipError_t launch_group_contacts(const GroupParams& G, hipStream_t stream);
+void set_last_error(const char* msg);
 }  // namespace sapx
 
 using namespace sapx;
@@ -31,6 +33,14 @@ using namespace sapx;
 namespace {
 
 thread_local std::string g_last_error = "";
+// device histogram: iteration bins 0..255, line searches, world-steps, learned starts taken / rejected
+constexpr int kHistSize = 260;
+}  // namespace
+
+// error text of the other translation units (sapx_mlp.hip)
+void sapx::set_last_error(const char* msg) { g_last_error = msg; }
+
+namespace {
 
 int32_t fail(int32_t code, const std::string& msg) {
     g_last_error = msg;
@@ -76,7 +86,7 @@ struct SapxSolver {
     double* v_guess = nullptr;
     int use_guess = 0;
     int* st = nullptr;  // 4 x W
-    unsigned long long* hist = nullptr;  // 258 accumulators
+    unsigned long long* hist = nullptr;  // kHistSize accumulators
     double* dbg = nullptr;
     size_t dbg_off[7] = {0};
     size_t dbg_size[7] = {0};
@@ -93,6 +103,12 @@ struct SapxSolver {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // most-expensive-first dispatch order of the worlds (wave kernels), rebuilt every step
     int* order = nullptr;
+    // learned warm start: feature layout and the caller's correction / target buffers
+    int feat_count = 0;
+    int frame_qs = -1;
+    int* feat_off = nullptr;  // device, J+1
+    const float* learn_delta = nullptr;
+    float* learn_target = nullptr;
     // grouping buffers for ungrouped contact lists
     int group_capacity = 0;
     int* g_keys = nullptr;
@@ -111,7 +127,7 @@ struct SapxSolver {
     size_t cub_tmp_bytes = 0;
     ~SapxSolver() {
         void* ptrs[] = {v_guess, st, hist, dbg, g_keys, g_keys_out, g_vals, g_vals_out, g_world_start, g_shape0, g_shape1,
-                        g_p0, g_p1, g_n, g_m0, g_m1, cub_tmp, order};
+                        g_p0, g_p1, g_n, g_m0, g_m1, cub_tmp, order, feat_off};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -144,6 +160,50 @@ extern "C" {
 int32_t sapx_version(void) { return 100; }
 
 const char* sapx_last_error(void) { return g_last_error.c_str(); }
+
+int32_t sapx_solver_feature_count(const SapxSolver* s) { return s ? s->feat_count : 0; }
+
+int32_t sapx_features(SapxSolver* s, const SapxStateView* in, const SapxControlView* ctl, const SapxContactsView* c,
+                      const float* prev_target, float* features, void* stream) {
+    if (!s || !in || !features) return fail(SAPX_ERR_INVALID, "null argument");
+    if (!in->joint_q || !in->joint_qd || !in->body_q) return fail(SAPX_ERR_INVALID, "state view needs joint_q, joint_qd and body_q");
+    const DevModel& dm = s->model->dm;
+    if (dm.NB > 64) return fail(SAPX_ERR_UNSUPPORTED, "features support at most 64 bodies per world");
+    FeatureParams F;
+    std::memset(&F, 0, sizeof(F));
+    F.m = dm;
+    F.jq = in->joint_q;
+    F.jqd = in->joint_qd;
+    F.bq = in->body_q;
+    if (ctl) {
+        F.tpos = ctl->joint_target_pos;
+        F.tvel = ctl->joint_target_vel;
+    }
+    if (c && c->count) {
+        if (!c->world_start) return fail(SAPX_ERR_UNSUPPORTED, "features need a world-grouped contact list (world_start)");
+        F.cworld_start = c->world_start;
+        F.cshape0 = c->shape0; F.cshape1 = c->shape1;
+        F.cpoint0 = c->point0; F.cpoint1 = c->point1; F.cnormal = c->normal;
+        F.cmargin0 = c->margin0; F.cmargin1 = c->margin1;
+    }
+    F.C = s->opts.max_rigid_contact;
+    F.feat_off = s->feat_off;
+    F.frame_qs = s->frame_qs;
+    F.prev_target = prev_target;
+    F.F = s->feat_count;
+    F.out = features;
+    HIP_TRY(launch_features(F, reinterpret_cast<hipStream_t>(stream)));
+    return SAPX_OK;
+}
+
+int32_t sapx_solver_set_warm_start_io(SapxSolver* s, const float* start_correction, float* target_out) {
+    if (!s) return fail(SAPX_ERR_INVALID, "null argument");
+    if ((start_correction || target_out) && !s->wave)
+        return fail(SAPX_ERR_UNSUPPORTED, "the learned warm start needs the one-wave step kernel");
+    s->learn_delta = start_correction;
+    s->learn_target = target_out;
+    return SAPX_OK;
+}
 
 int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     if (!d || !out) return fail(SAPX_ERR_INVALID, "null argument");
@@ -531,11 +591,39 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
     }
     (void)hipMemset(s->v_guess, 0, D * sizeof(double));
     (void)hipMemset(s->st, 0, 4 * W * sizeof(int));
-    if (hipMalloc(&s->hist, 258 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc(&s->hist, kHistSize * sizeof(unsigned long long)) != hipSuccess) {
         delete s;
         return fail(SAPX_ERR_HIP, "hipMalloc failed for the iteration histogram");
     }
-    (void)hipMemset(s->hist, 0, 258 * sizeof(unsigned long long));
+    (void)hipMemset(s->hist, 0, kHistSize * sizeof(unsigned long long));
+    {
+        // feature layout of the learned warm start (joint blocks, then one block per body)
+        const int J = dm.J;
+        std::vector<int> jt(J), jqs(J + 1), jds(J + 1), off(J + 1);
+        if (hipMemcpy(jt.data(), dm.t_jtype, (size_t)J * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(jqs.data(), dm.t_jqs, (size_t)(J + 1) * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(jds.data(), dm.t_jds, (size_t)(J + 1) * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            delete s;
+            return fail(SAPX_ERR_HIP, "reading the topology template failed");
+        }
+        int o = 0;
+        for (int j = 0; j < J; ++j) {
+            off[j] = o;
+            if (jt[j] == kJointFree || jt[j] == kJointDistance) {
+                o += 10;
+                if (s->frame_qs < 0) s->frame_qs = jqs[j];
+            } else {
+                o += 4 * (jds[j + 1] - jds[j]);
+            }
+        }
+        off[J] = o;
+        s->feat_count = o + 5 * dm.NB + dm.N;
+        if (hipMalloc(&s->feat_off, (size_t)(J + 1) * 4) != hipSuccess ||
+            hipMemcpy(s->feat_off, off.data(), (size_t)(J + 1) * 4, hipMemcpyHostToDevice) != hipSuccess) {
+            delete s;
+            return fail(SAPX_ERR_HIP, "hipMalloc failed for the feature layout");
+        }
+    }
     const size_t sizes[7] = {D, D, D, W * C * 3, W * C, W * C, D};
     size_t tot = 0;
     for (int i = 0; i < 7; ++i) {
@@ -709,6 +797,8 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
             }
         };
         P.world_order = nullptr;
+        P.learn_delta = s->learn_delta;
+        P.learn_target = s->learn_target;
         if (s->order) {
             HIP_TRY(launch_world_order(P.cworld_start, P.st_iters, (int)dm.W, P.C, s->order, st));
             P.world_order = s->order;
@@ -734,6 +824,8 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
             HIP_TRY(launch_on(P, cb, st));
         }
     } else {
+        if (s->learn_delta || s->learn_target)
+            return fail(SAPX_ERR_UNSUPPORTED, "the learned warm start needs the one-wave step kernel");
         HIP_TRY(launch_step(P, st));
     }
     s->use_guess = 1;
@@ -755,9 +847,9 @@ int32_t sapx_get_stats(SapxSolver* s, int32_t* newton_iterations, int32_t* line_
 int32_t sapx_get_histogram(SapxSolver* s, int64_t* hist, int32_t reset, void* stream) {
     if (!s || !hist) return fail(SAPX_ERR_INVALID, "null argument");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    HIP_TRY(hipMemcpyAsync(hist, s->hist, 258 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hist, s->hist, kHistSize * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (reset) HIP_TRY(hipMemsetAsync(s->hist, 0, 258 * sizeof(unsigned long long), st));
+    if (reset) HIP_TRY(hipMemsetAsync(s->hist, 0, kHistSize * sizeof(unsigned long long), st));
     return SAPX_OK;
 }
 

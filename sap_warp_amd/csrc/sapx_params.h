@@ -66,6 +66,34 @@ struct StepParams {
     WaveLayout wl;       // one-wave kernel layout for this launch's contact variant
     int wave_min_nc;     // the variant processes worlds with wave_min_nc <= nc <= its capacity
     const int* world_order;  // blockIdx -> world (a permutation of 0..W-1), or null for the identity
+    // learned warm start (wave kernels): predicted correction of the plain start (W x N, free-joint
+    // rows in the joint's body frame) and the training target written back (same layout)
+    const float* learn_delta;
+    float* learn_target;
+};
+
+// input features of the learned warm start (sapx_learned.hip)
+struct FeatureParams {
+    DevModel m;
+    const float* jq;
+    const float* jqd;
+    const float* bq;
+    const float* tpos;  // may be null (zero targets)
+    const float* tvel;
+    const int* cworld_start;  // may be null (no contacts)
+    const int* cshape0;
+    const int* cshape1;
+    const float* cpoint0;
+    const float* cpoint1;
+    const float* cnormal;
+    const float* cmargin0;
+    const float* cmargin1;
+    int C;                // contact capacity per world
+    const int* feat_off;  // J+1: first feature of each joint's block (bodies follow at feat_off[J])
+    int frame_qs;         // local coordinate start of the first free joint, -1 if none
+    const float* prev_target;  // W x N previous correction, or null (zeros)
+    int F;                // features per world
+    float* out;           // W x F
 };
 
 struct CollideParams {

@@ -190,6 +190,19 @@ struct DofTerms {
     float pda, pdg, pdl, lov, upv, lor, upr;
 };
 
+// component k (0..5) of the free-joint block that starts at dof ds of x (float64, LDS),
+// rotated by R(q) (joint body frame -> world) or by R(q)^T (world -> body frame); both
+// three-row halves of the block rotate alike
+__device__ __forceinline__ double free_rot(const double* x, int ds, int k, const float* q, bool to_body) {
+    const int g = k < 3 ? 0 : 3;
+    const D3 a = d3(x[ds + g], x[ds + g + 1], x[ds + g + 2]);
+    DQ r = dqnormalize(DQ{(double)q[0], (double)q[1], (double)q[2], (double)q[3]});
+    if (to_body) r = DQ{-r.x, -r.y, -r.z, r.w};
+    const D3 b = dqrot_unit(r, a);
+    const int c = k - g;
+    return c == 0 ? b.x : (c == 1 ? b.y : b.z);
+}
+
 __device__ __forceinline__ double dof_terms(const DofTerms& m, double vi, double dt, double* imp, double* hd) {
     double cost = 0.0;
     *imp = 0.0;
@@ -864,6 +877,58 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     }
     double avs = lds_row_dot<NP>(arow, vs);
     if (lane >= n) avs = 0.0;
+    const double* k8 = ck + (lane < CB ? lane : 0) * kCk;
+    // learned warm start: v_plain + R delta (free-joint rows rotated out of the joint's body
+    // frame) replaces the plain start when its SAP cost is lower; otherwise the plain start
+    // stays (the fallback)
+    const double v_plain = v;
+    int start_kind = 0;  // 0 plain, 1 learned start taken, 2 learned start rejected
+    int fr_ds = -1, fr_k = 0;
+    float fq[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+    if (lane < n) {
+        const int jd = T.dof_joint[lane];
+        const int jt = T.jtype[jd];
+        if (jt == kJointFree || jt == kJointDistance) {
+            fr_ds = T.jds[jd];
+            fr_k = lane - fr_ds;
+            const int qs = T.jqs[jd];
+            fq[0] = q_in[qs + 3]; fq[1] = q_in[qs + 4]; fq[2] = q_in[qs + 5]; fq[3] = q_in[qs + 6];
+        }
+    }
+    if (P.learn_delta && !unconstrained) {
+        auto sap_cost = [&](double x) -> double {
+            double ax, c0, c1, c2;
+            const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
+            matvec_AJ<NP>(arow, J0, js, x, ax, c0, c1, c2);
+            double cst = 0.0;
+            if (lane < nc) {
+                const double vcv[3] = {c0, c1, c2};
+                double gg[3];
+                float Gd[9];
+                cst += project_contact(k8, vcv, gg, Gd, false);
+            }
+            if (lane < n) {
+                double imp, hdd;
+                cst += dof_terms(mt, x, dt, &imp, &hdd);
+                cst += 0.5 * (x - vs) * (ax - avs);
+            }
+            return wave_sum(cst);
+        };
+        vec[lane] = lane < n ? (double)P.learn_delta[(size_t)w * n + lane] : 0.0;
+        wave_sync();
+        double dl = lane < n ? vec[lane] : 0.0;
+        if (fr_ds >= 0) dl = free_rot(vec, fr_ds, fr_k, fq, false);
+        wave_sync();
+        const double vp = (lane < n && part) ? v + dl : v;
+        const double c_plain = sap_cost(v);
+        const double c_pred = sap_cost(vp);
+        if (c_pred < c_plain) {
+            v = vp;
+            start_kind = 1;
+        } else {
+            start_kind = 2;
+        }
+    }
     int iteration = 0, ls_total = 0, flags = 0, ls_status = 0;
     // lane-c contact state: vc, dvc, gamma (float64), G in LDS (cG)
     double vc0 = 0, vc1 = 0, vc2 = 0, dvc0 = 0, dvc1 = 0, dvc2 = 0, g0 = 0, g1 = 0, g2 = 0;
@@ -872,7 +937,6 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     float hd = 0.0f;
     double alpha = 1.0, prev_cost = 0.0;
     const double cost_min_alpha = 0.5;
-    const double* k8 = ck + (lane < CB ? lane : 0) * kCk;
     if (unconstrained) {
         flags = 1;
     } else {
@@ -1169,6 +1233,17 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         }
     }
 
+    // training target of the learned warm start: converged v minus the plain start, with
+    // free-joint rows in the joint's body frame
+    if (P.learn_target) {
+        vec[lane] = lane < n ? v - v_plain : 0.0;
+        wave_sync();
+        double t = lane < n ? vec[lane] : 0.0;
+        if (fr_ds >= 0) t = free_rot(vec, fr_ds, fr_k, fq, true);
+        if (lane < n) P.learn_target[(size_t)w * n + lane] = (float)t;
+        wave_sync();
+    }
+
     // ------------------------------------------------------------------ 9
     // integration (solver_sap.py:288-519) + SAP -> public velocity
     vec[lane] = v0;
@@ -1327,6 +1402,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
             atomicAdd(&P.st_hist[iteration < 255 ? iteration : 255], 1ull);
             atomicAdd(&P.st_hist[256], (unsigned long long)ls_total);
             atomicAdd(&P.st_hist[257], 1ull);
+            if (start_kind) atomicAdd(&P.st_hist[start_kind == 1 ? 258 : 259], 1ull);
         }
     }
 }

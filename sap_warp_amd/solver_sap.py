@@ -400,8 +400,8 @@ class SolverSAP:
         return SapContactSolveResult(*arrs)
 
     def iteration_histogram(self, reset: bool = True) -> np.ndarray:
-        """Accumulated Newton-iteration histogram (``sapx_get_histogram``): 258 int64 entries."""
-        h = np.zeros(258, dtype=np.int64)
+        """Accumulated Newton-iteration histogram (``sapx_get_histogram``): 260 int64 entries."""
+        h = np.zeros(nat.HIST_SIZE, dtype=np.int64)
         nat.check(self._lib.sapx_get_histogram(self._handle, h.ctypes.data, int(bool(reset)), nat.stream_ptr()),
                   "sapx_get_histogram")
         return h
