@@ -422,11 +422,23 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
         add(TP_CHILD_START, child_start.data(), NB + 1);
         add(TP_CHILD, child_list.data(), child_list.size());
         if (topo.size() % 2) topo.push_back(0);  // 8-byte alignment of the u64 masks
-        dm.topo_off[TP_BODY_ANC] = (int)topo.size();
-        for (int b = 0; b < NB; ++b) {
-            topo.push_back((int)(anc[b] & 0xffffffffull));
-            topo.push_back((int)(anc[b] >> 32));
-        }
+        auto add64 = [&](int field, const std::vector<unsigned long long>& v) {
+            dm.topo_off[field] = (int)topo.size();
+            for (unsigned long long x : v) {
+                topo.push_back((int)(x & 0xffffffffull));
+                topo.push_back((int)(x >> 32));
+            }
+        };
+        // dof-level ancestry: bit d = dof d's joint lies on the root path
+        std::vector<unsigned long long> anc_dof(NB, 0ull), dof_anc(N, 0ull);
+        for (int b = 0; b < NB; ++b)
+            for (int d = 0; d < N; ++d)
+                if ((anc[b] >> dof_joint[d]) & 1ull) anc_dof[b] |= 1ull << d;
+        for (int d = 0; d < N; ++d) dof_anc[d] = anc_dof[t_jchild[dof_joint[d]]];
+        add64(TP_BODY_ANC, anc);
+        add64(TP_BODY_SUB, sub);
+        add64(TP_BODY_ANC_DOF, anc_dof);
+        add64(TP_DOF_ANC, dof_anc);
         dm.topo_ints = (int)topo.size();
         UP(topo, topo.data(), topo.size());
     }
@@ -550,7 +562,7 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
             // J rows then the factor rows; the kinematics scratch overlays both
             const size_t jbytes = round_up((size_t)3 * CB * L.js * 4, 16);
             const size_t ltbytes = ((size_t)NP * L.ls + 64) * 4;
-            const size_t kin = (size_t)(50 * nb + 14 * n + 2) * 4 + (size_t)dm.J * 128;
+            const size_t kin = (size_t)(54 * nb + 14 * n + 2) * 4 + (size_t)dm.J * 128;
             L.o_J = (int)o;
             L.o_Lt = (int)(o + jbytes);
             o = round_up(o + std::max(jbytes + ltbytes, kin), 16);

@@ -33,7 +33,7 @@ constexpr int kMaxDof = 64;       // dofs per world (one wavefront per row set)
 enum TopoField {
     TP_JTYPE = 0, TP_JPARENT, TP_JCHILD, TP_JQS, TP_JDS, TP_LEVEL_START, TP_LEVEL_JOINT, TP_DOF_JOINT,
     TP_DOF_COORD, TP_DOF_TARGET, TP_DOF_LIMIT_OK, TP_BODY_ART_START, TP_BODY_ART_COUNT, TP_CHILD_START,
-    TP_CHILD, TP_BODY_ANC, TP_COUNT
+    TP_CHILD, TP_BODY_ANC, TP_BODY_SUB, TP_BODY_ANC_DOF, TP_DOF_ANC, TP_COUNT
 };
 
 struct DevModel {
@@ -52,7 +52,7 @@ struct DevModel {
     // the step kernel); topo_off[k] = first int of field k (see enum TopoField)
     const int* topo;
     int topo_ints;
-    int topo_off[16];
+    int topo_off[TP_COUNT];
 
     // template, local indices
     const int* t_jtype;

@@ -272,12 +272,11 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     // kinematics scratch overlays the J / Lt region (phases 2-4 and 9)
     float* kV = Jm;
     float* kA = kV + nb * 6;
-    float* kF = kA + nb * 6;
-    float* kI = kF + nb * 6;
-    float* kIc = kI + nb * 10;
-    float* kFc = kIc + nb * 10;
-    float* kSO = kFc + nb * 6;
-    float* kFi = kSO + n * 6;
+    float* kI = kA + nb * 6;      // nb x 12 (16-byte rows): inertia about O
+    float* kIc = kI + nb * 12;    // nb x 12: composite inertia about O
+    float* kF = kIc + nb * 12;    // nb x 6: body force, torque about O
+    float* kFc = kF + nb * 6;     // nb x 6: composite force, torque about O
+    float* kFi = kFc + nb * 6;
     float* kExt = kFi + n * 6;
     float* kQdPub = kExt + nb * 6;
     double* kXl = reinterpret_cast<double*>(kQdPub + ((n + 1) & ~1));
@@ -513,12 +512,15 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         fv = fv - mg;
         const float* e = kExt + b * 6;
         float* F = kF + b * 6;
-        F[0] = fw.x + e[0]; F[1] = fw.y + e[1]; F[2] = fw.z + e[2];
-        F[3] = fv.x + e[3]; F[4] = fv.y + e[4]; F[5] = fv.z + e[5];
+        // torque about O (tau_O = tau_b + (p_b - O) x f): composites become plain subtree sums
+        const V3 fvt = fv + v3(e[3], e[4], e[5]);
+        const V3 two = fw + v3(e[0], e[1], e[2]) + cross(v3(bqb[0], bqb[1], bqb[2]) - O, fvt);
+        F[0] = two.x; F[1] = two.y; F[2] = two.z;
+        F[3] = fvt.x; F[4] = fvt.y; F[5] = fvt.z;
         const V3 co = (v3(bqb[0], bqb[1], bqb[2]) - O) + c;
         const float co2 = dot(co, co);
         const float co_[3] = {co.x, co.y, co.z};
-        float* I = kI + b * 10;
+        float* I = kI + b * 12;
         int e6 = 0;
         for (int r = 0; r < 3; ++r)
             for (int k = r; k < 3; ++k) I[e6++] = IW[r][k] + m * ((r == k ? co2 : 0.0f) - co_[r] * co_[k]);
@@ -527,71 +529,63 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     wave_sync();
 
     // ------------------------------------------------------------------ 4
-    // composites (reverse level sweep, each body pulls its children), tau, A rows
-    for (int Lv = M.num_levels - 1; Lv >= 0; --Lv) {
-        const int l0 = T.level_start[Lv], l1 = T.level_start[Lv + 1];
-        for (int t = l0 + lane; t < l1; t += 64) {
-            const int c = T.jchild[T.level_joint[t]];
-            float Ic[10];
-#pragma unroll
-            for (int k = 0; k < 10; ++k) Ic[k] = kI[c * 10 + k];
-            const float* F = kF + c * 6;
-            V3 tw = v3(F[0], F[1], F[2]), fv = v3(F[3], F[4], F[5]);
-            const V3 pc = v3(bq[c * 8], bq[c * 8 + 1], bq[c * 8 + 2]);
-            for (int e = T.child_start[c]; e < T.child_start[c + 1]; ++e) {
-                const int ch = T.child[e];
-#pragma unroll
-                for (int k = 0; k < 10; ++k) Ic[k] += kIc[ch * 10 + k];
-                const float* Fch = kFc + ch * 6;
-                const V3 f = v3(Fch[3], Fch[4], Fch[5]);
-                const V3 pch = v3(bq[ch * 8], bq[ch * 8 + 1], bq[ch * 8 + 2]);
-                tw = tw + v3(Fch[0], Fch[1], Fch[2]) - cross(pc - pch, f);
-                fv = fv + f;
-            }
-#pragma unroll
-            for (int k = 0; k < 10; ++k) kIc[c * 10 + k] = Ic[k];
-            float* Fc = kFc + c * 6;
-            Fc[0] = tw.x; Fc[1] = tw.y; Fc[2] = tw.z; Fc[3] = fv.x; Fc[4] = fv.y; Fc[5] = fv.z;
+    // composite inertia and force about O: plain sums over each body's subtree (one body per
+    // lane, every other body's row read by broadcast), the same totals as the reverse sweep
+    // over levels in the reference (free_motion.py:843-1179)
+    for (int c = lane; c < nb; c += 64) {
+        const unsigned long long sm = T.body_sub[c];
+        float4 i0 = make_float4(0.f, 0.f, 0.f, 0.f), i1 = i0, i2 = i0;
+        float f0 = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f, f4 = 0.f, f5 = 0.f;
+#pragma unroll 2
+        for (int d = 0; d < nb; ++d) {
+            const float sel = ((sm >> d) & 1ull) ? 1.0f : 0.0f;
+            const float4* Id = reinterpret_cast<const float4*>(kI + d * 12);
+            const float4 a0 = Id[0], a1 = Id[1], a2 = Id[2];
+            const float* Fd = kF + d * 6;
+            i0.x = fmaf(sel, a0.x, i0.x); i0.y = fmaf(sel, a0.y, i0.y); i0.z = fmaf(sel, a0.z, i0.z); i0.w = fmaf(sel, a0.w, i0.w);
+            i1.x = fmaf(sel, a1.x, i1.x); i1.y = fmaf(sel, a1.y, i1.y); i1.z = fmaf(sel, a1.z, i1.z); i1.w = fmaf(sel, a1.w, i1.w);
+            i2.x = fmaf(sel, a2.x, i2.x); i2.y = fmaf(sel, a2.y, i2.y);
+            f0 = fmaf(sel, Fd[0], f0); f1 = fmaf(sel, Fd[1], f1); f2 = fmaf(sel, Fd[2], f2);
+            f3 = fmaf(sel, Fd[3], f3); f4 = fmaf(sel, Fd[4], f4); f5 = fmaf(sel, Fd[5], f5);
         }
-        wave_sync();
+        float4* Ic = reinterpret_cast<float4*>(kIc + c * 12);
+        Ic[0] = i0; Ic[1] = i1; Ic[2] = i2;
+        float* Fc = kFc + c * 6;
+        Fc[0] = f0; Fc[1] = f1; Fc[2] = f2; Fc[3] = f3; Fc[4] = f4; Fc[5] = f5;
     }
+    wave_sync();
+    // tau and the dof's motion subspace / momentum about O; from here on the motion subspace
+    // is only needed about O, so it replaces S in place
     float tau = 0.0f;
     SV SOi = sv(v3(0, 0, 0), v3(0, 0, 0)), Fii = SOi;
-    int ji = 0;
     if (lane < n) {
-        ji = T.dof_joint[lane];
+        const int ji = T.dof_joint[lane];
         const int c = T.jchild[ji];
         const SV S = load_sv(Sg + lane * 6);
-        tau = jf - sv_dot(S, load_sv(kFc + c * 6));
         const V3 pc = v3(bq[c * 8], bq[c * 8 + 1], bq[c * 8 + 2]);
         SOi = shift_velocity(S, O - pc);
-        const float* I = kIc + c * 10;
+        tau = jf - sv_dot(SOi, load_sv(kFc + c * 6));  // power pairing is the same about any point
+        const float* I = kIc + c * 12;
         const V3 hv = v3(I[6], I[7], I[8]);
         const V3 Iw = v3(I[0] * SOi.w.x + I[1] * SOi.w.y + I[2] * SOi.w.z, I[1] * SOi.w.x + I[3] * SOi.w.y + I[4] * SOi.w.z,
                          I[2] * SOi.w.x + I[4] * SOi.w.y + I[5] * SOi.w.z);
         Fii = sv(Iw + cross(hv, SOi.v), SOi.v * I[9] - cross(hv, SOi.w));
-        store_sv(kSO + lane * 6, SOi);
+        store_sv(Sg + lane * 6, SOi);
         store_sv(kFi + lane * 6, Fii);
     }
     wave_sync();
-    // dof-ancestry mask of this lane: bit j set when joint(j) lies on joint(lane)'s root path
-    unsigned long long dmask = 0ull;
-    if (lane < n) {
-        const unsigned long long anc = T.body_anc[T.jchild[ji]];
-        for (int j = 0; j < n; ++j)
-            if (bit(anc, T.dof_joint[j])) dmask |= 1ull << j;
-    }
+    // A_ij = (I_c S_i) . S_j when joint(j) is on joint(i)'s root path, the mirror when joint(i)
+    // is on joint(j)'s, else 0 (dof-level ancestor masks from the topology template)
+    const unsigned long long dmask = lane < n ? T.dof_anc[lane] : 0ull;
     const float arm_i = (lane < n) ? M.armature[(size_t)w * n + lane] : 0.0f;
     float adiag = 1.0f;
+#pragma unroll 2
     for (int j = 0; j < n; ++j) {
-        const unsigned long long mj =
-            ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(dmask >> 32), j) << 32) |
-            (unsigned)__builtin_amdgcn_readlane((int)dmask, j);
-        const SV SOj = load_sv(kSO + j * 6);
+        const unsigned long long mj = T.dof_anc[j];
+        const SV SOj = load_sv(Sg + j * 6);
         const SV Fij = load_sv(kFi + j * 6);
-        float val = 0.0f;
-        if ((dmask >> j) & 1ull) val = sv_dot(Fii, SOj);
-        else if ((mj >> lane) & 1ull) val = sv_dot(Fij, SOi);
+        const float d1 = sv_dot(Fii, SOj), d2 = sv_dot(Fij, SOi);
+        float val = ((dmask >> j) & 1ull) ? d1 : (((mj >> lane) & 1ull) ? d2 : 0.0f);
         if (lane == j) { val += arm_i; adiag = val; }
         if (lane < n) Am[lane * ls + j] = val;
     }
@@ -703,17 +697,19 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         const bool wit = P.point_mode == 0;
         const V3 xs1 = wit ? cx1 : cpc;
         const V3 xs0 = wit ? cx0 : cpc;
-        const unsigned long long anc1 = cb1 >= 0 ? T.body_anc[cb1] : 0ull;
-        const unsigned long long anc0 = cb0 >= 0 ? T.body_anc[cb0] : 0ull;
+        // dof d moves the contact point x of a body on its root path at S_O.v + S_O.w x (x - O)
+        // (S_O: motion subspace about O; equal to S.v + S.w x (x - p_child))
+        const unsigned long long anc1 = cb1 >= 0 ? T.body_anc_dof[cb1] : 0ull;
+        const unsigned long long anc0 = cb0 >= 0 ? T.body_anc_dof[cb0] : 0ull;
+        const V3 r1 = xs1 - O, r0 = xs0 - O;
         float* J0 = Jm + (lane * 3) * js;
+#pragma unroll 2
         for (int d = 0; d < n; ++d) {
-            const int jd = T.dof_joint[d];
-            const int cd = T.jchild[jd];
             const SV S = load_sv(Sg + d * 6);
-            const V3 pcdv = v3(bq[cd * 8], bq[cd * 8 + 1], bq[cd * 8 + 2]);
-            V3 rel = v3(0, 0, 0);
-            if (bit(anc1, jd)) rel = rel + S.v + cross(S.w, xs1 - pcdv);
-            if (bit(anc0, jd)) rel = rel - (S.v + cross(S.w, xs0 - pcdv));
+            const float s1 = ((anc1 >> d) & 1ull) ? 1.0f : 0.0f;
+            const float s0 = ((anc0 >> d) & 1ull) ? 1.0f : 0.0f;
+            const V3 u1 = S.v + cross(S.w, r1), u0 = S.v + cross(S.w, r0);
+            const V3 rel = u1 * s1 - u0 * s0;
             J0[d] = dot(tx, rel);
             J0[js + d] = dot(ty, rel);
             J0[2 * js + d] = dot(nrm, rel);
