@@ -439,8 +439,51 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
         add64(TP_BODY_SUB, sub);
         add64(TP_BODY_ANC_DOF, anc_dof);
         add64(TP_DOF_ANC, dof_anc);
+        {
+            // ancestor tables for pointer jumping: round r links each body to the body 2^r
+            // joints up its root path
+            std::vector<int> cur(NB, -1), ups;
+            for (int b = 0; b < NB; ++b) cur[b] = body_joint[b] >= 0 ? t_jparent[body_joint[b]] : -1;
+            int rounds = 0;
+            while (std::any_of(cur.begin(), cur.end(), [](int u) { return u >= 0; })) {
+                ups.insert(ups.end(), cur.begin(), cur.end());
+                std::vector<int> nxt(NB, -1);
+                for (int b = 0; b < NB; ++b) nxt[b] = cur[b] >= 0 ? cur[cur[b]] : -1;
+                cur.swap(nxt);
+                ++rounds;
+            }
+            dm.fk_rounds = rounds;
+            add(TP_BODY_UP, ups.data(), ups.size());
+        }
+        while (topo.size() % 4) topo.push_back(0);  // 16-byte copies into LDS
         dm.topo_ints = (int)topo.size();
         UP(topo, topo.data(), topo.size());
+    }
+    {
+        // per-world parameter blocks in the one-wave kernel's LDS layout (one contiguous copy)
+        const int wpf = 16 * J + 4 * N + 14 * NB;
+        dm.wp_stride = (wpf + 3) / 4 * 4;
+        std::vector<float> wp((size_t)W * dm.wp_stride, 0.0f);
+        for (int w = 0; w < W; ++w) {
+            float* o = wp.data() + (size_t)w * dm.wp_stride;
+            for (int j = 0; j < J; ++j)
+                for (int k = 0; k < 7; ++k) {
+                    o[j * 8 + k] = d->joint_X_p[((size_t)w * J + j) * 7 + k];
+                    o[J * 8 + j * 8 + k] = d->joint_X_c[((size_t)w * J + j) * 7 + k];
+                }
+            float* ax = o + 16 * J;
+            for (int i = 0; i < N; ++i)
+                for (int k = 0; k < 3; ++k) ax[i * 4 + k] = d->joint_axis[((size_t)w * N + i) * 3 + k];
+            float* com = ax + 4 * N;
+            float* in = com + 4 * NB;
+            float* ms = in + 9 * NB;
+            for (int b = 0; b < NB; ++b) {
+                for (int k = 0; k < 3; ++k) com[b * 4 + k] = d->body_com[((size_t)w * NB + b) * 3 + k];
+                for (int k = 0; k < 9; ++k) in[b * 9 + k] = d->body_inertia[((size_t)w * NB + b) * 9 + k];
+                ms[b] = d->body_mass[(size_t)w * NB + b];
+            }
+        }
+        UP(world_params, wp.data(), wp.size());
     }
     UP(t_body_art_count, art_count.data(), NB);
     UP(t_pairs, per_world[0].data(), per_world[0].size());
@@ -550,7 +593,7 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
             L.js = NP + 1;  // zero-padded rows: the mat-vecs run over NP columns unguarded
             L.ls = NP + 1;
             L.o_topo = tk((size_t)dm.topo_ints * 4);
-            L.o_mp = tk((size_t)(dm.J * 16 + n * 4 + nb * 4 + nb * 9 + nb) * 4);
+            L.o_mp = tk((size_t)dm.wp_stride * 4);
             L.o_q = tk((size_t)2 * dm.NQ * 4);
             L.o_bq = tk((size_t)nb * 8 * 4);
             L.o_bqd = tk((size_t)nb * 8 * 8);
@@ -562,7 +605,7 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
             // J rows then the factor rows; the kinematics scratch overlays both
             const size_t jbytes = round_up((size_t)3 * CB * L.js * 4, 16);
             const size_t ltbytes = ((size_t)NP * L.ls + 64) * 4;
-            const size_t kin = (size_t)(54 * nb + 14 * n + 2) * 4 + (size_t)dm.J * 128;
+            const size_t kin = (size_t)(54 * nb + 14 * n + 2) * 4 + (size_t)dm.J * 128 + (size_t)nb * 64;
             L.o_J = (int)o;
             L.o_Lt = (int)(o + jbytes);
             o = round_up(o + std::max(jbytes + ltbytes, kin), 16);

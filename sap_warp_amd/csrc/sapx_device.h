@@ -33,7 +33,7 @@ constexpr int kMaxDof = 64;       // dofs per world (one wavefront per row set)
 enum TopoField {
     TP_JTYPE = 0, TP_JPARENT, TP_JCHILD, TP_JQS, TP_JDS, TP_LEVEL_START, TP_LEVEL_JOINT, TP_DOF_JOINT,
     TP_DOF_COORD, TP_DOF_TARGET, TP_DOF_LIMIT_OK, TP_BODY_ART_START, TP_BODY_ART_COUNT, TP_CHILD_START,
-    TP_CHILD, TP_BODY_ANC, TP_BODY_SUB, TP_BODY_ANC_DOF, TP_DOF_ANC, TP_COUNT
+    TP_CHILD, TP_BODY_ANC, TP_BODY_SUB, TP_BODY_ANC_DOF, TP_DOF_ANC, TP_BODY_UP, TP_COUNT
 };
 
 struct DevModel {
@@ -45,6 +45,7 @@ struct DevModel {
     int S;   // shapes per world
     int P;   // candidate pairs per world
     int num_levels;
+    int fk_rounds;  // pointer-jumping rounds over the root paths: ceil(log2(depth in bodies))
     int total_shapes;
     int global_shape_base;  // first global (world -1) shape index = W*S
 
@@ -74,6 +75,12 @@ struct DevModel {
     const int* t_child_start;     // NB+1: CSR of child bodies
     const int* t_child;           // child bodies, grouped by parent
     const int* t_pairs;           // P x 2: >=0 local shape, <0 => global shape (-v-1)
+
+    // per-world model parameters packed in the one-wave kernel's LDS order, wp_stride floats
+    // per world (a multiple of 4): X_p (J x 8), X_c (J x 8), axes (N x 4), COMs (NB x 4),
+    // inertias (NB x 9), masses (NB)
+    const float* world_params;
+    int wp_stride;
 
     // global per-item arrays
     const int* j_xc_identity;  // W*J

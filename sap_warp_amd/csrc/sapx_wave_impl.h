@@ -281,6 +281,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     float* kQdPub = kExt + nb * 6;
     double* kXl = reinterpret_cast<double*>(kQdPub + ((n + 1) & ~1));
     double* kXpn = kXl + nj * 8;  // nj x 8: X_p with its rotation normalised (float64)
+    double* kPose = kXpn + nj * 8;  // nb x 8: pointer-jumping partner buffer of bqd (float64)
     float* mXp = mp;
     float* mXc = mXp + nj * 8;
     float* mAx = mXc + nj * 8;
@@ -289,17 +290,57 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     float* mM = mI + nb * 9;
 
     // ------------------------------------------------------------------ 0/1
-    for (int k = lane; k < M.topo_ints; k += 64) topo[k] = M.topo[k];
-    for (int t = lane; t < nj * 7; t += 64) {
-        const int j = t / 7, k = t % 7;
-        mXp[j * 8 + k] = M.joint_X_p[(size_t)w * nj * 7 + t];
-        mXc[j * 8 + k] = M.joint_X_c[(size_t)w * nj * 7 + t];
+    // every global input of the prologue is requested in one batch (16-byte copies of the
+    // topology template and of this world's parameter block, the state and control rows),
+    // then written to LDS: one memory latency instead of one per dependent step
+    constexpr int kPre = 6;  // float4 prefetch slots per lane (parameter blocks up to 1536 floats)
+    const int wp4 = M.wp_stride >> 2, tp4 = M.topo_ints >> 2;
+    const float4* wsrc = reinterpret_cast<const float4*>(M.world_params + (size_t)w * M.wp_stride);
+    const int4* tsrc = reinterpret_cast<const int4*>(M.topo);
+    float4 wbuf[kPre];
+    int4 tbuf[2];
+#pragma unroll
+    for (int k = 0; k < kPre; ++k)
+        if (k * 64 + lane < wp4) wbuf[k] = wsrc[k * 64 + lane];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        if (k * 64 + lane < tp4) tbuf[k] = tsrc[k * 64 + lane];
+    float qd_l = 0.0f, jf_l = 0.0f;
+    if (lane < n) {
+        qd_l = P.jqd_in[(size_t)w * n + lane];
+        if (P.c_jf) jf_l = P.c_jf[(size_t)w * n + lane];
     }
-    for (int t = lane; t < n * 3; t += 64) mAx[(t / 3) * 4 + t % 3] = M.joint_axis[(size_t)w * n * 3 + t];
-    for (int t = lane; t < nb * 3; t += 64) mCom[(t / 3) * 4 + t % 3] = M.body_com[(size_t)w * nb * 3 + t];
-    for (int t = lane; t < nb * 9; t += 64) mI[t] = M.body_inertia[(size_t)w * nb * 9 + t];
-    for (int t = lane; t < nb; t += 64) mM[t] = M.body_mass[(size_t)w * nb + t];
+    float bq_l[7], bf_l[6];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) bq_l[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bf_l[k] = 0.0f;
+    if (lane < nb) {
+        const float* src = P.bq_in + ((size_t)w * nb + lane) * 7;
+#pragma unroll
+        for (int k = 0; k < 7; ++k) bq_l[k] = src[k];
+        if (P.bf_in) {
+            const float* f = P.bf_in + ((size_t)w * nb + lane) * 6;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) bf_l[k] = f[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kPre; ++k)
+        if (k * 64 + lane < wp4) reinterpret_cast<float4*>(mp)[k * 64 + lane] = wbuf[k];
+    for (int i = kPre * 64 + lane; i < wp4; i += 64) reinterpret_cast<float4*>(mp)[i] = wsrc[i];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        if (k * 64 + lane < tp4) reinterpret_cast<int4*>(topo)[k * 64 + lane] = tbuf[k];
+    for (int i = 128 + lane; i < tp4; i += 64) reinterpret_cast<int4*>(topo)[i] = tsrc[i];
     for (int k = lane; k < M.NQ; k += 64) q_in[k] = P.jq_in[(size_t)w * M.NQ + k];
+    // public body poses (bq is free until the kinematics fill it), public velocities and forces
+    if (lane < nb) {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) bq[lane * 8 + k] = bq_l[k];
+    }
+    vec[lane] = qd_l;
+    vec[64 + lane] = jf_l;
     wave_sync();
     const Topo T = topo_view(topo, M);
 
@@ -308,57 +349,65 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     float jf = 0.0f;
     if (lane < n) {
         const int i = lane;
-        const size_t g = (size_t)w * n + i;
         const int j = T.dof_joint[i];
         const int jt = T.jtype[j];
-        v0 = (double)P.jqd_in[g];
-        jf = P.c_jf ? P.c_jf[g] : 0.0f;
+        v0 = (double)qd_l;
+        jf = jf_l;
         if (jt == kJointFree || jt == kJointDistance) {
             const int ds = T.jds[j];
             const int a = i - ds;
             const int child = T.jchild[j];
-            const TF x = load_tf(P.bq_in + ((size_t)w * nb + child) * 7);
+            const float* xb = bq + child * 8;
+            const Q4 xq = Q4{xb[3], xb[4], xb[5], xb[6]};
             const float* com = mCom + child * 4;
-            const V3 r = qrot(qnormalize(x.q), v3(com[0], com[1], com[2]));
-            const size_t gd = (size_t)w * n + ds;
-            const double w0 = P.jqd_in[gd + 3], w1 = P.jqd_in[gd + 4], w2 = P.jqd_in[gd + 5];
+            const V3 r = qrot(qnormalize(xq), v3(com[0], com[1], com[2]));
+            const double w0 = vec[ds + 3], w1 = vec[ds + 4], w2 = vec[ds + 5];
             const double rx = r.x, ry = r.y, rz = r.z;
             if (a < 3) {
                 v0 = a == 0 ? w0 : (a == 1 ? w1 : w2);
             } else {
-                const double vc = P.jqd_in[gd + a - 3];
+                const double vc = vec[ds + a - 3];
                 const double wxr = a == 3 ? (w1 * rz - w2 * ry) : (a == 4 ? (w2 * rx - w0 * rz) : (w0 * ry - w1 * rx));
                 v0 = vc - wxr;
             }
             if (P.c_jf) {
-                const V3 fc = v3(P.c_jf[gd], P.c_jf[gd + 1], P.c_jf[gd + 2]);
-                const V3 tc = v3(P.c_jf[gd + 3], P.c_jf[gd + 4], P.c_jf[gd + 5]);
+                const double* fj = vec + 64 + ds;
+                const V3 fc = v3((float)fj[0], (float)fj[1], (float)fj[2]);
+                const V3 tc = v3((float)fj[3], (float)fj[4], (float)fj[5]);
                 const V3 to = tc + cross(r, fc);
                 jf = a == 0 ? to.x : a == 1 ? to.y : a == 2 ? to.z : a == 3 ? fc.x : a == 4 ? fc.y : fc.z;
             }
         }
     }
-    vec[lane] = v0;  // v0 by dof for the joint-parallel phases
-    for (int b = lane; b < nb; b += 64) {
-        float* e = kExt + b * 6;
-        for (int k = 0; k < 6; ++k) e[k] = 0.0f;
+    if (lane < nb) {
+        // external body force about the body origin (public [f, tau] at the COM)
+        float* e = kExt + lane * 6;
         if (P.bf_in) {
-            const size_t gb = (size_t)w * nb + b;
-            const float* f = P.bf_in + gb * 6;
-            const V3 fc = v3(f[0], f[1], f[2]);
-            const V3 tc = v3(f[3], f[4], f[5]);
-            const TF x = load_tf(P.bq_in + gb * 7);
-            const float* com = mCom + b * 4;
-            const V3 c = qrot(qnormalize(x.q), v3(com[0], com[1], com[2]));
+            const V3 fc = v3(bf_l[0], bf_l[1], bf_l[2]);
+            const V3 tc = v3(bf_l[3], bf_l[4], bf_l[5]);
+            const float* com = mCom + lane * 4;
+            const V3 c = qrot(qnormalize(Q4{bq_l[3], bq_l[4], bq_l[5], bq_l[6]}), v3(com[0], com[1], com[2]));
             const V3 to = tc + cross(c, fc);
             e[0] = -to.x; e[1] = -to.y; e[2] = -to.z;
             e[3] = -fc.x; e[4] = -fc.y; e[5] = -fc.z;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) e[k] = 0.0f;
         }
     }
     wave_sync();
+    vec[lane] = v0;  // v0 by dof for the joint-parallel phases
+    wave_sync();
 
     // ------------------------------------------------------------------ 2
-    // forward kinematics: joint-local parts in parallel, then the level walk
+    // bodies that are no joint's child keep the identity pose
+    for (int b = lane; b < nb; b += 64) store_dtf(kPose + b * 8, DTF{d3(0.0, 0.0, 0.0), DQ{0.0, 0.0, 0.0, 1.0}});
+    wave_sync();
+    // forward kinematics: joint-local parts in parallel; world poses in float64 by pointer
+    // jumping over the root paths (each body's pose is the product of its path's local
+    // transforms L = X_p X_j X_c^-1, combined in ceil(log2 depth) rounds); then the motion
+    // subspaces and relative velocities in the world frame in parallel, and the velocity /
+    // bias-acceleration recursion of the reference (free_motion.py:412-530) level by level
     for (int j = lane; j < nj; j += 64) {
         const int jt = T.jtype[j];
         const int child = T.jchild[j];
@@ -418,54 +467,74 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         }
         store_dtf(kXl + j * 8, Xl);
         kXl[j * 8 + 7] = plain;
-        {
-            DTF Xpn = dtf_from(load_tf(mXp + j * 8));
-            Xpn.q = dqnormalize(Xpn.q);
-            store_dtf(kXpn + j * 8, Xpn);
-        }
+        DTF Xpn = dtf_from(load_tf(mXp + j * 8));
+        Xpn.q = dqnormalize(Xpn.q);
+        store_dtf(kXpn + j * 8, Xpn);
+        // the child's pose relative to its parent body (for the root: relative to the world)
+        store_dtf(kPose + child * 8, dtf_compose_u(Xpn, Xl));
         store_sv(kV + child * 6, vl);
         store_sv(kA + child * 6, al);
     }
     wave_sync();
-    for (int Lv = 0; Lv < M.num_levels; ++Lv) {
+    {
+        double* src = kPose;
+        double* dst = bqd;
+        for (int r = 0; r < M.fk_rounds; ++r) {
+            if (lane < nb) {
+                const int u = T.body_up[r * nb + lane];
+                DTF x = load_dtf(src + lane * 8);
+                if (u >= 0) x = dtf_compose_u(load_dtf(src + u * 8), x);
+                store_dtf(dst + lane * 8, x);
+            }
+            wave_sync();
+            double* t = src;
+            src = dst;
+            dst = t;
+        }
+        if (src != bqd) {
+            if (lane < nb) store_dtf(bqd + lane * 8, load_dtf(src + lane * 8));
+            wave_sync();
+        }
+    }
+    // per joint, in parallel: the parent-joint frame X_wpj (rotation of the local quantities)
+    for (int j = lane; j < nj; j += 64) {
+        const int parent = T.jparent[j];
+        const int child = T.jchild[j];
+        const int ds = T.jds[j], nd = T.jds[j + 1] - ds;
+        // the root's X_wpj is X_p as given, like the reference
+        const DTF Xwpj_d = parent >= 0 ? dtf_compose_u(load_dtf(bqd + parent * 8), load_dtf(kXpn + j * 8))
+                                       : dtf_from(load_tf(mXp + j * 8));
+        const Q4 R = tf_from(Xwpj_d).q;
+        for (int k = 0; k < nd; ++k) {
+            const SV Sl = load_sv(Sg + (ds + k) * 6);
+            store_sv(Sg + (ds + k) * 6, sv(qrot(R, Sl.w), qrot(R, Sl.v)));
+        }
+        const SV vl = load_sv(kV + child * 6);
+        const SV al = load_sv(kA + child * 6);
+        store_sv(kV + child * 6, sv(qrot(R, vl.w), qrot(R, vl.v)));
+        store_sv(kA + child * 6, sv(qrot(R, al.w), qrot(R, al.v)));
+        const TF Xwc = tf_from(load_dtf(bqd + child * 8));
+        float* bqc = bq + child * 8;
+        bqc[0] = Xwc.p.x; bqc[1] = Xwc.p.y; bqc[2] = Xwc.p.z;
+        bqc[3] = Xwc.q.x; bqc[4] = Xwc.q.y; bqc[5] = Xwc.q.z; bqc[6] = Xwc.q.w;
+    }
+    wave_sync();
+    // velocities and bias accelerations, root to leaves (kV / kA hold the relative ones)
+    for (int Lv = 1; Lv < M.num_levels; ++Lv) {
         const int l0 = T.level_start[Lv], l1 = T.level_start[Lv + 1];
         for (int t = l0 + lane; t < l1; t += 64) {
             const int j = T.level_joint[t];
             const int parent = T.jparent[j];
             const int child = T.jchild[j];
-            const int ds = T.jds[j], nd = T.jds[j + 1] - ds;
-            // parent poses, normalised X_p and the local transforms are unit: only products are
-            // renormalised (the root's X_wpj is X_p as given, like the reference)
-            const DTF Xwpj_d = parent >= 0 ? dtf_compose_u(load_dtf(bqd + parent * 8), load_dtf(kXpn + j * 8))
-                                           : dtf_from(load_tf(mXp + j * 8));
-            const DTF Xl = load_dtf(kXl + j * 8);
-            const DQ qwpj = parent >= 0 ? Xwpj_d.q : dqnormalize(Xwpj_d.q);
-            const DTF Xwc_d = (kXl[j * 8 + 7] != 0.0) ? DTF{Xwpj_d.p, dqmul(qwpj, Xl.q)}
-                                                      : dtf_compose_u(DTF{Xwpj_d.p, qwpj}, Xl);
-            const Q4 R = tf_from(Xwpj_d).q;
-            for (int k = 0; k < nd; ++k) {
-                const SV Sl = load_sv(Sg + (ds + k) * 6);
-                store_sv(Sg + (ds + k) * 6, sv(qrot(R, Sl.w), qrot(R, Sl.v)));
-            }
-            const SV vl = load_sv(kV + child * 6);
-            const SV al = load_sv(kA + child * 6);
-            const SV vPB = sv(qrot(R, vl.w), qrot(R, vl.v));
-            const SV aPB = sv(qrot(R, al.w), qrot(R, al.v));
-            store_dtf(bqd + child * 8, Xwc_d);
-            const TF Xwc = tf_from(Xwc_d);
-            float* bqc = bq + child * 8;
-            bqc[0] = Xwc.p.x; bqc[1] = Xwc.p.y; bqc[2] = Xwc.p.z;
-            bqc[3] = Xwc.q.x; bqc[4] = Xwc.q.y; bqc[5] = Xwc.q.z; bqc[6] = Xwc.q.w;
-            SV V = vPB, A = aPB;
-            if (parent >= 0) {
-                const V3 pPB = Xwc.p - v3(bq[parent * 8], bq[parent * 8 + 1], bq[parent * 8 + 2]);
-                const SV Vp = load_sv(kV + parent * 6);
-                const SV Ap = load_sv(kA + parent * 6);
-                V = compose_velocity(Vp, pPB, vPB);
-                A = compose_acceleration(Ap, Vp, pPB, vPB, aPB);
-            }
-            store_sv(kV + child * 6, V);
-            store_sv(kA + child * 6, A);
+            if (parent < 0) continue;
+            const SV vPB = load_sv(kV + child * 6);
+            const SV aPB = load_sv(kA + child * 6);
+            const V3 pPB = v3(bq[child * 8], bq[child * 8 + 1], bq[child * 8 + 2]) -
+                           v3(bq[parent * 8], bq[parent * 8 + 1], bq[parent * 8 + 2]);
+            const SV Vp = load_sv(kV + parent * 6);
+            const SV Ap = load_sv(kA + parent * 6);
+            store_sv(kV + child * 6, compose_velocity(Vp, pPB, vPB));
+            store_sv(kA + child * 6, compose_acceleration(Ap, Vp, pPB, vPB, aPB));
         }
         wave_sync();
     }
