@@ -1375,10 +1375,18 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         }
     }
     wave_sync();
-    // public forward kinematics (sap_kinematics.py:84-229)
+    // public forward kinematics (sap_kinematics.py:84-229): world poses in float32 by pointer
+    // jumping (each body's local part L = X_p X_j X_c^-1), then the parent-joint frames and
+    // the joint velocity terms in parallel, and the body-velocity recursion per level
     float* pbq = bq;
     float* pbqd = kV;
-    float* kPl = reinterpret_cast<float*>(kXl);
+    float* kPl = reinterpret_cast<float*>(kXl);       // nj x 16: X_jc, then (rp, R jl, R ja)
+    float* kPB = reinterpret_cast<float*>(kPose);     // nb x 8: partner buffer of pbq
+    for (int b = lane; b < nb; b += 64) {
+        float* o = kPB + b * 8;
+        o[0] = 0.f; o[1] = 0.f; o[2] = 0.f; o[3] = 0.f; o[4] = 0.f; o[5] = 0.f; o[6] = 1.f;
+    }
+    wave_sync();
     for (int j = lane; j < nj; j += 64) {
         const int jt = T.jtype[j];
         const int qs = T.jqs[j], ds = T.jds[j];
@@ -1401,6 +1409,46 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         float* o = kPl + j * 16;
         store_tf(o, Xjc);
         o[7] = jl.x; o[8] = jl.y; o[9] = jl.z; o[10] = ja.x; o[11] = ja.y; o[12] = ja.z;
+        store_tf(kPB + T.jchild[j] * 8, tf_mul(load_tf(mXp + j * 8), Xjc));
+    }
+    wave_sync();
+    {
+        float* src = kPB;
+        float* dst = pbq;
+        for (int r = 0; r < M.fk_rounds; ++r) {
+            if (lane < nb) {
+                const int u = T.body_up[r * nb + lane];
+                TF x = load_tf(src + lane * 8);
+                if (u >= 0) x = tf_mul(load_tf(src + u * 8), x);
+                store_tf(dst + lane * 8, x);
+            }
+            wave_sync();
+            float* t = src;
+            src = dst;
+            dst = t;
+        }
+        if (src != pbq) {
+            if (lane < nb) store_tf(pbq + lane * 8, load_tf(src + lane * 8));
+            wave_sync();
+        }
+    }
+    for (int j = lane; j < nj; j += 64) {
+        const int parent = T.jparent[j];
+        const TF Xp = load_tf(mXp + j * 8);
+        TF Xwpj = Xp;
+        V3 rp = v3(0, 0, 0);
+        if (parent >= 0) {
+            const TF Xwp = load_tf(pbq + parent * 8);
+            Xwpj = tf_mul(Xwp, Xp);
+            const float* com = mCom + parent * 4;
+            rp = Xwpj.p - (Xwp.p + qrot(Xwp.q, v3(com[0], com[1], com[2])));
+        }
+        float* o = kPl + j * 16;
+        const V3 Rjl = qrot(Xwpj.q, v3(o[7], o[8], o[9]));
+        const V3 Rja = qrot(Xwpj.q, v3(o[10], o[11], o[12]));
+        o[7] = rp.x; o[8] = rp.y; o[9] = rp.z;
+        o[10] = Rjl.x; o[11] = Rjl.y; o[12] = Rjl.z;
+        o[13] = Rja.x; o[14] = Rja.y; o[15] = Rja.z;
     }
     wave_sync();
     for (int Lv = 0; Lv < M.num_levels; ++Lv) {
@@ -1408,24 +1456,15 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         for (int t = l0 + lane; t < l1; t += 64) {
             const int j = T.level_joint[t];
             const int parent = T.jparent[j], child = T.jchild[j];
-            const TF Xp = load_tf(mXp + j * 8);
-            TF Xwpj = Xp;
+            const float* pl = kPl + j * 16;
             V3 vlin = v3(0, 0, 0), vang = v3(0, 0, 0);
             if (parent >= 0) {
-                const TF Xwp = load_tf(pbq + parent * 8);
-                Xwpj = tf_mul(Xwp, Xp);
-                const float* com = mCom + parent * 4;
-                const V3 rp = Xwpj.p - (Xwp.p + qrot(Xwp.q, v3(com[0], com[1], com[2])));
                 const float* vp = pbqd + parent * 6;
                 vang = v3(vp[3], vp[4], vp[5]);
-                vlin = v3(vp[0], vp[1], vp[2]) + cross(vang, rp);
+                vlin = v3(vp[0], vp[1], vp[2]) + cross(vang, v3(pl[7], pl[8], pl[9]));
             }
-            const float* pl = kPl + j * 16;
-            const TF Xwc = tf_mul(Xwpj, load_tf(pl));
-            float* o = pbq + child * 8;
-            o[0] = Xwc.p.x; o[1] = Xwc.p.y; o[2] = Xwc.p.z; o[3] = Xwc.q.x; o[4] = Xwc.q.y; o[5] = Xwc.q.z; o[6] = Xwc.q.w;
-            const V3 lv = vlin + qrot(Xwpj.q, v3(pl[7], pl[8], pl[9]));
-            const V3 av3 = vang + qrot(Xwpj.q, v3(pl[10], pl[11], pl[12]));
+            const V3 lv = vlin + v3(pl[10], pl[11], pl[12]);
+            const V3 av3 = vang + v3(pl[13], pl[14], pl[15]);
             float* ov = pbqd + child * 6;
             ov[0] = lv.x; ov[1] = lv.y; ov[2] = lv.z; ov[3] = av3.x; ov[4] = av3.y; ov[5] = av3.z;
         }
