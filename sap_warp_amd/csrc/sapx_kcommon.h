@@ -184,9 +184,10 @@ __device__ __forceinline__ double project_contact(const double* k, const double 
     const double rt = k[0], rn = k[1], vhat = k[2], mu = k[3];
     const double rt_inv = k[4], rn_inv = k[5];
     const double y0 = -rt_inv * vc[0], y1 = -rt_inv * vc[1], y2 = rn_inv * (vhat - vc[2]);
-    // the soft norm keeps yr >= 1e-7 > 0
-    const double yr = sqrt(y0 * y0 + y1 * y1 + kSoftNormTol * kSoftNormTol);
-    const double yr_inv = 1.0 / yr;
+    // the soft norm keeps yr >= 1e-7 > 0; one reciprocal square root gives both yr and 1 / yr
+    const double yr2 = y0 * y0 + y1 * y1 + kSoftNormTol * kSoftNormTol;
+    const double yr_inv = rsqrt(yr2);
+    const double yr = yr2 * yr_inv;
     const double tx = y0 * yr_inv, ty = y1 * yr_inv;
     g[0] = g[1] = g[2] = 0.0;
     float Gm[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};

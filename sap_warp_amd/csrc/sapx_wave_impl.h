@@ -188,6 +188,7 @@ __device__ __forceinline__ float chol_solve(const float (&h)[NP], const float* L
 struct DofTerms {
     int pd, lo, up;
     float pda, pdg, pdl, lov, upv, lor, upr;
+    double pd_cw, lo_ri, up_ri;  // dt / gain (0 without gain), 1 / r_lower, 1 / r_upper
 };
 
 // component k (0..5) of the free-joint block that starts at dof ds of x (float64, LDS),
@@ -213,17 +214,17 @@ __device__ __forceinline__ double dof_terms(const DofTerms& m, double vi, double
         const double y = (double)m.pda - gain * vi;
         *imp += dt * clamp_s(y, lim);
         *hd += dt * gain * clamp_d(y, lim);
-        if (gain > 0.0) cost += (dt / gain) * clamp_a(y, lim);
+        cost += m.pd_cw * clamp_a(y, lim);
     }
     if (m.lo) {
         const double r = m.lor;
-        const double g = (1.0 / r) * ((double)m.lov - vi);
-        if (g > 0.0) { *imp += g; *hd += 1.0 / r; cost += 0.5 * r * g * g; }
+        const double g = m.lo_ri * ((double)m.lov - vi);
+        if (g > 0.0) { *imp += g; *hd += m.lo_ri; cost += 0.5 * r * g * g; }
     }
     if (m.up) {
         const double r = m.upr;
-        const double g = (1.0 / r) * ((double)m.upv + vi);
-        if (g > 0.0) { *imp -= g; *hd += 1.0 / r; cost += 0.5 * r * g * g; }
+        const double g = m.up_ri * ((double)m.upv + vi);
+        if (g > 0.0) { *imp -= g; *hd += m.up_ri; cost += 0.5 * r * g * g; }
     }
     return cost;
 }
@@ -850,7 +851,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
 
     // ------------------------------------------------------------------ 7
     // PD + limit terms (contact_solve.py:490-683) and participating dofs (422-487)
-    DofTerms mt = {0, 0, 0, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f};
+    DofTerms mt = {0, 0, 0, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, 0.0, 1.0, 1.0};
     int part = 0;
     if (lane < n) {
         const int i = lane;
@@ -894,6 +895,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                     mt.pda = (float)(kpe * (qt - q0) + kde * vd + u0);
                     mt.pdg = (float)gain;
                     mt.pdl = M.effort_limit[gt];
+                    mt.pd_cw = mt.pdg > 0.0f ? dt / (double)mt.pdg : 0.0;
                 }
             }
         }
@@ -909,9 +911,11 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 const double r = fmax(lbf * fmax(ainv, 1.0e-12), 1.0 / (dt * ke * (dt + tau_d)));
                 if (isfinite(lower) && q0 - lower <= window) {
                     mt.lo = 1; mt.lov = (float)(-(q0 - lower) / (dt + tau_d)); mt.lor = (float)r;
+                    mt.lo_ri = 1.0 / (double)mt.lor;
                 }
                 if (isfinite(upper) && upper - q0 <= window) {
                     mt.up = 1; mt.upv = (float)(-(upper - q0) / (dt + tau_d)); mt.upr = (float)r;
+                    mt.up_ri = 1.0 / (double)mt.upr;
                 }
             }
         }
