@@ -162,11 +162,13 @@ def train_learned(args, W, C, dt, dev):
     return lws, info
 
 
-def timed_run(args, W, C, dt, dev, lws_trained, graph=False):
+def timed_run(args, W, C, dt, dev, lws_trained, graph=False, schedule=None, tune=False):
     """Warmup + timed steps on the benchmark seed, plain (lws_trained None) or from the learned start.
 
     graph=True captures the timed steps (PD-target update, collide, step) into one HIP graph
     after the warmup and times one replay of it (no per-step events inside the graph).
+    schedule sets the solver's work schedule; tune=True runs the autotuner on the warmed-up
+    state first (sap_warp_amd/autotune.py) and keeps its choice.
     """
     import numpy as np
     import torch
@@ -174,6 +176,8 @@ def timed_run(args, W, C, dt, dev, lws_trained, graph=False):
     from sap_warp_amd.learned import LearnedWarmStart
 
     model, solver, pipe = _scene_and_solver(W, C, args.seed, dev, args.line_search)
+    if schedule is not None:
+        solver.set_schedule(**schedule)
     lws = None
     if lws_trained is not None:
         lws = LearnedWarmStart(solver, mlp=lws_trained.mlp)
@@ -197,6 +201,14 @@ def timed_run(args, W, C, dt, dev, lws_trained, graph=False):
         pipe.collide(s0, contacts)
         solve(s0, s1)
         s0, s1 = s1, s0
+    torch.cuda.synchronize()
+    tuning = None
+    if tune:
+        from sap_warp_amd.autotune import autotune
+
+        tuning = autotune(solver, pipe, s0, ctl, contacts, dt)
+        # the tuner's own steps restarted from s0; the timed run continues from s0 as well,
+        # with the warm-start guess of the last warmup step replaced by the plain start
     torch.cuda.synchronize()
     solver.iteration_histogram(reset=True)
     ev_start = torch.cuda.Event(enable_timing=True)
@@ -267,6 +279,7 @@ def timed_run(args, W, C, dt, dev, lws_trained, graph=False):
     cdf = np.cumsum(iters) / max(ws, 1)
     stats = solver.iteration_stats()
     return {
+        "schedule": solver.get_schedule(), "tuning": tuning,
         "model": model, "n": model.dof_per_env, "nb": model.bodies_per_env,
         "value": W * args.steps / (ms_total / 1000.0), "ms_per_step": ms_total / args.steps,
         "step_ms": step_ms, "pre_ms": pre_ms, "collide_ms": collide_ms, "wall": wall,
@@ -304,6 +317,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=24)
     ap.add_argument("--learned", type=int, default=1, help="train and time the learned warm start (1) or not (0)")
     ap.add_argument("--graph", type=int, default=1, help="time the plain steps as one HIP graph replay (1) or not (0)")
+    ap.add_argument("--autotune", type=int, default=1, help="choose the step's work schedule by timing (1) or keep the default (0)")
     ap.add_argument("--train-seeds", type=int, default=2)
     ap.add_argument("--train-steps", type=int, default=250)
     ap.add_argument("--train-epochs", type=int, default=12)
@@ -331,9 +345,10 @@ def main() -> int:
     train_info = None
     if args.learned:
         lws, train_info = train_learned(args, W, C, dt, dev)
-    plain = timed_run(args, W, C, dt, dev, None)
-    plain_graph = timed_run(args, W, C, dt, dev, None, graph=True) if args.graph else None
-    learned = timed_run(args, W, C, dt, dev, lws) if lws is not None else None
+    plain = timed_run(args, W, C, dt, dev, None, tune=bool(args.autotune))
+    sched = plain["schedule"]
+    plain_graph = timed_run(args, W, C, dt, dev, None, graph=True, schedule=sched) if args.graph else None
+    learned = timed_run(args, W, C, dt, dev, lws, schedule=sched) if lws is not None else None
     head = plain  # the benchmark workload is plain SAP; the learned start is reported beside it
     n, nb = head["n"], head["nb"]
     # roofline of the dominant kernel, the step (the wave kernel pair sapx_step launches on the
@@ -379,6 +394,8 @@ def main() -> int:
                    "kernel_ms and roofline from a second, eager run of the same steps with per-step HIP events")
                   if plain_graph is not None else "eager launches, HIP events around the timed steps",
         "eager": {"value": round(plain["value"], 1), "ms_per_step": round(plain["ms_per_step"], 4)},
+        "schedule": sched,
+        "autotune_ms_per_step": plain["tuning"]["ms_per_step"] if plain["tuning"] else None,
         "wall_s": round(head["wall"], 3),
     }
     if plain_graph is not None:

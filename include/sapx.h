@@ -286,6 +286,29 @@ int32_t sapx_features(SapxSolver* solver, const SapxStateView* state_in, const S
  */
 int32_t sapx_solver_set_warm_start_io(SapxSolver* solver, const float* start_correction, float* target_out);
 
+/* ---------------------------------------------------------------------------
+ * Work schedule of sapx_step.  No counterpart in the reference (it has one code
+ * path): how the worlds of a step map onto kernels, chosen per solver by an
+ * autotuner that times the candidates on the caller's batch
+ * (sap_warp_amd/autotune.py) or set directly.  Every schedule computes the
+ * same step.
+ * ------------------------------------------------------------------------- */
+#define SAPX_KERNEL_WAVE 0  /* one 64-lane wave per world (dofs <= 64, capacity <= 64)  */
+#define SAPX_KERNEL_FUSED 1 /* one 256-thread workgroup per world                       */
+
+typedef struct SapxSchedule {
+    int32_t kernel;         /* SAPX_KERNEL_*                                                   */
+    int32_t small_capacity; /* wave: contact slots (16, 32 or 64) of the variant every world
+                               with at most that many contacts runs in; the 64-slot variant
+                               takes the others                                                 */
+    int32_t fork;           /* wave: launch the 64-slot variant on a second stream beside it    */
+    int32_t world_order;    /* wave: dispatch worlds most-expensive-first                       */
+} SapxSchedule;
+
+int32_t sapx_solver_get_schedule(const SapxSolver* solver, SapxSchedule* out);
+/* SAPX_ERR_UNSUPPORTED when the model does not fit the requested kernel / variant. */
+int32_t sapx_solver_set_schedule(SapxSolver* solver, const SapxSchedule* schedule);
+
 typedef struct SapxMlp SapxMlp;
 
 /* buffer ids of sapx_mlp_write / sapx_mlp_read */

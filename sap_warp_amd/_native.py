@@ -88,6 +88,13 @@ class SapxContactsView(ctypes.Structure):
     ]
 
 
+class SapxSchedule(ctypes.Structure):
+    _fields_ = [("kernel", _I), ("small_capacity", _I), ("fork", _I), ("world_order", _I)]
+
+
+KERNEL_WAVE = 0
+KERNEL_FUSED = 1
+
 # declared entry points: name -> (restype, argtypes); tests check the export table
 EXPORTS = {
     "sapx_version": (_I, []),
@@ -109,6 +116,8 @@ EXPORTS = {
     "sapx_features": (_I, [_P, ctypes.POINTER(SapxStateView), ctypes.POINTER(SapxControlView),
                            ctypes.POINTER(SapxContactsView), _P, _P, _P]),
     "sapx_solver_set_warm_start_io": (_I, [_P, _P, _P]),
+    "sapx_solver_get_schedule": (_I, [_P, ctypes.POINTER(SapxSchedule)]),
+    "sapx_solver_set_schedule": (_I, [_P, ctypes.POINTER(SapxSchedule)]),
     "sapx_mlp_create": (_I, [_I, _P, _I, _I, ctypes.POINTER(_P)]),
     "sapx_mlp_destroy": (None, [_P]),
     "sapx_mlp_param_count": (ctypes.c_int64, [_P]),

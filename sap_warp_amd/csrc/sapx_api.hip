@@ -93,10 +93,11 @@ struct SapxSolver {
     int npad = 0, js = 0, ast = 0;
     int lds[17] = {0};
     int smem_bytes = 0;
-    // one-wave-per-world kernel: contact variants (capacity 16, and 64 for the rest)
-    bool wave = false;
-    WaveLayout wl_small{}, wl_big{};
-    int cb_small = 0, cb_big = 0;
+    // one-wave-per-world kernel: LDS layouts of the contact variants (16 / 32 / 64 slots; a
+    // variant whose layout does not fit has bytes == 0) and the current schedule
+    bool wave_ok = false;
+    WaveLayout wl16{}, wl32{}, wl64{};
+    SapxSchedule sched{};
     // the 64-slot variant runs on a second stream (default priority), forked from and joined
     // back into the caller's stream, so its few long worlds overlap the 16-slot launch
     hipStream_t aux = nullptr;
@@ -198,10 +199,38 @@ int32_t sapx_features(SapxSolver* s, const SapxStateView* in, const SapxControlV
 
 int32_t sapx_solver_set_warm_start_io(SapxSolver* s, const float* start_correction, float* target_out) {
     if (!s) return fail(SAPX_ERR_INVALID, "null argument");
-    if ((start_correction || target_out) && !s->wave)
+    if ((start_correction || target_out) && s->sched.kernel != SAPX_KERNEL_WAVE)
         return fail(SAPX_ERR_UNSUPPORTED, "the learned warm start needs the one-wave step kernel");
     s->learn_delta = start_correction;
     s->learn_target = target_out;
+    return SAPX_OK;
+}
+
+int32_t sapx_solver_get_schedule(const SapxSolver* s, SapxSchedule* out) {
+    if (!s || !out) return fail(SAPX_ERR_INVALID, "null argument");
+    *out = s->sched;
+    return SAPX_OK;
+}
+
+int32_t sapx_solver_set_schedule(SapxSolver* s, const SapxSchedule* in) {
+    if (!s || !in) return fail(SAPX_ERR_INVALID, "null argument");
+    if (in->kernel != SAPX_KERNEL_WAVE && in->kernel != SAPX_KERNEL_FUSED) return fail(SAPX_ERR_INVALID, "unknown kernel");
+    if (in->kernel == SAPX_KERNEL_WAVE) {
+        if (!s->wave_ok) return fail(SAPX_ERR_UNSUPPORTED, "the model does not fit the one-wave step kernel");
+        const int c = in->small_capacity;
+        if (c != 16 && c != 32 && c != 64) return fail(SAPX_ERR_INVALID, "small_capacity must be 16, 32 or 64");
+        const WaveLayout& L = c == 16 ? s->wl16 : (c == 32 ? s->wl32 : s->wl64);
+        if (L.bytes == 0) return fail(SAPX_ERR_UNSUPPORTED, "that contact variant does not fit in LDS");
+        if (s->opts.max_rigid_contact > c && s->wl64.bytes == 0)
+            return fail(SAPX_ERR_UNSUPPORTED, "the 64-slot variant does not fit in LDS");
+    } else if (s->learn_delta || s->learn_target) {
+        return fail(SAPX_ERR_UNSUPPORTED, "the learned warm start needs the one-wave step kernel");
+    }
+    SapxSchedule t = *in;
+    t.fork = t.fork ? 1 : 0;
+    t.world_order = t.world_order ? 1 : 0;
+    if (t.fork && !s->aux) t.fork = 0;  // capacity <= 16: a single variant, nothing to fork
+    s->sched = t;
     return SAPX_OK;
 }
 
@@ -612,25 +641,32 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
             L.bytes = (int)o;
             return L;
         };
-        // SAPX_STEP_KERNEL: "fused" selects the 256-thread kernel, "wave64" sends every world
-        // through the 64-slot wave variant (both for tests and comparisons)
+        if (NP <= 64 && C <= 64) {
+            auto fit = [&](int CB) {
+                WaveLayout L = layout(CB);
+                if (L.bytes > 160 * 1024) L.bytes = 0;
+                return L;
+            };
+            s->wl16 = fit(16);
+            s->wl32 = fit(32);
+            s->wl64 = fit(64);
+            // the 64-slot variant holds every world; the small ones need it beside them when C > slots
+            s->wave_ok = s->wl64.bytes > 0 || (C <= 16 && s->wl16.bytes > 0) || (C <= 32 && s->wl32.bytes > 0);
+        }
+        // default schedule; SAPX_STEP_KERNEL ("fused", "wave64") and SAPX_WORLD_ORDER=0 override it
+        // (comparisons and tests)
         const char* env = getenv("SAPX_STEP_KERNEL");
         const std::string kenv = env ? std::string(env) : std::string();
-        if (kenv != "fused" && NP <= 64 && C <= 64) {
-            const bool only_big = kenv == "wave64";
-            s->cb_small = only_big ? 0 : 16;
-            if (s->cb_small) s->wl_small = layout(16);
-            s->cb_big = (C > 16 || only_big) ? 64 : 0;
-            if (s->cb_big) s->wl_big = layout(64);
-            s->wave = (!s->cb_small || s->wl_small.bytes <= 160 * 1024) && (!s->cb_big || s->wl_big.bytes <= 160 * 1024);
-        }
-        // SAPX_WORLD_ORDER=0 keeps the identity order (comparisons)
         const char* oenv = getenv("SAPX_WORLD_ORDER");
-        if (s->wave && !(oenv && std::string(oenv) == "0") && hipMalloc(&s->order, (size_t)dm.W * sizeof(int)) != hipSuccess) {
+        s->sched.kernel = (s->wave_ok && kenv != "fused") ? SAPX_KERNEL_WAVE : SAPX_KERNEL_FUSED;
+        s->sched.small_capacity = kenv == "wave64" ? 64 : 16;
+        s->sched.fork = 1;
+        s->sched.world_order = (oenv && std::string(oenv) == "0") ? 0 : 1;
+        if (s->wave_ok && hipMalloc(&s->order, (size_t)dm.W * sizeof(int)) != hipSuccess) {
             delete s;
             return fail(SAPX_ERR_HIP, "hipMalloc failed for the world order");
         }
-        if (s->wave && s->cb_small && s->cb_big) {
+        if (s->wave_ok && C > 16) {
             if (hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
                 hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming) != hipSuccess) {
@@ -842,7 +878,7 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
     P.o_bqd = s->lds[15];
     P.o_topo = s->lds[16];
     P.smem_bytes = s->smem_bytes;
-    if (s->wave) {
+    if (s->sched.kernel == SAPX_KERNEL_WAVE) {
         auto launch_on = [&](const StepParams& Q, int cb, hipStream_t sq) {
             switch (s->npad) {
                 case 16: return launch_wave_np16(Q, cb, sq);
@@ -854,29 +890,37 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
         P.world_order = nullptr;
         P.learn_delta = s->learn_delta;
         P.learn_target = s->learn_target;
-        if (s->order) {
+        if (s->order && s->sched.world_order) {
             HIP_TRY(launch_world_order(P.cworld_start, P.st_iters, (int)dm.W, P.C, s->order, st));
             P.world_order = s->order;
         }
-        if (s->cb_small && s->cb_big) {
-            // fork: the worlds above 16 contacts start first on the second stream, the rest
-            // fill the remaining wave slots from the caller's stream; join before returning
-            StepParams Pb = P;
-            Pb.wl = s->wl_big;
-            Pb.wave_min_nc = s->cb_small + 1;
-            HIP_TRY(hipEventRecord(s->ev_fork, st));
-            HIP_TRY(hipStreamWaitEvent(s->aux, s->ev_fork, 0));
-            HIP_TRY(launch_on(Pb, s->cb_big, s->aux));
-            HIP_TRY(hipEventRecord(s->ev_join, s->aux));
-            P.wl = s->wl_small;
+        // contact variants: the small one holds every world when the capacity fits in it,
+        // otherwise the 64-slot variant takes the worlds above its slots
+        const int small = s->sched.small_capacity;
+        const WaveLayout& wls = small == 16 ? s->wl16 : (small == 32 ? s->wl32 : s->wl64);
+        if (P.C <= small || small == 64) {
+            P.wl = wls;
             P.wave_min_nc = 0;
-            HIP_TRY(launch_on(P, s->cb_small, st));
-            HIP_TRY(hipStreamWaitEvent(st, s->ev_join, 0));
+            HIP_TRY(launch_on(P, small, st));
         } else {
-            const int cb = s->cb_small ? s->cb_small : s->cb_big;
-            P.wl = s->cb_small ? s->wl_small : s->wl_big;
+            StepParams Pb = P;
+            Pb.wl = s->wl64;
+            Pb.wave_min_nc = small + 1;
+            P.wl = wls;
             P.wave_min_nc = 0;
-            HIP_TRY(launch_on(P, cb, st));
+            if (s->sched.fork) {
+                // fork: the worlds above `small` contacts start first on the second stream, the
+                // rest fill the remaining wave slots from the caller's stream; join before returning
+                HIP_TRY(hipEventRecord(s->ev_fork, st));
+                HIP_TRY(hipStreamWaitEvent(s->aux, s->ev_fork, 0));
+                HIP_TRY(launch_on(Pb, 64, s->aux));
+                HIP_TRY(hipEventRecord(s->ev_join, s->aux));
+                HIP_TRY(launch_on(P, small, st));
+                HIP_TRY(hipStreamWaitEvent(st, s->ev_join, 0));
+            } else {
+                HIP_TRY(launch_on(P, small, st));
+                HIP_TRY(launch_on(Pb, 64, st));
+            }
         }
     } else {
         if (s->learn_delta || s->learn_target)

@@ -4,6 +4,7 @@
 namespace sapx {
 hipError_t launch_wave_np16(const StepParams& P, int cb, hipStream_t stream) {
     if (cb == 16) return wv::launch_variant<16, 16>(P, stream);
+    if (cb == 32) return wv::launch_variant<16, 32>(P, stream);
     if (cb == 64) return wv::launch_variant<16, 64>(P, stream);
     return hipErrorInvalidValue;
 }

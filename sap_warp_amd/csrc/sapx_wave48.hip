@@ -4,6 +4,7 @@
 namespace sapx {
 hipError_t launch_wave_np48(const StepParams& P, int cb, hipStream_t stream) {
     if (cb == 16) return wv::launch_variant<48, 16>(P, stream);
+    if (cb == 32) return wv::launch_variant<48, 32>(P, stream);
     if (cb == 64) return wv::launch_variant<48, 64>(P, stream);
     return hipErrorInvalidValue;
 }

@@ -391,6 +391,20 @@ class SolverSAP:
         return state_out
 
     # ------------------------------------------------------------------
+    def get_schedule(self) -> dict:
+        """Current work schedule of the native step (``sapx_solver_get_schedule``)."""
+        sc = nat.SapxSchedule()
+        nat.check(self._lib.sapx_solver_get_schedule(self._handle, ctypes.byref(sc)), "sapx_solver_get_schedule")
+        return {"kernel": "wave" if sc.kernel == nat.KERNEL_WAVE else "fused", "small_capacity": sc.small_capacity,
+                "fork": bool(sc.fork), "world_order": bool(sc.world_order)}
+
+    def set_schedule(self, kernel: str = "wave", small_capacity: int = 16, fork: bool = True,
+                     world_order: bool = True) -> None:
+        """Set the work schedule (raises NotImplementedError when the model does not fit it)."""
+        sc = nat.SapxSchedule(nat.KERNEL_WAVE if kernel == "wave" else nat.KERNEL_FUSED, int(small_capacity),
+                              int(bool(fork)), int(bool(world_order)))
+        nat.check(self._lib.sapx_solver_set_schedule(self._handle, ctypes.byref(sc)), "sapx_solver_set_schedule")
+
     def iteration_stats(self) -> SapContactSolveResult:
         """Per-world Newton/line-search statistics of the last step (synchronises)."""
         W = self.num_envs

@@ -49,6 +49,7 @@ STRUCTS = {
     "SapxStateView": nat.SapxStateView,
     "SapxControlView": nat.SapxControlView,
     "SapxContactsView": nat.SapxContactsView,
+    "SapxSchedule": nat.SapxSchedule,
 }
 
 
@@ -134,3 +135,15 @@ def test_library_has_no_cpu_fallback_symbols():
     lib = nat.load()
     assert not hasattr(lib, "sapx_oracle_collide")
     assert np.dtype(np.int32).itemsize == 4
+
+
+def test_autotune_candidates_cover_the_schedule_space():
+    from sap_warp_amd.autotune import candidate_schedules
+
+    small = candidate_schedules(16)
+    assert {(c["kernel"], c["small_capacity"]) for c in small} == {("fused", 64), ("wave", 16)}
+    full = candidate_schedules(64)
+    keys = {(c["kernel"], c["small_capacity"], c["fork"], c["world_order"]) for c in full}
+    assert len(keys) == len(full) == 11
+    assert ("wave", 16, True, True) in keys and ("wave", 32, False, False) in keys and ("wave", 64, False, True) in keys
+    assert not any(c["small_capacity"] == 64 and c["fork"] for c in full)

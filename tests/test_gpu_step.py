@@ -45,7 +45,7 @@ def _control_np(c):
                 joint_target_vel=_np(c.joint_target_vel), joint_act=_np(c.joint_act))
 
 
-def _run_and_compare(scene, preset, ls, worlds, steps, seed=0):
+def _run_and_compare(scene, preset, ls, worlds, steps, seed=0, schedule=None):
     import torch
 
     from sap_warp_amd.collision import SapCollisionPipeline
@@ -55,6 +55,8 @@ def _run_and_compare(scene, preset, ls, worlds, steps, seed=0):
     solver = SolverSAP(model, max_rigid_contact=C, line_search_variant=ls, contact_preset_variant=preset,
                        capture_contact_jacobian_snapshots=True,
                        collect_iteration_stats=True)
+    if schedule is not None:
+        solver.set_schedule(**schedule)
     pipe = SapCollisionPipeline(model, max_contacts_per_world=C)
     contacts = pipe.contacts()
     s0, s1 = model.state(), model.state()
@@ -107,3 +109,10 @@ def test_humanoid_step_parity_other_kernels(gpu_device, monkeypatch, kernel):
 
 def test_ant_step_parity(gpu_device):
     _run_and_compare("ant", "approx32", "armijo_decay", worlds=8, steps=5)
+
+
+@pytest.mark.parametrize("small,fork,order", [(32, True, True), (32, False, False), (16, False, True)])
+def test_humanoid_step_parity_schedules(gpu_device, small, fork, order):
+    """Other work schedules of the wave kernel (32-slot variant, serial variants, identity order)."""
+    _run_and_compare("humanoid", "drake", "exact_root", worlds=6, steps=3,
+                     schedule={"kernel": "wave", "small_capacity": small, "fork": fork, "world_order": order})
