@@ -298,14 +298,19 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     const int wp4 = M.wp_stride >> 2, tp4 = M.topo_ints >> 2;
     const float4* wsrc = reinterpret_cast<const float4*>(M.world_params + (size_t)w * M.wp_stride);
     const int4* tsrc = reinterpret_cast<const int4*>(M.topo);
+    // (clamped indices: every slot is loaded unconditionally, so the buffers stay in VGPRs)
     float4 wbuf[kPre];
     int4 tbuf[2];
 #pragma unroll
-    for (int k = 0; k < kPre; ++k)
-        if (k * 64 + lane < wp4) wbuf[k] = wsrc[k * 64 + lane];
+    for (int k = 0; k < kPre; ++k) {
+        const int i = k * 64 + lane;
+        wbuf[k] = wsrc[i < wp4 ? i : 0];
+    }
 #pragma unroll
-    for (int k = 0; k < 2; ++k)
-        if (k * 64 + lane < tp4) tbuf[k] = tsrc[k * 64 + lane];
+    for (int k = 0; k < 2; ++k) {
+        const int i = k * 64 + lane;
+        tbuf[k] = tsrc[i < tp4 ? i : 0];
+    }
     float qd_l = 0.0f, jf_l = 0.0f;
     if (lane < n) {
         qd_l = P.jqd_in[(size_t)w * n + lane];
