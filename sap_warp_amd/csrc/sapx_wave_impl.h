@@ -72,7 +72,8 @@ __device__ __forceinline__ void matvec_AJ(const float* arow, const float* J0, in
 }
 
 // (J^T y)_lane = sum_c sum_r J[3c+r][lane] y_{c,r}, y broadcast from lane c (zero for
-// c >= nc; J rows of unused slots are zero).  Contacts in groups of four.
+// c >= nc; J rows of unused slots are zero).  Contacts in groups of four: the capacity CB is
+// a multiple of four, so the last group ends at slot CB - 1 at most.
 __device__ __forceinline__ double jt_times(const float* Jm, int js, int nc, int lane, double y0, double y1, double y2) {
     double acc0 = 0.0, acc1 = 0.0;
     const int col = lane < js ? lane : 0;
