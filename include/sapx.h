@@ -28,6 +28,7 @@ extern "C" {
 
 #define SAPX_LS_ARMIJO_DECAY 0
 #define SAPX_LS_EXACT_ROOT 1
+#define SAPX_LS_MONOTONE_DECAY 2 /* one-wave step kernel only */
 
 #define SAPX_WEIGHT_BODY_INERTIA 0
 #define SAPX_WEIGHT_DIAG_DELASSUS 1

@@ -974,7 +974,11 @@ class SolveOptions:
         if variant == "exact_root" and o.ls_max_iterations == 40:
             o.ls_max_iterations = EXACT_MAX_ITERATIONS
         if variant == "monotone_decay":
-            o.cost_abs_tol, o.cost_rel_tol = 0.0, 5.0e-3
+            # the reference's defaults for this variant (contact_solve.py:6179-6181)
+            if "cost_abs_tol" not in kw:
+                o.cost_abs_tol = 0.0
+            if "cost_rel_tol" not in kw:
+                o.cost_rel_tol = 5.0e-3
         return o
 
 
@@ -1119,6 +1123,12 @@ def sap_solve(model, wv: WorldView, A, v_star, v0, v_guess, cs: ContactSet, mt: 
             previous_cost = ev["cost"]
             v = v + alpha * dv
             ls_total += ls_it
+        elif opts.line_search == "monotone_decay":
+            # the fused decay search leaves v, records the tried step and the current cost as the
+            # previous one (contact_solve.py:2647-2662): the cost criterion ends the solve next
+            alpha = a_new
+            previous_cost = ev["cost"]
+            ls_total += ls_it
         iteration += 1
 
 
@@ -1152,7 +1162,35 @@ def line_search(prob, v, dv, ev, opts: SolveOptions):
         return exact_line_search(prob, v, dv, ev, opts)
     if opts.line_search == "armijo_decay":
         return armijo_line_search(prob, v, dv, ev, opts)
+    if opts.line_search == "monotone_decay":
+        return monotone_line_search(prob, v, dv, ev, opts)
     raise NotImplementedError(opts.line_search)
+
+
+MONOTONE_DECAY = 0.5
+MONOTONE_MIN_ALPHA = 1.0e-8
+MONOTONE_RELAX_R = 1.0e-12
+MONOTONE_RELAX_A = 1.0e-14
+
+
+def monotone_line_search(prob, v, dv, ev, opts: SolveOptions):
+    """Fused halving search (contact_solve.py:2486-2665, 5307-5415): the first alpha = 1, 1/2, 1/4 ...
+    whose cost does not exceed the current one (relaxed by 1e-14 + 1e-12 |cost|); no descent test.
+    Returns (accepted, alpha, trials, status); status -2 when alpha fell below 1e-8."""
+    ld = _line_data(prob, v, dv, ev)
+    current = ev["cost"]
+    tol = MONOTONE_RELAX_A + MONOTONE_RELAX_R * abs(current)
+    alpha = 1.0
+    trials = 0
+    for _ in range(opts.ls_max_iterations):
+        ell, _, _ = _trial(prob, v, dv, ld, alpha)
+        trials += 1
+        if ell <= current + tol:
+            return True, alpha, trials, 0
+        alpha *= MONOTONE_DECAY
+        if alpha < MONOTONE_MIN_ALPHA:
+            return False, alpha, trials, -2
+    return False, alpha, trials, 0
 
 
 def armijo_line_search(prob, v, dv, ev, opts: SolveOptions):

@@ -22,6 +22,7 @@ SAPX_ERR_CAPACITY = -4
 
 LS_ARMIJO_DECAY = 0
 LS_EXACT_ROOT = 1
+LS_MONOTONE_DECAY = 2
 WEIGHT_BODY_INERTIA = 0
 WEIGHT_DIAG_DELASSUS = 1
 POINT_WITNESS = 0

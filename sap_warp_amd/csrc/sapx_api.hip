@@ -225,6 +225,8 @@ int32_t sapx_solver_set_schedule(SapxSolver* s, const SapxSchedule* in) {
             return fail(SAPX_ERR_UNSUPPORTED, "the 64-slot variant does not fit in LDS");
     } else if (s->learn_delta || s->learn_target) {
         return fail(SAPX_ERR_UNSUPPORTED, "the learned warm start needs the one-wave step kernel");
+    } else if (s->opts.line_search == SAPX_LS_MONOTONE_DECAY) {
+        return fail(SAPX_ERR_UNSUPPORTED, "monotone_decay runs in the one-wave step kernel only");
     }
     SapxSchedule t = *in;
     t.fork = t.fork ? 1 : 0;
@@ -566,7 +568,8 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
     const DevModel& dm = model->dm;
     const int C = opts->max_rigid_contact;
     if (C <= 0) return fail(SAPX_ERR_INVALID, "max_rigid_contact must be positive");
-    if (opts->line_search != SAPX_LS_ARMIJO_DECAY && opts->line_search != SAPX_LS_EXACT_ROOT)
+    if (opts->line_search != SAPX_LS_ARMIJO_DECAY && opts->line_search != SAPX_LS_EXACT_ROOT &&
+        opts->line_search != SAPX_LS_MONOTONE_DECAY)
         return fail(SAPX_ERR_UNSUPPORTED, "line search variant not in the native step");
     if (opts->line_search == SAPX_LS_ARMIJO_DECAY && !(opts->rho > 0.0 && opts->rho < 1.0))
         return fail(SAPX_ERR_INVALID, "rho must lie in (0, 1)");
@@ -662,6 +665,10 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
         s->sched.small_capacity = kenv == "wave64" ? 64 : 16;
         s->sched.fork = 1;
         s->sched.world_order = (oenv && std::string(oenv) == "0") ? 0 : 1;
+        if (opts->line_search == SAPX_LS_MONOTONE_DECAY && s->sched.kernel != SAPX_KERNEL_WAVE) {
+            delete s;
+            return fail(SAPX_ERR_UNSUPPORTED, "monotone_decay runs in the one-wave step kernel only");
+        }
         if (s->wave_ok && hipMalloc(&s->order, (size_t)dm.W * sizeof(int)) != hipSuccess) {
             delete s;
             return fail(SAPX_ERR_HIP, "hipMalloc failed for the world order");

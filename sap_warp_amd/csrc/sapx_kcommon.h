@@ -16,6 +16,11 @@ constexpr double kLimitBeta = 0.1;        // contact_solve.py:49
 constexpr double kLimitWindow = 2.0;      // contact_solve.py:51
 constexpr double kExactAlphaMax = 1.5;    // contact_solve.py:52
 constexpr double kExactFTol = 1.0e-8;     // contact_solve.py:54
+constexpr int kLsMonotone = 2;            // SAPX_LS_MONOTONE_DECAY
+constexpr double kMonoDecay = 0.5;        // monotone_decay constants (contact_solve.py:5431-5443)
+constexpr double kMonoMinAlpha = 1.0e-8;
+constexpr double kMonoRelaxR = 1.0e-12;
+constexpr double kMonoRelaxA = 1.0e-14;
 constexpr int kNewtonRefine = 1;          // float64-residual refinement steps per Newton direction
 constexpr int kCk = 8;                    // doubles of per-contact projection constants
 

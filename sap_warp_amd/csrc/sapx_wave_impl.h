@@ -1011,7 +1011,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     double grad = 0.0, ar = 0.0, dvv = 0.0;
     float hd = 0.0f;
     double alpha = 1.0, prev_cost = 0.0;
-    const double cost_min_alpha = 0.5;
+    const double cost_min_alpha = P.ls_mode == kLsMonotone ? 0.0 : 0.5;  // contact_solve.py:6182
     if (unconstrained) {
         flags = 1;
     } else {
@@ -1195,7 +1195,20 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
             double a_acc = 1.0;
             int ls_it = 0;
             ls_status = 0;
-            if (!(dell0 < 0.0) || !isfinite(dell0)) {
+            if (P.ls_mode == kLsMonotone) {
+                // fused halving search (contact_solve.py:2486-2665): the first alpha = 1, 1/2, ...
+                // whose cost does not exceed the current one, relaxed by 1e-14 + 1e-12 |cost|
+                const double tol = kMonoRelaxA + kMonoRelaxR * fabs(cost);
+                double al = 1.0, dd1, dd2;
+                for (int it = 0; it < P.ls_max_iter; ++it) {
+                    const double ell = trial(al, false, &dd1, &dd2);
+                    ++ls_it;
+                    if (ell <= cost + tol) { accepted = true; a_acc = al; break; }
+                    al *= kMonoDecay;
+                    if (al < kMonoMinAlpha) { ls_status = -2; break; }
+                }
+                if (!accepted) a_acc = al;
+            } else if (!(dell0 < 0.0) || !isfinite(dell0)) {
                 ls_status = -1;
             } else if (P.ls_mode == 0) {
                 // Armijo backtracking (contact_solve.py:2670-2905, 5108-5305)
@@ -1303,6 +1316,12 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 prev_cost = cost;
                 ls_total += ls_it;
                 if (lane < n) v += alpha * dvv;
+            } else if (P.ls_mode == kLsMonotone) {
+                // a rejected decay search keeps v and records the current cost as the previous one
+                // (contact_solve.py:2647-2662): the cost criterion ends the solve next
+                alpha = a_acc;
+                prev_cost = cost;
+                ls_total += ls_it;
             }
             ++iteration;
         }

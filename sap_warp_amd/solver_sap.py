@@ -182,12 +182,11 @@ class SolverSAP:
         self.line_search_variant = normalize_sap_line_search_mode(line_search_variant)
         if self.line_search_variant == "armijo_decay" and not (0.0 < float(rho) < 1.0):
             raise ValueError(f"rho must lie in (0, 1), got {rho!r}.")
-        if self.line_search_variant == "monotone_decay":
-            raise NotImplementedError("line_search_variant='monotone_decay' is not in the native step yet")
+        # cost-criterion defaults per variant (contact_solve.py:6179-6182)
         if cost_abs_tol is None:
-            cost_abs_tol = 1.0e-30
+            cost_abs_tol = 0.0 if self.line_search_variant == "monotone_decay" else 1.0e-30
         if cost_rel_tol is None:
-            cost_rel_tol = 1.0e-15
+            cost_rel_tol = 5.0e-3 if self.line_search_variant == "monotone_decay" else 1.0e-15
         self.cost_abs_tol = float(cost_abs_tol)
         self.cost_rel_tol = float(cost_rel_tol)
         self.line_search_max_iterations = int(line_search_max_iterations)
@@ -246,7 +245,8 @@ class SolverSAP:
         opts.optimality_rel_tol = self.optimality_rel_tol
         opts.cost_abs_tol = self.cost_abs_tol
         opts.cost_rel_tol = self.cost_rel_tol
-        opts.line_search = nat.LS_EXACT_ROOT if self.line_search_variant == "exact_root" else nat.LS_ARMIJO_DECAY
+        opts.line_search = {"exact_root": nat.LS_EXACT_ROOT, "armijo_decay": nat.LS_ARMIJO_DECAY,
+                            "monotone_decay": nat.LS_MONOTONE_DECAY}[self.line_search_variant]
         opts.line_search_max_iterations = (self.static_substep_line_search_iterations if self.static_substep
                                            else self.line_search_max_iterations)
         opts.armijo_c = self.armijo_c

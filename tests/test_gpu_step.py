@@ -45,7 +45,7 @@ def _control_np(c):
                 joint_target_vel=_np(c.joint_target_vel), joint_act=_np(c.joint_act))
 
 
-def _run_and_compare(scene, preset, ls, worlds, steps, seed=0, schedule=None):
+def _run_and_compare(scene, preset, ls, worlds, steps, seed=0, schedule=None, cost_rel_tol=None, vel_tol=VEL_TOL):
     import torch
 
     from sap_warp_amd.collision import SapCollisionPipeline
@@ -53,6 +53,7 @@ def _run_and_compare(scene, preset, ls, worlds, steps, seed=0, schedule=None):
 
     model, dt, C = _scene(scene, worlds, seed)
     solver = SolverSAP(model, max_rigid_contact=C, line_search_variant=ls, contact_preset_variant=preset,
+                       cost_rel_tol=cost_rel_tol,
                        capture_contact_jacobian_snapshots=True,
                        collect_iteration_stats=True)
     if schedule is not None:
@@ -61,7 +62,8 @@ def _run_and_compare(scene, preset, ls, worlds, steps, seed=0, schedule=None):
     contacts = pipe.contacts()
     s0, s1 = model.state(), model.state()
     ctl = model.control()
-    opts = so.StepOptions(preset=preset, line_search=ls, max_rigid_contact=C)
+    solve = None if cost_rel_tol is None else so.SolveOptions.for_variant(ls, cost_rel_tol=cost_rel_tol)
+    opts = so.StepOptions(preset=preset, line_search=ls, max_rigid_contact=C, solve=solve)
     v_guess = None
     worst = dict(v=0.0, vstar=0.0, q=0.0, qd=0.0, bq=0.0)
     for k in range(steps):
@@ -88,8 +90,8 @@ def _run_and_compare(scene, preset, ls, worlds, steps, seed=0, schedule=None):
         s0, s1 = s1, s0
     print(scene, preset, ls, {k: f"{v:.2e}" for k, v in worst.items()})
     assert worst["vstar"] <= VEL_TOL, worst
-    assert worst["v"] <= VEL_TOL, worst
-    assert worst["qd"] <= VEL_TOL, worst
+    assert worst["v"] <= vel_tol, worst
+    assert worst["qd"] <= vel_tol, worst
     assert worst["q"] <= VEL_TOL, worst
     assert worst["bq"] <= VEL_TOL, worst
 
@@ -98,6 +100,15 @@ def _run_and_compare(scene, preset, ls, worlds, steps, seed=0, schedule=None):
                                        ("drake", "exact_root"), ("drake", "armijo_decay")])
 def test_humanoid_step_parity(gpu_device, preset, ls):
     _run_and_compare("humanoid", preset, ls, worlds=6, steps=5)
+
+
+@pytest.mark.parametrize("preset", ["approx32", "drake"])
+def test_humanoid_step_parity_monotone_decay(gpu_device, preset):
+    """monotone_decay at a tight cost tolerance meets the 1e-5 bar; at the reference's default stop
+    (relative cost change below 5e-3, contact_solve.py:6179-6181) the answer is only defined to that
+    looser stop, and two valid paths can end 2e-5 apart: checked against 1e-4 there."""
+    _run_and_compare("humanoid", preset, "monotone_decay", worlds=6, steps=5, cost_rel_tol=1.0e-15)
+    _run_and_compare("humanoid", preset, "monotone_decay", worlds=6, steps=5, vel_tol=1.0e-4)
 
 
 @pytest.mark.parametrize("kernel", ["wave64", "fused"])
