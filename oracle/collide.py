@@ -93,3 +93,27 @@ def collide(model, body_q, max_contacts_per_world: int):
     out["unsupported"] = int(unsupported)
     out["count"] = int(out["count"][0])
     return out
+
+
+def convex_pair(type_a, scale_a, xf_a, type_b, scale_b, xf_b, gap_sum=0.0, margin_a=0.0, margin_b=0.0):
+    """Contacts of one convex pair (box-box / capsule-box), float32.
+
+    ``xf`` = (px, py, pz, qx, qy, qz, qw).  Returns a dict with ``pos`` (n x 3
+    contact centres), ``dist`` (n signed distances), ``normal`` (world, A to B)
+    and ``radii`` (effective radii), as the per-contact writer receives them.
+    """
+    lib = _load()
+    f32 = lambda x: np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    sa, xa, sb, xb = f32(scale_a), f32(xf_a), f32(scale_b), f32(xf_b)
+    pos = np.zeros((5, 3), np.float32)
+    dist = np.zeros(5, np.float32)
+    normal = np.zeros(3, np.float32)
+    radii = np.zeros(2, np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    n = lib.sapx_oracle_convex_pair(
+        ctypes.c_int(type_a), _ptr(sa, ctypes.c_float), _ptr(xa, ctypes.c_float), ctypes.c_int(type_b),
+        _ptr(sb, ctypes.c_float), _ptr(xb, ctypes.c_float), ctypes.c_float(gap_sum), ctypes.c_float(margin_a),
+        ctypes.c_float(margin_b), _ptr(pos, ctypes.c_float), _ptr(dist, ctypes.c_float),
+        _ptr(normal, ctypes.c_float), _ptr(radii, ctypes.c_float))
+    del fp
+    return {"count": int(n), "pos": pos[:n].copy(), "dist": dist[:n].copy(), "normal": normal, "radii": radii}

@@ -259,6 +259,8 @@ static void sphere_box(v3 sp, float r, v3 bpos, q4 bq, v3 size, float* dist, v3*
     *pos = add(bpos, rmul(c, lp));
 }
 
+#include "convex.h"
+
 /* ------------------------------------------------------------ gap check */
 static void contact_points(v3 center, v3 n_in, float d, float ra, float rb, v3* xa, v3* xb, v3* n) {
     *n = normalize(n_in);
@@ -272,6 +274,14 @@ static int passes_gap(v3 center, v3 n_in, float d, float ra, float rb, float ma,
     float dd = dot(sub(xb, xa), n) - total;
     return dd <= gap_sum;
 }
+/* the writer's own rejection test: d > gap (a NaN distance is not rejected) */
+static int passes_gap_not(v3 center, v3 n_in, float d, float ra, float rb, float ma, float mb, float gap_sum) {
+    v3 xa, xb, n;
+    contact_points(center, n_in, d, ra, rb, &xa, &xb, &n);
+    float total = ra + rb + ma + mb;
+    float dd = dot(sub(xb, xa), n) - total;
+    return dd > gap_sum;
+}
 
 /*
  * Pair-level narrowphase.  Returns the number of candidate contacts (0..4)
@@ -279,8 +289,9 @@ static int passes_gap(v3 center, v3 n_in, float d, float ra, float rb, float ma,
  * *unsupported = 1.
  */
 static int narrow_pair(int sa, int sb, const int* shape_type, const float* shape_scale, const tf* X,
-                       float dist[4], v3 pos[4], v3* normal, float* ra_out, float* rb_out,
-                       int* shape_a_out, int* shape_b_out, float gap_sum, int* unsupported) {
+                       const float* shape_margin, float dist[5], v3 pos[5], v3* normal, float* ra_out,
+                       float* rb_out, int* shape_a_out, int* shape_b_out, float gap_sum, int* unsupported,
+                       int* convex) {
     int ta = shape_type[sa], tb = shape_type[sb];
     if (ta > tb) { int t = sa; sa = sb; sb = t; t = ta; ta = tb; tb = t; }
     *shape_a_out = sa; *shape_b_out = sb;
@@ -290,9 +301,10 @@ static int narrow_pair(int sa, int sb, const int* shape_type, const float* shape
     float ra = 0.0f, rb = 0.0f;
     if (ta == GEO_SPHERE || ta == GEO_CAPSULE) ra = scale_a.x;
     if (tb == GEO_SPHERE || tb == GEO_CAPSULE) rb = scale_b.x;
-    for (int i = 0; i < 4; ++i) { dist[i] = MAXVAL; pos[i] = mk(0.0f, 0.0f, 0.0f); }
+    for (int i = 0; i < 5; ++i) { dist[i] = MAXVAL; pos[i] = mk(0.0f, 0.0f, 0.0f); }
     *normal = mk(0.0f, 0.0f, 0.0f);
     *unsupported = 0;
+    *convex = 0;
     v3 ez = mk(0.0f, 0.0f, 1.0f);
     if (ta == GEO_PLANE && tb == GEO_SPHERE) {
         v3 n = qrot(Xa.q, ez);
@@ -317,6 +329,13 @@ static int narrow_pair(int sa, int sb, const int* shape_type, const float* shape
         capsule_capsule(Xa.p, a1, scale_a.x, scale_a.y, Xb.p, a2, scale_b.x, scale_b.y, dist, pos, normal);
     } else if (ta == GEO_SPHERE && tb == GEO_BOX) {
         sphere_box(Xa.p, scale_a.x, Xb.p, Xb.q, scale_b, &dist[0], &pos[0], normal);
+    } else if ((ta == GEO_CAPSULE || ta == GEO_BOX) && tb == GEO_BOX) {
+        /* every contact of a convex pair is kept, whatever its distance (the
+           writer's own gap check decides) */
+        *convex = 1;
+        int n = cvx_pair(ta, scale_a, Xa.p, Xa.q, tb, scale_b, Xb.p, Xb.q, gap_sum, shape_margin[sa],
+                         shape_margin[sb], ra_out, rb_out, pos, dist, normal);
+        return n;
     } else {
         *unsupported = 1;
         return 0;
@@ -324,7 +343,7 @@ static int narrow_pair(int sa, int sb, const int* shape_type, const float* shape
     *ra_out = ra; *rb_out = rb;
     int n = 0;
     for (int i = 0; i < 4; ++i) n += dist[i] < MAXVAL;
-    return n;
+    return n ? 4 : 0;
 }
 
 /*
@@ -364,30 +383,37 @@ int sapx_oracle_collide(int world_count, int shape_count, const float* body_q, c
             const float* B = scratch_aabb + 6 * s2;
             if (!aabb_overlap(mk(A[0], A[1], A[2]), mk(A[3], A[4], A[5]), mk(B[0], B[1], B[2]), mk(B[3], B[4], B[5])))
                 continue;
-            float dist[4];
-            v3 pos[4], normal;
+            float dist[5];
+            v3 pos[5], normal;
             float ra, rb;
-            int sa, sb, unsup;
+            int sa, sb, unsup, convex;
             float gap_sum = shape_gap[s1] + shape_gap[s2];
-            int nc = narrow_pair(s1, s2, shape_type, shape_scale, scratch_tf, dist, pos, &normal, &ra, &rb, &sa, &sb,
-                                 gap_sum, &unsup);
+            int nc = narrow_pair(s1, s2, shape_type, shape_scale, scratch_tf, shape_margin, dist, pos, &normal, &ra,
+                                 &rb, &sa, &sb, gap_sum, &unsup, &convex);
             unsupported_total += unsup;
             if (nc == 0) continue;
             float ma = shape_margin[sa], mb = shape_margin[sb];
-            int valid[4], nv = 0;
-            for (int i = 0; i < 4; ++i) {
-                valid[i] = dist[i] < MAXVAL && passes_gap(pos[i], normal, dist[i], ra, rb, ma, mb, gap_sum);
+            int valid[5], nv = 0;
+            for (int i = 0; i < nc; ++i) {
+                if (convex)  /* the per-contact writer drops d > gap (narrow_phase.py:580-747, pipeline.py:80-88) */
+                    valid[i] = !passes_gap_not(pos[i], normal, dist[i], ra, rb, ma, mb, gap_sum);
+                else
+                    valid[i] = dist[i] < MAXVAL && passes_gap(pos[i], normal, dist[i], ra, rb, ma, mb, gap_sum);
                 nv += valid[i];
             }
             if (nv == 0) continue;
-            /* batch append with the reference's overflow rule: the claim always
-               advances, a batch is written only if it fits entirely */
+            /* primitive pairs: batch append with the reference's overflow rule
+               (the claim always advances, a batch is written only if it fits
+               entirely); convex pairs: every contact claims a slot and is
+               written if that slot is inside the capacity */
+            int first = claimed;
             int fits = claimed + nv <= max_contacts_per_world;
             claimed += nv;
-            if (!fits) continue;
+            if (!fits && !convex) continue;
             int ba = shape_body[sa], bb = shape_body[sb];
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0, k = 0; i < nc; ++i) {
                 if (!valid[i]) continue;
+                if (first + k++ >= max_contacts_per_world) break;
                 int idx = total + wcount;
                 if (idx < out_capacity) {
                     v3 xa, xb, n;
@@ -413,4 +439,28 @@ int sapx_oracle_collide(int world_count, int shape_count, const float* body_q, c
     *out_count = total;
     (void)tf_inv_vector;
     return unsupported_total;
+}
+
+/*
+ * One convex pair in isolation (box-box or capsule-box, type_a <= type_b):
+ * contact centres (5 x 3), signed distances (5), the world normal (3) and the
+ * effective radii (2), as the per-contact writer receives them.  Returns the
+ * number of contacts.
+ */
+int sapx_oracle_convex_pair(int type_a, const float* scale_a, const float* xf_a, int type_b, const float* scale_b,
+                            const float* xf_b, float gap_sum, float margin_a, float margin_b, float* out_pos,
+                            float* out_dist, float* out_normal, float* out_radii) {
+    tf a = load_tf(xf_a), b = load_tf(xf_b);
+    v3 pos[5], n;
+    float dist[5], ra, rb;
+    int c = cvx_pair(type_a, mk(scale_a[0], scale_a[1], scale_a[2]), a.p, a.q, type_b,
+                     mk(scale_b[0], scale_b[1], scale_b[2]), b.p, b.q, gap_sum, margin_a, margin_b, &ra, &rb, pos,
+                     dist, &n);
+    for (int i = 0; i < c; ++i) {
+        out_pos[3 * i] = pos[i].x; out_pos[3 * i + 1] = pos[i].y; out_pos[3 * i + 2] = pos[i].z;
+        out_dist[i] = dist[i];
+    }
+    out_normal[0] = n.x; out_normal[1] = n.y; out_normal[2] = n.z;
+    out_radii[0] = ra; out_radii[1] = rb;
+    return c;
 }

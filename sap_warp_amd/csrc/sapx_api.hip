@@ -409,6 +409,13 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     dm.num_levels = nlev;
     dm.total_shapes = Stot;
     dm.global_shape_base = W * S;
+    dm.has_convex = 0;
+    for (size_t k = 0; k + 1 < per_world[0].size(); k += 2) {
+        const int ea = per_world[0][k], eb = per_world[0][k + 1];
+        int ta = d->shape_type[ea >= 0 ? ea : -ea - 1], tb = d->shape_type[eb >= 0 ? eb : -eb - 1];
+        if (ta > tb) std::swap(ta, tb);
+        if ((ta == kGeoCapsule || ta == kGeoBox) && tb == kGeoBox) dm.has_convex = 1;
+    }
     int32_t rc = SAPX_OK;
     auto& o = m->owned;
 #define UP(dst, src, cnt)                                   \

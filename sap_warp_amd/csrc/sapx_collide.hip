@@ -18,6 +18,8 @@
 
 #pragma clang fp contract(off)
 
+#include "sapx_convex.h"
+
 namespace sapx {
 
 constexpr float kMaxVal = 1e10f;
@@ -183,9 +185,12 @@ __device__ __forceinline__ void contact_points(V3 center, V3 n_in, float d, floa
     *xb = center + (*n) * (0.5f * d + rb);
 }
 
-template <int NT>
+// CVX: the model has capsule-box or box-box candidate pairs (MPR / GJK path, up to five
+// contacts per pair, a 16-slot polygon buffer per lane in LDS)
+template <int NT, bool CVX>
 __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
+    constexpr int NC = CVX ? 5 : 4;
     const DevModel& M = P.m;
     const int w = blockIdx.x;
     const int tid = threadIdx.x;
@@ -194,6 +199,7 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
     float* X = reinterpret_cast<float*>(smem_raw);  // ns x 8 world transforms
     float* AB = X + ns * 8;                          // ns x 6 aabbs
     int* scan = reinterpret_cast<int*>(AB + ns * 6); // NT/64 wave totals + 2
+    float2* poly = reinterpret_cast<float2*>(smem_raw + ((size_t)ns * 14 * 4 + (NT / 64 + 2) * 4 + 15) / 16 * 16);
     for (int s = tid; s < ns; s += NT) {
         const int g = (s < M.S) ? w * M.S + s : M.global_shape_base + (s - M.S);
         TF xs = load_tf(M.shape_transform + (size_t)g * 7);
@@ -215,12 +221,15 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
     for (int p0 = 0; p0 < M.P; p0 += NT) {
         const int p = p0 + tid;
         int nv = 0;
-        float dist[4] = {kMaxVal, kMaxVal, kMaxVal, kMaxVal};
-        V3 pos[4] = {v3(0, 0, 0), v3(0, 0, 0), v3(0, 0, 0), v3(0, 0, 0)};
+        float dist[NC];
+        V3 pos[NC];
+        bool valid[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) { dist[i] = kMaxVal; pos[i] = v3(0, 0, 0); valid[i] = false; }
         V3 normal = v3(0, 0, 0);
         float ra = 0.0f, rb = 0.0f;
         int la = -1, lb = -1, ga = -1, gb = -1;
-        bool valid[4] = {false, false, false, false};
+        int ncvx = -1;  // contact count of a convex pair; -1 for primitive pairs
         if (p < M.P) {
             int e0 = M.t_pairs[p * 2], e1 = M.t_pairs[p * 2 + 1];
             int l1 = e0 >= 0 ? e0 : M.S + (-e0 - 1 - M.global_shape_base);
@@ -274,17 +283,24 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
                     capsule_capsule(Xa.p, a1, scale_a.x, scale_a.y, Xb.p, a2, scale_b.x, scale_b.y, dist, pos, &normal);
                 } else if (ta == kGeoSphere && tb == kGeoBox) {
                     sphere_box(Xa.p, scale_a.x, Xb.p, Xb.q, scale_b, &dist[0], &pos[0], &normal);
+                } else if (CVX && (ta == kGeoCapsule || ta == kGeoBox) && tb == kGeoBox) {
+                    const cvx::Poly buf{poly + tid, NT};
+                    ncvx = cvx::pair(ta, scale_a, Xa.p, Xa.q, tb, scale_b, Xb.p, Xb.q, gap_sum, M.shape_margin[ga],
+                                     M.shape_margin[gb], buf, &ra, &rb, pos, dist, &normal);
                 } else {
                     atomicAdd(P.unsupported, 1);
                 }
                 const float ma = M.shape_margin[ga], mbm = M.shape_margin[gb];
-                for (int i = 0; i < 4; ++i) {
-                    if (dist[i] < kMaxVal) {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    // primitive contacts pass the gap check (d <= gap); a convex pair's contacts go
+                    // through the per-contact writer, which drops d > gap
+                    if (ncvx >= 0 ? i < ncvx : dist[i] < kMaxVal) {
                         V3 xa, xb, nn;
                         contact_points(pos[i], normal, dist[i], ra, rb, &xa, &xb, &nn);
                         float total = ((ra + rb) + ma) + mbm;
                         float dd = dot(xb - xa, nn) - total;
-                        valid[i] = dd <= gap_sum;
+                        valid[i] = ncvx >= 0 ? !(dd > gap_sum) : dd <= gap_sum;
                         nv += valid[i] ? 1 : 0;
                     }
                 }
@@ -305,13 +321,16 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
             chunk_total += scan[wv];
         }
         const int prefix = claim_base + wave_off + incl - nv;
-        if (nv > 0 && prefix + nv <= P.C) {
+        // a primitive pair's batch is written only if it fits entirely; a convex pair's
+        // contacts each claim a slot and are written while the slot is inside the capacity
+        if (nv > 0 && (prefix + nv <= P.C || (ncvx >= 0 && prefix < P.C))) {
             int idx = prefix;
             const size_t base = (size_t)w * P.C;
             const int bA = M.shape_body[ga], bB = M.shape_body[gb];
             const float ma = M.shape_margin[ga], mbm = M.shape_margin[gb];
-            for (int i = 0; i < 4; ++i) {
+            for (int i = 0; i < NC; ++i) {
                 if (!valid[i]) continue;
+                if (idx >= P.C) break;
                 V3 xa, xb, nn;
                 contact_points(pos[i], normal, dist[i], ra, rb, &xa, &xb, &nn);
                 V3 pa = xa, pb = xb;
@@ -333,7 +352,7 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
                 P.st_m1[o] = rb + mbm;
                 ++idx;
             }
-            atomicMax(&scan[NT / 64], prefix + nv);
+            atomicMax(&scan[NT / 64], idx);
         }
         claim_base += chunk_total;
         __syncthreads();
@@ -398,8 +417,13 @@ hipError_t launch_collide(const CollideParams& P, const CompactParams& Q, hipStr
     constexpr int NT = 128;
     const int nglob = P.m.total_shapes - P.m.global_shape_base;
     const int ns = P.m.S + nglob;
-    const size_t smem = (size_t)ns * 14 * sizeof(float) + (NT / 64 + 2) * sizeof(int) + 16;
-    hipLaunchKernelGGL(sapx_collide_kernel<NT>, dim3(P.m.W), dim3(NT), smem, stream, P);
+    const size_t base = ((size_t)ns * 14 * sizeof(float) + (NT / 64 + 2) * sizeof(int) + 15) / 16 * 16;
+    if (P.m.has_convex) {
+        const size_t smem = base + (size_t)NT * cvx::kBuf * sizeof(float2);
+        hipLaunchKernelGGL((sapx_collide_kernel<NT, true>), dim3(P.m.W), dim3(NT), smem, stream, P);
+    } else {
+        hipLaunchKernelGGL((sapx_collide_kernel<NT, false>), dim3(P.m.W), dim3(NT), base + 16, stream, P);
+    }
     hipLaunchKernelGGL(sapx_world_scan_kernel<1024>, dim3(1), dim3(1024), 0, stream, Q);
     const size_t total = (size_t)Q.W * Q.C;
     const int blk = 256;

@@ -48,6 +48,7 @@ struct DevModel {
     int fk_rounds;  // pointer-jumping rounds over the root paths: ceil(log2(depth in bodies))
     int total_shapes;
     int global_shape_base;  // first global (world -1) shape index = W*S
+    int has_convex;         // some candidate pair is capsule-box or box-box (MPR / GJK path)
 
     // the per-world topology template packed into one int array (copied into LDS by
     // the step kernel); topo_off[k] = first int of field k (see enum TopoField)

@@ -837,6 +837,87 @@ def build_humanoid_scene(
     return _with_consistent_body_q(model)
 
 
+def build_box_pile_scene(
+    num_worlds: int = 64,
+    seed: int = 5,
+    *,
+    spacing: float = 0.0,
+    contact_gap: float = 0.01,
+    device: str = "cpu",
+) -> SapModel:
+    """Free boxes and capsules piled on the ground: the convex (MPR / GJK) contact path.
+
+    Six free bodies per world, each on its own free joint and articulation:
+    a slab box (half 0.25 x 0.2 x 0.1) on the ground; a box (half 0.1 x 0.1 x
+    0.08) on the slab, yawed and slightly tilted; a capsule (r 0.04, half
+    0.12) lying on the slab; a capsule (r 0.03, half 0.06) standing on the
+    box; a small tilted box (half 0.06 x 0.05 x 0.04) above the box; a cube
+    (half 0.1) beside the slab, facing its side.  Every resting offset is a
+    penetration drawn from U(-0.005, 0.01) m (negative: a gap), positions
+    jitter by U(-0.02, 0.02) m and yaws are uniform, so the batch holds
+    face-face, edge-face, separated-within-gap and overhanging manifolds of
+    box-box and capsule-box pairs next to the plane-box / plane-capsule
+    primitives.  Density 500 kg/m^3, mu 0.8, per-shape stiffness 2e5 N/m and
+    dissipation 5 ms; bodies start at rest.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = SceneBuilder(rigid_gap=contact_gap)
+    cfg = ShapeCfg(density=500.0, ke=2.0e5, tau=0.005, mu=0.8, gap=contact_gap, collision_group=1)
+    b.world_count = num_worlds
+
+    def pen():
+        return rng.uniform(-0.005, 0.01)
+
+    def jitter():
+        return np.array([rng.uniform(-0.02, 0.02), rng.uniform(-0.02, 0.02), 0.0])
+
+    def yawq(tilt_axis=None, tilt=0.0):
+        q = quat_from_axis_angle((0, 0, 1), rng.uniform(-math.pi, math.pi))
+        if tilt_axis is not None:
+            q = quat_mul(q, quat_from_axis_angle(tilt_axis, tilt))
+        return q
+
+    for w in range(num_worlds):
+        b.current_world = w
+        off = _grid_offset(w, num_worlds, spacing)
+        bodies = []
+        # slab on the ground
+        slab_top = 0.2 - pen()
+        bodies.append((tf(off + np.array([0.0, 0.0, slab_top - 0.1]) + jitter(),
+                          quat_from_axis_angle((0, 0, 1), rng.uniform(-0.3, 0.3))),
+                       [(GEO_BOX, tf(), (0.25, 0.2, 0.1))]))
+        # box on the slab (yawed, tilted by up to 4 degrees)
+        box_top = slab_top + 0.16 - pen()
+        bodies.append((tf(off + np.array([0.05, 0.0, box_top - 0.08]) + jitter(),
+                          yawq((1, 0, 0), rng.uniform(-0.07, 0.07))),
+                       [(GEO_BOX, tf(), (0.1, 0.1, 0.08))]))
+        # capsule lying on the slab
+        bodies.append((tf(off + np.array([-0.12, 0.0, slab_top + 0.04 - pen()]) + jitter(), yawq()),
+                       [(GEO_CAPSULE, tf((0.0, 0.0, 0.0), quat_z_to((1.0, 0.0, 0.0))), (0.04, 0.12, 0.0))]))
+        # capsule standing on the box
+        bodies.append((tf(off + np.array([0.05, 0.0, box_top + 0.09 - pen()]) + jitter(),
+                          quat_from_axis_angle((1, 0, 0), rng.uniform(-0.05, 0.05))),
+                       [(GEO_CAPSULE, tf(), (0.03, 0.06, 0.0))]))
+        # small tilted box above the box
+        bodies.append((tf(off + np.array([0.12, 0.05, box_top + 0.045 - pen()]) + jitter(),
+                          yawq((0, 1, 0), rng.uniform(-0.6, 0.6))),
+                       [(GEO_BOX, tf(), (0.06, 0.05, 0.04))]))
+        # cube beside the slab, against its +x side face
+        bodies.append((tf(off + np.array([0.35 - pen(), 0.0, 0.1 - pen()]) + jitter(),
+                          quat_from_axis_angle((0, 0, 1), rng.uniform(-0.1, 0.1))),
+                       [(GEO_BOX, tf(), (0.1, 0.1, 0.1))]))
+        for pose, shapes in bodies:
+            body = b.add_body(pose)
+            for st, x, scale in shapes:
+                b.add_shape(body, st, x, scale, cfg)
+            j = b.add_free_joint(body)
+            b.add_articulation([j])
+            _set_free_joint_state(b, j, pose, np.zeros(6))
+    b.current_world = -1
+    b.add_ground_plane(ShapeCfg(density=0.0, ke=2.0e5, tau=0.005, mu=0.8, gap=contact_gap, collision_group=-1))
+    return _with_consistent_body_q(b.finalize(device))
+
+
 def _with_consistent_body_q(model: SapModel) -> SapModel:
     """Recompute body_q/body_qd from joint_q/joint_qd with the public FK."""
     from .kinematics_host import eval_fk_public
@@ -845,4 +926,5 @@ def _with_consistent_body_q(model: SapModel) -> SapModel:
     return replace(model, body_q=body_q.astype(np.float32), body_qd=body_qd.astype(np.float32))
 
 
-__all__ = ["SceneBuilder", "ShapeCfg", "DofCfg", "build_ant_scene", "build_humanoid_scene"]
+__all__ = ["SceneBuilder", "ShapeCfg", "DofCfg", "build_ant_scene", "build_box_pile_scene",
+           "build_humanoid_scene"]

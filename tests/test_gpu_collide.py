@@ -8,11 +8,27 @@ pytestmark = pytest.mark.gpu
 
 
 def _scene(name, worlds, seed):
-    from sap_warp_amd.scenes import build_ant_scene, build_humanoid_scene
+    from sap_warp_amd.scenes import build_ant_scene, build_box_pile_scene, build_humanoid_scene
 
     if name == "ant":
         return build_ant_scene(worlds, seed=seed, device="cuda:0")
+    if name == "boxpile":
+        return build_box_pile_scene(worlds, seed=seed, device="cuda:0")
     return build_humanoid_scene(worlds, seed=seed, device="cuda:0")
+
+
+def _jittered_body_q(model, seed):
+    """Every body moved by U(-5, 5) mm and turned by up to 3 degrees (pile contacts stay in touch)."""
+    from sap_warp_amd.scenes import quat_from_axis_angle, quat_mul
+
+    rng = np.random.default_rng(seed)
+    bq = np.asarray(model.body_q, dtype=np.float64).copy()
+    for b in range(bq.shape[0]):
+        bq[b, :3] += rng.uniform(-0.005, 0.005, 3)
+        dq = quat_from_axis_angle(rng.normal(size=3), rng.uniform(0.0, 0.05))
+        q = quat_mul(dq, bq[b, 3:])
+        bq[b, 3:] = q / np.linalg.norm(q)
+    return bq.astype(np.float32)
 
 
 def _perturbed_body_q(model, seed):
@@ -53,7 +69,7 @@ def _check_equal(gpu, ref, C, model):
         np.testing.assert_array_equal(g.view(np.uint32), r.view(np.uint32), err_msg=name)
 
 
-@pytest.mark.parametrize("scene,worlds,C", [("ant", 16, 16), ("humanoid", 16, 64)])
+@pytest.mark.parametrize("scene,worlds,C", [("ant", 16, 16), ("humanoid", 16, 64), ("boxpile", 32, 64)])
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_collide_bit_exact(gpu_device, scene, worlds, C, seed):
     import torch
@@ -65,11 +81,38 @@ def test_collide_bit_exact(gpu_device, scene, worlds, C, seed):
     contacts = pipe.contacts()
     state = model.state()
     for variant in range(2):
-        bq = model.body_q if variant == 0 else _perturbed_body_q(model, 100 + seed)
+        if variant == 0:
+            bq = model.body_q
+        elif scene == "boxpile":
+            bq = _jittered_body_q(model, 100 + seed)
+        else:
+            bq = _perturbed_body_q(model, 100 + seed)
+        print(f"collide {scene} seed {seed} variant {variant}", flush=True)
         state.body_q.copy_(torch.as_tensor(bq, device=gpu_device))
         pipe.collide(state, contacts)
         ref = oracle_contacts_dict(model, bq, C)
+        assert ref["unsupported"] == 0
         _check_equal(contacts, ref, C, model)
+
+
+def test_collide_convex_pairs_present(gpu_device):
+    """The pile's contact list holds box-box and capsule-box contacts (the MPR / GJK path)."""
+    import torch
+
+    from sap_warp_amd.collision import SapCollisionPipeline
+
+    model = _scene("boxpile", 8, 4)
+    pipe = SapCollisionPipeline(model, max_contacts_per_world=64)
+    contacts = pipe.contacts()
+    state = model.state()
+    pipe.collide(state, contacts)
+    torch.cuda.synchronize()
+    n = int(contacts.rigid_contact_count.cpu()[0])
+    st = np.asarray(model.shape_type)
+    t0 = st[contacts.rigid_contact_shape0.cpu().numpy()[:n]]
+    t1 = st[contacts.rigid_contact_shape1.cpu().numpy()[:n]]
+    kinds = set(zip(t0.tolist(), t1.tolist()))
+    assert (7, 7) in kinds and (4, 7) in kinds, kinds
 
 
 def test_collide_overflow_rule(gpu_device):
@@ -81,6 +124,26 @@ def test_collide_overflow_rule(gpu_device):
     model = _scene("humanoid", 8, 3)
     bq = _perturbed_body_q(model, 7)
     C = 5
+    pipe = SapCollisionPipeline(model, max_contacts_per_world=C)
+    contacts = pipe.contacts()
+    state = model.state()
+    state.body_q.copy_(torch.as_tensor(bq, device=gpu_device))
+    pipe.collide(state, contacts)
+    ref = oracle_contacts_dict(model, bq, C)
+    assert np.all(np.diff(ref["world_start"]) <= C)
+    _check_equal(contacts, ref, C, model)
+
+
+@pytest.mark.parametrize("C", [3, 7])
+def test_collide_overflow_rule_convex(gpu_device, C):
+    """Box pile at a tiny capacity: primitive batches are dropped whole, convex-pair contacts one by
+    one (each claims a slot and is written while the slot fits), identically to the oracle."""
+    import torch
+
+    from sap_warp_amd.collision import SapCollisionPipeline
+
+    model = _scene("boxpile", 16, 6)
+    bq = _jittered_body_q(model, 9)
     pipe = SapCollisionPipeline(model, max_contacts_per_world=C)
     contacts = pipe.contacts()
     state = model.state()

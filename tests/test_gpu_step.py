@@ -16,10 +16,12 @@ VEL_TOL = 1.0e-5
 
 
 def _scene(name, worlds, seed):
-    from sap_warp_amd.scenes import build_ant_scene, build_humanoid_scene
+    from sap_warp_amd.scenes import build_ant_scene, build_box_pile_scene, build_humanoid_scene
 
     if name == "ant":
         return build_ant_scene(worlds, seed=seed, device="cuda:0"), 1.0e-3, 16
+    if name == "boxpile":
+        return build_box_pile_scene(worlds, seed=seed, device="cuda:0"), 2.0e-3, 64
     return build_humanoid_scene(worlds, seed=seed, device="cuda:0"), 2.0e-3, 64
 
 
@@ -127,3 +129,9 @@ def test_humanoid_step_parity_schedules(gpu_device, small, fork, order):
     """Other work schedules of the wave kernel (32-slot variant, serial variants, identity order)."""
     _run_and_compare("humanoid", "drake", "exact_root", worlds=6, steps=3,
                      schedule={"kernel": "wave", "small_capacity": small, "fork": fork, "world_order": order})
+
+
+@pytest.mark.parametrize("preset,ls", [("approx32", "exact_root"), ("drake", "armijo_decay")])
+def test_box_pile_step_parity(gpu_device, preset, ls):
+    """Six free bodies per world (a forest of free joints) in box-box / capsule-box contact."""
+    _run_and_compare("boxpile", preset, ls, worlds=8, steps=4)
