@@ -245,9 +245,30 @@ int32_t sapx_pipeline_create(const SapxModel* model, int32_t max_contacts_per_wo
 void sapx_pipeline_destroy(SapxPipeline* pipeline);
 
 /*
+ * Collision pipeline with the all-pairs broadphase: SapCollisionPipeline(model,
+ * broad_phase="nxn") (sim/collision/pipeline.py:433-442; SapBroadPhaseAllPairs,
+ * sim/collision/broad_phase_nxn.py:122-325, world map and filters
+ * broad_phase_common.py:18-313).  broad_phase="sap" (sweep and prune,
+ * broad_phase_sap.py) yields the same candidate set and binds here too.
+ * Each world's slice is its shapes that carry the COLLIDE_SHAPES flag (bit 1
+ * of shape_flags; NULL: every shape) followed by the colliding global shapes;
+ * its pairs are enumerated once, in lower-triangular order (row r, columns
+ * c > r), canonicalised to (min, max), and kept when their worlds and
+ * collision groups may collide and the pair is not in filter_pairs
+ * (filter_count x 2, any order).  sapx_collide then tests the kept pairs'
+ * AABBs every call, in that deterministic order.  Global-global pairs are not
+ * generated.  All arrays are host memory; shape_collision_group and
+ * shape_flags have shape_count entries.
+ */
+int32_t sapx_pipeline_create_all_pairs(const SapxModel* model, int32_t max_contacts_per_world,
+                                       const int32_t* shape_collision_group, const int32_t* shape_flags,
+                                       const int32_t* filter_pairs, int32_t filter_count, SapxPipeline** out);
+
+/*
  * Rigid contact generation: SapCollisionPipeline.collide(state, contacts)
- * (sim/collision/pipeline.py:546-709) — shape AABBs, explicit-pair AABB test
- * and primitive narrowphase, written world-grouped in candidate-pair order
+ * (sim/collision/pipeline.py:546-709) — shape AABBs, the candidate pairs' AABB
+ * test, the primitive narrowphase and the MPR / GJK path for capsule-box and
+ * box-box pairs, written world-grouped in candidate-pair order
  * (deterministic).  body_q: device, body_count x 7.
  */
 int32_t sapx_collide(SapxPipeline* pipeline, const float* body_q, const SapxContactsView* contacts, void* stream);

@@ -53,10 +53,61 @@ def _ptr(a, ct):
     return a.ctypes.data_as(ctypes.POINTER(ct))
 
 
-def collide(model, body_q, max_contacts_per_world: int):
-    """Returns a dict of compact contact arrays in world-major / pair order."""
+def _group_pair_ok(ga: int, gb: int) -> bool:
+    """Collision-group rule (sim/collision/broad_phase_common.py:130-150)."""
+    if ga == 0 or gb == 0:
+        return False
+    if ga > 0:
+        return ga == gb or gb < 0
+    return ga != gb
+
+
+def all_pairs(model):
+    """Candidate pairs of the all-pairs broadphase, world by world, in lower-triangular order.
+
+    Restates SapBroadPhaseAllPairs (sim/collision/broad_phase_nxn.py:60-207,
+    215-325) and its world map (broad_phase_common.py:183-300): a world's slice
+    is its shapes with the COLLIDE_SHAPES flag followed by the colliding global
+    shapes; pair (r, c > r) of the slice is canonicalised to (min, max) and
+    kept when not both global (the dedicated global segment is not collided
+    here), the worlds match or one is global, the collision groups may
+    collide (broad_phase_common.py:153-180) and the pair is not excluded
+    (pipeline.py:533-544).  The AABB test is left to the narrowphase driver.
+    Returns (pairs, pair_world) like ``world_major_pairs``.
+    """
+    sw = np.asarray(model.shape_world, dtype=np.int64).reshape(-1)
+    grp = np.asarray(model.shape_collision_group, dtype=np.int64).reshape(-1)
+    flags = np.asarray(model.shape_flags, dtype=np.int64).reshape(-1)
+    fp = model.shape_collision_filter_pairs
+    excluded = set() if fp is None else {(min(a, b), max(a, b)) for a, b in np.asarray(fp).reshape(-1, 2).tolist()}
+    colliding = (flags & (1 << 1)) != 0
+    shared = [s for s in range(sw.shape[0]) if sw[s] == -1 and colliding[s]]
+    pairs, pw = [], []
+    for w in range(int(model.world_count)):
+        sl = [s for s in np.nonzero((sw == w) & colliding)[0].tolist()] + shared
+        for r in range(len(sl)):
+            for c in range(r + 1, len(sl)):
+                a, b = min(sl[r], sl[c]), max(sl[r], sl[c])
+                if sw[a] == -1 and sw[b] == -1:
+                    continue
+                if sw[a] != -1 and sw[b] != -1 and sw[a] != sw[b]:
+                    continue
+                if not _group_pair_ok(int(grp[a]), int(grp[b])) or (a, b) in excluded:
+                    continue
+                pairs.append((a, b))
+                pw.append(w)
+    return (np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2)),
+            np.ascontiguousarray(np.asarray(pw, dtype=np.int32)))
+
+
+def collide(model, body_q, max_contacts_per_world: int, pairs=None):
+    """Returns a dict of compact contact arrays in world-major / pair order.
+
+    ``pairs``: ``(pairs, pair_world)`` grouped by world (``all_pairs``); default:
+    the model's explicit pairs in world-major order.
+    """
     lib = _load()
-    pairs, pw = world_major_pairs(model)
+    pairs, pw = world_major_pairs(model) if pairs is None else pairs
     W = model.world_count
     S = model.shape_count
     cap = max(1, W * max_contacts_per_world)

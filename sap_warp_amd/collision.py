@@ -1,10 +1,18 @@
 """``SapCollisionPipeline`` — drop-in for ``sim/collision/pipeline.py:312-709``.
 
-``collide(state, contacts)`` computes shape AABBs, tests the model's explicit
-candidate pairs and runs the primitive narrowphase in one HIP kernel per
-world (``sapx_collide``), writing contacts grouped by world in candidate-pair
-order together with per-world offsets (``rigid_contact_world_start``) that
-``SolverSAP.step`` consumes directly.
+``collide(state, contacts)`` computes shape AABBs, tests the candidate pairs
+and runs the narrowphase (analytic primitives; MPR / GJK for capsule-box and
+box-box) in one HIP kernel per world (``sapx_collide``), writing contacts
+grouped by world in candidate-pair order together with per-world offsets
+(``rigid_contact_world_start``) that ``SolverSAP.step`` consumes directly.
+
+Broadphase modes (``pipeline.py:281-292, 421-454``): ``"explicit"`` (the
+default) tests the model's explicit pair list; ``"nxn"`` and ``"sap"`` test
+every pair of a world's colliding shapes plus the global shapes that the
+world / collision-group rules and the filter pairs allow.  The world, group
+and filter tests do not depend on the state, so the library applies them once
+when the pipeline is created and keeps the surviving pairs (lower-triangular
+order per world) as a template; each ``collide`` tests their AABBs.
 """
 from __future__ import annotations
 
@@ -17,7 +25,7 @@ from .runtime import SapContacts, SapModel, SapState, allocate_contacts
 
 
 class SapCollisionPipeline:
-    """Rigid contact generation for a ``SapModel`` (explicit broadphase + primitive narrowphase)."""
+    """Rigid contact generation for a ``SapModel`` (explicit or all-pairs broadphase + narrowphase)."""
 
     def __init__(
         self,
@@ -30,8 +38,10 @@ class SapCollisionPipeline:
     ):
         if not isinstance(model, SapModel):
             raise TypeError("SapCollisionPipeline requires SapModel.")
-        if broad_phase not in (None, "explicit"):
-            raise NotImplementedError("only the explicit-pair broadphase is in the native pipeline")
+        mode = "explicit" if broad_phase is None else str(broad_phase).strip().lower()
+        if mode not in ("explicit", "nxn", "sap"):
+            raise ValueError(f"broad_phase must be one of ('nxn', 'sap', 'explicit'), got {broad_phase!r}")
+        self.broad_phase_mode = mode
         W = max(int(model.world_count), 1)
         if max_contacts_per_world is None:
             max_contacts_per_world = (int(rigid_contact_max) // W) if rigid_contact_max else 128
@@ -44,8 +54,22 @@ class SapCollisionPipeline:
         self._native_model = nat.NativeModel(model)
         self._lib = nat.load()
         h = ctypes.c_void_p()
-        nat.check(self._lib.sapx_pipeline_create(self._native_model.handle, self.max_contacts_per_world,
-                                                 ctypes.byref(h)), "sapx_pipeline_create")
+        if mode == "explicit":
+            nat.check(self._lib.sapx_pipeline_create(self._native_model.handle, self.max_contacts_per_world,
+                                                     ctypes.byref(h)), "sapx_pipeline_create")
+        else:
+            groups = np.ascontiguousarray(np.asarray(model.shape_collision_group, dtype=np.int32).reshape(-1))
+            flags = np.ascontiguousarray(np.asarray(model.shape_flags, dtype=np.int32).reshape(-1))
+            fp = model.shape_collision_filter_pairs
+            fp = np.zeros((0, 2), np.int32) if fp is None else np.asarray(fp, dtype=np.int32).reshape(-1, 2)
+            fp = np.ascontiguousarray(fp)
+            if groups.shape[0] != model.shape_count or flags.shape[0] != model.shape_count:
+                raise ValueError("shape_collision_group and shape_flags need one entry per shape")
+            ip = ctypes.POINTER(ctypes.c_int32)
+            nat.check(self._lib.sapx_pipeline_create_all_pairs(
+                self._native_model.handle, self.max_contacts_per_world, groups.ctypes.data_as(ip),
+                flags.ctypes.data_as(ip), fp.ctypes.data_as(ip) if fp.shape[0] else None, int(fp.shape[0]),
+                ctypes.byref(h)), "sapx_pipeline_create_all_pairs")
         self._handle = h
 
     def close(self) -> None:

@@ -74,6 +74,7 @@ size_t round_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 struct SapxModel {
     DevModel dm;
     std::vector<void*> owned;
+    std::vector<int> h_shape_world, h_shape_type;  // host copies for pipeline set-up
     int32_t world_count = 0, joint_count = 0, dof_count = 0, coord_count = 0, body_count = 0, shape_count = 0;
     ~SapxModel() {
         for (void* p : owned) (void)hipFree(p);
@@ -149,8 +150,12 @@ struct SapxPipeline {
     float* st_m1 = nullptr;
     int* wcount = nullptr;
     int* unsupported = nullptr;
+    // all-pairs broadphase: the pipeline's own candidate template (P < 0: the model's explicit pairs)
+    int* t_pairs = nullptr;
+    int P = -1;
+    int has_convex = 0;
     ~SapxPipeline() {
-        void* ptrs[] = {st_shape0, st_shape1, st_p0, st_p1, st_n, st_m0, st_m1, wcount, unsupported};
+        void* ptrs[] = {st_shape0, st_shape1, st_p0, st_p1, st_n, st_m0, st_m1, wcount, unsupported, t_pairs};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
     }
@@ -404,6 +409,8 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     }
     SapxModel* m = new SapxModel();
     DevModel& dm = m->dm;
+    m->h_shape_world.assign(d->shape_world, d->shape_world + Stot);
+    m->h_shape_type.assign(d->shape_type, d->shape_type + Stot);
     dm.W = W; dm.J = J; dm.N = N; dm.NQ = NQ; dm.NB = NB; dm.S = S;
     dm.P = (int)per_world[0].size() / 2;
     dm.num_levels = nlev;
@@ -1000,6 +1007,83 @@ int32_t sapx_pipeline_create(const SapxModel* model, int32_t max_contacts_per_wo
 
 void sapx_pipeline_destroy(SapxPipeline* pipeline) { delete pipeline; }
 
+int32_t sapx_pipeline_create_all_pairs(const SapxModel* model, int32_t max_contacts_per_world,
+                                       const int32_t* shape_collision_group, const int32_t* shape_flags,
+                                       const int32_t* filter_pairs, int32_t filter_count, SapxPipeline** out) {
+    if (!model || !out || !shape_collision_group) return fail(SAPX_ERR_INVALID, "null argument");
+    if (filter_count < 0 || (filter_count > 0 && !filter_pairs)) return fail(SAPX_ERR_INVALID, "bad filter pairs");
+    const DevModel& dm = model->dm;
+    const int W = dm.W, S = dm.S, Stot = dm.total_shapes;
+    const std::vector<int>& sw = model->h_shape_world;
+    const std::vector<int>& st = model->h_shape_type;
+    constexpr int kCollideShapes = 1 << 1;
+    auto collides = [&](int s) { return !shape_flags || (shape_flags[s] & kCollideShapes) != 0; };
+    // excluded pairs, canonical (min, max), sorted for binary search
+    std::vector<std::pair<int, int>> excl;
+    excl.reserve(filter_count);
+    for (int k = 0; k < filter_count; ++k) {
+        const int a = filter_pairs[2 * k], b = filter_pairs[2 * k + 1];
+        excl.emplace_back(std::min(a, b), std::max(a, b));
+    }
+    std::sort(excl.begin(), excl.end());
+    auto group_ok = [](int ga, int gb) {
+        if (ga == 0 || gb == 0) return false;
+        if (ga > 0) return ga == gb || gb < 0;
+        return ga != gb;
+    };
+    std::vector<int> globals;
+    for (int s = dm.global_shape_base; s < Stot; ++s)
+        if (collides(s)) globals.push_back(s);
+    // each world's slice (its colliding shapes, then the colliding global shapes) in lower-triangular
+    // order; global-global pairs never collide in this path
+    std::vector<int> tmpl;
+    for (int w = 0; w < W; ++w) {
+        std::vector<int> slice;
+        for (int s = w * S; s < (w + 1) * S; ++s)
+            if (collides(s)) slice.push_back(s);
+        slice.insert(slice.end(), globals.begin(), globals.end());
+        std::vector<int> mine;
+        const int n = (int)slice.size();
+        for (int r = 0; r < n; ++r)
+            for (int c = r + 1; c < n; ++c) {
+                const int a = std::min(slice[r], slice[c]), b = std::max(slice[r], slice[c]);
+                const int wa = sw[a], wb = sw[b];
+                if (wa < 0 && wb < 0) continue;
+                if (wa >= 0 && wb >= 0 && wa != wb) continue;
+                if (!group_ok(shape_collision_group[a], shape_collision_group[b])) continue;
+                if (std::binary_search(excl.begin(), excl.end(), std::make_pair(a, b))) continue;
+                mine.push_back(wa >= 0 ? a - wa * S : -a - 1);
+                mine.push_back(wb >= 0 ? b - wb * S : -b - 1);
+            }
+        if (w == 0) tmpl = mine;
+        else if (mine != tmpl)
+            return fail(SAPX_ERR_UNSUPPORTED, "all-pairs candidates must repeat identically in every world");
+    }
+    int has_convex = 0;
+    for (size_t k = 0; k + 1 < tmpl.size(); k += 2) {
+        const int ea = tmpl[k], eb = tmpl[k + 1];
+        int ta = st[ea >= 0 ? ea : -ea - 1], tb = st[eb >= 0 ? eb : -eb - 1];
+        if (ta > tb) std::swap(ta, tb);
+        if ((ta == kGeoCapsule || ta == kGeoBox) && tb == kGeoBox) has_convex = 1;
+        if (ta == kGeoPlane && tb == kGeoPlane)
+            return fail(SAPX_ERR_UNSUPPORTED, "plane-plane candidate pairs are not in the native narrowphase");
+    }
+    int32_t rc = sapx_pipeline_create(model, max_contacts_per_world, out);
+    if (rc != SAPX_OK) return rc;
+    SapxPipeline* p = *out;
+    p->P = (int)tmpl.size() / 2;
+    p->has_convex = has_convex;
+    if (!tmpl.empty()) {
+        if (hipMalloc(&p->t_pairs, tmpl.size() * sizeof(int)) != hipSuccess ||
+            hipMemcpy(p->t_pairs, tmpl.data(), tmpl.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            delete p;
+            *out = nullptr;
+            return fail(SAPX_ERR_HIP, "pair template upload failed");
+        }
+    }
+    return SAPX_OK;
+}
+
 int32_t sapx_collide(SapxPipeline* p, const float* body_q, const SapxContactsView* c, void* stream) {
     if (!p || !body_q || !c) return fail(SAPX_ERR_INVALID, "null argument");
     if (!c->count || !c->world_start || !c->shape0 || !c->shape1 || !c->point0 || !c->point1 || !c->normal ||
@@ -1014,6 +1098,11 @@ int32_t sapx_collide(SapxPipeline* p, const float* body_q, const SapxContactsVie
     P.st_p0 = p->st_p0; P.st_p1 = p->st_p1; P.st_n = p->st_n; P.st_m0 = p->st_m0; P.st_m1 = p->st_m1;
     P.wcount = p->wcount;
     P.unsupported = p->unsupported;
+    if (p->P >= 0) {
+        P.m.t_pairs = p->t_pairs;
+        P.m.P = p->P;
+        P.m.has_convex = p->has_convex;
+    }
     CompactParams Q;
     Q.W = dm.W; Q.C = p->C; Q.wcount = p->wcount; Q.world_start = c->world_start; Q.count = c->count;
     Q.st_shape0 = p->st_shape0; Q.st_shape1 = p->st_shape1; Q.st_p0 = p->st_p0; Q.st_p1 = p->st_p1; Q.st_n = p->st_n;

@@ -128,6 +128,8 @@ class SapModel:
     shape_collision_group: np.ndarray
     shape_world: np.ndarray
     shape_pairs: np.ndarray  # (P, 2) int32 explicit candidate pairs
+    # (F, 2) int32 excluded pairs (min, max): same-body and parent-child shapes (scene.py:333-373, 409-414)
+    shape_collision_filter_pairs: np.ndarray | None = None
     up_axis: int = 2
     labels: dict = field(default_factory=dict)
 

@@ -516,6 +516,7 @@ class SceneBuilder:
             shape_collision_group=np.asarray([c.collision_group for c in self.shape_cfg], dtype=i32),
             shape_world=np.asarray(self.shape_world, dtype=i32),
             shape_pairs=self.shape_pairs(),
+            shape_collision_filter_pairs=np.asarray(sorted(self.filter_pairs), dtype=np.int32).reshape(-1, 2),
         )
         return model
 
@@ -771,6 +772,9 @@ def build_humanoid_scene(
     pairs = np.concatenate(pairs, axis=0)
     is_ground = np.any(pairs == W * S, axis=1)
     pairs = np.concatenate([pairs[is_ground], pairs[~is_ground]], axis=0).astype(np.int32)
+    filt_t = np.asarray(t.shape_collision_filter_pairs, dtype=np.int64).reshape(-1, 2)
+    filt = np.concatenate([np.where(filt_t == ground, W * S, filt_t + w * S) for w in range(W)], axis=0)
+    filt = np.unique(filt, axis=0).astype(np.int32)
 
     def cat_shapes(a_world, a_ground):
         return np.concatenate([np.tile(a_world, (W,) + (1,) * (a_world.ndim - 1)), a_ground], axis=0)
@@ -833,6 +837,7 @@ def build_humanoid_scene(
         shape_collision_group=cat_shapes(t.shape_collision_group[:S], t.shape_collision_group[S:]).astype(np.int32),
         shape_world=np.concatenate([np.repeat(np.arange(W), S), [-1]]).astype(np.int32),
         shape_pairs=pairs,
+        shape_collision_filter_pairs=filt,
     )
     return _with_consistent_body_q(model)
 

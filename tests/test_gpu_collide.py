@@ -152,3 +152,47 @@ def test_collide_overflow_rule_convex(gpu_device, C):
     ref = oracle_contacts_dict(model, bq, C)
     assert np.all(np.diff(ref["world_start"]) <= C)
     _check_equal(contacts, ref, C, model)
+
+
+@pytest.mark.parametrize("scene,worlds,C", [("humanoid", 16, 64), ("boxpile", 16, 64)])
+@pytest.mark.parametrize("mode", ["nxn", "sap"])
+def test_collide_all_pairs_broadphase(gpu_device, scene, worlds, C, mode):
+    """broad_phase="nxn" / "sap": bit-exact vs the oracle fed its all-pairs enumeration, and per
+    world the same contacts as the explicit-pair pipeline (in the all-pairs order)."""
+    import torch
+
+    from oracle.collide import all_pairs, collide
+    from sap_warp_amd.collision import SapCollisionPipeline
+
+    model = _scene(scene, worlds, 1)
+    pipe = SapCollisionPipeline(model, max_contacts_per_world=C, broad_phase=mode)
+    explicit = SapCollisionPipeline(model, max_contacts_per_world=C)
+    contacts, contacts_e = pipe.contacts(), explicit.contacts()
+    state = model.state()
+    pairs = all_pairs(model)
+    for variant in range(2):
+        if variant == 0:
+            bq = model.body_q
+        elif scene == "boxpile":
+            bq = _jittered_body_q(model, 30)
+        else:
+            bq = _perturbed_body_q(model, 30)
+        print(f"all-pairs {mode} {scene} variant {variant}", flush=True)
+        state.body_q.copy_(torch.as_tensor(bq, device=gpu_device))
+        pipe.collide(state, contacts)
+        explicit.collide(state, contacts_e)
+        ref = collide(model, np.asarray(bq, dtype=np.float32), C, pairs=pairs)
+        assert ref["unsupported"] == 0
+        _check_equal(contacts, ref, C, model)
+        torch.cuda.synchronize()
+        ws = contacts.rigid_contact_world_start.cpu().numpy()
+        we = contacts_e.rigid_contact_world_start.cpu().numpy()
+        np.testing.assert_array_equal(np.diff(ws), np.diff(we))
+        def rows(c):
+            s0, s1 = c.rigid_contact_shape0.cpu().numpy(), c.rigid_contact_shape1.cpu().numpy()
+            p0, nn = c.rigid_contact_point0.cpu().numpy(), c.rigid_contact_normal.cpu().numpy()
+            return [(int(s0[i]), int(s1[i])) + tuple(p0[i].tolist()) + tuple(nn[i].tolist()) for i in range(len(s0))]
+
+        ra, rb = rows(contacts), rows(contacts_e)
+        for w in range(model.world_count):
+            assert sorted(ra[ws[w]:ws[w + 1]]) == sorted(rb[we[w]:we[w + 1]])
