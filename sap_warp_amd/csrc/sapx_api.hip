@@ -466,6 +466,19 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
         add(TP_BODY_ART_COUNT, art_count.data(), NB);
         add(TP_CHILD_START, child_start.data(), NB + 1);
         add(TP_CHILD, child_list.data(), child_list.size());
+        {
+            // root body of each body's tree: the dynamics of a tree are written about its root's
+            // origin, so a forest (several free bodies, several fixed-base trees) keeps every
+            // body's spatial inertia about a nearby point
+            std::vector<int> root(NB);
+            for (int b = 0; b < NB; ++b) {
+                int r = b;
+                for (int guard = 0; guard <= NB && body_joint[r] >= 0 && t_jparent[body_joint[r]] >= 0; ++guard)
+                    r = t_jparent[body_joint[r]];
+                root[b] = r;
+            }
+            add(TP_BODY_ROOT, root.data(), NB);
+        }
         if (topo.size() % 2) topo.push_back(0);  // 8-byte alignment of the u64 masks
         auto add64 = [&](int field, const std::vector<unsigned long long>& v) {
             dm.topo_off[field] = (int)topo.size();

@@ -573,7 +573,13 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
 
     // ------------------------------------------------------------------ 3
     // body dynamics (free_motion.py:680-719) + inertia about O = root origin
-    const V3 O = v3(s.bq[0], s.bq[1], s.bq[2]);
+    // inertia about O = the origin of the body's tree root (each tree of a forest about its own root)
+    // p_b - O, formed from the float64 poses
+    auto tree_offset = [&](int b) {
+        const int r = T.body_root[b];
+        return v3((float)(s.bqd[b * 8] - s.bqd[r * 8]), (float)(s.bqd[b * 8 + 1] - s.bqd[r * 8 + 1]),
+                  (float)(s.bqd[b * 8 + 2] - s.bqd[r * 8 + 2]));
+    };
     for (int b = tid; b < nb; b += NT) {
         const size_t gb = (size_t)w * nb + b;
         const float m = mM[b];
@@ -621,7 +627,7 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
         F[0] = fw.x + e[0]; F[1] = fw.y + e[1]; F[2] = fw.z + e[2];
         F[3] = fv.x + e[3]; F[4] = fv.y + e[4]; F[5] = fv.z + e[5];
         // inertia about O: c_O = (p_b - O) + c
-        V3 co = (v3(bqb[0], bqb[1], bqb[2]) - O) + c;
+        V3 co = tree_offset(b) + c;
         float co2 = dot(co, co);
         float co_[3] = {co.x, co.y, co.z};
         float* I = kI + b * 10;
@@ -677,8 +683,7 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
         // tau = joint_f - S . f_s  (free_motion.py:93-147, sap_helpers.py:2090-2118)
         d32[F_TAU * npad + d] = d32[F_JF * npad + d] - sv_dot(S, load_sv(kFc + c * 6));
         // S about O
-        V3 pc = v3(s.bq[c * 8], s.bq[c * 8 + 1], s.bq[c * 8 + 2]);
-        SV SO = shift_velocity(S, O - pc);
+        SV SO = shift_velocity(S, -tree_offset(c));
         store_sv(kSO + d * 6, SO);
         // composite inertia of the joint's child subtree times SO
         const float* I = kIc + c * 10;

@@ -547,9 +547,17 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     }
 
     // ------------------------------------------------------------------ 3
-    // body dynamics about each body origin, inertia about O = root origin
-    const V3 O = v3(bq[0], bq[1], bq[2]);
+    // body dynamics about each body origin; inertia and torque about O = the origin of the
+    // body's tree root (each tree of a forest about its own root, so no body sits far from its
+    // reference point and float32 spatial algebra keeps small inertias)
+    // p_b - O, formed from the float64 poses so far-away worlds lose no lever-arm digits
+    auto tree_offset = [&](int b) {
+        const int r = T.body_root[b];
+        return v3((float)(bqd[b * 8] - bqd[r * 8]), (float)(bqd[b * 8 + 1] - bqd[r * 8 + 1]),
+                  (float)(bqd[b * 8 + 2] - bqd[r * 8 + 2]));
+    };
     for (int b = lane; b < nb; b += 64) {
+        const V3 pO = tree_offset(b);
         const float m = mM[b];
         const float* Ip = mI + b * 9;
         const float* cp = mCom + b * 4;
@@ -590,10 +598,10 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         float* F = kF + b * 6;
         // torque about O (tau_O = tau_b + (p_b - O) x f): composites become plain subtree sums
         const V3 fvt = fv + v3(e[3], e[4], e[5]);
-        const V3 two = fw + v3(e[0], e[1], e[2]) + cross(v3(bqb[0], bqb[1], bqb[2]) - O, fvt);
+        const V3 two = fw + v3(e[0], e[1], e[2]) + cross(pO, fvt);
         F[0] = two.x; F[1] = two.y; F[2] = two.z;
         F[3] = fvt.x; F[4] = fvt.y; F[5] = fvt.z;
-        const V3 co = (v3(bqb[0], bqb[1], bqb[2]) - O) + c;
+        const V3 co = pO + c;
         const float co2 = dot(co, co);
         const float co_[3] = {co.x, co.y, co.z};
         float* I = kI + b * 12;
@@ -638,8 +646,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         const int ji = T.dof_joint[lane];
         const int c = T.jchild[ji];
         const SV S = load_sv(Sg + lane * 6);
-        const V3 pc = v3(bq[c * 8], bq[c * 8 + 1], bq[c * 8 + 2]);
-        SOi = shift_velocity(S, O - pc);
+        SOi = shift_velocity(S, -tree_offset(c));
         tau = jf - sv_dot(SOi, load_sv(kFc + c * 6));  // power pairing is the same about any point
         const float* I = kIc + c * 12;
         const V3 hv = v3(I[6], I[7], I[8]);
@@ -697,6 +704,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     int cb0 = -1, cb1 = -1;
     V3 cx0 = v3(0, 0, 0), cx1 = cx0, cpc = cx0, tx = cx0, ty = cx0, nrm = cx0;
     double phi = 0.0, kcomb = 1.0, taus = 0.0, muc = 0.0;
+    D3 xs0d = d3(0.0, 0.0, 0.0), xs1d = xs0d;  // the Jacobian's points (witnesses or midpoint), float64
     if (lane < nc) {
         const int idx = c_begin + lane;
         const int sh0 = P.cshape0[idx], sh1 = P.cshape1[idx];
@@ -741,6 +749,8 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         if (kden != 0.0) { w0 = k0 / kden; w1 = k1 / kden; }
         const D3 pcd = dadd(dscale(x0d, w0), dscale(x1d, w1));
         cpc = v3((float)pcd.x, (float)pcd.y, (float)pcd.z);
+        xs0d = P.point_mode == 0 ? x0d : pcd;
+        xs1d = P.point_mode == 0 ? x1d : pcd;
         if (!isfinite(k0)) kcomb = k1;
         else if (!isfinite(k1)) kcomb = k0;
         else if (kden <= 1.0e-12) kcomb = fmax(k0, k1);
@@ -770,14 +780,18 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     wave_sync();
     vec[64 + lane] = (lane < n) ? ainv : 0.0;   // diag(A)^-1 by dof for the Delassus weights
     if (lane < nc) {
-        const bool wit = P.point_mode == 0;
-        const V3 xs1 = wit ? cx1 : cpc;
-        const V3 xs0 = wit ? cx0 : cpc;
+        // x - O of the body's tree root, from float64 points and poses
+        auto lever = [&](D3 x, int b) {
+            const int r = T.body_root[b];
+            return v3((float)(x.x - bqd[r * 8]), (float)(x.y - bqd[r * 8 + 1]), (float)(x.z - bqd[r * 8 + 2]));
+        };
         // dof d moves the contact point x of a body on its root path at S_O.v + S_O.w x (x - O)
-        // (S_O: motion subspace about O; equal to S.v + S.w x (x - p_child))
+        // (S_O: motion subspace about the origin O of the dof's tree, which is the body's tree;
+        // equal to S.v + S.w x (x - p_child))
         const unsigned long long anc1 = cb1 >= 0 ? T.body_anc_dof[cb1] : 0ull;
         const unsigned long long anc0 = cb0 >= 0 ? T.body_anc_dof[cb0] : 0ull;
-        const V3 r1 = xs1 - O, r0 = xs0 - O;
+        const V3 r1 = cb1 >= 0 ? lever(xs1d, cb1) : v3(0, 0, 0);
+        const V3 r0 = cb0 >= 0 ? lever(xs0d, cb0) : v3(0, 0, 0);
         float* J0 = Jm + (lane * 3) * js;
 #pragma unroll 2
         for (int d = 0; d < n; ++d) {

@@ -190,9 +190,13 @@ class ShapeCfg:
     collision_group: int = 1
     collision_filter_parent: bool = True
     has_shape_collision: bool = True
+    has_particle_collision: bool = True
+    is_visible: bool = True
 
     def flags(self) -> int:
-        f = SHAPE_FLAG_VISIBLE | SHAPE_FLAG_COLLIDE_PARTICLES
+        f = SHAPE_FLAG_VISIBLE if self.is_visible else 0
+        if self.has_particle_collision:
+            f |= SHAPE_FLAG_COLLIDE_PARTICLES
         if self.has_shape_collision:
             f |= SHAPE_FLAG_COLLIDE_SHAPES
         return f
@@ -248,6 +252,7 @@ class SceneBuilder:
     def __init__(self, *, rigid_gap: float = 0.1, gravity: float = -9.81):
         self.rigid_gap = float(rigid_gap)
         self.gravity = float(gravity)
+        self.default_body_armature = 0.0
         self.current_world = -1
         self.world_count = 1
         # bodies
@@ -284,12 +289,16 @@ class SceneBuilder:
         self.filter_pairs: set = set()
 
     # ------------------------------------------------------------------ bodies
-    def add_body(self, xform, *, mass=0.0, com=None, inertia=None, lock_inertia=False) -> int:
+    def add_body(self, xform, *, mass=0.0, com=None, inertia=None, lock_inertia=False, armature=None) -> int:
         bid = len(self.body_q)
         self.body_q.append(np.asarray(xform, dtype=np.float64).copy())
         self.body_mass.append(float(mass))
         self.body_com.append(np.zeros(3) if com is None else np.asarray(com, dtype=np.float64))
-        self.body_inertia.append(np.zeros((3, 3)) if inertia is None else np.asarray(inertia, dtype=np.float64))
+        body_inertia = np.zeros((3, 3)) if inertia is None else np.asarray(inertia, dtype=np.float64).reshape(3, 3)
+        arm = self.default_body_armature if armature is None else float(armature)
+        if arm != 0.0:  # scene.py:301-303
+            body_inertia = body_inertia + np.eye(3) * arm
+        self.body_inertia.append(body_inertia)
         self.body_world.append(self.current_world)
         self.body_lock_inertia.append(bool(lock_inertia))
         self.body_shapes[bid] = []
@@ -417,6 +426,93 @@ class SceneBuilder:
         for j in joints:
             self.joint_articulation[j] = art
 
+    # -------------------------------------------------------------- replicate
+    def replicate(self, world_count: int, *, spacing=(2.0, 2.0, 0.0)) -> None:
+        """Copy the built (single) world ``world_count`` times (``_SapSceneBuilder.replicate``,
+        scene.py:513-760).
+
+        World w's bodies, free-joint root coordinates, root non-free joint anchors and world-attached
+        shapes move by the w-th grid offset (``_compute_sap_world_offsets``, scene.py:5650-5685: a
+        grid over the non-zero spacing axes, centred except along z).  Everything already in the
+        builder becomes world 0's template; global shapes (the ground) are added afterwards.
+        """
+        W = max(int(world_count), 1)
+        nb, nj, ns = len(self.body_q), len(self.joint_type), len(self.shape_type)
+        nq, nd, na = len(self.joint_q), len(self.joint_qd), len(self.articulation_start)
+        base = {k: list(getattr(self, k)) for k in (
+            "body_q", "body_mass", "body_inertia", "body_com", "body_lock_inertia", "joint_type", "joint_parent",
+            "joint_child", "joint_X_p", "joint_X_c", "joint_q", "joint_qd", "joint_q_start", "joint_qd_start",
+            "joint_dof_dim", "joint_articulation", "dofs", "articulation_start", "shape_transform", "shape_body",
+            "shape_type", "shape_scale", "shape_cfg")}
+        base_filters = sorted(self.filter_pairs)
+        offsets = world_offsets(W, spacing)
+        for k in base:
+            setattr(self, k, [])
+        self.body_world, self.shape_world = [], []
+        self.body_shapes = {-1: []}
+        self.joint_parents, self.joint_children = {}, {}
+        self.filter_pairs = set()
+        for w in range(W):
+            off = offsets[w]
+            bo, jo, so, qo, do, ao = w * nb, w * nj, w * ns, w * nq, w * nd, w * na
+            for x in base["body_q"]:
+                y = np.asarray(x, np.float64).copy()
+                y[:3] += off
+                self.body_q.append(y)
+            self.body_mass.extend(base["body_mass"])
+            self.body_inertia.extend([np.array(i) for i in base["body_inertia"]])
+            self.body_com.extend([np.array(c) for c in base["body_com"]])
+            self.body_lock_inertia.extend(base["body_lock_inertia"])
+            self.body_world.extend([w] * nb)
+            for b in range(nb):
+                self.body_shapes[bo + b] = []
+            self.joint_type.extend(base["joint_type"])
+            self.joint_parent.extend(p + bo if p >= 0 else -1 for p in base["joint_parent"])
+            self.joint_child.extend(c + bo for c in base["joint_child"])
+            self.joint_X_c.extend([np.array(x) for x in base["joint_X_c"]])
+            self.joint_q_start.extend(q + qo for q in base["joint_q_start"])
+            self.joint_qd_start.extend(d + do for d in base["joint_qd_start"])
+            self.joint_dof_dim.extend(base["joint_dof_dim"])
+            self.joint_articulation.extend(a + ao if a >= 0 else -1 for a in base["joint_articulation"])
+            self.dofs.extend(replace(d) for d in base["dofs"])
+            self.articulation_start.extend(a + jo for a in base["articulation_start"])
+            q = list(base["joint_q"])
+            for j, jt in enumerate(base["joint_type"]):
+                root = base["joint_parent"][j] == -1
+                xp = np.asarray(base["joint_X_p"][j], np.float64).copy()
+                if root and jt in (SAP_JOINT_FREE, SAP_JOINT_DISTANCE):
+                    qs = base["joint_q_start"][j]
+                    for k in range(3):
+                        q[qs + k] += float(off[k])
+                elif root:
+                    xp[:3] += off
+                self.joint_X_p.append(xp)
+            self.joint_q.extend(q)
+            self.joint_qd.extend(base["joint_qd"])
+            for si in range(ns):
+                b = base["shape_body"][si]
+                rb = b + bo if b >= 0 else -1
+                x = np.asarray(base["shape_transform"][si], np.float64).copy()
+                if rb < 0:
+                    x[:3] += off
+                self.shape_transform.append(x)
+                self.shape_body.append(rb)
+                self.shape_type.append(base["shape_type"][si])
+                self.shape_scale.append(base["shape_scale"][si])
+                self.shape_cfg.append(replace(base["shape_cfg"][si]))
+                self.shape_world.append(w)
+                self.body_shapes.setdefault(rb, []).append(so + si)
+            for a, b in base_filters:
+                self._filter(a + so, b + so)
+            for j in range(nj):
+                p, c = self.joint_parent[jo + j], self.joint_child[jo + j]
+                self.joint_parents.setdefault(c, []).append(p)
+                ch = self.joint_children.setdefault(p, [])
+                if c not in ch:
+                    ch.append(c)
+        self.world_count = W
+        self.current_world = -1
+
     # --------------------------------------------------------------- finalize
     def shape_pairs(self) -> np.ndarray:
         """Explicit candidate pairs (``_find_shape_contact_pairs`` scene.py:1188-1212)."""
@@ -524,6 +620,35 @@ class SceneBuilder:
 # ----------------------------------------------------------------------------
 # scenes
 # ----------------------------------------------------------------------------
+def world_offsets(world_count: int, spacing, up_axis: int = 2) -> np.ndarray:
+    """Per-world origin offsets of a replicated scene (``_compute_sap_world_offsets``,
+    scene.py:5650-5685): worlds fill a grid over the axes with non-zero spacing (a line, a square
+    or a cube of side ceil(W^(1/dims))), then the grid is centred on the axes other than up."""
+    sp = np.asarray(spacing, dtype=np.float32)
+    W = int(world_count)
+    out = np.zeros((max(W, 0), 3), dtype=np.float32)
+    if W <= 0:
+        return out
+    nz = np.nonzero(sp)[0]
+    dims = int(nz.shape[0])
+    if dims:
+        side = int(np.ceil(W ** (1.0 / dims)))
+        for i in range(W):
+            if dims == 1:
+                out[i] = i * sp
+            elif dims == 2:
+                out[i, nz[0]] = (i % side) * sp[nz[0]]
+                out[i, nz[1]] = (i // side) * sp[nz[1]]
+            else:
+                out[i, 0] = (i % side) * sp[0]
+                out[i, 1] = ((i // side) % side) * sp[1]
+                out[i, 2] = (i // (side * side)) * sp[2]
+    lo = out.min(axis=0)
+    corr = lo + (out.max(axis=0) - lo) / 2.0
+    corr[int(up_axis)] = 0.0
+    return out - corr
+
+
 def _grid_offset(world: int, world_count: int, spacing: float) -> np.ndarray:
     if spacing == 0.0:
         return np.zeros(3)
