@@ -291,6 +291,51 @@ def timed_run(args, W, C, dt, dev, lws_trained, graph=False, schedule=None, tune
     }
 
 
+def hetero_run(worlds: int, steps: int, warmup: int, seed: int, dt: float, line_search: str):
+    """The heterogeneous batch (arm+gripper+box 15 DOF, quadruped 18, humanoid 35, pile of 8 free
+    bodies 48; kinds drawn per world by seed): one stream per kind forked and joined per step, timed
+    eagerly with HIP events on the caller's stream (the joins end there)."""
+    import numpy as np
+    import torch
+
+    from sap_warp_amd.hetero import HeteroBatch, build_hetero_models
+
+    t0 = time.time()
+    models = build_hetero_models(worlds, seed=seed, device="cuda:0")
+    build_s = time.time() - t0
+    batch = HeteroBatch(models, dt=dt, line_search_variant=line_search, collect_iteration_stats=True)
+    for _ in range(warmup):
+        batch.step()
+    torch.cuda.synchronize()
+    # per-kind Newton iterations of one eager step
+    batch.step()
+    torch.cuda.synchronize()
+    iters = {n: float(np.mean(g["solver"].last_contact_solve_result.newton_iterations_env))
+             for n, g in batch.groups.items()}
+    for g in batch.groups.values():
+        g["solver"].collect_iteration_stats = False  # no per-step host reads in the timed loop
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        batch.step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    finite = all(bool(torch.isfinite(g["states"][0].joint_q).all()) for g in batch.groups.values())
+    return {
+        "value": round(batch.world_count * 1000.0 / ms, 1),
+        "ms_per_step": round(ms, 4),
+        "worlds": batch.world_count,
+        "steps_timed": steps,
+        "kinds": {n: {"worlds": g["model"].world_count, "dof_per_world": g["model"].joint_dof_count // g["model"].world_count,
+                      "mean_newton_iterations_one_step": round(iters[n], 3)} for n, g in batch.groups.items()},
+        "finite_after": finite,
+        "build_s": round(build_s, 2),
+        "timing": "eager launches (collide + step per kind, one stream per kind), HIP events on the caller's stream",
+    }
+
+
 def copy_calibration() -> int:
     """Three copies of a 512 MiB buffer: the HBM counters of a kernel of known traffic."""
     import torch
@@ -322,6 +367,9 @@ def main() -> int:
     ap.add_argument("--train-steps", type=int, default=250)
     ap.add_argument("--train-epochs", type=int, default=12)
     ap.add_argument("--train-lr", type=float, default=1e-3)
+    ap.add_argument("--hetero", type=int, default=1, help="also time the heterogeneous batch (1) or not (0)")
+    ap.add_argument("--hetero-worlds", type=int, default=8192)
+    ap.add_argument("--hetero-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 512 MiB device copies (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -416,6 +464,9 @@ def main() -> int:
             "fallback_rate": round(rejected / max(taken + rejected, 1), 4),
             "train": train_info,
         }
+    if args.hetero:
+        out["hetero"] = hetero_run(args.hetero_worlds, args.hetero_steps, min(args.warmup, 20), args.seed, dt,
+                                   args.line_search)
     if not args.no_cpu_baseline:
         v_cpu, el = cpu_baseline(args.cpu_worlds, args.cpu_steps, args.seed, dt, args.line_search)
         out["cpu_baseline"] = {"value": round(v_cpu, 3), "unit": "world-steps/s", "cores": 1, "kind": "port",

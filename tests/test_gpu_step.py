@@ -22,6 +22,12 @@ def _scene(name, worlds, seed):
         return build_ant_scene(worlds, seed=seed, device="cuda:0"), 1.0e-3, 16
     if name == "boxpile":
         return build_box_pile_scene(worlds, seed=seed, device="cuda:0"), 2.0e-3, 64
+    if name in ("quadruped", "arm_gripper", "pile"):
+        from sap_warp_amd import hetero
+
+        build = {"quadruped": hetero.build_quadruped_scene, "arm_gripper": hetero.build_arm_gripper_scene,
+                 "pile": hetero.build_pile_scene}[name]
+        return build(worlds, seed=seed + 3, device="cuda:0"), 2.0e-3, 64
     if name in ("hinge_slider_stack", "cube_toss"):
         import os
 
@@ -151,3 +157,9 @@ def test_yaml_scene_step_parity(gpu_device, scene, preset, ls):
     """Scenes loaded from the inline scene files: fixed anchors, a limited revolute and prismatic
     joint with drives, free boxes stacked in box-box / capsule-box contact, replicated worlds."""
     _run_and_compare(scene, preset, ls, worlds=8, steps=4, cost_rel_tol=1.0e-15 if ls == "monotone_decay" else None)
+
+
+@pytest.mark.parametrize("scene", ["quadruped", "arm_gripper", "pile"])
+def test_hetero_kind_step_parity(gpu_device, scene):
+    """The other world kinds of the heterogeneous batch, each against homogeneous oracle runs."""
+    _run_and_compare(scene, "approx32", "exact_root", worlds=8, steps=4)
