@@ -317,7 +317,7 @@ class FreeMotion:
 
 
 def _mobilizer_transform(jt, axes, q):
-    """``_sap_calc_across_mobilizer_transform`` sap_helpers.py:1164-1243 (no D6/ball here)."""
+    """``_sap_calc_across_mobilizer_transform`` sap_helpers.py:1164-1243 (no D6 here)."""
     if jt == JOINT_PRISMATIC:
         return np.concatenate([axes[0] * q[0], [0.0, 0.0, 0.0, 1.0]])
     if jt == JOINT_REVOLUTE:
@@ -389,6 +389,15 @@ def free_motion(model, wv: WorldView, joint_q, v0, joint_f_sap, body_f_sap, dt) 
                 wFM = axes[0] * qd[0]
                 A_PB = sp(np.zeros(3), qrot_unit(R_WF, np.cross(wFM, np.cross(wFM, p_MoBo_F))))
                 v_PB = S[ds] * qd[0]
+            elif jt == JOINT_BALL:
+                # three angular mobilities in the joint frame (sap_helpers.py:1345-1361)
+                V_FM = sp(qd[:3], np.zeros(3))
+                v_PB = mobilizer_to_body(X_wpj, p_MoBo_F, V_FM)
+                for k in range(3):
+                    e = np.zeros(3)
+                    e[k] = 1.0
+                    S[ds + k] = mobilizer_to_body(X_wpj, p_MoBo_F, sp(e, np.zeros(3)))
+                A_PB = mobilizer_bias_to_body(X_wpj, p_MoBo_F, V_FM, np.zeros(6))
             elif jt in (JOINT_FREE, JOINT_DISTANCE):
                 V_FM = sp(qd[:3], qd[3:6])
                 v_PB = mobilizer_to_body(X_wpj, p_MoBo_F, V_FM)
@@ -1355,6 +1364,15 @@ def integrate(model, wv: WorldView, joint_q, v_prev, v_new, dt, mode="midpoint")
                 vmid = 0.5 * (v_prev[ds + a] + v_new[ds + a]) if mode == "midpoint" else v_new[ds + a]
                 q_out[qs + a] = q_out[qs + a] + dt * vmid
                 qd_out[ds + a] = v_new[ds + a]
+        elif jt == JOINT_BALL:
+            # quaternion coordinates, angular velocity in the joint frame (solver_sap.py:316-330, 425-450)
+            q = q_out[qs:qs + 4].copy()
+            w_new = v_new[ds:ds + 3]
+            if mode == "midpoint":
+                q_out[qs:qs + 4] = _quat_midpoint(q, 0.5 * (v_prev[ds:ds + 3] + w_new), dt)
+            else:
+                q_out[qs:qs + 4] = _quat_sap_euler(q, w_new, dt)
+            qd_out[ds:ds + 3] = w_new
         elif jt in (JOINT_FREE, JOINT_DISTANCE):
             q = q_out[qs + 3:qs + 7].copy()
             w_new = v_new[ds:ds + 3]
@@ -1417,6 +1435,9 @@ def public_fk(model, wv: WorldView, joint_q_world, joint_qd_world):
             ax = model.joint_axis[wv.d0 + ds].astype(np.float64)
             X_j = np.concatenate([np.zeros(3), q_from_axis_angle(ax, joint_q_world[qs])])
             v_j = np.concatenate([np.zeros(3), ax * joint_qd_world[ds]])
+        elif jt == JOINT_BALL:
+            X_j = np.concatenate([np.zeros(3), np.asarray(joint_q_world[qs:qs + 4], dtype=np.float64)])
+            v_j = np.concatenate([np.zeros(3), np.asarray(joint_qd_world[ds:ds + 3], dtype=np.float64)])
         elif jt in (JOINT_FREE, JOINT_DISTANCE):
             X_j = np.asarray(joint_q_world[qs:qs + 7], dtype=np.float64)
             v_j = np.asarray(joint_qd_world[ds:ds + 6], dtype=np.float64)

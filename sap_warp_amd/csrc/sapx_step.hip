@@ -703,9 +703,14 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
             const int ji = T.dof_joint[i], jj = T.dof_joint[j];
             const unsigned long long anc_i = T.body_anc[T.jchild[ji]];
             const unsigned long long anc_j = T.body_anc[T.jchild[jj]];
+            // two dofs of one joint lie on each other's path: both triangles take
+            // (Ic S_max(i,j)) . S_min(i,j), so A is exactly symmetric in float32 (the two
+            // products differ by rounding, and the solve's gradient and line search read A
+            // from opposite sides)
+            const bool up_i = bit(anc_i, jj), up_j = bit(anc_j, ji);
             val = 0.0f;
-            if (bit(anc_i, jj)) val = sv_dot(load_sv(kFi + i * 6), load_sv(kSO + j * 6));
-            else if (bit(anc_j, ji)) val = sv_dot(load_sv(kFi + j * 6), load_sv(kSO + i * 6));
+            if (up_i && (!up_j || j <= i)) val = sv_dot(load_sv(kFi + i * 6), load_sv(kSO + j * 6));
+            else if (up_j) val = sv_dot(load_sv(kFi + j * 6), load_sv(kSO + i * 6));
             if (i == j) val += M.armature[(size_t)w * n + i];
             s.A[i * ast + j] = val;
         } else {

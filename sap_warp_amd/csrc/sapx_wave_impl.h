@@ -274,12 +274,11 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     // kinematics scratch overlays the J / Lt region (phases 2-4 and 9)
     float* kV = Jm;
     float* kA = kV + nb * 6;
-    float* kI = kA + nb * 6;      // nb x 12 (16-byte rows): inertia about O
-    float* kIc = kI + nb * 12;    // nb x 12: composite inertia about O
-    float* kF = kIc + nb * 12;    // nb x 6: body force, torque about O
-    float* kFc = kF + nb * 6;     // nb x 6: composite force, torque about O
-    float* kFi = kFc + nb * 6;
-    float* kExt = kFi + n * 6;
+    float* kI = kA + nb * 6;      // nb x 12 (16-byte rows): inertia about the COM, COM offset, mass
+    float* kIc = kI + nb * 12;    // nb x 12: subtree composite inertia about the body origin
+    float* kF = kIc + nb * 12;    // nb x 6: body force, torque about the body origin
+    float* kFc = kF + nb * 6;     // nb x 6: subtree composite force, torque about the body origin
+    float* kExt = kFc + nb * 6 + n * 6;
     float* kQdPub = kExt + nb * 6;
     double* kXl = reinterpret_cast<double*>(kQdPub + ((n + 1) & ~1));
     double* kXpn = kXl + nj * 8;  // nj x 8: X_p with its rotation normalised (float64)
@@ -444,6 +443,8 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 Xj.q = dq_axis_angle(axd, (double)q_in[qs]);
             } else if (jt == kJointFree || jt == kJointDistance) {
                 Xj = DTF{d3(q_in[qs], q_in[qs + 1], q_in[qs + 2]), DQ{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]}};
+            } else if (jt == kJointBall) {
+                Xj.q = DQ{q_in[qs], q_in[qs + 1], q_in[qs + 2], q_in[qs + 3]};
             }
             Xl = xcid ? Xj : dtf_compose_n(Xj, dtf_inverse_n(dtf_from(Xc)));
             const TF Xmb = tf_inverse_n(Xc);
@@ -460,6 +461,15 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 const V3 wf = axis * qd;
                 al = sv(v3(0, 0, 0), cross(wf, cross(wf, pmb)));
                 vl = sv_scale(S, qd);
+            } else if (jt == kJointBall) {
+                // three angular mobilities in the joint frame (sap_helpers.py:1345-1361)
+                const V3 wv = v3((float)vec[ds], (float)vec[ds + 1], (float)vec[ds + 2]);
+                vl = sv(wv, cross(wv, pmb));
+                al = sv(v3(0, 0, 0), cross(wv, cross(wv, pmb)));
+                for (int k = 0; k < 3; ++k) {
+                    const V3 e = v3(k == 0, k == 1, k == 2);
+                    store_sv(Sg + (ds + k) * 6, sv(e, cross(e, pmb)));
+                }
             } else if (jt == kJointFree || jt == kJointDistance) {
                 const V3 wv = v3((float)vec[ds], (float)vec[ds + 1], (float)vec[ds + 2]);
                 const V3 vv = v3((float)vec[ds + 3], (float)vec[ds + 4], (float)vec[ds + 5]);
@@ -547,17 +557,18 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     }
 
     // ------------------------------------------------------------------ 3
-    // body dynamics about each body origin; inertia and torque about O = the origin of the
-    // body's tree root (each tree of a forest about its own root, so no body sits far from its
-    // reference point and float32 spatial algebra keeps small inertias)
-    // p_b - O, formed from the float64 poses so far-away worlds lose no lever-arm digits
+    // body dynamics about each body origin. The composite inertias, the bias forces and the
+    // dynamics matrix are taken about the origin of the child body of each dof's joint, with the
+    // lever arms between bodies formed from the float64 poses: a light link far down a chain
+    // keeps its small inertia (about one reference point for the whole tree, float32 loses it
+    // to the parallel-axis terms of the bodies between). The motion subspace about O = the
+    // origin of the body's tree root is what the contact Jacobian needs from phase 6 on.
     auto tree_offset = [&](int b) {
         const int r = T.body_root[b];
         return v3((float)(bqd[b * 8] - bqd[r * 8]), (float)(bqd[b * 8 + 1] - bqd[r * 8 + 1]),
                   (float)(bqd[b * 8 + 2] - bqd[r * 8 + 2]));
     };
     for (int b = lane; b < nb; b += 64) {
-        const V3 pO = tree_offset(b);
         const float m = mM[b];
         const float* Ip = mI + b * 9;
         const float* cp = mCom + b * 4;
@@ -596,84 +607,107 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         fv = fv - mg;
         const float* e = kExt + b * 6;
         float* F = kF + b * 6;
-        // torque about O (tau_O = tau_b + (p_b - O) x f): composites become plain subtree sums
-        const V3 fvt = fv + v3(e[3], e[4], e[5]);
-        const V3 two = fw + v3(e[0], e[1], e[2]) + cross(pO, fvt);
-        F[0] = two.x; F[1] = two.y; F[2] = two.z;
-        F[3] = fvt.x; F[4] = fvt.y; F[5] = fvt.z;
-        const V3 co = pO + c;
-        const float co2 = dot(co, co);
-        const float co_[3] = {co.x, co.y, co.z};
-        float* I = kI + b * 12;
+        F[0] = fw.x + e[0]; F[1] = fw.y + e[1]; F[2] = fw.z + e[2];
+        F[3] = fv.x + e[3]; F[4] = fv.y + e[4]; F[5] = fv.z + e[5];
+        float* I = kI + b * 12;  // inertia about the COM (world axes), COM offset, mass
         int e6 = 0;
         for (int r = 0; r < 3; ++r)
-            for (int k = r; k < 3; ++k) I[e6++] = IW[r][k] + m * ((r == k ? co2 : 0.0f) - co_[r] * co_[k]);
-        I[6] = m * co.x; I[7] = m * co.y; I[8] = m * co.z; I[9] = m;
+            for (int k = r; k < 3; ++k) I[e6++] = IW[r][k];
+        I[6] = c.x; I[7] = c.y; I[8] = c.z; I[9] = m;
     }
     wave_sync();
 
     // ------------------------------------------------------------------ 4
-    // composite inertia and force about O: plain sums over each body's subtree (one body per
-    // lane, every other body's row read by broadcast), the same totals as the reverse sweep
-    // over levels in the reference (free_motion.py:843-1179)
+    // composite inertia and force of each body's subtree about the body's origin (one body per
+    // lane, every other body's row read by broadcast, each with r = p_d - p_c + c_d its COM
+    // relative to p_c), the same totals as the reverse sweep over levels in the reference
+    // (free_motion.py:843-1179):
+    //   I += I_d,com + m (|r|^2 1 - r r^T),  h += m r,  torque += torque_d + (p_d - p_c) x f_d,  f += f_d
     for (int c = lane; c < nb; c += 64) {
         const unsigned long long sm = T.body_sub[c];
-        float4 i0 = make_float4(0.f, 0.f, 0.f, 0.f), i1 = i0, i2 = i0;
-        float f0 = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f, f4 = 0.f, f5 = 0.f;
+        const double pcx = bqd[c * 8], pcy = bqd[c * 8 + 1], pcz = bqd[c * 8 + 2];
+        float I00 = 0.f, I01 = 0.f, I02 = 0.f, I11 = 0.f, I12 = 0.f, I22 = 0.f;
+        V3 hs = v3(0, 0, 0), ts = v3(0, 0, 0), fs = v3(0, 0, 0);
+        float ms = 0.f;
 #pragma unroll 2
         for (int d = 0; d < nb; ++d) {
-            const float sel = ((sm >> d) & 1ull) ? 1.0f : 0.0f;
-            const float4* Id = reinterpret_cast<const float4*>(kI + d * 12);
-            const float4 a0 = Id[0], a1 = Id[1], a2 = Id[2];
+            if (!((sm >> d) & 1ull)) continue;
+            const float* Id = kI + d * 12;
             const float* Fd = kF + d * 6;
-            i0.x = fmaf(sel, a0.x, i0.x); i0.y = fmaf(sel, a0.y, i0.y); i0.z = fmaf(sel, a0.z, i0.z); i0.w = fmaf(sel, a0.w, i0.w);
-            i1.x = fmaf(sel, a1.x, i1.x); i1.y = fmaf(sel, a1.y, i1.y); i1.z = fmaf(sel, a1.z, i1.z); i1.w = fmaf(sel, a1.w, i1.w);
-            i2.x = fmaf(sel, a2.x, i2.x); i2.y = fmaf(sel, a2.y, i2.y);
-            f0 = fmaf(sel, Fd[0], f0); f1 = fmaf(sel, Fd[1], f1); f2 = fmaf(sel, Fd[2], f2);
-            f3 = fmaf(sel, Fd[3], f3); f4 = fmaf(sel, Fd[4], f4); f5 = fmaf(sel, Fd[5], f5);
+            const V3 dl = v3((float)(bqd[d * 8] - pcx), (float)(bqd[d * 8 + 1] - pcy), (float)(bqd[d * 8 + 2] - pcz));
+            const float md = Id[9];
+            const V3 r = dl + v3(Id[6], Id[7], Id[8]);
+            const V3 mr = r * md;
+            const float mrr = dot(mr, r);
+            I00 += Id[0] + (mrr - mr.x * r.x);
+            I01 += Id[1] - mr.x * r.y;
+            I02 += Id[2] - mr.x * r.z;
+            I11 += Id[3] + (mrr - mr.y * r.y);
+            I12 += Id[4] - mr.y * r.z;
+            I22 += Id[5] + (mrr - mr.z * r.z);
+            hs = hs + mr;
+            ms += md;
+            const V3 fd = v3(Fd[3], Fd[4], Fd[5]);
+            ts = ts + v3(Fd[0], Fd[1], Fd[2]) + cross(dl, fd);
+            fs = fs + fd;
         }
-        float4* Ic = reinterpret_cast<float4*>(kIc + c * 12);
-        Ic[0] = i0; Ic[1] = i1; Ic[2] = i2;
+        float* Ic = kIc + c * 12;
+        Ic[0] = I00; Ic[1] = I01; Ic[2] = I02; Ic[3] = I11; Ic[4] = I12; Ic[5] = I22;
+        Ic[6] = hs.x; Ic[7] = hs.y; Ic[8] = hs.z; Ic[9] = ms;
         float* Fc = kFc + c * 6;
-        Fc[0] = f0; Fc[1] = f1; Fc[2] = f2; Fc[3] = f3; Fc[4] = f4; Fc[5] = f5;
+        Fc[0] = ts.x; Fc[1] = ts.y; Fc[2] = ts.z; Fc[3] = fs.x; Fc[4] = fs.y; Fc[5] = fs.z;
     }
     wave_sync();
-    // tau and the dof's motion subspace / momentum about O; from here on the motion subspace
-    // is only needed about O, so it replaces S in place
+    // tau and the dof's composite momentum, both about the origin of the dof's child body c
     float tau = 0.0f;
-    SV SOi = sv(v3(0, 0, 0), v3(0, 0, 0)), Fii = SOi;
+    SV Si = sv(v3(0, 0, 0), v3(0, 0, 0)), SOi = Si, Fii = Si;
+    V3 pOi = v3(0, 0, 0);
     if (lane < n) {
-        const int ji = T.dof_joint[lane];
-        const int c = T.jchild[ji];
-        const SV S = load_sv(Sg + lane * 6);
-        SOi = shift_velocity(S, -tree_offset(c));
-        tau = jf - sv_dot(SOi, load_sv(kFc + c * 6));  // power pairing is the same about any point
-        const float* I = kIc + c * 12;
+        const int ci = T.jchild[T.dof_joint[lane]];
+        Si = load_sv(Sg + lane * 6);
+        pOi = tree_offset(ci);
+        SOi = shift_velocity(Si, -pOi);
+        tau = jf - sv_dot(Si, load_sv(kFc + ci * 6));
+        const float* I = kIc + ci * 12;
         const V3 hv = v3(I[6], I[7], I[8]);
-        const V3 Iw = v3(I[0] * SOi.w.x + I[1] * SOi.w.y + I[2] * SOi.w.z, I[1] * SOi.w.x + I[3] * SOi.w.y + I[4] * SOi.w.z,
-                         I[2] * SOi.w.x + I[4] * SOi.w.y + I[5] * SOi.w.z);
-        Fii = sv(Iw + cross(hv, SOi.v), SOi.v * I[9] - cross(hv, SOi.w));
-        store_sv(Sg + lane * 6, SOi);
-        store_sv(kFi + lane * 6, Fii);
+        const V3 Iw = v3(I[0] * Si.w.x + I[1] * Si.w.y + I[2] * Si.w.z, I[1] * Si.w.x + I[3] * Si.w.y + I[4] * Si.w.z,
+                         I[2] * Si.w.x + I[4] * Si.w.y + I[5] * Si.w.z);
+        Fii = sv(Iw + cross(hv, Si.v), Si.v * I[9] - cross(hv, Si.w));
     }
-    wave_sync();
-    // A_ij = (I_c S_i) . S_j when joint(j) is on joint(i)'s root path, the mirror when joint(i)
-    // is on joint(j)'s, else 0 (dof-level ancestor masks from the topology template)
+    // A_ij = A_ji = (I_c S_i) . S_j about p_ci when joint(j) is on joint(i)'s root path: the lane
+    // of dof i writes both entries (S_j moves to p_ci by v + w x (p_ci - p_cj)); entries with
+    // neither dof on the other's path are 0 (dof-level ancestor masks from the topology
+    // template). Two dofs of one joint are each on the other's path; the higher index writes.
+    // A is exactly symmetric: the gradient (A rows) and the line search (A dv) read A from
+    // opposite sides, and an asymmetric A stalls the line search near the optimum.
     const unsigned long long dmask = lane < n ? T.dof_anc[lane] : 0ull;
     const float arm_i = (lane < n) ? M.armature[(size_t)w * n + lane] : 0.0f;
     float adiag = 1.0f;
-#pragma unroll 2
+    // p_ci - p_cj as a difference of the two tree offsets (column j's from lane j): I_c S_i about
+    // p_ci is small, so the lever's float32 rounding (1e-7 m at 1 m from the root) stays at
+    // float32 precision of A_ij; same-tree pairs only (other pairs are 0)
     for (int j = 0; j < n; ++j) {
         const unsigned long long mj = T.dof_anc[j];
-        const SV SOj = load_sv(Sg + j * 6);
-        const SV Fij = load_sv(kFi + j * 6);
-        const float d1 = sv_dot(Fii, SOj), d2 = sv_dot(Fij, SOi);
-        float val = ((dmask >> j) & 1ull) ? d1 : (((mj >> lane) & 1ull) ? d2 : 0.0f);
+        const SV Sj = load_sv(Sg + j * 6);
+        const V3 dl = v3(pOi.x - rl(pOi.x, j), pOi.y - rl(pOi.y, j), pOi.z - rl(pOi.z, j));
+        float val = sv_dot(Fii, sv(Sj.w, Sj.v + cross(Sj.w, dl)));
+        const bool up_i = (dmask >> j) & 1ull, up_j = (mj >> lane) & 1ull;
+        const bool mine = up_i && (!up_j || j <= lane);
         if (lane == j) { val += arm_i; adiag = val; }
-        if (lane < n) Am[lane * ls + j] = val;
+        if (lane < n) {
+            if (mine) {
+                Am[lane * ls + j] = val;
+                Am[j * ls + lane] = val;
+            } else if (!up_j) {
+                Am[lane * ls + j] = 0.0f;
+            }
+        }
     }
     if (lane < n)
         for (int j = n; j < NP; ++j) Am[lane * ls + j] = 0.0f;
+    wave_sync();
+    // from here on the motion subspace is only needed about O
+    if (lane < n) store_sv(Sg + lane * 6, SOi);
     wave_sync();
 
     // ------------------------------------------------------------------ 5
@@ -1364,12 +1398,15 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     for (int j = lane; j < nj; j += 64) {
         const int jt = T.jtype[j];
         const int qs = T.jqs[j], ds = T.jds[j], nd = T.jds[j + 1] - T.jds[j];
-        if (jt == kJointFree || jt == kJointDistance) {
-            const Q4 q = Q4{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]};
+        if (jt == kJointFree || jt == kJointDistance || jt == kJointBall) {
+            // free: (p, quaternion) with [w, v_O]; ball: quaternion with w (solver_sap.py:288-519)
+            const bool ball = jt == kJointBall;
+            const int qr = ball ? qs : qs + 3;
+            const Q4 q = Q4{q_in[qr], q_in[qr + 1], q_in[qr + 2], q_in[qr + 3]};
             const V3 wn = v3((float)vv[ds], (float)vv[ds + 1], (float)vv[ds + 2]);
-            const V3 von = v3((float)vv[ds + 3], (float)vv[ds + 4], (float)vv[ds + 5]);
+            const V3 von = ball ? v3(0, 0, 0) : v3((float)vv[ds + 3], (float)vv[ds + 4], (float)vv[ds + 5]);
             const V3 wo = v3((float)v0v[ds], (float)v0v[ds + 1], (float)v0v[ds + 2]);
-            const V3 voo = v3((float)v0v[ds + 3], (float)v0v[ds + 4], (float)v0v[ds + 5]);
+            const V3 voo = ball ? v3(0, 0, 0) : v3((float)v0v[ds + 3], (float)v0v[ds + 4], (float)v0v[ds + 5]);
             Q4 qn;
             V3 vmid;
             if (P.integ_mode == 0) {
@@ -1397,6 +1434,11 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 qn = Q4{x, y, z, sw};
                 vmid = von;
             }
+            if (ball) {
+                q_out[qs] = qn.x; q_out[qs + 1] = qn.y; q_out[qs + 2] = qn.z; q_out[qs + 3] = qn.w;
+                qd_pub[ds] = wn.x; qd_pub[ds + 1] = wn.y; qd_pub[ds + 2] = wn.z;
+                continue;
+            }
             q_out[qs] = q_in[qs] + h * vmid.x;
             q_out[qs + 1] = q_in[qs + 1] + h * vmid.y;
             q_out[qs + 2] = q_in[qs + 2] + h * vmid.z;
@@ -1406,7 +1448,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
             const V3 vcom = von + cross(wn, rc);
             qd_pub[ds] = vcom.x; qd_pub[ds + 1] = vcom.y; qd_pub[ds + 2] = vcom.z;
             qd_pub[ds + 3] = wn.x; qd_pub[ds + 4] = wn.y; qd_pub[ds + 5] = wn.z;
-        } else if (jt == kJointFixed || jt == kJointBall) {
+        } else if (jt == kJointFixed) {
             for (int a2 = 0; a2 < nd; ++a2) qd_pub[ds + a2] = (float)vv[ds + a2];
         } else {
             for (int a2 = 0; a2 < nd; ++a2) {
@@ -1447,6 +1489,9 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
             Xj = TF{v3(q_out[qs], q_out[qs + 1], q_out[qs + 2]), Q4{q_out[qs + 3], q_out[qs + 4], q_out[qs + 5], q_out[qs + 6]}};
             jl = v3(qd_pub[ds], qd_pub[ds + 1], qd_pub[ds + 2]);
             ja = v3(qd_pub[ds + 3], qd_pub[ds + 4], qd_pub[ds + 5]);
+        } else if (jt == kJointBall) {
+            Xj.q = Q4{q_out[qs], q_out[qs + 1], q_out[qs + 2], q_out[qs + 3]};
+            ja = v3(qd_pub[ds], qd_pub[ds + 1], qd_pub[ds + 2]);
         }
         const TF Xjc = tf_mul(Xj, tf_inverse(load_tf(mXc + j * 8)));
         float* o = kPl + j * 16;

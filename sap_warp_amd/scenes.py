@@ -1048,6 +1048,60 @@ def build_box_pile_scene(
     return _with_consistent_body_q(b.finalize(device))
 
 
+def build_ball_chain_scene(
+    num_worlds: int = 64,
+    seed: int = 7,
+    *,
+    links: int = 4,
+    anchor_height: float = 1.19,
+    max_tilt: float = 0.35,
+    contact_gap: float = 0.01,
+    device: str = "cpu",
+) -> SapModel:
+    """Chains of capsules on ball joints swinging onto a free box.
+
+    Each world: ``links`` capsules (r 0.04, 0.24 m long) joined end to end by ball joints, the
+    first hung from a world anchor ``anchor_height`` up; a free box (half 0.12 m) on the ground
+    under the chain.  Every ball joint starts at a random rotation (angle U(0, ``max_tilt``) rad)
+    with a random angular velocity N(0, 1) rad/s and a velocity-damping drive (kd 0.05); so
+    3 * links + 6 DOF, capsule-box and box-plane contacts.  With the defaults a
+    hanging chain's tip starts at most 1 cm into the box top (a chain hung lower starts several
+    cm inside the box, and its first step is an ejection at 100+ rad/s rather than a contact).
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = SceneBuilder(rigid_gap=contact_gap)
+    b.world_count = num_worlds
+    cfg = ShapeCfg(density=700.0, ke=2.0e5, tau=0.005, mu=0.8, gap=contact_gap)
+    half_len = 0.12
+    for w in range(num_worlds):
+        b.current_world = w
+        parent, x = -1, tf((0.0, 0.0, anchor_height))
+        joints = []
+        for k in range(links):
+            body = b.add_body(x)
+            b.add_shape(body, GEO_CAPSULE, tf((0.0, 0.0, -half_len)), (0.04, half_len - 0.04, 0.0), cfg)
+            axes = [DofCfg(axis=a, target_kd=0.05, actuator_mode=2, limit_ke=0.0, limit_kd=0.0)
+                    for a in ((1, 0, 0), (0, 1, 0), (0, 0, 1))]
+            px = tf((0.0, 0.0, anchor_height)) if parent < 0 else tf((0.0, 0.0, -2.0 * half_len))
+            j = b.add_joint(parent, body, SAP_JOINT_BALL, angular_axes=axes, parent_xform=px)
+            qs, ds = b.joint_q_start[j], b.joint_qd_start[j]
+            b.joint_q[qs:qs + 4] = list(quat_from_axis_angle(rng.normal(size=3), rng.uniform(0.0, max_tilt)))
+            b.joint_qd[ds:ds + 3] = list(rng.normal(0.0, 1.0, 3))
+            joints.append(j)
+            parent, x = body, tf_mul(x, tf((0.0, 0.0, -2.0 * half_len)))
+        b.add_articulation(joints)
+        box_pose = tf((rng.uniform(-0.15, 0.15), rng.uniform(-0.15, 0.15), 0.121),
+                      quat_from_axis_angle((0, 0, 1), rng.uniform(-math.pi, math.pi)))
+        box = b.add_body(box_pose)
+        b.add_shape(box, GEO_BOX, tf(), (0.12, 0.12, 0.12), ShapeCfg(density=300.0, ke=2.0e5, tau=0.005, mu=0.8,
+                                                                    gap=contact_gap))
+        jb = b.add_free_joint(box)
+        b.add_articulation([jb])
+    b.current_world = -1
+    b.add_ground_plane(ShapeCfg(density=0.0, ke=2.0e5, tau=0.005, mu=0.8, gap=contact_gap, collision_group=-1))
+    return _with_consistent_body_q(b.finalize(device))
+
+
 def _with_consistent_body_q(model: SapModel) -> SapModel:
     """Recompute body_q/body_qd from joint_q/joint_qd with the public FK."""
     from .kinematics_host import eval_fk_public
@@ -1056,5 +1110,5 @@ def _with_consistent_body_q(model: SapModel) -> SapModel:
     return replace(model, body_q=body_q.astype(np.float32), body_qd=body_qd.astype(np.float32))
 
 
-__all__ = ["SceneBuilder", "ShapeCfg", "DofCfg", "build_ant_scene", "build_box_pile_scene",
+__all__ = ["SceneBuilder", "ShapeCfg", "DofCfg", "build_ant_scene", "build_ball_chain_scene", "build_box_pile_scene",
            "build_humanoid_scene"]

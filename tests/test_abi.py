@@ -81,10 +81,10 @@ def _desc_from(model):
     return nat.model_desc(model, keep), keep
 
 
-def test_model_create_rejects_ball_joints():
-    from sap_warp_amd.runtime import SAP_JOINT_BALL
+def test_model_create_rejects_d6_joints():
+    """D6 joints are outside the native step (ball joints are in it); rejected before any GPU work."""
+    from sap_warp_amd.runtime import GEO_SPHERE, SAP_JOINT_D6
     from sap_warp_amd.scenes import DofCfg, SceneBuilder, ShapeCfg, tf
-    from sap_warp_amd.runtime import GEO_SPHERE
 
     b = SceneBuilder()
     b.current_world = 0
@@ -93,14 +93,14 @@ def test_model_create_rejects_ball_joints():
     j0 = b.add_free_joint(root)
     child = b.add_body(tf((0, 0, 0.5)))
     b.add_shape(child, GEO_SPHERE, tf(), (0.1, 0, 0), ShapeCfg())
-    j1 = b.add_joint(root, child, SAP_JOINT_BALL, angular_axes=[DofCfg(axis=a) for a in ((1, 0, 0), (0, 1, 0), (0, 0, 1))],
+    j1 = b.add_joint(root, child, SAP_JOINT_D6, angular_axes=[DofCfg(axis=a) for a in ((1, 0, 0), (0, 1, 0))],
                      parent_xform=tf((0, 0, -0.5)))
     b.add_articulation([j0, j1])
     model = b.finalize()
     lib = nat.load()
     desc, _keep = _desc_from(model)
     h = ctypes.c_void_p()
-    with pytest.raises(NotImplementedError, match="ball"):
+    with pytest.raises(NotImplementedError, match="D6"):
         nat.check(lib.sapx_model_create(ctypes.byref(desc), ctypes.byref(h)), "sapx_model_create")
 
 

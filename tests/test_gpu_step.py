@@ -22,6 +22,10 @@ def _scene(name, worlds, seed):
         return build_ant_scene(worlds, seed=seed, device="cuda:0"), 1.0e-3, 16
     if name == "boxpile":
         return build_box_pile_scene(worlds, seed=seed, device="cuda:0"), 2.0e-3, 64
+    if name == "ball_chain":
+        from sap_warp_amd.scenes import build_ball_chain_scene
+
+        return build_ball_chain_scene(worlds, seed=seed + 7, device="cuda:0"), 2.0e-3, 64
     if name in ("quadruped", "arm_gripper", "pile"):
         from sap_warp_amd import hetero
 
@@ -163,3 +167,19 @@ def test_yaml_scene_step_parity(gpu_device, scene, preset, ls):
 def test_hetero_kind_step_parity(gpu_device, scene):
     """The other world kinds of the heterogeneous batch, each against homogeneous oracle runs."""
     _run_and_compare(scene, "approx32", "exact_root", worlds=8, steps=4)
+
+
+@pytest.mark.parametrize("preset,ls", [("approx32", "exact_root"), ("drake", "armijo_decay")])
+def test_ball_chain_step_parity(gpu_device, preset, ls):
+    """Capsule chains on ball joints (quaternion coordinates, three angular mobilities, velocity
+    damping drives) swinging onto a free box."""
+    _run_and_compare("ball_chain", preset, ls, worlds=8, steps=5)
+
+
+def test_ball_joints_reject_the_fused_kernel(gpu_device):
+    from sap_warp_amd.scenes import build_ball_chain_scene
+    from sap_warp_amd.solver_sap import SolverSAP
+
+    solver = SolverSAP(build_ball_chain_scene(2, device="cuda:0"), max_rigid_contact=64)
+    with pytest.raises(NotImplementedError):
+        solver.set_schedule(kernel="fused", small_capacity=64, fork=False, world_order=False)
