@@ -1060,19 +1060,23 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     float hd = 0.0f;
     double alpha = 1.0, prev_cost = 0.0;
     const double cost_min_alpha = P.ls_mode == kLsMonotone ? 0.0 : 0.5;  // contact_solve.py:6182
+    // A v and J v carried between iterations: after a step v + alpha dv they are updated with the
+    // line search's A dv and J dv (float64), so only the first evaluation runs the mat-vec
+    double av_run = 0.0, adv_ls = 0.0;
+    bool have_av = false;
     if (unconstrained) {
         flags = 1;
     } else {
         while (true) {
             // ---- evaluate at v: vc = J v, A v; projection + model terms; impulses / gradient / norms
-            double av;
-            {
+            double av = av_run;
+            if (!have_av) {
                 const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
                 double c0, c1, c2;
                 matvec_AJ<NP>(arow, J0, js, v, av, c0, c1, c2);
-                if (lane >= n) av = 0.0;
                 vc0 = c0; vc1 = c1; vc2 = c2;
             }
+            if (lane >= n) av = 0.0;
             ar = av - avs;
             double part4[4] = {0.0, 0.0, 0.0, 0.0};  // cost, |D g|^2, |D A v|^2, |D j|^2
             if (lane < nc) {
@@ -1200,6 +1204,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                     matvec_AJ<NP>(arow, J0, js, dvv, adv, d0, d1, d2);
                 }
                 dvc0 = d0; dvc1 = d1; dvc2 = d2;
+                adv_ls = adv;
                 if (lane < n) {
                     const double ri = v - vs;
                     ld[0] = grad * dvv;
@@ -1364,13 +1369,19 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 prev_cost = cost;
                 ls_total += ls_it;
                 if (lane < n) v += alpha * dvv;
-            } else if (P.ls_mode == kLsMonotone) {
-                // a rejected decay search keeps v and records the current cost as the previous one
-                // (contact_solve.py:2647-2662): the cost criterion ends the solve next
-                alpha = a_acc;
-                prev_cost = cost;
-                ls_total += ls_it;
+                av_run = av + alpha * adv_ls;
+                vc0 += alpha * dvc0; vc1 += alpha * dvc1; vc2 += alpha * dvc2;
+            } else {
+                av_run = av;  // v unchanged
+                if (P.ls_mode == kLsMonotone) {
+                    // a rejected decay search keeps v and records the current cost as the previous
+                    // one (contact_solve.py:2647-2662): the cost criterion ends the solve next
+                    alpha = a_acc;
+                    prev_cost = cost;
+                    ls_total += ls_it;
+                }
             }
+            have_av = true;
             ++iteration;
         }
     }
