@@ -75,7 +75,6 @@ struct SapxModel {
     DevModel dm;
     std::vector<void*> owned;
     std::vector<int> h_shape_world, h_shape_type;  // host copies for pipeline set-up
-    bool has_ball = false;                          // ball joints: one-wave step kernel only
     int32_t world_count = 0, joint_count = 0, dof_count = 0, coord_count = 0, body_count = 0, shape_count = 0;
     ~SapxModel() {
         for (void* p : owned) (void)hipFree(p);
@@ -233,8 +232,6 @@ int32_t sapx_solver_set_schedule(SapxSolver* s, const SapxSchedule* in) {
         return fail(SAPX_ERR_UNSUPPORTED, "the learned warm start needs the one-wave step kernel");
     } else if (s->opts.line_search == SAPX_LS_MONOTONE_DECAY) {
         return fail(SAPX_ERR_UNSUPPORTED, "monotone_decay runs in the one-wave step kernel only");
-    } else if (s->model->has_ball) {
-        return fail(SAPX_ERR_UNSUPPORTED, "ball joints run in the one-wave step kernel only");
     }
     SapxSchedule t = *in;
     t.fork = t.fork ? 1 : 0;
@@ -412,7 +409,6 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     SapxModel* m = new SapxModel();
     DevModel& dm = m->dm;
     m->h_shape_world.assign(d->shape_world, d->shape_world + Stot);
-    m->has_ball = std::find(t_jtype.begin(), t_jtype.end(), kJointBall) != t_jtype.end();
     m->h_shape_type.assign(d->shape_type, d->shape_type + Stot);
     dm.W = W; dm.J = J; dm.N = N; dm.NQ = NQ; dm.NB = NB; dm.S = S;
     dm.P = (int)per_world[0].size() / 2;
@@ -698,10 +694,6 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
         if (opts->line_search == SAPX_LS_MONOTONE_DECAY && s->sched.kernel != SAPX_KERNEL_WAVE) {
             delete s;
             return fail(SAPX_ERR_UNSUPPORTED, "monotone_decay runs in the one-wave step kernel only");
-        }
-        if (s->model->has_ball && s->sched.kernel != SAPX_KERNEL_WAVE) {
-            delete s;
-            return fail(SAPX_ERR_UNSUPPORTED, "ball joints run in the one-wave step kernel only");
         }
         if (s->wave_ok && hipMalloc(&s->order, (size_t)dm.W * sizeof(int)) != hipSuccess) {
             delete s;

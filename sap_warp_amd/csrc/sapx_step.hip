@@ -488,6 +488,8 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
                 Xj.q = dq_axis_angle(axd, (double)q_in[qs]);
             } else if (jt == kJointFree || jt == kJointDistance) {
                 Xj = DTF{d3(q_in[qs], q_in[qs + 1], q_in[qs + 2]), DQ{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]}};
+            } else if (jt == kJointBall) {
+                Xj.q = DQ{q_in[qs], q_in[qs + 1], q_in[qs + 2], q_in[qs + 3]};
             }
             Xl = xcid ? Xj : dtf_compose_n(Xj, dtf_inverse_n(dtf_from(Xc)));
             // mobilizer velocity / bias in the parent joint frame (sap_helpers.py:1289-1433)
@@ -515,6 +517,16 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
                     const V3 e = v3(k == 0, k == 1, k == 2);
                     store_sv(s.S + (ds + k) * 6, sv(e, cross(e, pmb)));
                     store_sv(s.S + (ds + 3 + k) * 6, sv(v3(0, 0, 0), e));
+                }
+            } else if (jt == kJointBall) {
+                // three angular dofs in the joint frame (solver_sap.py:316-330)
+                const double* v0 = d64 + V_V0 * npad + ds;
+                const V3 wv = v3((float)v0[0], (float)v0[1], (float)v0[2]);
+                vl = sv(wv, cross(wv, pmb));
+                al = sv(v3(0, 0, 0), cross(wv, cross(wv, pmb)));
+                for (int k = 0; k < 3; ++k) {
+                    const V3 e = v3(k == 0, k == 1, k == 2);
+                    store_sv(s.S + (ds + k) * 6, sv(e, cross(e, pmb)));
                 }
             }
         }
@@ -572,9 +584,12 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
     __syncthreads();
 
     // ------------------------------------------------------------------ 3
-    // body dynamics (free_motion.py:680-719) + inertia about O = root origin
-    // inertia about O = the origin of the body's tree root (each tree of a forest about its own root)
-    // p_b - O, formed from the float64 poses
+    // body dynamics (free_motion.py:680-719) about each body origin. The composite inertias,
+    // the bias forces and the dynamics matrix are taken about the origin of the child body of
+    // each dof's joint, with lever arms between bodies from the float64 poses (about one point
+    // for a whole tree, float32 loses a light link's small inertia to the parallel-axis terms
+    // of the bodies between); the motion subspace about O = the origin of the body's tree root
+    // serves the contact Jacobian. p_b - O, formed from the float64 poses:
     auto tree_offset = [&](int b) {
         const int r = T.body_root[b];
         return v3((float)(s.bqd[b * 8] - s.bqd[r * 8]), (float)(s.bqd[b * 8 + 1] - s.bqd[r * 8 + 1]),
@@ -626,22 +641,20 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
         float* F = kF + b * 6;
         F[0] = fw.x + e[0]; F[1] = fw.y + e[1]; F[2] = fw.z + e[2];
         F[3] = fv.x + e[3]; F[4] = fv.y + e[4]; F[5] = fv.z + e[5];
-        // inertia about O: c_O = (p_b - O) + c
-        V3 co = tree_offset(b) + c;
-        float co2 = dot(co, co);
-        float co_[3] = {co.x, co.y, co.z};
+        // inertia about the body origin, first moment m c, mass
         float* I = kI + b * 10;
         int e6 = 0;
         for (int r = 0; r < 3; ++r)
-            for (int k = r; k < 3; ++k) I[e6++] = IW[r][k] + m * ((r == k ? co2 : 0.0f) - co_[r] * co_[k]);
-        I[6] = m * co.x; I[7] = m * co.y; I[8] = m * co.z; I[9] = m;
+            for (int k = r; k < 3; ++k) I[e6++] = IB[r][k];
+        I[6] = m * c.x; I[7] = m * c.y; I[8] = m * c.z; I[9] = m;
     }
     __syncthreads();
 
     // ------------------------------------------------------------------ 4
-    // composites over subtrees (inertia about O, force at the body origin): reverse
-    // level sweep on wave 0, each body pulls its children's finished composites
-    // (deterministic order; wave-level ordering between levels)
+    // composites over subtrees about each body origin: reverse level sweep on wave 0, each
+    // body pulls its children's finished composites shifted by d = p_ch - p_c (float64
+    // poses): I += I_ch + m (|d|^2 1 - d d^T) + 2 (d.h) 1 - d h^T - h d^T, h += h_ch + m d,
+    // torque += torque_ch + d x f (deterministic order; wave-level ordering between levels)
     if (tid < 64) {
         const int lane = tid;
         for (int L = M.num_levels - 1; L >= 0; --L) {
@@ -653,16 +666,27 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
                 for (int k = 0; k < 10; ++k) Ic[k] = kI[c * 10 + k];
                 const float* F = kF + c * 6;
                 V3 tw = v3(F[0], F[1], F[2]), fv = v3(F[3], F[4], F[5]);
-                const V3 pc = v3(s.bq[c * 8], s.bq[c * 8 + 1], s.bq[c * 8 + 2]);
+                const double pcx = s.bqd[c * 8], pcy = s.bqd[c * 8 + 1], pcz = s.bqd[c * 8 + 2];
                 for (int e = T.child_start[c]; e < T.child_start[c + 1]; ++e) {
                     const int ch = T.child[e];
-#pragma unroll
-                    for (int k = 0; k < 10; ++k) Ic[k] += kIc[ch * 10 + k];
+                    const float* Ich = kIc + ch * 10;
+                    const V3 dl = v3((float)(s.bqd[ch * 8] - pcx), (float)(s.bqd[ch * 8 + 1] - pcy),
+                                     (float)(s.bqd[ch * 8 + 2] - pcz));
+                    const float md = Ich[9];
+                    const V3 hd = v3(Ich[6], Ich[7], Ich[8]);
+                    const float dd = dot(dl, dl), dh2 = 2.0f * dot(dl, hd);
+                    Ic[0] += Ich[0] + md * (dd - dl.x * dl.x) + dh2 - 2.0f * dl.x * hd.x;
+                    Ic[1] += Ich[1] - md * dl.x * dl.y - dl.x * hd.y - hd.x * dl.y;
+                    Ic[2] += Ich[2] - md * dl.x * dl.z - dl.x * hd.z - hd.x * dl.z;
+                    Ic[3] += Ich[3] + md * (dd - dl.y * dl.y) + dh2 - 2.0f * dl.y * hd.y;
+                    Ic[4] += Ich[4] - md * dl.y * dl.z - dl.y * hd.z - hd.y * dl.z;
+                    Ic[5] += Ich[5] + md * (dd - dl.z * dl.z) + dh2 - 2.0f * dl.z * hd.z;
+                    Ic[6] += hd.x + md * dl.x; Ic[7] += hd.y + md * dl.y; Ic[8] += hd.z + md * dl.z;
+                    Ic[9] += md;
                     const float* Fch = kFc + ch * 6;
                     const V3 f = v3(Fch[3], Fch[4], Fch[5]);
-                    const V3 pch = v3(s.bq[ch * 8], s.bq[ch * 8 + 1], s.bq[ch * 8 + 2]);
-                    // shift_force(F_ch, p_c - p_ch) = [tau - (p_c - p_ch) x f, f] (sap_helpers.py:962-966)
-                    tw = tw + v3(Fch[0], Fch[1], Fch[2]) - cross(pc - pch, f);
+                    // shift_force(F_ch, p_c - p_ch) = [tau + (p_ch - p_c) x f, f] (sap_helpers.py:962-966)
+                    tw = tw + v3(Fch[0], Fch[1], Fch[2]) + cross(dl, f);
                     fv = fv + f;
                 }
 #pragma unroll
@@ -682,20 +706,26 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
         SV S = load_sv(s.S + d * 6);
         // tau = joint_f - S . f_s  (free_motion.py:93-147, sap_helpers.py:2090-2118)
         d32[F_TAU * npad + d] = d32[F_JF * npad + d] - sv_dot(S, load_sv(kFc + c * 6));
-        // S about O
+        // S about O (contact Jacobian)
         SV SO = shift_velocity(S, -tree_offset(c));
         store_sv(kSO + d * 6, SO);
-        // composite inertia of the joint's child subtree times SO
+        // composite inertia of the joint's child subtree times S, both about p_c
         const float* I = kIc + c * 10;
         V3 hv = v3(I[6], I[7], I[8]);
-        V3 Iw = v3(I[0] * SO.w.x + I[1] * SO.w.y + I[2] * SO.w.z, I[1] * SO.w.x + I[3] * SO.w.y + I[4] * SO.w.z,
-                   I[2] * SO.w.x + I[4] * SO.w.y + I[5] * SO.w.z);
-        SV Fi = sv(Iw + cross(hv, SO.v), SO.v * I[9] - cross(hv, SO.w));
+        V3 Iw = v3(I[0] * S.w.x + I[1] * S.w.y + I[2] * S.w.z, I[1] * S.w.x + I[3] * S.w.y + I[4] * S.w.z,
+                   I[2] * S.w.x + I[4] * S.w.y + I[5] * S.w.z);
+        SV Fi = sv(Iw + cross(hv, S.v), S.v * I[9] - cross(hv, S.w));
         store_sv(kFi + d * 6, Fi);
     }
     __syncthreads();
-    // mass matrix by composite rigid bodies: H_ij = (Ic S_i) . S_j for joint(j) on
-    // joint(i)'s root path (equals J^T M J of free_motion.py:843-1179)
+    // mass matrix by composite rigid bodies: H_ij = (Ic S_i) . S_j about p_ci for joint(j) on
+    // joint(i)'s root path (equals J^T M J of free_motion.py:843-1179); S_j moves to p_ci by
+    // v + w x (p_ci - p_cj), the difference of the two float32 tree offsets
+    auto crb = [&](int a, int b) {  // (Ic S_a) . S_b about p_ca
+        const V3 dl = tree_offset(T.jchild[T.dof_joint[a]]) - tree_offset(T.jchild[T.dof_joint[b]]);
+        const SV Sb = load_sv(s.S + b * 6);
+        return sv_dot(load_sv(kFi + a * 6), sv(Sb.w, Sb.v + cross(Sb.w, dl)));
+    };
     for (int t = tid; t < npad * npad; t += NT) {
         const int i = t / npad, j = t % npad;
         float val;
@@ -709,8 +739,8 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
             // from opposite sides)
             const bool up_i = bit(anc_i, jj), up_j = bit(anc_j, ji);
             val = 0.0f;
-            if (up_i && (!up_j || j <= i)) val = sv_dot(load_sv(kFi + i * 6), load_sv(kSO + j * 6));
-            else if (up_j) val = sv_dot(load_sv(kFi + j * 6), load_sv(kSO + i * 6));
+            if (up_i && (!up_j || j <= i)) val = crb(i, j);
+            else if (up_j) val = crb(j, i);
             if (i == j) val += M.armature[(size_t)w * n + i];
             s.A[i * ast + j] = val;
         } else {
@@ -1424,13 +1454,16 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
     for (int j = tid; j < nj; j += NT) {
         const int jt = T.jtype[j];
         const int qs = T.jqs[j], ds = T.jds[j], nd = T.jds[j + 1] - T.jds[j];
-        if (jt == kJointFree || jt == kJointDistance) {
-            Q4 q = Q4{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]};
+        if (jt == kJointFree || jt == kJointDistance || jt == kJointBall) {
+            // free: (p, quaternion) with [w, v_O]; ball: quaternion with w (solver_sap.py:288-519)
+            const bool ball = jt == kJointBall;
+            const int qr = ball ? qs : qs + 3;
+            Q4 q = Q4{q_in[qr], q_in[qr + 1], q_in[qr + 2], q_in[qr + 3]};
             V3 wn = v3((float)v[ds], (float)v[ds + 1], (float)v[ds + 2]);
-            V3 von = v3((float)v[ds + 3], (float)v[ds + 4], (float)v[ds + 5]);
+            V3 von = ball ? v3(0, 0, 0) : v3((float)v[ds + 3], (float)v[ds + 4], (float)v[ds + 5]);
             const double* v0 = d64 + V_V0 * npad + ds;
             V3 wo = v3((float)v0[0], (float)v0[1], (float)v0[2]);
-            V3 voo = v3((float)v0[3], (float)v0[4], (float)v0[5]);
+            V3 voo = ball ? v3(0, 0, 0) : v3((float)v0[3], (float)v0[4], (float)v0[5]);
             Q4 qn;
             V3 vmid;
             if (P.integ_mode == 0) {
@@ -1458,6 +1491,11 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
                 qn = Q4{x, y, z, sw};
                 vmid = von;
             }
+            if (ball) {
+                q_out[qs] = qn.x; q_out[qs + 1] = qn.y; q_out[qs + 2] = qn.z; q_out[qs + 3] = qn.w;
+                qd_pub[ds] = wn.x; qd_pub[ds + 1] = wn.y; qd_pub[ds + 2] = wn.z;
+                continue;
+            }
             q_out[qs] = q_in[qs] + h * vmid.x;
             q_out[qs + 1] = q_in[qs + 1] + h * vmid.y;
             q_out[qs + 2] = q_in[qs + 2] + h * vmid.z;
@@ -1468,7 +1506,7 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
             V3 vcom = von + cross(wn, rc);
             qd_pub[ds] = vcom.x; qd_pub[ds + 1] = vcom.y; qd_pub[ds + 2] = vcom.z;
             qd_pub[ds + 3] = wn.x; qd_pub[ds + 4] = wn.y; qd_pub[ds + 5] = wn.z;
-        } else if (jt == kJointFixed || jt == kJointBall) {
+        } else if (jt == kJointFixed) {
             for (int a = 0; a < nd; ++a) qd_pub[ds + a] = (float)v[ds + a];
         } else {
             for (int a = 0; a < nd; ++a) {
@@ -1502,6 +1540,9 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
             Xj = TF{v3(q_out[qs], q_out[qs + 1], q_out[qs + 2]), Q4{q_out[qs + 3], q_out[qs + 4], q_out[qs + 5], q_out[qs + 6]}};
             jl = v3(qd_pub[ds], qd_pub[ds + 1], qd_pub[ds + 2]);
             ja = v3(qd_pub[ds + 3], qd_pub[ds + 4], qd_pub[ds + 5]);
+        } else if (jt == kJointBall) {
+            Xj.q = Q4{q_out[qs], q_out[qs + 1], q_out[qs + 2], q_out[qs + 3]};
+            ja = v3(qd_pub[ds], qd_pub[ds + 1], qd_pub[ds + 2]);
         }
         const TF Xjc = tf_mul(Xj, tf_inverse(load_tf(mXc + j * 8)));
         float* o = kPl + j * 16;

@@ -176,10 +176,8 @@ def test_ball_chain_step_parity(gpu_device, preset, ls):
     _run_and_compare("ball_chain", preset, ls, worlds=8, steps=5)
 
 
-def test_ball_joints_reject_the_fused_kernel(gpu_device):
-    from sap_warp_amd.scenes import build_ball_chain_scene
-    from sap_warp_amd.solver_sap import SolverSAP
-
-    solver = SolverSAP(build_ball_chain_scene(2, device="cuda:0"), max_rigid_contact=64)
-    with pytest.raises(NotImplementedError):
-        solver.set_schedule(kernel="fused", small_capacity=64, fork=False, world_order=False)
+@pytest.mark.parametrize("kernel", ["fused", "wave64"])
+def test_ball_chain_step_parity_other_kernels(gpu_device, monkeypatch, kernel):
+    """Ball joints in the 256-thread kernel and in the 64-slot wave variant alone, same bar."""
+    monkeypatch.setenv("SAPX_STEP_KERNEL", kernel)
+    _run_and_compare("ball_chain", "approx32", "exact_root", worlds=8, steps=5)
