@@ -618,46 +618,56 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     wave_sync();
 
     // ------------------------------------------------------------------ 4
-    // composite inertia and force of each body's subtree about the body's origin (one body per
-    // lane, every other body's row read by broadcast, each with r = p_d - p_c + c_d its COM
-    // relative to p_c), the same totals as the reverse sweep over levels in the reference
-    // (free_motion.py:843-1179):
-    //   I += I_d,com + m (|r|^2 1 - r r^T),  h += m r,  torque += torque_d + (p_d - p_c) x f_d,  f += f_d
-    for (int c = lane; c < nb; c += 64) {
-        const unsigned long long sm = T.body_sub[c];
-        const double pcx = bqd[c * 8], pcy = bqd[c * 8 + 1], pcz = bqd[c * 8 + 2];
-        float I00 = 0.f, I01 = 0.f, I02 = 0.f, I11 = 0.f, I12 = 0.f, I22 = 0.f;
-        V3 hs = v3(0, 0, 0), ts = v3(0, 0, 0), fs = v3(0, 0, 0);
-        float ms = 0.f;
-#pragma unroll 2
-        for (int d = 0; d < nb; ++d) {
-            if (!((sm >> d) & 1ull)) continue;
-            const float* Id = kI + d * 12;
-            const float* Fd = kF + d * 6;
-            const V3 dl = v3((float)(bqd[d * 8] - pcx), (float)(bqd[d * 8 + 1] - pcy), (float)(bqd[d * 8 + 2] - pcz));
-            const float md = Id[9];
-            const V3 r = dl + v3(Id[6], Id[7], Id[8]);
-            const V3 mr = r * md;
-            const float mrr = dot(mr, r);
-            I00 += Id[0] + (mrr - mr.x * r.x);
-            I01 += Id[1] - mr.x * r.y;
-            I02 += Id[2] - mr.x * r.z;
-            I11 += Id[3] + (mrr - mr.y * r.y);
-            I12 += Id[4] - mr.y * r.z;
-            I22 += Id[5] + (mrr - mr.z * r.z);
-            hs = hs + mr;
-            ms += md;
-            const V3 fd = v3(Fd[3], Fd[4], Fd[5]);
-            ts = ts + v3(Fd[0], Fd[1], Fd[2]) + cross(dl, fd);
-            fs = fs + fd;
+    // composite inertia and force of each body's subtree about the body's origin: reverse sweep
+    // over the levels (free_motion.py:843-1179), each body pulling its children's finished
+    // composites shifted by d = p_ch - p_c (float64 poses):
+    //   own: I = I_com + m (|c|^2 1 - c c^T), h = m c;
+    //   child: I += I_ch + m (|d|^2 1 - d d^T) + 2 (d.h) 1 - d h^T - h d^T, h += h_ch + m d,
+    //          torque += torque_ch + d x f_ch, f += f_ch
+    for (int Lv = M.num_levels - 1; Lv >= 0; --Lv) {
+        const int l0 = T.level_start[Lv], l1 = T.level_start[Lv + 1];
+        for (int t = l0 + lane; t < l1; t += 64) {
+            const int c = T.jchild[T.level_joint[t]];
+            const float* Io = kI + c * 12;
+            const float m0 = Io[9];
+            const V3 c0 = v3(Io[6], Io[7], Io[8]);
+            const V3 h0 = c0 * m0;
+            const float hc = dot(h0, c0);
+            float I00 = Io[0] + (hc - h0.x * c0.x), I01 = Io[1] - h0.x * c0.y, I02 = Io[2] - h0.x * c0.z;
+            float I11 = Io[3] + (hc - h0.y * c0.y), I12 = Io[4] - h0.y * c0.z, I22 = Io[5] + (hc - h0.z * c0.z);
+            V3 hs = h0;
+            float ms = m0;
+            const float* Fo = kF + c * 6;
+            V3 ts = v3(Fo[0], Fo[1], Fo[2]), fs = v3(Fo[3], Fo[4], Fo[5]);
+            const double pcx = bqd[c * 8], pcy = bqd[c * 8 + 1], pcz = bqd[c * 8 + 2];
+            for (int e = T.child_start[c]; e < T.child_start[c + 1]; ++e) {
+                const int ch = T.child[e];
+                const float* Ich = kIc + ch * 12;
+                const V3 dl = v3((float)(bqd[ch * 8] - pcx), (float)(bqd[ch * 8 + 1] - pcy), (float)(bqd[ch * 8 + 2] - pcz));
+                const float md = Ich[9];
+                const V3 hd = v3(Ich[6], Ich[7], Ich[8]);
+                const float dd = dot(dl, dl), dh2 = 2.0f * dot(dl, hd);
+                I00 += Ich[0] + md * (dd - dl.x * dl.x) + dh2 - 2.0f * dl.x * hd.x;
+                I01 += Ich[1] - md * dl.x * dl.y - dl.x * hd.y - hd.x * dl.y;
+                I02 += Ich[2] - md * dl.x * dl.z - dl.x * hd.z - hd.x * dl.z;
+                I11 += Ich[3] + md * (dd - dl.y * dl.y) + dh2 - 2.0f * dl.y * hd.y;
+                I12 += Ich[4] - md * dl.y * dl.z - dl.y * hd.z - hd.y * dl.z;
+                I22 += Ich[5] + md * (dd - dl.z * dl.z) + dh2 - 2.0f * dl.z * hd.z;
+                hs = hs + hd + dl * md;
+                ms += md;
+                const float* Fch = kFc + ch * 6;
+                const V3 fd = v3(Fch[3], Fch[4], Fch[5]);
+                ts = ts + v3(Fch[0], Fch[1], Fch[2]) + cross(dl, fd);
+                fs = fs + fd;
+            }
+            float* Ic = kIc + c * 12;
+            Ic[0] = I00; Ic[1] = I01; Ic[2] = I02; Ic[3] = I11; Ic[4] = I12; Ic[5] = I22;
+            Ic[6] = hs.x; Ic[7] = hs.y; Ic[8] = hs.z; Ic[9] = ms;
+            float* Fc = kFc + c * 6;
+            Fc[0] = ts.x; Fc[1] = ts.y; Fc[2] = ts.z; Fc[3] = fs.x; Fc[4] = fs.y; Fc[5] = fs.z;
         }
-        float* Ic = kIc + c * 12;
-        Ic[0] = I00; Ic[1] = I01; Ic[2] = I02; Ic[3] = I11; Ic[4] = I12; Ic[5] = I22;
-        Ic[6] = hs.x; Ic[7] = hs.y; Ic[8] = hs.z; Ic[9] = ms;
-        float* Fc = kFc + c * 6;
-        Fc[0] = ts.x; Fc[1] = ts.y; Fc[2] = ts.z; Fc[3] = fs.x; Fc[4] = fs.y; Fc[5] = fs.z;
+        wave_sync();
     }
-    wave_sync();
     // tau and the dof's composite momentum, both about the origin of the dof's child body c
     float tau = 0.0f;
     SV Si = sv(v3(0, 0, 0), v3(0, 0, 0)), SOi = Si, Fii = Si;
