@@ -301,20 +301,13 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
             if (depth[j] == L) level_joint.push_back(j);
     }
     level_start[nlev] = (int)level_joint.size();
-    std::vector<unsigned long long> anc(NB, 0ull), sub(NB, 0ull);
+    std::vector<unsigned long long> anc(NB, 0ull);
     for (int b = 0; b < NB; ++b) {
         int j = body_joint[b];
         while (j >= 0) {
             anc[b] |= 1ull << j;
             const int p = t_jparent[j];
             j = p >= 0 ? body_joint[p] : -1;
-        }
-        // subtree: walk up from b marking b in every ancestor body's subtree
-        int cur = b;
-        while (cur >= 0) {
-            sub[cur] |= 1ull << b;
-            const int jj = body_joint[cur];
-            cur = jj >= 0 ? t_jparent[jj] : -1;
         }
     }
     // child-body lists (CSR) for the deterministic reverse-level composite sweep
@@ -439,7 +432,6 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     UP(t_dof_limit_ok, dof_limit.data(), N);
     UP(t_body_joint, body_joint.data(), NB);
     UP(t_body_anc, anc.data(), NB);
-    UP(t_body_sub, sub.data(), NB);
     UP(t_body_art_start, art_start.data(), NB);
     UP(t_child_start, child_start.data(), NB + 1);
     UP(t_child, child_list.data(), child_list.size());
@@ -493,7 +485,6 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
                 if ((anc[b] >> dof_joint[d]) & 1ull) anc_dof[b] |= 1ull << d;
         for (int d = 0; d < N; ++d) dof_anc[d] = anc_dof[t_jchild[dof_joint[d]]];
         add64(TP_BODY_ANC, anc);
-        add64(TP_BODY_SUB, sub);
         add64(TP_BODY_ANC_DOF, anc_dof);
         add64(TP_DOF_ANC, dof_anc);
         {

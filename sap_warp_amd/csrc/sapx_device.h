@@ -33,7 +33,7 @@ constexpr int kMaxDof = 64;       // dofs per world (one wavefront per row set)
 enum TopoField {
     TP_JTYPE = 0, TP_JPARENT, TP_JCHILD, TP_JQS, TP_JDS, TP_LEVEL_START, TP_LEVEL_JOINT, TP_DOF_JOINT,
     TP_DOF_COORD, TP_DOF_TARGET, TP_DOF_LIMIT_OK, TP_BODY_ART_START, TP_BODY_ART_COUNT, TP_CHILD_START,
-    TP_CHILD, TP_BODY_ANC, TP_BODY_SUB, TP_BODY_ANC_DOF, TP_DOF_ANC, TP_BODY_UP, TP_BODY_ROOT, TP_COUNT
+    TP_CHILD, TP_BODY_ANC, TP_BODY_ANC_DOF, TP_DOF_ANC, TP_BODY_UP, TP_BODY_ROOT, TP_COUNT
 };
 
 struct DevModel {
@@ -70,7 +70,6 @@ struct DevModel {
     const int* t_dof_limit_ok; // N
     const int* t_body_joint;   // NB: joint with this child body, -1 if none
     const unsigned long long* t_body_anc;  // NB: bit j = joint j on the root path (incl. own)
-    const unsigned long long* t_body_sub;  // NB: bit b = body b in subtree (incl. self)
     const int* t_body_art_start;  // NB: first local dof of the body's articulation (-1 none)
     const int* t_body_art_count;  // NB
     const int* t_child_start;     // NB+1: CSR of child bodies

@@ -150,7 +150,6 @@ struct Topo {
     const int* child_start;
     const int* child;
     const unsigned long long* body_anc;      // NB: bit j = joint j on the body's root path
-    const unsigned long long* body_sub;      // NB: bit b = body b in the body's subtree (incl. itself)
     const unsigned long long* body_anc_dof;  // NB: bit d = dof d's joint on the body's root path
     const unsigned long long* dof_anc;       // N: body_anc_dof of the child body of dof d's joint
     const int* body_up;                      // fk_rounds x NB: ancestor body 2^r levels up, -1 past the root
@@ -175,7 +174,6 @@ __device__ __forceinline__ Topo topo_view(const int* t, const DevModel& M) {
     T.child_start = t + M.topo_off[TP_CHILD_START];
     T.child = t + M.topo_off[TP_CHILD];
     T.body_anc = reinterpret_cast<const unsigned long long*>(t + M.topo_off[TP_BODY_ANC]);
-    T.body_sub = reinterpret_cast<const unsigned long long*>(t + M.topo_off[TP_BODY_SUB]);
     T.body_anc_dof = reinterpret_cast<const unsigned long long*>(t + M.topo_off[TP_BODY_ANC_DOF]);
     T.dof_anc = reinterpret_cast<const unsigned long long*>(t + M.topo_off[TP_DOF_ANC]);
     T.body_up = t + M.topo_off[TP_BODY_UP];
