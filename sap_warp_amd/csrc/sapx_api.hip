@@ -647,7 +647,7 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
             L.o_bq = tk((size_t)nb * 8 * 4);
             L.o_bqd = tk((size_t)nb * 8 * 8);
             L.o_S = tk((size_t)n * 6 * 4);
-            L.o_vec = tk((size_t)128 * 8);  // also the Cholesky panel broadcast (64 x float4)
+            L.o_vec = tk((size_t)128 * 8);
             L.o_ck = tk((size_t)CB * 8 * 8);
             L.o_cG = tk((size_t)CB * 9 * 4);
             L.o_A = tk((size_t)n * L.ls * 4);

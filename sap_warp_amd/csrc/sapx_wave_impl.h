@@ -90,20 +90,20 @@ __device__ __forceinline__ double jt_times(const float* Jm, int js, int nc, int 
     return acc0 + acc1;
 }
 
-// Right-looking Cholesky of the register rows in panels of four columns: h becomes L
+// Right-looking Cholesky of the register rows in panels of eight columns: h becomes L
 // (row `lane`, lower, strictly upper part zero), dinv = 1 / L_ii.  Inside a panel the
 // pivots and the panel's own columns use v_readlane; the panel is then broadcast
-// through `pan` (one float4 per row) and every trailing column k takes its four
-// updates from one broadcast read.  Panels and trailing columns beyond the dof count
-// n are skipped (the padding is the identity and needs no work).
+// through `pan` (two float4 per row, rows < NP: 8 NP floats) and every trailing column k
+// takes its eight updates from two broadcast reads.  Panels and trailing columns beyond
+// the dof count n are skipped (the padding is the identity and needs no work).
 template <int NP>
 __device__ __forceinline__ void chol_rows(float (&h)[NP], float& dinv, int lane, int n, float4* pan) {
     dinv = 1.0f;
 #pragma unroll
-    for (int j0 = 0; j0 < NP; j0 += 4) {
+    for (int j0 = 0; j0 < NP; j0 += 8) {
         if (j0 < n) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < 8; ++t) {
                 const int j = j0 + t;
                 const float pj = fmaxf(rl(h[j], j), 1.0e-30f);
                 const float dj = __builtin_amdgcn_rsqf(pj);
@@ -112,21 +112,28 @@ __device__ __forceinline__ void chol_rows(float (&h)[NP], float& dinv, int lane,
                 h[j] = lij;
                 if (lane == j) dinv = dj;
 #pragma unroll
-                for (int u = t + 1; u < 4; ++u) h[j0 + u] = fmaf(-lij, rl(lij, j0 + u), h[j0 + u]);
+                for (int u = t + 1; u < 8; ++u) h[j0 + u] = fmaf(-lij, rl(lij, j0 + u), h[j0 + u]);
             }
-            if (j0 + 4 < NP && j0 + 4 < n) {
-                pan[lane] = make_float4(h[j0], h[j0 + 1], h[j0 + 2], h[j0 + 3]);
+            if (j0 + 8 < NP && j0 + 8 < n) {
+                if (lane < NP) {
+                    pan[2 * lane] = make_float4(h[j0], h[j0 + 1], h[j0 + 2], h[j0 + 3]);
+                    pan[2 * lane + 1] = make_float4(h[j0 + 4], h[j0 + 5], h[j0 + 6], h[j0 + 7]);
+                }
                 wave_sync();
 #pragma unroll
-                for (int k4 = j0 + 4; k4 < NP; k4 += 4) {
-                    if (k4 < n) {
+                for (int k8 = j0 + 8; k8 < NP; k8 += 8) {
+                    if (k8 < n) {
 #pragma unroll
-                        for (int k = k4; k < k4 + 4; ++k) {
-                            const float4 pk = pan[k];
-                            float a = fmaf(-h[j0], pk.x, h[k]);
-                            a = fmaf(-h[j0 + 1], pk.y, a);
-                            a = fmaf(-h[j0 + 2], pk.z, a);
-                            h[k] = fmaf(-h[j0 + 3], pk.w, a);
+                        for (int k = k8; k < k8 + 8; ++k) {
+                            const float4 pa = pan[2 * k], pb = pan[2 * k + 1];
+                            float a = fmaf(-h[j0], pa.x, h[k]);
+                            a = fmaf(-h[j0 + 1], pa.y, a);
+                            a = fmaf(-h[j0 + 2], pa.z, a);
+                            a = fmaf(-h[j0 + 3], pa.w, a);
+                            a = fmaf(-h[j0 + 4], pb.x, a);
+                            a = fmaf(-h[j0 + 5], pb.y, a);
+                            a = fmaf(-h[j0 + 6], pb.z, a);
+                            h[k] = fmaf(-h[j0 + 7], pb.w, a);
                         }
                     }
                 }
@@ -726,7 +733,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     float dinv;
 #pragma unroll
     for (int j = 0; j < NP; ++j) hrow[j] = (lane < n) ? (j < n ? arow[j] : 0.0f) : ((lane == j) ? 1.0f : 0.0f);
-    chol_rows<NP>(hrow, dinv, lane, n, reinterpret_cast<float4*>(vec));
+    chol_rows<NP>(hrow, dinv, lane, n, reinterpret_cast<float4*>(Lt));  // Lt is free until store_rows
     store_rows<NP>(hrow, Lt, ls, lane);
     wave_sync();
     const float x1 = chol_solve<NP>(hrow, Lt, ls, dinv, lane < n ? tau : 0.0f, lane);
@@ -1180,7 +1187,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 hrow[j] = (lane < n) ? (j < n ? arow[j] + hc + ((lane == j) ? hdiag : 0.0f) : 0.0f) : ((lane == j) ? 1.0f : 0.0f);
             }
             wave_sync();
-            chol_rows<NP>(hrow, dinv, lane, n, reinterpret_cast<float4*>(vec));
+            chol_rows<NP>(hrow, dinv, lane, n, reinterpret_cast<float4*>(Lt));  // Lt is free until store_rows
             store_rows<NP>(hrow, Lt, ls, lane);
             wave_sync();
             float dvf = chol_solve<NP>(hrow, Lt, ls, dinv, lane < n ? (float)(-grad) : 0.0f, lane);
