@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import sap_oracle as so
-from sap_warp_amd.runtime import GEO_CAPSULE, SAP_JOINT_BALL, SAP_JOINT_REVOLUTE
+from sap_warp_amd.runtime import GEO_BOX, GEO_CAPSULE, SAP_JOINT_BALL, SAP_JOINT_REVOLUTE
 from sap_warp_amd.scenes import DofCfg, SceneBuilder, ShapeCfg, _with_consistent_body_q, quat_from_axis_angle, tf
 from tests.helpers import control_dict_from_model, state_dict_from_model
 
@@ -99,3 +99,37 @@ def test_ball_matches_revolute_in_plane(integration):
     np.testing.assert_allclose(ob["joint_qd"][1:], 0.0, atol=1e-9)
     angle = 2.0 * math.atan2(ob["joint_q"][0], ob["joint_q"][3])
     assert angle == pytest.approx(float(orv["joint_q"][0]), abs=2e-6)
+
+
+def test_ball_chain_scene_starts_in_light_contact():
+    """The ball-chain scene hangs each chain just above its box: every world's contacts start at
+    most about 1 cm deep (a chain hung lower starts several cm inside the box and its first step
+    is an ejection, not a contact), the chain touches the box in some worlds, and one oracle
+    step converges everywhere."""
+    from oracle.collide import collide
+    from sap_warp_amd.scenes import build_ball_chain_scene
+
+    m = build_ball_chain_scene(8, seed=7)
+    assert m.joint_dof_count == 8 * 18
+    c = collide(m, m.body_q, 64)
+    assert c["unsupported"] == 0
+    bq = np.asarray(m.body_q, dtype=np.float64)
+    depths, chain_box = [], 0
+    for k in range(int(c["count"])):
+        p = []
+        for side in (0, 1):
+            s = int(c[f"shape{side}"][k])
+            b = int(m.shape_body[s])
+            x = np.asarray(c[f"point{side}"][k], dtype=np.float64)
+            if b >= 0:
+                x = bq[b, :3] + qmat(bq[b, 3:7]) @ x
+            p.append(x)
+        d = float(np.dot(p[1] - p[0], c["normal"][k]) - c["margin0"][k] - c["margin1"][k])
+        depths.append(d)
+        types = {int(m.shape_type[int(c["shape0"][k])]), int(m.shape_type[int(c["shape1"][k])])}
+        chain_box += int(types == {GEO_CAPSULE, GEO_BOX})
+    assert min(depths) > -0.012, min(depths)
+    assert chain_box >= 2
+    opts = so.StepOptions(preset="approx32", line_search="exact_root", max_rigid_contact=64)
+    out = so.step(m, state_dict_from_model(m), control_dict_from_model(m), c, 2.0e-3, opts)
+    assert all(r["result"].converged for r in out["per_world"].values())
