@@ -33,33 +33,44 @@ namespace wv {
 // ---------------------------------------------------------------------------
 
 // sum_j A[row][j] x_j for the LDS row of this lane (A zero-padded to NP columns);
-// x is lane j's value (zero on lanes >= n), broadcast by v_readlane
+// x is lane j's value (zero on lanes >= n), staged in LDS (xs, 64 doubles) and read back two
+// columns per uniform-address load
 template <int NP>
-__device__ __forceinline__ double lds_row_dot(const float* arow, double x) {
+__device__ __forceinline__ double lds_row_dot(const float* arow, double x, double* xs, int lane) {
+    xs[lane] = x;
+    wave_sync();
+    const double2* x2 = reinterpret_cast<const double2*>(xs);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
     for (int j = 0; j < NP; j += 4) {
-        s0 += (double)arow[j] * readlane_d(x, j);
-        s1 += (double)arow[j + 1] * readlane_d(x, j + 1);
-        s2 += (double)arow[j + 2] * readlane_d(x, j + 2);
-        s3 += (double)arow[j + 3] * readlane_d(x, j + 3);
+        const double2 xa = x2[j / 2], xb = x2[j / 2 + 1];
+        s0 += (double)arow[j] * xa.x;
+        s1 += (double)arow[j + 1] * xa.y;
+        s2 += (double)arow[j + 2] * xb.x;
+        s3 += (double)arow[j + 3] * xb.y;
     }
+    wave_sync();
     return (s0 + s1) + (s2 + s3);
 }
 
 // One pass over the dofs: lane i gets (A x)_i from its LDS row, lane c gets the three
-// contact-Jacobian rows of contact c times x.  x_j is broadcast from lane j and must be
-// zero on lanes >= n; A and J rows are zero-padded to NP columns, so the loop is fully
-// unrolled without guards (every lane's loads are independent of the chain).
+// contact-Jacobian rows of contact c times x.  x_j (zero on lanes >= n) is staged in LDS (xs,
+// 64 doubles) and read back two columns per uniform-address load; A and J rows are
+// zero-padded to NP columns, so the loop is fully unrolled without guards (every lane's loads
+// are independent of the chain).
 template <int NP>
 __device__ __forceinline__ void matvec_AJ(const float* arow, const float* J0, int js, double x, double& ax,
-                                          double& jx0, double& jx1, double& jx2) {
+                                          double& jx0, double& jx1, double& jx2, double* xs, int lane) {
+    xs[lane] = x;
+    wave_sync();
+    const double2* x2 = reinterpret_cast<const double2*>(xs);
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0;
 #pragma unroll
     for (int j = 0; j < NP; j += 4) {
+        const double2 xa = x2[j / 2], xb = x2[j / 2 + 1];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const double xj = readlane_d(x, j + u);
+            const double xj = u == 0 ? xa.x : (u == 1 ? xa.y : (u == 2 ? xb.x : xb.y));
             const double t = (double)arow[j + u] * xj;
             if (u == 0) s0 += t; else if (u == 1) s1 += t; else if (u == 2) s2 += t; else s3 += t;
             c0 += (double)J0[j + u] * xj;
@@ -67,6 +78,7 @@ __device__ __forceinline__ void matvec_AJ(const float* arow, const float* J0, in
             c2 += (double)J0[2 * js + j + u] * xj;
         }
     }
+    wave_sync();
     ax = (s0 + s1) + (s2 + s3);
     jx0 = c0; jx1 = c1; jx2 = c2;
 }
@@ -738,7 +750,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
     wave_sync();
     const float x1 = chol_solve<NP>(hrow, Lt, ls, dinv, lane < n ? tau : 0.0f, lane);
     const double x1d = lane < n ? (double)x1 : 0.0;
-    const double ax1 = lds_row_dot<NP>(arow, x1d);
+    const double ax1 = lds_row_dot<NP>(arow, x1d, vec, lane);
     const double r1 = lane < n ? (double)tau - ax1 : 0.0;
     const float x2 = chol_solve<NP>(hrow, Lt, ls, dinv, (float)r1, lane);
     const double vs = lane < n ? v0 + dt * (x1d + (double)x2) : 0.0;
@@ -1015,7 +1027,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         v = P.use_guess ? P.v_guess[(size_t)w * n + lane] : v0;
         if (unconstrained || !part) v = vs;
     }
-    double avs = lds_row_dot<NP>(arow, vs);
+    double avs = lds_row_dot<NP>(arow, vs, vec, lane);
     if (lane >= n) avs = 0.0;
     const double* k8 = ck + (lane < CB ? lane : 0) * kCk;
     // learned warm start: v_plain + R delta (free-joint rows rotated out of the joint's body
@@ -1039,7 +1051,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
         auto sap_cost = [&](double x) -> double {
             double ax, c0, c1, c2;
             const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
-            matvec_AJ<NP>(arow, J0, js, x, ax, c0, c1, c2);
+            matvec_AJ<NP>(arow, J0, js, x, ax, c0, c1, c2, vec, lane);
             double cst = 0.0;
             if (lane < nc) {
                 const double vcv[3] = {c0, c1, c2};
@@ -1090,7 +1102,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
             if (!have_av) {
                 const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
                 double c0, c1, c2;
-                matvec_AJ<NP>(arow, J0, js, v, av, c0, c1, c2);
+                matvec_AJ<NP>(arow, J0, js, v, av, c0, c1, c2, vec, lane);
                 vc0 = c0; vc1 = c1; vc2 = c2;
             }
             if (lane >= n) av = 0.0;
@@ -1197,7 +1209,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 double t0, t1, t2, adv;
                 {
                     const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
-                    matvec_AJ<NP>(arow, J0, js, dvv, adv, t0, t1, t2);
+                    matvec_AJ<NP>(arow, J0, js, dvv, adv, t0, t1, t2, vec, lane);
                 }
                 double u0 = 0.0, u1 = 0.0, u2 = 0.0;
                 if (lane < nc) {
@@ -1218,7 +1230,7 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
                 double adv, d0, d1, d2;
                 {
                     const float* J0 = Jm + ((lane < CB ? lane : 0) * 3) * js;
-                    matvec_AJ<NP>(arow, J0, js, dvv, adv, d0, d1, d2);
+                    matvec_AJ<NP>(arow, J0, js, dvv, adv, d0, d1, d2, vec, lane);
                 }
                 dvc0 = d0; dvc1 = d1; dvc2 = d2;
                 adv_ls = adv;
