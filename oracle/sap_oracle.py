@@ -1080,6 +1080,31 @@ def _participating(model, wv: WorldView, cs: ContactSet, mt: ModelTerms):
     return part
 
 
+def _optimality_norms(ev, part, D, opts: SolveOptions):
+    """SAP optimality test terms (contact_solve.py:1970-2087): ||D grad|| and
+    eps_a + eps_r max(||D A v||, ||D j||) over the participating dofs, D = diag(A)^(-1/2)."""
+    mask = part == 1
+    gn = math.sqrt(float(np.sum((D[mask] * ev["grad"][mask]) ** 2)))
+    pn = math.sqrt(float(np.sum((D[mask] * ev["Av"][mask]) ** 2)))
+    jn = math.sqrt(float(np.sum((D[mask] * ev["jimp"][mask]) ** 2)))
+    return gn, opts.optimality_abs_tol + opts.optimality_rel_tol * max(pn, jn)
+
+
+def optimality_at(model, w, world_result: dict, v, dt, opts: SolveOptions | None = None):
+    """Evaluate the SAP optimality test of world ``w``'s problem (as ``step_world`` built it,
+    float64) at an arbitrary velocity ``v``: returns (||D grad||, tolerance, cost, gamma).
+    Used to check a velocity computed elsewhere against the reference stopping rule."""
+    opts = opts or SolveOptions()
+    wv = WorldView(model, w)
+    fm, cs, mt = world_result["free_motion"], world_result["contacts"], world_result["model_terms"]
+    prob = SapProblem(fm.A_env, fm.v_star, cs, mt, dt, opts.beta, opts.sigma)
+    part = _participating(model, wv, cs, mt)
+    a_diag = np.array([d if (d >= 1.0e-12 and math.isfinite(d)) else 1.0e-12 for d in np.diag(fm.A_env)])
+    ev = prob.evaluate(np.asarray(v, dtype=np.float64))
+    gn, tol = _optimality_norms(ev, part, 1.0 / np.sqrt(a_diag), opts)
+    return gn, tol, ev["cost"], ev["gamma"]
+
+
 def sap_solve(model, wv: WorldView, A, v_star, v0, v_guess, cs: ContactSet, mt: ModelTerms, dt,
               opts: SolveOptions) -> SolveResult:
     """SAP Newton solve of one world (contact_solve.py:6129-6285, 5474-5655)."""
@@ -1103,11 +1128,7 @@ def sap_solve(model, wv: WorldView, A, v_star, v0, v_guess, cs: ContactSet, mt: 
     status = 0
     while True:
         ev = prob.evaluate(v)
-        mask = part == 1
-        gn = math.sqrt(float(np.sum((D[mask] * ev["grad"][mask]) ** 2)))
-        pn = math.sqrt(float(np.sum((D[mask] * ev["Av"][mask]) ** 2)))
-        jn = math.sqrt(float(np.sum((D[mask] * ev["jimp"][mask]) ** 2)))
-        opt_tol = opts.optimality_abs_tol + opts.optimality_rel_tol * max(pn, jn)
+        gn, opt_tol = _optimality_norms(ev, part, D, opts)
         opt_reached = gn <= opt_tol
         cost_reached = False
         if iteration > 0 and alpha > cost_min_alpha:
@@ -1512,8 +1533,10 @@ def step_world(model, w, state: dict, control: dict, contacts: dict, dt, opts: S
 
 
 def step(model, state: dict, control: dict, contacts: dict, dt, opts: StepOptions, v_guess=None, worlds=None):
-    """All worlds (or ``worlds``): returns global arrays + per-world results."""
+    """All worlds (or ``worlds``): returns global arrays + per-world results.  ``dt`` is a scalar or
+    one timestep per world (``SolverSAP._update_dt_world``, solver_sap.py:975-992)."""
     worlds = range(model.world_count) if worlds is None else worlds
+    dts = np.broadcast_to(np.asarray(dt, dtype=np.float64), (model.world_count,))
     out = dict(joint_q=np.asarray(state["joint_q"], dtype=np.float64).copy(),
                joint_qd=np.asarray(state["joint_qd"], dtype=np.float64).copy(),
                body_q=np.asarray(state["body_q"], dtype=np.float64).copy(),
@@ -1522,7 +1545,7 @@ def step(model, state: dict, control: dict, contacts: dict, dt, opts: StepOption
     for w in worlds:
         wv = WorldView(model, w)
         guess = None if v_guess is None else v_guess[wv.d0:wv.d1]
-        r = step_world(model, w, state, control, contacts, dt, opts, guess)
+        r = step_world(model, w, state, control, contacts, float(dts[w]), opts, guess)
         out["joint_q"][wv.c0:wv.c1] = r["joint_q"]
         out["joint_qd"][wv.d0:wv.d1] = r["joint_qd"]
         out["body_q"][wv.b0:wv.b1] = r["body_q"]

@@ -250,16 +250,98 @@ def model_desc(model, keep: HostArrays) -> SapxModelDesc:
     return d
 
 
-class NativeModel:
-    """Owns a ``SapxModel*`` (device-resident copy of a SapModel)."""
+def require_gpu_device(device, who: str):
+    """The torch device a native object runs on; raises unless ``device`` names a visible GPU."""
+    import torch
 
-    def __init__(self, model):
+    d = torch.device(device)
+    if d.type != "cuda":
+        raise ValueError(f"{who} runs on the GPU, but the model lives on {str(device)!r}: build it with a GPU "
+                         "device (e.g. device='cuda:0')")
+    if not torch.cuda.is_available():
+        raise RuntimeError(f"{who}: no GPU is visible to this process")
+    if d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    if d.index >= torch.cuda.device_count():
+        raise ValueError(f"{who}: device {d} does not exist")
+    return d
+
+
+def require_on(t, device, name: str) -> None:
+    """Raise before any launch unless tensor ``t`` lives on ``device`` (a GPU)."""
+    import torch
+
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name} must be a torch tensor on {device}")
+    if t.device != device:
+        raise ValueError(f"{name} is on {t.device}, but the native object runs on {device}")
+
+
+def on_device(device):
+    """Context that makes ``device`` current for native calls (allocations, launches)."""
+    import torch
+
+    return torch.cuda.device(device)
+
+
+_CONTACT_FIELDS = (("rigid_contact_shape0", 1, "int32"), ("rigid_contact_shape1", 1, "int32"),
+                   ("rigid_contact_point0", 3, "float32"), ("rigid_contact_point1", 3, "float32"),
+                   ("rigid_contact_normal", 3, "float32"), ("rigid_contact_margin0", 1, "float32"),
+                   ("rigid_contact_margin1", 1, "float32"))
+
+
+def contact_capacity(contacts) -> int:
+    if contacts.rigid_contact_max:
+        return int(contacts.rigid_contact_max)
+    return int(contacts.rigid_contact_shape0.shape[0]) if contacts.rigid_contact_shape0 is not None else 0
+
+
+def check_contacts(contacts, device, world_count: int, need_world_start: bool = False) -> None:
+    """Device, dtype and size checks of a ``SapContacts`` bundle before its pointers reach a kernel."""
+    import torch
+
+    if contacts.rigid_contact_count is None:
+        return
+    require_on(contacts.rigid_contact_count, device, "contacts.rigid_contact_count")
+    if contacts.rigid_contact_count.dtype != torch.int32 or contacts.rigid_contact_count.numel() < 1:
+        raise TypeError("contacts.rigid_contact_count must be an int32 tensor with one entry")
+    cap = contact_capacity(contacts)
+    for f, width, dt in _CONTACT_FIELDS:
+        t = getattr(contacts, f)
+        if t is None:
+            raise ValueError(f"contacts.{f} is missing")
+        require_on(t, device, f"contacts.{f}")
+        if str(t.dtype) != f"torch.{dt}":
+            raise TypeError(f"contacts.{f} must be {dt}")
+        if not t.is_contiguous():
+            raise ValueError(f"contacts.{f} must be contiguous")
+        if t.numel() < cap * width:
+            raise ValueError(f"contacts.{f} holds {t.numel() // width} contacts, fewer than the capacity {cap}")
+    ws = contacts.rigid_contact_world_start
+    if ws is None:
+        if need_world_start:
+            raise ValueError("contacts need rigid_contact_world_start; allocate them with pipeline.contacts()")
+        return
+    require_on(ws, device, "contacts.rigid_contact_world_start")
+    if ws.dtype != torch.int32 or ws.numel() < world_count + 1 or not ws.is_contiguous():
+        raise ValueError(f"contacts.rigid_contact_world_start must be a contiguous int32 tensor of "
+                         f"world_count + 1 = {world_count + 1} entries")
+
+
+class NativeModel:
+    """Owns a ``SapxModel*`` (device-resident copy of a SapModel) on ``device``."""
+
+    def __init__(self, model, device=None):
         lib = load()
         keep = HostArrays()
         desc = model_desc(model, keep)
         h = _P()
-        check(lib.sapx_model_create(ctypes.byref(desc), ctypes.byref(h)), "sapx_model_create")
+        if device is None:
+            device = require_gpu_device(model.device, "NativeModel")
+        with on_device(device):
+            check(lib.sapx_model_create(ctypes.byref(desc), ctypes.byref(h)), "sapx_model_create")
         self.handle = h
+        self.device = device
         self._lib = lib
 
     def __del__(self):
@@ -290,5 +372,5 @@ def contacts_view(contacts) -> SapxContactsView:
     v.normal = _ptr(contacts.rigid_contact_normal)
     v.margin0 = _ptr(contacts.rigid_contact_margin0)
     v.margin1 = _ptr(contacts.rigid_contact_margin1)
-    v.capacity = int(contacts.rigid_contact_max or (contacts.rigid_contact_shape0.shape[0] if contacts.rigid_contact_shape0 is not None else 0))
+    v.capacity = contact_capacity(contacts)
     return v

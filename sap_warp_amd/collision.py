@@ -6,6 +6,10 @@ box-box) in one HIP kernel per world (``sapx_collide``), writing contacts
 grouped by world in candidate-pair order together with per-world offsets
 (``rigid_contact_world_start``) that ``SolverSAP.step`` consumes directly.
 
+``reduce_contacts`` is accepted and has no effect: the reference applies it
+only to mesh contacts on the GPU and switches it off for models without
+meshes (``narrow_phase.py:1387-1388``); this pipeline has no mesh shapes.
+
 Broadphase modes (``pipeline.py:281-292, 421-454``): ``"explicit"`` (the
 default) tests the model's explicit pair list; ``"nxn"`` and ``"sap"`` test
 every pair of a world's colliding shapes plus the global shapes that the
@@ -42,6 +46,7 @@ class SapCollisionPipeline:
         if mode not in ("explicit", "nxn", "sap"):
             raise ValueError(f"broad_phase must be one of ('nxn', 'sap', 'explicit'), got {broad_phase!r}")
         self.broad_phase_mode = mode
+        self._device = nat.require_gpu_device(model.device, "SapCollisionPipeline")
         W = max(int(model.world_count), 1)
         if max_contacts_per_world is None:
             max_contacts_per_world = (int(rigid_contact_max) // W) if rigid_contact_max else 128
@@ -51,9 +56,14 @@ class SapCollisionPipeline:
         self.reduce_contacts = bool(reduce_contacts)
         self.max_contacts_per_world = int(max_contacts_per_world)
         self.rigid_contact_max = int(rigid_contact_max) if rigid_contact_max else self.max_contacts_per_world * W
-        self._native_model = nat.NativeModel(model)
+        self._native_model = nat.NativeModel(model, self._device)
         self._lib = nat.load()
         h = ctypes.c_void_p()
+        with nat.on_device(self._device):
+            self._create(model, mode, h)
+        self._handle = h
+
+    def _create(self, model, mode, h):
         if mode == "explicit":
             nat.check(self._lib.sapx_pipeline_create(self._native_model.handle, self.max_contacts_per_world,
                                                      ctypes.byref(h)), "sapx_pipeline_create")
@@ -70,7 +80,6 @@ class SapCollisionPipeline:
                 self._native_model.handle, self.max_contacts_per_world, groups.ctypes.data_as(ip),
                 flags.ctypes.data_as(ip), fp.ctypes.data_as(ip) if fp.shape[0] else None, int(fp.shape[0]),
                 ctypes.byref(h)), "sapx_pipeline_create_all_pairs")
-        self._handle = h
 
     def close(self) -> None:
         if getattr(self, "_handle", None):
@@ -93,14 +102,18 @@ class SapCollisionPipeline:
 
         if not isinstance(contacts, SapContacts):
             raise TypeError("collide requires SapContacts")
-        if contacts.rigid_contact_world_start is None:
-            raise ValueError("contacts need rigid_contact_world_start; allocate them with pipeline.contacts()")
+        if contacts.rigid_contact_count is None:
+            raise ValueError("contacts need rigid_contact_count; allocate them with pipeline.contacts()")
+        nat.check_contacts(contacts, self._device, self.model.world_count, need_world_start=True)
         bq = state.body_q
-        if bq.dtype != torch.float32 or bq.numel() != self.model.body_count * 7 or not bq.is_contiguous():
+        if (not isinstance(bq, torch.Tensor) or bq.dtype != torch.float32 or bq.numel() != self.model.body_count * 7
+                or not bq.is_contiguous()):
             raise TypeError("state.body_q must be a contiguous float32 (body_count, 7) tensor")
+        nat.require_on(bq, self._device, "state.body_q")
         view = nat.contacts_view(contacts)
-        nat.check(self._lib.sapx_collide(self._handle, nat._ptr(bq), ctypes.byref(view), nat.stream_ptr()),
-                  "sapx_collide")
+        with nat.on_device(self._device):
+            rc = self._lib.sapx_collide(self._handle, nat._ptr(bq), ctypes.byref(view), nat.stream_ptr())
+        nat.check(rc, "sapx_collide")
         return contacts
 
 

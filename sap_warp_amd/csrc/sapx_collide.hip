@@ -384,14 +384,16 @@ __global__ __launch_bounds__(NT) void sapx_world_scan_kernel(CompactParams P) {
             if (wv < wave) off += part[wv];
             tot += part[wv];
         }
-        if (wi < P.W) P.world_start[wi] = off + incl - x;
+        // offsets past the contact buffer's capacity are clamped to it: the contacts there are
+        // dropped by the compaction, and every world's range stays inside the buffer
+        if (wi < P.W) P.world_start[wi] = min(off + incl - x, P.capacity);
         __syncthreads();
         if (tid == 0) carry += tot;
         __syncthreads();
     }
     if (tid == 0) {
-        P.world_start[P.W] = carry;
-        P.count[0] = carry;
+        P.world_start[P.W] = min(carry, P.capacity);
+        P.count[0] = min(carry, P.capacity);
     }
 }
 

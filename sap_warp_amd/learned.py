@@ -96,14 +96,29 @@ class Mlp:
     def reserve(self, batch: int):
         nat.check(self._lib.sapx_mlp_reserve(self._h, int(batch)), "sapx_mlp_reserve")
 
+    def _check_io(self, B, **tensors):
+        import torch
+
+        widths = dict(x=self.in_features, y=self.out_features, target=self.out_features, loss=1)
+        for name, t in tensors.items():
+            if t is None:
+                continue
+            if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float32 or not t.is_contiguous():
+                raise TypeError(f"{name} must be a contiguous float32 GPU tensor")
+            need = 1 if name == "loss" else B * widths[name]
+            if t.numel() < need:
+                raise ValueError(f"{name} holds {t.numel()} floats, the batch needs {need}")
+
     def forward(self, x, y, batch: int | None = None):
         """y[:B] = network(x[:B]) (device float32 tensors, row-major)."""
         B = int(batch if batch is not None else x.shape[0])
+        self._check_io(B, x=x, y=y)
         nat.check(self._lib.sapx_mlp_forward(self._h, nat._ptr(x), B, nat._ptr(y), nat.stream_ptr()),
                   "sapx_mlp_forward")
 
     def gradients(self, x, target, loss=None, batch: int | None = None):
         B = int(batch if batch is not None else x.shape[0])
+        self._check_io(B, x=x, target=target, loss=loss)
         nat.check(self._lib.sapx_mlp_gradients(self._h, nat._ptr(x), nat._ptr(target), B, nat._ptr(loss),
                                                nat.stream_ptr()), "sapx_mlp_gradients")
 
@@ -147,6 +162,10 @@ class LearnedWarmStart:
 
     # ------------------------------------------------------------------ plumbing
     def compute_features(self, state, control, contacts):
+        self.solver._check_state(state, "state")
+        if control is not None:
+            self.solver._check_control(control)
+        nat.check_contacts(contacts, self.solver._device, self.W, need_world_start=True)
         sin = nat.SapxStateView(nat._ptr(state.joint_q), nat._ptr(state.joint_qd), nat._ptr(state.body_q),
                                 nat._ptr(state.body_qd), nat._ptr(state.body_f))
         ctl = nat.SapxControlView()

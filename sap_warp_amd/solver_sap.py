@@ -151,6 +151,7 @@ class SolverSAP:
             raise TypeError("SolverSAP requires SapModel; convert frontend data before constructing the solver.")
         if int(model.joint_count) <= 0 or int(model.joint_dof_count) <= 0:
             raise ValueError("SolverSAP requires a model with articulated joint DOFs.")
+        self._device = nat.require_gpu_device(model.device, "SolverSAP")
         preset = _CONTACT_PRESET_KWARGS[_normalize_contact_preset_variant(contact_preset_variant)]
         self.contact_preset_variant = _normalize_contact_preset_variant(contact_preset_variant)
         self.use_f64_boundary_pose = bool(preset["use_f64_boundary_pose"] if use_f64_boundary_pose is None
@@ -230,7 +231,7 @@ class SolverSAP:
             from dataclasses import replace
 
             native_model_src = replace(model, shape_material_mu=np.full_like(mu, self.fallback_mu))
-        self._native_model = nat.NativeModel(native_model_src)
+        self._native_model = nat.NativeModel(native_model_src, self._device)
         self._lib = nat.load()
         opts = nat.SapxSolverOpts()
         opts.max_rigid_contact = self.max_rigid_contact
@@ -260,12 +261,17 @@ class SolverSAP:
         opts.capture_snapshots = int(self.capture_contact_jacobian_snapshots)
         self._opts = opts
         h = ctypes.c_void_p()
-        nat.check(self._lib.sapx_solver_create(self._native_model.handle, ctypes.byref(opts), ctypes.byref(h)),
-                  "sapx_solver_create")
+        with nat.on_device(self._device):
+            nat.check(self._lib.sapx_solver_create(self._native_model.handle, ctypes.byref(opts), ctypes.byref(h)),
+                      "sapx_solver_create")
         self._handle = h
         self.dof_count = int(model.joint_dof_count)
         self.num_envs = int(model.world_count)
         self.dof_per_env = self.dof_count // max(self.num_envs, 1)
+        import torch
+
+        # per-world timestep, solver-owned (_update_dt_world, solver_sap.py:975-992)
+        self._dt_world = torch.zeros(max(self.num_envs, 1), dtype=torch.float64, device=self._device)
         self.last_contacts = None
         self.last_contact_solve_result = None
         self.reset_runtime_state()
@@ -314,6 +320,9 @@ class SolverSAP:
         import torch
 
         m = self.model
+        for f in ("joint_q", "joint_qd", "body_q"):
+            if not isinstance(getattr(st, f), torch.Tensor):
+                raise TypeError(f"{name}.{f} must be a torch tensor on {self._device}")
         if st.joint_q.dtype != torch.float32 or st.joint_qd.dtype != torch.float32:
             raise TypeError(f"{name}: joint_q / joint_qd must be float32 tensors")
         if st.body_q.dtype != torch.float32:
@@ -322,13 +331,44 @@ class SolverSAP:
             raise ValueError(f"{name}: joint_q / joint_qd sizes do not match the model")
         if st.body_q.numel() != m.body_count * 7:
             raise ValueError(f"{name}: body_q size does not match the model")
-        for t in (st.joint_q, st.joint_qd, st.body_q, st.body_qd, st.body_f):
-            if t is not None and not t.is_contiguous():
-                raise ValueError(f"{name}: tensors must be contiguous")
+        for f, size in (("body_qd", m.body_count * 6), ("body_f", m.body_count * 6)):
+            t = getattr(st, f)
+            if t is not None and (not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or t.numel() != size):
+                raise TypeError(f"{name}.{f} must be a float32 tensor of {size} entries (or None)")
+        for f in ("joint_q", "joint_qd", "body_q", "body_qd", "body_f"):
+            t = getattr(st, f)
+            if t is not None:
+                nat.require_on(t, self._device, f"{name}.{f}")
+                if not t.is_contiguous():
+                    raise ValueError(f"{name}: tensors must be contiguous")
         if getattr(st, "joint_qd_order", "public") != "public":
             raise NotImplementedError("the native step takes public-order joint_qd (joint_qd_order='public')")
         if st.body_f is not None and getattr(st, "body_f_order", "public") != "public":
             raise NotImplementedError("the native step takes public-order body_f (body_f_order='public')")
+
+    def _check_control(self, control: SapControl) -> None:
+        import torch
+
+        n = self.model.joint_dof_count
+        for f in ("joint_f", "joint_target_pos", "joint_target_vel", "joint_act"):
+            t = getattr(control, f)
+            if t is None:
+                continue
+            if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or t.numel() != n:
+                raise TypeError(f"control.{f} must be a float32 tensor of joint_dof_count={n} entries (or None)")
+            nat.require_on(t, self._device, f"control.{f}")
+            if not t.is_contiguous():
+                raise ValueError(f"control.{f} must be contiguous")
+
+    def _set_dt_world(self, dt):
+        """Per-world dt into the solver's float64 buffer (``_update_dt_world``, solver_sap.py:975-992)."""
+        if dt.numel() != self.num_envs:
+            raise ValueError(f"per-world dt length {dt.numel()} != num_envs {self.num_envs}")
+        nat.require_on(dt, self._device, "dt")
+        if not dt.dtype.is_floating_point:
+            raise TypeError("per-world dt must be a float32 or float64 tensor")
+        self._dt_world.copy_(dt.reshape(-1))
+        return self._dt_world
 
     def step(self, state_in: SapState, state_out: SapState, control: SapControl | None,
              contacts: SapContacts | None, dt) -> SapState:
@@ -345,10 +385,19 @@ class SolverSAP:
             raise TypeError("SolverSAP.step requires SapControl.")
         if contacts is not None and not isinstance(contacts, SapContacts):
             raise TypeError("SolverSAP.step requires SapContacts.")
+        if self.collect_iteration_stats and torch.cuda.is_current_stream_capturing():
+            raise RuntimeError("collect_iteration_stats reads the statistics back to the host after every step, "
+                               "which a HIP graph capture cannot record; construct the solver with "
+                               "collect_iteration_stats=False for captured steps (iteration_histogram() reads "
+                               "the accumulated statistics after a replay)")
         self._check_state(state_in, "state_in")
         self._check_state(state_out, "state_out")
         if control is not None and getattr(control, "joint_f_order", "public") != "public":
             raise NotImplementedError("the native step takes public-order joint_f (joint_f_order='public')")
+        if control is not None:
+            self._check_control(control)
+        if contacts is not None:
+            nat.check_contacts(contacts, self._device, self.model.world_count)
         sin = nat.SapxStateView(nat._ptr(state_in.joint_q), nat._ptr(state_in.joint_qd), nat._ptr(state_in.body_q),
                                 nat._ptr(state_in.body_qd), nat._ptr(state_in.body_f))
         sout = nat.SapxStateView(nat._ptr(state_out.joint_q), nat._ptr(state_out.joint_qd),
@@ -360,18 +409,16 @@ class SolverSAP:
         cview = nat.contacts_view(contacts if contacts is not None else SapContacts())
         dt_world = None
         if isinstance(dt, torch.Tensor):
-            if dt.numel() != self.num_envs:
-                raise ValueError(f"per-world dt length {dt.numel()} != num_envs {self.num_envs}")
-            dt_world = dt.to(torch.float64).contiguous()
+            dt_world = self._set_dt_world(dt)
             dt_scalar = 0.0
         else:
             dt_scalar = float(dt)
             if not dt_scalar > 0.0:
                 raise ValueError("dt must be positive")
-        rc = self._lib.sapx_step(self._handle, ctypes.byref(sin), ctypes.byref(sout), ctypes.byref(ctl),
-                                 ctypes.byref(cview), dt_scalar, nat._ptr(dt_world), nat.stream_ptr())
+        with nat.on_device(self._device):
+            rc = self._lib.sapx_step(self._handle, ctypes.byref(sin), ctypes.byref(sout), ctypes.byref(ctl),
+                                     ctypes.byref(cview), dt_scalar, nat._ptr(dt_world), nat.stream_ptr())
         nat.check(rc, "sapx_step")
-        self._dt_world_keep = dt_world
         self.last_contacts = contacts
         self._has_contact_solve_v_guess = True
         if self.collect_iteration_stats:

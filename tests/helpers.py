@@ -51,5 +51,18 @@ def rel_err(a, b, floor=1.0):
     return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), floor))
 
 
+def world_rel_err(a, b):
+    """max|a - b| / max|b| over one world's entries, with no unit floor (0 when both are empty or zero)."""
+    a = np.asarray(a, dtype=np.float64).reshape(-1)
+    b = np.asarray(b, dtype=np.float64).reshape(-1)
+    if b.size == 0:
+        return 0.0
+    diff = float(np.max(np.abs(a - b)))
+    scale = float(np.max(np.abs(b)))
+    if scale == 0.0:
+        return 0.0 if diff == 0.0 else float("inf")
+    return diff / scale
+
+
 def oracle_step_options(preset="approx32", line_search="armijo_decay", C=64):
     return so.StepOptions(preset=preset, line_search=line_search, max_rigid_contact=C)

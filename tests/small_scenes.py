@@ -14,6 +14,7 @@ from sap_warp_amd.runtime import (
     GEO_BOX,
     GEO_CAPSULE,
     GEO_SPHERE,
+    SAP_JOINT_BALL,
     SAP_JOINT_FREE,
     SAP_JOINT_PRISMATIC,
     SAP_JOINT_REVOLUTE,
@@ -98,7 +99,11 @@ def advance_coords(model, q, v_sap, t):
 
     Free joints move their origin with the linear part and rotate with the
     world-frame angular part (q(t) = exp(t w / 2) q); revolute / prismatic
-    coordinates move linearly.  The derivative at t = 0 is exactly ``v_sap``.
+    coordinates move linearly; a ball joint turns its child frame with the
+    angular velocity w given in the parent's joint frame (r(t) = exp(t w / 2) r,
+    the rate the public FK maps to the world by the parent joint frame's
+    rotation, ``sap_kinematics.py:203-206``).  The derivative at t = 0 is
+    exactly ``v_sap``.
     """
     q = np.asarray(q, dtype=np.float64).copy()
     for j in range(model.joint_count):
@@ -110,6 +115,8 @@ def advance_coords(model, q, v_sap, t):
         elif jt == SAP_JOINT_FREE:
             q[qs:qs + 3] += t * v_sap[ds + 3:ds + 6]
             q[qs + 3:qs + 7] = quat_mul(quat_exp(t * v_sap[ds:ds + 3]), q[qs + 3:qs + 7])
+        elif jt == SAP_JOINT_BALL:
+            q[qs:qs + 4] = quat_mul(quat_exp(t * v_sap[ds:ds + 3]), q[qs:qs + 4])
     return q
 
 

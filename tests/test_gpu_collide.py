@@ -196,3 +196,40 @@ def test_collide_all_pairs_broadphase(gpu_device, scene, worlds, C, mode):
         ra, rb = rows(contacts), rows(contacts_e)
         for w in range(model.world_count):
             assert sorted(ra[ws[w]:ws[w + 1]]) == sorted(rb[we[w]:we[w + 1]])
+
+
+def test_collide_small_contact_buffer(gpu_device):
+    """A contact buffer with fewer slots than worlds x per-world capacity: the list is the oracle's
+    world-grouped list cut at the buffer's capacity, every world offset is clamped into the buffer,
+    and the step reads only inside it."""
+    import torch
+
+    from sap_warp_amd.collision import SapCollisionPipeline
+    from sap_warp_amd.runtime import allocate_contacts
+    from sap_warp_amd.solver_sap import SolverSAP
+
+    model = _scene("humanoid", 8, 3)
+    bq = _perturbed_body_q(model, 11)
+    C = 64
+    ref = oracle_contacts_dict(model, bq, C)
+    cap = max(1, ref["count"] // 2)
+    pipe = SapCollisionPipeline(model, max_contacts_per_world=C)
+    contacts = allocate_contacts(cap, model.world_count, gpu_device)
+    state = model.state()
+    state.body_q.copy_(torch.as_tensor(bq, device=gpu_device))
+    print(f"small buffer: {ref['count']} contacts into {cap} slots", flush=True)
+    pipe.collide(state, contacts)
+    torch.cuda.synchronize()
+    assert int(contacts.rigid_contact_count.cpu()[0]) == cap
+    ws = contacts.rigid_contact_world_start.cpu().numpy()
+    np.testing.assert_array_equal(ws, np.minimum(ref["world_start"], cap))
+    for name, key in (("rigid_contact_shape0", "shape0"), ("rigid_contact_shape1", "shape1")):
+        np.testing.assert_array_equal(getattr(contacts, name).cpu().numpy()[:cap], ref[key][:cap])
+    g = contacts.rigid_contact_point0.cpu().numpy()[:cap]
+    np.testing.assert_array_equal(g.view(np.uint32), ref["point0"][:cap].view(np.uint32))
+    solver = SolverSAP(model, max_rigid_contact=C, line_search_variant="exact_root", collect_iteration_stats=True)
+    s1 = model.state()
+    solver.step(state, s1, model.control(), contacts, 2.0e-3)
+    nc = solver.last_contact_solve_result.contact_count_env
+    np.testing.assert_array_equal(nc, np.diff(np.minimum(ref["world_start"], cap)))
+    assert np.all(np.isfinite(s1.joint_q.cpu().numpy()))
