@@ -64,38 +64,75 @@ def read_traffic():
     return None
 
 
-def cpu_baseline(worlds: int, steps: int, seed: int, dt: float, line_search: str):
-    """Time the float64 oracle (CPU restatement of the reference step) on a bounded sample."""
+def _cpu_model_name() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_rollout(model, C, steps, dt, line_search, threads, walk_seed):
+    """collide + step on the CPU for every world, ``steps`` times; returns (world-steps/s, seconds,
+    mean Newton iterations).  PD targets follow a random walk (sigma 0.05 rad per step)."""
     import numpy as np
 
-    try:
-        from threadpoolctl import threadpool_limits
-    except Exception:  # pragma: no cover
-        threadpool_limits = None
     from oracle import collide as oc
+    from oracle import cpu_step as cs
     from oracle import sap_oracle as so
-    from sap_warp_amd.scenes import build_humanoid_scene
 
-    model = build_humanoid_scene(worlds, seed=seed)
-    opts = so.StepOptions(preset="approx32", line_search=line_search, max_rigid_contact=64)
+    opts = so.StepOptions(preset="approx32", line_search=line_search, max_rigid_contact=C)
+    stepper = cs.CpuStepper(model, opts)
+    n = model.joint_dof_count // model.world_count
+    mask = np.zeros(n, np.float32)
+    mask[6:] = 1.0 if n > 6 else 0.0
+    rng = np.random.default_rng(walk_seed)
     state = dict(joint_q=model.joint_q, joint_qd=model.joint_qd, body_q=model.body_q, body_qd=model.body_qd,
                  body_f=np.zeros((model.body_count, 6), np.float32))
     control = dict(joint_f=model.joint_f, joint_target_pos=model.joint_target_pos.copy(),
                    joint_target_vel=model.joint_target_vel, joint_act=model.joint_act)
-    import contextlib
-
     guess = None
-    with (threadpool_limits(1) if threadpool_limits else contextlib.nullcontext()):
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            c = oc.collide(model, state["body_q"], 64)
-            out = so.step(model, state, control, c, dt, opts, v_guess=guess)
-            guess = out["v_sap"]
-            state = dict(joint_q=out["joint_q"].astype(np.float32), joint_qd=out["joint_qd"].astype(np.float32),
-                         body_q=out["body_q"].astype(np.float32), body_qd=out["body_qd"].astype(np.float32),
-                         body_f=state["body_f"])
-        el = time.perf_counter() - t0
-    return worlds * steps / el, el
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        control["joint_target_pos"] = control["joint_target_pos"] + (
+            rng.standard_normal((model.world_count, n)).astype(np.float32) * 0.05 * mask).reshape(-1)
+        c = oc.collide(model, state["body_q"], C, threads=threads)
+        out = stepper.step(state, control, c, dt, v_guess=guess, threads=threads)
+        guess = out["v_sap"]
+        its.append(float(np.mean(out["iterations"])))
+        state = dict(joint_q=out["joint_q"], joint_qd=out["joint_qd"], body_q=out["body_q"], body_qd=out["body_qd"],
+                     body_f=state["body_f"])
+    el = time.perf_counter() - t0
+    return model.world_count * steps / el, el, float(np.mean(its))
+
+
+def cpu_baseline(worlds: int, steps: int, seed: int, dt: float, line_search: str, threads: int):
+    """The C++17 restatement of the step (oracle/cpu_step.cpp, OpenMP over worlds; collision by the
+    C oracle, threaded over worlds) on the host: a sample of the benchmark workload at ``threads``
+    threads and at 1 thread, and the 64-ant CPU configuration (dt 1e-3, 200 steps) in full."""
+    from sap_warp_amd.scenes import build_ant_scene, build_humanoid_scene
+
+    model = build_humanoid_scene(worlds, seed=seed)
+    v_all, el_all, it_all = _cpu_rollout(model, 64, steps, dt, line_search, threads, seed)
+    v_one, el_one, it_one = _cpu_rollout(model, 64, steps, dt, line_search, 1, seed)
+    ants = build_ant_scene(64, seed=1)
+    a_all, a_el_all, _ = _cpu_rollout(ants, 16, 200, 1.0e-3, "armijo_decay", threads, 1)
+    a_one, a_el_one, _ = _cpu_rollout(ants, 16, 200, 1.0e-3, "armijo_decay", 1, 1)
+    return {
+        "value": round(v_all, 1), "unit": "world-steps/s", "cores": threads, "kind": "port",
+        "value_1_thread": round(v_one, 1),
+        "cpu": _cpu_model_name(),
+        "sample": (f"C++17 CPU restatement of the step (float64, OpenMP over worlds, {threads} threads) + C collide "
+                   f"oracle: {worlds} humanoid worlds of the benchmark workload x {steps} steps (collide + step), "
+                   f"{el_all:.2f} s; 1 thread {el_one:.2f} s; mean Newton iterations {it_all:.2f}"),
+        "ants_64x200": {"value": round(a_all, 1), "value_1_thread": round(a_one, 1), "seconds": round(a_el_all, 3),
+                        "seconds_1_thread": round(a_el_one, 3),
+                        "workload": "64 ant worlds (14 DOF), dt 1e-3, 200 steps, armijo_decay, approx32"},
+    }
 
 
 def _scene_and_solver(W, C, seed, dev, line_search):
@@ -358,8 +395,9 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--line-search", default="exact_root", choices=["exact_root", "armijo_decay"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-worlds", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=24)
+    ap.add_argument("--cpu-worlds", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count())
     ap.add_argument("--learned", type=int, default=1, help="train and time the learned warm start (1) or not (0)")
     ap.add_argument("--graph", type=int, default=1, help="time the plain steps as one HIP graph replay (1) or not (0)")
     ap.add_argument("--autotune", type=int, default=1, help="choose the step's work schedule by timing (1) or keep the default (0)")
@@ -468,10 +506,8 @@ def main() -> int:
         out["hetero"] = hetero_run(args.hetero_worlds, args.hetero_steps, min(args.warmup, 20), args.seed, dt,
                                    args.line_search)
     if not args.no_cpu_baseline:
-        v_cpu, el = cpu_baseline(args.cpu_worlds, args.cpu_steps, args.seed, dt, args.line_search)
-        out["cpu_baseline"] = {"value": round(v_cpu, 3), "unit": "world-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"float64 numpy oracle, first {args.cpu_worlds} humanoid worlds x "
-                                         f"{args.cpu_steps} steps (collide + step), {el:.1f} s"}
+        out["cpu_baseline"] = cpu_baseline(args.cpu_worlds, args.cpu_steps, args.seed, dt, args.line_search,
+                                           args.cpu_threads)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

@@ -235,7 +235,11 @@ int32_t sapx_get_histogram(SapxSolver* solver, int64_t* hist, int32_t reset, voi
  * order), 1 = v_star (f64 dof_count), 2 = v0 (f64 dof_count), 3 = contact
  * impulses gamma (f64 world_count x C x 3), 4 = contact phi0 (f64 world_count
  * x C), 5 = contact weight (f64 world_count x C), 6 = dynamics matrix A
- * diagonal (f64 dof_count).  bytes must equal the buffer size.  Requires a
+ * diagonal (f64 dof_count), 7 = dynamics matrix A = H + diag(armature) as the
+ * Newton solve uses it (f64 world_count x n x n, n = dofs per world), 8 =
+ * contact Jacobian rows t1, t2, n of each slot (f64 world_count x C x 3 x n,
+ * zero past the world's contact count; contact_jacobian.py:51-84
+ * contact_env_jac).  bytes must equal the buffer size.  Requires a
  * solver created with capture_snapshots != 0 (SAPX_ERR_INVALID otherwise).
  */
 int32_t sapx_solver_read(SapxSolver* solver, int32_t which, void* dst, int64_t bytes, void* stream);
@@ -269,7 +273,11 @@ int32_t sapx_pipeline_create_all_pairs(const SapxModel* model, int32_t max_conta
  * (sim/collision/pipeline.py:546-709) — shape AABBs, the candidate pairs' AABB
  * test, the primitive narrowphase and the MPR / GJK path for capsule-box and
  * box-box pairs, written world-grouped in candidate-pair order
- * (deterministic).  body_q: device, body_count x 7.
+ * (deterministic).  body_q: device, body_count x 7.  Two kernel launches
+ * (contacts, then their placement); the pipeline's staging area (world_count x
+ * max_contacts_per_world x 52 bytes) carries the contacts between them, so
+ * calls on one pipeline must not overlap in time (different pipelines may run
+ * concurrently).  Graph-capturable.
  */
 int32_t sapx_collide(SapxPipeline* pipeline, const float* body_q, const SapxContactsView* contacts, void* stream);
 
@@ -330,6 +338,41 @@ typedef struct SapxSchedule {
 int32_t sapx_solver_get_schedule(const SapxSolver* solver, SapxSchedule* out);
 /* SAPX_ERR_UNSUPPORTED when the model does not fit the requested kernel / variant. */
 int32_t sapx_solver_set_schedule(SapxSolver* solver, const SapxSchedule* schedule);
+
+/* ---------------------------------------------------------------------------
+ * Batch step.  No counterpart in the reference (its step needs equally shaped
+ * worlds, sim/contact_jacobian.py:1273-1281): the worlds of up to four solvers
+ * (world kinds of different dof counts, at most 48 dofs per world, each with its
+ * own model) advance in one kernel launch per contact variant, one wave per
+ * world, each group reading its own kernel parameters.  Same per-world result
+ * as sapx_step on each solver (identity world order).
+ * ------------------------------------------------------------------------- */
+typedef struct SapxBatch SapxBatch;
+
+int32_t sapx_batch_create(SapxSolver* const* solvers, int32_t count, SapxBatch** out);
+void sapx_batch_destroy(SapxBatch* batch);
+/*
+ * Contact capacity of the launch over every world (16, 32 or 64); with a solver
+ * capacity above it, a second launch of the 64-slot variant takes the worlds
+ * with more contacts, on a second stream forked from the caller's (fork != 0)
+ * or after the first.  Re-uploads the bound slots (synchronous).
+ */
+int32_t sapx_batch_set_schedule(SapxBatch* batch, int32_t small_capacity, int32_t fork);
+/*
+ * Binds slot 0 or 1 to one set of per-solver views (arrays of `count` entries,
+ * device memory as for sapx_step; contacts world-grouped) and a timestep, and
+ * uploads the kernel parameters (synchronous).  Two slots serve the usual
+ * ping-pong of two states.
+ */
+int32_t sapx_batch_bind(SapxBatch* batch, int32_t slot, const SapxStateView* state_in,
+                        const SapxStateView* state_out, const SapxControlView* control,
+                        const SapxContactsView* contacts, double dt);
+/*
+ * One step of every world of every solver with the views bound to `slot`.
+ * Graph-capturable once every solver has stepped (the first step after a reset
+ * re-uploads the slot's warm-start flags, which a capture refuses).
+ */
+int32_t sapx_batch_step(SapxBatch* batch, int32_t slot, void* stream);
 
 typedef struct SapxMlp SapxMlp;
 

@@ -100,11 +100,12 @@ def all_pairs(model):
             np.ascontiguousarray(np.asarray(pw, dtype=np.int32)))
 
 
-def collide(model, body_q, max_contacts_per_world: int, pairs=None):
+def collide(model, body_q, max_contacts_per_world: int, pairs=None, threads: int = 1):
     """Returns a dict of compact contact arrays in world-major / pair order.
 
     ``pairs``: ``(pairs, pair_world)`` grouped by world (``all_pairs``); default:
-    the model's explicit pairs in world-major order.
+    the model's explicit pairs in world-major order.  ``threads`` > 1 runs the
+    worlds on that many OpenMP threads (same output; the CPU baseline's collide).
     """
     lib = _load()
     pairs, pw = world_major_pairs(model) if pairs is None else pairs
@@ -140,7 +141,17 @@ def collide(model, body_q, max_contacts_per_world: int, pairs=None):
         _ptr(out["margin1"], ctypes.c_float), _ptr(out["pair"], ctypes.c_int),
         scratch_tf.ctypes.data_as(ctypes.c_void_p), _ptr(scratch_aabb, ctypes.c_float),
     ]
-    unsupported = lib.sapx_oracle_collide(*args)
+    if threads > 1:
+        C = max_contacts_per_world
+        wps = np.ascontiguousarray(np.searchsorted(pw, np.arange(W + 1)).astype(i32))
+        stage_i = np.zeros(W * C * 3, i32)
+        stage_f = np.zeros(W * C * 11, f32)
+        wcount = np.zeros(W, i32)
+        args += [_ptr(wps, ctypes.c_int), _ptr(stage_i, ctypes.c_int), _ptr(stage_f, ctypes.c_float),
+                 _ptr(wcount, ctypes.c_int), ctypes.c_int(int(threads))]
+        unsupported = lib.sapx_oracle_collide_mt(*args)
+    else:
+        unsupported = lib.sapx_oracle_collide(*args)
     out["unsupported"] = int(unsupported)
     out["count"] = int(out["count"][0])
     return out

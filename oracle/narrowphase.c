@@ -10,8 +10,9 @@
  *   primitives              sim/collision/primitive.py:37-90 (segment helpers),
  *                           89-129 (plane-sphere, sphere-sphere), 132-166
  *                           (sphere-capsule), 169-264 (capsule-capsule),
- *                           267-319 (plane-capsule), 354-428 (plane-box),
- *                           1150-1205 (sphere-box)
+ *                           267-319 (plane-capsule), 322-351 (plane-ellipsoid),
+ *                           354-428 (plane-box), 431-501 (sphere-cylinder),
+ *                           504-657 (plane-cylinder), 1150-1205 (sphere-box)
  *   gap check + writer      sim/collision/contact_data.py:36-56,
  *                           sim/collision/pipeline.py:51-118
  * Contacts are appended per world in candidate-pair order; a pair whose whole
@@ -28,7 +29,10 @@
 #define GEO_PLANE 1
 #define GEO_SPHERE 3
 #define GEO_CAPSULE 4
+#define GEO_ELLIPSOID 5
+#define GEO_CYLINDER 6
 #define GEO_BOX 7
+#define CYL_FLAT_COS 0.9238795325112867f /* cos(22.5 deg), primitive.py:31-34 */
 #define MAXVAL 1e10f
 #define MINVAL 1e-15f
 
@@ -110,6 +114,19 @@ static void shape_aabb(int type, v3 pos, q4 q, v3 scale, float radius, float eff
         h = mk(fabsf(c[0].x) * scale.x + fabsf(c[1].x) * scale.y + fabsf(c[2].x) * scale.z,
                fabsf(c[0].y) * scale.x + fabsf(c[1].y) * scale.y + fabsf(c[2].y) * scale.z,
                fabsf(c[0].z) * scale.x + fabsf(c[1].z) * scale.y + fabsf(c[2].z) * scale.z);
+    } else if (type == GEO_ELLIPSOID) {
+        /* extent along world axis i: the support value sqrt(sum_k (R_ik s_k)^2) */
+        v3 c[3];
+        rot_cols(q, c);
+        v3 a = scl(c[0], scale.x), b = scl(c[1], scale.y), d = scl(c[2], scale.z);
+        h = mk(sqrtf(a.x * a.x + b.x * b.x + d.x * d.x), sqrtf(a.y * a.y + b.y * b.y + d.y * d.y),
+               sqrtf(a.z * a.z + b.z * b.z + d.z * d.z));
+    } else if (type == GEO_CYLINDER) {
+        /* extent along world axis i: r sqrt(1 - a_i^2) + h |a_i| for the unit axis a */
+        v3 a = qrot(q, mk(0.0f, 0.0f, 1.0f));
+        h = mk(scale.x * sqrtf(fmaxf(1.0f - a.x * a.x, 0.0f)) + scale.y * fabsf(a.x),
+               scale.x * sqrtf(fmaxf(1.0f - a.y * a.y, 0.0f)) + scale.y * fabsf(a.y),
+               scale.x * sqrtf(fmaxf(1.0f - a.z * a.z, 0.0f)) + scale.y * fabsf(a.z));
     } else {
         h = mk(radius, radius, radius);
     }
@@ -192,6 +209,110 @@ static void plane_capsule(v3 n, v3 ppos, v3 cp, v3 axis, float r, float hh, floa
     v3 seg = scl(axis, hh);
     plane_sphere(n, ppos, add(cp, seg), r, &dist[0], &pos[0]);
     plane_sphere(n, ppos, sub(cp, seg), r, &dist[1], &pos[1]);
+}
+
+static void plane_ellipsoid(v3 n, v3 ppos, v3 epos, q4 eq, v3 size, float* dist, v3* pos) {
+    v3 c[3];
+    rot_cols(eq, c);
+    v3 ln = rtmul(c, n);
+    v3 sup = normalize(mk(ln.x * size.x, ln.y * size.y, ln.z * size.z));
+    sup = mk(-sup.x, -sup.y, -sup.z);
+    v3 p = add(epos, rmul(c, mk(sup.x * size.x, sup.y * size.y, sup.z * size.z)));
+    float d = dot(n, sub(p, ppos));
+    *dist = d;
+    *pos = sub(p, scl(n, d * 0.5f));
+}
+
+static void sphere_cylinder(v3 sp, float sr, v3 cp, v3 axis, float cr, float hh, float* dist, v3* pos, v3* n) {
+    v3 vec = sub(sp, cp);
+    float x = dot(vec, axis);
+    v3 a_proj = scl(axis, x);
+    v3 p_proj = sub(vec, a_proj);
+    float p_sqr = dot(p_proj, p_proj);
+    int side = fabsf(x) < hh;
+    int cap = p_sqr < cr * cr;
+    if (side && cap) {
+        float d_cap = hh - fabsf(x);
+        float d_rad = cr - sqrtf(p_sqr);
+        if (d_cap < d_rad) side = 0;
+        else cap = 0;
+    }
+    if (side) {
+        sphere_sphere(sp, sr, add(cp, a_proj), cr, dist, pos, n);
+    } else if (cap) {
+        v3 cpos, pn;
+        if (x > 0.0f) { cpos = add(cp, scl(axis, hh)); pn = axis; }
+        else { cpos = sub(cp, scl(axis, hh)); pn = mk(-axis.x, -axis.y, -axis.z); }
+        plane_sphere(pn, cpos, sp, sr, dist, pos);
+        *n = mk(-pn.x, -pn.y, -pn.z);
+    } else {
+        float l = sqrtf(p_sqr);
+        float inv = 1.0f / (l != 0.0f ? l : 1e-15f);
+        p_proj = scl(p_proj, cr * inv);
+        v3 cap_off = scl(axis, (x < 0.0f ? -1.0f : 1.0f) * hh);
+        sphere_sphere(sp, sr, add(add(cp, cap_off), p_proj), 0.0f, dist, pos, n);
+    }
+}
+
+/* contacts of a cylinder on a plane: the deepest rim point first, then a fixed tripod on the near
+   cap (near upright) or the two side generators and the cap-rim point facing the plane (rolling),
+   each kept when it is farther than 1 % of the larger dimension from the deepest point */
+static void plane_cylinder(v3 n, v3 ppos, v3 cc, v3 axis, float r, float hh, float dist[4], v3 pos[4]) {
+    for (int i = 0; i < 4; ++i) { dist[i] = MAXVAL; pos[i] = mk(0.0f, 0.0f, 0.0f); }
+    float dna = dot(n, axis);
+    if (dna > 0.0f) { axis = mk(-axis.x, -axis.y, -axis.z); dna = -dna; }
+    v3 cap = add(cc, scl(axis, hh));
+    v3 pa = add(mk(-n.x, -n.y, -n.z), scl(axis, dna));
+    float pa_sq = dot(pa, pa);
+    int has_align = pa_sq > 1e-10f;
+    if (has_align) pa = scl(pa, 1.0f / sqrtf(pa_sq));
+    v3 ref = mk(1.0f, 0.0f, 0.0f);
+    if (fabsf(dot(axis, ref)) > 0.9f) ref = mk(0.0f, 1.0f, 0.0f);
+    v3 pf = normalize(sub(ref, scl(axis, dot(axis, ref))));
+    float abs_dot = -dna;
+    int flat = abs_dot >= CYL_FLAT_COS;
+    v3 dperp = has_align ? pa : pf;
+    v3 dpt = add(cap, scl(dperp, r));
+    float dd = dot(sub(dpt, ppos), n);
+    v3 dpos = sub(dpt, scl(n, dd * 0.5f));
+    dist[0] = dd;
+    pos[0] = dpos;
+    int nc = 1;
+    float thr = 0.01f * fmaxf(r, hh);
+    float thr_sq = thr * thr;
+    if (flat) {
+        v3 u = scl(pf, r);
+        v3 v = scl(cross(axis, pf), r);
+        const float c120 = -0.5f, s120 = 0.8660254f;
+        v3 pts[3];
+        pts[0] = add(cap, u);
+        pts[1] = add(add(cap, scl(u, c120)), scl(v, s120));
+        pts[2] = sub(add(cap, scl(u, c120)), scl(v, s120));
+        for (int k = 0; k < 3; ++k) {
+            float d = dot(sub(pts[k], ppos), n);
+            v3 p = sub(pts[k], scl(n, d * 0.5f));
+            if (nc < 4 && dot(sub(p, dpos), sub(p, dpos)) > thr_sq) { dist[nc] = d; pos[nc] = p; ++nc; }
+        }
+    } else {
+        v3 pr = has_align ? pa : pf;
+        v3 u = scl(pr, r);
+        v3 v = scl(cross(axis, pr), r);
+        v3 pts[2];
+        pts[0] = add(add(cc, scl(axis, hh)), u);
+        pts[1] = add(sub(cc, scl(axis, hh)), u);
+        for (int k = 0; k < 2; ++k) {
+            float d = dot(sub(pts[k], ppos), n);
+            v3 p = sub(pts[k], scl(n, d * 0.5f));
+            if (nc < 4 && dot(sub(p, dpos), sub(p, dpos)) > thr_sq) { dist[nc] = d; pos[nc] = p; ++nc; }
+        }
+        v3 ppv = add(cap, v), pnv = sub(cap, v);
+        float dpv = dot(sub(ppv, ppos), n), dnv = dot(sub(pnv, ppos), n);
+        int use_p = dpv <= dnv;
+        v3 pt = use_p ? ppv : pnv;
+        float d = use_p ? dpv : dnv;
+        v3 p = sub(pt, scl(n, d * 0.5f));
+        if (nc < 4 && dot(sub(p, dpos), sub(p, dpos)) > thr_sq) { dist[nc] = d; pos[nc] = p; ++nc; }
+    }
 }
 
 static void plane_box(v3 n, v3 ppos, v3 bpos, q4 bq, v3 size, float margin, float dist[4], v3 pos[4]) {
@@ -329,6 +450,18 @@ static int narrow_pair(int sa, int sb, const int* shape_type, const float* shape
         capsule_capsule(Xa.p, a1, scale_a.x, scale_a.y, Xb.p, a2, scale_b.x, scale_b.y, dist, pos, normal);
     } else if (ta == GEO_SPHERE && tb == GEO_BOX) {
         sphere_box(Xa.p, scale_a.x, Xb.p, Xb.q, scale_b, &dist[0], &pos[0], normal);
+    } else if (ta == GEO_PLANE && tb == GEO_ELLIPSOID) {
+        v3 n = qrot(Xa.q, ez);
+        plane_ellipsoid(n, Xa.p, Xb.p, Xb.q, scale_b, &dist[0], &pos[0]);
+        *normal = n;
+    } else if (ta == GEO_PLANE && tb == GEO_CYLINDER) {
+        v3 n = qrot(Xa.q, ez);
+        v3 ax = qrot(Xb.q, ez);
+        plane_cylinder(n, Xa.p, Xb.p, ax, scale_b.x, scale_b.y, dist, pos);
+        *normal = n;
+    } else if (ta == GEO_SPHERE && tb == GEO_CYLINDER) {
+        v3 ax = qrot(Xb.q, ez);
+        sphere_cylinder(Xa.p, scale_a.x, Xb.p, ax, scale_b.x, scale_b.y, &dist[0], &pos[0], normal);
     } else if ((ta == GEO_CAPSULE || ta == GEO_BOX) && tb == GEO_BOX) {
         /* every contact of a convex pair is kept, whatever its distance (the
            writer's own gap check decides) */
@@ -352,15 +485,80 @@ static int narrow_pair(int sa, int sb, const int* shape_type, const float* shape
  * Outputs are written compact (flat) in world-major order; world_start has
  * world_count+1 entries.  Returns the number of unsupported pair encounters.
  */
-int sapx_oracle_collide(int world_count, int shape_count, const float* body_q, const float* shape_transform,
+/* Contacts of one world's candidate pairs [p_begin, p_end), appended at out[base ...] (at most
+   max_contacts_per_world; writes stop at out_capacity).  Returns the world's contact count. */
+static int collide_world(int p_begin, int p_end, const float* body_q, const int* shape_body, const int* shape_type,
+                         const float* shape_scale, const float* shape_margin, const float* shape_gap, const int* pairs,
+                         int max_contacts_per_world, int base, int out_capacity, int* out_shape0, int* out_shape1,
+                         float* out_point0, float* out_point1, float* out_normal, float* out_margin0,
+                         float* out_margin1, int* out_pair, const tf* scratch_tf, const float* scratch_aabb,
+                         int* unsupported_total) {
+    int wcount = 0, claimed = 0;
+    for (int p = p_begin; p < p_end; ++p) {
+        int s1 = pairs[2 * p], s2 = pairs[2 * p + 1];
+        if (s1 == s2 || s1 < 0 || s2 < 0) continue;
+        const float* A = scratch_aabb + 6 * s1;
+        const float* B = scratch_aabb + 6 * s2;
+        if (!aabb_overlap(mk(A[0], A[1], A[2]), mk(A[3], A[4], A[5]), mk(B[0], B[1], B[2]), mk(B[3], B[4], B[5])))
+            continue;
+        float dist[5];
+        v3 pos[5], normal;
+        float ra, rb;
+        int sa, sb, unsup, convex;
+        float gap_sum = shape_gap[s1] + shape_gap[s2];
+        int nc = narrow_pair(s1, s2, shape_type, shape_scale, scratch_tf, shape_margin, dist, pos, &normal, &ra,
+                             &rb, &sa, &sb, gap_sum, &unsup, &convex);
+        *unsupported_total += unsup;
+        if (nc == 0) continue;
+        float ma = shape_margin[sa], mb = shape_margin[sb];
+        int valid[5], nv = 0;
+        for (int i = 0; i < nc; ++i) {
+            if (convex)  /* the per-contact writer drops d > gap (narrow_phase.py:580-747, pipeline.py:80-88) */
+                valid[i] = !passes_gap_not(pos[i], normal, dist[i], ra, rb, ma, mb, gap_sum);
+            else
+                valid[i] = dist[i] < MAXVAL && passes_gap(pos[i], normal, dist[i], ra, rb, ma, mb, gap_sum);
+            nv += valid[i];
+        }
+        if (nv == 0) continue;
+        /* primitive pairs: batch append with the reference's overflow rule
+           (the claim always advances, a batch is written only if it fits
+           entirely); convex pairs: every contact claims a slot and is
+           written if that slot is inside the capacity */
+        int first = claimed;
+        int fits = claimed + nv <= max_contacts_per_world;
+        claimed += nv;
+        if (!fits && !convex) continue;
+        int ba = shape_body[sa], bb = shape_body[sb];
+        for (int i = 0, k = 0; i < nc; ++i) {
+            if (!valid[i]) continue;
+            if (first + k++ >= max_contacts_per_world) break;
+            int idx = base + wcount;
+            if (idx < out_capacity) {
+                v3 xa, xb, n;
+                contact_points(pos[i], normal, dist[i], ra, rb, &xa, &xb, &n);
+                v3 la = xa, lb = xb;
+                if (ba >= 0) la = tf_inv_point(load_tf(body_q + 7 * ba), xa);
+                if (bb >= 0) lb = tf_inv_point(load_tf(body_q + 7 * bb), xb);
+                out_shape0[idx] = sa;
+                out_shape1[idx] = sb;
+                out_point0[3 * idx] = la.x; out_point0[3 * idx + 1] = la.y; out_point0[3 * idx + 2] = la.z;
+                out_point1[3 * idx] = lb.x; out_point1[3 * idx + 1] = lb.y; out_point1[3 * idx + 2] = lb.z;
+                out_normal[3 * idx] = n.x; out_normal[3 * idx + 1] = n.y; out_normal[3 * idx + 2] = n.z;
+                out_margin0[idx] = ra + ma;
+                out_margin1[idx] = rb + mb;
+                out_pair[idx] = p;
+            }
+            wcount++;
+        }
+    }
+    return wcount;
+}
+
+static void shape_poses(int s_begin, int s_end, const float* body_q, const float* shape_transform,
                         const int* shape_body, const int* shape_type, const float* shape_scale,
-                        const float* shape_margin, const float* shape_gap, const float* shape_radius,
-                        const int* pairs, const int* pair_world, int pair_count, int max_contacts_per_world,
-                        int out_capacity, int* out_count, int* world_start, int* out_shape0, int* out_shape1,
-                        float* out_point0, float* out_point1, float* out_normal, float* out_margin0,
-                        float* out_margin1, int* out_pair, tf* scratch_tf, float* scratch_aabb) {
-    int unsupported_total = 0;
-    for (int s = 0; s < shape_count; ++s) {
+                        const float* shape_margin, const float* shape_gap, const float* shape_radius, tf* scratch_tf,
+                        float* scratch_aabb) {
+    for (int s = s_begin; s < s_end; ++s) {
         tf xs = load_tf(shape_transform + 7 * s);
         int b = shape_body[s];
         if (b >= 0) xs = tf_mul(load_tf(body_q + 7 * b), xs);
@@ -371,73 +569,92 @@ int sapx_oracle_collide(int world_count, int shape_count, const float* body_q, c
         scratch_aabb[6 * s + 0] = lo.x; scratch_aabb[6 * s + 1] = lo.y; scratch_aabb[6 * s + 2] = lo.z;
         scratch_aabb[6 * s + 3] = hi.x; scratch_aabb[6 * s + 4] = hi.y; scratch_aabb[6 * s + 5] = hi.z;
     }
+}
+
+int sapx_oracle_collide(int world_count, int shape_count, const float* body_q, const float* shape_transform,
+                        const int* shape_body, const int* shape_type, const float* shape_scale,
+                        const float* shape_margin, const float* shape_gap, const float* shape_radius,
+                        const int* pairs, const int* pair_world, int pair_count, int max_contacts_per_world,
+                        int out_capacity, int* out_count, int* world_start, int* out_shape0, int* out_shape1,
+                        float* out_point0, float* out_point1, float* out_normal, float* out_margin0,
+                        float* out_margin1, int* out_pair, tf* scratch_tf, float* scratch_aabb) {
+    int unsupported_total = 0;
+    shape_poses(0, shape_count, body_q, shape_transform, shape_body, shape_type, shape_scale, shape_margin, shape_gap,
+                shape_radius, scratch_tf, scratch_aabb);
     int total = 0;
     int p = 0;
     for (int w = 0; w < world_count; ++w) {
         world_start[w] = total;
-        int wcount = 0, claimed = 0;
-        for (; p < pair_count && pair_world[p] == w; ++p) {
-            int s1 = pairs[2 * p], s2 = pairs[2 * p + 1];
-            if (s1 == s2 || s1 < 0 || s2 < 0) continue;
-            const float* A = scratch_aabb + 6 * s1;
-            const float* B = scratch_aabb + 6 * s2;
-            if (!aabb_overlap(mk(A[0], A[1], A[2]), mk(A[3], A[4], A[5]), mk(B[0], B[1], B[2]), mk(B[3], B[4], B[5])))
-                continue;
-            float dist[5];
-            v3 pos[5], normal;
-            float ra, rb;
-            int sa, sb, unsup, convex;
-            float gap_sum = shape_gap[s1] + shape_gap[s2];
-            int nc = narrow_pair(s1, s2, shape_type, shape_scale, scratch_tf, shape_margin, dist, pos, &normal, &ra,
-                                 &rb, &sa, &sb, gap_sum, &unsup, &convex);
-            unsupported_total += unsup;
-            if (nc == 0) continue;
-            float ma = shape_margin[sa], mb = shape_margin[sb];
-            int valid[5], nv = 0;
-            for (int i = 0; i < nc; ++i) {
-                if (convex)  /* the per-contact writer drops d > gap (narrow_phase.py:580-747, pipeline.py:80-88) */
-                    valid[i] = !passes_gap_not(pos[i], normal, dist[i], ra, rb, ma, mb, gap_sum);
-                else
-                    valid[i] = dist[i] < MAXVAL && passes_gap(pos[i], normal, dist[i], ra, rb, ma, mb, gap_sum);
-                nv += valid[i];
-            }
-            if (nv == 0) continue;
-            /* primitive pairs: batch append with the reference's overflow rule
-               (the claim always advances, a batch is written only if it fits
-               entirely); convex pairs: every contact claims a slot and is
-               written if that slot is inside the capacity */
-            int first = claimed;
-            int fits = claimed + nv <= max_contacts_per_world;
-            claimed += nv;
-            if (!fits && !convex) continue;
-            int ba = shape_body[sa], bb = shape_body[sb];
-            for (int i = 0, k = 0; i < nc; ++i) {
-                if (!valid[i]) continue;
-                if (first + k++ >= max_contacts_per_world) break;
-                int idx = total + wcount;
-                if (idx < out_capacity) {
-                    v3 xa, xb, n;
-                    contact_points(pos[i], normal, dist[i], ra, rb, &xa, &xb, &n);
-                    v3 la = xa, lb = xb;
-                    if (ba >= 0) la = tf_inv_point(load_tf(body_q + 7 * ba), xa);
-                    if (bb >= 0) lb = tf_inv_point(load_tf(body_q + 7 * bb), xb);
-                    out_shape0[idx] = sa;
-                    out_shape1[idx] = sb;
-                    out_point0[3 * idx] = la.x; out_point0[3 * idx + 1] = la.y; out_point0[3 * idx + 2] = la.z;
-                    out_point1[3 * idx] = lb.x; out_point1[3 * idx + 1] = lb.y; out_point1[3 * idx + 2] = lb.z;
-                    out_normal[3 * idx] = n.x; out_normal[3 * idx + 1] = n.y; out_normal[3 * idx + 2] = n.z;
-                    out_margin0[idx] = ra + ma;
-                    out_margin1[idx] = rb + mb;
-                    out_pair[idx] = p;
-                }
-                wcount++;
-            }
-        }
-        total += wcount;
+        int pe = p;
+        while (pe < pair_count && pair_world[pe] == w) ++pe;
+        total += collide_world(p, pe, body_q, shape_body, shape_type, shape_scale, shape_margin, shape_gap, pairs,
+                               max_contacts_per_world, total, out_capacity, out_shape0, out_shape1, out_point0,
+                               out_point1, out_normal, out_margin0, out_margin1, out_pair, scratch_tf, scratch_aabb,
+                               &unsupported_total);
+        p = pe;
     }
     world_start[world_count] = total;
     *out_count = total;
     (void)tf_inv_vector;
+    return unsupported_total;
+}
+
+/*
+ * The same contacts with worlds on `threads` OpenMP threads: each world is generated into its own
+ * block of max_contacts_per_world staging slots (stage_* arrays, world_count x C), then the blocks
+ * are compacted in world order.  Identical output to sapx_oracle_collide (CPU baseline only).
+ */
+int sapx_oracle_collide_mt(int world_count, int shape_count, const float* body_q, const float* shape_transform,
+                           const int* shape_body, const int* shape_type, const float* shape_scale,
+                           const float* shape_margin, const float* shape_gap, const float* shape_radius,
+                           const int* pairs, const int* pair_world, int pair_count, int max_contacts_per_world,
+                           int out_capacity, int* out_count, int* world_start, int* out_shape0, int* out_shape1,
+                           float* out_point0, float* out_point1, float* out_normal, float* out_margin0,
+                           float* out_margin1, int* out_pair, tf* scratch_tf, float* scratch_aabb,
+                           const int* world_pair_start, int* stage_i, float* stage_f, int* world_count_out,
+                           int threads) {
+    const int C = max_contacts_per_world;
+    int unsupported_total = 0;
+#pragma omp parallel for schedule(static) num_threads(threads)
+    for (int chunk = 0; chunk < threads; ++chunk) {
+        int s0 = (int)((long long)shape_count * chunk / threads), s1 = (int)((long long)shape_count * (chunk + 1) / threads);
+        shape_poses(s0, s1, body_q, shape_transform, shape_body, shape_type, shape_scale, shape_margin, shape_gap,
+                    shape_radius, scratch_tf, scratch_aabb);
+    }
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads) reduction(+ : unsupported_total)
+    for (int w = 0; w < world_count; ++w) {
+        int* si = stage_i + (size_t)w * C * 3;
+        float* sf = stage_f + (size_t)w * C * 11;
+        int unsup = 0;
+        world_count_out[w] = collide_world(world_pair_start[w], world_pair_start[w + 1], body_q, shape_body,
+                                           shape_type, shape_scale, shape_margin, shape_gap, pairs, C, 0, C, si,
+                                           si + C, sf, sf + 3 * C, sf + 6 * C, sf + 9 * C, sf + 10 * C, si + 2 * C,
+                                           scratch_tf, scratch_aabb, &unsup);
+        unsupported_total += unsup;
+    }
+    int total = 0;
+    for (int w = 0; w < world_count; ++w) {
+        world_start[w] = total;
+        const int* si = stage_i + (size_t)w * C * 3;
+        const float* sf = stage_f + (size_t)w * C * 11;
+        for (int k = 0; k < world_count_out[w]; ++k) {
+            int idx = total + k;
+            if (idx >= out_capacity) break;
+            out_shape0[idx] = si[k];
+            out_shape1[idx] = si[C + k];
+            out_pair[idx] = si[2 * C + k];
+            for (int e = 0; e < 3; ++e) {
+                out_point0[3 * idx + e] = sf[3 * k + e];
+                out_point1[3 * idx + e] = sf[3 * C + 3 * k + e];
+                out_normal[3 * idx + e] = sf[6 * C + 3 * k + e];
+            }
+            out_margin0[idx] = sf[9 * C + k];
+            out_margin1[idx] = sf[10 * C + k];
+        }
+        total += world_count_out[w];
+    }
+    world_start[world_count] = total;
+    *out_count = total;
     return unsupported_total;
 }
 

@@ -28,7 +28,7 @@ Everything operates on one world at a time; the model is the global
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -1105,6 +1105,50 @@ def optimality_at(model, w, world_result: dict, v, dt, opts: SolveOptions | None
     return gn, tol, ev["cost"], ev["gamma"]
 
 
+def optimality_on_data(model, w, world_result: dict, v, dt, opts: SolveOptions | None = None, *, A, v_star, J,
+                       weight, phi0):
+    """The SAP optimality test (contact_solve.py:1970-2087) in float64 at velocity ``v`` for world
+    ``w``'s problem rebuilt from data formed elsewhere: a solver's dynamics matrix ``A`` (n x n),
+    free-motion velocity ``v_star``, contact Jacobian rows ``J`` (nc x 3 x n), contact weights and
+    signed distances, in place of the oracle's own (PD / limit terms are rebuilt from that A and v*,
+    as build_model_terms does).  Checks that ``v`` is the optimum of the problem that solver
+    formed, independently of how closely its float32 data rounds the oracle's.
+    Returns (||D grad||, tolerance)."""
+    opts = opts or SolveOptions()
+    wv = WorldView(model, w)
+    cs = world_result["contacts"]
+    nc = cs.count
+    A = np.asarray(A, dtype=np.float64)
+    v_star = np.asarray(v_star, dtype=np.float64)
+    cs2 = replace(cs, J=np.asarray(J, dtype=np.float64)[:nc].reshape(nc, 3, wv.n),
+                  w=np.asarray(weight, dtype=np.float64)[:nc], phi0=np.asarray(phi0, dtype=np.float64)[:nc])
+    mt = build_model_terms(model, wv, world_result["joint_q_in"], world_result["control_in"], A, world_result["v0"],
+                           v_star, dt)
+    prob = SapProblem(A, v_star, cs2, mt, dt, opts.beta, opts.sigma)
+    part = _participating(model, wv, cs2, mt)
+    a_diag = np.array([d if (d >= 1.0e-12 and math.isfinite(d)) else 1.0e-12 for d in np.diag(A)])
+    ev = prob.evaluate(np.asarray(v, dtype=np.float64))
+    return _optimality_norms(ev, part, 1.0 / np.sqrt(a_diag), opts)
+
+
+def solve_on_data(model, w, world_result: dict, dt, opts: SolveOptions | None = None, *, A, v_star, J, weight, phi0,
+                  v_guess=None) -> SolveResult:
+    """SAP solve of world ``w``'s problem rebuilt from data formed elsewhere (as ``optimality_on_data``
+    rebuilds it): the oracle's float64 answer to the problem a solver formed, so that solver's
+    Newton, projection and line search can be checked apart from how its data rounds."""
+    opts = opts or SolveOptions()
+    wv = WorldView(model, w)
+    cs = world_result["contacts"]
+    nc = cs.count
+    A = np.asarray(A, dtype=np.float64)
+    v_star = np.asarray(v_star, dtype=np.float64)
+    cs2 = replace(cs, J=np.asarray(J, dtype=np.float64)[:nc].reshape(nc, 3, wv.n),
+                  w=np.asarray(weight, dtype=np.float64)[:nc], phi0=np.asarray(phi0, dtype=np.float64)[:nc])
+    mt = build_model_terms(model, wv, world_result["joint_q_in"], world_result["control_in"], A, world_result["v0"],
+                           v_star, dt)
+    return sap_solve(model, wv, A, v_star, world_result["v0"], v_guess, cs2, mt, dt, opts)
+
+
 def sap_solve(model, wv: WorldView, A, v_star, v0, v_guess, cs: ContactSet, mt: ModelTerms, dt,
               opts: SolveOptions) -> SolveResult:
     """SAP Newton solve of one world (contact_solve.py:6129-6285, 5474-5655)."""
@@ -1529,7 +1573,7 @@ def step_world(model, w, state: dict, control: dict, contacts: dict, dt, opts: S
     q_out, qd_out = integrate(model, wv, jq, v0, res.v, dt, mode=integration)
     body_q, body_qd = public_fk(model, wv, q_out, qd_out)
     return dict(joint_q=q_out, joint_qd=qd_out, body_q=body_q, body_qd=body_qd, v_sap=res.v, v_star=fm.v_star,
-                v0=v0, result=res, contacts=cs, free_motion=fm, model_terms=mt)
+                v0=v0, result=res, contacts=cs, free_motion=fm, model_terms=mt, joint_q_in=jq, control_in=control)
 
 
 def step(model, state: dict, control: dict, contacts: dict, dt, opts: StepOptions, v_guess=None, worlds=None):

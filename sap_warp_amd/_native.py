@@ -38,6 +38,8 @@ READ_GAMMA = 3
 READ_PHI0 = 4
 READ_WEIGHT = 5
 READ_A_DIAG = 6
+READ_A = 7
+READ_J = 8
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int32
@@ -120,6 +122,12 @@ EXPORTS = {
     "sapx_solver_set_warm_start_io": (_I, [_P, _P, _P]),
     "sapx_solver_get_schedule": (_I, [_P, ctypes.POINTER(SapxSchedule)]),
     "sapx_solver_set_schedule": (_I, [_P, ctypes.POINTER(SapxSchedule)]),
+    "sapx_batch_create": (_I, [_P, _I, ctypes.POINTER(_P)]),
+    "sapx_batch_destroy": (None, [_P]),
+    "sapx_batch_set_schedule": (_I, [_P, _I, _I]),
+    "sapx_batch_bind": (_I, [_P, _I, ctypes.POINTER(SapxStateView), ctypes.POINTER(SapxStateView),
+                             ctypes.POINTER(SapxControlView), ctypes.POINTER(SapxContactsView), _D]),
+    "sapx_batch_step": (_I, [_P, _I, _P]),
     "sapx_mlp_create": (_I, [_I, _P, _I, _I, ctypes.POINTER(_P)]),
     "sapx_mlp_destroy": (None, [_P]),
     "sapx_mlp_param_count": (ctypes.c_int64, [_P]),

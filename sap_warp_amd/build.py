@@ -37,6 +37,9 @@ UNITS = {
     "sapx_wave32.hip": [],
     "sapx_wave48.hip": [],
     "sapx_wave64.hip": [],
+    "sapx_batch16.hip": [],
+    "sapx_batch32.hip": [],
+    "sapx_batch64.hip": [],
     "sapx_order.hip": [],
     "sapx_learned.hip": [],
     "sapx_mlp.hip": [],

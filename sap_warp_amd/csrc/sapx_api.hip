@@ -21,7 +21,11 @@ hipError_t launch_wave_np16(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_wave_np32(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_wave_np48(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_wave_np64(const StepParams& P, int cb, hipStream_t stream);
-hipError_t launch_collide(const CollideParams& P, const CompactParams& Q, hipStream_t stream);
+hipError_t launch_collide(const CollideParams& P, const PlaceParams& Q, hipStream_t stream);
+hipError_t launch_batch_16(const BatchMap& B, int blocks, size_t smem, hipStream_t stream);
+hipError_t launch_batch_32(const BatchMap& B, int blocks, size_t smem, hipStream_t stream);
+hipError_t launch_batch_64(const BatchMap& B, int blocks, size_t smem, hipStream_t stream);
+size_t collide_lds_bytes(const DevModel& m, int C);
 hipError_t launch_world_order(const int* cworld_start, const int* iters, int W, int C, int* order, hipStream_t stream);
 hipError_t launch_features(const FeatureParams& P, hipStream_t stream);
 hipError_t launch_group_contacts(const GroupParams& G, hipStream_t stream);
@@ -89,8 +93,8 @@ struct SapxSolver {
     int* st = nullptr;  // 4 x W
     unsigned long long* hist = nullptr;  // kHistSize accumulators
     double* dbg = nullptr;
-    size_t dbg_off[7] = {0};
-    size_t dbg_size[7] = {0};
+    size_t dbg_off[9] = {0};
+    size_t dbg_size[9] = {0};
     int npad = 0, js = 0, ast = 0;
     int lds[17] = {0};
     int smem_bytes = 0;
@@ -141,23 +145,61 @@ struct SapxSolver {
 struct SapxPipeline {
     const SapxModel* model = nullptr;
     int C = 0;
-    int* st_shape0 = nullptr;
-    int* st_shape1 = nullptr;
-    float* st_p0 = nullptr;
-    float* st_p1 = nullptr;
-    float* st_n = nullptr;
-    float* st_m0 = nullptr;
-    float* st_m1 = nullptr;
-    int* wcount = nullptr;
+    int* st_i = nullptr;  // W x 2C staged contact shape ids
+    float* st_f = nullptr;  // W x 11C staged contact margins and points
+    int* wcount = nullptr;  // W staged counts
     int* unsupported = nullptr;
     // all-pairs broadphase: the pipeline's own candidate template (P < 0: the model's explicit pairs)
     int* t_pairs = nullptr;
     int P = -1;
     int has_convex = 0;
     ~SapxPipeline() {
-        void* ptrs[] = {st_shape0, st_shape1, st_p0, st_p1, st_n, st_m0, st_m1, wcount, unsupported, t_pairs};
+        void* ptrs[] = {st_i, st_f, wcount, unsupported, t_pairs};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
+    }
+};
+
+// shape-type pairs (sorted, ta <= tb) the collide kernel handles: the analytic primitives of
+// narrow_phase.py:304-480 and the MPR / GJK path for capsule-box and box-box
+static bool pair_supported(int ta, int tb) {
+    switch (ta) {
+        case kGeoPlane:
+            return tb == kGeoSphere || tb == kGeoCapsule || tb == kGeoBox || tb == kGeoEllipsoid || tb == kGeoCylinder;
+        case kGeoSphere:
+            return tb == kGeoSphere || tb == kGeoCapsule || tb == kGeoBox || tb == kGeoCylinder;
+        case kGeoCapsule:
+            return tb == kGeoCapsule || tb == kGeoBox;
+        case kGeoBox:
+            return tb == kGeoBox;
+        default:
+            return false;
+    }
+}
+
+// Several solvers (world kinds) stepped by one launch per contact variant.  Each bound slot holds
+// the kernel parameters of every group for one set of state / control / contact views.
+struct SapxBatch {
+    std::vector<SapxSolver*> s;
+    StepParams* dparams = nullptr;  // device: [slot 0..1][variant small, 64][group]
+    std::vector<StepParams> hparams;
+    struct Views {
+        std::vector<SapxStateView> in, out;
+        std::vector<SapxControlView> ctl;
+        std::vector<SapxContactsView> c;
+        double dt = 0.0;
+        int bound = 0;
+        std::vector<int> use_guess;  // each solver's warm-start flag when the slot was uploaded
+    } views[2];
+    int small = 16;
+    int fork = 0;
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    ~SapxBatch() {
+        if (dparams) (void)hipFree(dparams);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (aux) (void)hipStreamDestroy(aux);
     }
 };
 
@@ -372,7 +414,8 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
         if (d->shape_world[s] != s / S) return fail(SAPX_ERR_UNSUPPORTED, "shapes must be grouped by world");
     for (int s = 0; s < Stot; ++s) {
         const int t = d->shape_type[s];
-        if (t != kGeoPlane && t != kGeoSphere && t != kGeoCapsule && t != kGeoBox)
+        if (t != kGeoPlane && t != kGeoSphere && t != kGeoCapsule && t != kGeoBox && t != kGeoEllipsoid &&
+            t != kGeoCylinder)
             return fail(SAPX_ERR_UNSUPPORTED, "shape type " + std::to_string(t) + " is not in the native narrowphase");
     }
     // pair template: pairs grouped by world (stable), encoded local / global
@@ -413,6 +456,11 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
         const int ea = per_world[0][k], eb = per_world[0][k + 1];
         int ta = d->shape_type[ea >= 0 ? ea : -ea - 1], tb = d->shape_type[eb >= 0 ? eb : -eb - 1];
         if (ta > tb) std::swap(ta, tb);
+        if (!pair_supported(ta, tb)) {
+            delete m;
+            return fail(SAPX_ERR_UNSUPPORTED, "candidate pair of shape types " + std::to_string(ta) + " and " +
+                                                  std::to_string(tb) + " is not in the native narrowphase");
+        }
         if ((ta == kGeoCapsule || ta == kGeoBox) && tb == kGeoBox) dm.has_convex = 1;
     }
     int32_t rc = SAPX_OK;
@@ -739,9 +787,10 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
             return fail(SAPX_ERR_HIP, "hipMalloc failed for the feature layout");
         }
     }
-    const size_t sizes[7] = {D, D, D, W * C * 3, W * C, W * C, D};
+    const size_t nd = (size_t)dm.N;
+    const size_t sizes[9] = {D, D, D, W * C * 3, W * C, W * C, D, W * nd * nd, W * C * 3 * nd};
     size_t tot = 0;
-    for (int i = 0; i < 7; ++i) {
+    for (int i = 0; i < 9; ++i) {
         s->dbg_off[i] = tot;
         s->dbg_size[i] = sizes[i] * sizeof(double);
         tot += sizes[i];
@@ -797,15 +846,17 @@ static int32_t ensure_group_buffers(SapxSolver* s, int capacity) {
     return SAPX_OK;
 }
 
-int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* out, const SapxControlView* ctl,
-                  const SapxContactsView* c, double dt, const double* dt_world, void* stream) {
+// Kernel parameters of one step of solver s (all but the wave layout and the world order); an
+// ungrouped contact list is grouped by world first (launches on st).
+static int32_t fill_step_params(SapxSolver* s, const SapxStateView* in, const SapxStateView* out,
+                                const SapxControlView* ctl, const SapxContactsView* c, double dt,
+                                const double* dt_world, hipStream_t st, StepParams* out_params) {
     if (!s || !in || !out) return fail(SAPX_ERR_INVALID, "null argument");
     if (!in->joint_q || !in->joint_qd || !in->body_q || !out->joint_q || !out->joint_qd || !out->body_q)
         return fail(SAPX_ERR_INVALID, "state views need joint_q, joint_qd and body_q");
     if (!(dt > 0.0) && !dt_world) return fail(SAPX_ERR_INVALID, "dt must be positive");
     const DevModel& dm = s->model->dm;
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    StepParams P;
+    StepParams& P = *out_params;
     std::memset(&P, 0, sizeof(P));
     P.m = dm;
     P.jq_in = in->joint_q;
@@ -869,6 +920,8 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
         P.dbg_phi0 = s->dbg + s->dbg_off[4];
         P.dbg_w = s->dbg + s->dbg_off[5];
         P.dbg_adiag = s->dbg + s->dbg_off[6];
+        P.dbg_A = s->dbg + s->dbg_off[7];
+        P.dbg_J = s->dbg + s->dbg_off[8];
     }
     P.dt = dt;
     P.dt_world = dt_world;
@@ -902,6 +955,18 @@ int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* o
     P.o_bqd = s->lds[15];
     P.o_topo = s->lds[16];
     P.smem_bytes = s->smem_bytes;
+    return SAPX_OK;
+}
+
+int32_t sapx_step(SapxSolver* s, const SapxStateView* in, const SapxStateView* out, const SapxControlView* ctl,
+                  const SapxContactsView* c, double dt, const double* dt_world, void* stream) {
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    StepParams P;
+    {
+        const int32_t rc = fill_step_params(s, in, out, ctl, c, dt, dt_world, st, &P);
+        if (rc != SAPX_OK) return rc;
+    }
+    const DevModel& dm = s->model->dm;
     if (s->sched.kernel == SAPX_KERNEL_WAVE) {
         auto launch_on = [&](const StepParams& Q, int cb, hipStream_t sq) {
             switch (s->npad) {
@@ -978,7 +1043,7 @@ int32_t sapx_get_histogram(SapxSolver* s, int64_t* hist, int32_t reset, void* st
 
 int32_t sapx_solver_read(SapxSolver* s, int32_t which, void* dst, int64_t bytes, void* stream) {
     if (!s || !dst) return fail(SAPX_ERR_INVALID, "null argument");
-    if (which < 0 || which > 6) return fail(SAPX_ERR_INVALID, "unknown buffer id");
+    if (which < 0 || which > 8) return fail(SAPX_ERR_INVALID, "unknown buffer id");
     if (!s->dbg) return fail(SAPX_ERR_INVALID, "solver was created without capture_snapshots");
     if ((size_t)bytes != s->dbg_size[which]) return fail(SAPX_ERR_INVALID, "byte count does not match the buffer");
     HIP_TRY(hipMemcpyAsync(dst, s->dbg + s->dbg_off[which], (size_t)bytes, hipMemcpyDeviceToDevice,
@@ -993,17 +1058,20 @@ int32_t sapx_pipeline_create(const SapxModel* model, int32_t max_contacts_per_wo
     SapxPipeline* p = new SapxPipeline();
     p->model = model;
     p->C = max_contacts_per_world;
-    const size_t n = (size_t)model->dm.W * p->C;
-    bool ok = hipMalloc(&p->st_shape0, n * 4) == hipSuccess && hipMalloc(&p->st_shape1, n * 4) == hipSuccess &&
-              hipMalloc(&p->st_p0, n * 12) == hipSuccess && hipMalloc(&p->st_p1, n * 12) == hipSuccess &&
-              hipMalloc(&p->st_n, n * 12) == hipSuccess && hipMalloc(&p->st_m0, n * 4) == hipSuccess &&
-              hipMalloc(&p->st_m1, n * 4) == hipSuccess && hipMalloc(&p->wcount, (size_t)model->dm.W * 4) == hipSuccess &&
-              hipMalloc(&p->unsupported, 4) == hipSuccess;
+    if (collide_lds_bytes(model->dm, p->C) > kCollideMaxLds) {
+        delete p;
+        return fail(SAPX_ERR_CAPACITY, "max_contacts_per_world too large for the collide kernel's LDS staging");
+    }
+    const size_t slots = (size_t)model->dm.W * p->C;
+    bool ok = hipMalloc(&p->st_i, slots * 2 * sizeof(int)) == hipSuccess &&
+              hipMalloc(&p->st_f, slots * 11 * sizeof(float)) == hipSuccess &&
+              hipMalloc(&p->wcount, (size_t)model->dm.W * sizeof(int)) == hipSuccess &&
+              hipMalloc(&p->unsupported, sizeof(int)) == hipSuccess &&
+              hipMemset(p->unsupported, 0, sizeof(int)) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
     if (!ok) {
         delete p;
         return fail(SAPX_ERR_HIP, "hipMalloc failed for pipeline buffers");
     }
-    (void)hipMemset(p->unsupported, 0, 4);
     *out = p;
     return SAPX_OK;
 }
@@ -1068,14 +1136,22 @@ int32_t sapx_pipeline_create_all_pairs(const SapxModel* model, int32_t max_conta
         int ta = st[ea >= 0 ? ea : -ea - 1], tb = st[eb >= 0 ? eb : -eb - 1];
         if (ta > tb) std::swap(ta, tb);
         if ((ta == kGeoCapsule || ta == kGeoBox) && tb == kGeoBox) has_convex = 1;
-        if (ta == kGeoPlane && tb == kGeoPlane)
-            return fail(SAPX_ERR_UNSUPPORTED, "plane-plane candidate pairs are not in the native narrowphase");
+        if (!pair_supported(ta, tb))
+            return fail(SAPX_ERR_UNSUPPORTED, "candidate pair of shape types " + std::to_string(ta) + " and " +
+                                                  std::to_string(tb) + " is not in the native narrowphase");
     }
     int32_t rc = sapx_pipeline_create(model, max_contacts_per_world, out);
     if (rc != SAPX_OK) return rc;
     SapxPipeline* p = *out;
     p->P = (int)tmpl.size() / 2;
     p->has_convex = has_convex;
+    DevModel dm_all = dm;
+    dm_all.has_convex = has_convex;
+    if (collide_lds_bytes(dm_all, p->C) > kCollideMaxLds) {
+        delete p;
+        *out = nullptr;
+        return fail(SAPX_ERR_CAPACITY, "max_contacts_per_world too large for the collide kernel's LDS staging");
+    }
     if (!tmpl.empty()) {
         if (hipMalloc(&p->t_pairs, tmpl.size() * sizeof(int)) != hipSuccess ||
             hipMemcpy(p->t_pairs, tmpl.data(), tmpl.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
@@ -1087,32 +1163,200 @@ int32_t sapx_pipeline_create_all_pairs(const SapxModel* model, int32_t max_conta
     return SAPX_OK;
 }
 
-int32_t sapx_collide(SapxPipeline* p, const float* body_q, const SapxContactsView* c, void* stream) {
+static int32_t collide_params(const SapxPipeline* p, const float* body_q, const SapxContactsView* c,
+                              CollideParams* P, PlaceParams* Q) {
     if (!p || !body_q || !c) return fail(SAPX_ERR_INVALID, "null argument");
     if (!c->count || !c->world_start || !c->shape0 || !c->shape1 || !c->point0 || !c->point1 || !c->normal ||
         !c->margin0 || !c->margin1)
         return fail(SAPX_ERR_INVALID, "contact view is missing arrays");
+    if (c->capacity < 0) return fail(SAPX_ERR_INVALID, "negative contact capacity");
     const DevModel& dm = p->model->dm;
+    std::memset(P, 0, sizeof(*P));
+    std::memset(Q, 0, sizeof(*Q));
+    P->m = dm;
+    P->body_q = body_q;
+    P->C = p->C;
+    if (p->P >= 0) {
+        P->m.t_pairs = p->t_pairs;
+        P->m.P = p->P;
+        P->m.has_convex = p->has_convex;
+    }
+    Q->W = dm.W;
+    Q->C = p->C;
+    Q->capacity = c->capacity;
+    Q->world_start = c->world_start;
+    Q->count = c->count;
+    Q->shape0 = c->shape0; Q->shape1 = c->shape1;
+    Q->p0 = c->point0; Q->p1 = c->point1; Q->nrm = c->normal;
+    Q->m0 = c->margin0; Q->m1 = c->margin1;
+    return SAPX_OK;
+}
+
+int32_t sapx_collide(SapxPipeline* p, const float* body_q, const SapxContactsView* c, void* stream) {
     CollideParams P;
-    P.m = dm;
-    P.body_q = body_q;
-    P.C = p->C;
-    P.st_shape0 = p->st_shape0; P.st_shape1 = p->st_shape1;
-    P.st_p0 = p->st_p0; P.st_p1 = p->st_p1; P.st_n = p->st_n; P.st_m0 = p->st_m0; P.st_m1 = p->st_m1;
+    PlaceParams Q;
+    const int32_t rc = collide_params(p, body_q, c, &P, &Q);
+    if (rc != SAPX_OK) return rc;
+    P.st_i = p->st_i;
+    P.st_f = p->st_f;
     P.wcount = p->wcount;
     P.unsupported = p->unsupported;
-    if (p->P >= 0) {
-        P.m.t_pairs = p->t_pairs;
-        P.m.P = p->P;
-        P.m.has_convex = p->has_convex;
-    }
-    CompactParams Q;
-    Q.W = dm.W; Q.C = p->C; Q.wcount = p->wcount; Q.world_start = c->world_start; Q.count = c->count;
-    Q.st_shape0 = p->st_shape0; Q.st_shape1 = p->st_shape1; Q.st_p0 = p->st_p0; Q.st_p1 = p->st_p1; Q.st_n = p->st_n;
-    Q.st_m0 = p->st_m0; Q.st_m1 = p->st_m1;
-    Q.shape0 = c->shape0; Q.shape1 = c->shape1; Q.p0 = c->point0; Q.p1 = c->point1; Q.nrm = c->normal;
-    Q.m0 = c->margin0; Q.m1 = c->margin1; Q.capacity = c->capacity;
+    Q.st_i = p->st_i;
+    Q.st_f = p->st_f;
+    Q.wcount = p->wcount;
     HIP_TRY(launch_collide(P, Q, reinterpret_cast<hipStream_t>(stream)));
+    return SAPX_OK;
+}
+
+static const WaveLayout& wave_layout(const SapxSolver* s, int cb) {
+    return cb == 16 ? s->wl16 : (cb == 32 ? s->wl32 : s->wl64);
+}
+
+static int32_t batch_upload(SapxBatch* b, int slot) {
+    const size_t G = b->s.size();
+    SapxBatch::Views& v = b->views[slot];
+    for (size_t g = 0; g < G; ++g) {
+        SapxSolver* s = b->s[g];
+        StepParams P;
+        const int32_t rc = fill_step_params(s, &v.in[g], &v.out[g], &v.ctl[g], &v.c[g], v.dt, nullptr, nullptr, &P);
+        if (rc != SAPX_OK) return rc;
+        P.world_order = nullptr;
+        P.learn_delta = nullptr;
+        P.learn_target = nullptr;
+        StepParams Pb = P;
+        P.wl = wave_layout(s, b->small);
+        P.wave_min_nc = 0;
+        Pb.wl = s->wl64;
+        Pb.wave_min_nc = b->small + 1;
+        b->hparams[((size_t)slot * 2 + 0) * G + g] = P;
+        b->hparams[((size_t)slot * 2 + 1) * G + g] = Pb;
+        v.use_guess[g] = s->use_guess;
+    }
+    HIP_TRY(hipMemcpy(b->dparams + (size_t)slot * 2 * G, b->hparams.data() + (size_t)slot * 2 * G,
+                      2 * G * sizeof(StepParams), hipMemcpyHostToDevice));
+    v.bound = 1;
+    return SAPX_OK;
+}
+
+int32_t sapx_batch_create(SapxSolver* const* solvers, int32_t count, SapxBatch** out) {
+    if (!solvers || !out) return fail(SAPX_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (count < 1 || count > kMaxBatchGroups)
+        return fail(SAPX_ERR_CAPACITY, "a batch holds 1 to " + std::to_string(kMaxBatchGroups) + " solvers");
+    for (int g = 0; g < count; ++g) {
+        const SapxSolver* s = solvers[g];
+        if (!s) return fail(SAPX_ERR_INVALID, "null solver");
+        if (!s->wave_ok || s->npad > 48)
+            return fail(SAPX_ERR_UNSUPPORTED, "batch steps need the one-wave kernel layout (at most 48 dofs per world)");
+        if (s->learn_delta || s->learn_target)
+            return fail(SAPX_ERR_UNSUPPORTED, "the learned warm start is not part of batch steps");
+        if (!s->wl16.bytes) return fail(SAPX_ERR_UNSUPPORTED, "a solver's 16-slot layout does not fit the LDS");
+    }
+    SapxBatch* b = new SapxBatch();
+    b->s.assign(solvers, solvers + count);
+    b->hparams.resize((size_t)4 * count);
+    for (auto& v : b->views) v.use_guess.assign(count, 0);
+    if (hipMalloc(&b->dparams, (size_t)4 * count * sizeof(StepParams)) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess) {
+        delete b;
+        return fail(SAPX_ERR_HIP, "batch allocation failed");
+    }
+    *out = b;
+    return SAPX_OK;
+}
+
+void sapx_batch_destroy(SapxBatch* b) { delete b; }
+
+int32_t sapx_batch_set_schedule(SapxBatch* b, int32_t small_capacity, int32_t fork) {
+    if (!b) return fail(SAPX_ERR_INVALID, "null batch");
+    if (small_capacity != 16 && small_capacity != 32 && small_capacity != 64)
+        return fail(SAPX_ERR_INVALID, "small_capacity must be 16, 32 or 64");
+    for (SapxSolver* s : b->s)
+        if (!wave_layout(s, small_capacity).bytes || (small_capacity < s->opts.max_rigid_contact && !s->wl64.bytes))
+            return fail(SAPX_ERR_UNSUPPORTED, "a solver's layout for this schedule does not fit the LDS");
+    b->small = small_capacity;
+    b->fork = fork ? 1 : 0;
+    for (int slot = 0; slot < 2; ++slot)
+        if (b->views[slot].bound) {
+            const int32_t rc = batch_upload(b, slot);
+            if (rc != SAPX_OK) return rc;
+        }
+    return SAPX_OK;
+}
+
+int32_t sapx_batch_bind(SapxBatch* b, int32_t slot, const SapxStateView* in, const SapxStateView* out,
+                        const SapxControlView* ctl, const SapxContactsView* contacts, double dt) {
+    if (!b || !in || !out || !ctl || !contacts) return fail(SAPX_ERR_INVALID, "null argument");
+    if (slot < 0 || slot > 1) return fail(SAPX_ERR_INVALID, "slot must be 0 or 1");
+    if (!(dt > 0.0)) return fail(SAPX_ERR_INVALID, "dt must be positive");
+    const size_t G = b->s.size();
+    for (size_t g = 0; g < G; ++g)
+        if (!contacts[g].world_start || !contacts[g].count)
+            return fail(SAPX_ERR_INVALID, "batch steps take world-grouped contacts (world_start)");
+    SapxBatch::Views& v = b->views[slot];
+    v.in.assign(in, in + G);
+    v.out.assign(out, out + G);
+    v.ctl.assign(ctl, ctl + G);
+    v.c.assign(contacts, contacts + G);
+    v.dt = dt;
+    return batch_upload(b, slot);
+}
+
+int32_t sapx_batch_step(SapxBatch* b, int32_t slot, void* stream) {
+    if (!b) return fail(SAPX_ERR_INVALID, "null batch");
+    if (slot < 0 || slot > 1 || !b->views[slot].bound) return fail(SAPX_ERR_INVALID, "slot is not bound");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const size_t G = b->s.size();
+    // a solver whose warm start became valid since the upload (its first step): refresh the slot
+    bool stale = false;
+    for (size_t g = 0; g < G; ++g) stale |= b->views[slot].use_guess[g] != b->s[g]->use_guess;
+    if (stale) {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIP_TRY(hipStreamIsCapturing(st, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            return fail(SAPX_ERR_INVALID, "step the batch once before capturing it (warm-start state changes)");
+        HIP_TRY(hipStreamSynchronize(st));
+        const int32_t rc = batch_upload(b, slot);
+        if (rc != SAPX_OK) return rc;
+    }
+    BatchMap M0{}, M1{};
+    int blocks = 0;
+    size_t smem0 = 0, smem1 = 0;
+    bool large = false;
+    for (size_t g = 0; g < G; ++g) {
+        const SapxSolver* s = b->s[g];
+        M0.block_start[g] = blocks;
+        M0.npad[g] = s->npad;
+        blocks += s->model->dm.W;
+        smem0 = std::max(smem0, (size_t)wave_layout(s, b->small).bytes);
+        smem1 = std::max(smem1, (size_t)s->wl64.bytes);
+        large |= b->small < 64 && s->opts.max_rigid_contact > b->small;
+    }
+    M0.block_start[G] = blocks;
+    M0.groups = (int)G;
+    M1 = M0;
+    M0.params = b->dparams + (size_t)slot * 2 * G;
+    M1.params = b->dparams + ((size_t)slot * 2 + 1) * G;
+    auto launch = [&](const BatchMap& M, int cb, size_t smem, hipStream_t sq) {
+        return cb == 16 ? launch_batch_16(M, blocks, smem, sq)
+                        : (cb == 32 ? launch_batch_32(M, blocks, smem, sq) : launch_batch_64(M, blocks, smem, sq));
+    };
+    if (!large) {
+        HIP_TRY(launch(M0, b->small, smem0, st));
+    } else if (b->fork) {
+        HIP_TRY(hipEventRecord(b->ev_fork, st));
+        HIP_TRY(hipStreamWaitEvent(b->aux, b->ev_fork, 0));
+        HIP_TRY(launch(M1, 64, smem1, b->aux));
+        HIP_TRY(hipEventRecord(b->ev_join, b->aux));
+        HIP_TRY(launch(M0, b->small, smem0, st));
+        HIP_TRY(hipStreamWaitEvent(st, b->ev_join, 0));
+    } else {
+        HIP_TRY(launch(M0, b->small, smem0, st));
+        HIP_TRY(launch(M1, 64, smem1, st));
+    }
+    for (SapxSolver* s : b->s) s->use_guess = 1;
     return SAPX_OK;
 }
 

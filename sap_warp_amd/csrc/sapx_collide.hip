@@ -6,13 +6,16 @@
 //
 // Reference: sim/collision/pipeline.py:51-179 (writer, AABBs),
 // broad_phase_common.py:18-35 / broad_phase_nxn.py:21-59 (pair test),
-// narrow_phase.py:141-577 (primitive kernel), primitive.py:37-428, 1150-1205,
+// narrow_phase.py:141-577 (primitive kernel), primitive.py:37-657, 1150-1205,
 // contact_data.py:36-56 (gap check).
 //
 // Output: contacts of world w are appended in candidate-pair order; a pair's
 // contacts are written only if the whole batch fits into the world's capacity
-// (the reference's batch-overflow rule, narrow_phase.py:534-539).  A second
-// pass scans per-world counts and compacts into the flat SapContacts arrays.
+// (the reference's batch-overflow rule, narrow_phase.py:534-539).  The world's
+// contacts are staged in LDS and copied out coalesced into the world's block of
+// a W x C staging area with its count; a second kernel places every world's block
+// at its offset in the flat SapContacts arrays (sum of the earlier counts) and
+// writes the per-world offsets.
 #include "sapx_device.h"
 #include "sapx_params.h"
 
@@ -56,6 +59,19 @@ __device__ void shape_aabb(int type, V3 pos, Q4 q, V3 scale, float radius, float
         h = v3(fabsf(c[0].x) * scale.x + fabsf(c[1].x) * scale.y + fabsf(c[2].x) * scale.z,
                fabsf(c[0].y) * scale.x + fabsf(c[1].y) * scale.y + fabsf(c[2].y) * scale.z,
                fabsf(c[0].z) * scale.x + fabsf(c[1].z) * scale.y + fabsf(c[2].z) * scale.z);
+    } else if (type == kGeoEllipsoid) {
+        // extent along world axis i: the support value sqrt(sum_k (R_ik s_k)^2)
+        V3 c[3];
+        rot_cols(q, c);
+        const V3 a = c[0] * scale.x, b = c[1] * scale.y, d = c[2] * scale.z;
+        h = v3(sqrtf(a.x * a.x + b.x * b.x + d.x * d.x), sqrtf(a.y * a.y + b.y * b.y + d.y * d.y),
+               sqrtf(a.z * a.z + b.z * b.z + d.z * d.z));
+    } else if (type == kGeoCylinder) {
+        // extent along world axis i: r sqrt(1 - a_i^2) + h |a_i| for the unit axis a
+        const V3 a = qrot(q, v3(0.0f, 0.0f, 1.0f));
+        h = v3(scale.x * sqrtf(fmaxf(1.0f - a.x * a.x, 0.0f)) + scale.y * fabsf(a.x),
+               scale.x * sqrtf(fmaxf(1.0f - a.y * a.y, 0.0f)) + scale.y * fabsf(a.y),
+               scale.x * sqrtf(fmaxf(1.0f - a.z * a.z, 0.0f)) + scale.y * fabsf(a.z));
     } else {
         h = v3(radius, radius, radius);
     }
@@ -145,6 +161,110 @@ __device__ void plane_box(V3 n, V3 pp, V3 bp, Q4 bq, V3 size, float margin, floa
         }
     }
 }
+// plane-ellipsoid (primitive.py:322-351): the ellipsoid's support point against the plane normal
+__device__ void plane_ellipsoid(V3 n, V3 pp, V3 ep, Q4 eq, V3 size, float* dist, V3* pos) {
+    V3 c[3];
+    rot_cols(eq, c);
+    const V3 ln = rtmul(c, n);
+    V3 sup = nsafe(v3(ln.x * size.x, ln.y * size.y, ln.z * size.z));
+    sup = v3(-sup.x, -sup.y, -sup.z);
+    const V3 p = ep + rmul(c, v3(sup.x * size.x, sup.y * size.y, sup.z * size.z));
+    const float d = dot(n, p - pp);
+    *dist = d;
+    *pos = p - n * (d * 0.5f);
+}
+
+// sphere-cylinder (primitive.py:431-501): side, cap or rim, whichever the sphere centre is nearest
+__device__ void sphere_cylinder(V3 sp, float sr, V3 cp, V3 axis, float cr, float hh, float* dist, V3* pos,
+                                V3* normal) {
+    const V3 vec = sp - cp;
+    const float x = dot(vec, axis);
+    const V3 a_proj = axis * x;
+    V3 p_proj = vec - a_proj;
+    const float p_sqr = dot(p_proj, p_proj);
+    bool side = fabsf(x) < hh;
+    bool cap = p_sqr < cr * cr;
+    if (side && cap) {
+        const float d_cap = hh - fabsf(x);
+        const float d_rad = cr - sqrtf(p_sqr);
+        if (d_cap < d_rad) side = false;
+        else cap = false;
+    }
+    if (side) {
+        sphere_sphere(sp, sr, cp + a_proj, cr, dist, pos, normal);
+    } else if (cap) {
+        V3 cpos, pn;
+        if (x > 0.0f) { cpos = cp + axis * hh; pn = axis; }
+        else { cpos = cp - axis * hh; pn = v3(-axis.x, -axis.y, -axis.z); }
+        plane_sphere(pn, cpos, sp, sr, dist, pos);
+        *normal = v3(-pn.x, -pn.y, -pn.z);
+    } else {
+        const float l = sqrtf(p_sqr);
+        const float inv = 1.0f / (l != 0.0f ? l : 1e-15f);
+        p_proj = p_proj * (cr * inv);
+        const V3 cap_off = axis * ((x < 0.0f ? -1.0f : 1.0f) * hh);
+        sphere_sphere(sp, sr, (cp + cap_off) + p_proj, 0.0f, dist, pos, normal);
+    }
+}
+
+// plane-cylinder (primitive.py:504-657): the deepest rim point first, then a fixed tripod on the
+// near cap (axis within 22.5 degrees of the normal) or the two side generators and the cap-rim
+// point facing the plane, each kept when farther than 1 % of the larger dimension from the first
+__device__ void plane_cylinder(V3 n, V3 pp, V3 cc, V3 axis, float r, float hh, float dist[4], V3 pos[4]) {
+    constexpr float kFlatCos = 0.9238795325112867f;  // cos(22.5 deg)
+    float dna = dot(n, axis);
+    if (dna > 0.0f) { axis = v3(-axis.x, -axis.y, -axis.z); dna = -dna; }
+    const V3 cap = cc + axis * hh;
+    V3 pa = v3(-n.x, -n.y, -n.z) + axis * dna;
+    const float pa_sq = dot(pa, pa);
+    const bool has_align = pa_sq > 1e-10f;
+    if (has_align) pa = pa * (1.0f / sqrtf(pa_sq));
+    V3 ref = v3(1.0f, 0.0f, 0.0f);
+    if (fabsf(dot(axis, ref)) > 0.9f) ref = v3(0.0f, 1.0f, 0.0f);
+    const V3 pf = nsafe(ref - axis * dot(axis, ref));
+    const bool flat = -dna >= kFlatCos;
+    const V3 dperp = has_align ? pa : pf;
+    const V3 dpt = cap + dperp * r;
+    const float dd = dot(dpt - pp, n);
+    const V3 dpos = dpt - n * (dd * 0.5f);
+    dist[0] = dd;
+    pos[0] = dpos;
+    int nc = 1;
+    const float thr = 0.01f * fmaxf(r, hh);
+    const float thr_sq = thr * thr;
+    V3 pts[3];
+    int np;
+    if (flat) {
+        const V3 u = pf * r;
+        const V3 v = cross(axis, pf) * r;
+        const float c120 = -0.5f, s120 = 0.8660254f;
+        pts[0] = cap + u;
+        pts[1] = (cap + u * c120) + v * s120;
+        pts[2] = (cap + u * c120) - v * s120;
+        np = 3;
+    } else {
+        const V3 pr = has_align ? pa : pf;
+        const V3 u = pr * r;
+        const V3 v = cross(axis, pr) * r;
+        pts[0] = (cc + axis * hh) + u;
+        pts[1] = (cc - axis * hh) + u;
+        const V3 ppv = cap + v, pnv = cap - v;
+        const float dpv = dot(ppv - pp, n), dnv = dot(pnv - pp, n);
+        pts[2] = dpv <= dnv ? ppv : pnv;
+        np = 3;
+    }
+    for (int k = 0; k < np; ++k) {
+        const float d = dot(pts[k] - pp, n);
+        const V3 p = pts[k] - n * (d * 0.5f);
+        const V3 e = p - dpos;
+        if (nc < 4 && dot(e, e) > thr_sq) {
+            dist[nc] = d;
+            pos[nc] = p;
+            ++nc;
+        }
+    }
+}
+
 __device__ __forceinline__ float vget(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 __device__ void sphere_box(V3 sp, float r, V3 bp, Q4 bq, V3 size, float* dist, V3* pos, V3* normal) {
     V3 c[3];
@@ -185,6 +305,15 @@ __device__ __forceinline__ void contact_points(V3 center, V3 n_in, float d, floa
     *xb = center + (*n) * (0.5f * d + rb);
 }
 
+// LDS layout of the collide kernel: transforms and AABBs (14 floats per shape), NT/64 + 4 ints,
+// then the contact staging (13 words per slot), then the convex polygon buffer
+__host__ __device__ inline size_t collide_stage_offset(int ns, int nt) {
+    return ((size_t)ns * 14 * 4 + (size_t)(nt / 64 + 4) * 4 + 15) / 16 * 16;
+}
+__host__ __device__ inline size_t collide_poly_offset(int ns, int nt, int C) {
+    return (collide_stage_offset(ns, nt) + (size_t)C * 13 * 4 + 15) / 16 * 16;
+}
+
 // CVX: the model has capsule-box or box-box candidate pairs (MPR / GJK path, up to five
 // contacts per pair, a 16-slot polygon buffer per lane in LDS)
 template <int NT, bool CVX>
@@ -192,14 +321,22 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     constexpr int NC = CVX ? 5 : 4;
     const DevModel& M = P.m;
-    const int w = blockIdx.x;
     const int tid = threadIdx.x;
     const int nglob = M.total_shapes - M.global_shape_base;
     const int ns = M.S + nglob;
     float* X = reinterpret_cast<float*>(smem_raw);  // ns x 8 world transforms
     float* AB = X + ns * 8;                          // ns x 6 aabbs
-    int* scan = reinterpret_cast<int*>(AB + ns * 6); // NT/64 wave totals + 2
-    float2* poly = reinterpret_cast<float2*>(smem_raw + ((size_t)ns * 14 * 4 + (NT / 64 + 2) * 4 + 15) / 16 * 16);
+    int* scan = reinterpret_cast<int*>(AB + ns * 6); // NT/64 wave totals, staged count
+    // the world's contacts, staged: shape0, shape1, margin0, margin1 (C each), point0, point1, normal (3C each)
+    int* s_sh0 = reinterpret_cast<int*>(smem_raw + collide_stage_offset(ns, NT));
+    int* s_sh1 = s_sh0 + P.C;
+    float* s_m0 = reinterpret_cast<float*>(s_sh1 + P.C);
+    float* s_m1 = s_m0 + P.C;
+    float* s_p0 = s_m1 + P.C;
+    float* s_p1 = s_p0 + 3 * P.C;
+    float* s_n = s_p1 + 3 * P.C;
+    float2* poly = reinterpret_cast<float2*>(smem_raw + collide_poly_offset(ns, NT, P.C));
+    const int w = blockIdx.x;
     for (int s = tid; s < ns; s += NT) {
         const int g = (s < M.S) ? w * M.S + s : M.global_shape_base + (s - M.S);
         TF xs = load_tf(M.shape_transform + (size_t)g * 7);
@@ -214,7 +351,7 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
         float* a = AB + s * 6;
         a[0] = lo.x; a[1] = lo.y; a[2] = lo.z; a[3] = hi.x; a[4] = hi.y; a[5] = hi.z;
     }
-    if (tid == 0) { scan[NT / 64] = 0; scan[NT / 64 + 1] = 0; }
+    if (tid == 0) scan[NT / 64] = 0;  // running count of staged contacts
     __syncthreads();
     int claim_base = 0;  // uniform running claim of earlier chunks
     const int lane = tid & 63, wave = tid >> 6;
@@ -283,6 +420,17 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
                     capsule_capsule(Xa.p, a1, scale_a.x, scale_a.y, Xb.p, a2, scale_b.x, scale_b.y, dist, pos, &normal);
                 } else if (ta == kGeoSphere && tb == kGeoBox) {
                     sphere_box(Xa.p, scale_a.x, Xb.p, Xb.q, scale_b, &dist[0], &pos[0], &normal);
+                } else if (ta == kGeoPlane && tb == kGeoEllipsoid) {
+                    V3 nn = qrot(Xa.q, ez);
+                    plane_ellipsoid(nn, Xa.p, Xb.p, Xb.q, scale_b, &dist[0], &pos[0]);
+                    normal = nn;
+                } else if (ta == kGeoPlane && tb == kGeoCylinder) {
+                    V3 nn = qrot(Xa.q, ez);
+                    plane_cylinder(nn, Xa.p, Xb.p, qrot(Xb.q, ez), scale_b.x, scale_b.y, dist, pos);
+                    normal = nn;
+                } else if (ta == kGeoSphere && tb == kGeoCylinder) {
+                    sphere_cylinder(Xa.p, scale_a.x, Xb.p, qrot(Xb.q, ez), scale_b.x, scale_b.y, &dist[0], &pos[0],
+                                    &normal);
                 } else if (CVX && (ta == kGeoCapsule || ta == kGeoBox) && tb == kGeoBox) {
                     const cvx::Poly buf{poly + tid, NT};
                     ncvx = cvx::pair(ta, scale_a, Xa.p, Xa.q, tb, scale_b, Xb.p, Xb.q, gap_sum, M.shape_margin[ga],
@@ -325,7 +473,6 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
         // contacts each claim a slot and are written while the slot is inside the capacity
         if (nv > 0 && (prefix + nv <= P.C || (ncvx >= 0 && prefix < P.C))) {
             int idx = prefix;
-            const size_t base = (size_t)w * P.C;
             const int bA = M.shape_body[ga], bB = M.shape_body[gb];
             const float ma = M.shape_margin[ga], mbm = M.shape_margin[gb];
             for (int i = 0; i < NC; ++i) {
@@ -342,14 +489,14 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
                     TF t = load_tf(P.body_q + (size_t)bB * 7);
                     pb = qrot(qconj(t.q), xb - t.p);
                 }
-                const size_t o = base + idx;
-                P.st_shape0[o] = ga;
-                P.st_shape1[o] = gb;
-                P.st_p0[o * 3] = pa.x; P.st_p0[o * 3 + 1] = pa.y; P.st_p0[o * 3 + 2] = pa.z;
-                P.st_p1[o * 3] = pb.x; P.st_p1[o * 3 + 1] = pb.y; P.st_p1[o * 3 + 2] = pb.z;
-                P.st_n[o * 3] = nn.x; P.st_n[o * 3 + 1] = nn.y; P.st_n[o * 3 + 2] = nn.z;
-                P.st_m0[o] = ra + ma;
-                P.st_m1[o] = rb + mbm;
+                const int o = idx;
+                s_sh0[o] = ga;
+                s_sh1[o] = gb;
+                s_p0[o * 3] = pa.x; s_p0[o * 3 + 1] = pa.y; s_p0[o * 3 + 2] = pa.z;
+                s_p1[o * 3] = pb.x; s_p1[o * 3 + 1] = pb.y; s_p1[o * 3 + 2] = pb.z;
+                s_n[o * 3] = nn.x; s_n[o * 3 + 1] = nn.y; s_n[o * 3 + 2] = nn.z;
+                s_m0[o] = ra + ma;
+                s_m1[o] = rb + mbm;
                 ++idx;
             }
             atomicMax(&scan[NT / 64], idx);
@@ -357,79 +504,115 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
         claim_base += chunk_total;
         __syncthreads();
     }
-    if (tid == 0) P.wcount[w] = scan[NT / 64];
+    // ---- the world's block of the staging area (shape ids, margins, xyz triples), coalesced
+    const int nw = scan[NT / 64];  // contacts staged (at most C)
+    const size_t C = (size_t)P.C;
+    int* gi = P.st_i + (size_t)w * C * 2;
+    float* gf = P.st_f + (size_t)w * C * 11;
+    for (int k = tid; k < nw; k += NT) {
+        gi[k] = s_sh0[k];
+        gi[C + k] = s_sh1[k];
+        gf[k] = s_m0[k];
+        gf[C + k] = s_m1[k];
+    }
+    for (int e = tid; e < 3 * nw; e += NT) {
+        gf[2 * C + e] = s_p0[e];
+        gf[5 * C + e] = s_p1[e];
+        gf[8 * C + e] = s_n[e];
+    }
+    if (tid == 0) P.wcount[w] = nw;
 }
 
-// exclusive scan of per-world counts (single workgroup)
-template <int NT>
-__global__ __launch_bounds__(NT) void sapx_world_scan_kernel(CompactParams P) {
-    __shared__ int part[NT / 64];
-    __shared__ int carry;
+// Places every world's staged contacts at its offset in the flat, world-grouped contact arrays
+// (the sum of the earlier worlds' counts, cut at the buffer's capacity) and writes world_start /
+// count.  One workgroup per kPlaceWorlds consecutive worlds (enough workgroups to spread the copy
+// over the CUs); each sums the counts before its range itself, so workgroups never wait on each
+// other.
+constexpr int kPlaceWorlds = 32;
+constexpr int kPlaceThreads = 256;
+__global__ __launch_bounds__(kPlaceThreads) void sapx_place_kernel(PlaceParams P) {
+    __shared__ int part[kPlaceThreads / 64];
+    __shared__ int incl_s[kPlaceWorlds];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid == 0) carry = 0;
+    const int w0 = blockIdx.x * kPlaceWorlds;
+    // counts of all earlier worlds
+    int acc = 0;
+    for (int i = tid; i < w0; i += kPlaceThreads) acc += P.wcount[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) part[wave] = acc;
     __syncthreads();
-    for (int b0 = 0; b0 < P.W; b0 += NT) {
-        const int wi = b0 + tid;
-        const int x = wi < P.W ? P.wcount[wi] : 0;
+    int base = 0;
+#pragma unroll
+    for (int k = 0; k < kPlaceThreads / 64; ++k) base += part[k];
+    const int cap = P.capacity;
+    // inclusive scan of this range's counts (wave 0)
+    if (wave == 0) {
+        const int w = w0 + lane;
+        const int x = (lane < kPlaceWorlds && w < P.W) ? P.wcount[w] : 0;
         int incl = x;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
-            int y = __shfl_up(incl, off, 64);
+            const int y = __shfl_up(incl, off, 64);
             if (lane >= off) incl += y;
         }
-        if (lane == 63) part[wave] = incl;
-        __syncthreads();
-        int off = carry, tot = 0;
-        for (int wv = 0; wv < NT / 64; ++wv) {
-            if (wv < wave) off += part[wv];
-            tot += part[wv];
+        if (lane < kPlaceWorlds) incl_s[lane] = incl;
+        if (lane < kPlaceWorlds && w < P.W) P.world_start[w] = min(base + incl - x, cap);
+        if (w == P.W - 1) {
+            P.world_start[P.W] = min(base + incl, cap);
+            P.count[0] = min(base + incl, cap);
         }
-        // offsets past the contact buffer's capacity are clamped to it: the contacts there are
-        // dropped by the compaction, and every world's range stays inside the buffer
-        if (wi < P.W) P.world_start[wi] = min(off + incl - x, P.capacity);
-        __syncthreads();
-        if (tid == 0) carry += tot;
-        __syncthreads();
     }
-    if (tid == 0) {
-        P.world_start[P.W] = min(carry, P.capacity);
-        P.count[0] = min(carry, P.capacity);
+    __syncthreads();
+    // copy: contact t of the range belongs to the first world whose inclusive count exceeds t
+    const int nrange = min(kPlaceWorlds, P.W - w0);
+    const int total = incl_s[nrange - 1];
+    const size_t C = (size_t)P.C;
+    for (int t = tid; t < total; t += kPlaceThreads) {
+        int lo = 0, hi = nrange - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (incl_s[mid] > t) hi = mid;
+            else lo = mid + 1;
+        }
+        const int k = t - (lo > 0 ? incl_s[lo - 1] : 0);
+        const int dst = base + t;
+        if (dst >= cap) break;
+        const size_t ww = (size_t)(w0 + lo);
+        const int* gi = P.st_i + ww * C * 2;
+        const float* gf = P.st_f + ww * C * 11;
+        P.shape0[dst] = gi[k];
+        P.shape1[dst] = gi[C + k];
+        P.m0[dst] = gf[k];
+        P.m1[dst] = gf[C + k];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            P.p0[(size_t)dst * 3 + e] = gf[2 * C + 3 * k + e];
+            P.p1[(size_t)dst * 3 + e] = gf[5 * C + 3 * k + e];
+            P.nrm[(size_t)dst * 3 + e] = gf[8 * C + 3 * k + e];
+        }
     }
 }
 
-__global__ void sapx_compact_kernel(CompactParams P) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int w = (int)(t / P.C), k = (int)(t % P.C);
-    if (w >= P.W) return;
-    if (k >= P.wcount[w]) return;
-    const int dst = P.world_start[w] + k;
-    if (dst >= P.capacity) return;
-    P.shape0[dst] = P.st_shape0[t];
-    P.shape1[dst] = P.st_shape1[t];
-    for (int e = 0; e < 3; ++e) {
-        P.p0[(size_t)dst * 3 + e] = P.st_p0[t * 3 + e];
-        P.p1[(size_t)dst * 3 + e] = P.st_p1[t * 3 + e];
-        P.nrm[(size_t)dst * 3 + e] = P.st_n[t * 3 + e];
-    }
-    P.m0[dst] = P.st_m0[t];
-    P.m1[dst] = P.st_m1[t];
+size_t collide_lds_bytes(const DevModel& m, int C) {
+    const int ns = m.S + (m.total_shapes - m.global_shape_base);
+    const size_t base = collide_poly_offset(ns, kCollideThreads, C);
+    return m.has_convex ? base + (size_t)kCollideThreads * cvx::kBuf * sizeof(float2) : base;
 }
 
-hipError_t launch_collide(const CollideParams& P, const CompactParams& Q, hipStream_t stream) {
-    constexpr int NT = 128;
+hipError_t launch_collide(const CollideParams& P, const PlaceParams& Q, hipStream_t stream) {
+    constexpr int NT = kCollideThreads;
     const int nglob = P.m.total_shapes - P.m.global_shape_base;
     const int ns = P.m.S + nglob;
-    const size_t base = ((size_t)ns * 14 * sizeof(float) + (NT / 64 + 2) * sizeof(int) + 15) / 16 * 16;
+    const size_t base = collide_poly_offset(ns, NT, P.C);
     if (P.m.has_convex) {
         const size_t smem = base + (size_t)NT * cvx::kBuf * sizeof(float2);
         hipLaunchKernelGGL((sapx_collide_kernel<NT, true>), dim3(P.m.W), dim3(NT), smem, stream, P);
     } else {
-        hipLaunchKernelGGL((sapx_collide_kernel<NT, false>), dim3(P.m.W), dim3(NT), base + 16, stream, P);
+        hipLaunchKernelGGL((sapx_collide_kernel<NT, false>), dim3(P.m.W), dim3(NT), base, stream, P);
     }
-    hipLaunchKernelGGL(sapx_world_scan_kernel<1024>, dim3(1), dim3(1024), 0, stream, Q);
-    const size_t total = (size_t)Q.W * Q.C;
-    const int blk = 256;
-    hipLaunchKernelGGL(sapx_compact_kernel, dim3((unsigned)((total + blk - 1) / blk)), dim3(blk), 0, stream, Q);
+    const unsigned blocks = (unsigned)((Q.W + kPlaceWorlds - 1) / kPlaceWorlds);
+    hipLaunchKernelGGL(sapx_place_kernel, dim3(blocks), dim3(kPlaceThreads), 0, stream, Q);
     return hipGetLastError();
 }
 

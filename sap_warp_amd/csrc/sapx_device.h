@@ -21,6 +21,8 @@ constexpr int kJointD6 = 6;
 constexpr int kGeoPlane = 1;
 constexpr int kGeoSphere = 3;
 constexpr int kGeoCapsule = 4;
+constexpr int kGeoEllipsoid = 5;
+constexpr int kGeoCylinder = 6;
 constexpr int kGeoBox = 7;
 
 constexpr int kMaxTemplate = 64;  // joints / bodies per world (64-bit masks)

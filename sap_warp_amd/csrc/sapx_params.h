@@ -50,6 +50,8 @@ struct StepParams {
     double* dbg_phi0;
     double* dbg_w;
     double* dbg_adiag;
+    double* dbg_A;      // W x n x n: dynamics matrix A = H + diag(armature) as the solve uses it
+    double* dbg_J;      // W x C x 3 x n: contact Jacobian rows (t1, t2, n per slot; zero past the count)
     double dt;
     const double* dt_world;
     int C;
@@ -96,33 +98,40 @@ struct FeatureParams {
     float* out;           // W x F
 };
 
+// Collision: one workgroup per world stages its contacts in LDS, then copies them (coalesced) into
+// the world's block of a W x C staging area with its count; the place kernel moves every world's
+// block to its offset in the flat, world-grouped contact arrays.
+// Batch step: the worlds of up to kMaxBatchGroups models (kinds of different dof counts) in one
+// launch; block b belongs to the group g with block_start[g] <= b < block_start[g + 1].
+constexpr int kMaxBatchGroups = 4;
+struct BatchMap {
+    const StepParams* params;  // device, one per group
+    int groups;
+    int block_start[kMaxBatchGroups + 1];
+    int npad[kMaxBatchGroups];
+};
+
+constexpr int kCollideThreads = 128;
+constexpr size_t kCollideMaxLds = 64 * 1024;  // dynamic LDS of one collide workgroup
+
 struct CollideParams {
     DevModel m;
     const float* body_q;
-    int C;
-    int* st_shape0;
-    int* st_shape1;
-    float* st_p0;
-    float* st_p1;
-    float* st_n;
-    float* st_m0;
-    float* st_m1;
-    int* wcount;
+    int C;          // contact capacity per world
+    int* st_i;      // W x (2 C): shape0[C], shape1[C] per world
+    float* st_f;    // W x (11 C): margin0[C], margin1[C], point0[3C], point1[3C], normal[3C] per world
+    int* wcount;    // W: staged contacts per world
     int* unsupported;
 };
 
-struct CompactParams {
+struct PlaceParams {
     int W, C;
+    int capacity;  // slots of the flat contact arrays
     const int* wcount;
-    int* world_start;
+    const int* st_i;
+    const float* st_f;
+    int* world_start;  // W + 1
     int* count;
-    const int* st_shape0;
-    const int* st_shape1;
-    const float* st_p0;
-    const float* st_p1;
-    const float* st_n;
-    const float* st_m0;
-    const float* st_m1;
     int* shape0;
     int* shape1;
     float* p0;
@@ -130,7 +139,6 @@ struct CompactParams {
     float* nrm;
     float* m0;
     float* m1;
-    int capacity;
 };
 
 // grouping of an arbitrary flat contact list by world (stable radix sort)

@@ -1085,6 +1085,16 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
     double* av = d64 + V_AV * npad;
     int iteration = 0, ls_total = 0, flags = 0, ls_status = 0;  // meaningful on wave 0
     bool unconstrained = false;
+    if (P.dbg_A) {
+        for (int t = tid; t < n * n; t += NT) P.dbg_A[(size_t)w * n * n + t] = s.A[(t / n) * ast + (t % n)];
+    }
+    if (P.dbg_J) {
+        const size_t jb = (size_t)w * C * 3 * n;
+        for (int t = tid; t < C * 3 * n; t += NT) {
+            const int r = t / n;
+            P.dbg_J[jb + t] = (r < 3 * nc) ? (double)s.J[r * js + (t % n)] : 0.0;
+        }
+    }
     if (wave == 0) {
         int am = 0;
         for (int i = lane; i < n; i += 64)

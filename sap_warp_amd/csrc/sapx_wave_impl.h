@@ -252,12 +252,14 @@ __device__ __forceinline__ double dof_terms(const DofTerms& m, double vi, double
 // ---------------------------------------------------------------------------
 // the kernel
 // ---------------------------------------------------------------------------
+// One world of the step on one wave: `blk` is the world's slot in the launch (mapped through
+// P.world_order when set).  Shared by the per-model kernel and the batch kernel.
 template <int NP, int CB>
-__global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
+__device__ __forceinline__ void wave_body(const StepParams& P, const int blk) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     const DevModel& M = P.m;
     const WaveLayout& L = P.wl;
-    const int w = P.world_order ? P.world_order[blockIdx.x] : (int)blockIdx.x;
+    const int w = P.world_order ? P.world_order[blk] : blk;
     const int lane = threadIdx.x;
     const int n = M.N, nb = M.NB, nj = M.J;
     const int js = L.js, ls = L.ls;
@@ -1021,6 +1023,16 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
 
     // ------------------------------------------------------------------ 8
     // SAP Newton solve (contact_solve.py:6129-6285, 5474-5655)
+    if (P.dbg_A) {
+        for (int t = lane; t < n * n; t += 64) P.dbg_A[(size_t)w * n * n + t] = Am[(t / n) * ls + (t % n)];
+    }
+    if (P.dbg_J) {
+        const size_t jb = (size_t)w * C * 3 * n;
+        for (int t = lane; t < C * 3 * n; t += 64) {
+            const int r = t / n;
+            P.dbg_J[jb + t] = (r < 3 * nc) ? (double)Jm[r * js + (t % n)] : 0.0;
+        }
+    }
     const bool unconstrained = (nc == 0) && !__any(mt.pd | mt.lo | mt.up);
     double v = 0.0;
     if (lane < n) {
@@ -1637,6 +1649,11 @@ __global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
             if (start_kind) atomicAdd(&P.st_hist[start_kind == 1 ? 258 : 259], 1ull);
         }
     }
+}
+
+template <int NP, int CB>
+__global__ __launch_bounds__(64, 1) void sapx_wave_kernel(StepParams P) {
+    wave_body<NP, CB>(P, (int)blockIdx.x);
 }
 
 template <int NP, int CB>

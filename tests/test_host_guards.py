@@ -67,11 +67,26 @@ def test_stats_readback_refused_under_capture(cpu_model, monkeypatch):
 
 
 def test_hetero_batch_refused_under_capture(monkeypatch):
+    """The streams mode (nested fork) raises under capture; the batch mode raises under capture
+    until both slots are bound, and when statistics are collected."""
     from sap_warp_amd.hetero import HeteroBatch
 
     batch = object.__new__(HeteroBatch)
     batch.groups = {}
     batch.dt = 2.0e-3
+    batch.mode = "streams"
     monkeypatch.setattr(torch.cuda, "is_current_stream_capturing", lambda: True)
     with pytest.raises(RuntimeError, match="captured"):
+        batch.step()
+    batch.mode = "batch"
+    batch._k = 0
+    batch._bound = [None, None]
+    with pytest.raises(RuntimeError, match="before capturing"):
+        batch.step()
+
+    class _Solver:
+        collect_iteration_stats = True
+
+    batch.groups = {"kind": {"solver": _Solver()}}
+    with pytest.raises(RuntimeError, match="collect_iteration_stats"):
         batch.step()

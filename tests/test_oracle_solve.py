@@ -240,3 +240,51 @@ def test_frictionless_box_accelerates_at_f_over_m():
     vx = np.array([s["qd"][0] for s in hist])
     np.testing.assert_allclose(vx, F / m * dt * np.arange(1, 201), rtol=1e-5)
     assert np.max(np.abs(np.array([s["gamma"][:, :2] for s in hist]))) == 0.0
+
+
+def test_optimality_on_solver_data():
+    """optimality_on_data rebuilds the problem from data formed elsewhere: fed the oracle's own A,
+    v*, J, weights and phi0 it gives optimality_at's norms; a v optimal for a perturbed problem
+    (A scaled by 1 + 1e-4, as a float32-formed A would differ, only larger) passes on that problem
+    and fails on the oracle's."""
+    model = build_humanoid_scene(num_worlds=2, seed=4)
+    st, ctl = state_dict_from_model(model), control_dict_from_model(model)
+    contacts = oc.collide(model, st["body_q"], 64)
+    dt = 2.0e-3
+    opts = so.SolveOptions.for_variant("exact_root")
+    for w in range(2):
+        r = so.step_world(model, w, st, ctl, contacts, dt, so.StepOptions(line_search="exact_root"))
+        fm, cs, res = r["free_motion"], r["contacts"], r["result"]
+        assert res.iterations > 0
+        gn, tol = so.optimality_on_data(model, w, r, res.v, dt, opts, A=fm.A_env, v_star=fm.v_star, J=cs.J,
+                                        weight=cs.w, phi0=cs.phi0)
+        gn0, tol0, _, _ = so.optimality_at(model, w, r, res.v, dt, opts)
+        assert gn == pytest.approx(gn0, rel=1e-12) and tol == pytest.approx(tol0, rel=1e-12)
+        assert gn <= tol
+        # the optimum of a perturbed problem: optimal there, not on the original
+        A2 = fm.A_env * (1.0 + 1.0e-4)
+        wv = so.WorldView(model, w)
+        mt2 = so.build_model_terms(model, wv, r["joint_q_in"], ctl, A2, r["v0"], fm.v_star, dt)
+        tight = so.SolveOptions.for_variant("exact_root", optimality_rel_tol=1e-12, cost_rel_tol=0.0)
+        r2 = so.sap_solve(model, wv, A2, fm.v_star, r["v0"], None, cs, mt2, dt, tight)
+        gn2, tol2 = so.optimality_on_data(model, w, r, r2.v, dt, opts, A=A2, v_star=fm.v_star, J=cs.J,
+                                          weight=cs.w, phi0=cs.phi0)
+        assert gn2 <= tol2
+        gn3, tol3, _, _ = so.optimality_at(model, w, r, r2.v, dt, opts)
+        assert gn3 > 2.0 * tol3
+
+
+def test_solve_on_data_reproduces_own_solve():
+    """solve_on_data fed the oracle's own problem data returns the oracle's own solution."""
+    model = build_humanoid_scene(num_worlds=2, seed=4)
+    st, ctl = state_dict_from_model(model), control_dict_from_model(model)
+    contacts = oc.collide(model, st["body_q"], 64)
+    dt = 2.0e-3
+    opts = so.SolveOptions.for_variant("exact_root")
+    for w in range(2):
+        r = so.step_world(model, w, st, ctl, contacts, dt, so.StepOptions(line_search="exact_root"))
+        fm, cs = r["free_motion"], r["contacts"]
+        res = so.solve_on_data(model, w, r, dt, opts, A=fm.A_env, v_star=fm.v_star, J=cs.J, weight=cs.w,
+                               phi0=cs.phi0)
+        np.testing.assert_array_equal(res.v, r["result"].v)
+        np.testing.assert_array_equal(res.gamma, r["result"].gamma)
