@@ -157,8 +157,19 @@ def collide(model, body_q, max_contacts_per_world: int, pairs=None, threads: int
     return out
 
 
+def support(shape_type, scale, direction):
+    """Support point of one shape in its own frame, float32 (the MPR / GJK support maps)."""
+    lib = _load()
+    f32 = lambda x: np.ascontiguousarray(np.asarray(x, dtype=np.float32))  # noqa: E731
+    sc, d = f32(scale), f32(direction)
+    out = np.zeros(3, np.float32)
+    lib.sapx_oracle_support(ctypes.c_int(shape_type), _ptr(sc, ctypes.c_float), _ptr(d, ctypes.c_float),
+                            _ptr(out, ctypes.c_float))
+    return out
+
+
 def convex_pair(type_a, scale_a, xf_a, type_b, scale_b, xf_b, gap_sum=0.0, margin_a=0.0, margin_b=0.0):
-    """Contacts of one convex pair (box-box / capsule-box), float32.
+    """Contacts of one pair on the MPR / GJK path (sorted types, type_a <= type_b), float32.
 
     ``xf`` = (px, py, pz, qx, qy, qz, qw).  Returns a dict with ``pos`` (n x 3
     contact centres), ``dist`` (n signed distances), ``normal`` (world, A to B)

@@ -1,14 +1,17 @@
 /*
- * Convex-pair contacts (box-box, capsule-box) for the float32 oracle — TEST
- * INFRASTRUCTURE ONLY; included by narrowphase.c after its vector helpers.
+ * Contacts of the pairs the primitive routines leave to MPR / GJK (two
+ * support-mapped shapes among sphere, capsule, ellipsoid, cylinder, box, cone;
+ * an infinite plane against a cone as a box proxy) for the float32 oracle —
+ * TEST INFRASTRUCTURE ONLY; included by narrowphase.c after its vector helpers.
  *
  * Restates the reference's GJK/MPR path for pairs the primitive kernel routes on:
  *   routing + per-pair driver   sim/collision/narrow_phase.py:580-747,
  *                               core.py:625-724, core.py:278-392
  *   fused MPR-then-GJK solver   sim/collision/convex.py:26-118
  *   support maps                sim/collision/support.py:94-276 (box, sphere,
- *                               capsule; capsule/sphere radius replaced by 1e-4
- *                               and restored afterwards, core.py:340-351)
+ *                               capsule, ellipsoid, cylinder, cone; capsule /
+ *                               sphere radius replaced by 1e-4 and restored
+ *                               afterwards, core.py:340-351)
  *   Minkowski portal refinement sim/collision/mpr.py:27-237 (30 iterations,
  *                               epsilon 1e-5, inflate 1e-4)
  *   GJK distance                sim/collision/simplex.py:39-436 (30
@@ -22,7 +25,15 @@
  *                               when the face normals deviate by more than 0.1
  *                               degrees)
  *   tangent basis               sim/collision/math.py:20-44
- *   Minkowski post-process      sim/collision/core.py:143-169
+ *   post-process                sim/collision/core.py:172-275 (Minkowski shift;
+ *                               a cylinder / cone rolling on a box or plane:
+ *                               contact points projected onto the plane
+ *                               through its axis and the normal)
+ *   single contact              core.py:360-367 (a sphere or ellipsoid pair)
+ *   infinite plane as a box     core.py:507-566 (half extents 10 x the other
+ *                               shape's bounding-sphere radius + gap, top face
+ *                               on the plane under it), narrow_phase.py:640-720
+ *                               (bounding-sphere half-space test)
  *
  * The 2D polygon buffers keep the reference's layout: the shape-B polygon
  * starts at slot 0 and the shape-A polygon at slot 5 of one buffer, so the
@@ -65,6 +76,39 @@ static v3 cvx_support(const cvx_geom* g, v3 d) {
         v3 r = scl(n, g->scale.x);
         if (g->type == GEO_CAPSULE) r = add(r, mk(0.0f, 0.0f, d.z >= 0.0f ? g->scale.y : -g->scale.y));
         return r;
+    }
+    if (g->type == GEO_ELLIPSOID) {
+        const float a = g->scale.x, b = g->scale.y, c = g->scale.z;
+        if (len_sq(d) > 1e-12f) {
+            const float adx = a * d.x, bdy = b * d.y, cdz = c * d.z;
+            const float den2 = adx * adx + bdy * bdy + cdz * cdz;
+            if (den2 > 1e-12f) {
+                const float den = sqrtf(den2);
+                return mk((a * a) * d.x / den, (b * b) * d.y / den, (c * c) * d.z / den);
+            }
+        }
+        return mk(a, 0.0f, 0.0f);
+    }
+    if (g->type == GEO_CYLINDER) {
+        const float r = g->scale.x, hh = g->scale.y;
+        v3 lat = mk(r, 0.0f, 0.0f);
+        v3 dxy = mk(d.x, d.y, 0.0f);
+        if (len_sq(dxy) > 1e-12f) {
+            v3 nxy = normalize(dxy);
+            lat = mk(nxy.x * r, nxy.y * r, 0.0f);
+        }
+        if (d.z > 0.0f) return mk(lat.x, lat.y, hh);
+        if (d.z < 0.0f) return mk(lat.x, lat.y, -hh);
+        return lat;
+    }
+    if (g->type == GEO_CONE) {
+        /* apex at +hh, base disk at -hh: the apex when dz >= r / (2 hh) |d_xy| */
+        const float r = g->scale.x, hh = g->scale.y;
+        const float lxy = length(mk(d.x, d.y, 0.0f));
+        const float k = hh > 1e-12f ? r / (2.0f * hh) : 0.0f;
+        if (lxy <= 1e-12f) return d.z >= 0.0f ? mk(0.0f, 0.0f, hh) : mk(r, 0.0f, -hh);
+        if (d.z >= k * lxy) return mk(0.0f, 0.0f, hh);
+        return mk(d.x / lxy * r, d.y / lxy * r, -hh);
     }
     return mk(0.0f, 0.0f, 0.0f);
 }
@@ -625,6 +669,40 @@ static int cvx_manifold(const cvx_geom* ga, const cvx_geom* gb, q4 qa, v3 pa_w, 
  * box).  Fills up to five contact centres and signed distances (after the
  * Minkowski radius shift), the shared world normal and the effective radii.
  */
+/* A cylinder or cone against a box (or a plane's box proxy) in rolling contact: the contact points
+   move onto the plane through the shape's centre spanned by its axis and the normal (core.py:215-273).
+   A cylinder rolls when its axis is within 2 degrees of perpendicular to the normal, a cone when the
+   angle between axis and normal is within 2 degrees of its half angle atan2(r, 2 hh); sin(theta +- 2
+   deg) is formed from sin / cos of the half angle (r and 2 hh over their hypotenuse). */
+#define CVX_SIN2 0.034899496702500969f
+#define CVX_COS2 0.99939082701909576f
+static void cvx_axial_rolling(int ta_, v3 sa_, v3 pa_w, q4 qa, int tb_, v3 sb_, v3 pb_w, q4 qb, v3 nw, v3 pos[5],
+                              int count) {
+    const int disc_a = ta_ == GEO_BOX, disc_b = tb_ == GEO_BOX;
+    const int ax_a = ta_ == GEO_CYLINDER || ta_ == GEO_CONE, ax_b = tb_ == GEO_CYLINDER || tb_ == GEO_CONE;
+    if (!((disc_a && ax_b) || (disc_b && ax_a))) return;
+    v3 axis, spos, an;
+    float r, hh;
+    int cone;
+    if (disc_a && ax_b) {
+        axis = qrot(qb, mk(0.0f, 0.0f, 1.0f)); r = sb_.x; hh = sb_.y; cone = tb_ == GEO_CONE; spos = pb_w; an = nw;
+    } else {
+        axis = qrot(qa, mk(0.0f, 0.0f, 1.0f)); r = sa_.x; hh = sa_.y; cone = ta_ == GEO_CONE; spos = pa_w; an = v3neg(nw);
+    }
+    const float adot = fabsf(dot(axis, an));
+    int rolling;
+    if (cone) {
+        const float hyp = sqrtf(r * r + 4.0f * hh * hh);
+        const float sh = r / hyp, ch = (2.0f * hh) / hyp;
+        rolling = adot >= sh * CVX_COS2 - ch * CVX_SIN2 && adot <= sh * CVX_COS2 + ch * CVX_SIN2;
+    } else {
+        rolling = adot <= CVX_SIN2;
+    }
+    if (!rolling) return;
+    const v3 pn = normalize(cross(axis, an));
+    for (int i = 0; i < count; ++i) pos[i] = sub(pos[i], scl(pn, dot(sub(pos[i], spos), pn)));
+}
+
 static int cvx_pair(int ta_, v3 sa_, v3 pa_w, q4 qa, int tb_, v3 sb_, v3 pb_w, q4 qb, float gap_sum, float ma,
                     float mb, float* ra_out, float* rb_out, v3 pos[5], float dist[5], v3* normal_out) {
     cvx_geom ga = {ta_, sa_}, gb = {tb_, sb_};
@@ -632,7 +710,7 @@ static int cvx_pair(int ta_, v3 sa_, v3 pa_w, q4 qa, int tb_, v3 sb_, v3 pb_w, q
     if (ta_ == GEO_SPHERE || ta_ == GEO_CAPSULE) { ra = sa_.x; ga.scale.x = 0.0001f; }
     if (tb_ == GEO_SPHERE || tb_ == GEO_CAPSULE) { rb = sb_.x; gb.scale.x = 0.0001f; }
     const float threshold = gap_sum + ra + rb + ma + mb;
-    const int skip_multi = ta_ == GEO_SPHERE || tb_ == GEO_SPHERE;
+    const int skip_multi = ta_ == GEO_SPHERE || tb_ == GEO_SPHERE || ta_ == GEO_ELLIPSOID || tb_ == GEO_ELLIPSOID;
     q4 qr = qmul(qconj4(qa), qb);
     v3 pr = qrot_inv(qa, sub(pb_w, pa_w));
     const float enlarge = 1e-4f;
@@ -662,6 +740,7 @@ static int cvx_pair(int ta_, v3 sa_, v3 pa_w, q4 qa, int tb_, v3 sb_, v3 pb_w, q
             dist[i] = dist[i] - rb;
         }
     }
+    cvx_axial_rolling(ta_, sa_, pa_w, qa, tb_, sb_, pb_w, qb, nw, pos, count);
     *ra_out = ra; *rb_out = rb; *normal_out = nw;
     return count;
 }

@@ -32,6 +32,7 @@
 #define GEO_ELLIPSOID 5
 #define GEO_CYLINDER 6
 #define GEO_BOX 7
+#define GEO_CONE 9
 #define CYL_FLAT_COS 0.9238795325112867f /* cos(22.5 deg), primitive.py:31-34 */
 #define MAXVAL 1e10f
 #define MINVAL 1e-15f
@@ -127,6 +128,20 @@ static void shape_aabb(int type, v3 pos, q4 q, v3 scale, float radius, float eff
         h = mk(scale.x * sqrtf(fmaxf(1.0f - a.x * a.x, 0.0f)) + scale.y * fabsf(a.x),
                scale.x * sqrtf(fmaxf(1.0f - a.y * a.y, 0.0f)) + scale.y * fabsf(a.y),
                scale.x * sqrtf(fmaxf(1.0f - a.z * a.z, 0.0f)) + scale.y * fabsf(a.z));
+    } else if (type == GEO_CONE) {
+        /* apex a_w = +h along the axis, base rim radius r at -h: along world axis i the extremes
+           are +-h a_i (apex / base centre) and -h a_i +- r sqrt(1 - a_i^2) (rim) */
+        v3 a = qrot(q, mk(0.0f, 0.0f, 1.0f));
+        float av[3] = {a.x, a.y, a.z}, lo3[3], hi3[3];
+        for (int i = 0; i < 3; ++i) {
+            const float rim = scale.x * sqrtf(fmaxf(1.0f - av[i] * av[i], 0.0f));
+            const float ap = scale.y * av[i], bc = -scale.y * av[i];
+            hi3[i] = fmaxf(ap, bc + rim);
+            lo3[i] = fminf(ap, bc - rim);
+        }
+        *lo = sub(add(pos, mk(lo3[0], lo3[1], lo3[2])), mk(eff_gap, eff_gap, eff_gap));
+        *hi = add(add(pos, mk(hi3[0], hi3[1], hi3[2])), mk(eff_gap, eff_gap, eff_gap));
+        return;
     } else {
         h = mk(radius, radius, radius);
     }
@@ -382,6 +397,20 @@ static void sphere_box(v3 sp, float r, v3 bpos, q4 bq, v3 size, float* dist, v3*
 
 #include "convex.h"
 
+/* pairs (sorted types) the primitive chain leaves to the MPR / GJK path (narrow_phase.py:573-575) */
+static int cvx_type(int t) {
+    return t == GEO_SPHERE || t == GEO_CAPSULE || t == GEO_ELLIPSOID || t == GEO_CYLINDER || t == GEO_BOX ||
+           t == GEO_CONE;
+}
+static int cvx_routed(int ta, int tb) { return (cvx_type(ta) && cvx_type(tb)) || (ta == GEO_PLANE && tb == GEO_CONE); }
+
+/* Support point of a shape in its own frame (a test hook for the support maps). */
+void sapx_oracle_support(int type, const float* scale, const float* d, float* out) {
+    cvx_geom g = {type, mk(scale[0], scale[1], scale[2])};
+    v3 p = cvx_support(&g, mk(d[0], d[1], d[2]));
+    out[0] = p.x; out[1] = p.y; out[2] = p.z;
+}
+
 /* ------------------------------------------------------------ gap check */
 static void contact_points(v3 center, v3 n_in, float d, float ra, float rb, v3* xa, v3* xb, v3* n) {
     *n = normalize(n_in);
@@ -410,9 +439,8 @@ static int passes_gap_not(v3 center, v3 n_in, float d, float ra, float rb, float
  * *unsupported = 1.
  */
 static int narrow_pair(int sa, int sb, const int* shape_type, const float* shape_scale, const tf* X,
-                       const float* shape_margin, float dist[5], v3 pos[5], v3* normal, float* ra_out,
-                       float* rb_out, int* shape_a_out, int* shape_b_out, float gap_sum, int* unsupported,
-                       int* convex) {
+                       const float* shape_margin, const float* aabb, float dist[5], v3 pos[5], v3* normal, float* ra_out, float* rb_out, int* shape_a_out, int* shape_b_out,
+                       float gap_sum, int* unsupported, int* convex) {
     int ta = shape_type[sa], tb = shape_type[sb];
     if (ta > tb) { int t = sa; sa = sb; sb = t; t = ta; ta = tb; tb = t; }
     *shape_a_out = sa; *shape_b_out = sb;
@@ -462,11 +490,35 @@ static int narrow_pair(int sa, int sb, const int* shape_type, const float* shape
     } else if (ta == GEO_SPHERE && tb == GEO_CYLINDER) {
         v3 ax = qrot(Xb.q, ez);
         sphere_cylinder(Xa.p, scale_a.x, Xb.p, ax, scale_b.x, scale_b.y, &dist[0], &pos[0], normal);
-    } else if ((ta == GEO_CAPSULE || ta == GEO_BOX) && tb == GEO_BOX) {
+    } else if (cvx_routed(ta, tb)) {
         /* every contact of a convex pair is kept, whatever its distance (the
            writer's own gap check decides) */
         *convex = 1;
-        int n = cvx_pair(ta, scale_a, Xa.p, Xa.q, tb, scale_b, Xb.p, Xb.q, gap_sum, shape_margin[sa],
+        int tta = ta;
+        v3 ssa = scale_a, ppa = Xa.p;
+        if (ta == GEO_PLANE) {
+            /* only an infinite plane (zero half extents) becomes a box proxy (core.py:680-697) */
+            if (scale_a.x != 0.0f || scale_a.y != 0.0f) {
+                *unsupported = 1;
+                return 0;
+            }
+            /* the other shape's bounding sphere (centre and half diagonal of its collision AABB,
+               narrow_phase.py:661-720) must reach the plane's half-space; the plane becomes a box
+               of half extent 10 (R + gap) whose top face lies on the plane under the other shape
+               (core.py:507-566) */
+            const float* B = aabb + 6 * sb;
+            const v3 lo = mk(B[0], B[1], B[2]), hi = mk(B[3], B[4], B[5]);
+            const v3 c = scl(add(lo, hi), 0.5f);
+            const float R = length(scl(sub(hi, lo), 0.5f));
+            const v3 n = qrot(Xa.q, ez);
+            if (dot(sub(c, Xa.p), n) > R) return 0;
+            const float ext = (R + gap_sum) * 10.0f;
+            const v3 surf = sub(Xb.p, scl(n, dot(sub(Xb.p, Xa.p), n)));
+            ppa = sub(surf, scl(n, ext));
+            tta = GEO_BOX;
+            ssa = mk(ext, ext, ext);
+        }
+        int n = cvx_pair(tta, ssa, ppa, Xa.q, tb, scale_b, Xb.p, Xb.q, gap_sum, shape_margin[sa],
                          shape_margin[sb], ra_out, rb_out, pos, dist, normal);
         return n;
     } else {
@@ -506,8 +558,8 @@ static int collide_world(int p_begin, int p_end, const float* body_q, const int*
         float ra, rb;
         int sa, sb, unsup, convex;
         float gap_sum = shape_gap[s1] + shape_gap[s2];
-        int nc = narrow_pair(s1, s2, shape_type, shape_scale, scratch_tf, shape_margin, dist, pos, &normal, &ra,
-                             &rb, &sa, &sb, gap_sum, &unsup, &convex);
+        int nc = narrow_pair(s1, s2, shape_type, shape_scale, scratch_tf, shape_margin, scratch_aabb, dist, pos,
+                             &normal, &ra, &rb, &sa, &sb, gap_sum, &unsup, &convex);
         *unsupported_total += unsup;
         if (nc == 0) continue;
         float ma = shape_margin[sa], mb = shape_margin[sb];

@@ -79,6 +79,7 @@ struct SapxModel {
     DevModel dm;
     std::vector<void*> owned;
     std::vector<int> h_shape_world, h_shape_type;  // host copies for pipeline set-up
+    std::vector<float> h_shape_scale;
     int32_t world_count = 0, joint_count = 0, dof_count = 0, coord_count = 0, body_count = 0, shape_count = 0;
     ~SapxModel() {
         for (void* p : owned) (void)hipFree(p);
@@ -161,21 +162,14 @@ struct SapxPipeline {
 };
 
 // shape-type pairs (sorted, ta <= tb) the collide kernel handles: the analytic primitives of
-// narrow_phase.py:304-480 and the MPR / GJK path for capsule-box and box-box
-static bool pair_supported(int ta, int tb) {
-    switch (ta) {
-        case kGeoPlane:
-            return tb == kGeoSphere || tb == kGeoCapsule || tb == kGeoBox || tb == kGeoEllipsoid || tb == kGeoCylinder;
-        case kGeoSphere:
-            return tb == kGeoSphere || tb == kGeoCapsule || tb == kGeoBox || tb == kGeoCylinder;
-        case kGeoCapsule:
-            return tb == kGeoCapsule || tb == kGeoBox;
-        case kGeoBox:
-            return tb == kGeoBox;
-        default:
-            return false;
-    }
+// narrow_phase.py:304-567 and the MPR / GJK path for the other support-mapped pairs; a plane
+// reaches the MPR / GJK path only when it is infinite (zero half extents, core.py:680-697)
+static bool pair_supported(int ta, int tb, bool plane_infinite) {
+    if (pair_primitive(ta, tb)) return true;
+    if (!pair_convex(ta, tb)) return false;
+    return ta != kGeoPlane || plane_infinite;
 }
+static bool plane_infinite(const float* scale) { return scale[0] == 0.0f && scale[1] == 0.0f; }
 
 // Several solvers (world kinds) stepped by one launch per contact variant.  Each bound slot holds
 // the kernel parameters of every group for one set of state / control / contact views.
@@ -414,8 +408,7 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
         if (d->shape_world[s] != s / S) return fail(SAPX_ERR_UNSUPPORTED, "shapes must be grouped by world");
     for (int s = 0; s < Stot; ++s) {
         const int t = d->shape_type[s];
-        if (t != kGeoPlane && t != kGeoSphere && t != kGeoCapsule && t != kGeoBox && t != kGeoEllipsoid &&
-            t != kGeoCylinder)
+        if (t != kGeoPlane && !support_type(t))
             return fail(SAPX_ERR_UNSUPPORTED, "shape type " + std::to_string(t) + " is not in the native narrowphase");
     }
     // pair template: pairs grouped by world (stable), encoded local / global
@@ -446,6 +439,7 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     DevModel& dm = m->dm;
     m->h_shape_world.assign(d->shape_world, d->shape_world + Stot);
     m->h_shape_type.assign(d->shape_type, d->shape_type + Stot);
+    m->h_shape_scale.assign(d->shape_scale, d->shape_scale + 3 * (size_t)Stot);
     dm.W = W; dm.J = J; dm.N = N; dm.NQ = NQ; dm.NB = NB; dm.S = S;
     dm.P = (int)per_world[0].size() / 2;
     dm.num_levels = nlev;
@@ -454,14 +448,15 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     dm.has_convex = 0;
     for (size_t k = 0; k + 1 < per_world[0].size(); k += 2) {
         const int ea = per_world[0][k], eb = per_world[0][k + 1];
-        int ta = d->shape_type[ea >= 0 ? ea : -ea - 1], tb = d->shape_type[eb >= 0 ? eb : -eb - 1];
-        if (ta > tb) std::swap(ta, tb);
-        if (!pair_supported(ta, tb)) {
+        int sa = ea >= 0 ? ea : -ea - 1, sb = eb >= 0 ? eb : -eb - 1;
+        int ta = d->shape_type[sa], tb = d->shape_type[sb];
+        if (ta > tb) { std::swap(ta, tb); std::swap(sa, sb); }
+        if (!pair_supported(ta, tb, plane_infinite(d->shape_scale + 3 * (size_t)sa))) {
             delete m;
             return fail(SAPX_ERR_UNSUPPORTED, "candidate pair of shape types " + std::to_string(ta) + " and " +
                                                   std::to_string(tb) + " is not in the native narrowphase");
         }
-        if ((ta == kGeoCapsule || ta == kGeoBox) && tb == kGeoBox) dm.has_convex = 1;
+        if (pair_convex(ta, tb)) dm.has_convex = 1;
     }
     int32_t rc = SAPX_OK;
     auto& o = m->owned;
@@ -1133,10 +1128,11 @@ int32_t sapx_pipeline_create_all_pairs(const SapxModel* model, int32_t max_conta
     int has_convex = 0;
     for (size_t k = 0; k + 1 < tmpl.size(); k += 2) {
         const int ea = tmpl[k], eb = tmpl[k + 1];
-        int ta = st[ea >= 0 ? ea : -ea - 1], tb = st[eb >= 0 ? eb : -eb - 1];
-        if (ta > tb) std::swap(ta, tb);
-        if ((ta == kGeoCapsule || ta == kGeoBox) && tb == kGeoBox) has_convex = 1;
-        if (!pair_supported(ta, tb))
+        int sa = ea >= 0 ? ea : -ea - 1, sb = eb >= 0 ? eb : -eb - 1;
+        int ta = st[sa], tb = st[sb];
+        if (ta > tb) { std::swap(ta, tb); std::swap(sa, sb); }
+        if (pair_convex(ta, tb)) has_convex = 1;
+        if (!pair_supported(ta, tb, plane_infinite(model->h_shape_scale.data() + 3 * (size_t)sa)))
             return fail(SAPX_ERR_UNSUPPORTED, "candidate pair of shape types " + std::to_string(ta) + " and " +
                                                   std::to_string(tb) + " is not in the native narrowphase");
     }

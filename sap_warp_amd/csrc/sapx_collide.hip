@@ -72,6 +72,22 @@ __device__ void shape_aabb(int type, V3 pos, Q4 q, V3 scale, float radius, float
         h = v3(scale.x * sqrtf(fmaxf(1.0f - a.x * a.x, 0.0f)) + scale.y * fabsf(a.x),
                scale.x * sqrtf(fmaxf(1.0f - a.y * a.y, 0.0f)) + scale.y * fabsf(a.y),
                scale.x * sqrtf(fmaxf(1.0f - a.z * a.z, 0.0f)) + scale.y * fabsf(a.z));
+    } else if (type == kGeoCone) {
+        // apex at +h along the axis, base rim of radius r at -h: along world axis i the extremes
+        // are +-h a_i (apex / base centre) and -h a_i +- r sqrt(1 - a_i^2) (rim)
+        const V3 a = qrot(q, v3(0.0f, 0.0f, 1.0f));
+        const float av[3] = {a.x, a.y, a.z};
+        float lo3[3], hi3[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const float rim = scale.x * sqrtf(fmaxf(1.0f - av[i] * av[i], 0.0f));
+            const float ap = scale.y * av[i], bc = -scale.y * av[i];
+            hi3[i] = fmaxf(ap, bc + rim);
+            lo3[i] = fminf(ap, bc - rim);
+        }
+        *lo = (pos + v3(lo3[0], lo3[1], lo3[2])) - v3(eff, eff, eff);
+        *hi = (pos + v3(hi3[0], hi3[1], hi3[2])) + v3(eff, eff, eff);
+        return;
     } else {
         h = v3(radius, radius, radius);
     }
@@ -314,8 +330,8 @@ __host__ __device__ inline size_t collide_poly_offset(int ns, int nt, int C) {
     return (collide_stage_offset(ns, nt) + (size_t)C * 13 * 4 + 15) / 16 * 16;
 }
 
-// CVX: the model has capsule-box or box-box candidate pairs (MPR / GJK path, up to five
-// contacts per pair, a 16-slot polygon buffer per lane in LDS)
+// CVX: the model has candidate pairs on the MPR / GJK path (up to five contacts per pair, a
+// 16-slot polygon buffer per lane in LDS)
 template <int NT, bool CVX>
 __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
@@ -431,10 +447,38 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
                 } else if (ta == kGeoSphere && tb == kGeoCylinder) {
                     sphere_cylinder(Xa.p, scale_a.x, Xb.p, qrot(Xb.q, ez), scale_b.x, scale_b.y, &dist[0], &pos[0],
                                     &normal);
-                } else if (CVX && (ta == kGeoCapsule || ta == kGeoBox) && tb == kGeoBox) {
-                    const cvx::Poly buf{poly + tid, NT};
-                    ncvx = cvx::pair(ta, scale_a, Xa.p, Xa.q, tb, scale_b, Xb.p, Xb.q, gap_sum, M.shape_margin[ga],
-                                     M.shape_margin[gb], buf, &ra, &rb, pos, dist, &normal);
+                } else if (CVX && pair_convex(ta, tb)) {
+                    int tta = ta;
+                    V3 ssa = scale_a, ppa = Xa.p;
+                    bool reach = true;
+                    if (ta == kGeoPlane) {
+                        // the other shape's bounding sphere (centre and half diagonal of its collision
+                        // AABB) must reach the infinite plane's half-space; the plane becomes a box of
+                        // half extent 10 (R + gap) whose top face lies on the plane under the shape
+                        const float* Bb = AB + lb * 6;
+                        const V3 blo = v3(Bb[0], Bb[1], Bb[2]), bhi = v3(Bb[3], Bb[4], Bb[5]);
+                        const V3 c = (blo + bhi) * 0.5f;
+                        const float R = length((bhi - blo) * 0.5f);
+                        const V3 nn = qrot(Xa.q, ez);
+                        if (scale_a.x != 0.0f || scale_a.y != 0.0f) {
+                            atomicAdd(P.unsupported, 1);
+                            reach = false;
+                        } else if (dot(c - Xa.p, nn) > R) {
+                            reach = false;
+                        } else {
+                            const float ext = (R + gap_sum) * 10.0f;
+                            const V3 surf = Xb.p - nn * dot(Xb.p - Xa.p, nn);
+                            ppa = surf - nn * ext;
+                            tta = kGeoBox;
+                            ssa = v3(ext, ext, ext);
+                        }
+                    }
+                    ncvx = 0;
+                    if (reach) {
+                        const cvx::Poly buf{poly + tid, NT};
+                        ncvx = cvx::pair(tta, ssa, ppa, Xa.q, tb, scale_b, Xb.p, Xb.q, gap_sum, M.shape_margin[ga],
+                                         M.shape_margin[gb], buf, &ra, &rb, pos, dist, &normal);
+                    }
                 } else {
                     atomicAdd(P.unsupported, 1);
                 }
@@ -558,7 +602,9 @@ __global__ __launch_bounds__(kPlaceThreads) void sapx_place_kernel(PlaceParams P
         }
         if (lane < kPlaceWorlds) incl_s[lane] = incl;
         if (lane < kPlaceWorlds && w < P.W) P.world_start[w] = min(base + incl - x, cap);
-        if (w == P.W - 1) {
+        // only a lane of this range closes the list: lanes kPlaceWorlds..63 carry the next range's
+        // world ids with a zero count, and one of them may be the last world
+        if (lane < kPlaceWorlds && w == P.W - 1) {
             P.world_start[P.W] = min(base + incl, cap);
             P.count[0] = min(base + incl, cap);
         }

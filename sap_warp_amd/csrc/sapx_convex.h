@@ -1,5 +1,6 @@
-// Convex-pair contacts (capsule-box, box-box) on gfx950: fused MPR + GJK and the
-// tilted-probe contact manifold, one lane per candidate pair.
+// Contacts of the pairs the analytic routines leave (two support-mapped shapes among sphere,
+// capsule, ellipsoid, cylinder, box, cone; an infinite plane against a cone enters as a box) on
+// gfx950: fused MPR + GJK and the tilted-probe contact manifold, one lane per candidate pair.
 //
 // Included by sapx_collide.hip (compiled with -ffp-contract=off and IEEE
 // float32 division / square root) so the results are bit-identical to the CPU
@@ -12,8 +13,9 @@
 // refinement, 30 iterations, epsilon 1e-5), simplex.py:39-436 (GJK distance,
 // 30 iterations, epsilon 1e-4, Johnson sub-simplex selection with the 1e-8
 // degeneracy guard), multicontact.py:20-955 (manifold), support.py:94-276
-// (support maps), math.py:20-44 (tangent basis), core.py:143-169 (Minkowski
-// post-process).
+// (support maps), math.py:20-44 (tangent basis), core.py:172-275 (Minkowski
+// post-process and the rolling projection of cylinders and cones), core.py:360-378
+// (one contact for sphere and ellipsoid pairs).
 //
 // Device layout: the simplex (4 Minkowski vertices, barycentric weights) lives
 // in registers, indexed through unrolled selects; the manifold's 2D polygon
@@ -61,6 +63,39 @@ __device__ __forceinline__ V3 support(const Geom& g, V3 d) {
         V3 r = n * g.scale.x;
         if (g.type == kGeoCapsule) r = r + v3(0.0f, 0.0f, d.z >= 0.0f ? g.scale.y : -g.scale.y);
         return r;
+    }
+    if (g.type == kGeoEllipsoid) {
+        const float a = g.scale.x, b = g.scale.y, c = g.scale.z;
+        if (len_sq(d) > 1e-12f) {
+            const float adx = a * d.x, bdy = b * d.y, cdz = c * d.z;
+            const float den2 = adx * adx + bdy * bdy + cdz * cdz;
+            if (den2 > 1e-12f) {
+                const float den = sqrtf(den2);
+                return v3((a * a) * d.x / den, (b * b) * d.y / den, (c * c) * d.z / den);
+            }
+        }
+        return v3(a, 0.0f, 0.0f);
+    }
+    if (g.type == kGeoCylinder) {
+        const float r = g.scale.x, hh = g.scale.y;
+        V3 lat = v3(r, 0.0f, 0.0f);
+        const V3 dxy = v3(d.x, d.y, 0.0f);
+        if (len_sq(dxy) > 1e-12f) {
+            const V3 nxy = nrm(dxy);
+            lat = v3(nxy.x * r, nxy.y * r, 0.0f);
+        }
+        if (d.z > 0.0f) return v3(lat.x, lat.y, hh);
+        if (d.z < 0.0f) return v3(lat.x, lat.y, -hh);
+        return lat;
+    }
+    if (g.type == kGeoCone) {
+        // apex at +hh, base disk at -hh: the apex when dz >= r / (2 hh) |d_xy|
+        const float r = g.scale.x, hh = g.scale.y;
+        const float lxy = length(v3(d.x, d.y, 0.0f));
+        const float k = hh > 1e-12f ? r / (2.0f * hh) : 0.0f;
+        if (lxy <= 1e-12f) return d.z >= 0.0f ? v3(0.0f, 0.0f, hh) : v3(r, 0.0f, -hh);
+        if (d.z >= k * lxy) return v3(0.0f, 0.0f, hh);
+        return v3(d.x / lxy * r, d.y / lxy * r, -hh);
     }
     return v3(0.0f, 0.0f, 0.0f);
 }
@@ -674,6 +709,44 @@ __device__ int manifold(const Geom& ga, const Geom& gb, Q4 qa, V3 pa_w, Q4 qr, V
     return count;
 }
 
+// A cylinder or cone against a box (or an infinite plane's box proxy) in rolling contact: the
+// contact points move onto the plane through the shape's centre spanned by its axis and the
+// normal.  A cylinder rolls when its axis is within 2 degrees of perpendicular to the normal, a
+// cone when the angle between axis and normal is within 2 degrees of its half angle atan2(r, 2 hh);
+// sin(half angle +- 2 deg) is formed from the half angle's sine and cosine (r and 2 hh over their
+// hypotenuse).
+constexpr float kSin2 = 0.034899496702500969f;
+constexpr float kCos2 = 0.99939082701909576f;
+__device__ void axial_rolling(int ta_, V3 sa_, V3 pa_w, Q4 qa, int tb_, V3 sb_, V3 pb_w, Q4 qb, V3 nw, V3 pos[5],
+                              int count) {
+    const bool disc_a = ta_ == kGeoBox, disc_b = tb_ == kGeoBox;
+    const bool ax_a = ta_ == kGeoCylinder || ta_ == kGeoCone, ax_b = tb_ == kGeoCylinder || tb_ == kGeoCone;
+    if (!((disc_a && ax_b) || (disc_b && ax_a))) return;
+    V3 axis, spos, an;
+    float r, hh;
+    bool cone;
+    if (disc_a && ax_b) {
+        axis = qrot(qb, v3(0.0f, 0.0f, 1.0f)); r = sb_.x; hh = sb_.y; cone = tb_ == kGeoCone; spos = pb_w; an = nw;
+    } else {
+        axis = qrot(qa, v3(0.0f, 0.0f, 1.0f)); r = sa_.x; hh = sa_.y; cone = ta_ == kGeoCone; spos = pa_w; an = -nw;
+    }
+    const float adot = fabsf(dot(axis, an));
+    bool rolling;
+    if (cone) {
+        const float hyp = sqrtf(r * r + 4.0f * hh * hh);
+        const float sh = r / hyp, ch = (2.0f * hh) / hyp;
+        rolling = adot >= sh * kCos2 - ch * kSin2 && adot <= sh * kCos2 + ch * kSin2;
+    } else {
+        rolling = adot <= kSin2;
+    }
+    if (!rolling) return;
+    const V3 pn = nrm(cross(axis, an));
+    for (int i = 0; i < 5; ++i) {
+        if (i >= count) break;
+        pos[i] = pos[i] - pn * dot(pos[i] - spos, pn);
+    }
+}
+
 // Pair driver (type_a <= type_b): up to five contact centres and signed distances after
 // the Minkowski radius shift, the shared world normal and the effective radii.
 __device__ int pair(int ta_, V3 sa_, V3 pa_w, Q4 qa, int tb_, V3 sb_, V3 pb_w, Q4 qb, float gap_sum, float ma,
@@ -683,7 +756,8 @@ __device__ int pair(int ta_, V3 sa_, V3 pa_w, Q4 qa, int tb_, V3 sb_, V3 pb_w, Q
     if (ta_ == kGeoSphere || ta_ == kGeoCapsule) { ra = sa_.x; ga.scale.x = 0.0001f; }
     if (tb_ == kGeoSphere || tb_ == kGeoCapsule) { rb = sb_.x; gb.scale.x = 0.0001f; }
     const float threshold = gap_sum + ra + rb + ma + mb;
-    const bool skip_multi = ta_ == kGeoSphere || tb_ == kGeoSphere;
+    const bool skip_multi =
+        ta_ == kGeoSphere || tb_ == kGeoSphere || ta_ == kGeoEllipsoid || tb_ == kGeoEllipsoid;
     Q4 qr = qmul(qconj(qa), qb);
     V3 pr = qrot_inv(qa, pb_w - pa_w);
     const float enlarge = 1e-4f;
@@ -714,6 +788,7 @@ __device__ int pair(int ta_, V3 sa_, V3 pa_w, Q4 qa, int tb_, V3 sb_, V3 pb_w, Q
             dist[i] = dist[i] - rb;
         }
     }
+    axial_rolling(ta_, sa_, pa_w, qa, tb_, sb_, pb_w, qb, nw, pos, count);
     *ra_out = ra;
     *rb_out = rb;
     *normal_out = nw;

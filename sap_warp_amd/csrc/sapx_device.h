@@ -24,6 +24,25 @@ constexpr int kGeoCapsule = 4;
 constexpr int kGeoEllipsoid = 5;
 constexpr int kGeoCylinder = 6;
 constexpr int kGeoBox = 7;
+constexpr int kGeoCone = 9;
+
+// Shape-type pairs (sorted, ta <= tb) with an analytic contact routine (narrow_phase.py:304-567).
+__host__ __device__ inline bool pair_primitive(int ta, int tb) {
+    if (ta == kGeoPlane)
+        return tb == kGeoSphere || tb == kGeoCapsule || tb == kGeoBox || tb == kGeoEllipsoid || tb == kGeoCylinder;
+    if (ta == kGeoSphere) return tb == kGeoSphere || tb == kGeoCapsule || tb == kGeoBox || tb == kGeoCylinder;
+    return ta == kGeoCapsule && tb == kGeoCapsule;
+}
+// Shapes with a support map (support.py:94-258).
+__host__ __device__ inline bool support_type(int t) {
+    return t == kGeoSphere || t == kGeoCapsule || t == kGeoEllipsoid || t == kGeoCylinder || t == kGeoBox ||
+           t == kGeoCone;
+}
+// Sorted pairs the primitive routines leave to the MPR / GJK path (narrow_phase.py:570-575): two
+// support-mapped shapes, or an infinite plane (turned into a box) against a cone.
+__host__ __device__ inline bool pair_convex(int ta, int tb) {
+    return !pair_primitive(ta, tb) && ((support_type(ta) && support_type(tb)) || (ta == kGeoPlane && tb == kGeoCone));
+}
 
 constexpr int kMaxTemplate = 64;  // joints / bodies per world (64-bit masks)
 constexpr int kMaxDof = 64;       // dofs per world (one wavefront per row set)

@@ -51,6 +51,7 @@ GEO_CAPSULE = 4
 GEO_ELLIPSOID = 5
 GEO_CYLINDER = 6
 GEO_BOX = 7
+GEO_CONE = 9
 
 # Shape flags (sim/collision/flags.py:12-19)
 SHAPE_FLAG_VISIBLE = 1 << 0
@@ -298,4 +299,5 @@ __all__ = [
     "GEO_ELLIPSOID",
     "GEO_CYLINDER",
     "GEO_BOX",
+    "GEO_CONE",
 ]

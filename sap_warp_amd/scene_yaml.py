@@ -2,7 +2,8 @@
 
 Reads the reference's scene schema (version 1) for scenes described inline —
 ``simulation``, ``builder`` (gravity, rigid gap, default body armature, joint
-and shape defaults), ``bodies`` with box / sphere / capsule shapes, ``joints``
+and shape defaults), ``bodies`` with box / sphere / capsule / cylinder / cone /
+ellipsoid shapes, ``joints``
 (fixed, free, revolute, prismatic), ``articulations``, ``initial_joint_q``,
 ``post_build`` operations, ``replicate`` and ``ground`` — and builds the model
 with ``SceneBuilder`` (``sim/loader/scene.py:1372-1509, 5100-5780``).  Asset
@@ -24,9 +25,9 @@ from dataclasses import dataclass, field, fields
 
 import numpy as np
 
-from .runtime import (GEO_BOX, GEO_CAPSULE, GEO_SPHERE, SAP_JOINT_FIXED, SAP_JOINT_FREE, SAP_JOINT_PRISMATIC,
-                      SAP_JOINT_REVOLUTE, SHAPE_FLAG_COLLIDE_PARTICLES, SHAPE_FLAG_COLLIDE_SHAPES,
-                      SHAPE_FLAG_VISIBLE)
+from .runtime import (GEO_BOX, GEO_CAPSULE, GEO_CONE, GEO_CYLINDER, GEO_ELLIPSOID, GEO_SPHERE, SAP_JOINT_FIXED,
+                      SAP_JOINT_FREE, SAP_JOINT_PRISMATIC, SAP_JOINT_REVOLUTE, SHAPE_FLAG_COLLIDE_PARTICLES,
+                      SHAPE_FLAG_COLLIDE_SHAPES, SHAPE_FLAG_VISIBLE)
 from .scenes import DofCfg, SceneBuilder, ShapeCfg, _with_consistent_body_q, quat_from_axis_angle
 
 SCHEMA_VERSION = 1
@@ -252,9 +253,13 @@ class _Build:
             return self.b.add_shape(body, GEO_BOX, x, scale, sc)
         if kind == "sphere":
             return self.b.add_shape(body, GEO_SPHERE, x, (number(cfg.get("radius", 1.0)), 0.0, 0.0), sc)
-        if kind == "capsule":
+        if kind in ("capsule", "cylinder", "cone"):
             scale = (number(cfg.get("radius", 1.0)), number(cfg.get("half_height", 0.5)), 0.0)
-            return self.b.add_shape(body, GEO_CAPSULE, x, scale, sc)
+            typ = {"capsule": GEO_CAPSULE, "cylinder": GEO_CYLINDER, "cone": GEO_CONE}[kind]
+            return self.b.add_shape(body, typ, x, scale, sc)
+        if kind == "ellipsoid":
+            scale = (number(cfg.get("a", 1.0)), number(cfg.get("b", 0.75)), number(cfg.get("c", 0.5)))
+            return self.b.add_shape(body, GEO_ELLIPSOID, x, scale, sc)
         raise SceneLoaderError(f"Unsupported inline shape type: {kind}")
 
     def body_ref(self, value) -> int:

@@ -17,6 +17,10 @@ Scenes:
 * :func:`build_humanoid_scene` — the 35-DOF humanoid batch (free pelvis + 29
   revolute joints, 30 bodies, capsule/sphere/box links) used as the GPU
   workload.
+* :func:`build_box_pile_scene`, :func:`build_round_shapes_scene`,
+  :func:`build_convex_shapes_scene`, :func:`build_ball_chain_scene` — parity
+  scenes for the box pairs, the round-shape primitives, the MPR / GJK pairs
+  (cones, cylinders, ellipsoids) and ball-joint chains.
 
 Worlds are independent (no cross-world pairs), so by default every world sits
 at the origin (``spacing=0``): positions stay O(1 m) and float32 geometry keeps
@@ -33,6 +37,7 @@ import numpy as np
 from .runtime import (
     GEO_BOX,
     GEO_CAPSULE,
+    GEO_CONE,
     GEO_CYLINDER,
     GEO_ELLIPSOID,
     GEO_PLANE,
@@ -158,6 +163,15 @@ def inertia_ellipsoid(density, rx, ry, rz):
                           0.2 * mass * (rx * rx + ry * ry)])
 
 
+def inertia_cone(density, r, half_height):
+    """Solid cone, apex at +half_height: mass, centre of mass (0, 0, -h/4) for the height h, and
+    the inertia about that centre (``_sap_compute_inertia_cone`` scene.py:5878-5884)."""
+    h = 2.0 * half_height
+    mass = density * math.pi * r * r * h / 3.0
+    ixy = 3.0 / 20.0 * mass * r * r + 3.0 / 80.0 * mass * h * h
+    return mass, np.array([0.0, 0.0, -h / 4.0]), np.diag([ixy, ixy, 3.0 / 10.0 * mass * r * r])
+
+
 def shape_volume(shape_type, scale) -> float:
     if shape_type == GEO_SPHERE:
         return 4.0 / 3.0 * math.pi * scale[0] ** 3
@@ -170,6 +184,8 @@ def shape_volume(shape_type, scale) -> float:
         return math.pi * scale[0] ** 2 * 2.0 * scale[1]
     if shape_type == GEO_ELLIPSOID:
         return 4.0 / 3.0 * math.pi * scale[0] * scale[1] * scale[2]
+    if shape_type == GEO_CONE:
+        return math.pi * scale[0] ** 2 * 2.0 * scale[1] / 3.0
     return 0.0
 
 
@@ -186,7 +202,7 @@ def shape_radius(shape_type, scale) -> float:
         return float(scale[0])
     if shape_type == GEO_BOX:
         return float(np.linalg.norm(scale))
-    if shape_type in (GEO_CAPSULE, GEO_CYLINDER):
+    if shape_type in (GEO_CAPSULE, GEO_CYLINDER, GEO_CONE):
         return float(scale[0] + scale[1])
     if shape_type == GEO_ELLIPSOID:
         return float(max(scale[0], scale[1], scale[2]))
@@ -378,11 +394,16 @@ class SceneBuilder:
                 m, inertia = inertia_cylinder(cfg.density, scale[0], scale[1])
             elif shape_type == GEO_ELLIPSOID:
                 m, inertia = inertia_ellipsoid(cfg.density, *scale)
+            elif shape_type == GEO_CONE:
+                m, com, inertia = inertia_cone(cfg.density, scale[0], scale[1])
             else:
                 m, inertia = 0.0, np.zeros((3, 3))
             if m > 0.0:
                 x = self.shape_transform[sid]
-                self._update_body_mass(body, m, inertia, x[:3].copy(), x[3:])
+                p = x[:3].copy()
+                if shape_type == GEO_CONE:  # the shape's centre of mass in body coordinates
+                    p = p + quat_rotate(x[3:], com)
+                self._update_body_mass(body, m, inertia, p, x[3:])
         return sid
 
     def add_ground_plane(self, cfg: ShapeCfg, height: float = 0.0) -> int:
@@ -1201,6 +1222,100 @@ def build_round_shapes_scene(
     return _with_consistent_body_q(b.finalize(device))
 
 
+def build_convex_shapes_scene(
+    num_worlds: int = 64,
+    seed: int = 11,
+    *,
+    spacing: float = 0.0,
+    contact_gap: float = 0.01,
+    device: str = "cpu",
+) -> SapModel:
+    """Cones, cylinders, ellipsoids and a capsule resting on each other: the pairs the analytic
+    routines leave to MPR / GJK (``narrow_phase.py:570-747``), including an infinite plane against a
+    cone (box proxy, ``core.py:507-566``), the single-contact rule of sphere and ellipsoid pairs
+    (``core.py:360-378``) and the rolling projection of cylinders and cones on boxes and planes
+    (``core.py:172-275``).
+
+    Nine free bodies per world, in four collision groups:
+
+    * group 1: a box slab (half extents 0.2 x 0.2 x 0.05 m) on the ground, a cylinder lying on it
+      (axis within 1.5 degrees of horizontal: the rolling mode) and an upright cone standing on its
+      base on it (box-cylinder, box-cone, and cone-cylinder out of reach);
+    * group 2: a cone lying on its slant on the ground (plane-cone in the rolling mode);
+    * group 3: two ellipsoids of equal vertical semi-axis side by side on the ground, touching at
+      their equators (ellipsoid-ellipsoid), and a sphere resting on the first (sphere-ellipsoid);
+    * group 4: a cylinder lying on the ground and a capsule lying across it (capsule-cylinder).
+
+    Radii and half heights U(0.04, 0.08) m, ellipsoid semi-axes U(0.05, 0.1) m; every resting offset
+    is a penetration U(0, 3) mm; density U(400, 1000) kg/m^3, mu 0.6, stiffness 2e5 N/m, dissipation
+    5 ms; bodies start at rest.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = SceneBuilder(rigid_gap=contact_gap)
+    b.world_count = num_worlds
+
+    def cfg(group):
+        return ShapeCfg(density=rng.uniform(400.0, 1000.0), ke=2.0e5, tau=0.005, mu=0.6, gap=contact_gap,
+                        collision_group=group)
+
+    def pen():
+        return rng.uniform(0.0, 0.003)
+
+    def yaw_q(extra=None):
+        q = quat_from_axis_angle((0, 0, 1), rng.uniform(-math.pi, math.pi))
+        return q if extra is None else quat_mul(q, extra)
+
+    for w in range(num_worlds):
+        b.current_world = w
+        off = _grid_offset(w, num_worlds, spacing)
+        bodies = []
+        # group 1: slab, rolling cylinder and upright cone on it
+        c0 = off + np.array([0.0, 0.0, 0.05 - pen()])
+        bodies.append((tf(c0, yaw_q()), GEO_BOX, (0.2, 0.2, 0.05), 1))
+        top = c0[2] + 0.05
+        r, hh = rng.uniform(0.04, 0.08), rng.uniform(0.04, 0.08)
+        tilt = math.radians(rng.uniform(0.0, 1.5))
+        q = quat_mul(quat_from_axis_angle((0, 0, 1), rng.uniform(-0.3, 0.3)),
+                     quat_from_axis_angle((1, 0, 0), math.pi / 2 - tilt))
+        a = quat_rotate(q, (0.0, 0.0, 1.0))
+        low = r * math.sqrt(max(1.0 - a[2] * a[2], 0.0)) + hh * abs(a[2])
+        bodies.append((tf(c0 + np.array([0.0, -0.1, top - c0[2] + low - pen()]), q), GEO_CYLINDER, (r, hh, 0.0), 1))
+        r, hh = rng.uniform(0.04, 0.08), rng.uniform(0.04, 0.08)
+        bodies.append((tf(c0 + np.array([0.0, 0.1, top - c0[2] + hh - pen()]), yaw_q()), GEO_CONE, (r, hh, 0.0), 1))
+        # group 2: a cone lying on its slant on the ground: the axis points down by the half angle
+        r, hh = rng.uniform(0.04, 0.08), rng.uniform(0.04, 0.08)
+        alpha = math.atan2(r, 2.0 * hh)
+        q = yaw_q(quat_from_axis_angle((0, 1, 0), math.pi / 2 + alpha))
+        bodies.append((tf(off + np.array([0.45, 0.0, hh * math.sin(alpha) - pen()]), q), GEO_CONE, (r, hh, 0.0), 2))
+        # group 3: two ellipsoids side by side, a sphere against the first
+        e1 = tuple(rng.uniform(0.05, 0.1, size=3))
+        e2 = (rng.uniform(0.05, 0.1), rng.uniform(0.05, 0.1), e1[2])  # same height: equators touch
+        z = e1[2] - pen()
+        p1 = off + np.array([-0.45, 0.0, z])
+        p2 = off + np.array([-0.45, e1[1] + e2[1] - pen(), z])
+        bodies.append((tf(p1), GEO_ELLIPSOID, e1, 3))
+        bodies.append((tf(p2), GEO_ELLIPSOID, e2, 3))
+        rs = rng.uniform(0.03, 0.05)
+        bodies.append((tf(p1 + np.array([0.0, 0.0, e1[2] + rs - pen()])), GEO_SPHERE, (rs, 0.0, 0.0), 3))
+        # group 4: a cylinder on its side on the ground, a capsule across it
+        r, hh = rng.uniform(0.04, 0.08), rng.uniform(0.04, 0.08)
+        q = quat_from_axis_angle((1, 0, 0), math.pi / 2)
+        pc = off + np.array([0.0, 0.45, r - pen()])
+        bodies.append((tf(pc, q), GEO_CYLINDER, (r, hh, 0.0), 4))
+        rc, hc = rng.uniform(0.02, 0.04), rng.uniform(0.06, 0.1)
+        qc = quat_from_axis_angle((0, 1, 0), math.pi / 2 + rng.uniform(-0.2, 0.2))
+        bodies.append((tf(pc + np.array([0.0, 0.0, r + rc - pen()]), qc), GEO_CAPSULE, (rc, hc, 0.0), 4))
+        for pose, st, scale, group in bodies:
+            body = b.add_body(pose)
+            b.add_shape(body, st, tf(), scale, cfg(group))
+            j = b.add_free_joint(body)
+            b.add_articulation([j])
+            _set_free_joint_state(b, j, pose, np.zeros(6))
+    b.current_world = -1
+    b.add_ground_plane(ShapeCfg(density=0.0, ke=2.0e5, tau=0.005, mu=0.6, gap=contact_gap, collision_group=-1))
+    return _with_consistent_body_q(b.finalize(device))
+
+
 def build_ball_chain_scene(
     num_worlds: int = 64,
     seed: int = 7,
@@ -1264,4 +1379,4 @@ def _with_consistent_body_q(model: SapModel) -> SapModel:
 
 
 __all__ = ["SceneBuilder", "ShapeCfg", "DofCfg", "build_ant_scene", "build_ball_chain_scene", "build_box_pile_scene",
-           "build_humanoid_scene", "build_round_shapes_scene"]
+           "build_humanoid_scene", "build_round_shapes_scene", "build_convex_shapes_scene"]

@@ -18,6 +18,10 @@ def _scene(name, worlds, seed):
         from sap_warp_amd.scenes import build_round_shapes_scene
 
         return build_round_shapes_scene(worlds, seed=seed + 9, device="cuda:0")
+    if name == "convex":
+        from sap_warp_amd.scenes import build_convex_shapes_scene
+
+        return build_convex_shapes_scene(worlds, seed=seed + 11, device="cuda:0")
     if name == "humanoid_lying":
         return build_humanoid_scene(worlds, seed=seed, lying_fraction=1.0, contact_gap=0.05, device="cuda:0")
     return build_humanoid_scene(worlds, seed=seed, device="cuda:0")
@@ -76,7 +80,7 @@ def _check_equal(gpu, ref, C, model):
 
 
 @pytest.mark.parametrize("scene,worlds,C", [("ant", 16, 16), ("humanoid", 16, 64), ("boxpile", 32, 64),
-                                            ("round", 32, 16), ("humanoid_lying", 16, 64)])
+                                            ("round", 32, 16), ("humanoid_lying", 16, 64), ("convex", 32, 32)])
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_collide_bit_exact(gpu_device, scene, worlds, C, seed):
     import torch
@@ -90,7 +94,7 @@ def test_collide_bit_exact(gpu_device, scene, worlds, C, seed):
     for variant in range(2):
         if variant == 0:
             bq = model.body_q
-        elif scene in ("boxpile", "round"):
+        elif scene in ("boxpile", "round", "convex"):
             bq = _jittered_body_q(model, 100 + seed)
         else:
             bq = _perturbed_body_q(model, 100 + seed)
@@ -120,6 +124,57 @@ def test_collide_convex_pairs_present(gpu_device):
     t1 = st[contacts.rigid_contact_shape1.cpu().numpy()[:n]]
     kinds = set(zip(t0.tolist(), t1.tolist()))
     assert (7, 7) in kinds and (4, 7) in kinds, kinds
+
+
+def test_collide_convex_shape_pairs_present(gpu_device):
+    """The convex-shapes scene reaches every pair kind it is built for: plane-cone through the box
+    proxy, box-cone, box-cylinder, ellipsoid-ellipsoid, sphere-ellipsoid and capsule-cylinder on the
+    MPR / GJK path, plus the plane primitives."""
+    import torch
+
+    from sap_warp_amd.collision import SapCollisionPipeline
+
+    model = _scene("convex", 8, 0)
+    pipe = SapCollisionPipeline(model, max_contacts_per_world=32)
+    contacts = pipe.contacts()
+    pipe.collide(model.state(), contacts)
+    torch.cuda.synchronize()
+    n = int(contacts.rigid_contact_count.cpu()[0])
+    st = np.asarray(model.shape_type)
+    t0 = st[contacts.rigid_contact_shape0.cpu().numpy()[:n]]
+    t1 = st[contacts.rigid_contact_shape1.cpu().numpy()[:n]]
+    kinds = set(zip(t0.tolist(), t1.tolist()))
+    for k in ((1, 9), (7, 9), (6, 7), (5, 5), (3, 5), (4, 6), (1, 5), (1, 6), (1, 7)):
+        assert k in kinds, (k, kinds)
+
+
+@pytest.mark.parametrize("worlds", [33, 64, 96, 100, 4096])
+def test_collide_count_closes_list(gpu_device, worlds):
+    """count and world_start[W] equal the total of the per-world counts on every call, for world
+    counts whose last world also falls in the upper half of the previous placement workgroup's
+    wave (a regression test: those lanes once raced the owning workgroup's write of the total)."""
+    import torch
+
+    from sap_warp_amd.collision import SapCollisionPipeline
+
+    model = _scene("humanoid", worlds, 5)
+    pipe = SapCollisionPipeline(model, max_contacts_per_world=64)
+    contacts = pipe.contacts()
+    state = model.state()
+    ref = oracle_contacts_dict(model, model.body_q, 64) if worlds <= 128 else None
+    for rep in range(20):
+        pipe.collide(state, contacts)
+        torch.cuda.synchronize()
+        ws = contacts.rigid_contact_world_start.cpu().numpy()
+        count = int(contacts.rigid_contact_count.cpu()[0])
+        assert count == ws[-1], rep
+        assert np.all(np.diff(ws) >= 0), rep
+        if ref is not None:
+            np.testing.assert_array_equal(ws, ref["world_start"])
+        elif rep == 0:
+            first = ws.copy()
+        else:
+            np.testing.assert_array_equal(ws, first)
 
 
 def test_collide_overflow_rule(gpu_device):

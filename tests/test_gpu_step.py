@@ -55,6 +55,10 @@ def _scene(name, worlds, seed):
         from sap_warp_amd.scenes import build_round_shapes_scene
 
         return build_round_shapes_scene(worlds, seed=seed + 9, device="cuda:0"), 2.0e-3, 16
+    if name == "convex":
+        from sap_warp_amd.scenes import build_convex_shapes_scene
+
+        return build_convex_shapes_scene(worlds, seed=seed + 11, device="cuda:0"), 2.0e-3, 32
     if name == "ant":
         return build_ant_scene(worlds, seed=seed, device="cuda:0"), 1.0e-3, 16
     if name == "boxpile":
@@ -351,6 +355,13 @@ def test_box_pile_step_parity(gpu_device, preset, ls):
 def test_round_shapes_step_parity(gpu_device, preset, ls):
     """Free cylinders, ellipsoids and spheres (plane-cylinder, plane-ellipsoid, sphere-cylinder)."""
     _run_and_compare("round", preset, ls, worlds=8, steps=4)
+
+
+@pytest.mark.parametrize("preset,ls", [("approx32", "exact_root"), ("drake", "armijo_decay")])
+def test_convex_shapes_step_parity(gpu_device, preset, ls):
+    """Cones, cylinders, ellipsoids and a capsule on the MPR / GJK path (plane-cone through the box
+    proxy, box-cone, rolling box-cylinder, ellipsoid-ellipsoid, sphere-ellipsoid, capsule-cylinder)."""
+    _run_and_compare("convex", preset, ls, worlds=8, steps=4)
 
 
 @pytest.mark.parametrize("scene,preset,ls", [("hinge_slider_stack", "drake", "exact_root"),

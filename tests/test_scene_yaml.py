@@ -104,7 +104,7 @@ def test_replicated_worlds():
 
 
 def test_strict_unsupported_features():
-    cfg = {"bodies": [{"id": "b", "shapes": [{"type": "cylinder"}]}],
+    cfg = {"bodies": [{"id": "b", "shapes": [{"type": "mesh"}]}],
            "joints": [{"id": "j", "type": "ball", "parent": "world", "child": "b"}]}
     with pytest.raises(UnsupportedSceneFeature) as e:
         build_scene(cfg)
@@ -118,6 +118,28 @@ def test_strict_unsupported_features():
         build_scene({"bodies": [{"shapes": []}]})
     with pytest.raises(SceneLoaderError):
         build_scene({"articulations": [{"id": "a", "joints": ["missing"]}]})
+
+
+def test_round_shape_kinds():
+    """Cylinder, cone and ellipsoid shapes (scene.py:5135-5152) with their masses and the cone's
+    centre of mass a quarter of its height below the shape origin (scene.py:5878-5884)."""
+    cfg = {"bodies": [{"id": "cyl", "shapes": [{"type": "cylinder", "radius": 0.1, "half_height": 0.2,
+                                                 "cfg": {"density": 500.0}}]},
+                      {"id": "cone", "shapes": [{"type": "cone", "radius": 0.1, "half_height": 0.2,
+                                                  "cfg": {"density": 500.0}}]},
+                      {"id": "ell", "shapes": [{"type": "ellipsoid", "a": 0.3, "b": 0.2, "c": 0.1,
+                                                 "cfg": {"density": 500.0}}]}]}
+    m = build_scene(cfg).model
+    ids = build_scene(cfg).body_ids
+    assert m.body_mass[ids["cyl"]] == pytest.approx(500.0 * math.pi * 0.01 * 0.4)
+    assert m.body_mass[ids["cone"]] == pytest.approx(500.0 * math.pi * 0.01 * 0.4 / 3.0)
+    assert m.body_mass[ids["ell"]] == pytest.approx(500.0 * 4.0 / 3.0 * math.pi * 0.3 * 0.2 * 0.1)
+    np.testing.assert_allclose(m.body_com[ids["cone"]], [0.0, 0.0, -0.1], atol=1e-7)
+    np.testing.assert_allclose(m.body_com[ids["cyl"]], [0.0, 0.0, 0.0], atol=1e-7)
+    # the cone's inertia about its centre of mass: 3/20 m r^2 + 3/80 m h^2 across, 3/10 m r^2 along
+    mc = 500.0 * math.pi * 0.01 * 0.4 / 3.0
+    np.testing.assert_allclose(np.diag(np.asarray(m.body_inertia[ids["cone"]]).reshape(3, 3)),
+                               [0.15 * mc * 0.01 + 3.0 / 80.0 * mc * 0.16] * 2 + [0.3 * mc * 0.01], rtol=1e-5)
 
 
 @pytest.mark.skipif(not os.path.isdir(REF_YAML), reason="reference scene files not present")
