@@ -281,6 +281,24 @@ int32_t sapx_pipeline_create_all_pairs(const SapxModel* model, int32_t max_conta
  */
 int32_t sapx_collide(SapxPipeline* pipeline, const float* body_q, const SapxContactsView* contacts, void* stream);
 
+/*
+ * sapx_collide without allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same contacts as
+ * sapx_collide, bit for bit, but the contact staging area and per-world counts
+ * live in the library's static device scratch (55 MB, held by every device that
+ * loads the library until the process ends) instead of the pipeline's buffers.
+ * Calls no hipMalloc / hipFree.  Limits: at most 65536 worlds and 2^20 contact
+ * slots (worlds x max_contacts_per_world) per call (SAPX_ERR_CAPACITY beyond).
+ * Every call on one device shares that scratch: calls that overlap in time on
+ * one device race (serialise them on one stream).  Each call rewrites every
+ * count and staged contact it reads, so nothing needs clearing between calls;
+ * the scratch's unsupported-pair counter is not reported (pair kinds are checked
+ * when the pipeline is created).
+ */
+int32_t sapx_collide_noalloc(SapxPipeline* pipeline, const float* body_q, const SapxContactsView* contacts,
+                             void* stream);
+
 /* ---------------------------------------------------------------------------
  * Learned warm start.  No counterpart in the reference: an MLP predicts a
  * correction of SolverSAP's plain initial velocity (the previous step's

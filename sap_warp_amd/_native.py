@@ -116,6 +116,7 @@ EXPORTS = {
     "sapx_pipeline_destroy": (None, [_P]),
     "sapx_pipeline_create_all_pairs": (_I, [_P, _I, _P, _P, _P, _I, ctypes.POINTER(_P)]),
     "sapx_collide": (_I, [_P, _P, ctypes.POINTER(SapxContactsView), _P]),
+    "sapx_collide_noalloc": (_I, [_P, _P, ctypes.POINTER(SapxContactsView), _P]),
     "sapx_solver_feature_count": (_I, [_P]),
     "sapx_features": (_I, [_P, ctypes.POINTER(SapxStateView), ctypes.POINTER(SapxControlView),
                            ctypes.POINTER(SapxContactsView), _P, _P, _P]),

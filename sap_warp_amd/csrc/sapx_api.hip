@@ -22,6 +22,7 @@ hipError_t launch_wave_np32(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_wave_np48(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_wave_np64(const StepParams& P, int cb, hipStream_t stream);
 hipError_t launch_collide(const CollideParams& P, const PlaceParams& Q, hipStream_t stream);
+hipError_t collide_static_scratch(int** out);
 hipError_t launch_batch_16(const BatchMap& B, int blocks, size_t smem, hipStream_t stream);
 hipError_t launch_batch_32(const BatchMap& B, int blocks, size_t smem, hipStream_t stream);
 hipError_t launch_batch_64(const BatchMap& B, int blocks, size_t smem, hipStream_t stream);
@@ -1200,6 +1201,29 @@ int32_t sapx_collide(SapxPipeline* p, const float* body_q, const SapxContactsVie
     Q.st_i = p->st_i;
     Q.st_f = p->st_f;
     Q.wcount = p->wcount;
+    HIP_TRY(launch_collide(P, Q, reinterpret_cast<hipStream_t>(stream)));
+    return SAPX_OK;
+}
+
+int32_t sapx_collide_noalloc(SapxPipeline* p, const float* body_q, const SapxContactsView* c, void* stream) {
+    CollideParams P;
+    PlaceParams Q;
+    const int32_t rc = collide_params(p, body_q, c, &P, &Q);
+    if (rc != SAPX_OK) return rc;
+    const size_t W = (size_t)p->model->dm.W;
+    if (W > (size_t)kNoAllocWorlds || W * (size_t)p->C > kNoAllocSlots)
+        return fail(SAPX_ERR_CAPACITY, "sapx_collide_noalloc takes at most " + std::to_string(kNoAllocWorlds) +
+                                           " worlds and " + std::to_string(kNoAllocSlots) +
+                                           " contact slots (worlds x max_contacts_per_world) per call");
+    int* base = nullptr;
+    HIP_TRY(collide_static_scratch(&base));
+    P.unsupported = base;
+    P.wcount = base + 64;
+    P.st_i = base + 64 + kNoAllocWorlds;
+    P.st_f = reinterpret_cast<float*>(base + 64 + kNoAllocWorlds + 2 * kNoAllocSlots);
+    Q.wcount = P.wcount;
+    Q.st_i = P.st_i;
+    Q.st_f = P.st_f;
     HIP_TRY(launch_collide(P, Q, reinterpret_cast<hipStream_t>(stream)));
     return SAPX_OK;
 }

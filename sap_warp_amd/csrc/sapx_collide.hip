@@ -23,6 +23,8 @@
 
 #include "sapx_convex.h"
 
+#include <mutex>
+
 namespace sapx {
 
 constexpr float kMaxVal = 1e10f;
@@ -638,6 +640,35 @@ __global__ __launch_bounds__(kPlaceThreads) void sapx_place_kernel(PlaceParams P
             P.nrm[(size_t)dst * 3 + e] = gf[8 * C + 3 * k + e];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Static scratch of sapx_collide_noalloc: the W x C contact staging area and the per-world counts
+// in one file-scope device array (each device that loads the library holds its own copy), for
+// callers that cannot allocate.  Layout in 4-byte words: [0] unsupported-pair counter,
+// [64, 64 + kNoAllocWorlds) counts, then kNoAllocSlots x 2 shape ids, then kNoAllocSlots x 11 floats.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) int scratch[kNoAllocWords];
+
+namespace {
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+hipError_t collide_static_scratch(int** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<int*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
 }
 
 size_t collide_lds_bytes(const DevModel& m, int C) {

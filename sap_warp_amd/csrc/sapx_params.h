@@ -111,6 +111,12 @@ struct BatchMap {
     int npad[kMaxBatchGroups];
 };
 
+// sapx_collide_noalloc: limits of its static scratch (55 MB per device)
+constexpr int kNoAllocWorlds = 65536;          // worlds per call
+constexpr size_t kNoAllocSlots = 1u << 20;     // worlds x contacts per world per call
+constexpr size_t kNoAllocWords = 64 + kNoAllocWorlds + 13 * kNoAllocSlots;
+constexpr int kNoAllocMaxDevices = 64;
+
 constexpr int kCollideThreads = 128;
 constexpr size_t kCollideMaxLds = 64 * 1024;  // dynamic LDS of one collide workgroup
 
