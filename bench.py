@@ -43,12 +43,16 @@ def step_flops(n, nb, nc, k, t):
 
 
 def collide_bytes(model, contacts_written, S, P):
-    """Compulsory HBM bytes of one collide launch (body poses, shape data, pair template, contacts out)."""
+    """HBM bytes of one collide call as its two kernels lay them out: the collide kernel reads body
+    poses, shape data and the pair template and writes each world's staged contacts (52 bytes each)
+    and count; the placement kernel reads the counts and staged contacts and writes the final
+    contacts and world_start."""
     W = model.world_count
     body = model.body_count * 28
     shapes = W * S * (28 + 12 + 4 * 5 + 4)
     pairs = P * 8
-    out = contacts_written * (4 * 2 + 12 * 3 + 8) * 2 + (W + 1) * 4  # staging write + compact read/write
+    per_contact = 4 * 2 + 12 * 3 + 4 * 2
+    out = contacts_written * per_contact * 3 + W * 4 * 2 + (W + 1) * 4
     return body + shapes + pairs + out
 
 
@@ -62,6 +66,12 @@ def read_traffic():
             if line.startswith("hbm_bytes_per_launch"):
                 return float(line.split("=")[1])
     return None
+
+
+def _log(msg: str) -> None:
+    """A progress line on stderr (stdout carries only the JSON result)."""
+    sys.stderr.write(f"[bench {time.strftime('%H:%M:%S')}] {msg}\n")
+    sys.stderr.flush()
 
 
 def _cpu_model_name() -> str:
@@ -328,10 +338,12 @@ def timed_run(args, W, C, dt, dev, lws_trained, graph=False, schedule=None, tune
     }
 
 
-def hetero_run(worlds: int, steps: int, warmup: int, seed: int, dt: float, line_search: str):
+def hetero_run(worlds: int, steps: int, warmup: int, seed: int, dt: float, line_search: str, graph_steps: int):
     """The heterogeneous batch (arm+gripper+box 15 DOF, quadruped 18, humanoid 35, pile of 8 free
-    bodies 48; kinds drawn per world by seed): one stream per kind forked and joined per step, timed
-    eagerly with HIP events on the caller's stream (the joins end there)."""
+    bodies 48; kinds drawn per world by seed) in the batch mode: each kind's collide, then one step
+    launch per contact variant over every kind's worlds.  The contact variants (16- or 32-slot small
+    variant, forked or serial 64-slot variant) are chosen by timing eager steps; the timed steps are
+    replays of one HIP graph holding ``graph_steps`` steps, timed with HIP events."""
     import numpy as np
     import torch
 
@@ -340,36 +352,70 @@ def hetero_run(worlds: int, steps: int, warmup: int, seed: int, dt: float, line_
     t0 = time.time()
     models = build_hetero_models(worlds, seed=seed, device="cuda:0")
     build_s = time.time() - t0
-    batch = HeteroBatch(models, dt=dt, line_search_variant=line_search, collect_iteration_stats=True)
-    for _ in range(warmup):
+    _log(f"heterogeneous models built ({build_s:.1f} s)")
+    batch = HeteroBatch(models, dt=dt, line_search_variant=line_search, collect_iteration_stats=True, mode="batch")
+    for _ in range(max(warmup, 3)):  # three eager steps bind both parameter slots
         batch.step()
     torch.cuda.synchronize()
-    # per-kind Newton iterations of one eager step
-    batch.step()
+    batch.step()  # per-kind Newton iterations of one eager step (the batch kernel fills each solver's counters)
     torch.cuda.synchronize()
-    iters = {n: float(np.mean(g["solver"].last_contact_solve_result.newton_iterations_env))
-             for n, g in batch.groups.items()}
+    iters = {n: float(np.mean(g["solver"].iteration_stats().newton_iterations_env)) for n, g in batch.groups.items()}
     for g in batch.groups.values():
-        g["solver"].collect_iteration_stats = False  # no per-step host reads in the timed loop
-    torch.cuda.synchronize()
+        g["solver"].collect_iteration_stats = False  # no per-step host reads in the timed or captured steps
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(steps):
+    tuning = {}
+    for small in (16, 32):
+        for fork in (False, True):
+            batch.set_schedule(small_capacity=small, fork=fork)
+            for _ in range(2):
+                batch.step()
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(10):
+                batch.step()
+            e1.record()
+            torch.cuda.synchronize()
+            tuning[f"small{small}_{'fork' if fork else 'serial'}"] = round(e0.elapsed_time(e1) / 10, 4)
+    best = min(tuning, key=tuning.get)
+    batch.set_schedule(small_capacity=int(best[5:7]), fork=best.endswith("fork"))
+    for _ in range(2):
         batch.step()
+    torch.cuda.synchronize()
+    k = max(2, graph_steps - graph_steps % 2)  # even: the state slots return to where they started
+    reps = max(1, steps // k)
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            for _ in range(k):
+                batch.step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g.replay()  # first replay uploads the graph
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        g.replay()
     e1.record()
     torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / steps
-    finite = all(bool(torch.isfinite(g["states"][0].joint_q).all()) for g in batch.groups.values())
+    ms = e0.elapsed_time(e1) / (reps * k)
+    finite = all(bool(torch.isfinite(gr["states"][0].joint_q).all()) and
+                 bool(torch.isfinite(gr["states"][1].joint_q).all()) for gr in batch.groups.values())
     return {
         "value": round(batch.world_count * 1000.0 / ms, 1),
         "ms_per_step": round(ms, 4),
         "worlds": batch.world_count,
-        "steps_timed": steps,
-        "kinds": {n: {"worlds": g["model"].world_count, "dof_per_world": g["model"].joint_dof_count // g["model"].world_count,
-                      "mean_newton_iterations_one_step": round(iters[n], 3)} for n, g in batch.groups.items()},
+        "steps_timed": reps * k,
+        "kinds": {n: {"worlds": gr["model"].world_count,
+                      "dof_per_world": gr["model"].joint_dof_count // gr["model"].world_count,
+                      "mean_newton_iterations_one_step": round(iters[n], 3)} for n, gr in batch.groups.items()},
+        "schedule": best,
+        "autotune_ms_per_step": tuning,
         "finite_after": finite,
         "build_s": round(build_s, 2),
-        "timing": "eager launches (collide + step per kind, one stream per kind), HIP events on the caller's stream",
+        "timing": f"{reps} replays of one HIP graph holding {k} batch steps (each kind's collide, one step "
+                  "launch per contact variant over all kinds), HIP events around the replays",
     }
 
 
@@ -407,7 +453,9 @@ def main() -> int:
     ap.add_argument("--train-lr", type=float, default=1e-3)
     ap.add_argument("--hetero", type=int, default=1, help="also time the heterogeneous batch (1) or not (0)")
     ap.add_argument("--hetero-worlds", type=int, default=8192)
-    ap.add_argument("--hetero-steps", type=int, default=200)
+    ap.add_argument("--hetero-steps", type=int, default=1000)
+    ap.add_argument("--hetero-graph-steps", type=int, default=100,
+                    help="batch steps captured in the heterogeneous batch's HIP graph (replayed to cover the steps)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 512 MiB device copies (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -431,10 +479,16 @@ def main() -> int:
     train_info = None
     if args.learned:
         lws, train_info = train_learned(args, W, C, dt, dev)
+        _log("learned warm start trained")
     plain = timed_run(args, W, C, dt, dev, None, tune=bool(args.autotune))
     sched = plain["schedule"]
+    _log(f"plain eager run done ({plain['ms_per_step']:.4f} ms/step)")
     plain_graph = timed_run(args, W, C, dt, dev, None, graph=True, schedule=sched) if args.graph else None
+    if plain_graph is not None:
+        _log(f"plain graph run done ({plain_graph['ms_per_step']:.4f} ms/step)")
     learned = timed_run(args, W, C, dt, dev, lws, schedule=sched) if lws is not None else None
+    if learned is not None:
+        _log("learned run done")
     head = plain  # the benchmark workload is plain SAP; the learned start is reported beside it
     n, nb = head["n"], head["nb"]
     # roofline of the dominant kernel, the step (the wave kernel pair sapx_step launches on the
@@ -504,10 +558,12 @@ def main() -> int:
         }
     if args.hetero:
         out["hetero"] = hetero_run(args.hetero_worlds, args.hetero_steps, min(args.warmup, 20), args.seed, dt,
-                                   args.line_search)
+                                   args.line_search, args.hetero_graph_steps)
+        _log(f"heterogeneous batch done ({out['hetero']['ms_per_step']:.4f} ms/step)")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_worlds, args.cpu_steps, args.seed, dt, args.line_search,
                                            args.cpu_threads)
+        _log("CPU baseline done")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

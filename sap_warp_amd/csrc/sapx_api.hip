@@ -100,6 +100,9 @@ struct SapxSolver {
     int npad = 0, js = 0, ast = 0;
     int lds[17] = {0};
     int smem_bytes = 0;
+    // large worlds: A, H and J of every world in device memory (fused kernel), floats
+    float* big_ws = nullptr;
+    size_t big_stride = 0, big_A = 0, big_H = 0, big_J = 0;
     // one-wave-per-world kernel: LDS layouts of the contact variants (16 / 32 / 64 slots; a
     // variant whose layout does not fit has bytes == 0) and the current schedule
     bool wave_ok = false;
@@ -135,7 +138,7 @@ struct SapxSolver {
     size_t cub_tmp_bytes = 0;
     ~SapxSolver() {
         void* ptrs[] = {v_guess, st, hist, dbg, g_keys, g_keys_out, g_vals, g_vals_out, g_world_start, g_shape0, g_shape1,
-                        g_p0, g_p1, g_n, g_m0, g_m1, cub_tmp, order, feat_off};
+                        g_p0, g_p1, g_n, g_m0, g_m1, cub_tmp, order, feat_off, big_ws};
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -290,7 +293,7 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     const int J = d->joint_count / W, N = d->dof_count / W, NQ = d->coord_count / W, NB = d->body_count / W;
     if (J > kMaxTemplate || NB > kMaxTemplate)
         return fail(SAPX_ERR_CAPACITY, "more than 64 joints or bodies per world");
-    if (N > kMaxDof) return fail(SAPX_ERR_CAPACITY, "more than 64 dofs per world");
+    if (N > kMaxDofFused) return fail(SAPX_ERR_CAPACITY, "more than 256 dofs per world");
     // ---- topology template + equal-world validation
     std::vector<int> t_jtype(J), t_jparent(J), t_jchild(J), t_jqs(J + 1), t_jds(J + 1);
     for (int w = 0; w < W; ++w) {
@@ -525,7 +528,7 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
         // dof-level ancestry: bit d = dof d's joint lies on the root path
         std::vector<unsigned long long> anc_dof(NB, 0ull), dof_anc(N, 0ull);
         for (int b = 0; b < NB; ++b)
-            for (int d = 0; d < N; ++d)
+            for (int d = 0; d < N && d < 64; ++d)  // read only by the one-wave kernel (n <= 64)
                 if ((anc[b] >> dof_joint[d]) & 1ull) anc_dof[b] |= 1ull << d;
         for (int d = 0; d < N; ++d) dof_anc[d] = anc_dof[t_jchild[dof_joint[d]]];
         add64(TP_BODY_ANC, anc);
@@ -650,9 +653,25 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
         off = round_up(off + bytes, 16);
         return (int)o;
     };
-    s->lds[0] = take((size_t)n * s->ast * 4);                                            // A
-    s->lds[1] = take(std::max((size_t)npad * hs * 4, (size_t)C * 104));                  // H / contact scratch
-    s->lds[2] = take(std::max((size_t)3 * C * js * 4, (size_t)(50 * nb + 14 * n + 2) * 4 + (size_t)dm.J * 64));  // J / kinematics
+    const size_t a_bytes = (size_t)n * s->ast * 4;                                                  // A
+    const size_t h_bytes = std::max((size_t)npad * hs * 4, (size_t)C * 104);                        // H / contact scratch
+    const size_t j_bytes = std::max((size_t)3 * C * js * 4, (size_t)(50 * nb + 14 * n + 2) * 4 + (size_t)dm.J * 64);  // J / kinematics
+    // A, H and J stay in LDS while the whole world fits (with room for the rest); a world above 64
+    // dofs, or one whose matrices alone pass 112 KiB, keeps them in a per-world block of device
+    // memory (L2 / MALL resident while its workgroup runs) and the vectors in LDS
+    const bool big = npad > 64 || a_bytes + h_bytes + j_bytes > 112 * 1024;
+    if (big) {
+        auto fl = [](size_t bytes) { return round_up(bytes, 64) / 4; };  // 64-byte aligned regions
+        s->big_A = 0;
+        s->big_H = fl(a_bytes);
+        s->big_J = s->big_H + fl(h_bytes);
+        s->big_stride = s->big_J + fl(j_bytes);
+        s->lds[0] = s->lds[1] = s->lds[2] = 0;
+    } else {
+        s->lds[0] = take(a_bytes);
+        s->lds[1] = take(h_bytes);
+        s->lds[2] = take(j_bytes);
+    }
     s->lds[3] = take((size_t)nb * 8 * 4);                                                // bq
     s->lds[4] = take((size_t)npad * 6 * 4);                                              // S
     s->lds[5] = take((size_t)C * 8 * 8);                                                 // ck (kCk = 8)
@@ -671,6 +690,13 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
     if (s->smem_bytes > 160 * 1024) {
         delete s;
         return fail(SAPX_ERR_CAPACITY, "per-world LDS footprint exceeds 160 KiB; lower max_rigid_contact");
+    }
+    if (big) {
+        const size_t bytes = (size_t)dm.W * s->big_stride * sizeof(float);
+        if (hipMalloc(&s->big_ws, bytes) != hipSuccess) {
+            delete s;
+            return fail(SAPX_ERR_HIP, "hipMalloc failed for the large-world matrices");
+        }
     }
     {
         // one wave per world: LDS layout per contact variant (see sapx_wave_impl.h)
@@ -951,6 +977,11 @@ static int32_t fill_step_params(SapxSolver* s, const SapxStateView* in, const Sa
     P.o_bqd = s->lds[15];
     P.o_topo = s->lds[16];
     P.smem_bytes = s->smem_bytes;
+    P.g_work = s->big_ws;
+    P.g_stride = s->big_stride;
+    P.g_A = s->big_A;
+    P.g_H = s->big_H;
+    P.g_J = s->big_J;
     return SAPX_OK;
 }
 

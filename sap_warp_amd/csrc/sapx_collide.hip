@@ -152,11 +152,28 @@ __device__ void capsule_capsule(V3 p1, V3 a1, float r1, float h1, V3 p2, V3 a2, 
         sphere_sphere(v1, r1, v2, r2, &dist[1], &pos[1], &n2);
     }
 }
-__device__ void plane_box(V3 n, V3 pp, V3 bp, Q4 bq, V3 size, float margin, float dist[4], V3 pos[4]) {
+// The contact slots of plane-box / plane-cylinder are separate scalars selected by compares: an
+// array (or a struct of adjacent floats) read or written at a runtime slot index becomes an
+// indexed access to scratch memory.
+__device__ __forceinline__ float sel4(int k, float a, float b, float c, float d) {
+    return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
+}
+// component-wise: a ?: between two structs is emitted as a copy through memory
+__device__ __forceinline__ V3 vsel(bool c, V3 a, V3 b) { return v3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
+__device__ __forceinline__ void put4(int k, float v, V3 p, float& d0, float& d1, float& d2, float& d3, V3& p0, V3& p1,
+                                     V3& p2, V3& p3) {
+    d0 = k == 0 ? v : d0; d1 = k == 1 ? v : d1; d2 = k == 2 ? v : d2; d3 = k == 3 ? v : d3;
+    p0 = vsel(k == 0, p, p0); p1 = vsel(k == 1, p, p1); p2 = vsel(k == 2, p, p2); p3 = vsel(k == 3, p, p3);
+}
+template <int K>
+__device__ void plane_box(V3 n, V3 pp, V3 bp, Q4 bq, V3 size, float margin, float (&dist)[K], V3 (&pos)[K]) {
     V3 c[3];
     rot_cols(bq, c);
     float center_dist = dot(bp - pp, n);
+    float d0 = dist[0], d1 = dist[1], d2 = dist[2], d3 = dist[3];
+    V3 p0 = pos[0], p1 = pos[1], p2 = pos[2], p3 = pos[3];
     int ncontact = 0, worst = 0;
+#pragma unroll
     for (int i = 0; i < 8; ++i) {
         V3 corner = v3((i & 1) ? size.x : -size.x, (i & 2) ? size.y : -size.y, (i & 4) ? size.z : -size.z);
         corner = rmul(c, corner);
@@ -165,19 +182,19 @@ __device__ void plane_box(V3 n, V3 pp, V3 bp, Q4 bq, V3 size, float margin, floa
         if (cdist > margin) continue;
         V3 cpos = (corner + bp) - n * (0.5f * cdist);
         if (ncontact < 4) {
-            dist[ncontact] = cdist;
-            pos[ncontact] = cpos;
-            if (ncontact == 0 || cdist > dist[worst]) worst = ncontact;
+            put4(ncontact, cdist, cpos, d0, d1, d2, d3, p0, p1, p2, p3);
+            if (ncontact == 0 || cdist > sel4(worst, d0, d1, d2, d3)) worst = ncontact;
             ncontact++;
-        } else if (cdist < dist[worst]) {
-            dist[worst] = cdist;
-            pos[worst] = cpos;
+        } else if (cdist < sel4(worst, d0, d1, d2, d3)) {
+            put4(worst, cdist, cpos, d0, d1, d2, d3, p0, p1, p2, p3);
             worst = 0;
-            if (dist[1] > dist[worst]) worst = 1;
-            if (dist[2] > dist[worst]) worst = 2;
-            if (dist[3] > dist[worst]) worst = 3;
+            if (d1 > sel4(worst, d0, d1, d2, d3)) worst = 1;
+            if (d2 > sel4(worst, d0, d1, d2, d3)) worst = 2;
+            if (d3 > sel4(worst, d0, d1, d2, d3)) worst = 3;
         }
     }
+    dist[0] = d0; dist[1] = d1; dist[2] = d2; dist[3] = d3;
+    pos[0] = p0; pos[1] = p1; pos[2] = p2; pos[3] = p3;
 }
 // plane-ellipsoid (primitive.py:322-351): the ellipsoid's support point against the plane normal
 __device__ void plane_ellipsoid(V3 n, V3 pp, V3 ep, Q4 eq, V3 size, float* dist, V3* pos) {
@@ -228,7 +245,8 @@ __device__ void sphere_cylinder(V3 sp, float sr, V3 cp, V3 axis, float cr, float
 // plane-cylinder (primitive.py:504-657): the deepest rim point first, then a fixed tripod on the
 // near cap (axis within 22.5 degrees of the normal) or the two side generators and the cap-rim
 // point facing the plane, each kept when farther than 1 % of the larger dimension from the first
-__device__ void plane_cylinder(V3 n, V3 pp, V3 cc, V3 axis, float r, float hh, float dist[4], V3 pos[4]) {
+template <int K>
+__device__ void plane_cylinder(V3 n, V3 pp, V3 cc, V3 axis, float r, float hh, float (&dist)[K], V3 (&pos)[K]) {
     constexpr float kFlatCos = 0.9238795325112867f;  // cos(22.5 deg)
     float dna = dot(n, axis);
     if (dna > 0.0f) { axis = v3(-axis.x, -axis.y, -axis.z); dna = -dna; }
@@ -241,7 +259,7 @@ __device__ void plane_cylinder(V3 n, V3 pp, V3 cc, V3 axis, float r, float hh, f
     if (fabsf(dot(axis, ref)) > 0.9f) ref = v3(0.0f, 1.0f, 0.0f);
     const V3 pf = nsafe(ref - axis * dot(axis, ref));
     const bool flat = -dna >= kFlatCos;
-    const V3 dperp = has_align ? pa : pf;
+    const V3 dperp = vsel(has_align, pa, pf);
     const V3 dpt = cap + dperp * r;
     const float dd = dot(dpt - pp, n);
     const V3 dpos = dpt - n * (dd * 0.5f);
@@ -261,26 +279,31 @@ __device__ void plane_cylinder(V3 n, V3 pp, V3 cc, V3 axis, float r, float hh, f
         pts[2] = (cap + u * c120) - v * s120;
         np = 3;
     } else {
-        const V3 pr = has_align ? pa : pf;
+        const V3 pr = vsel(has_align, pa, pf);
         const V3 u = pr * r;
         const V3 v = cross(axis, pr) * r;
         pts[0] = (cc + axis * hh) + u;
         pts[1] = (cc - axis * hh) + u;
         const V3 ppv = cap + v, pnv = cap - v;
         const float dpv = dot(ppv - pp, n), dnv = dot(pnv - pp, n);
-        pts[2] = dpv <= dnv ? ppv : pnv;
+        pts[2] = vsel(dpv <= dnv, ppv, pnv);
         np = 3;
     }
-    for (int k = 0; k < np; ++k) {
+    float d0 = dist[0], d1 = dist[1], d2 = dist[2], d3 = dist[3];
+    V3 p0 = pos[0], p1 = pos[1], p2 = pos[2], p3 = pos[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (k >= np) break;
         const float d = dot(pts[k] - pp, n);
         const V3 p = pts[k] - n * (d * 0.5f);
         const V3 e = p - dpos;
         if (nc < 4 && dot(e, e) > thr_sq) {
-            dist[nc] = d;
-            pos[nc] = p;
+            put4(nc, d, p, d0, d1, d2, d3, p0, p1, p2, p3);
             ++nc;
         }
     }
+    dist[1] = d1; dist[2] = d2; dist[3] = d3;
+    pos[1] = p1; pos[2] = p2; pos[3] = p3;
 }
 
 __device__ __forceinline__ float vget(V3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
@@ -521,6 +544,7 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
             int idx = prefix;
             const int bA = M.shape_body[ga], bB = M.shape_body[gb];
             const float ma = M.shape_margin[ga], mbm = M.shape_margin[gb];
+#pragma unroll
             for (int i = 0; i < NC; ++i) {
                 if (!valid[i]) continue;
                 if (idx >= P.C) break;

@@ -45,7 +45,8 @@ __host__ __device__ inline bool pair_convex(int ta, int tb) {
 }
 
 constexpr int kMaxTemplate = 64;  // joints / bodies per world (64-bit masks)
-constexpr int kMaxDof = 64;       // dofs per world (one wavefront per row set)
+constexpr int kMaxDof = 64;       // dofs per world of the one-wave kernel (one row per lane)
+constexpr int kMaxDofFused = 256; // dofs per world of the 256-thread kernel (one dof per thread)
 
 // --------------------------------------------------------------------------
 // Device model: per-world topology template (local indices, shared by all

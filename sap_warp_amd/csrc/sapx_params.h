@@ -65,6 +65,12 @@ struct StepParams {
     // LDS byte offsets
     int o_A, o_H, o_J, o_bq, o_S, o_ck, o_cv, o_cG, o_cb, o_dv64, o_dv32, o_dflag, o_red, o_q, o_misc, o_bqd, o_topo;
     int smem_bytes;
+    // large worlds (fused kernel): A, H (with the contact set-up temporaries) and J (with the
+    // kinematics scratch) live in a per-world block of device memory instead of LDS; g_work is
+    // null when they fit in LDS.  Offsets and stride in floats.
+    float* g_work;
+    size_t g_stride;
+    size_t g_A, g_H, g_J;
     WaveLayout wl;       // one-wave kernel layout for this launch's contact variant
     int wave_min_nc;     // the variant processes worlds with wave_min_nc <= nc <= its capacity
     const int* world_order;  // blockIdx -> world (a permutation of 0..W-1), or null for the identity

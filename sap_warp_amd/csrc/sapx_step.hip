@@ -46,6 +46,7 @@ enum { V_V = 0, V_V0, V_VS, V_DV, V_G, V_JI, V_DP, V_AV, V_T1, V_T2, V_AVS, V_AR
 enum { F_HD = 0, F_AINV, F_DSC, F_PDA, F_PDG, F_PDL, F_LOV, F_UPV, F_LOR, F_UPR, F_TAU, F_JF, F_DINV, F_NUM };
 enum { I_PART = 0, I_PD, I_LO, I_UP, I_NUM };
 
+template <bool BIG>
 __device__ __forceinline__ Smem smem_view(char* base, const StepParams& P) {
     Smem s;
     s.A = reinterpret_cast<float*>(base + P.o_A);
@@ -64,6 +65,12 @@ __device__ __forceinline__ Smem smem_view(char* base, const StepParams& P) {
     s.q = reinterpret_cast<float*>(base + P.o_q);
     s.mp = reinterpret_cast<float*>(base + P.o_misc);
     s.bqd = reinterpret_cast<double*>(base + P.o_bqd);
+    if constexpr (BIG) {
+        float* g = P.g_work + (size_t)blockIdx.x * P.g_stride;
+        s.A = g + P.g_A;
+        s.H = g + P.g_H;
+        s.J = g + P.g_J;
+    }
     return s;
 }
 
@@ -306,6 +313,67 @@ __device__ __forceinline__ void chol_solve_blocks(const float* H, int npad, int 
     if (lane < n) b[lane] = (double)x;
 }
 
+// Solve L L^T x = b for any npad (a multiple of 16) with the blocked factor and its inverted
+// diagonal blocks (chol_block_inverses), wave 0 only; b is the float64 working vector in LDS or
+// device memory and holds x on return.  Forward: per 16-row block, lanes 0..15 apply the block's
+// inverse to its finished right-hand side, then the wave subtracts L_ik y_k from every row below.
+// Backward: the transposed sweep from the last block.  Rows >= n are identity (b_i = 0 there).
+__device__ __forceinline__ void chol_solve_wave_big(const float* H, int npad, int hs, int n, const float* dinv,
+                                                    double* b) {
+    const int lane = threadIdx.x & 63;
+    if ((threadIdx.x >> 6) != 0) return;
+    const int nblk = npad >> 4;
+    for (int i = lane; i < npad; i += 64)
+        if (i >= n) b[i] = 0.0;
+    wave_sync();
+    for (int kb = 0; kb < nblk; ++kb) {
+        const int k0 = kb * 16;
+        float yk = 0.0f;
+        if (lane < 16) {
+            float acc = dinv[k0 + lane] * (float)b[k0 + lane];
+            for (int c = 0; c < lane; ++c) acc += H[(k0 + c) * hs + k0 + lane] * (float)b[k0 + c];
+            yk = acc;
+        }
+        wave_sync();
+        if (lane < 16) b[k0 + lane] = (double)yk;
+        wave_sync();
+        for (int i = k0 + 16 + lane; i < npad; i += 64) {
+            float acc = (float)b[i];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) acc -= H[i * hs + k0 + t] * (float)b[k0 + t];
+            b[i] = (double)acc;
+        }
+        wave_sync();
+    }
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+        const int k0 = kb * 16;
+        float xk = 0.0f;
+        if (lane < 16) {
+            float acc = dinv[k0 + lane] * (float)b[k0 + lane];
+            for (int i = lane + 1; i < 16; ++i) acc += H[(k0 + lane) * hs + k0 + i] * (float)b[k0 + i];
+            xk = acc;
+        }
+        wave_sync();
+        if (lane < 16) b[k0 + lane] = (double)xk;
+        wave_sync();
+        for (int i = lane; i < k0; i += 64) {
+            float acc = (float)b[i];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) acc -= H[(k0 + t) * hs + i] * (float)b[k0 + t];
+            b[i] = (double)acc;
+        }
+        wave_sync();
+    }
+    for (int i = lane; i < npad; i += 64)
+        if (i >= n) b[i] = 0.0;
+}
+
+// the factor solve for the world's padded size: register rows up to 64, blocked beyond
+__device__ __forceinline__ void chol_solve(const float* H, int npad, int hs, int n, const float* dinv, double* b) {
+    if (npad <= 64) chol_solve_blocks(H, npad, hs, n, dinv, b);
+    else chol_solve_wave_big(H, npad, hs, n, dinv, b);
+}
+
 __device__ __forceinline__ double model_terms(const Smem& s, int npad, int i, double vi, double dt, double* imp,
                                               double* hd) {
     double cost = 0.0;
@@ -336,7 +404,8 @@ __device__ __forceinline__ double model_terms(const Smem& s, int npad, int i, do
 // ---------------------------------------------------------------------------
 // The fused step kernel
 // ---------------------------------------------------------------------------
-template <int NT>
+// BIG: A, H and J in the world's block of device memory (P.g_work), the rest in LDS
+template <int NT, bool BIG>
 __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     const DevModel& M = P.m;
@@ -350,7 +419,7 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
     const int js = P.js;
     const int ast = P.ast;  // row stride of A (odd: conflict-free row-per-thread access)
     const int C = P.C;
-    Smem s = smem_view(smem_raw, P);
+    Smem s = smem_view<BIG>(smem_raw, P);
     const Topo T = topo_view(reinterpret_cast<const int*>(smem_raw + P.o_topo), M);
     double* d64 = s.d64;
     float* d32 = s.d32;
@@ -756,13 +825,13 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
     chol_block_inverses<NT>(s.H, npad, hs, d32 + F_DINV * npad);
     for (int d = tid; d < npad; d += NT) d64[V_T1 * npad + d] = (d < n) ? (double)d32[F_TAU * npad + d] : 0.0;
     __syncthreads();
-    chol_solve_blocks(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
+    chol_solve(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
     __syncthreads();
     for (int i = tid; i < n; i += NT) {
         d64[V_T2 * npad + i] = (double)d32[F_TAU * npad + i] - dot_fd(s.A + i * ast, d64 + V_T1 * npad, n);
     }
     __syncthreads();
-    chol_solve_blocks(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T2 * npad);
+    chol_solve(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T2 * npad);
     __syncthreads();
     for (int d = tid; d < npad; d += NT) {
         double qdd = (d < n) ? d64[V_T1 * npad + d] + d64[V_T2 * npad + d] : 0.0;
@@ -1249,7 +1318,7 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
         if (wave == 0) {
             for (int i = lane; i < npad; i += 64) dvv[i] = (i < n) ? -grad[i] : 0.0;
             wave_sync();
-            chol_solve_blocks(s.H, npad, hs, n, d32 + F_DINV * npad, dvv);
+            chol_solve(s.H, npad, hs, n, d32 + F_DINV * npad, dvv);
             wave_sync();
             // ---- mixed-precision refinement of the Newton direction: residual
             //      r = -grad - (A + J^T G J + diag(hd + shift)) dv in float64 from the stored
@@ -1282,7 +1351,7 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
                     d64[V_T1 * npad + i] = r;
                 }
                 wave_sync();
-                chol_solve_blocks(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
+                chol_solve(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
                 wave_sync();
                 for (int i = lane; i < n; i += 64) dvv[i] += d64[V_T1 * npad + i];
                 wave_sync();
@@ -1639,10 +1708,16 @@ hipError_t launch_step(const StepParams& P, hipStream_t stream) {
     constexpr int NT = 256;
     static bool attr_set = false;
     if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)sapx_step_kernel<NT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute((const void*)sapx_step_kernel<NT, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void*)sapx_step_kernel<NT, true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
         attr_set = true;
     }
-    hipLaunchKernelGGL(sapx_step_kernel<NT>, dim3(P.m.W), dim3(NT), P.smem_bytes, stream, P);
+    if (P.g_work)
+        hipLaunchKernelGGL((sapx_step_kernel<NT, true>), dim3(P.m.W), dim3(NT), P.smem_bytes, stream, P);
+    else
+        hipLaunchKernelGGL((sapx_step_kernel<NT, false>), dim3(P.m.W), dim3(NT), P.smem_bytes, stream, P);
     return hipGetLastError();
 }
 }  // namespace sapx

@@ -23,7 +23,14 @@ World kinds (the heterogeneous workload of the benchmark plan):
 * ``quadruped`` — a floating box torso with 4 legs of 3 revolute joints: 18 DOF;
 * ``humanoid`` — ``scenes.build_humanoid_scene``: 35 DOF;
 * ``pile`` — 8 free boxes / spheres of size U(0.05, 0.2) m dropped on the
-  ground: 48 DOF.
+  ground: 48 DOF; with ``build_hetero_models(pile_sizes=(8, 16, 32))`` the pile
+  worlds split into groups of 8, 16 and 32 bodies (``pile``, ``pile16``,
+  ``pile32``: 48, 96 and 192 DOF, 8 contact slots per body above 8 bodies).
+
+Groups the batch launch cannot hold (above 48 dofs or 64 contact slots: the large
+piles, which run the 256-thread kernel with their matrices in device memory) are
+stepped by their own solver inside the same call, on the caller's stream, so the
+whole batch step still captures into one graph.
 """
 from __future__ import annotations
 
@@ -187,17 +194,20 @@ def build_arm_gripper_scene(num_worlds: int = 64, seed: int = 12, *, contact_gap
 def build_pile_scene(num_worlds: int = 64, seed: int = 13, *, bodies: int = 8, contact_gap: float = 0.01,
                      device: str = "cpu"):
     """Piles of free boxes and spheres (alternating by slot, sizes U(0.05, 0.2) m, density
-    U(500, 2000)) dropped from a loose lattice 0.3 m apart onto the ground; 6 DOF per body."""
+    U(500, 2000)) dropped onto the ground from a loose lattice: 2 x 2 per layer, layers 0.3 m
+    apart, for up to 8 bodies; 4 x 4 per layer 0.25 m apart for larger piles (16 bodies: one
+    layer, 32: two); 6 DOF per body."""
     rng = np.random.Generator(np.random.PCG64(seed))
     b = SceneBuilder(rigid_gap=contact_gap)
     b.world_count = num_worlds
-    side = 2
+    side, pitch = (2, 0.3) if bodies <= 8 else (4, 0.25)
     for w in range(num_worlds):
         b.current_world = w
         for k in range(bodies):
             i, j, l = k % side, (k // side) % side, k // (side * side)
             size = rng.uniform(0.05, 0.2)
-            p = np.array([(i - 0.5) * 0.3, (j - 0.5) * 0.3, 0.12 + 0.3 * l]) + rng.uniform(-0.04, 0.04, 3)
+            c = 0.5 * (side - 1)
+            p = np.array([(i - c) * pitch, (j - c) * pitch, 0.12 + pitch * l]) + rng.uniform(-0.04, 0.04, 3)
             q = quat_from_axis_angle(rng.normal(size=3), rng.uniform(0.0, math.pi))
             pose = tf(p, q)
             body = b.add_body(pose)
@@ -225,11 +235,35 @@ def draw_world_kinds(num_worlds: int, seed: int = 0) -> dict:
     return {k: int(np.sum(kinds == i)) for i, k in enumerate(WORLD_KINDS) if np.any(kinds == i)}
 
 
-def build_hetero_models(num_worlds: int = 8192, seed: int = 0, *, device: str = "cpu") -> dict:
-    """One model per world kind, sized by ``draw_world_kinds`` (each group seeded from ``seed``)."""
+def build_hetero_models(num_worlds: int = 8192, seed: int = 0, *, device: str = "cpu",
+                        pile_sizes: tuple = (8,)) -> dict:
+    """One model per world kind, sized by ``draw_world_kinds`` (each group seeded from ``seed``).
+
+    ``pile_sizes``: body counts of the pile worlds; with several sizes each pile world draws its
+    size (seeded), and every size is a group of its own (``pile`` for 8 bodies, ``pile<b>``
+    otherwise: 16 bodies are 96 dofs, 32 bodies 192)."""
     counts = draw_world_kinds(num_worlds, seed)
-    return {k: _BUILDERS[k](n, seed=1000 * seed + 17 * i + 1, device=device)
-            for i, (k, n) in enumerate(counts.items())}
+    out = {}
+    for i, (k, n) in enumerate(counts.items()):
+        s = 1000 * seed + 17 * i + 1
+        if k != "pile" or tuple(pile_sizes) == (8,):
+            out[k] = _BUILDERS[k](n, seed=s, device=device)
+            continue
+        rng = np.random.Generator(np.random.PCG64(s))
+        draw = rng.integers(0, len(pile_sizes), size=n)
+        for j, b in enumerate(pile_sizes):
+            m = int(np.sum(draw == j))
+            if m:
+                out["pile" if b == 8 else f"pile{b}"] = build_pile_scene(m, seed=s + 101 * j, bodies=int(b),
+                                                                          device=device)
+    return out
+
+
+def contacts_for(name: str, default: int) -> int:
+    """Contact capacity per world of a group: 8 per body for piles above 8 bodies."""
+    if name.startswith("pile") and name != "pile":
+        return 8 * int(name[4:])
+    return default
 
 
 class HeteroBatch:
@@ -257,21 +291,28 @@ class HeteroBatch:
         self.dt = float(dt)
         self.groups = {}
         for name, model in models.items():
-            solver = SolverSAP(model, max_rigid_contact=max_contacts_per_world,
-                               contact_preset_variant=contact_preset_variant,
+            C = contacts_for(name, max_contacts_per_world)
+            solver = SolverSAP(model, max_rigid_contact=C, contact_preset_variant=contact_preset_variant,
                                line_search_variant=line_search_variant, **solver_kwargs)
-            pipe = SapCollisionPipeline(model, max_contacts_per_world=max_contacts_per_world)
+            pipe = SapCollisionPipeline(model, max_contacts_per_world=C)
             self.groups[name] = dict(model=model, solver=solver, pipeline=pipe, contacts=pipe.contacts(),
                                      states=[model.state(), model.state()], control=model.control(),
                                      stream=torch.cuda.Stream())
         self._k = 0
         self._batch = None
         self._bound = [None, None]
-        if mode == "batch":
+        # groups the batch launch holds (one-wave kernel, at most 48 dofs and 64 contact slots);
+        # the others (large piles: the 256-thread kernel) step with their own solver in the call
+        self.batched = [n for n, g in self.groups.items()
+                        if g["solver"].get_schedule()["kernel"] == "wave" and g["solver"].dof_per_env <= 48
+                        and g["solver"].max_rigid_contact <= 64]
+        self.own = [n for n in self.groups if n not in self.batched]
+        if mode == "batch" and self.batched:
             self._lib = nat.load()
-            handles = (ctypes.c_void_p * len(self.groups))(*[g["solver"]._handle.value for g in self.groups.values()])
+            gs = [self.groups[n] for n in self.batched]
+            handles = (ctypes.c_void_p * len(gs))(*[g["solver"]._handle.value for g in gs])
             h = ctypes.c_void_p()
-            nat.check(self._lib.sapx_batch_create(handles, len(self.groups), ctypes.byref(h)), "sapx_batch_create")
+            nat.check(self._lib.sapx_batch_create(handles, len(gs), ctypes.byref(h)), "sapx_batch_create")
             self._batch = h
 
     def close(self) -> None:
@@ -289,8 +330,10 @@ class HeteroBatch:
         """Contact variants of the batch launch (``sapx_batch_set_schedule``)."""
         from . import _native as nat
 
-        if self._batch is None:
+        if self.mode != "batch":
             raise ValueError("set_schedule applies to mode='batch'")
+        if self._batch is None:
+            return  # no group in the batch launch
         nat.check(self._lib.sapx_batch_set_schedule(self._batch, int(small_capacity), int(bool(fork))),
                   "sapx_batch_set_schedule")
 
@@ -299,7 +342,7 @@ class HeteroBatch:
 
         from . import _native as nat
 
-        gs = list(self.groups.values())
+        gs = [self.groups[n] for n in self.batched]
         G = len(gs)
 
         def sview(st):
@@ -333,14 +376,18 @@ class HeteroBatch:
             slot = self._k % 2
             if capturing and any(g["solver"].collect_iteration_stats for g in self.groups.values()):
                 raise RuntimeError("construct the batch with collect_iteration_stats=False to capture it")
-            if self._bound[slot] is None:
+            if self._batch is not None and self._bound[slot] is None:
                 if capturing:
                     raise RuntimeError("step the batch eagerly (three steps) before capturing it: binds both slots")
                 self._bind(slot)
             for g in self.groups.values():
                 g["pipeline"].collide(g["states"][slot], g["contacts"])
-            with nat.on_device(next(iter(self.groups.values()))["solver"]._device):
-                nat.check(self._lib.sapx_batch_step(self._batch, slot, nat.stream_ptr()), "sapx_batch_step")
+            if self._batch is not None:
+                with nat.on_device(self.groups[self.batched[0]]["solver"]._device):
+                    nat.check(self._lib.sapx_batch_step(self._batch, slot, nat.stream_ptr()), "sapx_batch_step")
+            for n in self.own:
+                g = self.groups[n]
+                g["solver"].step(g["states"][slot], g["states"][1 - slot], g["control"], g["contacts"], self.dt)
             self._k += 1
             return
         if capturing:
@@ -362,4 +409,4 @@ class HeteroBatch:
 
 
 __all__ = ["HeteroBatch", "WORLD_KINDS", "build_arm_gripper_scene", "build_hetero_models", "build_pile_scene",
-           "build_quadruped_scene", "draw_world_kinds"]
+           "build_quadruped_scene", "contacts_for", "draw_world_kinds"]

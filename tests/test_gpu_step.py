@@ -67,6 +67,11 @@ def _scene(name, worlds, seed):
         from sap_warp_amd.scenes import build_ball_chain_scene
 
         return build_ball_chain_scene(worlds, seed=seed + 7, device="cuda:0"), 2.0e-3, 64
+    if name in ("pile16", "pile32"):
+        from sap_warp_amd import hetero
+
+        bodies = int(name[4:])
+        return hetero.build_pile_scene(worlds, seed=seed + 5, bodies=bodies, device="cuda:0"), 2.0e-3, 8 * bodies
     if name in ("quadruped", "arm_gripper", "pile"):
         from sap_warp_amd import hetero
 
@@ -355,6 +360,23 @@ def test_box_pile_step_parity(gpu_device, preset, ls):
 def test_round_shapes_step_parity(gpu_device, preset, ls):
     """Free cylinders, ellipsoids and spheres (plane-cylinder, plane-ellipsoid, sphere-cylinder)."""
     _run_and_compare("round", preset, ls, worlds=8, steps=4)
+
+
+@pytest.mark.parametrize("scene,preset,ls", [("pile16", "approx32", "exact_root"), ("pile32", "approx32", "exact_root"),
+                                             ("pile32", "drake", "armijo_decay")])
+def test_large_pile_step_parity(gpu_device, scene, preset, ls):
+    """Piles of 16 and 32 free boxes / spheres (96 and 192 dofs, beyond the one-wave kernel): the
+    256-thread kernel with A, H and J in device memory and the blocked solves, after the piles
+    settle for 120 GPU steps into 40-70 contacts per world."""
+    _run_and_compare(scene, preset, ls, worlds=3, steps=3, warm_steps=120, min_max_contacts=30)
+
+
+def test_matrices_in_device_memory_step_parity(gpu_device):
+    """A world whose matrices pass the LDS budget (lying humanoids with a 320-slot contact
+    capacity: J alone is 3 x 320 rows) runs the 256-thread kernel with A, H and J in device
+    memory and the register-row solves; same bar."""
+    _run_and_compare("humanoid_lying", "approx32", "exact_root", worlds=4, steps=3, solver_C=320, pipe_C=64,
+                     min_max_contacts=20)
 
 
 @pytest.mark.parametrize("preset,ls", [("approx32", "exact_root"), ("drake", "armijo_decay")])

@@ -81,6 +81,8 @@ def test_hetero_batch_refused_under_capture(monkeypatch):
     batch.mode = "batch"
     batch._k = 0
     batch._bound = [None, None]
+    batch._batch = object()  # a batch launch exists but no slot is bound yet
+    batch.batched, batch.own = [], []
     with pytest.raises(RuntimeError, match="before capturing"):
         batch.step()
 
