@@ -338,7 +338,8 @@ def timed_run(args, W, C, dt, dev, lws_trained, graph=False, schedule=None, tune
     }
 
 
-def hetero_run(worlds: int, steps: int, warmup: int, seed: int, dt: float, line_search: str, graph_steps: int):
+def hetero_run(worlds: int, steps: int, warmup: int, seed: int, dt: float, line_search: str, graph_steps: int,
+               pile_sizes=(8,)):
     """The heterogeneous batch (arm+gripper+box 15 DOF, quadruped 18, humanoid 35, pile of 8 free
     bodies 48; kinds drawn per world by seed) in the batch mode: each kind's collide, then one step
     launch per contact variant over every kind's worlds.  The contact variants (16- or 32-slot small
@@ -350,7 +351,7 @@ def hetero_run(worlds: int, steps: int, warmup: int, seed: int, dt: float, line_
     from sap_warp_amd.hetero import HeteroBatch, build_hetero_models
 
     t0 = time.time()
-    models = build_hetero_models(worlds, seed=seed, device="cuda:0")
+    models = build_hetero_models(worlds, seed=seed, device="cuda:0", pile_sizes=tuple(pile_sizes))
     build_s = time.time() - t0
     _log(f"heterogeneous models built ({build_s:.1f} s)")
     batch = HeteroBatch(models, dt=dt, line_search_variant=line_search, collect_iteration_stats=True, mode="batch")
@@ -409,6 +410,8 @@ def hetero_run(worlds: int, steps: int, warmup: int, seed: int, dt: float, line_
         "steps_timed": reps * k,
         "kinds": {n: {"worlds": gr["model"].world_count,
                       "dof_per_world": gr["model"].joint_dof_count // gr["model"].world_count,
+                      "contact_slots": gr["solver"].max_rigid_contact,
+                      "kernel": "batch launch" if n in batch.batched else "own solver (256-thread kernel)",
                       "mean_newton_iterations_one_step": round(iters[n], 3)} for n, gr in batch.groups.items()},
         "schedule": best,
         "autotune_ms_per_step": tuning,
@@ -456,6 +459,9 @@ def main() -> int:
     ap.add_argument("--hetero-steps", type=int, default=1000)
     ap.add_argument("--hetero-graph-steps", type=int, default=100,
                     help="batch steps captured in the heterogeneous batch's HIP graph (replayed to cover the steps)")
+    ap.add_argument("--hetero-large", type=int, default=1,
+                    help="also time the heterogeneous batch with piles of 8, 16 and 32 bodies (1) or not (0)")
+    ap.add_argument("--hetero-large-steps", type=int, default=200)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 512 MiB device copies (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -560,6 +566,11 @@ def main() -> int:
         out["hetero"] = hetero_run(args.hetero_worlds, args.hetero_steps, min(args.warmup, 20), args.seed, dt,
                                    args.line_search, args.hetero_graph_steps)
         _log(f"heterogeneous batch done ({out['hetero']['ms_per_step']:.4f} ms/step)")
+        if args.hetero_large:
+            out["hetero_large_piles"] = hetero_run(args.hetero_worlds, args.hetero_large_steps, min(args.warmup, 20),
+                                                   args.seed, dt, args.line_search, args.hetero_graph_steps,
+                                                   pile_sizes=(8, 16, 32))
+            _log(f"heterogeneous batch with large piles done ({out['hetero_large_piles']['ms_per_step']:.4f} ms/step)")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_worlds, args.cpu_steps, args.seed, dt, args.line_search,
                                            args.cpu_threads)

@@ -158,8 +158,6 @@ __device__ void capsule_capsule(V3 p1, V3 a1, float r1, float h1, V3 p2, V3 a2, 
 __device__ __forceinline__ float sel4(int k, float a, float b, float c, float d) {
     return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d));
 }
-// component-wise: a ?: between two structs is emitted as a copy through memory
-__device__ __forceinline__ V3 vsel(bool c, V3 a, V3 b) { return v3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
 __device__ __forceinline__ void put4(int k, float v, V3 p, float& d0, float& d1, float& d2, float& d3, V3& p0, V3& p1,
                                      V3& p2, V3& p3) {
     d0 = k == 0 ? v : d0; d1 = k == 1 ? v : d1; d2 = k == 2 ? v : d2; d3 = k == 3 ? v : d3;

@@ -224,6 +224,14 @@ struct DTF {
     D3 p;
     DQ q;
 };
+
+// Component-wise selects: C++ ?: between two structs is emitted as a copy through a stack slot,
+// which the backend keeps in scratch memory.
+__device__ __forceinline__ V3 vsel(bool c, V3 a, V3 b) { return V3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+__device__ __forceinline__ D3 dsel(bool c, D3 a, D3 b) { return D3{c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z}; }
+__device__ __forceinline__ DTF dtfsel(bool c, DTF a, DTF b) {
+    return DTF{dsel(c, a.p, b.p), DQ{c ? a.q.x : b.q.x, c ? a.q.y : b.q.y, c ? a.q.z : b.q.z, c ? a.q.w : b.q.w}};
+}
 __device__ __forceinline__ D3 d3(double x, double y, double z) { return D3{x, y, z}; }
 __device__ __forceinline__ D3 dadd(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
 __device__ __forceinline__ D3 dsub(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }

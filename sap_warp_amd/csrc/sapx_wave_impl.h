@@ -467,7 +467,7 @@ __device__ __forceinline__ void wave_body(const StepParams& P, const int blk) {
             } else if (jt == kJointBall) {
                 Xj.q = DQ{q_in[qs], q_in[qs + 1], q_in[qs + 2], q_in[qs + 3]};
             }
-            Xl = xcid ? Xj : dtf_compose_n(Xj, dtf_inverse_n(dtf_from(Xc)));
+            Xl = dtfsel(xcid, Xj, dtf_compose_n(Xj, dtf_inverse_n(dtf_from(Xc))));
             const TF Xmb = tf_inverse_n(Xc);
             const Q4 qj = Q4{(float)Xj.q.x, (float)Xj.q.y, (float)Xj.q.z, (float)Xj.q.w};
             const V3 pmb = qrot(qj, Xmb.p);
@@ -784,7 +784,7 @@ __device__ __forceinline__ void wave_body(const StepParams& P, const int blk) {
         if (cb1 >= 0) { const DTF X = load_dtf(bqd + cb1 * 8); x1d = dadd(X.p, dqrot_unit(X.q, x1d)); }
         D3 nd = d3(P.cnormal[idx * 3], P.cnormal[idx * 3 + 1], P.cnormal[idx * 3 + 2]);
         const double nd2 = nd.x * nd.x + nd.y * nd.y + nd.z * nd.z;
-        nd = (nd2 <= 1.0e-32) ? d3(0.0, 0.0, 0.0) : dscale(nd, 1.0 / sqrt(nd2));
+        nd = dsel(nd2 <= 1.0e-32, d3(0.0, 0.0, 0.0), dscale(nd, 1.0 / sqrt(nd2)));
         cx0 = v3((float)x0d.x, (float)x0d.y, (float)x0d.z);
         cx1 = v3((float)x1d.x, (float)x1d.y, (float)x1d.z);
         nrm = v3((float)nd.x, (float)nd.y, (float)nd.z);
@@ -814,8 +814,8 @@ __device__ __forceinline__ void wave_body(const StepParams& P, const int blk) {
         if (kden != 0.0) { w0 = k0 / kden; w1 = k1 / kden; }
         const D3 pcd = dadd(dscale(x0d, w0), dscale(x1d, w1));
         cpc = v3((float)pcd.x, (float)pcd.y, (float)pcd.z);
-        xs0d = P.point_mode == 0 ? x0d : pcd;
-        xs1d = P.point_mode == 0 ? x1d : pcd;
+        xs0d = dsel(P.point_mode == 0, x0d, pcd);
+        xs1d = dsel(P.point_mode == 0, x1d, pcd);
         if (!isfinite(k0)) kcomb = k1;
         else if (!isfinite(k1)) kcomb = k0;
         else if (kden <= 1.0e-12) kcomb = fmax(k0, k1);
@@ -855,8 +855,8 @@ __device__ __forceinline__ void wave_body(const StepParams& P, const int blk) {
         // equal to S.v + S.w x (x - p_child))
         const unsigned long long anc1 = cb1 >= 0 ? T.body_anc_dof[cb1] : 0ull;
         const unsigned long long anc0 = cb0 >= 0 ? T.body_anc_dof[cb0] : 0ull;
-        const V3 r1 = cb1 >= 0 ? lever(xs1d, cb1) : v3(0, 0, 0);
-        const V3 r0 = cb0 >= 0 ? lever(xs0d, cb0) : v3(0, 0, 0);
+        const V3 r1 = vsel(cb1 >= 0, lever(xs1d, cb1 >= 0 ? cb1 : 0), v3(0, 0, 0));
+        const V3 r0 = vsel(cb0 >= 0, lever(xs0d, cb0 >= 0 ? cb0 : 0), v3(0, 0, 0));
         float* J0 = Jm + (lane * 3) * js;
 #pragma unroll 2
         for (int d = 0; d < n; ++d) {
@@ -886,13 +886,13 @@ __device__ __forceinline__ void wave_body(const StepParams& P, const int blk) {
                 const Q4 q = qnormalize(Q4{bb[3], bb[4], bb[5], bb[6]});
                 const float* com = mCom + lb * 4;
                 const V3 comw = v3(bb[0], bb[1], bb[2]) + qrot(q, v3(com[0], com[1], com[2]));
-                const V3 r = (side == 0 ? cx0 : cx1) - comw;
+                const V3 r = vsel(side == 0, cx0, cx1) - comw;
                 const float* I = mI + lb * 9;
                 const double ix = fmax((double)I[0], 1.0e-12), iy = fmax((double)I[4], 1.0e-12), iz = fmax((double)I[8], 1.0e-12);
                 for (int k = 0; k < 3; ++k) {
                     V3 dn = dirs[k];
                     const float dl = sqrtf(dot(dn, dn));
-                    dn = (dl * dl <= 1.0e-32f) ? v3(0, 0, 0) : dn * (1.0f / dl);
+                    dn = vsel(dl * dl <= 1.0e-32f, v3(0, 0, 0), dn * (1.0f / dl));
                     const V3 cbv = qrot(qconj(q), cross(r, dn));
                     acc += 1.0 / m + (double)cbv.x * cbv.x / ix + (double)cbv.y * cbv.y / iy + (double)cbv.z * cbv.z / iz;
                 }
