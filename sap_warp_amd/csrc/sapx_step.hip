@@ -1139,10 +1139,11 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
     __syncthreads();
 
     // ------------------------------------------------------------------ 8
-    // SAP Newton solve (contact_solve.py:6129-6285, 5474-5655).  Every stage that needs at
-    // most one row / contact / dof per lane runs on wave 0 alone with wave-level ordering
-    // (no block barriers); the Hessian assembly and its factorization use the whole block.
-    // ctl[0] carries wave 0's continue / stop decision to the other waves.
+    // SAP Newton solve (contact_solve.py:6129-6285, 5474-5655).  The mat-vecs (J v, A v, J^T
+    // gamma, the refinement residual, the line-search data), the Hessian assembly and its
+    // factorization use the whole block; projections, reductions, the triangular solves and
+    // the line search run on wave 0 with wave-level ordering.  ctl[0] carries wave 0's
+    // continue / stop decision to the other waves, ctl[1] the unconstrained flag.
     const int lane = tid & 63, wave = tid >> 6;
     int* ctl = reinterpret_cast<int*>(s.red);
     double* v = d64 + V_V * npad;
@@ -1183,55 +1184,70 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
     }
     double alpha = 1.0, prev_cost = 0.0, cost = 0.0;
     const double cost_min_alpha = 0.5;
+    // the mat-vecs of each stage run on the whole workgroup (one row / dof / contact per thread,
+    // each value computed as before); projections, reductions and the line search stay on wave 0
+    if (wave == 0 && lane == 0) ctl[1] = unconstrained ? 1 : 0;
+    __syncthreads();
+    const bool free_world = ctl[1] != 0;
     while (true) {
+        // ---- evaluate at v: vc = J v and A v, projection + model terms, then
+        //      impulses / gradient / norms (contact_solve.py:743-921, 1493-1627)
+        if (!free_world) {
+            for (int t = tid; t < 3 * nc + n; t += NT) {
+                if (t < 3 * nc) {
+                    s.cv[(t / 3) * 9 + (t % 3)] = dot_fd(s.J + t * js, v, n);
+                } else {
+                    const int i = t - 3 * nc;
+                    const double avi = dot_fd(s.A + i * ast, v, n);
+                    av[i] = avi;
+                    d64[V_AR * npad + i] = avi - d64[V_AVS * npad + i];
+                }
+            }
+        }
+        __syncthreads();
+        double part3[4] = {0.0, 0.0, 0.0, 0.0};  // cost, |D g|^2, |D A v|^2, |D j|^2 (wave 0)
+        if (wave == 0 && !free_world) {
+            for (int t = lane; t < nc + n; t += 64) {
+                if (t < nc) {
+                    const int c = t;
+                    double vc[3] = {s.cv[c * 9], s.cv[c * 9 + 1], s.cv[c * 9 + 2]};
+                    double g[3];
+                    part3[0] += project_contact(s.ck + c * kCk, vc, g, s.cG + c * 9, true);
+                    s.cv[c * 9 + 6] = g[0]; s.cv[c * 9 + 7] = g[1]; s.cv[c * 9 + 8] = g[2];
+                } else {
+                    const int i = t - nc;
+                    double imp, hd;
+                    part3[0] += model_terms(s, npad, i, v[i], dt, &imp, &hd);
+                    d32[F_HD * npad + i] = (float)hd;
+                    dp[i] = imp;  // temporarily holds the model impulse
+                }
+            }
+        }
+        __syncthreads();
+        if (!free_world) {
+            for (int i = tid; i < n; i += NT) {
+                double j0a = 0.0, j1a = 0.0, j2a = 0.0;
+                for (int c = 0; c < nc; ++c) {
+                    j0a += (double)s.J[(c * 3) * js + i] * s.cv[c * 9 + 6];
+                    j1a += (double)s.J[(c * 3 + 1) * js + i] * s.cv[c * 9 + 7];
+                    j2a += (double)s.J[(c * 3 + 2) * js + i] * s.cv[c * 9 + 8];
+                }
+                const double jc = dp[i] + ((j0a + j1a) + j2a);
+                jimp[i] = jc;
+                grad[i] = d64[V_AR * npad + i] - jc;
+            }
+        }
+        __syncthreads();
         if (wave == 0) {
             bool stop = false;
-            if (unconstrained) {
+            if (free_world) {
                 flags = 1;
                 stop = true;
             } else {
-                // ---- evaluate at v: vc = J v and A v, projection + model terms, then
-                //      impulses / gradient / norms (contact_solve.py:743-921, 1493-1627)
-                for (int t = lane; t < 3 * nc + n; t += 64) {
-                    if (t < 3 * nc) {
-                        s.cv[(t / 3) * 9 + (t % 3)] = dot_fd(s.J + t * js, v, n);
-                    } else {
-                        const int i = t - 3 * nc;
-                        const double avi = dot_fd(s.A + i * ast, v, n);
-                        av[i] = avi;
-                        d64[V_AR * npad + i] = avi - d64[V_AVS * npad + i];
-                    }
-                }
-                wave_sync();
-                double part3[4] = {0.0, 0.0, 0.0, 0.0};  // cost, |D g|^2, |D A v|^2, |D j|^2
-                for (int t = lane; t < nc + n; t += 64) {
-                    if (t < nc) {
-                        const int c = t;
-                        double vc[3] = {s.cv[c * 9], s.cv[c * 9 + 1], s.cv[c * 9 + 2]};
-                        double g[3];
-                        part3[0] += project_contact(s.ck + c * kCk, vc, g, s.cG + c * 9, true);
-                        s.cv[c * 9 + 6] = g[0]; s.cv[c * 9 + 7] = g[1]; s.cv[c * 9 + 8] = g[2];
-                    } else {
-                        const int i = t - nc;
-                        double imp, hd;
-                        part3[0] += model_terms(s, npad, i, v[i], dt, &imp, &hd);
-                        d32[F_HD * npad + i] = (float)hd;
-                        dp[i] = imp;  // temporarily holds the model impulse
-                    }
-                }
-                wave_sync();
                 for (int i = lane; i < n; i += 64) {
-                    double j0a = 0.0, j1a = 0.0, j2a = 0.0;
-                    for (int c = 0; c < nc; ++c) {
-                        j0a += (double)s.J[(c * 3) * js + i] * s.cv[c * 9 + 6];
-                        j1a += (double)s.J[(c * 3 + 1) * js + i] * s.cv[c * 9 + 7];
-                        j2a += (double)s.J[(c * 3 + 2) * js + i] * s.cv[c * 9 + 8];
-                    }
-                    const double jc = dp[i] + ((j0a + j1a) + j2a);
+                    const double jc = jimp[i];
                     const double ar = d64[V_AR * npad + i];
                     const double avv = av[i];
-                    jimp[i] = jc;
-                    grad[i] = ar - jc;
                     part3[0] += 0.5 * (v[i] - vs[i]) * ar;
                     if (s.dflag[I_PART * npad + i]) {
                         const double sc = d32[F_DSC * npad + i];
@@ -1283,19 +1299,28 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
                 f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
                 const int ia = i0 + (hl & 15);
                 const int jbb = j0 + (hl & 15);
-                for (int k0 = 0; k0 < K; k0 += 4) {
-                    const int k = k0 + (hl >> 4);
-                    float a = 0.0f, b = 0.0f;
-                    if (k < K) {
-                        const int c = k / 3, r = k % 3;
-                        if (ia < n) a = s.J[k * js + ia];
-                        if (jbb < n) {
-                            const float* G = s.cG + c * 9 + r * 3;
-                            const float* Jc = s.J + (c * 3) * js + jbb;
-                            b = G[0] * Jc[0] + G[1] * Jc[js] + G[2] * Jc[2 * js];
+                // four k-steps' operands are loaded before their four MFMAs, so the loads (device
+                // memory for large worlds) overlap instead of each MFMA waiting on its own
+                for (int k0 = 0; k0 < K; k0 += 16) {
+                    float a4[4], b4[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int k = k0 + 4 * u + (hl >> 4);
+                        float a = 0.0f, b = 0.0f;
+                        if (k < K) {
+                            const int c = k / 3, r = k % 3;
+                            if (ia < n) a = s.J[k * js + ia];
+                            if (jbb < n) {
+                                const float* G = s.cG + c * 9 + r * 3;
+                                const float* Jc = s.J + (c * 3) * js + jbb;
+                                b = G[0] * Jc[0] + G[1] * Jc[js] + G[2] * Jc[2 * js];
+                            }
                         }
+                        a4[u] = a;
+                        b4[u] = b;
                     }
-                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a4[u], b4[u], acc, 0, 0, 0);
                 }
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
@@ -1319,52 +1344,67 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
             for (int i = lane; i < npad; i += 64) dvv[i] = (i < n) ? -grad[i] : 0.0;
             wave_sync();
             chol_solve(s.H, npad, hs, n, d32 + F_DINV * npad, dvv);
-            wave_sync();
-            // ---- mixed-precision refinement of the Newton direction: residual
-            //      r = -grad - (A + J^T G J + diag(hd + shift)) dv in float64 from the stored
-            //      float32 operands, correction solved with the float32 factor
-            for (int rep = 0; rep < kNewtonRefine; ++rep) {
-                for (int t = lane; t < 3 * nc + n; t += 64) {
-                    if (t < 3 * nc) {
-                        s.cv[(t / 3) * 9 + 3 + (t % 3)] = dot_fd(s.J + t * js, dvv, n);
-                    } else {
-                        const int i = t - 3 * nc;
-                        dp[i] = dot_fd(s.A + i * ast, dvv, n);
-                    }
-                }
-                wave_sync();
-                for (int i = lane; i < npad; i += 64) {
-                    double r = 0.0;
-                    if (i < n) {
-                        double acc = dp[i] + ((double)d32[F_HD * npad + i] + P.diag_shift) * dvv[i];
-                        for (int c = 0; c < nc; ++c) {
-                            const float* G = s.cG + c * 9;
-                            const double t0 = s.cv[c * 9 + 3], t1 = s.cv[c * 9 + 4], t2 = s.cv[c * 9 + 5];
-                            const double u0 = (double)G[0] * t0 + (double)G[1] * t1 + (double)G[2] * t2;
-                            const double u1 = (double)G[3] * t0 + (double)G[4] * t1 + (double)G[5] * t2;
-                            const double u2 = (double)G[6] * t0 + (double)G[7] * t1 + (double)G[8] * t2;
-                            acc += (double)s.J[(c * 3) * js + i] * u0 + (double)s.J[(c * 3 + 1) * js + i] * u1 +
-                                   (double)s.J[(c * 3 + 2) * js + i] * u2;
-                        }
-                        r = -grad[i] - acc;
-                    }
-                    d64[V_T1 * npad + i] = r;
-                }
-                wave_sync();
-                chol_solve(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
-                wave_sync();
-                for (int i = lane; i < n; i += 64) dvv[i] += d64[V_T1 * npad + i];
-                wave_sync();
-            }
-            // ---- line search data: dvc = J dv and dp = A dv in one stage (contact_solve.py:2234-2263)
-            double ld[4] = {0, 0, 0, 0};  // dell0, dell_a0, d2ell_a, m0
-            for (int t = lane; t < 3 * nc + n; t += 64) {
+        }
+        __syncthreads();
+        // ---- mixed-precision refinement of the Newton direction: residual
+        //      r = -grad - (A + J^T G J + diag(hd + shift)) dv in float64 from the stored
+        //      float32 operands, correction solved with the float32 factor
+        for (int rep = 0; rep < kNewtonRefine; ++rep) {
+            for (int t = tid; t < 3 * nc + n; t += NT) {
                 if (t < 3 * nc) {
                     s.cv[(t / 3) * 9 + 3 + (t % 3)] = dot_fd(s.J + t * js, dvv, n);
                 } else {
                     const int i = t - 3 * nc;
-                    const double acc = dot_fd(s.A + i * ast, dvv, n);
-                    dp[i] = acc;
+                    dp[i] = dot_fd(s.A + i * ast, dvv, n);
+                }
+            }
+            __syncthreads();
+            // G_c (J_c dv) once per contact, in place of J_c dv (recomputed before it is read again)
+            for (int c = tid; c < nc; c += NT) {
+                const float* G = s.cG + c * 9;
+                const double t0 = s.cv[c * 9 + 3], t1 = s.cv[c * 9 + 4], t2 = s.cv[c * 9 + 5];
+                s.cv[c * 9 + 3] = (double)G[0] * t0 + (double)G[1] * t1 + (double)G[2] * t2;
+                s.cv[c * 9 + 4] = (double)G[3] * t0 + (double)G[4] * t1 + (double)G[5] * t2;
+                s.cv[c * 9 + 5] = (double)G[6] * t0 + (double)G[7] * t1 + (double)G[8] * t2;
+            }
+            __syncthreads();
+            for (int i = tid; i < npad; i += NT) {
+                double r = 0.0;
+                if (i < n) {
+                    double acc = dp[i] + ((double)d32[F_HD * npad + i] + P.diag_shift) * dvv[i];
+                    for (int c = 0; c < nc; ++c) {
+                        const double u0 = s.cv[c * 9 + 3], u1 = s.cv[c * 9 + 4], u2 = s.cv[c * 9 + 5];
+                        acc += (double)s.J[(c * 3) * js + i] * u0 + (double)s.J[(c * 3 + 1) * js + i] * u1 +
+                               (double)s.J[(c * 3 + 2) * js + i] * u2;
+                    }
+                    r = -grad[i] - acc;
+                }
+                d64[V_T1 * npad + i] = r;
+            }
+            __syncthreads();
+            if (wave == 0) {
+                chol_solve(s.H, npad, hs, n, d32 + F_DINV * npad, d64 + V_T1 * npad);
+                wave_sync();
+                for (int i = lane; i < n; i += 64) dvv[i] += d64[V_T1 * npad + i];
+            }
+            __syncthreads();
+        }
+        // ---- line search data: dvc = J dv and dp = A dv (contact_solve.py:2234-2263)
+        for (int t = tid; t < 3 * nc + n; t += NT) {
+            if (t < 3 * nc) {
+                s.cv[(t / 3) * 9 + 3 + (t % 3)] = dot_fd(s.J + t * js, dvv, n);
+            } else {
+                const int i = t - 3 * nc;
+                dp[i] = dot_fd(s.A + i * ast, dvv, n);
+            }
+        }
+        __syncthreads();
+        if (wave == 0) {
+            double ld[4] = {0, 0, 0, 0};  // dell0, dell_a0, d2ell_a, m0
+            for (int t = lane; t < 3 * nc + n; t += 64) {
+                if (t >= 3 * nc) {
+                    const int i = t - 3 * nc;
+                    const double acc = dp[i];
                     const double ri = v[i] - vs[i];
                     ld[0] += grad[i] * dvv[i];
                     ld[1] += acc * ri;
@@ -1522,6 +1562,7 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
             wave_sync();
             ++iteration;
         }
+        __syncthreads();  // the committed v before the next evaluation's mat-vecs
     }
     __syncthreads();
 
