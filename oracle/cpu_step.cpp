@@ -29,7 +29,7 @@
 
 namespace {
 
-constexpr int kPrismatic = 0, kRevolute = 1, kBall = 2, kFixed = 3, kFree = 4, kDistance = 5;
+constexpr int kPrismatic = 0, kRevolute = 1, kBall = 2, kFixed = 3, kFree = 4, kDistance = 5, kD6 = 6;
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kSoftNormTol = 1.0e-7;     // contact_solve.py SOFT_NORM_TOL
 constexpr double kPdBeta = 0.1;             // sap_helpers.py near-rigid PD beta
@@ -89,6 +89,47 @@ inline TF tcompose(TF a, TF b) {
 inline TF tinverse(TF x) {
     const Q qi = qconj(qnormalize(x.q));
     return TF{-qrot_unit(qi, x.p), qi};
+}
+// Quaternion of the matrix with columns c0, c1, c2, normalised (``_quatd_from_matrix_cols``,
+// sap_helpers.py:607-652).
+inline Q q_from_cols(V3 c0, V3 c1, V3 c2) {
+    const double tr = c0.x + c1.y + c2.z;
+    Q q;
+    if (tr > 0.0) {
+        const double s = std::sqrt(tr + 1.0) * 2.0;
+        q = Q{(c1.z - c2.y) / s, (c2.x - c0.z) / s, (c0.y - c1.x) / s, 0.25 * s};
+    } else if (c0.x > c1.y && c0.x > c2.z) {
+        const double s = std::sqrt(1.0 + c0.x - c1.y - c2.z) * 2.0;
+        q = Q{0.25 * s, (c1.x + c0.y) / s, (c2.x + c0.z) / s, (c1.z - c2.y) / s};
+    } else if (c1.y > c2.z) {
+        const double s = std::sqrt(1.0 + c1.y - c0.x - c2.z) * 2.0;
+        q = Q{(c1.x + c0.y) / s, 0.25 * s, (c2.y + c1.z) / s, (c2.x - c0.z) / s};
+    } else {
+        const double s = std::sqrt(1.0 + c2.z - c0.x - c1.y) * 2.0;
+        q = Q{(c2.x + c0.z) / s, (c2.y + c1.z) / s, 0.25 * s, (c0.y - c1.x) / s};
+    }
+    return qnormalize(q);
+}
+// Rotation of a D6 joint's na angular coordinates q and each angular dof's current axis (joint
+// frame): one axis turns by its angle; two or three are first replaced by the frame their
+// matrix defines (third column a0 x a1 for two), then turned in sequence, each later axis
+// carried by the rotations before it (sap_helpers.py:1109-1161, 1247-1286).
+template <class T>
+inline Q d6_rotation(int na, const V3* a, const T* q, V3* axis) {
+    if (na <= 0) return Q{};
+    if (na == 1) {
+        axis[0] = a[0];
+        return q_axis_angle(a[0], (double)q[0]);
+    }
+    const Q off = q_from_cols(a[0], a[1], na == 2 ? cross(a[0], a[1]) : a[2]);
+    const V3 l0 = qrot(off, v3(1, 0, 0)), l1 = qrot(off, v3(0, 1, 0)), l2 = qrot(off, v3(0, 0, 1));
+    const Q q0 = q_axis_angle(l0, (double)q[0]);
+    axis[0] = l0;
+    axis[1] = qrot(q0, l1);
+    const Q q10 = qmul(q_axis_angle(axis[1], (double)q[1]), q0);
+    if (na == 2) return q10;
+    axis[2] = qrot(q10, l2);
+    return qmul(q_axis_angle(axis[2], (double)q[2]), q10);
 }
 inline V3 tpoint(TF x, V3 p) { return x.p + qrot(x.q, p); }
 inline bool is_identity(const float* x) {
@@ -432,6 +473,8 @@ struct World {
             const TF Xwpj = parent < 0 ? Xp : tcompose(bq[parent], Xp);
             const double* qd = v0.data() + ds;
             V3 ax0 = nd > 0 ? ld3(M.joint_axis + (size_t)(d0 + ds) * 3) : V3{};
+            V3 d6ax[6], d6cur[3];
+            int nlin = 0;
             TF Xwc;
             SV vPB, aPB;
             if (jt == kRevolute && xc_id) {
@@ -448,7 +491,13 @@ struct World {
                 else if (jt == kFree || jt == kDistance)
                     Xj = TF{v3(joint_q[qs], joint_q[qs + 1], joint_q[qs + 2]),
                             Q{joint_q[qs + 3], joint_q[qs + 4], joint_q[qs + 5], joint_q[qs + 6]}};
-                else Xj = TF{};
+                else if (jt == kD6) {
+                    nlin = M.joint_dof_dim[2 * j];
+                    for (int k = 0; k < nd; ++k) d6ax[k] = ld3(M.joint_axis + (size_t)(d0 + ds + k) * 3);
+                    V3 pos{};
+                    for (int k = 0; k < nlin; ++k) pos = pos + d6ax[k] * joint_q[qs + k];
+                    Xj = TF{pos, d6_rotation(nd - nlin, d6ax + nlin, joint_q + qs + nlin, d6cur)};
+                } else Xj = TF{};
                 const TF XWM = tcompose(Xwpj, Xj);
                 Xwc = xc_id ? XWM : tcompose(XWM, tinverse(Xc));
                 const TF XMB = tinverse(Xc);
@@ -481,6 +530,17 @@ struct World {
                         S[ds + k] = mob_to_body(Xwpj, p, SV{e, V3{}});
                         S[ds + 3 + k] = mob_to_body(Xwpj, p, SV{V3{}, e});
                     }
+                    aPB = mob_bias_to_body(Xwpj, p, VFM, SV{});
+                } else if (jt == kD6) {
+                    // linear axes fixed in the joint frame, angular axes at the current angles
+                    // (sap_helpers.py:1387-1432)
+                    SV VFM{};
+                    for (int k = 0; k < nd; ++k) {
+                        const SV col = k < nlin ? SV{V3{}, d6ax[k]} : SV{d6cur[k - nlin], V3{}};
+                        S[ds + k] = mob_to_body(Xwpj, p, col);
+                        VFM = VFM + col * qd[k];
+                    }
+                    vPB = mob_to_body(Xwpj, p, VFM);
                     aPB = mob_bias_to_body(Xwpj, p, VFM, SV{});
                 } else {
                     vPB = SV{};
@@ -1227,6 +1287,18 @@ struct World {
                 Xj = TF{v3(q[qs], q[qs + 1], q[qs + 2]), Q{q[qs + 3], q[qs + 4], q[qs + 5], q[qs + 6]}};
                 jl = v3(qd[ds], qd[ds + 1], qd[ds + 2]);
                 ja = v3(qd[ds + 3], qd[ds + 4], qd[ds + 5]);
+            } else if (jt == kD6) {
+                // sap_kinematics.py:170-218
+                const int nl = M.joint_dof_dim[2 * j], na = M.joint_dof_dim[2 * j + 1];
+                V3 a[6], cur[3];
+                for (int k = 0; k < nl + na; ++k) a[k] = ld3(M.joint_axis + (size_t)(d0 + ds + k) * 3);
+                V3 pos{};
+                for (int k = 0; k < nl; ++k) {
+                    pos = pos + a[k] * q[qs + k];
+                    jl = jl + a[k] * qd[ds + k];
+                }
+                Xj = TF{pos, d6_rotation(na, a + nl, q.data() + qs + nl, cur)};
+                for (int k = 0; k < na; ++k) ja = ja + cur[k] * qd[ds + nl + k];
             }
             X[child] = tmul(tmul(Xwpj, Xj), tinv(Xcj));
             T[child] = SV{vlin + qrot_unit(Xwpj.q, jl), vang + qrot_unit(Xwpj.q, ja)};
@@ -1266,7 +1338,7 @@ int sapx_cpu_step(const Model* m, const State* st, const Control* ctl, const Con
                 target[ds + a] = ds + a + 3;
                 target[ds + a + 3] = ds + a;
             }
-        if (jt != kPrismatic && jt != kRevolute) continue;
+        if (jt != kPrismatic && jt != kRevolute && jt != kD6) continue;
         const int ac = M.joint_dof_dim[2 * j] + M.joint_dof_dim[2 * j + 1];
         for (int a = 0; a < ac; ++a) {
             coord[ds + a] = M.joint_q_start[j] + a;

@@ -316,8 +316,66 @@ class FreeMotion:
     body_jac: np.ndarray    # (nb, 6, n) body-origin Jacobian per body
 
 
-def _mobilizer_transform(jt, axes, q):
-    """``_sap_calc_across_mobilizer_transform`` sap_helpers.py:1164-1243 (no D6 here)."""
+def quat_from_cols(c0, c1, c2):
+    """Quaternion of the matrix with columns c0, c1, c2, normalised (``_quatd_from_matrix_cols``
+    sap_helpers.py:607-652: the trace branch, else the largest diagonal entry's)."""
+    r00, r10, r20 = c0
+    r01, r11, r21 = c1
+    r02, r12, r22 = c2
+    tr = r00 + r11 + r22
+    if tr > 0.0:
+        s = math.sqrt(tr + 1.0) * 2.0
+        q = [(r21 - r12) / s, (r02 - r20) / s, (r10 - r01) / s, 0.25 * s]
+    elif r00 > r11 and r00 > r22:
+        s = math.sqrt(1.0 + r00 - r11 - r22) * 2.0
+        q = [0.25 * s, (r01 + r10) / s, (r02 + r20) / s, (r21 - r12) / s]
+    elif r11 > r22:
+        s = math.sqrt(1.0 + r11 - r00 - r22) * 2.0
+        q = [(r01 + r10) / s, 0.25 * s, (r12 + r21) / s, (r02 - r20) / s]
+    else:
+        s = math.sqrt(1.0 + r22 - r00 - r11) * 2.0
+        q = [(r02 + r20) / s, (r12 + r21) / s, 0.25 * s, (r10 - r01) / s]
+    return qnormalize(np.array(q, dtype=np.float64))
+
+
+def d6_rotation(axes, q):
+    """Rotation of a D6 joint's angular coordinates and the current axis of each angular dof
+    (``_sap_compute_2d/3d_rotational_dofs`` sap_helpers.py:1109-1161, ``_d6_angular_axis``
+    1247-1286): one axis rotates by q about it; two or three are first carried onto the frame
+    whose columns they form (the third column of two is their cross product), then rotated in
+    turn, each axis moved by the rotations before it (intrinsic sequence)."""
+    m = len(axes)
+    if m == 0:
+        return np.array([0.0, 0.0, 0.0, 1.0]), []
+    if m == 1:
+        a0 = np.asarray(axes[0], dtype=np.float64)
+        return q_from_axis_angle(a0, q[0]), [a0]
+    a0, a1 = np.asarray(axes[0], dtype=np.float64), np.asarray(axes[1], dtype=np.float64)
+    a2 = np.cross(a0, a1) if m == 2 else np.asarray(axes[2], dtype=np.float64)
+    q_off = quat_from_cols(a0, a1, a2)
+    l0 = qrot(q_off, np.array([1.0, 0.0, 0.0]))
+    l1 = qrot(q_off, np.array([0.0, 1.0, 0.0]))
+    q0 = q_from_axis_angle(l0, q[0])
+    ax1 = qrot(q0, l1)
+    q1 = q_from_axis_angle(ax1, q[1])
+    q10 = qmul(q1, q0)
+    if m == 2:
+        return q10, [l0, ax1]
+    l2 = qrot(q_off, np.array([0.0, 0.0, 1.0]))
+    ax2 = qrot(q10, l2)
+    q2 = q_from_axis_angle(ax2, q[2])
+    return qmul(q2, q10), [l0, ax1, ax2]
+
+
+def _mobilizer_transform(jt, axes, q, nlin=0):
+    """``_sap_calc_across_mobilizer_transform`` sap_helpers.py:1164-1243 (``nlin``: a D6 joint's
+    linear axis count; its axes list is linear then angular)."""
+    if jt == JOINT_D6:
+        pos = np.zeros(3)
+        for k in range(nlin):
+            pos = pos + np.asarray(axes[k], dtype=np.float64) * q[k]
+        rot, _ = d6_rotation(axes[nlin:], q[nlin:])
+        return np.concatenate([pos, rot])
     if jt == JOINT_PRISMATIC:
         return np.concatenate([axes[0] * q[0], [0.0, 0.0, 0.0, 1.0]])
     if jt == JOINT_REVOLUTE:
@@ -372,7 +430,8 @@ def free_motion(model, wv: WorldView, joint_q, v0, joint_f_sap, body_f_sap, dt) 
             v_PB = S[ds] * qd[0]
             A_PB = np.zeros(6)
         else:
-            X_j = _mobilizer_transform(jt, axes, joint_q[qs:])
+            nlin = int(model.joint_dof_dim[j, 0]) if jt == JOINT_D6 else 0
+            X_j = _mobilizer_transform(jt, axes, joint_q[qs:], nlin)
             X_WM = tcompose(X_wpj, X_j)
             X_wc = X_WM if X_c_ident else tcompose(X_WM, tinverse(X_c))
             # _sap_calc_mobilizer_velocity_and_bias_body_world (sap_helpers.py:1289-1433)
@@ -406,6 +465,17 @@ def free_motion(model, wv: WorldView, joint_q, v0, joint_f_sap, body_f_sap, dt) 
                     e[k] = 1.0
                     S[ds + k] = mobilizer_to_body(X_wpj, p_MoBo_F, sp(e, np.zeros(3)))
                     S[ds + 3 + k] = mobilizer_to_body(X_wpj, p_MoBo_F, sp(np.zeros(3), e))
+                A_PB = mobilizer_bias_to_body(X_wpj, p_MoBo_F, V_FM, np.zeros(6))
+            elif jt == JOINT_D6:
+                # linear axes, then the angular axes at the current coordinates (sap_helpers.py:
+                # 1387-1432); the bias is the body offset's velocity product only
+                _, ang = d6_rotation(axes[nlin:], joint_q[qs + nlin:])
+                V_FM = np.zeros(6)
+                for k in range(nd):
+                    col = sp(np.zeros(3), axes[k]) if k < nlin else sp(ang[k - nlin], np.zeros(3))
+                    S[ds + k] = mobilizer_to_body(X_wpj, p_MoBo_F, col)
+                    V_FM = V_FM + col * qd[k]
+                v_PB = mobilizer_to_body(X_wpj, p_MoBo_F, V_FM)
                 A_PB = mobilizer_bias_to_body(X_wpj, p_MoBo_F, V_FM, np.zeros(6))
             elif jt == JOINT_FIXED:
                 v_PB = np.zeros(6)
@@ -1506,6 +1576,21 @@ def public_fk(model, wv: WorldView, joint_q_world, joint_qd_world):
         elif jt in (JOINT_FREE, JOINT_DISTANCE):
             X_j = np.asarray(joint_q_world[qs:qs + 7], dtype=np.float64)
             v_j = np.asarray(joint_qd_world[ds:ds + 6], dtype=np.float64)
+        elif jt == JOINT_D6:
+            # sap_kinematics.py:170-218: linear axes sum into the offset, the angular ones rotate
+            nl, na = int(model.joint_dof_dim[j, 0]), int(model.joint_dof_dim[j, 1])
+            axes = [model.joint_axis[wv.d0 + ds + k].astype(np.float64) for k in range(nl + na)]
+            pos = np.zeros(3)
+            vv = np.zeros(3)
+            for k in range(nl):
+                pos = pos + axes[k] * joint_q_world[qs + k]
+                vv = vv + axes[k] * joint_qd_world[ds + k]
+            rot, ang = d6_rotation(axes[nl:], joint_q_world[qs + nl:qs + nl + na])
+            vw = np.zeros(3)
+            for k in range(na):
+                vw = vw + ang[k] * joint_qd_world[ds + nl + k]
+            X_j = np.concatenate([pos, rot])
+            v_j = np.concatenate([vv, vw])
         X_wc = tmul(tmul(X_wpj, X_j), tinv(X_cj))
         body_q[child] = X_wc
         body_qd[child] = v_wpj + np.concatenate([qrot_unit(X_wpj[3:], v_j[:3]), qrot_unit(X_wpj[3:], v_j[3:])])

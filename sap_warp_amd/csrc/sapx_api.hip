@@ -295,7 +295,7 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
         return fail(SAPX_ERR_CAPACITY, "more than 64 joints or bodies per world");
     if (N > kMaxDofFused) return fail(SAPX_ERR_CAPACITY, "more than 256 dofs per world");
     // ---- topology template + equal-world validation
-    std::vector<int> t_jtype(J), t_jparent(J), t_jchild(J), t_jqs(J + 1), t_jds(J + 1);
+    std::vector<int> t_jtype(J), t_jparent(J), t_jchild(J), t_jqs(J + 1), t_jds(J + 1), t_jlin(J, 0);
     for (int w = 0; w < W; ++w) {
         for (int j = 0; j < J; ++j) {
             const int g = w * J + j;
@@ -306,10 +306,22 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
             const int ds = d->joint_qd_start[g] - w * N;
             if (w == 0) {
                 t_jtype[j] = jt; t_jparent[j] = par; t_jchild[j] = chi; t_jqs[j] = qs; t_jds[j] = ds;
-                if (jt == kJointD6) return fail(SAPX_ERR_UNSUPPORTED, "D6 joints are not in the native step");
+                if (jt == kJointD6) {
+                    // D6: linear then angular axes, one coordinate per dof (sap_helpers.py:1164-1243)
+                    if (!d->joint_dof_dim) return fail(SAPX_ERR_INVALID, "D6 joints need joint_dof_dim");
+                    const int nl = d->joint_dof_dim[2 * g], na = d->joint_dof_dim[2 * g + 1];
+                    const int nd = d->joint_qd_start[g + 1] - d->joint_qd_start[g];
+                    const int nq = d->joint_q_start[g + 1] - d->joint_q_start[g];
+                    if (nl < 0 || nl > 3 || na < 0 || na > 3 || nl + na != nd || nq != nd)
+                        return fail(SAPX_ERR_INVALID, "D6 joint " + std::to_string(j) +
+                                    ": needs 0-3 linear and 0-3 angular axes, one coordinate per dof");
+                    if (nd == 0) return fail(SAPX_ERR_UNSUPPORTED, "D6 joint without axes (use a fixed joint)");
+                    t_jlin[j] = nl;
+                }
                 if (chi < 0 || chi >= NB) return fail(SAPX_ERR_INVALID, "joint child outside its world");
                 if (par >= NB) return fail(SAPX_ERR_INVALID, "joint parent outside its world");
-            } else if (jt != t_jtype[j] || par != t_jparent[j] || chi != t_jchild[j] || qs != t_jqs[j] || ds != t_jds[j]) {
+            } else if (jt != t_jtype[j] || par != t_jparent[j] || chi != t_jchild[j] || qs != t_jqs[j] || ds != t_jds[j] ||
+                       (jt == kJointD6 && d->joint_dof_dim[2 * g] != t_jlin[j])) {
                 return fail(SAPX_ERR_UNSUPPORTED, "worlds must share one topology (world " + std::to_string(w) + ")");
             }
         }
@@ -375,7 +387,7 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
                 dof_target[t_jds[j] + a + 3] = t_jds[j] + a;
             }
         }
-        if (jt == kJointPrismatic || jt == kJointRevolute) {
+        if (jt == kJointPrismatic || jt == kJointRevolute || jt == kJointD6) {
             for (int a = 0; a < nd; ++a) {
                 dof_coord[t_jds[j] + a] = t_jqs[j] + a;
                 if (nd == 1) dof_limit[t_jds[j] + a] = 1;
@@ -517,6 +529,7 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
             }
             add(TP_BODY_ROOT, root.data(), NB);
         }
+        add(TP_JLIN, t_jlin.data(), J);
         if (topo.size() % 2) topo.push_back(0);  // 8-byte alignment of the u64 masks
         auto add64 = [&](int field, const std::vector<unsigned long long>& v) {
             dm.topo_off[field] = (int)topo.size();

@@ -55,7 +55,7 @@ constexpr int kMaxDofFused = 256; // dofs per world of the 256-thread kernel (on
 enum TopoField {
     TP_JTYPE = 0, TP_JPARENT, TP_JCHILD, TP_JQS, TP_JDS, TP_LEVEL_START, TP_LEVEL_JOINT, TP_DOF_JOINT,
     TP_DOF_COORD, TP_DOF_TARGET, TP_DOF_LIMIT_OK, TP_BODY_ART_START, TP_BODY_ART_COUNT, TP_CHILD_START,
-    TP_CHILD, TP_BODY_ANC, TP_BODY_ANC_DOF, TP_DOF_ANC, TP_BODY_UP, TP_BODY_ROOT, TP_COUNT
+    TP_CHILD, TP_BODY_ANC, TP_BODY_ANC_DOF, TP_DOF_ANC, TP_BODY_UP, TP_BODY_ROOT, TP_JLIN, TP_COUNT
 };
 
 struct DevModel {
@@ -289,5 +289,56 @@ __device__ __forceinline__ DTF dtf_inverse_n(DTF x) {
 }
 // x.p + rotate(normalize(x.q), p)  (_transform_point_d, sap_helpers.py:240-266)
 __device__ __forceinline__ D3 dtf_point(DTF x, D3 p) { return dadd(x.p, dqrot_unit(dqnormalize(x.q), p)); }
+
+// ---------------------------------------------------------------------------
+// D6 joints: up to three linear axes fixed in the joint frame, then up to three
+// angular axes.  One angular axis turns by its angle; two or three are first
+// replaced by the frame that the matrix of their columns defines (the third
+// column a0 x a1 for two), then applied in turn, each later axis carried by the
+// rotations before it (sap_helpers.py:1109-1161, 1247-1286).  float64, as the
+// reference's free motion.
+// ---------------------------------------------------------------------------
+// unit quaternion of the matrix [c0 c1 c2]: trace branch, else the largest diagonal entry's
+// (sap_helpers.py:607-652)
+__device__ inline DQ dq_from_cols(D3 c0, D3 c1, D3 c2) {
+    const double tr = c0.x + c1.y + c2.z;
+    DQ q;
+    if (tr > 0.0) {
+        const double s = sqrt(tr + 1.0) * 2.0;
+        q = DQ{(c1.z - c2.y) / s, (c2.x - c0.z) / s, (c0.y - c1.x) / s, 0.25 * s};
+    } else if (c0.x > c1.y && c0.x > c2.z) {
+        const double s = sqrt(1.0 + c0.x - c1.y - c2.z) * 2.0;
+        q = DQ{0.25 * s, (c1.x + c0.y) / s, (c2.x + c0.z) / s, (c1.z - c2.y) / s};
+    } else if (c1.y > c2.z) {
+        const double s = sqrt(1.0 + c1.y - c0.x - c2.z) * 2.0;
+        q = DQ{(c1.x + c0.y) / s, 0.25 * s, (c2.y + c1.z) / s, (c2.x - c0.z) / s};
+    } else {
+        const double s = sqrt(1.0 + c2.z - c0.x - c1.y) * 2.0;
+        q = DQ{(c2.x + c0.z) / s, (c2.y + c1.z) / s, 0.25 * s, (c0.y - c1.x) / s};
+    }
+    return dqnormalize(q);
+}
+struct D6Ang {
+    DQ rot;           // rotation of the angular coordinates
+    D3 a0, a1, a2;    // current axis of angular dof 0, 1, 2 (joint frame)
+};
+// na angular axes c0..c2 (unused ones ignored) at angles q0..q2
+__device__ inline D6Ang d6_angular(int na, D3 c0, D3 c1, D3 c2, double q0, double q1, double q2) {
+    D6Ang r{DQ{0.0, 0.0, 0.0, 1.0}, c0, c1, c2};
+    if (na == 1) {
+        r.rot = dq_axis_angle(c0, q0);
+    } else if (na >= 2) {
+        const DQ off = dq_from_cols(c0, c1, na == 2 ? dcross(c0, c1) : c2);
+        r.a0 = dqrot_unit(off, d3(1.0, 0.0, 0.0));
+        const D3 l1 = dqrot_unit(off, d3(0.0, 1.0, 0.0));
+        const D3 l2 = dqrot_unit(off, d3(0.0, 0.0, 1.0));
+        const DQ r0 = dq_axis_angle(r.a0, q0);
+        r.a1 = dqrot_unit(dqnormalize(r0), l1);
+        const DQ r10 = dqmul(dq_axis_angle(r.a1, q1), r0);
+        r.a2 = dqrot_unit(dqnormalize(r10), l2);
+        r.rot = na == 2 ? r10 : dqmul(dq_axis_angle(r.a2, q2), r10);
+    }
+    return r;
+}
 
 }  // namespace sapx

@@ -154,6 +154,7 @@ struct Topo {
     const unsigned long long* dof_anc;       // N: body_anc_dof of the child body of dof d's joint
     const int* body_up;                      // fk_rounds x NB: ancestor body 2^r levels up, -1 past the root
     const int* body_root;                    // NB: root body of the body's tree
+    const int* jlin;                         // J: a D6 joint's linear axis count (0 otherwise)
 };
 
 __device__ __forceinline__ Topo topo_view(const int* t, const DevModel& M) {
@@ -177,6 +178,7 @@ __device__ __forceinline__ Topo topo_view(const int* t, const DevModel& M) {
     T.body_anc_dof = reinterpret_cast<const unsigned long long*>(t + M.topo_off[TP_BODY_ANC_DOF]);
     T.dof_anc = reinterpret_cast<const unsigned long long*>(t + M.topo_off[TP_DOF_ANC]);
     T.body_up = t + M.topo_off[TP_BODY_UP];
+    T.jlin = t + M.topo_off[TP_JLIN];
     T.body_root = t + M.topo_off[TP_BODY_ROOT];
     return T;
 }

@@ -541,6 +541,7 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
         double plain = 0.0;  // 1: X_wc = [X_wpj.p, X_wpj.q * q_FM] without normalisation
         SV vl = sv(v3(0, 0, 0), v3(0, 0, 0));
         SV al = vl;
+        D6Ang d6;
         if (jt == kJointRevolute && xcid) {
             // _sap_revolute_identity_child_kinematics (sap_helpers.py:704-720)
             Xl.q = dq_axis_angle(axd, (double)q_in[qs]);
@@ -559,6 +560,24 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
                 Xj = DTF{d3(q_in[qs], q_in[qs + 1], q_in[qs + 2]), DQ{q_in[qs + 3], q_in[qs + 4], q_in[qs + 5], q_in[qs + 6]}};
             } else if (jt == kJointBall) {
                 Xj.q = DQ{q_in[qs], q_in[qs + 1], q_in[qs + 2], q_in[qs + 3]};
+            } else if (jt == kJointD6) {
+                // linear coordinates along their axes, then the angular rotation (indices clamped
+                // into the joint's own dofs; validated at model creation: 1 <= nd <= 6)
+                const int nl = T.jlin[j], last = T.jds[j + 1] - ds - 1;
+                D3 c[3];
+                double qa[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int kl = min(k, last), ka = min(nl + k, last);
+                    const float* al = mAx + (ds + kl) * 4;
+                    const double ql = k < nl ? (double)q_in[qs + kl] : 0.0;
+                    Xj.p = dadd(Xj.p, dscale(d3(al[0], al[1], al[2]), ql));
+                    const float* aa = mAx + (ds + ka) * 4;
+                    c[k] = d3(aa[0], aa[1], aa[2]);
+                    qa[k] = (double)q_in[qs + ka];
+                }
+                d6 = d6_angular(last + 1 - nl, c[0], c[1], c[2], qa[0], qa[1], qa[2]);
+                Xj.q = d6.rot;
             }
             Xl = xcid ? Xj : dtf_compose_n(Xj, dtf_inverse_n(dtf_from(Xc)));
             // mobilizer velocity / bias in the parent joint frame (sap_helpers.py:1289-1433)
@@ -576,6 +595,25 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
                 const V3 wf = axis * qd;
                 al = sv(v3(0, 0, 0), cross(wf, cross(wf, pmb)));
                 vl = sv_scale(S, qd);
+            } else if (jt == kJointD6) {
+                // linear axes fixed in the joint frame, angular axes at the current angles
+                // (sap_helpers.py:1387-1432); the bias is w x (w x p) of the summed angular rate
+                const int nl = T.jlin[j], nd = T.jds[j + 1] - ds;
+                V3 wv = v3(0, 0, 0), vv = v3(0, 0, 0);
+                for (int k = 0; k < nd; ++k) {
+                    const bool lin = k < nl;
+                    const int m = k - nl;
+                    const D3 ad = dsel(m <= 0, d6.a0, dsel(m == 1, d6.a1, d6.a2));
+                    const float* ap = mAx + (ds + k) * 4;
+                    const V3 a = vsel(lin, v3(ap[0], ap[1], ap[2]), v3((float)ad.x, (float)ad.y, (float)ad.z));
+                    const V3 z = v3(0, 0, 0);
+                    store_sv(s.S + (ds + k) * 6, sv(vsel(lin, z, a), vsel(lin, a, cross(a, pmb))));
+                    const float qd = (float)d64[V_V0 * npad + ds + k];
+                    wv = wv + vsel(lin, z, a * qd);
+                    vv = vv + vsel(lin, a * qd, z);
+                }
+                vl = sv(wv, vv + cross(wv, pmb));
+                al = sv(v3(0, 0, 0), cross(wv, cross(wv, pmb)));
             } else if (jt == kJointFree || jt == kJointDistance) {
                 const double* v0 = d64 + V_V0 * npad + ds;
                 const V3 wv = v3((float)v0[0], (float)v0[1], (float)v0[2]);
@@ -1663,6 +1701,30 @@ __global__ __launch_bounds__(NT, 2) void sapx_step_kernel(StepParams P) {
         } else if (jt == kJointBall) {
             Xj.q = Q4{q_out[qs], q_out[qs + 1], q_out[qs + 2], q_out[qs + 3]};
             ja = v3(qd_pub[ds], qd_pub[ds + 1], qd_pub[ds + 2]);
+        } else if (jt == kJointD6) {
+            // sap_kinematics.py:170-218 (the rotation in float64, rounded once)
+            const int nl = T.jlin[j], last = T.jds[j + 1] - ds - 1;
+            D3 c[3];
+            double qa[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int kl = min(k, last), ka = min(nl + k, last);
+                const float* al = mAx + (ds + kl) * 4;
+                const float on = k < nl ? 1.0f : 0.0f;
+                Xj.p = Xj.p + v3(al[0], al[1], al[2]) * (on * q_out[qs + kl]);
+                jl = jl + v3(al[0], al[1], al[2]) * (on * qd_pub[ds + kl]);
+                const float* aa = mAx + (ds + ka) * 4;
+                c[k] = d3(aa[0], aa[1], aa[2]);
+                qa[k] = (double)q_out[qs + ka];
+            }
+            const int na = last + 1 - nl;
+            const D6Ang r = d6_angular(na, c[0], c[1], c[2], qa[0], qa[1], qa[2]);
+            Xj.q = Q4{(float)r.rot.x, (float)r.rot.y, (float)r.rot.z, (float)r.rot.w};
+            const float w0 = na > 0 ? qd_pub[ds + min(nl, last)] : 0.0f;
+            const float w1 = na > 1 ? qd_pub[ds + min(nl + 1, last)] : 0.0f;
+            const float w2 = na > 2 ? qd_pub[ds + min(nl + 2, last)] : 0.0f;
+            ja = v3((float)r.a0.x, (float)r.a0.y, (float)r.a0.z) * w0 + v3((float)r.a1.x, (float)r.a1.y, (float)r.a1.z) * w1 +
+                 v3((float)r.a2.x, (float)r.a2.y, (float)r.a2.z) * w2;
         }
         const TF Xjc = tf_mul(Xj, tf_inverse(load_tf(mXc + j * 8)));
         float* o = kPl + j * 16;

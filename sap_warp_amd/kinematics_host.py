@@ -12,7 +12,14 @@ from __future__ import annotations
 
 import numpy as np
 
-from .runtime import SAP_JOINT_BALL, SAP_JOINT_DISTANCE, SAP_JOINT_FREE, SAP_JOINT_PRISMATIC, SAP_JOINT_REVOLUTE
+from .runtime import (
+    SAP_JOINT_BALL,
+    SAP_JOINT_D6,
+    SAP_JOINT_DISTANCE,
+    SAP_JOINT_FREE,
+    SAP_JOINT_PRISMATIC,
+    SAP_JOINT_REVOLUTE,
+)
 
 
 def _qmul(a, b):
@@ -35,6 +42,75 @@ def _tmul(a, b):
 def _tinv(a):
     qi = a[..., 3:] * np.array([-1.0, -1.0, -1.0, 1.0])
     return np.concatenate([-_qrot(qi, a[..., :3]), qi], axis=-1)
+
+
+def quat_from_cols(c0, c1, c2):
+    """Unit quaternion (xyzw) of the 3x3 matrix with columns c0, c1, c2: the trace branch when the
+    trace is positive, else the branch of the largest diagonal entry, then normalised (the
+    construction of ``_quatd_from_matrix_cols``, sap_helpers.py:607-652)."""
+    r00, r10, r20 = (float(x) for x in c0)
+    r01, r11, r21 = (float(x) for x in c1)
+    r02, r12, r22 = (float(x) for x in c2)
+    tr = r00 + r11 + r22
+    if tr > 0.0:
+        s = np.sqrt(tr + 1.0) * 2.0
+        q = np.array([r21 - r12, r02 - r20, r10 - r01, 0.25 * s * s]) / s
+    elif r00 > r11 and r00 > r22:
+        s = np.sqrt(1.0 + r00 - r11 - r22) * 2.0
+        q = np.array([0.25 * s * s, r01 + r10, r02 + r20, r21 - r12]) / s
+    elif r11 > r22:
+        s = np.sqrt(1.0 + r11 - r00 - r22) * 2.0
+        q = np.array([r01 + r10, 0.25 * s * s, r12 + r21, r02 - r20]) / s
+    else:
+        s = np.sqrt(1.0 + r22 - r00 - r11) * 2.0
+        q = np.array([r02 + r20, r12 + r21, 0.25 * s * s, r10 - r01]) / s
+    n = np.linalg.norm(q)
+    return q / n if n > 0.0 else np.array([0.0, 0.0, 0.0, 1.0])
+
+
+def d6_frame_axes(ang_axes):
+    """The rotation axes of a D6 joint's angular coordinates at zero angles.
+
+    One axis is used as given.  Two or three axes are replaced by the columns of the rotation
+    whose quaternion is built from the matrix [a0, a1, a2] (a2 = a0 x a1 for two axes), so
+    orthonormal right-handed axes come back unchanged (sap_helpers.py:1109-1161)."""
+    m = len(ang_axes)
+    if m <= 1:
+        return [np.asarray(a, dtype=np.float64) for a in ang_axes]
+    a0, a1 = np.asarray(ang_axes[0], np.float64), np.asarray(ang_axes[1], np.float64)
+    a2 = np.cross(a0, a1) if m == 2 else np.asarray(ang_axes[2], np.float64)
+    q = quat_from_cols(a0, a1, a2)
+    return [_qrot(q, e) for e in np.eye(3)[:m]]
+
+
+def _axis_angle(ax, ang):
+    """(W, 4) quaternions of rotations by ``ang`` (W,) about the fixed axis ``ax``."""
+    h = 0.5 * np.asarray(ang, dtype=np.float64)
+    return np.concatenate([np.asarray(ax)[None, :] * np.sin(h)[:, None], np.cos(h)[:, None]], axis=-1)
+
+
+def d6_joint_motion(lin_axes, ang_axes, q, qd):
+    """Joint transform and joint-frame velocity of D6 joints, vectorised over worlds.
+
+    ``q``/``qd`` are (W, nl + na).  The angular coordinates are intrinsic rotations about the
+    frame axes l_k of :func:`d6_frame_axes`: R = R(l0, q0) R(l1, q1) R(l2, q2), whose angular
+    velocity is l0 qd0 + R(l0, q0) l1 qd1 + R(l0, q0) R(l1, q1) l2 qd2; the linear coordinates
+    translate along their fixed axes (sap_kinematics.py:170-218).  Returns ((W, 7), (W, 6)) with
+    the velocity as [linear, angular]."""
+    W = q.shape[0]
+    nl = len(lin_axes)
+    X = np.zeros((W, 7))
+    X[:, 6] = 1.0
+    v = np.zeros((W, 6))
+    for k, a in enumerate(lin_axes):
+        X[:, :3] += np.asarray(a, np.float64)[None, :] * q[:, k:k + 1]
+        v[:, :3] += np.asarray(a, np.float64)[None, :] * qd[:, k:k + 1]
+    rot = X[:, 3:].copy()
+    for k, l in enumerate(d6_frame_axes(ang_axes)):
+        v[:, 3:] += _qrot(rot, np.broadcast_to(l, (W, 3))) * qd[:, nl + k:nl + k + 1]
+        rot = _qmul(rot, _axis_angle(l, q[:, nl + k]))
+    X[:, 3:] = rot
+    return X, v
 
 
 def eval_fk_public(model, joint_q, joint_qd):
@@ -86,6 +162,11 @@ def eval_fk_public(model, joint_q, joint_qd):
         elif jt in (SAP_JOINT_FREE, SAP_JOINT_DISTANCE):
             X_j = jq[qs[:, None] + np.arange(7)]
             v_j = jqd[ds[:, None] + np.arange(6)]
+        elif jt == SAP_JOINT_D6:
+            nl, na = int(model.joint_dof_dim[j, 0]), int(model.joint_dof_dim[j, 1])
+            axes = model.joint_axis[int(model.joint_qd_start[j]) + np.arange(nl + na)].astype(np.float64)
+            X_j, v_j = d6_joint_motion(axes[:nl], axes[nl:], jq[qs[:, None] + np.arange(nl + na)],
+                                       jqd[ds[:, None] + np.arange(nl + na)])
         X_wc = _tmul(_tmul(X_wpj, X_j), _tinv(X_cj))
         lin = _qrot(X_wpj[:, 3:], v_j[:, :3])
         ang = _qrot(X_wpj[:, 3:], v_j[:, 3:])

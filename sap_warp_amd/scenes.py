@@ -18,9 +18,10 @@ Scenes:
   revolute joints, 30 bodies, capsule/sphere/box links) used as the GPU
   workload.
 * :func:`build_box_pile_scene`, :func:`build_round_shapes_scene`,
-  :func:`build_convex_shapes_scene`, :func:`build_ball_chain_scene` — parity
-  scenes for the box pairs, the round-shape primitives, the MPR / GJK pairs
-  (cones, cylinders, ellipsoids) and ball-joint chains.
+  :func:`build_convex_shapes_scene`, :func:`build_ball_chain_scene`,
+  :func:`build_d6_ant_scene` — parity scenes for the box pairs, the round-shape
+  primitives, the MPR / GJK pairs (cones, cylinders, ellipsoids), ball-joint
+  chains and D6 joints.
 
 Worlds are independent (no cross-world pairs), so by default every world sits
 at the origin (``spacing=0``): positions stay O(1 m) and float32 geometry keeps
@@ -44,6 +45,7 @@ from .runtime import (
     GEO_SPHERE,
     SAP_BODY_FLAG_DYNAMIC,
     SAP_JOINT_BALL,
+    SAP_JOINT_D6,
     SAP_JOINT_DISTANCE,
     SAP_JOINT_FIXED,
     SAP_JOINT_FREE,
@@ -1370,6 +1372,91 @@ def build_ball_chain_scene(
     return _with_consistent_body_q(b.finalize(device))
 
 
+def build_d6_ant_scene(
+    num_worlds: int = 64,
+    seed: int = 3,
+    *,
+    device: str = "cpu",
+) -> SapModel:
+    """Ant batch whose legs use D6 joints (the joints the reference's MJCF / USD / URDF loaders
+    make from bodies with several hinges or slides, ``loader/scene.py:2239-2251, 2856-2894,
+    4930-4937``).
+
+    Torso sphere r=0.25 on a free joint; four two-capsule legs (r=0.04, segments U(0.2, 0.35)):
+
+    * leg 0: hip D6 with two angular axes (z, tangent), revolute knee;
+    * leg 1: hip D6 with three angular axes (z, tangent, radial), one-axis D6 knee with limits
+      (+-0.4 rad) and a PD drive (ke 20, kd 1);
+    * leg 2: hip D6 with one linear (radial) and one angular (z) axis, revolute knee;
+    * leg 3: hip D6 with two skew angular axes (orthogonalised by the joint), knee D6 with one
+      linear axis along the shin and one angular axis, PD drives on both (ke 40, kd 2).
+
+    6 + 3 + 4 + 3 + 4 = 20 DOF.  Hinge angles U(-0.3, 0.3), slides U(-0.02, 0.02), base height
+    U(0.36, 0.6) (the lowest worlds start with a foot on the ground), velocities N(0, 0.5); mu 1, per-shape stiffness 2e5 N/m, tau 5 s, as the ant.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = SceneBuilder(rigid_gap=0.02)
+    shape = ShapeCfg(density=1000.0, ke=2.0e5, tau=5.0, mu=1.0, gap=0.02)
+    b.world_count = num_worlds
+    z = np.array([0.0, 0.0, 1.0])
+
+    def dof(axis, **kw):
+        a = np.asarray(axis, dtype=np.float64)
+        return DofCfg(axis=tuple(a / np.linalg.norm(a)), **({"limit_ke": 0.0} | kw))
+
+    for w in range(num_worlds):
+        b.current_world = w
+        height = rng.uniform(0.36, 0.6)
+        root_pose = tf(np.array([0.0, 0.0, height]), quat_from_axis_angle((0, 0, 1), rng.uniform(-math.pi, math.pi)))
+        torso = b.add_body(root_pose)
+        b.add_shape(torso, GEO_SPHERE, tf(), (0.25, 0.0, 0.0), shape)
+        joints = [b.add_free_joint(torso)]
+        for leg in range(4):
+            ang = math.pi / 4 + leg * math.pi / 2
+            radial = np.array([math.cos(ang), math.sin(ang), 0.0])
+            tangent = np.array([-math.sin(ang), math.cos(ang), 0.0])
+            l1, l2 = rng.uniform(0.2, 0.35), rng.uniform(0.2, 0.35)
+            hip = 0.25 * radial
+            down = 0.5 * radial - 0.866 * z
+            down /= np.linalg.norm(down)
+            thigh = b.add_body(tf_mul(root_pose, tf(hip)))
+            b.add_shape(thigh, GEO_CAPSULE, tf(0.5 * l1 * radial, quat_z_to(radial)), (0.04, 0.5 * l1, 0.0), shape)
+            shin = b.add_body(tf_mul(root_pose, tf(hip + l1 * radial)))
+            b.add_shape(shin, GEO_CAPSULE, tf(0.5 * l2 * down, quat_z_to(down)), (0.04, 0.5 * l2, 0.0), shape)
+            hip_x = dict(parent_xform=tf(hip))
+            knee_x = dict(parent_xform=tf(l1 * radial))
+            if leg == 0:
+                jh = b.add_joint(torso, thigh, SAP_JOINT_D6, angular_axes=[dof(z), dof(tangent)], **hip_x)
+                jk = b.add_joint(thigh, shin, SAP_JOINT_REVOLUTE, angular_axes=[dof(tangent)], **knee_x)
+            elif leg == 1:
+                jh = b.add_joint(torso, thigh, SAP_JOINT_D6, angular_axes=[dof(z), dof(tangent), dof(radial)], **hip_x)
+                jk = b.add_joint(thigh, shin, SAP_JOINT_D6, angular_axes=[dof(tangent, limit_lower=-0.4, limit_upper=0.4,
+                                                                              limit_ke=1.0e4, target_ke=20.0, target_kd=1.0,
+                                                                              actuator_mode=1)],
+                                 **knee_x)
+            elif leg == 2:
+                jh = b.add_joint(torso, thigh, SAP_JOINT_D6, linear_axes=[dof(radial)], angular_axes=[dof(z)], **hip_x)
+                jk = b.add_joint(thigh, shin, SAP_JOINT_REVOLUTE, angular_axes=[dof(tangent)], **knee_x)
+            else:
+                jh = b.add_joint(torso, thigh, SAP_JOINT_D6, angular_axes=[dof(z + 0.3 * radial), dof(tangent + 0.2 * z)],
+                                 **hip_x)
+                pd = dict(target_ke=40.0, target_kd=2.0, actuator_mode=1)
+                jk = b.add_joint(thigh, shin, SAP_JOINT_D6, linear_axes=[dof(down, **pd)],
+                                 angular_axes=[dof(tangent, **pd)], **knee_x)
+            joints += [jh, jk]
+        b.add_articulation(joints)
+        _set_free_joint_state(b, joints[0], root_pose, rng.normal(0.0, 0.5, size=6))
+        for j in joints[1:]:
+            qs, ds = b.joint_q_start[j], b.joint_qd_start[j]
+            nl, na = b.joint_dof_dim[j]
+            for k in range(nl + na):
+                b.joint_q[qs + k] = rng.uniform(-0.02, 0.02) if k < nl else rng.uniform(-0.3, 0.3)
+                b.joint_qd[ds + k] = rng.normal(0.0, 0.5)
+    b.current_world = -1
+    b.add_ground_plane(ShapeCfg(density=0.0, ke=2.0e5, tau=5.0, mu=1.0, gap=0.02, collision_group=-1))
+    return _with_consistent_body_q(b.finalize(device))
+
+
 def _with_consistent_body_q(model: SapModel) -> SapModel:
     """Recompute body_q/body_qd from joint_q/joint_qd with the public FK."""
     from .kinematics_host import eval_fk_public
@@ -1379,4 +1466,4 @@ def _with_consistent_body_q(model: SapModel) -> SapModel:
 
 
 __all__ = ["SceneBuilder", "ShapeCfg", "DofCfg", "build_ant_scene", "build_ball_chain_scene", "build_box_pile_scene",
-           "build_humanoid_scene", "build_round_shapes_scene", "build_convex_shapes_scene"]
+           "build_humanoid_scene", "build_round_shapes_scene", "build_convex_shapes_scene", "build_d6_ant_scene"]

@@ -15,6 +15,7 @@ from sap_warp_amd.runtime import (
     GEO_CAPSULE,
     GEO_SPHERE,
     SAP_JOINT_BALL,
+    SAP_JOINT_D6,
     SAP_JOINT_FREE,
     SAP_JOINT_PRISMATIC,
     SAP_JOINT_REVOLUTE,
@@ -77,6 +78,69 @@ def fixed_base_tree_scene(armature=0.02):
     return b.finalize()
 
 
+def _unit(v):
+    v = np.asarray(v, dtype=np.float64)
+    return tuple(v / np.linalg.norm(v))
+
+
+def d6_tree_scene(kind="mixed", armature=0.02, as_plain=False, limits=False):
+    """Fixed-base tree built from D6 joints.
+
+    ``kind``:
+      * "single": one-axis D6 joints only (two angular, one linear), the tree of
+        :func:`fixed_base_tree_scene`'s shape; with ``as_plain`` the same tree with revolute /
+        prismatic joints instead, for the equivalence checks;
+      * "one_angle": D6 joints with at most one angular axis (1 lin + 1 ang, 2 lin + 1 ang, 3 lin);
+      * "mixed": 3 angular (about a rotated orthonormal frame), 1 lin + 2 ang, 2 ang (skew pair,
+        orthogonalised by the joint) and a revolute.
+    ``limits`` gives every one-axis dof a finite range (lower 0.05 below, upper 0.1 above zero).
+    """
+    b = SceneBuilder(gravity=-9.81)
+    cfg = ShapeCfg(density=900.0)
+    rng_axes = quat_to_matrix(quat_from_axis_angle((0.3, -0.8, 0.5), 0.9))
+
+    def dof(axis):
+        kw = dict(axis=_unit(axis), armature=armature)
+        if limits:
+            kw.update(limit_lower=-0.05, limit_upper=0.1, limit_ke=1.0e4, limit_kd=10.0)
+        return DofCfg(**kw)
+
+    def joint(parent, child, lin, ang, **kw):
+        if as_plain:
+            assert len(lin) + len(ang) == 1
+            jt = SAP_JOINT_PRISMATIC if lin else SAP_JOINT_REVOLUTE
+            return b.add_joint(parent, child, jt, linear_axes=[dof(a) for a in lin], angular_axes=[dof(a) for a in ang], **kw)
+        return b.add_joint(parent, child, SAP_JOINT_D6, linear_axes=[dof(a) for a in lin],
+                           angular_axes=[dof(a) for a in ang], **kw)
+
+    root = b.add_body(tf())
+    b.add_shape(root, GEO_CAPSULE, tf((0.02, 0.0, -0.25), quat_from_axis_angle((1, 0, 0), 0.3)), (0.05, 0.18, 0.0), cfg)
+    arm = b.add_body(tf())
+    b.add_shape(arm, GEO_BOX, tf((0.0, 0.05, -0.2)), (0.04, 0.06, 0.2), cfg)
+    hand = b.add_body(tf())
+    b.add_shape(hand, GEO_SPHERE, tf((0.05, 0.0, -0.08)), (0.06, 0.0, 0.0), cfg)
+    slider = b.add_body(tf())
+    b.add_shape(slider, GEO_BOX, tf((0.03, 0.0, 0.0)), (0.05, 0.05, 0.05), cfg)
+    skew = (0.3, 0.5, 0.8)
+    pax = (1.0, 0.0, 0.2)
+    if kind == "single":
+        specs = [((), ((0, 1, 0),)), ((), ((1, 0, 0),)), ((), (skew,)), ((pax,), ())]
+    elif kind == "one_angle":
+        specs = [(((0, 0, 1),), ((0, 1, 0),)), ((pax, (0, 1, 0)), ((1, 0, 0),)), ((), (skew,)),
+                 (((1, 0, 0), (0, 1, 0), (0, 0, 1)), ())]
+    elif kind == "mixed":
+        specs = [((), tuple(tuple(c) for c in rng_axes.T)), ((pax,), ((1, 0, 0), (0, 0, 1))),
+                 ((), ((0.2, 1.0, 0.1), (1.0, 0.3, -0.2))), ((), ((0, 1, 0),))]
+    else:
+        raise ValueError(kind)
+    j0 = joint(-1, root, *specs[0], parent_xform=tf((0.0, 0.0, 1.5)))
+    j1 = joint(root, arm, *specs[1], parent_xform=tf((0.0, 0.1, -0.5)), child_xform=tf((0.0, 0.0, 0.03)))
+    j2 = joint(arm, hand, *specs[2], parent_xform=tf((0.0, 0.0, -0.42), quat_from_axis_angle((0, 0, 1), 0.4)))
+    j3 = joint(root, slider, *specs[3], parent_xform=tf((0.1, -0.1, -0.3), quat_from_axis_angle((0, 1, 0), 0.5)))
+    b.add_articulation([j0, j1, j2, j3])
+    return b.finalize()
+
+
 def single_world(model):
     assert model.world_count == 1
     return model
@@ -98,7 +162,7 @@ def advance_coords(model, q, v_sap, t):
     """Coordinates at time ``t`` when the SAP velocity ``v_sap`` is held constant.
 
     Free joints move their origin with the linear part and rotate with the
-    world-frame angular part (q(t) = exp(t w / 2) q); revolute / prismatic
+    world-frame angular part (q(t) = exp(t w / 2) q); revolute / prismatic / D6
     coordinates move linearly; a ball joint turns its child frame with the
     angular velocity w given in the parent's joint frame (r(t) = exp(t w / 2) r,
     the rate the public FK maps to the world by the parent joint frame's
@@ -110,7 +174,7 @@ def advance_coords(model, q, v_sap, t):
         jt = int(model.joint_type[j])
         qs, ds = int(model.joint_q_start[j]), int(model.joint_qd_start[j])
         nd = int(model.joint_qd_start[j + 1]) - ds
-        if jt in (SAP_JOINT_REVOLUTE, SAP_JOINT_PRISMATIC):
+        if jt in (SAP_JOINT_REVOLUTE, SAP_JOINT_PRISMATIC, SAP_JOINT_D6):
             q[qs:qs + nd] += t * v_sap[ds:ds + nd]
         elif jt == SAP_JOINT_FREE:
             q[qs:qs + 3] += t * v_sap[ds + 3:ds + 6]

@@ -81,8 +81,7 @@ def _desc_from(model):
     return nat.model_desc(model, keep), keep
 
 
-def test_model_create_rejects_d6_joints():
-    """D6 joints are outside the native step (ball joints are in it); rejected before any GPU work."""
+def _d6_model(angular, linear=()):
     from sap_warp_amd.runtime import GEO_SPHERE, SAP_JOINT_D6
     from sap_warp_amd.scenes import DofCfg, SceneBuilder, ShapeCfg, tf
 
@@ -93,14 +92,29 @@ def test_model_create_rejects_d6_joints():
     j0 = b.add_free_joint(root)
     child = b.add_body(tf((0, 0, 0.5)))
     b.add_shape(child, GEO_SPHERE, tf(), (0.1, 0, 0), ShapeCfg())
-    j1 = b.add_joint(root, child, SAP_JOINT_D6, angular_axes=[DofCfg(axis=a) for a in ((1, 0, 0), (0, 1, 0))],
-                     parent_xform=tf((0, 0, -0.5)))
+    j1 = b.add_joint(root, child, SAP_JOINT_D6, angular_axes=[DofCfg(axis=a) for a in angular],
+                     linear_axes=[DofCfg(axis=a) for a in linear], parent_xform=tf((0, 0, -0.5)))
     b.add_articulation([j0, j1])
-    model = b.finalize()
+    return b.finalize()
+
+
+@pytest.mark.parametrize("case", ["four_angles", "dof_dim_mismatch", "no_axes"])
+def test_model_create_rejects_malformed_d6_joints(case):
+    """D6 joints take 0-3 linear and 0-3 angular axes with one coordinate per dof and a
+    joint_dof_dim that matches the dof count; anything else is rejected before any GPU work."""
+    axes = ((1, 0, 0), (0, 1, 0), (0, 0, 1))
+    if case == "four_angles":
+        model = _d6_model(axes + ((1, 1, 0),))
+    elif case == "no_axes":
+        model = _d6_model(())
+    else:
+        model = _d6_model(axes[:2])
+        model.joint_dof_dim[1] = (1, 2)
     lib = nat.load()
     desc, _keep = _desc_from(model)
     h = ctypes.c_void_p()
-    with pytest.raises(NotImplementedError, match="D6"):
+    err = NotImplementedError if case == "no_axes" else ValueError
+    with pytest.raises(err, match="D6"):
         nat.check(lib.sapx_model_create(ctypes.byref(desc), ctypes.byref(h)), "sapx_model_create")
 
 

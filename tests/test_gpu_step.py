@@ -63,6 +63,10 @@ def _scene(name, worlds, seed):
         return build_ant_scene(worlds, seed=seed, device="cuda:0"), 1.0e-3, 16
     if name == "boxpile":
         return build_box_pile_scene(worlds, seed=seed, device="cuda:0"), 2.0e-3, 64
+    if name == "d6_ant":
+        from sap_warp_amd.scenes import build_d6_ant_scene
+
+        return build_d6_ant_scene(worlds, seed=seed + 13, device="cuda:0"), 2.0e-3, 16
     if name == "ball_chain":
         from sap_warp_amd.scenes import build_ball_chain_scene
 
@@ -413,6 +417,20 @@ def test_ball_chain_step_parity_other_kernels(gpu_device, monkeypatch, kernel):
     """Ball joints in the 256-thread kernel and in the 64-slot wave variant alone, same bar."""
     monkeypatch.setenv("SAPX_STEP_KERNEL", kernel)
     _run_and_compare("ball_chain", "approx32", "exact_root", worlds=8, steps=5)
+
+
+@pytest.mark.parametrize("preset,ls", [("approx32", "exact_root"), ("drake", "armijo_decay")])
+def test_d6_ant_step_parity(gpu_device, preset, ls):
+    """D6 joints: two- and three-angle hips, a sliding hip, a limited one-axis knee with a drive and
+    a slide-and-hinge knee with drives, on a free torso falling onto the ground."""
+    _run_and_compare("d6_ant", preset, ls, worlds=8, steps=4, warm_steps=100, min_max_contacts=2)
+
+
+@pytest.mark.parametrize("kernel", ["fused", "wave64"])
+def test_d6_ant_step_parity_other_kernels(gpu_device, monkeypatch, kernel):
+    """D6 joints in the 256-thread kernel and in the 64-slot wave variant alone, same bar."""
+    monkeypatch.setenv("SAPX_STEP_KERNEL", kernel)
+    _run_and_compare("d6_ant", "approx32", "exact_root", worlds=8, steps=4, warm_steps=100, min_max_contacts=2)
 
 
 def test_host_tensors_rejected_before_launch(gpu_device):
