@@ -429,6 +429,46 @@ int32_t sapx_mlp_gradients(SapxMlp* mlp, const float* x, const float* target, in
 /* One bias-corrected Adam step with the last gradients. */
 int32_t sapx_mlp_adam(SapxMlp* mlp, float lr, float beta1, float beta2, float eps, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Blocked Cholesky of large symmetric positive definite matrices.
+ * Replaces BlockCholeskySolverBatched (sim/blocked_cholesky.py:725-901), whose
+ * factorisation runs one workgroup per matrix: here each block column's panel
+ * and Schur-complement update spread over the whole GPU (64 x 64 blocks,
+ * float32, MFMA updates), for the dynamics / Newton matrices of worlds far
+ * above the step kernels' 256 dofs.  Rows from n to the next multiple of 64
+ * are identity (the reference's padding, blocked_cholesky.py:299-313).
+ * ------------------------------------------------------------------------- */
+typedef struct SapxChol SapxChol;
+
+/*
+ * Workspace for batch_size matrices of order up to max_num_equations
+ * (BlockCholeskySolverBatched.__init__, blocked_cholesky.py:726-779): the
+ * factor, a kept copy of the input (for float64-residual refinement) and the
+ * inverted diagonal blocks.  ceil(max_num_equations / 64) * batch_size <= 512.
+ */
+int32_t sapx_chol_create(int32_t max_num_equations, int32_t batch_size, SapxChol** out);
+void sapx_chol_destroy(SapxChol* chol);
+/*
+ * Factor A (device float32; batch item b, row i at A + b * a_stride + i * lda;
+ * the full symmetric n x n matrix) with n <= max_num_equations active
+ * equations (factorize, blocked_cholesky.py:781-809).  Enqueued on stream; no
+ * allocation, so it can be captured in a graph.
+ */
+int32_t sapx_chol_factorize(SapxChol* chol, const float* A, int32_t n, int32_t lda, int64_t a_stride,
+                            void* stream);
+/*
+ * result = A^-1 rhs for every batch item (solve, blocked_cholesky.py:811-823);
+ * rhs / result device float64, batch_size x n contiguous, not aliased.
+ * refine = number of extra sweeps r = rhs - A result (float64), result += A^-1 r.
+ */
+int32_t sapx_chol_solve(SapxChol* chol, const double* rhs, double* result, int32_t refine, void* stream);
+/*
+ * Synchronises, copies the per-matrix status of the last factorisation into
+ * info (host, batch_size entries: 0, or 1 + the first column whose pivot was
+ * not positive), and fails if a solve's wait for a block timed out.
+ */
+int32_t sapx_chol_info(SapxChol* chol, int32_t* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -138,6 +138,11 @@ EXPORTS = {
     "sapx_mlp_forward": (_I, [_P, _P, _I, _P, _P]),
     "sapx_mlp_gradients": (_I, [_P, _P, _P, _I, _P, _P]),
     "sapx_mlp_adam": (_I, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, _P]),
+    "sapx_chol_create": (_I, [_I, _I, ctypes.POINTER(_P)]),
+    "sapx_chol_destroy": (None, [_P]),
+    "sapx_chol_factorize": (_I, [_P, _P, _I, _I, ctypes.c_int64, _P]),
+    "sapx_chol_solve": (_I, [_P, _P, _P, _I, _P]),
+    "sapx_chol_info": (_I, [_P, _P]),
 }
 
 HIST_SIZE = 260  # sapx_get_histogram entries

@@ -43,6 +43,7 @@ UNITS = {
     "sapx_order.hip": [],
     "sapx_learned.hip": [],
     "sapx_mlp.hip": [],
+    "sapx_chol.hip": [],
 }
 
 

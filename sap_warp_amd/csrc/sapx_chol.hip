@@ -1,0 +1,504 @@
+// Blocked Cholesky factor and solve of large symmetric positive definite matrices (gfx950).
+//
+// Counterpart of BlockCholeskySolverBatched (blocked_cholesky.py:725-901), whose
+// factorisation runs one workgroup per matrix (blocked_cholesky.py:803-809): fine for a
+// 35-dof world, a serial bottleneck for a single world of thousands of dofs.  Here every
+// stage spreads over the whole GPU:
+//
+//   right-looking, 64 x 64 blocks, float32, per block column k:
+//     chol_panel  (T - k workgroups)  each factors the diagonal block A_kk in LDS and
+//                                     inverts L_kk (redundantly: no grid barrier);
+//                                     workgroup 0 stores L_kk and L_kk^-1, the others the
+//                                     panel blocks L_ik = A_ik L_kk^-T by MFMA
+//     chol_schur  ((T-k-1)(T-k)/2)    the Schur-complement update A_ij -= L_ik L_jk^T of
+//                                     every lower trailing block, v_mfma_f32_16x16x4f32
+//   solves, one launch each: workgroup i computes block i of y (forward) or x (backward)
+//   as soon as the blocks it depends on are published (per-block ready flags with
+//   release / acquire ordering and a bounded wait), using the stored diagonal inverses, so
+//   the dependent chain is one 64 x 64 mat-vec per block; then float64-residual refinement
+//   sweeps r = b - A x, x += solve(r) against a kept copy of the input.
+//   The factor is of D A D with D = diag(A)^-1/2 (the solves scale in and out), which
+//   takes the spread of masses, inertias and contact stiffnesses out of the float32
+//   condition number.
+//
+// Rows and columns from n to the padded order are identity, as in the reference's
+// padded layout (blocked_cholesky.py:299-313).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <string>
+
+#include "../../include/sapx.h"
+
+namespace sapx {
+void set_last_error(const char* msg);
+}
+
+namespace {
+
+constexpr int NB = 64;       // block order
+constexpr int LS = NB + 1;   // LDS row stride (floats): conflict-free column access
+constexpr int NT = 256;      // threads per workgroup
+constexpr int kSpinLimit = 1 << 24;  // polls before a wait gives up (about a second)
+constexpr int kMaxResident = 512;    // workgroups of one solve launch (all resident at once)
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+int32_t cfail(int32_t code, const std::string& msg) {
+    sapx::set_last_error(msg.c_str());
+    return code;
+}
+
+__device__ __forceinline__ int flag_load(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void flag_store(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float coherent_load(const float* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// thread 0 waits until flag is set (bounded); the whole workgroup then proceeds
+__device__ __forceinline__ void wait_flag(const int* flag, int* timed_out) {
+    if (threadIdx.x == 0) {
+        int polls = 0;
+        while (flag_load(flag) == 0) {
+            if (++polls > kSpinLimit) {
+                __hip_atomic_store(timed_out, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
+
+// publish: every thread's stores are visible device-wide before the flag
+__device__ __forceinline__ void publish(int* flag) {
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) flag_store(flag, 1);
+}
+
+// symmetric diagonal scaling d_i = 1 / sqrt(A_ii) (1 where A_ii is not positive, and on the
+// padding): the factor works on D A D, whose unit diagonal removes the scale spread of masses,
+// rotational inertias and stiff contact blocks from the float32 condition number
+__global__ __launch_bounds__(NT) void chol_scale(const float* __restrict__ A, int lda, long long a_stride, int n,
+                                                 float* __restrict__ dsc, int npad) {
+    const int b = blockIdx.y;
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= npad) return;
+    const float a = i < n ? A[b * a_stride + (long long)i * lda + i] : 1.0f;
+    dsc[(size_t)b * npad + i] = (a > 0.0f && a <= FLT_MAX) ? 1.0f / sqrtf(a) : 1.0f;
+}
+
+// the scaled padded work matrix D A D and the kept original A (identity outside n x n)
+__global__ __launch_bounds__(NT) void chol_load(const float* __restrict__ A, int lda, long long a_stride, int n,
+                                                const float* __restrict__ dsc, float* __restrict__ W,
+                                                float* __restrict__ A0, int npad, int* __restrict__ info) {
+    const int b = blockIdx.y;
+    const long long total = (long long)npad * npad;
+    const float* Ab = A + b * a_stride;
+    const float* db = dsc + (size_t)b * npad;
+    for (long long e = (long long)blockIdx.x * NT + threadIdx.x; e < total; e += (long long)gridDim.x * NT) {
+        const int i = (int)(e / npad), j = (int)(e - (long long)i * npad);
+        const float v = (i < n && j < n) ? Ab[(long long)i * lda + j] : (i == j ? 1.0f : 0.0f);
+        W[b * total + e] = (v * db[i]) * db[j];
+        A0[b * total + e] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) info[b] = 0;
+}
+
+// 64 x 64 x 64 product on four waves: out = P Q^T with P, Q in LDS (row stride LS); wave w
+// owns the 32 x 32 quadrant (w >> 1, w & 1) as 2 x 2 MFMA tiles
+__device__ __forceinline__ void mfma_pqt(const float* P, const float* Q, f4 (&acc)[2][2]) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int qr = (w >> 1) * 32, qc = (w & 1) * 32;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[a][c] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
+    for (int kk = 0; kk < NB; kk += 4) {
+        const int col = kk + (lane >> 4);
+        float pa[2], qb[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            pa[t] = P[(qr + t * 16 + (lane & 15)) * LS + col];
+            qb[t] = Q[(qc + t * 16 + (lane & 15)) * LS + col];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[a][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(pa[a], qb[c], acc[a][c], 0, 0, 0);
+    }
+}
+
+// block column k: factor and invert the diagonal block; store L_kk, L_kk^-1 (workgroup 0)
+// or the panel block L_ik = A_ik L_kk^-T (workgroup i - k)
+__global__ __launch_bounds__(NT) void chol_panel(float* __restrict__ W, float* __restrict__ Xi, int npad, int T, int k,
+                                                 int* __restrict__ info) {
+    __shared__ float D[NB * LS];
+    __shared__ float X[NB * LS];
+    __shared__ int first_bad;
+    const int b = blockIdx.y, tid = threadIdx.x;
+    float* Wb = W + (size_t)b * npad * npad;
+    const int k0 = k * NB;
+    for (int e = tid; e < NB * NB; e += NT) {
+        const int i = e >> 6, j = e & 63;
+        D[i * LS + j] = Wb[(size_t)(k0 + i) * npad + k0 + j];
+        X[i * LS + j] = i == j ? 1.0f : 0.0f;
+    }
+    if (tid == 0) first_bad = -1;
+    __syncthreads();
+    // unblocked Cholesky of the diagonal block (lower triangle)
+    for (int j = 0; j < NB; ++j) {
+        const float d = D[j * LS + j];
+        const bool ok = d > 0.0f && d <= FLT_MAX;
+        const float s = sqrtf(ok ? d : FLT_MIN);
+        const float inv = 1.0f / s;
+        __syncthreads();
+        if (tid == 0) {
+            D[j * LS + j] = s;
+            if (!ok && first_bad < 0) first_bad = j;
+        }
+        if (tid > j && tid < NB) D[tid * LS + j] *= inv;
+        __syncthreads();
+        for (int e = tid; e < NB * NB; e += NT) {
+            const int i = e >> 6, l = e & 63;
+            if (l > j && l <= i) D[i * LS + l] -= D[i * LS + j] * D[l * LS + j];
+        }
+        __syncthreads();
+    }
+    // X = L^-1 (lower): column-oriented forward substitution of L X = I
+    for (int j = 0; j < NB; ++j) {
+        if (tid <= j) X[j * LS + tid] *= 1.0f / D[j * LS + j];
+        __syncthreads();
+        for (int e = tid; e < NB * NB; e += NT) {
+            const int i = e >> 6, c = e & 63;
+            if (i > j && c <= j) X[i * LS + c] -= D[i * LS + j] * X[j * LS + c];
+        }
+        __syncthreads();
+    }
+    if (blockIdx.x == 0) {
+        float* Xo = Xi + ((size_t)b * T + k) * NB * NB;
+        for (int e = tid; e < NB * NB; e += NT) {
+            const int i = e >> 6, j = e & 63;
+            Wb[(size_t)(k0 + i) * npad + k0 + j] = j <= i ? D[i * LS + j] : 0.0f;
+            Xo[e] = j <= i ? X[i * LS + j] : 0.0f;
+        }
+        if (tid == 0 && first_bad >= 0) atomicCAS(info + b, 0, k0 + first_bad + 1);
+        return;
+    }
+    // panel block: D <- A_ik, then L_ik = A_ik X^T
+    const int i0 = (k + blockIdx.x) * NB;
+    __syncthreads();
+    for (int e = tid; e < NB * NB; e += NT) {
+        const int r = e >> 6, m = e & 63;
+        D[r * LS + m] = Wb[(size_t)(i0 + r) * npad + k0 + m];
+    }
+    __syncthreads();
+    f4 acc[2][2];
+    mfma_pqt(D, X, acc);
+    const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = (w >> 1) * 32 + a * 16 + 4 * (lane >> 4) + e;
+                const int cc = (w & 1) * 32 + c * 16 + (lane & 15);
+                Wb[(size_t)(i0 + r) * npad + k0 + cc] = acc[a][c][e];
+            }
+}
+
+// Schur-complement update of block column k: A_ij -= L_ik L_jk^T for k < j <= i < T
+__global__ __launch_bounds__(NT) void chol_schur(float* __restrict__ W, int npad, int k) {
+    __shared__ float Li[NB * LS];
+    __shared__ float Lj[NB * LS];
+    const int b = blockIdx.y, tid = threadIdx.x;
+    float* Wb = W + (size_t)b * npad * npad;
+    const int t = blockIdx.x;
+    int ib = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while ((ib + 1) * (ib + 2) / 2 <= t) ++ib;
+    while (ib * (ib + 1) / 2 > t) --ib;
+    const int jb = t - ib * (ib + 1) / 2;
+    const int k0 = k * NB, i0 = (k + 1 + ib) * NB, j0 = (k + 1 + jb) * NB;
+    for (int e = tid; e < NB * NB; e += NT) {
+        const int r = e >> 6, m = e & 63;
+        Li[r * LS + m] = Wb[(size_t)(i0 + r) * npad + k0 + m];
+        Lj[r * LS + m] = Wb[(size_t)(j0 + r) * npad + k0 + m];
+    }
+    __syncthreads();
+    f4 acc[2][2];
+    mfma_pqt(Li, Lj, acc);
+    const int lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int r = (w >> 1) * 32 + a * 16 + 4 * (lane >> 4) + e;
+                const int cc = (w & 1) * 32 + c * 16 + (lane & 15);
+                Wb[(size_t)(i0 + r) * npad + j0 + cc] -= acc[a][c][e];
+            }
+}
+
+__global__ __launch_bounds__(NT) void chol_clear(int* __restrict__ flags, int count) {
+    for (int e = blockIdx.x * NT + threadIdx.x; e < count; e += gridDim.x * NT) flags[e] = 0;
+}
+
+// forward solve L y = r: workgroup i forms y_i = L_ii^-1 (r_i - sum_{j<i} L_ij y_j)
+__global__ __launch_bounds__(NT) void chol_fwd(const float* __restrict__ W, const float* __restrict__ Xi,
+                                               const float* __restrict__ dsc, const double* __restrict__ rhs, int n,
+                                               float* __restrict__ y,
+                                               int* __restrict__ flags, int npad, int T, int* __restrict__ timed_out) {
+    __shared__ float Lb[NB * LS];
+    __shared__ float Xs[NB * LS];
+    __shared__ float part[4][NB];
+    __shared__ float vj[NB];
+    const int b = blockIdx.y, i = blockIdx.x, tid = threadIdx.x;
+    const int r = tid & 63, q = tid >> 6;
+    const float* Wb = W + (size_t)b * npad * npad;
+    const float* Xb = Xi + ((size_t)b * T + i) * NB * NB;
+    float* yb = y + (size_t)b * npad;
+    int* fb = flags + (size_t)b * T;
+    for (int e = tid; e < NB * NB; e += NT) Xs[(e >> 6) * LS + (e & 63)] = Xb[e];
+    float acc = 0.0f;
+    for (int j = 0; j < i; ++j) {
+        for (int e = tid; e < NB * NB; e += NT) {
+            const int rr = e >> 6, c = e & 63;
+            Lb[rr * LS + c] = Wb[(size_t)(i * NB + rr) * npad + j * NB + c];
+        }
+        wait_flag(fb + j, timed_out);
+        if (tid < NB) vj[tid] = coherent_load(yb + j * NB + tid);
+        __syncthreads();
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s += Lb[r * LS + q * 16 + c] * vj[q * 16 + c];
+        acc += s;
+        __syncthreads();
+    }
+    part[q][r] = acc;
+    __syncthreads();
+    if (tid < NB) {
+        const int row = i * NB + tid;
+        const float ri = row < n ? (float)(rhs[(size_t)b * n + row] * (double)dsc[(size_t)b * npad + row]) : 0.0f;
+        vj[tid] = ri - (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
+    }
+    __syncthreads();
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) s += Xs[r * LS + q * 16 + c] * vj[q * 16 + c];
+    part[q][r] = s;
+    __syncthreads();
+    if (tid < NB) yb[i * NB + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+    publish(fb + i);
+}
+
+// backward solve L^T x = y: workgroup w forms block i = T - 1 - w,
+// x_i = L_ii^-T (y_i - sum_{j>i} L_ji^T x_j); x is written (or added, accumulate) for rows < n
+__global__ __launch_bounds__(NT) void chol_bwd(const float* __restrict__ W, const float* __restrict__ Xi,
+                                               const float* __restrict__ dsc, const float* __restrict__ y,
+                                               float* __restrict__ xs,
+                                               double* __restrict__ x, int n, int accumulate,
+                                               int* __restrict__ flags, int npad, int T, int* __restrict__ timed_out) {
+    __shared__ float Lb[NB * LS];
+    __shared__ float part[4][NB];
+    __shared__ float vj[NB];
+    const int b = blockIdx.y, i = T - 1 - (int)blockIdx.x, tid = threadIdx.x;
+    const int c = tid & 63, q = tid >> 6;
+    const float* Wb = W + (size_t)b * npad * npad;
+    const float* Xb = Xi + ((size_t)b * T + i) * NB * NB;
+    float* xb = xs + (size_t)b * npad;
+    int* fb = flags + (size_t)b * T;
+    float acc = 0.0f;
+    for (int j = T - 1; j > i; --j) {
+        for (int e = tid; e < NB * NB; e += NT) {
+            const int rr = e >> 6, cc = e & 63;
+            Lb[rr * LS + cc] = Wb[(size_t)(j * NB + rr) * npad + i * NB + cc];
+        }
+        wait_flag(fb + j, timed_out);
+        if (tid < NB) vj[tid] = coherent_load(xb + j * NB + tid);
+        __syncthreads();
+        float s = 0.0f;
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) s += Lb[(q * 16 + rr) * LS + c] * vj[q * 16 + rr];
+        acc += s;
+        __syncthreads();
+    }
+    part[q][c] = acc;
+    __syncthreads();
+    if (tid < NB) vj[tid] = y[(size_t)b * npad + i * NB + tid] - (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
+    __syncthreads();
+    float s = 0.0f;
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) s += Xb[(q * 16 + rr) * NB + c] * vj[q * 16 + rr];
+    part[q][c] = s;
+    __syncthreads();
+    if (tid < NB) {
+        const float xv = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+        xb[i * NB + tid] = xv;
+        const int row = i * NB + tid;
+        if (row < n) {
+            double* o = x + (size_t)b * n + row;
+            const double xr = (double)xv * (double)dsc[(size_t)b * npad + row];
+            *o = accumulate ? *o + xr : xr;
+        }
+    }
+    publish(fb + i);
+}
+
+// r = rhs - A0 x in float64, one wave per row
+__global__ __launch_bounds__(NT) void chol_residual(const float* __restrict__ A0, const double* __restrict__ rhs,
+                                                    const double* __restrict__ x, double* __restrict__ r, int n,
+                                                    int npad) {
+    const int b = blockIdx.y;
+    const int row = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const float* a = A0 + (size_t)b * npad * npad + (size_t)row * npad;
+    const double* xb = x + (size_t)b * n;
+    double s = 0.0;
+    for (int c = lane; c < n; c += 64) s += (double)a[c] * xb[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) r[(size_t)b * n + row] = rhs[(size_t)b * n + row] - s;
+}
+
+}  // namespace
+
+struct SapxChol {
+    int max_n = 0, npad_max = 0, batch = 0;
+    int n = 0, npad = 0, T = 0;
+    float* W = nullptr;
+    float* A0 = nullptr;
+    float* Xi = nullptr;
+    float* y = nullptr;
+    float* xs = nullptr;
+    float* dsc = nullptr;
+    double* r = nullptr;
+    int* flags = nullptr;  // 2 x batch x T: forward, backward
+    int* info = nullptr;
+    int* timed_out = nullptr;
+};
+
+extern "C" {
+
+int32_t sapx_chol_create(int32_t max_num_equations, int32_t batch_size, SapxChol** out) {
+    if (!out) return cfail(SAPX_ERR_INVALID, "sapx_chol_create: null output");
+    *out = nullptr;
+    if (max_num_equations <= 0 || batch_size <= 0)
+        return cfail(SAPX_ERR_INVALID, "sapx_chol_create: max_num_equations and batch_size must be positive");
+    const long long npad = ((long long)max_num_equations + NB - 1) / NB * NB;
+    const long long T = npad / NB;
+    if (T * batch_size > kMaxResident)
+        return cfail(SAPX_ERR_CAPACITY, "sapx_chol_create: (padded order / 64) x batch_size must be at most " +
+                                            std::to_string(kMaxResident));
+    SapxChol* s = new SapxChol();
+    s->max_n = max_num_equations;
+    s->npad_max = (int)npad;
+    s->batch = batch_size;
+    const size_t mat = (size_t)batch_size * npad * npad;
+    bool ok = hipMalloc(&s->W, mat * sizeof(float)) == hipSuccess &&
+              hipMalloc(&s->A0, mat * sizeof(float)) == hipSuccess &&
+              hipMalloc(&s->Xi, (size_t)batch_size * T * NB * NB * sizeof(float)) == hipSuccess &&
+              hipMalloc(&s->y, (size_t)batch_size * npad * sizeof(float)) == hipSuccess &&
+              hipMalloc(&s->xs, (size_t)batch_size * npad * sizeof(float)) == hipSuccess &&
+              hipMalloc(&s->dsc, (size_t)batch_size * npad * sizeof(float)) == hipSuccess &&
+              hipMalloc(&s->r, (size_t)batch_size * npad * sizeof(double)) == hipSuccess &&
+              hipMalloc(&s->flags, (size_t)2 * batch_size * T * sizeof(int)) == hipSuccess &&
+              hipMalloc(&s->info, (size_t)batch_size * sizeof(int)) == hipSuccess &&
+              hipMalloc(&s->timed_out, sizeof(int)) == hipSuccess &&
+              hipMemset(s->info, 0, (size_t)batch_size * sizeof(int)) == hipSuccess &&
+              hipMemset(s->timed_out, 0, sizeof(int)) == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+    if (!ok) {
+        sapx_chol_destroy(s);
+        return cfail(SAPX_ERR_HIP, "sapx_chol_create: device allocation failed");
+    }
+    *out = s;
+    return SAPX_OK;
+}
+
+void sapx_chol_destroy(SapxChol* s) {
+    if (!s) return;
+    (void)hipFree(s->W);
+    (void)hipFree(s->A0);
+    (void)hipFree(s->Xi);
+    (void)hipFree(s->y);
+    (void)hipFree(s->xs);
+    (void)hipFree(s->dsc);
+    (void)hipFree(s->r);
+    (void)hipFree(s->flags);
+    (void)hipFree(s->info);
+    (void)hipFree(s->timed_out);
+    delete s;
+}
+
+int32_t sapx_chol_factorize(SapxChol* s, const float* A, int32_t n, int32_t lda, int64_t a_stride, void* stream) {
+    if (!s || !A) return cfail(SAPX_ERR_INVALID, "sapx_chol_factorize: null argument");
+    if (n <= 0 || n > s->max_n)
+        return cfail(SAPX_ERR_INVALID, "sapx_chol_factorize: n must be in 1.." + std::to_string(s->max_n));
+    if (lda < n || a_stride < (int64_t)lda * n) return cfail(SAPX_ERR_INVALID, "sapx_chol_factorize: lda < n or stride too small");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    s->n = n;
+    s->npad = (n + NB - 1) / NB * NB;
+    s->T = s->npad / NB;
+    const int npad = s->npad, T = s->T, B = s->batch;
+    const long long total = (long long)npad * npad;
+    const int lgrid = (int)std::min<long long>((total + NT - 1) / NT, 4096);
+    hipLaunchKernelGGL(chol_scale, dim3(T, B), dim3(NT), 0, st, A, lda, (long long)a_stride, n, s->dsc, npad);
+    hipLaunchKernelGGL(chol_load, dim3(lgrid, B), dim3(NT), 0, st, A, lda, (long long)a_stride, n, s->dsc, s->W, s->A0,
+                       npad, s->info);
+    for (int k = 0; k < T; ++k) {
+        hipLaunchKernelGGL(chol_panel, dim3(T - k, B), dim3(NT), 0, st, s->W, s->Xi, npad, T, k, s->info);
+        const int m = T - k - 1;
+        if (m > 0) hipLaunchKernelGGL(chol_schur, dim3(m * (m + 1) / 2, B), dim3(NT), 0, st, s->W, npad, k);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return cfail(SAPX_ERR_HIP, std::string("sapx_chol_factorize: ") + hipGetErrorString(e));
+    return SAPX_OK;
+}
+
+int32_t sapx_chol_solve(SapxChol* s, const double* rhs, double* result, int32_t refine, void* stream) {
+    if (!s || !rhs || !result) return cfail(SAPX_ERR_INVALID, "sapx_chol_solve: null argument");
+    if (s->n <= 0) return cfail(SAPX_ERR_INVALID, "sapx_chol_solve: factorize first");
+    if (refine < 0 || refine > 8) return cfail(SAPX_ERR_INVALID, "sapx_chol_solve: refine must be 0..8");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int n = s->n, npad = s->npad, T = s->T, B = s->batch;
+    int* ff = s->flags;
+    int* fbk = s->flags + (size_t)B * T;
+    const int nflags = 2 * B * T;
+    for (int pass = 0; pass <= refine; ++pass) {
+        const double* b = rhs;
+        if (pass > 0) {
+            hipLaunchKernelGGL(chol_residual, dim3((n + 3) / 4, B), dim3(NT), 0, st, s->A0, rhs, result, s->r, n, npad);
+            b = s->r;
+        }
+        hipLaunchKernelGGL(chol_clear, dim3((nflags + NT - 1) / NT), dim3(NT), 0, st, s->flags, nflags);
+        hipLaunchKernelGGL(chol_fwd, dim3(T, B), dim3(NT), 0, st, s->W, s->Xi, s->dsc, b, n, s->y, ff, npad, T,
+                           s->timed_out);
+        hipLaunchKernelGGL(chol_bwd, dim3(T, B), dim3(NT), 0, st, s->W, s->Xi, s->dsc, s->y, s->xs, result, n,
+                           pass > 0 ? 1 : 0, fbk, npad, T, s->timed_out);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return cfail(SAPX_ERR_HIP, std::string("sapx_chol_solve: ") + hipGetErrorString(e));
+    return SAPX_OK;
+}
+
+int32_t sapx_chol_info(SapxChol* s, int32_t* info) {
+    if (!s || !info) return cfail(SAPX_ERR_INVALID, "sapx_chol_info: null argument");
+    int to = 0;
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(info, s->info, (size_t)s->batch * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&to, s->timed_out, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return cfail(SAPX_ERR_HIP, "sapx_chol_info: reading the status failed");
+    if (to) return cfail(SAPX_ERR_HIP, "sapx_chol_info: a solve's wait for a block timed out");
+    return SAPX_OK;
+}
+
+}  // extern "C"
