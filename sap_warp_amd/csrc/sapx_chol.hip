@@ -60,19 +60,54 @@ __device__ __forceinline__ float coherent_load(const float* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// thread 0 waits until flag is set (bounded); the whole workgroup then proceeds
-__device__ __forceinline__ void wait_flag(const int* flag, int* timed_out) {
-    if (threadIdx.x == 0) {
-        int polls = 0;
-        while (flag_load(flag) == 0) {
-            if (++polls > kSpinLimit) {
-                __hip_atomic_store(timed_out, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+__device__ __forceinline__ float bcast(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+// One wave factors the 64 x 64 diagonal block held in D (lower triangle) and inverts the
+// factor: lane r keeps row r of L and of X = L^-1 in registers; the right-looking sweep
+// and the row-by-row substitution of L X = I take their other rows' entries by readlane.
+// L goes back to D and X to Xs (upper triangles zero).  Returns the first column whose
+// pivot was not positive (replaced by FLT_MIN), or -1.
+__device__ int diag_factor_invert(float* D, float* Xs) {
+    const int lane = threadIdx.x & 63;
+    float a[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) a[c] = D[lane * LS + c];
+    float dinv = 1.0f;
+    int bad = -1;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const float d = bcast(a[j], j);
+        const bool ok = d > 0.0f && d <= FLT_MAX;
+        if (!ok && bad < 0) bad = j;
+        const float sq = sqrtf(ok ? d : FLT_MIN);
+        const float inv = 1.0f / sq;
+        dinv = lane == j ? inv : dinv;
+        a[j] = lane == j ? sq : (lane > j ? a[j] * inv : a[j]);
+        // rows above j only touch their (unused) upper entries
+#pragma unroll
+        for (int l = j + 1; l < NB; ++l) a[l] -= a[j] * bcast(a[j], l);
+    }
+    float x[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) x[c] = lane == c ? 1.0f : 0.0f;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+#pragma unroll
+        for (int c = 0; c <= k; ++c) x[c] = lane == k ? x[c] * dinv : x[c];
+#pragma unroll
+        for (int c = 0; c <= k; ++c) {
+            const float xk = bcast(x[c], k);
+            x[c] = lane > k ? x[c] - a[k] * xk : x[c];
         }
     }
-    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        D[lane * LS + c] = c <= lane ? a[c] : 0.0f;
+        Xs[lane * LS + c] = c <= lane ? x[c] : 0.0f;
+    }
+    return bad;
 }
 
 // publish: every thread's stores are visible device-wide before the flag
@@ -149,39 +184,13 @@ __global__ __launch_bounds__(NT) void chol_panel(float* __restrict__ W, float* _
     for (int e = tid; e < NB * NB; e += NT) {
         const int i = e >> 6, j = e & 63;
         D[i * LS + j] = Wb[(size_t)(k0 + i) * npad + k0 + j];
-        X[i * LS + j] = i == j ? 1.0f : 0.0f;
     }
-    if (tid == 0) first_bad = -1;
     __syncthreads();
-    // unblocked Cholesky of the diagonal block (lower triangle)
-    for (int j = 0; j < NB; ++j) {
-        const float d = D[j * LS + j];
-        const bool ok = d > 0.0f && d <= FLT_MAX;
-        const float s = sqrtf(ok ? d : FLT_MIN);
-        const float inv = 1.0f / s;
-        __syncthreads();
-        if (tid == 0) {
-            D[j * LS + j] = s;
-            if (!ok && first_bad < 0) first_bad = j;
-        }
-        if (tid > j && tid < NB) D[tid * LS + j] *= inv;
-        __syncthreads();
-        for (int e = tid; e < NB * NB; e += NT) {
-            const int i = e >> 6, l = e & 63;
-            if (l > j && l <= i) D[i * LS + l] -= D[i * LS + j] * D[l * LS + j];
-        }
-        __syncthreads();
+    if (tid < 64) {
+        const int bad = diag_factor_invert(D, X);
+        if (tid == 0) first_bad = bad;
     }
-    // X = L^-1 (lower): column-oriented forward substitution of L X = I
-    for (int j = 0; j < NB; ++j) {
-        if (tid <= j) X[j * LS + tid] *= 1.0f / D[j * LS + j];
-        __syncthreads();
-        for (int e = tid; e < NB * NB; e += NT) {
-            const int i = e >> 6, c = e & 63;
-            if (i > j && c <= j) X[i * LS + c] -= D[i * LS + j] * X[j * LS + c];
-        }
-        __syncthreads();
-    }
+    __syncthreads();
     if (blockIdx.x == 0) {
         float* Xo = Xi + ((size_t)b * T + k) * NB * NB;
         for (int e = tid; e < NB * NB; e += NT) {
@@ -252,15 +261,41 @@ __global__ __launch_bounds__(NT) void chol_clear(int* __restrict__ flags, int co
     for (int e = blockIdx.x * NT + threadIdx.x; e < count; e += gridDim.x * NT) flags[e] = 0;
 }
 
-// forward solve L y = r: workgroup i forms y_i = L_ii^-1 (r_i - sum_{j<i} L_ij y_j)
+constexpr int kChunk = 16;  // published blocks a solve workgroup takes in one pass
+
+// thread 0 waits for block j, then extends the run of published blocks (up to kChunk and
+// below `limit` in the direction `dir`); every thread gets the run's end
+__device__ __forceinline__ int ready_run(const int* fb, int j, int limit, int dir, int* timed_out, int* s_end) {
+    if (threadIdx.x == 0) {
+        int polls = 0;
+        while (flag_load(fb + j) == 0) {
+            if (++polls > kSpinLimit) {
+                __hip_atomic_store(timed_out, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        int e = j + dir, cnt = 1;
+        while (e != limit && cnt < kChunk && flag_load(fb + e) != 0) {
+            e += dir;
+            ++cnt;
+        }
+        *s_end = e;
+    }
+    __syncthreads();
+    return *s_end;
+}
+
+// forward solve L y = r: workgroup i forms y_i = L_ii^-1 (r_i - sum_{j<i} L_ij y_j), taking
+// the published y_j in runs; thread (r, q) owns row r and column quarter q of each block
 __global__ __launch_bounds__(NT) void chol_fwd(const float* __restrict__ W, const float* __restrict__ Xi,
                                                const float* __restrict__ dsc, const double* __restrict__ rhs, int n,
-                                               float* __restrict__ y,
-                                               int* __restrict__ flags, int npad, int T, int* __restrict__ timed_out) {
-    __shared__ float Lb[NB * LS];
+                                               float* __restrict__ y, int* __restrict__ flags, int npad, int T,
+                                               int* __restrict__ timed_out) {
     __shared__ float Xs[NB * LS];
     __shared__ float part[4][NB];
-    __shared__ float vj[NB];
+    __shared__ float vbuf[kChunk * NB];
+    __shared__ int s_end;
     const int b = blockIdx.y, i = blockIdx.x, tid = threadIdx.x;
     const int r = tid & 63, q = tid >> 6;
     const float* Wb = W + (size_t)b * npad * npad;
@@ -268,48 +303,54 @@ __global__ __launch_bounds__(NT) void chol_fwd(const float* __restrict__ W, cons
     float* yb = y + (size_t)b * npad;
     int* fb = flags + (size_t)b * T;
     for (int e = tid; e < NB * NB; e += NT) Xs[(e >> 6) * LS + (e & 63)] = Xb[e];
+    const float* Lrow = Wb + (size_t)(i * NB + r) * npad + q * 16;
     float acc = 0.0f;
-    for (int j = 0; j < i; ++j) {
-        for (int e = tid; e < NB * NB; e += NT) {
-            const int rr = e >> 6, c = e & 63;
-            Lb[rr * LS + c] = Wb[(size_t)(i * NB + rr) * npad + j * NB + c];
-        }
-        wait_flag(fb + j, timed_out);
-        if (tid < NB) vj[tid] = coherent_load(yb + j * NB + tid);
+    for (int j = 0; j < i;) {
+        const int je = ready_run(fb, j, i, 1, timed_out, &s_end);
+        for (int e = tid; e < (je - j) * NB; e += NT) vbuf[e] = coherent_load(yb + j * NB + e);
         __syncthreads();
-        float s = 0.0f;
+        for (int jj = j; jj < je; ++jj) {
+            const float4* L4 = reinterpret_cast<const float4*>(Lrow + jj * NB);
+            const float* v = vbuf + (jj - j) * NB + q * 16;
+            float sacc = 0.0f;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) s += Lb[r * LS + q * 16 + c] * vj[q * 16 + c];
-        acc += s;
+            for (int u = 0; u < 4; ++u) {
+                const float4 l = L4[u];
+                sacc += l.x * v[4 * u] + l.y * v[4 * u + 1] + l.z * v[4 * u + 2] + l.w * v[4 * u + 3];
+            }
+            acc += sacc;
+        }
         __syncthreads();
+        j = je;
     }
     part[q][r] = acc;
     __syncthreads();
     if (tid < NB) {
         const int row = i * NB + tid;
         const float ri = row < n ? (float)(rhs[(size_t)b * n + row] * (double)dsc[(size_t)b * npad + row]) : 0.0f;
-        vj[tid] = ri - (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
+        vbuf[tid] = ri - (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
     }
     __syncthreads();
-    float s = 0.0f;
+    float sacc = 0.0f;
 #pragma unroll
-    for (int c = 0; c < 16; ++c) s += Xs[r * LS + q * 16 + c] * vj[q * 16 + c];
-    part[q][r] = s;
+    for (int c = 0; c < 16; ++c) sacc += Xs[r * LS + q * 16 + c] * vbuf[q * 16 + c];
+    part[q][r] = sacc;
     __syncthreads();
     if (tid < NB) yb[i * NB + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
     publish(fb + i);
 }
 
 // backward solve L^T x = y: workgroup w forms block i = T - 1 - w,
-// x_i = L_ii^-T (y_i - sum_{j>i} L_ji^T x_j); x is written (or added, accumulate) for rows < n
+// x_i = L_ii^-T (y_i - sum_{j>i} L_ji^T x_j), taking the published x_j in runs; thread (c, q)
+// owns column c and row quarter q of each block; x (float64, scaled back) is written, or added
+// when accumulating, for rows < n
 __global__ __launch_bounds__(NT) void chol_bwd(const float* __restrict__ W, const float* __restrict__ Xi,
                                                const float* __restrict__ dsc, const float* __restrict__ y,
-                                               float* __restrict__ xs,
-                                               double* __restrict__ x, int n, int accumulate,
+                                               float* __restrict__ xs, double* __restrict__ x, int n, int accumulate,
                                                int* __restrict__ flags, int npad, int T, int* __restrict__ timed_out) {
-    __shared__ float Lb[NB * LS];
     __shared__ float part[4][NB];
-    __shared__ float vj[NB];
+    __shared__ float vbuf[kChunk * NB];
+    __shared__ int s_end;
     const int b = blockIdx.y, i = T - 1 - (int)blockIdx.x, tid = threadIdx.x;
     const int c = tid & 63, q = tid >> 6;
     const float* Wb = W + (size_t)b * npad * npad;
@@ -317,28 +358,33 @@ __global__ __launch_bounds__(NT) void chol_bwd(const float* __restrict__ W, cons
     float* xb = xs + (size_t)b * npad;
     int* fb = flags + (size_t)b * T;
     float acc = 0.0f;
-    for (int j = T - 1; j > i; --j) {
-        for (int e = tid; e < NB * NB; e += NT) {
-            const int rr = e >> 6, cc = e & 63;
-            Lb[rr * LS + cc] = Wb[(size_t)(j * NB + rr) * npad + i * NB + cc];
+    for (int j = T - 1; j > i;) {
+        const int je = ready_run(fb, j, i, -1, timed_out, &s_end);  // blocks j, j-1, ..., je+1
+        const int cnt = j - je;
+        for (int e = tid; e < cnt * NB; e += NT) {
+            const int blk = e >> 6;  // 0 -> block j
+            vbuf[e] = coherent_load(xb + (j - blk) * NB + (e & 63));
         }
-        wait_flag(fb + j, timed_out);
-        if (tid < NB) vj[tid] = coherent_load(xb + j * NB + tid);
         __syncthreads();
-        float s = 0.0f;
+        for (int t = 0; t < cnt; ++t) {
+            const float* Lc = Wb + (size_t)((j - t) * NB + q * 16) * npad + i * NB + c;
+            const float* v = vbuf + t * NB + q * 16;
+            float sacc = 0.0f;
 #pragma unroll
-        for (int rr = 0; rr < 16; ++rr) s += Lb[(q * 16 + rr) * LS + c] * vj[q * 16 + rr];
-        acc += s;
+            for (int rr = 0; rr < 16; ++rr) sacc += Lc[(size_t)rr * npad] * v[rr];
+            acc += sacc;
+        }
         __syncthreads();
+        j = je;
     }
     part[q][c] = acc;
     __syncthreads();
-    if (tid < NB) vj[tid] = y[(size_t)b * npad + i * NB + tid] - (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
+    if (tid < NB) vbuf[tid] = y[(size_t)b * npad + i * NB + tid] - (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
     __syncthreads();
-    float s = 0.0f;
+    float sacc = 0.0f;
 #pragma unroll
-    for (int rr = 0; rr < 16; ++rr) s += Xb[(q * 16 + rr) * NB + c] * vj[q * 16 + rr];
-    part[q][c] = s;
+    for (int rr = 0; rr < 16; ++rr) sacc += Xb[(q * 16 + rr) * NB + c] * vbuf[q * 16 + rr];
+    part[q][c] = sacc;
     __syncthreads();
     if (tid < NB) {
         const float xv = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
