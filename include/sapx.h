@@ -469,6 +469,89 @@ int32_t sapx_chol_solve(SapxChol* chol, const double* rhs, double* result, int32
  */
 int32_t sapx_chol_info(SapxChol* chol, int32_t* info);
 
+/* ---------------------------------------------------------------------------
+ * SAP Newton solve of one large world (thousands of dofs, tens of thousands of
+ * contacts) spread over the whole GPU.  Replaces the contact solve of the
+ * reference (SapContactSolver, contact_solve.py:6129-6285, with the projection
+ * 743-921, optimality and cost tests 1970-2087, exact-root 2913-3140 and
+ * Armijo 2670-2905 line searches) for a world too large for the per-world
+ * step kernels; the problem comes from that world's free motion and contact
+ * Jacobian (contact_jacobian.py:51-84 fields).  Contact rows are sparse: each
+ * contact lists at most kmax dofs (those of the two articulations it touches).
+ * All arrays are device memory.
+ * ------------------------------------------------------------------------- */
+typedef struct SapxLarge SapxLarge;
+
+typedef struct SapxLargeProblem {
+    int32_t n;                      /* dofs */
+    int32_t contact_count;          /* nc */
+    int32_t kmax;                   /* dofs listed per contact (the solver's kmax) */
+    int32_t pad_;
+    const float* A;                 /* n x n dynamics matrix, row-major, full symmetric */
+    const double* v_star;           /* n: free-motion velocity */
+    const double* v_guess;          /* n: initial velocity (v0 or a warm start) */
+    const int32_t* participating;   /* n: 1 where the dof takes part, else it is held at v* */
+    const int32_t* J_cols;          /* nc x kmax dof indices, -1 = unused */
+    const float* J_vals;            /* nc x 3 x kmax: rows t1, t2, n of each contact */
+    const double* phi0;             /* nc signed distances */
+    const double* weight;           /* nc contact weights w */
+    const double* stiffness;        /* nc combined stiffness k */
+    const double* tau;              /* nc dissipation time scales */
+    const double* mu;               /* nc friction coefficients */
+    /* PD / limit terms (contact_solve.py:490-741), all NULL or all set */
+    const int32_t* pd_active;
+    const double* pd_a;
+    const double* pd_gain;
+    const double* pd_limit;
+    const int32_t* lo_active;
+    const int32_t* up_active;
+    const double* lo_vhat;
+    const double* up_vhat;
+    const double* lo_r;
+    const double* up_r;
+    double dt;
+} SapxLargeProblem;
+
+typedef struct SapxLargeOpts {
+    int32_t line_search;            /* SAPX_LS_EXACT_ROOT or SAPX_LS_ARMIJO_DECAY */
+    int32_t max_iterations;
+    int32_t ls_max_iterations;
+    int32_t refine;                 /* float64-residual sweeps of each Newton solve */
+    double optimality_abs_tol;
+    double optimality_rel_tol;
+    double cost_abs_tol;
+    double cost_rel_tol;
+    double armijo_c;
+    double rho;
+    double relative_slop;
+    double beta;
+    double sigma;
+    double diag_shift;
+} SapxLargeOpts;
+
+typedef struct SapxLargeStats {
+    int32_t iterations;
+    int32_t ls_iterations;
+    int32_t converged;
+    int32_t optimality_reached;
+    int32_t status;                 /* last line search: 0 or its failure code */
+    int32_t pad_;
+    double cost;
+    double grad_norm;
+    double opt_tol;
+} SapxLargeStats;
+
+/* Workspace for worlds up to max_dofs dofs and max_contacts contacts; kmax <= 256. */
+int32_t sapx_large_create(int32_t max_dofs, int32_t max_contacts, int32_t kmax, SapxLarge** out);
+void sapx_large_destroy(SapxLarge* large);
+/*
+ * Solve; v_out (n) receives v, gamma_out (nc x 3, may be NULL) the impulses at
+ * the last evaluation, stats (host) the iteration record.  Synchronises the
+ * stream once per Newton iteration (the host reads the convergence test).
+ */
+int32_t sapx_large_solve(SapxLarge* large, const SapxLargeProblem* problem, const SapxLargeOpts* opts, double* v_out,
+                         double* gamma_out, SapxLargeStats* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

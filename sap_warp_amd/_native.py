@@ -95,6 +95,26 @@ class SapxSchedule(ctypes.Structure):
     _fields_ = [("kernel", _I), ("small_capacity", _I), ("fork", _I), ("world_order", _I)]
 
 
+class SapxLargeProblem(ctypes.Structure):
+    _fields_ = [("n", _I), ("contact_count", _I), ("kmax", _I), ("pad_", _I),
+                ("A", _P), ("v_star", _P), ("v_guess", _P), ("participating", _P), ("J_cols", _P), ("J_vals", _P),
+                ("phi0", _P), ("weight", _P), ("stiffness", _P), ("tau", _P), ("mu", _P),
+                ("pd_active", _P), ("pd_a", _P), ("pd_gain", _P), ("pd_limit", _P), ("lo_active", _P),
+                ("up_active", _P), ("lo_vhat", _P), ("up_vhat", _P), ("lo_r", _P), ("up_r", _P), ("dt", _D)]
+
+
+class SapxLargeOpts(ctypes.Structure):
+    _fields_ = [("line_search", _I), ("max_iterations", _I), ("ls_max_iterations", _I), ("refine", _I),
+                ("optimality_abs_tol", _D), ("optimality_rel_tol", _D), ("cost_abs_tol", _D), ("cost_rel_tol", _D),
+                ("armijo_c", _D), ("rho", _D), ("relative_slop", _D), ("beta", _D), ("sigma", _D),
+                ("diag_shift", _D)]
+
+
+class SapxLargeStats(ctypes.Structure):
+    _fields_ = [("iterations", _I), ("ls_iterations", _I), ("converged", _I), ("optimality_reached", _I),
+                ("status", _I), ("pad_", _I), ("cost", _D), ("grad_norm", _D), ("opt_tol", _D)]
+
+
 KERNEL_WAVE = 0
 KERNEL_FUSED = 1
 
@@ -143,6 +163,10 @@ EXPORTS = {
     "sapx_chol_factorize": (_I, [_P, _P, _I, _I, ctypes.c_int64, _P]),
     "sapx_chol_solve": (_I, [_P, _P, _P, _I, _P]),
     "sapx_chol_info": (_I, [_P, _P]),
+    "sapx_large_create": (_I, [_I, _I, _I, ctypes.POINTER(_P)]),
+    "sapx_large_destroy": (None, [_P]),
+    "sapx_large_solve": (_I, [_P, ctypes.POINTER(SapxLargeProblem), ctypes.POINTER(SapxLargeOpts), _P, _P,
+                              ctypes.POINTER(SapxLargeStats), _P]),
 }
 
 HIST_SIZE = 260  # sapx_get_histogram entries

@@ -44,6 +44,7 @@ UNITS = {
     "sapx_learned.hip": [],
     "sapx_mlp.hip": [],
     "sapx_chol.hip": [],
+    "sapx_large.hip": [],
 }
 
 

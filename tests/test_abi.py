@@ -50,6 +50,9 @@ STRUCTS = {
     "SapxControlView": nat.SapxControlView,
     "SapxContactsView": nat.SapxContactsView,
     "SapxSchedule": nat.SapxSchedule,
+    "SapxLargeProblem": nat.SapxLargeProblem,
+    "SapxLargeOpts": nat.SapxLargeOpts,
+    "SapxLargeStats": nat.SapxLargeStats,
 }
 
 

@@ -1,0 +1,136 @@
+"""Large-world SAP solve (``sapx_large_*``) against the oracle's Newton solve of the same problem.
+
+Worlds: one pile of 32 or 64 free boxes and spheres (192 / 384 dofs) settled onto the ground by
+the C++ restatement of the step (oracle), then one step formed by the float64 oracle: its free
+motion (A, v*), contact set (Jacobian rows, phi0, weights, stiffness, tau, mu), PD / limit terms
+and participation mask are handed to the GPU solver (A and J rounded to float32, J in the
+sparse per-contact layout).  Checks, per line search (exact_root, armijo_decay):
+
+* v within 1e-5 (relative max-norm) of the oracle's float64 solve of that same float32-rounded
+  problem (``sap_solve``), and gamma within 1e-4 (redundant pile contacts: gamma is set by the
+  regularisation and is ill-conditioned, v is not);
+* the reference optimality test, evaluated by the oracle in float64 at the GPU's v: at most
+  the tolerance when the GPU reports optimality;
+* the oracle's own float64-formed solution (the step's v) within 1e-5, as the step tests;
+* a world without contacts or active terms returns v* at once.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KMAX = 16
+
+
+def _settled_pile(bodies, seed, steps, ls):
+    from oracle import collide as oc
+    from oracle import cpu_step as cs
+    from oracle import sap_oracle as so
+    from sap_warp_amd import hetero
+
+    model = hetero.build_pile_scene(1, seed=seed, bodies=bodies)
+    C = 8 * bodies
+    dt = 2.0e-3
+    opts = so.StepOptions(preset="approx32", line_search=ls, max_rigid_contact=C,
+                          solve=so.SolveOptions.for_variant(ls))
+    stepper = cs.CpuStepper(model, opts)
+    state = dict(joint_q=model.joint_q, joint_qd=model.joint_qd, body_q=model.body_q, body_qd=model.body_qd,
+                 body_f=np.zeros((model.body_count, 6), np.float32))
+    control = dict(joint_f=model.joint_f, joint_target_pos=model.joint_target_pos,
+                   joint_target_vel=model.joint_target_vel, joint_act=model.joint_act)
+    for _ in range(steps):
+        c = oc.collide(model, state["body_q"], C, threads=8)
+        out = stepper.step(state, control, c, dt, threads=8)
+        state = dict(joint_q=out["joint_q"], joint_qd=out["joint_qd"], body_q=out["body_q"], body_qd=out["body_qd"],
+                     body_f=state["body_f"])
+    c = oc.collide(model, state["body_q"], C, threads=8)
+    ref = so.step(model, state, control, c, dt, opts)
+    return model, ref["per_world"][0], opts, dt
+
+
+def _gpu_inputs(model, r, dt):
+    import torch
+
+    from oracle import sap_oracle as so
+    from sap_warp_amd.large_world import sparse_contact_rows
+
+    fm, cs, mt = r["free_motion"], r["contacts"], r["model_terms"]
+    wv = so.WorldView(model, 0)
+    part = so._participating(model, wv, cs, mt)
+    dev = "cuda:0"
+    A32 = fm.A_env.astype(np.float32)
+    J32 = cs.J.astype(np.float32)
+    cols, vals = sparse_contact_rows(J32, KMAX)
+    T = lambda x, dt_: torch.tensor(np.ascontiguousarray(x), dtype=dt_, device=dev)  # noqa: E731
+    f64, i32 = torch.float64, torch.int32
+    mterms = dict(pd_active=T(mt.pd_active, i32), pd_a=T(mt.pd_a, f64), pd_gain=T(mt.pd_gain, f64),
+                  pd_limit=T(mt.pd_limit, f64), lo_active=T(mt.lo_active, i32), up_active=T(mt.up_active, i32),
+                  lo_vhat=T(mt.lo_vhat, f64), up_vhat=T(mt.up_vhat, f64), lo_r=T(mt.lo_r, f64), up_r=T(mt.up_r, f64))
+    args = dict(A=T(A32, torch.float32), v_star=T(fm.v_star, f64), v_guess=T(r["v0"], f64),
+                participating=T(part, i32), J_cols=T(cols, i32), J_vals=T(vals, torch.float32), phi0=T(cs.phi0, f64),
+                weight=T(cs.w, f64), stiffness=T(cs.k, f64), tau=T(cs.tau, f64), mu=T(cs.mu, f64), dt=dt,
+                model_terms=mterms)
+    return args, A32.astype(np.float64), J32.astype(np.float64), part
+
+
+def _rel(x, ref):
+    return float(np.max(np.abs(np.asarray(x) - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+@pytest.mark.parametrize("bodies,ls", [(32, "exact_root"), (32, "armijo_decay"), (64, "exact_root")])
+def test_pile_solve_matches_oracle(gpu_device, bodies, ls):
+    from dataclasses import replace
+
+    from oracle import sap_oracle as so
+    from sap_warp_amd.large_world import LargeWorldSolver
+
+    print(f"large-world solve: pile {bodies} {ls}", flush=True)
+    model, r, opts, dt = _settled_pile(bodies, seed=3, steps=250, ls=ls)
+    cs, mt, fm = r["contacts"], r["model_terms"], r["free_motion"]
+    nc = cs.count
+    assert nc >= bodies, f"pile not settled: {nc} contacts"
+    args, A64, J64, part = _gpu_inputs(model, r, dt)
+    solver = LargeWorldSolver(model.joint_dof_count, nc, KMAX)
+    v, gamma, st = solver.solve(line_search=ls, **args)
+    v = v.cpu().numpy()
+    gamma = gamma.cpu().numpy()
+    solver.close()
+    # the oracle's float64 solve of the same (float32-rounded) problem
+    wv = so.WorldView(model, 0)
+    cs32 = replace(cs, J=J64)
+    sol = so.sap_solve(model, wv, A64, fm.v_star, r["v0"], None, cs32, mt, dt, opts.solve)
+    print(f"n {model.joint_dof_count} nc {nc} gpu it {st.iterations} ls {st.ls_iterations} conv {st.converged} "
+          f"opt {st.optimality_reached} | oracle it {sol.iterations} | v err {_rel(v, sol.v):.2e} "
+          f"gamma err {_rel(gamma, sol.gamma):.2e} vs own {_rel(v, r['v_sap']):.2e}", flush=True)
+    assert st.converged and sol.converged
+    assert _rel(v, sol.v) <= 1e-5
+    assert _rel(gamma, sol.gamma) <= 1e-4
+    assert _rel(v, r["v_sap"]) <= 1e-5
+    if st.optimality_reached:
+        prob = so.SapProblem(A64, fm.v_star, cs32, mt, dt, opts.solve.beta, opts.solve.sigma)
+        ev = prob.evaluate(v)
+        a_diag = np.array([d if d >= 1e-12 else 1e-12 for d in np.diag(A64)])
+        gn, tol = so._optimality_norms(ev, part, 1.0 / np.sqrt(a_diag), opts.solve)
+        assert gn <= tol * 1.000001, (gn, tol)
+
+
+def test_unconstrained_world_returns_v_star(gpu_device):
+    import torch
+
+    from sap_warp_amd.large_world import LargeWorldSolver
+
+    n = 70
+    rng = np.random.default_rng(2)
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    A = (Q * rng.uniform(1.0, 10.0, n)) @ Q.T
+    dev = "cuda:0"
+    vs = rng.normal(size=n)
+    solver = LargeWorldSolver(n, 4, KMAX)
+    v, gamma, st = solver.solve(A=torch.tensor(A, dtype=torch.float32, device=dev),
+                                v_star=torch.tensor(vs, device=dev), v_guess=torch.zeros(n, dtype=torch.float64, device=dev),
+                                participating=torch.zeros(n, dtype=torch.int32, device=dev), J_cols=None, J_vals=None,
+                                phi0=torch.zeros(0, dtype=torch.float64, device=dev), weight=None, stiffness=None,
+                                tau=None, mu=None, dt=1e-3)
+    assert st.converged and st.iterations == 0
+    np.testing.assert_array_equal(v.cpu().numpy(), vs)
+    solver.close()
