@@ -422,6 +422,70 @@ def hetero_run(worlds: int, steps: int, warmup: int, seed: int, dt: float, line_
     }
 
 
+def large_world_run(nx: int, ny: int, nz: int, repeats: int, line_search: str) -> dict:
+    """One large world on the whole GPU: the SAP Newton solve of a resting block of nx*ny*nz free
+    cubes (sap_warp_amd.large_world.stacked_boxes_problem, synthetic; 16x8x8 = 6144 dofs and
+    11 520 contacts, the size of a settled thousand-body bin), timed with HIP events around whole
+    solves, and the blocked Cholesky of that order timed on its own (its cost does not depend on
+    the values: dense float32, 64 x 64 blocks)."""
+    import numpy as np
+    import torch
+
+    from sap_warp_amd.blocked_cholesky import BlockCholeskySolverBatched
+    from sap_warp_amd.large_world import LargeWorldSolver, stacked_boxes_problem
+
+    dev = "cuda:0"
+    P = stacked_boxes_problem(nx, ny, nz, seed=0)
+    nb = nx * ny * nz
+    n, nc = 6 * nb, len(P["phi0"])
+    A = torch.zeros((n, n), dtype=torch.float32, device=dev)
+    idx = torch.arange(nb, device=dev)
+    A.view(nb, 6, nb, 6)[idx, :, idx, :] = torch.tensor(P["A_blocks"], dtype=torch.float32, device=dev)
+    T = lambda x, d: torch.tensor(np.ascontiguousarray(x), dtype=d, device=dev)  # noqa: E731
+    f64 = torch.float64
+    args = dict(A=A, v_star=T(P["v_star"], f64), v_guess=T(P["v_guess"], f64),
+                participating=T(P["participating"], torch.int32), J_cols=T(P["J_cols"], torch.int32),
+                J_vals=T(P["J_vals"], torch.float32), phi0=T(P["phi0"], f64), weight=T(P["weight"], f64),
+                stiffness=T(P["stiffness"], f64), tau=T(P["tau"], f64), mu=T(P["mu"], f64), dt=P["dt"])
+    solver = LargeWorldSolver(n, nc, 12)
+    _, _, st = solver.solve(line_search=line_search, **args)  # warm-up (code objects, first touches)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(repeats):
+        _, _, st = solver.solve(line_search=line_search, **args)
+    e1.record()
+    torch.cuda.synchronize()
+    solve_ms = e0.elapsed_time(e1) / repeats
+    solver.close()
+    chol = BlockCholeskySolverBatched(n, 1, refine=1)
+    Ab = A.unsqueeze(0)
+    chol.factorize(Ab, n)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(repeats):
+        chol.factorize(Ab, n)
+    e1.record()
+    torch.cuda.synchronize()
+    factor_ms = e0.elapsed_time(e1) / repeats
+    chol.close()
+    flops = n ** 3 / 3.0
+    return {
+        "workload": f"one world: {nb} resting free cubes ({nx}x{ny}x{nz}), {n} dofs, {nc} contacts, "
+                    f"{line_search}, synthetic (large_world.stacked_boxes_problem)",
+        "solve_ms": round(solve_ms, 3),
+        "newton_iterations": st.iterations,
+        "line_search_trials": st.ls_iterations,
+        "converged": st.converged,
+        "ms_per_newton_iteration": round(solve_ms / max(st.iterations, 1), 3),
+        "factor_ms": round(factor_ms, 3),
+        "factor_tflops": round(flops / (factor_ms * 1e-3) / 1e12, 3),
+        "factor_frac_of_fp32_peak": round(flops / (factor_ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 5),
+        "timing": f"HIP events around {repeats} whole solves (host-driven Newton loop, one sync per iteration) "
+                  f"and {repeats} factorisations; flops = n^3/3",
+    }
+
+
 def copy_calibration() -> int:
     """Three copies of a 512 MiB buffer: the HBM counters of a kernel of known traffic."""
     import torch
@@ -462,6 +526,9 @@ def main() -> int:
     ap.add_argument("--hetero-large", type=int, default=1,
                     help="also time the heterogeneous batch with piles of 8, 16 and 32 bodies (1) or not (0)")
     ap.add_argument("--hetero-large-steps", type=int, default=200)
+    ap.add_argument("--large-world", type=int, default=1,
+                    help="also time the single large world (1024 cubes, 6144 dofs) solve (1) or not (0)")
+    ap.add_argument("--large-world-repeats", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 512 MiB device copies (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -571,6 +638,10 @@ def main() -> int:
                                                    args.seed, dt, args.line_search, args.hetero_graph_steps,
                                                    pile_sizes=(8, 16, 32))
             _log(f"heterogeneous batch with large piles done ({out['hetero_large_piles']['ms_per_step']:.4f} ms/step)")
+    if args.large_world:
+        out["large_world"] = large_world_run(16, 8, 8, args.large_world_repeats, args.line_search)
+        _log(f"large world done ({out['large_world']['solve_ms']:.1f} ms/solve, "
+             f"factor {out['large_world']['factor_ms']:.2f} ms)")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_worlds, args.cpu_steps, args.seed, dt, args.line_search,
                                            args.cpu_threads)

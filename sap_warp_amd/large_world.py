@@ -139,3 +139,105 @@ class LargeWorldSolver:
                                 bool(st.optimality_reached), int(st.status), float(st.cost), float(st.grad_norm),
                                 float(st.opt_tol))
         return v, gamma[:nc], stats
+
+
+def stacked_boxes_problem(nx: int = 16, ny: int = 8, nz: int = 8, *, seed: int = 0, dt: float = 1.0e-3,
+                          size: float = 0.1, stiffness: float = 1.0e5, tau: float = 0.01, mu: float = 0.7,
+                          penetration: float = 1.0e-4, gravity: float = -9.81):
+    """The SAP problem of one world holding a resting block of nx x ny x nz free cubes on the ground.
+
+    Cubes of edge ``size`` (density U(500, 2000) kg/m^3, COM at the body origin, unrotated) sit on a
+    touching lattice; every pair of face neighbours and every bottom cube against the ground
+    touches at the four corners of the shared face (inset by 10 % of the edge), ``penetration``
+    deep.  Starting at rest, v* = dt g for every body.  Returns numpy arrays: A_blocks (nb, 6, 6:
+    each body's spatial inertia in SAP [w, v_O] order; A is their block diagonal), v_star,
+    v_guess (zeros), participating, the contact rows sparse (J_cols (nc, 12): body 0's then body
+    1's six dofs, -1 for the ground; J_vals (nc, 3, 12): rows t1, t2, n of the relative velocity
+    of body 1 minus body 0 at the contact point), phi0, weight (the body_inertia contact weight,
+    sap_helpers.py:307-338), stiffness, tau, mu.  1024 cubes (16 x 8 x 8) give 6144 dofs and
+    11 520 contacts: the size of a settled bin of a thousand bodies."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nb = nx * ny * nz
+    n = 6 * nb
+    h = 0.5 * size
+    idx = lambda i, j, k: (k * ny + j) * nx + i  # noqa: E731
+    pos = np.zeros((nb, 3))
+    mass = np.zeros(nb)
+    inert = np.zeros(nb)
+    blocks = np.zeros((nb, 6, 6))
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                b = idx(i, j, k)
+                pos[b] = ((i - 0.5 * (nx - 1)) * size, (j - 0.5 * (ny - 1)) * size, h + k * size)
+                m = rng.uniform(500.0, 2000.0) * size ** 3
+                mass[b] = m
+                inert[b] = m * size * size / 6.0
+                blocks[b, :3, :3] = np.eye(3) * inert[b]
+                blocks[b, 3:, 3:] = np.eye(3) * m
+    v_star = np.zeros(n)
+    v_star[5::6] = dt * gravity
+    contacts = []  # (body0, body1, point, normal)
+    inset = 0.4 * size
+    axes = np.eye(3)
+    for b in range(nb):
+        k, rem = divmod(b, nx * ny)
+        j, i = divmod(rem, nx)
+        for ax, (di, dj, dk) in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))):
+            ii, jj, kk = i + di, j + dj, k + dk
+            if ii >= nx or jj >= ny or kk >= nz:
+                continue
+            nrm = axes[ax]
+            u, w2 = axes[(ax + 1) % 3], axes[(ax + 2) % 3]
+            centre = pos[b] + h * nrm
+            for su in (-1.0, 1.0):
+                for sw in (-1.0, 1.0):
+                    contacts.append((b, idx(ii, jj, kk), centre + inset * (su * u + sw * w2), nrm))
+        if k == 0:
+            centre = pos[b] - h * axes[2]
+            for su in (-1.0, 1.0):
+                for sw in (-1.0, 1.0):
+                    contacts.append((-1, b, centre + inset * (su * axes[0] + sw * axes[1]), axes[2]))
+    nc = len(contacts)
+    cols = np.full((nc, 12), -1, dtype=np.int32)
+    vals = np.zeros((nc, 3, 12), dtype=np.float64)
+    weight = np.zeros(nc)
+    for c, (b0, b1, p, nrm) in enumerate(contacts):
+        t1 = np.cross(nrm, axes[0] if abs(nrm[0]) < 0.9 else axes[1])
+        t1 /= np.linalg.norm(t1)
+        frame = (t1, np.cross(nrm, t1), nrm)
+        wsum = 0.0
+        for slot, (b, sgn) in enumerate(((b0, -1.0), (b1, 1.0))):
+            if b < 0:
+                continue
+            cols[c, 6 * slot:6 * slot + 6] = np.arange(6 * b, 6 * b + 6)
+            r = p - pos[b]
+            for row, e in enumerate(frame):
+                cb = np.cross(r, e)
+                vals[c, row, 6 * slot:6 * slot + 3] = sgn * cb
+                vals[c, row, 6 * slot + 3:6 * slot + 6] = sgn * e
+                wsum += 1.0 / mass[b] + float(cb @ cb) / inert[b]
+        weight[c] = max(wsum / 3.0, 1.0e-12)
+    return dict(A_blocks=blocks, v_star=v_star, v_guess=np.zeros(n), participating=np.ones(n, dtype=np.int32),
+                J_cols=cols, J_vals=vals, phi0=np.full(nc, -penetration), weight=weight,
+                stiffness=np.full(nc, stiffness), tau=np.full(nc, tau), mu=np.full(nc, mu), dt=dt)
+
+
+def dense_rows(cols: np.ndarray, vals: np.ndarray, n: int) -> np.ndarray:
+    """Sparse contact rows back to dense (nc, 3, n)."""
+    nc, kmax = cols.shape
+    J = np.zeros((nc, 3, n), dtype=vals.dtype)
+    for c in range(nc):
+        for k in range(kmax):
+            if cols[c, k] >= 0:
+                J[c, :, cols[c, k]] += vals[c, :, k]
+    return J
+
+
+def block_diagonal(blocks: np.ndarray) -> np.ndarray:
+    """(nb, 6, 6) blocks -> the dense (6 nb, 6 nb) block diagonal."""
+    nb = blocks.shape[0]
+    A = np.zeros((6 * nb, 6 * nb), dtype=blocks.dtype)
+    for b in range(nb):
+        A[6 * b:6 * b + 6, 6 * b:6 * b + 6] = blocks[b]
+    return A

@@ -134,3 +134,47 @@ def test_unconstrained_world_returns_v_star(gpu_device):
     assert st.converged and st.iterations == 0
     np.testing.assert_array_equal(v.cpu().numpy(), vs)
     solver.close()
+
+
+@pytest.mark.parametrize("ls", ["exact_root", "armijo_decay"])
+def test_stacked_boxes_matches_oracle(gpu_device, monkeypatch, ls):
+    """The synthetic resting block of cubes (the bench's large-world problem, here 4 x 4 x 4 cubes:
+    384 dofs, 640 contacts): GPU solve against the oracle's Newton solve of the same data."""
+    import torch
+
+    from oracle import sap_oracle as so
+    from sap_warp_amd.large_world import LargeWorldSolver, block_diagonal, dense_rows, stacked_boxes_problem
+
+    P = stacked_boxes_problem(4, 4, 4, seed=1)
+    n, nc = 6 * 64, len(P["phi0"])
+    A32 = block_diagonal(P["A_blocks"]).astype(np.float32)
+    vals32 = P["J_vals"].astype(np.float32)
+    J64 = dense_rows(P["J_cols"], vals32.astype(np.float64), n)
+    z3 = np.zeros((nc, 3))
+    cs = so.ContactSet(body0=np.zeros(nc, int), body1=np.zeros(nc, int), x0=z3, x1=z3, point=z3,
+                       R=np.zeros((nc, 3, 3)), phi0=P["phi0"], mu=P["mu"], k=P["stiffness"], tau=P["tau"],
+                       w=P["weight"], J=J64, flat_index=np.zeros(nc, int))
+    mt = so.ModelTerms(*[np.zeros(n) for _ in range(10)])
+    mt.pd_active, mt.lo_active, mt.up_active = (np.zeros(n, int) for _ in range(3))
+    mt.lo_r[:] = 1.0
+    mt.up_r[:] = 1.0
+    # one world of free bodies: every dof takes part
+    monkeypatch.setattr(so, "_participating", lambda *a: np.ones(n, dtype=int))
+    sol = so.sap_solve(None, None, A32.astype(np.float64), P["v_star"], P["v_guess"], None, cs, mt, P["dt"],
+                       so.SolveOptions.for_variant(ls))
+    dev = "cuda:0"
+    T = lambda x, d: torch.tensor(np.ascontiguousarray(x), dtype=d, device=dev)  # noqa: E731
+    solver = LargeWorldSolver(n, nc, 12)
+    v, gamma, st = solver.solve(A=T(A32, torch.float32), v_star=T(P["v_star"], torch.float64),
+                                v_guess=T(P["v_guess"], torch.float64), participating=T(P["participating"], torch.int32),
+                                J_cols=T(P["J_cols"], torch.int32), J_vals=T(vals32, torch.float32),
+                                phi0=T(P["phi0"], torch.float64), weight=T(P["weight"], torch.float64),
+                                stiffness=T(P["stiffness"], torch.float64), tau=T(P["tau"], torch.float64),
+                                mu=T(P["mu"], torch.float64), dt=P["dt"], line_search=ls)
+    v, gamma = v.cpu().numpy(), gamma.cpu().numpy()
+    solver.close()
+    print(f"stacked boxes {ls}: gpu it {st.iterations} oracle it {sol.iterations} v err {_rel(v, sol.v):.2e} "
+          f"gamma err {_rel(gamma, sol.gamma):.2e}", flush=True)
+    assert st.converged and sol.converged and st.optimality_reached
+    assert _rel(v, sol.v) <= 1e-5
+    assert _rel(gamma, sol.gamma) <= 1e-4
