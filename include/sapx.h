@@ -472,7 +472,7 @@ int32_t sapx_chol_info(SapxChol* chol, int32_t* info);
 /* ---------------------------------------------------------------------------
  * SAP Newton solve of one large world (thousands of dofs, tens of thousands of
  * contacts) spread over the whole GPU.  Replaces the contact solve of the
- * reference (SapContactSolver, contact_solve.py:6129-6285, with the projection
+ * reference (SapContactSolve, contact_solve.py:6129-6285, with the projection
  * 743-921, optimality and cost tests 1970-2087, exact-root 2913-3140 and
  * Armijo 2670-2905 line searches) for a world too large for the per-world
  * step kernels; the problem comes from that world's free motion and contact

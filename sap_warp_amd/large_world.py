@@ -5,7 +5,7 @@ world of thousands of dofs -- a bin of free bodies, a robot among them -- is sol
 stage spread over the GPU: contact evaluation one thread per contact, gradient and line-search
 products one wave per dof row, the Newton matrix assembled densely and factored by the blocked
 Cholesky (``blocked_cholesky.py``), and every line-search trial run on the device.  The solve
-mirrors the reference's ``SapContactSolver`` iteration (``contact_solve.py:6129-6285``) on the
+mirrors the reference's ``SapContactSolve`` iteration (``contact_solve.py:6129-6285``) on the
 problem a world's free motion and contact Jacobian produced (``contact_jacobian.py:51-84``).
 
 Contact rows are passed sparse: each contact lists at most ``kmax`` dofs (those of the two
