@@ -64,49 +64,136 @@ __device__ __forceinline__ float bcast(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// One wave factors the 64 x 64 diagonal block held in D (lower triangle) and inverts the
-// factor: lane r keeps row r of L and of X = L^-1 in registers; the right-looking sweep
-// and the row-by-row substitution of L X = I take their other rows' entries by readlane.
-// L goes back to D and X to Xs (upper triangles zero).  Returns the first column whose
-// pivot was not positive (replaced by FLT_MIN), or -1.
-__device__ int diag_factor_invert(float* D, float* Xs) {
+constexpr int SB = 16;       // sub-block order inside the 64 x 64 diagonal block
+constexpr int YS = SB + 1;   // LDS row stride of the 16 x 16 work tiles
+
+// acc += P Q^T (qt) or P Q over one 16 x 16 x 16 product on one wave (v_mfma_f32_16x16x4f32):
+// P rows at P + i * ps, Q rows at Q + r * qs; acc[e] is C[4 (lane >> 4) + e][lane & 15]
+__device__ __forceinline__ void mfma16(const float* P, int ps, const float* Q, int qs, bool qt, f4& acc) {
     const int lane = threadIdx.x & 63;
-    float a[NB];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) a[c] = D[lane * LS + c];
-    float dinv = 1.0f;
-    int bad = -1;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const float d = bcast(a[j], j);
-        const bool ok = d > 0.0f && d <= FLT_MAX;
-        if (!ok && bad < 0) bad = j;
-        const float sq = sqrtf(ok ? d : FLT_MIN);
-        const float inv = 1.0f / sq;
-        dinv = lane == j ? inv : dinv;
-        a[j] = lane == j ? sq : (lane > j ? a[j] * inv : a[j]);
-        // rows above j only touch their (unused) upper entries
-#pragma unroll
-        for (int l = j + 1; l < NB; ++l) a[l] -= a[j] * bcast(a[j], l);
+    for (int kk = 0; kk < SB; kk += 4) {
+        const int k = kk + (lane >> 4), r = lane & 15;
+        const float a = P[r * ps + k];
+        const float b = qt ? Q[r * qs + k] : Q[k * qs + r];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
     }
-    float x[NB];
+}
+__device__ __forceinline__ void store16(float* C, int cs, const f4& acc, float scale) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) x[c] = lane == c ? 1.0f : 0.0f;
+    for (int e = 0; e < 4; ++e) C[(4 * (lane >> 4) + e) * cs + (lane & 15)] = scale * acc[e];
+}
+
+// Factor the 64 x 64 diagonal block held in D (lower triangle) and invert the factor, on the
+// workgroup's four waves, in 16 x 16 sub-blocks: per sub-block column, wave 0 factors the
+// 16 x 16 diagonal block and inverts it in registers (lane r = row r, other rows by readlane),
+// the waves form the sub-blocks below it (A Y^T, MFMA) and apply their Schur updates (MFMA);
+// then X = L^-1 by block forward substitution, X_ij = -Y_i sum_{m=j..i-1} L_im X_mj, one
+// column of blocks per wave.  L stays in D and X goes to Xs (upper triangles are zeroed when
+// stored).  Y (4 x 16 x YS) and tmp (4 x 16 x YS) are LDS work tiles.  Returns the first column
+// whose pivot was not positive (replaced by FLT_MIN), or -1 (valid on wave 0).
+__device__ int diag_factor_invert(float* D, float* Xs, float* Y, float* tmp) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int bad = -1;
+    for (int kb = 0; kb < NB / SB; ++kb) {
+        const int k0 = kb * SB;
+        if (wave == 0) {
+            const int r = lane & 15;
+            float a[SB];
 #pragma unroll
-    for (int k = 0; k < NB; ++k) {
+            for (int c = 0; c < SB; ++c) a[c] = D[(k0 + r) * LS + k0 + c];
+            float dinv = 1.0f;
 #pragma unroll
-        for (int c = 0; c <= k; ++c) x[c] = lane == k ? x[c] * dinv : x[c];
+            for (int j = 0; j < SB; ++j) {
+                const float d = bcast(a[j], j);
+                const bool ok = d > 0.0f && d <= FLT_MAX;
+                if (!ok && bad < 0) bad = k0 + j;
+                const float sq = sqrtf(ok ? d : FLT_MIN);
+                const float inv = 1.0f / sq;
+                dinv = r == j ? inv : dinv;
+                a[j] = r == j ? sq : (r > j ? a[j] * inv : a[j]);
 #pragma unroll
-        for (int c = 0; c <= k; ++c) {
-            const float xk = bcast(x[c], k);
-            x[c] = lane > k ? x[c] - a[k] * xk : x[c];
+                for (int l = j + 1; l < SB; ++l) a[l] -= a[j] * bcast(a[j], l);
+            }
+            float x[SB];
+#pragma unroll
+            for (int c = 0; c < SB; ++c) x[c] = r == c ? 1.0f : 0.0f;
+#pragma unroll
+            for (int k = 0; k < SB; ++k) {
+#pragma unroll
+                for (int c = 0; c <= k; ++c) x[c] = r == k ? x[c] * dinv : x[c];
+#pragma unroll
+                for (int c = 0; c <= k; ++c) {
+                    const float xk = bcast(x[c], k);
+                    x[c] = r > k ? x[c] - a[k] * xk : x[c];
+                }
+            }
+            if (lane < SB) {
+#pragma unroll
+                for (int c = 0; c < SB; ++c) {
+                    D[(k0 + r) * LS + k0 + c] = c <= r ? a[c] : 0.0f;
+                    Y[(kb * SB + r) * YS + c] = c <= r ? x[c] : 0.0f;
+                }
+            }
+        }
+        __syncthreads();
+        // sub-blocks below: L_ib = A_ib Y^T
+        const int below = NB / SB - 1 - kb;
+        if (wave < below) {
+            const int i0 = k0 + SB * (wave + 1);
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            mfma16(D + i0 * LS + k0, LS, Y + kb * SB * YS, YS, true, acc);
+            store16(tmp + wave * SB * YS, YS, acc, 1.0f);
+        }
+        __syncthreads();
+        if (wave < below) {
+            const int i0 = k0 + SB * (wave + 1);
+            for (int e = lane; e < SB * SB; e += 64) {
+                const int rr = e >> 4, cc = e & 15;
+                D[(i0 + rr) * LS + k0 + cc] = tmp[(wave * SB + rr) * YS + cc];
+            }
+        }
+        __syncthreads();
+        // Schur updates of the trailing sub-blocks (i, j), kb < j <= i
+        const int ntile = below * (below + 1) / 2;
+        for (int t = wave; t < ntile; t += 4) {
+            int ib = 0, rem = t;
+            while (rem > ib) {
+                rem -= ib + 1;
+                ++ib;
+            }
+            const int i0 = k0 + SB * (ib + 1), j0 = k0 + SB * (rem + 1);
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            mfma16(D + i0 * LS + k0, LS, D + j0 * LS + k0, LS, true, acc);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) D[(i0 + 4 * (lane >> 4) + e) * LS + j0 + (lane & 15)] -= acc[e];
+        }
+        __syncthreads();
+    }
+    // X = L^-1: diagonal blocks Y_i, below them X_ij for i > j (wave j), upper blocks zero
+    for (int e = threadIdx.x; e < NB * NB; e += NT) {
+        const int r = e >> 6, c = e & 63, bi = r >> 4, bj = c >> 4;
+        Xs[r * LS + c] = bi == bj ? Y[(bi * SB + (r & 15)) * YS + (c & 15)] : 0.0f;
+    }
+    __syncthreads();
+    {
+        const int j = wave, j0 = j * SB;
+        float* tw = tmp + wave * SB * YS;
+        for (int i = j + 1; i < NB / SB; ++i) {
+            const int i0 = i * SB;
+            f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int m = j; m < i; ++m) mfma16(D + i0 * LS + m * SB, LS, Xs + m * SB * LS + j0, LS, false, acc);
+            store16(tw, YS, acc, 1.0f);
+            __builtin_amdgcn_wave_barrier();
+            f4 acc2 = {0.0f, 0.0f, 0.0f, 0.0f};
+            mfma16(Y + i * SB * YS, YS, tw, YS, false, acc2);
+            __builtin_amdgcn_wave_barrier();
+            store16(Xs + i0 * LS + j0, LS, acc2, -1.0f);
+            __builtin_amdgcn_wave_barrier();
         }
     }
-#pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        D[lane * LS + c] = c <= lane ? a[c] : 0.0f;
-        Xs[lane * LS + c] = c <= lane ? x[c] : 0.0f;
-    }
+    __syncthreads();
     return bad;
 }
 
@@ -177,6 +264,8 @@ __global__ __launch_bounds__(NT) void chol_panel(float* __restrict__ W, float* _
                                                  int* __restrict__ info) {
     __shared__ float D[NB * LS];
     __shared__ float X[NB * LS];
+    __shared__ float Y[NB * YS];
+    __shared__ float tmp[4 * SB * YS];
     __shared__ int first_bad;
     const int b = blockIdx.y, tid = threadIdx.x;
     float* Wb = W + (size_t)b * npad * npad;
@@ -186,8 +275,8 @@ __global__ __launch_bounds__(NT) void chol_panel(float* __restrict__ W, float* _
         D[i * LS + j] = Wb[(size_t)(k0 + i) * npad + k0 + j];
     }
     __syncthreads();
-    if (tid < 64) {
-        const int bad = diag_factor_invert(D, X);
+    {
+        const int bad = diag_factor_invert(D, X, Y, tmp);
         if (tid == 0) first_bad = bad;
     }
     __syncthreads();
