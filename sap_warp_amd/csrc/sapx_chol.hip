@@ -6,12 +6,15 @@
 // stage spreads over the whole GPU:
 //
 //   right-looking, 64 x 64 blocks, float32, per block column k:
-//     chol_panel  (T - k workgroups)  each factors the diagonal block A_kk in LDS and
-//                                     inverts L_kk (redundantly: no grid barrier);
-//                                     workgroup 0 stores L_kk and L_kk^-1, the others the
-//                                     panel blocks L_ik = A_ik L_kk^-T by MFMA
-//     chol_schur  ((T-k-1)(T-k)/2)    the Schur-complement update A_ij -= L_ik L_jk^T of
-//                                     every lower trailing block, v_mfma_f32_16x16x4f32
+//     panel        (T - k workgroups)  each factors the diagonal block A_kk in LDS and
+//                                      inverts L_kk (redundantly: no grid barrier);
+//                                      workgroup 0 stores L_kk and L_kk^-1, the others the
+//                                      panel blocks L_ik = A_ik L_kk^-T by MFMA
+//     Schur update                     A_ij -= L_ik L_jk^T of every lower trailing block,
+//                                      v_mfma_f32_16x16x4f32, split with look-ahead: the
+//                                      tiles of column k + 1 first (chol_schur_next), then
+//                                      the other tiles in the same launch as panel k + 1
+//                                      (chol_panel_schur), which touches none of them
 //   solves, one launch each: workgroup i computes block i of y (forward) or x (backward)
 //   as soon as the blocks it depends on are published (per-block ready flags with
 //   release / acquire ordering and a bounded wait), using the stored diagonal inverses, so
@@ -93,7 +96,7 @@ __device__ __forceinline__ void store16(float* C, int cs, const f4& acc, float s
 // column of blocks per wave.  L stays in D and X goes to Xs (upper triangles are zeroed when
 // stored).  Y (4 x 16 x YS) and tmp (4 x 16 x YS) are LDS work tiles.  Returns the first column
 // whose pivot was not positive (replaced by FLT_MIN), or -1 (valid on wave 0).
-__device__ int diag_factor_invert(float* D, float* Xs, float* Y, float* tmp) {
+__device__ __forceinline__ int diag_factor_invert(float* D, float* Xs, float* Y, float* tmp) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int bad = -1;
     for (int kb = 0; kb < NB / SB; ++kb) {
@@ -258,17 +261,20 @@ __device__ __forceinline__ void mfma_pqt(const float* P, const float* Q, f4 (&ac
     }
 }
 
-// block column k: factor and invert the diagonal block; store L_kk, L_kk^-1 (workgroup 0)
-// or the panel block L_ik = A_ik L_kk^-T (workgroup i - k)
-__global__ __launch_bounds__(NT) void chol_panel(float* __restrict__ W, float* __restrict__ Xi, int npad, int T, int k,
-                                                 int* __restrict__ info) {
-    __shared__ float D[NB * LS];
-    __shared__ float X[NB * LS];
-    __shared__ float Y[NB * YS];
-    __shared__ float tmp[4 * SB * YS];
-    __shared__ int first_bad;
-    const int b = blockIdx.y, tid = threadIdx.x;
-    float* Wb = W + (size_t)b * npad * npad;
+// LDS of the panel role (D, X, Y, tmp) and of the Schur role (two 64 x 64 operands)
+constexpr int kPanelFloats = 2 * NB * LS + NB * YS + 4 * SB * YS;
+constexpr int kSmemFloats = kPanelFloats > 2 * NB * LS ? kPanelFloats : 2 * NB * LS;
+
+// Block column k, block row k + blk: factor and invert the diagonal block (every panel
+// workgroup, redundantly: no grid barrier); blk 0 stores L_kk and L_kk^-1, the others the
+// panel block L_ik = A_ik L_kk^-T (MFMA).
+__device__ __forceinline__ void panel_body(float* smem, int* first_bad, float* Wb, float* Xi, int b, int npad, int T, int k, int blk,
+                           int* info) {
+    float* D = smem;
+    float* X = smem + NB * LS;
+    float* Y = smem + 2 * NB * LS;
+    float* tmp = Y + NB * YS;
+    const int tid = threadIdx.x;
     const int k0 = k * NB;
     for (int e = tid; e < NB * NB; e += NT) {
         const int i = e >> 6, j = e & 63;
@@ -277,22 +283,21 @@ __global__ __launch_bounds__(NT) void chol_panel(float* __restrict__ W, float* _
     __syncthreads();
     {
         const int bad = diag_factor_invert(D, X, Y, tmp);
-        if (tid == 0) first_bad = bad;
+        if (tid == 0) *first_bad = bad;
     }
     __syncthreads();
-    if (blockIdx.x == 0) {
+    if (blk == 0) {
         float* Xo = Xi + ((size_t)b * T + k) * NB * NB;
         for (int e = tid; e < NB * NB; e += NT) {
             const int i = e >> 6, j = e & 63;
             Wb[(size_t)(k0 + i) * npad + k0 + j] = j <= i ? D[i * LS + j] : 0.0f;
             Xo[e] = j <= i ? X[i * LS + j] : 0.0f;
         }
-        if (tid == 0 && first_bad >= 0) atomicCAS(info + b, 0, k0 + first_bad + 1);
+        if (tid == 0 && *first_bad >= 0) atomicCAS(info + b, 0, k0 + *first_bad + 1);
         return;
     }
     // panel block: D <- A_ik, then L_ik = A_ik X^T
-    const int i0 = (k + blockIdx.x) * NB;
-    __syncthreads();
+    const int i0 = (k + blk) * NB;
     for (int e = tid; e < NB * NB; e += NT) {
         const int r = e >> 6, m = e & 63;
         D[r * LS + m] = Wb[(size_t)(i0 + r) * npad + k0 + m];
@@ -313,18 +318,12 @@ __global__ __launch_bounds__(NT) void chol_panel(float* __restrict__ W, float* _
             }
 }
 
-// Schur-complement update of block column k: A_ij -= L_ik L_jk^T for k < j <= i < T
-__global__ __launch_bounds__(NT) void chol_schur(float* __restrict__ W, int npad, int k) {
-    __shared__ float Li[NB * LS];
-    __shared__ float Lj[NB * LS];
-    const int b = blockIdx.y, tid = threadIdx.x;
-    float* Wb = W + (size_t)b * npad * npad;
-    const int t = blockIdx.x;
-    int ib = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-    while ((ib + 1) * (ib + 2) / 2 <= t) ++ib;
-    while (ib * (ib + 1) / 2 > t) --ib;
-    const int jb = t - ib * (ib + 1) / 2;
-    const int k0 = k * NB, i0 = (k + 1 + ib) * NB, j0 = (k + 1 + jb) * NB;
+// one Schur-complement tile of block column k: A_ij -= L_ik L_jk^T (rows i0, columns j0)
+__device__ __forceinline__ void schur_tile(float* smem, float* Wb, int npad, int k, int i0, int j0) {
+    float* Li = smem;
+    float* Lj = smem + NB * LS;
+    const int tid = threadIdx.x;
+    const int k0 = k * NB;
     for (int e = tid; e < NB * NB; e += NT) {
         const int r = e >> 6, m = e & 63;
         Li[r * LS + m] = Wb[(size_t)(i0 + r) * npad + k0 + m];
@@ -344,6 +343,49 @@ __global__ __launch_bounds__(NT) void chol_schur(float* __restrict__ W, int npad
                 const int cc = (w & 1) * 32 + c * 16 + (lane & 15);
                 Wb[(size_t)(i0 + r) * npad + j0 + cc] -= acc[a][c][e];
             }
+}
+
+// lower-triangle tile t -> (ib, jb), jb <= ib
+__device__ __forceinline__ void tri_index(int t, int* ib, int* jb) {
+    int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while ((i + 1) * (i + 2) / 2 <= t) ++i;
+    while (i * (i + 1) / 2 > t) --i;
+    *ib = i;
+    *jb = t - i * (i + 1) / 2;
+}
+
+// the first panel (block column 0)
+__global__ __launch_bounds__(NT) void chol_panel(float* __restrict__ W, float* __restrict__ Xi, int npad, int T,
+                                                 int* __restrict__ info) {
+    __shared__ float smem[kSmemFloats];
+    __shared__ int first_bad;
+    const int b = blockIdx.y;
+    panel_body(smem, &first_bad, W + (size_t)b * npad * npad, Xi, b, npad, T, 0, blockIdx.x, info);
+}
+
+// update of block column k + 1 by block column k: A_i,k+1 -= L_ik L_k+1,k^T, i > k
+__global__ __launch_bounds__(NT) void chol_schur_next(float* __restrict__ W, int npad, int k) {
+    __shared__ float smem[kSmemFloats];
+    const int b = blockIdx.y;
+    schur_tile(smem, W + (size_t)b * npad * npad, npad, k, (k + 1 + (int)blockIdx.x) * NB, (k + 1) * NB);
+}
+
+// look-ahead: the panel of block column kp (T - kp workgroups) next to the rest of block
+// column kp - 1's Schur update (tiles kp < j <= i), which touches no block of column kp
+__global__ __launch_bounds__(NT) void chol_panel_schur(float* __restrict__ W, float* __restrict__ Xi, int npad, int T,
+                                                       int kp, int* __restrict__ info) {
+    __shared__ float smem[kSmemFloats];
+    __shared__ int first_bad;
+    const int b = blockIdx.y;
+    float* Wb = W + (size_t)b * npad * npad;
+    const int npanel = T - kp;
+    if ((int)blockIdx.x < npanel) {
+        panel_body(smem, &first_bad, Wb, Xi, b, npad, T, kp, blockIdx.x, info);
+        return;
+    }
+    int ib, jb;
+    tri_index((int)blockIdx.x - npanel, &ib, &jb);
+    schur_tile(smem, Wb, npad, kp - 1, (kp + 1 + ib) * NB, (kp + 1 + jb) * NB);
 }
 
 __global__ __launch_bounds__(NT) void chol_clear(int* __restrict__ flags, int count) {
@@ -589,10 +631,14 @@ int32_t sapx_chol_factorize(SapxChol* s, const float* A, int32_t n, int32_t lda,
     hipLaunchKernelGGL(chol_scale, dim3(T, B), dim3(NT), 0, st, A, lda, (long long)a_stride, n, s->dsc, npad);
     hipLaunchKernelGGL(chol_load, dim3(lgrid, B), dim3(NT), 0, st, A, lda, (long long)a_stride, n, s->dsc, s->W, s->A0,
                        npad, s->info);
-    for (int k = 0; k < T; ++k) {
-        hipLaunchKernelGGL(chol_panel, dim3(T - k, B), dim3(NT), 0, st, s->W, s->Xi, npad, T, k, s->info);
-        const int m = T - k - 1;
-        if (m > 0) hipLaunchKernelGGL(chol_schur, dim3(m * (m + 1) / 2, B), dim3(NT), 0, st, s->W, npad, k);
+    // panel 0; then per block column k: update column k + 1, then factor its panel while the
+    // rest of column k's update runs in the same launch
+    hipLaunchKernelGGL(chol_panel, dim3(T, B), dim3(NT), 0, st, s->W, s->Xi, npad, T, s->info);
+    for (int k = 0; k + 1 < T; ++k) {
+        hipLaunchKernelGGL(chol_schur_next, dim3(T - k - 1, B), dim3(NT), 0, st, s->W, npad, k);
+        const int m2 = T - k - 2;
+        hipLaunchKernelGGL(chol_panel_schur, dim3((T - k - 1) + m2 * (m2 + 1) / 2, B), dim3(NT), 0, st, s->W, s->Xi,
+                           npad, T, k + 1, s->info);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return cfail(SAPX_ERR_HIP, std::string("sapx_chol_factorize: ") + hipGetErrorString(e));
