@@ -436,6 +436,13 @@ __global__ __launch_bounds__(NT) void chol_fwd(const float* __restrict__ W, cons
     for (int e = tid; e < NB * NB; e += NT) Xs[(e >> 6) * LS + (e & 63)] = Xb[e];
     const float* Lrow = Wb + (size_t)(i * NB + r) * npad + q * 16;
     float acc = 0.0f;
+    // the L entries of the next block to wait for are loaded before the wait, so a block
+    // published last costs one y load and the arithmetic
+    float4 pre[4];
+    if (i > 0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pre[u] = reinterpret_cast<const float4*>(Lrow)[u];
+    }
     for (int j = 0; j < i;) {
         const int je = ready_run(fb, j, i, 1, timed_out, &s_end);
         for (int e = tid; e < (je - j) * NB; e += NT) vbuf[e] = coherent_load(yb + j * NB + e);
@@ -446,13 +453,17 @@ __global__ __launch_bounds__(NT) void chol_fwd(const float* __restrict__ W, cons
             float sacc = 0.0f;
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const float4 l = L4[u];
+                const float4 l = jj == j ? pre[u] : L4[u];
                 sacc += l.x * v[4 * u] + l.y * v[4 * u + 1] + l.z * v[4 * u + 2] + l.w * v[4 * u + 3];
             }
             acc += sacc;
         }
         __syncthreads();
         j = je;
+        if (j < i) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pre[u] = reinterpret_cast<const float4*>(Lrow + j * NB)[u];
+        }
     }
     part[q][r] = acc;
     __syncthreads();
@@ -489,6 +500,13 @@ __global__ __launch_bounds__(NT) void chol_bwd(const float* __restrict__ W, cons
     float* xb = xs + (size_t)b * npad;
     int* fb = flags + (size_t)b * T;
     float acc = 0.0f;
+    // the L entries of the next block to wait for are loaded before the wait
+    float pre[16];
+    if (i < T - 1) {
+        const float* Lc = Wb + (size_t)((T - 1) * NB + q * 16) * npad + i * NB + c;
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) pre[rr] = Lc[(size_t)rr * npad];
+    }
     for (int j = T - 1; j > i;) {
         const int je = ready_run(fb, j, i, -1, timed_out, &s_end);  // blocks j, j-1, ..., je+1
         const int cnt = j - je;
@@ -497,7 +515,14 @@ __global__ __launch_bounds__(NT) void chol_bwd(const float* __restrict__ W, cons
             vbuf[e] = coherent_load(xb + (j - blk) * NB + (e & 63));
         }
         __syncthreads();
-        for (int t = 0; t < cnt; ++t) {
+        {
+            const float* v = vbuf + q * 16;
+            float sacc = 0.0f;
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr) sacc += pre[rr] * v[rr];
+            acc += sacc;
+        }
+        for (int t = 1; t < cnt; ++t) {
             const float* Lc = Wb + (size_t)((j - t) * NB + q * 16) * npad + i * NB + c;
             const float* v = vbuf + t * NB + q * 16;
             float sacc = 0.0f;
@@ -507,6 +532,11 @@ __global__ __launch_bounds__(NT) void chol_bwd(const float* __restrict__ W, cons
         }
         __syncthreads();
         j = je;
+        if (j > i) {
+            const float* Lc = Wb + (size_t)(j * NB + q * 16) * npad + i * NB + c;
+#pragma unroll
+            for (int rr = 0; rr < 16; ++rr) pre[rr] = Lc[(size_t)rr * npad];
+        }
     }
     part[q][c] = acc;
     __syncthreads();
