@@ -1,7 +1,8 @@
 """Large-world SAP solve (``sapx_large_*``) against the oracle's Newton solve of the same problem.
 
 Worlds: one pile of 32 or 64 free boxes and spheres (192 / 384 dofs) settled onto the ground by
-the C++ restatement of the step (oracle), then one step formed by the float64 oracle: its free
+the C++ restatement of the step (oracle), a lying humanoid (35 dofs, PD drives, contact rows over
+its chain) and a settled quadruped, then one step formed by the float64 oracle: its free
 motion (A, v*), contact set (Jacobian rows, phi0, weights, stiffness, tau, mu), PD / limit terms
 and participation mask are handed to the GPU solver (A and J rounded to float32, J in the
 sparse per-contact layout).  Checks, per line search (exact_root, armijo_decay):
@@ -22,14 +23,22 @@ pytestmark = pytest.mark.gpu
 KMAX = 16
 
 
-def _settled_pile(bodies, seed, steps, ls):
+def _settled_pile(bodies, seed, steps, ls, scene="pile"):
     from oracle import collide as oc
     from oracle import cpu_step as cs
     from oracle import sap_oracle as so
     from sap_warp_amd import hetero
+    from sap_warp_amd.scenes import build_humanoid_scene
 
-    model = hetero.build_pile_scene(1, seed=seed, bodies=bodies)
-    C = 8 * bodies
+    if scene == "pile":
+        model = hetero.build_pile_scene(1, seed=seed, bodies=bodies)
+        C = 8 * bodies
+    elif scene == "humanoid":
+        model = build_humanoid_scene(1, seed=seed, lying_fraction=1.0, contact_gap=0.05)
+        C = 64
+    else:
+        model = hetero.build_quadruped_scene(1, seed=seed)
+        C = 64
     dt = 2.0e-3
     opts = so.StepOptions(preset="approx32", line_search=ls, max_rigid_contact=C,
                           solve=so.SolveOptions.for_variant(ls))
@@ -48,7 +57,7 @@ def _settled_pile(bodies, seed, steps, ls):
     return model, ref["per_world"][0], opts, dt
 
 
-def _gpu_inputs(model, r, dt):
+def _gpu_inputs(model, r, dt, kmax=KMAX):
     import torch
 
     from oracle import sap_oracle as so
@@ -60,7 +69,7 @@ def _gpu_inputs(model, r, dt):
     dev = "cuda:0"
     A32 = fm.A_env.astype(np.float32)
     J32 = cs.J.astype(np.float32)
-    cols, vals = sparse_contact_rows(J32, KMAX)
+    cols, vals = sparse_contact_rows(J32, kmax)
     T = lambda x, dt_: torch.tensor(np.ascontiguousarray(x), dtype=dt_, device=dev)  # noqa: E731
     f64, i32 = torch.float64, torch.int32
     mterms = dict(pd_active=T(mt.pd_active, i32), pd_a=T(mt.pd_a, f64), pd_gain=T(mt.pd_gain, f64),
@@ -112,6 +121,36 @@ def test_pile_solve_matches_oracle(gpu_device, bodies, ls):
         a_diag = np.array([d if d >= 1e-12 else 1e-12 for d in np.diag(A64)])
         gn, tol = so._optimality_norms(ev, part, 1.0 / np.sqrt(a_diag), opts.solve)
         assert gn <= tol * 1.000001, (gn, tol)
+
+
+@pytest.mark.parametrize("scene", ["humanoid", "quadruped"])
+def test_articulated_world_solve_matches_oracle(gpu_device, scene):
+    """Articulated worlds through the large-world solver: contact rows over the chain's dofs (up
+    to 35, kmax 48), PD drives and joint limits as model terms."""
+    from dataclasses import replace
+
+    from oracle import sap_oracle as so
+    from sap_warp_amd.large_world import LargeWorldSolver
+
+    print(f"large-world solve: {scene}", flush=True)
+    model, r, opts, dt = _settled_pile(0, seed=4, steps=0 if scene == "humanoid" else 150, ls="exact_root",
+                                       scene=scene)
+    cs, mt, fm = r["contacts"], r["model_terms"], r["free_motion"]
+    nc = cs.count
+    assert nc >= 4 and np.any(mt.pd_active == 1)
+    args, A64, J64, part = _gpu_inputs(model, r, dt, kmax=48)
+    solver = LargeWorldSolver(model.joint_dof_count, nc, 48)
+    v, gamma, st = solver.solve(line_search="exact_root", **args)
+    v, gamma = v.cpu().numpy(), gamma.cpu().numpy()
+    solver.close()
+    sol = so.sap_solve(model, so.WorldView(model, 0), A64, fm.v_star, r["v0"], None, replace(cs, J=J64), mt, dt,
+                       opts.solve)
+    print(f"{scene}: n {model.joint_dof_count} nc {nc} gpu it {st.iterations} oracle it {sol.iterations} "
+          f"v err {_rel(v, sol.v):.2e} gamma err {_rel(gamma, sol.gamma):.2e}", flush=True)
+    assert st.converged and sol.converged
+    assert _rel(v, sol.v) <= 1e-5
+    assert _rel(gamma, sol.gamma) <= 1e-4
+    assert _rel(v, r["v_sap"]) <= 1e-5
 
 
 def test_unconstrained_world_returns_v_star(gpu_device):
