@@ -552,6 +552,37 @@ void sapx_large_destroy(SapxLarge* large);
 int32_t sapx_large_solve(SapxLarge* large, const SapxLargeProblem* problem, const SapxLargeOpts* opts, double* v_out,
                          double* gamma_out, SapxLargeStats* stats, void* stream);
 
+/* -------------------------------------------------------------------------
+ * One world of free rigid bodies on the whole GPU (a bin of a thousand boxes, spheres
+ * and capsules): the collision of SapCollisionPipeline.collide (sim/collision/pipeline.py)
+ * and the step of SolverSAP.step (sim/solver_sap.py:1388-1494) for a single large world,
+ * every stage spread over the GPU; its Newton solve is sapx_large_solve.  The model
+ * (sapx_model_create) must hold one world whose joints are all free joints rooted at the
+ * world with identity joint frames, one per body, without drives or limits
+ * (SAPX_ERR_UNSUPPORTED otherwise); opts are the solver knobs (line searches exact_root
+ * and armijo_decay).
+ * ------------------------------------------------------------------------- */
+typedef struct SapxBigWorld SapxBigWorld;
+
+int32_t sapx_bigworld_create(const SapxModel* model, const SapxSolverOpts* opts, SapxBigWorld** out);
+void sapx_bigworld_destroy(SapxBigWorld* big);
+/*
+ * Contacts of the model's explicit candidate pairs at body poses body_q (device,
+ * body_count x 7) into the view (capacity >= max_rigid_contact), in candidate order with
+ * the per-world capacity rule of sapx_collide; asynchronous on stream.
+ */
+int32_t sapx_bigworld_collide(SapxBigWorld* big, const float* body_q, const SapxContactsView* contacts,
+                              void* stream);
+/*
+ * One step of the world on those contacts: state_out receives joint_q, joint_qd and (when
+ * set) body_q / body_qd; v_out (dof_count doubles, SAP order) and gamma_out (contacts x 3)
+ * are optional device outputs; stats (host) the Newton record.  Synchronises the stream
+ * (contact count, then once per Newton iteration).
+ */
+int32_t sapx_bigworld_step(SapxBigWorld* big, const SapxStateView* state_in, const SapxStateView* state_out,
+                           const SapxControlView* control, const SapxContactsView* contacts, float dt, double* v_out,
+                           double* gamma_out, SapxLargeStats* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -167,6 +167,12 @@ EXPORTS = {
     "sapx_large_destroy": (None, [_P]),
     "sapx_large_solve": (_I, [_P, ctypes.POINTER(SapxLargeProblem), ctypes.POINTER(SapxLargeOpts), _P, _P,
                               ctypes.POINTER(SapxLargeStats), _P]),
+    "sapx_bigworld_create": (_I, [_P, ctypes.POINTER(SapxSolverOpts), ctypes.POINTER(_P)]),
+    "sapx_bigworld_destroy": (None, [_P]),
+    "sapx_bigworld_collide": (_I, [_P, _P, ctypes.POINTER(SapxContactsView), _P]),
+    "sapx_bigworld_step": (_I, [_P, ctypes.POINTER(SapxStateView), ctypes.POINTER(SapxStateView),
+                                ctypes.POINTER(SapxControlView), ctypes.POINTER(SapxContactsView), ctypes.c_float,
+                                _P, _P, ctypes.POINTER(SapxLargeStats), _P]),
 }
 
 HIST_SIZE = 260  # sapx_get_histogram entries

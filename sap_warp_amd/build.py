@@ -45,6 +45,7 @@ UNITS = {
     "sapx_mlp.hip": [],
     "sapx_chol.hip": [],
     "sapx_large.hip": [],
+    "sapx_bigworld.hip": [],
 }
 
 

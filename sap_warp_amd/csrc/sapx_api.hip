@@ -87,6 +87,9 @@ struct SapxModel {
     }
 };
 
+// device tables of a model, for the single-large-world unit
+const sapx::DevModel& sapx::model_dev(const SapxModel* m) { return m->dm; }
+
 struct SapxSolver {
     const SapxModel* model = nullptr;
     SapxSolverOpts opts{};
@@ -291,9 +294,11 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     if (d->joint_count % W || d->dof_count % W || d->coord_count % W || d->body_count % W)
         return fail(SAPX_ERR_INVALID, "joints/dofs/coords/bodies must divide evenly across worlds");
     const int J = d->joint_count / W, N = d->dof_count / W, NQ = d->coord_count / W, NB = d->body_count / W;
-    if (J > kMaxTemplate || NB > kMaxTemplate)
-        return fail(SAPX_ERR_CAPACITY, "more than 64 joints or bodies per world");
-    if (N > kMaxDofFused) return fail(SAPX_ERR_CAPACITY, "more than 256 dofs per world");
+    // a world above the step kernels' limits (64 joints / bodies, 256 dofs) is stepped alone, over
+    // the whole GPU (sapx_bigworld); sapx_solver_create rejects it
+    const bool large = J > kMaxTemplate || NB > kMaxTemplate || N > kMaxDofFused;
+    if (large && W != 1)
+        return fail(SAPX_ERR_CAPACITY, "more than 64 joints or bodies or 256 dofs per world: one world only");
     // ---- topology template + equal-world validation
     std::vector<int> t_jtype(J), t_jparent(J), t_jchild(J), t_jqs(J + 1), t_jds(J + 1), t_jlin(J, 0);
     for (int w = 0; w < W; ++w) {
@@ -357,7 +362,7 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
     for (int b = 0; b < NB; ++b) {
         int j = body_joint[b];
         while (j >= 0) {
-            anc[b] |= 1ull << j;
+            if (j < 64) anc[b] |= 1ull << j;  // masks of the step kernels (64 joints at most)
             const int p = t_jparent[j];
             j = p >= 0 ? body_joint[p] : -1;
         }
@@ -542,7 +547,7 @@ int32_t sapx_model_create(const SapxModelDesc* d, SapxModel** out) {
         std::vector<unsigned long long> anc_dof(NB, 0ull), dof_anc(N, 0ull);
         for (int b = 0; b < NB; ++b)
             for (int d = 0; d < N && d < 64; ++d)  // read only by the one-wave kernel (n <= 64)
-                if ((anc[b] >> dof_joint[d]) & 1ull) anc_dof[b] |= 1ull << d;
+                if (dof_joint[d] < 64 && ((anc[b] >> dof_joint[d]) & 1ull)) anc_dof[b] |= 1ull << d;
         for (int d = 0; d < N; ++d) dof_anc[d] = anc_dof[t_jchild[dof_joint[d]]];
         add64(TP_BODY_ANC, anc);
         add64(TP_BODY_ANC_DOF, anc_dof);
@@ -645,6 +650,9 @@ int32_t sapx_solver_create(const SapxModel* model, const SapxSolverOpts* opts, S
     const DevModel& dm = model->dm;
     const int C = opts->max_rigid_contact;
     if (C <= 0) return fail(SAPX_ERR_INVALID, "max_rigid_contact must be positive");
+    if (dm.J > kMaxTemplate || dm.NB > kMaxTemplate || dm.N > kMaxDofFused)
+        return fail(SAPX_ERR_CAPACITY, "more than 64 joints or bodies or 256 dofs per world: step the world with "
+                                       "sapx_bigworld");
     if (opts->line_search != SAPX_LS_ARMIJO_DECAY && opts->line_search != SAPX_LS_EXACT_ROOT &&
         opts->line_search != SAPX_LS_MONOTONE_DECAY)
         return fail(SAPX_ERR_UNSUPPORTED, "line search variant not in the native step");

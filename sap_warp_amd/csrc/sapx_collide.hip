@@ -355,6 +355,126 @@ __host__ __device__ inline size_t collide_poly_offset(int ns, int nt, int C) {
 
 // CVX: the model has candidate pairs on the MPR / GJK path (up to five contacts per pair, a
 // 16-slot polygon buffer per lane in LDS)
+// Narrowphase of one candidate pair whose AABBs overlap (g1 != g2): orders the pair by shape
+// type, runs the primitive routine or the convex path, and marks which of its contacts pass the
+// gap test (primitive contacts: d <= gap; a convex pair's: not d > gap).  X / AB hold the
+// shapes' world transforms (8 floats) and AABBs (6 floats) at the local indices l1 / l2; S holds
+// the per-shape tables at the global indices g1 / g2.
+struct ShapeTable {
+    const int* type;
+    const float* scale;
+    const float* margin;
+    const float* gap;
+};
+template <bool CVX, int NC>
+__device__ __forceinline__ void narrow_pair(const ShapeTable& S, const float* X, const float* AB, int l1, int l2, int g1,
+                                            int g2, const cvx::Poly& buf, int* unsupported, int& la, int& lb, int& ga,
+                                            int& gb, float& ra, float& rb, float (&dist)[NC], V3 (&pos)[NC], V3& normal,
+                                            int& ncvx, bool (&valid)[NC], int& nv) {
+    int ta = S.type[g1], tb = S.type[g2];
+    la = l1; lb = l2; ga = g1; gb = g2;
+    if (ta > tb) {
+        int t = la; la = lb; lb = t;
+        t = ga; ga = gb; gb = t;
+        t = ta; ta = tb; tb = t;
+    }
+    const float* sca = S.scale + (size_t)ga * 3;
+    const float* scb = S.scale + (size_t)gb * 3;
+    V3 scale_a = v3(sca[0], sca[1], sca[2]);
+    V3 scale_b = v3(scb[0], scb[1], scb[2]);
+    TF Xa = load_tf(X + la * 8), Xb = load_tf(X + lb * 8);
+    if (ta == kGeoSphere || ta == kGeoCapsule) ra = scale_a.x;
+    if (tb == kGeoSphere || tb == kGeoCapsule) rb = scale_b.x;
+    const float gap_sum = S.gap[g1] + S.gap[g2];
+    const V3 ez = v3(0.0f, 0.0f, 1.0f);
+    if (ta == kGeoPlane && tb == kGeoSphere) {
+        V3 nn = qrot(Xa.q, ez);
+        plane_sphere(nn, Xa.p, Xb.p, scale_b.x, &dist[0], &pos[0]);
+        normal = nn;
+    } else if (ta == kGeoPlane && tb == kGeoBox) {
+        V3 nn = qrot(Xa.q, ez);
+        plane_box(nn, Xa.p, Xb.p, Xb.q, scale_b, gap_sum, dist, pos);
+        normal = nn;
+    } else if (ta == kGeoSphere && tb == kGeoSphere) {
+        sphere_sphere(Xa.p, scale_a.x, Xb.p, scale_b.x, &dist[0], &pos[0], &normal);
+    } else if (ta == kGeoPlane && tb == kGeoCapsule) {
+        V3 nn = qrot(Xa.q, ez);
+        V3 ax = qrot(Xb.q, ez);
+        V3 seg = ax * scale_b.y;
+        plane_sphere(nn, Xa.p, Xb.p + seg, scale_b.x, &dist[0], &pos[0]);
+        plane_sphere(nn, Xa.p, Xb.p - seg, scale_b.x, &dist[1], &pos[1]);
+        normal = nn;
+    } else if (ta == kGeoSphere && tb == kGeoCapsule) {
+        V3 ax = qrot(Xb.q, ez);
+        V3 seg = ax * scale_b.y;
+        V3 pt = closest_segment_point(Xb.p - seg, Xb.p + seg, Xa.p);
+        sphere_sphere(Xa.p, scale_a.x, pt, scale_b.x, &dist[0], &pos[0], &normal);
+    } else if (ta == kGeoCapsule && tb == kGeoCapsule) {
+        V3 a1 = qrot(Xa.q, ez), a2 = qrot(Xb.q, ez);
+        capsule_capsule(Xa.p, a1, scale_a.x, scale_a.y, Xb.p, a2, scale_b.x, scale_b.y, dist, pos, &normal);
+    } else if (ta == kGeoSphere && tb == kGeoBox) {
+        sphere_box(Xa.p, scale_a.x, Xb.p, Xb.q, scale_b, &dist[0], &pos[0], &normal);
+    } else if (ta == kGeoPlane && tb == kGeoEllipsoid) {
+        V3 nn = qrot(Xa.q, ez);
+        plane_ellipsoid(nn, Xa.p, Xb.p, Xb.q, scale_b, &dist[0], &pos[0]);
+        normal = nn;
+    } else if (ta == kGeoPlane && tb == kGeoCylinder) {
+        V3 nn = qrot(Xa.q, ez);
+        plane_cylinder(nn, Xa.p, Xb.p, qrot(Xb.q, ez), scale_b.x, scale_b.y, dist, pos);
+        normal = nn;
+    } else if (ta == kGeoSphere && tb == kGeoCylinder) {
+        sphere_cylinder(Xa.p, scale_a.x, Xb.p, qrot(Xb.q, ez), scale_b.x, scale_b.y, &dist[0], &pos[0],
+                        &normal);
+    } else if (CVX && pair_convex(ta, tb)) {
+        int tta = ta;
+        V3 ssa = scale_a, ppa = Xa.p;
+        bool reach = true;
+        if (ta == kGeoPlane) {
+            // the other shape's bounding sphere (centre and half diagonal of its collision
+            // AABB) must reach the infinite plane's half-space; the plane becomes a box of
+            // half extent 10 (R + gap) whose top face lies on the plane under the shape
+            const float* Bb = AB + lb * 6;
+            const V3 blo = v3(Bb[0], Bb[1], Bb[2]), bhi = v3(Bb[3], Bb[4], Bb[5]);
+            const V3 c = (blo + bhi) * 0.5f;
+            const float R = length((bhi - blo) * 0.5f);
+            const V3 nn = qrot(Xa.q, ez);
+            if (scale_a.x != 0.0f || scale_a.y != 0.0f) {
+                atomicAdd(unsupported, 1);
+                reach = false;
+            } else if (dot(c - Xa.p, nn) > R) {
+                reach = false;
+            } else {
+                const float ext = (R + gap_sum) * 10.0f;
+                const V3 surf = Xb.p - nn * dot(Xb.p - Xa.p, nn);
+                ppa = surf - nn * ext;
+                tta = kGeoBox;
+                ssa = v3(ext, ext, ext);
+            }
+        }
+        ncvx = 0;
+        if (reach) {
+            ncvx = cvx::pair(tta, ssa, ppa, Xa.q, tb, scale_b, Xb.p, Xb.q, gap_sum, S.margin[ga],
+                             S.margin[gb], buf, &ra, &rb, pos, dist, &normal);
+        }
+    } else {
+        atomicAdd(unsupported, 1);
+    }
+    const float ma = S.margin[ga], mbm = S.margin[gb];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        // primitive contacts pass the gap check (d <= gap); a convex pair's contacts go
+        // through the per-contact writer, which drops d > gap
+        if (ncvx >= 0 ? i < ncvx : dist[i] < kMaxVal) {
+            V3 xa, xb, nn;
+            contact_points(pos[i], normal, dist[i], ra, rb, &xa, &xb, &nn);
+            float total = ((ra + rb) + ma) + mbm;
+            float dd = dot(xb - xa, nn) - total;
+            valid[i] = ncvx >= 0 ? !(dd > gap_sum) : dd <= gap_sum;
+            nv += valid[i] ? 1 : 0;
+        }
+    }
+}
+
 template <int NT, bool CVX>
 __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
@@ -416,109 +536,9 @@ __global__ __launch_bounds__(NT) void sapx_collide_kernel(CollideParams P) {
             const float* B = AB + l2 * 6;
             bool ov = A[0] <= B[3] && A[3] >= B[0] && A[1] <= B[4] && A[4] >= B[1] && A[2] <= B[5] && A[5] >= B[2];
             if (ov && g1 != g2) {
-                int ta = M.shape_type[g1], tb = M.shape_type[g2];
-                la = l1; lb = l2; ga = g1; gb = g2;
-                if (ta > tb) {
-                    int t = la; la = lb; lb = t;
-                    t = ga; ga = gb; gb = t;
-                    t = ta; ta = tb; tb = t;
-                }
-                const float* sca = M.shape_scale + (size_t)ga * 3;
-                const float* scb = M.shape_scale + (size_t)gb * 3;
-                V3 scale_a = v3(sca[0], sca[1], sca[2]);
-                V3 scale_b = v3(scb[0], scb[1], scb[2]);
-                TF Xa = load_tf(X + la * 8), Xb = load_tf(X + lb * 8);
-                if (ta == kGeoSphere || ta == kGeoCapsule) ra = scale_a.x;
-                if (tb == kGeoSphere || tb == kGeoCapsule) rb = scale_b.x;
-                const float gap_sum = M.shape_gap[g1] + M.shape_gap[g2];
-                const V3 ez = v3(0.0f, 0.0f, 1.0f);
-                if (ta == kGeoPlane && tb == kGeoSphere) {
-                    V3 nn = qrot(Xa.q, ez);
-                    plane_sphere(nn, Xa.p, Xb.p, scale_b.x, &dist[0], &pos[0]);
-                    normal = nn;
-                } else if (ta == kGeoPlane && tb == kGeoBox) {
-                    V3 nn = qrot(Xa.q, ez);
-                    plane_box(nn, Xa.p, Xb.p, Xb.q, scale_b, gap_sum, dist, pos);
-                    normal = nn;
-                } else if (ta == kGeoSphere && tb == kGeoSphere) {
-                    sphere_sphere(Xa.p, scale_a.x, Xb.p, scale_b.x, &dist[0], &pos[0], &normal);
-                } else if (ta == kGeoPlane && tb == kGeoCapsule) {
-                    V3 nn = qrot(Xa.q, ez);
-                    V3 ax = qrot(Xb.q, ez);
-                    V3 seg = ax * scale_b.y;
-                    plane_sphere(nn, Xa.p, Xb.p + seg, scale_b.x, &dist[0], &pos[0]);
-                    plane_sphere(nn, Xa.p, Xb.p - seg, scale_b.x, &dist[1], &pos[1]);
-                    normal = nn;
-                } else if (ta == kGeoSphere && tb == kGeoCapsule) {
-                    V3 ax = qrot(Xb.q, ez);
-                    V3 seg = ax * scale_b.y;
-                    V3 pt = closest_segment_point(Xb.p - seg, Xb.p + seg, Xa.p);
-                    sphere_sphere(Xa.p, scale_a.x, pt, scale_b.x, &dist[0], &pos[0], &normal);
-                } else if (ta == kGeoCapsule && tb == kGeoCapsule) {
-                    V3 a1 = qrot(Xa.q, ez), a2 = qrot(Xb.q, ez);
-                    capsule_capsule(Xa.p, a1, scale_a.x, scale_a.y, Xb.p, a2, scale_b.x, scale_b.y, dist, pos, &normal);
-                } else if (ta == kGeoSphere && tb == kGeoBox) {
-                    sphere_box(Xa.p, scale_a.x, Xb.p, Xb.q, scale_b, &dist[0], &pos[0], &normal);
-                } else if (ta == kGeoPlane && tb == kGeoEllipsoid) {
-                    V3 nn = qrot(Xa.q, ez);
-                    plane_ellipsoid(nn, Xa.p, Xb.p, Xb.q, scale_b, &dist[0], &pos[0]);
-                    normal = nn;
-                } else if (ta == kGeoPlane && tb == kGeoCylinder) {
-                    V3 nn = qrot(Xa.q, ez);
-                    plane_cylinder(nn, Xa.p, Xb.p, qrot(Xb.q, ez), scale_b.x, scale_b.y, dist, pos);
-                    normal = nn;
-                } else if (ta == kGeoSphere && tb == kGeoCylinder) {
-                    sphere_cylinder(Xa.p, scale_a.x, Xb.p, qrot(Xb.q, ez), scale_b.x, scale_b.y, &dist[0], &pos[0],
-                                    &normal);
-                } else if (CVX && pair_convex(ta, tb)) {
-                    int tta = ta;
-                    V3 ssa = scale_a, ppa = Xa.p;
-                    bool reach = true;
-                    if (ta == kGeoPlane) {
-                        // the other shape's bounding sphere (centre and half diagonal of its collision
-                        // AABB) must reach the infinite plane's half-space; the plane becomes a box of
-                        // half extent 10 (R + gap) whose top face lies on the plane under the shape
-                        const float* Bb = AB + lb * 6;
-                        const V3 blo = v3(Bb[0], Bb[1], Bb[2]), bhi = v3(Bb[3], Bb[4], Bb[5]);
-                        const V3 c = (blo + bhi) * 0.5f;
-                        const float R = length((bhi - blo) * 0.5f);
-                        const V3 nn = qrot(Xa.q, ez);
-                        if (scale_a.x != 0.0f || scale_a.y != 0.0f) {
-                            atomicAdd(P.unsupported, 1);
-                            reach = false;
-                        } else if (dot(c - Xa.p, nn) > R) {
-                            reach = false;
-                        } else {
-                            const float ext = (R + gap_sum) * 10.0f;
-                            const V3 surf = Xb.p - nn * dot(Xb.p - Xa.p, nn);
-                            ppa = surf - nn * ext;
-                            tta = kGeoBox;
-                            ssa = v3(ext, ext, ext);
-                        }
-                    }
-                    ncvx = 0;
-                    if (reach) {
-                        const cvx::Poly buf{poly + tid, NT};
-                        ncvx = cvx::pair(tta, ssa, ppa, Xa.q, tb, scale_b, Xb.p, Xb.q, gap_sum, M.shape_margin[ga],
-                                         M.shape_margin[gb], buf, &ra, &rb, pos, dist, &normal);
-                    }
-                } else {
-                    atomicAdd(P.unsupported, 1);
-                }
-                const float ma = M.shape_margin[ga], mbm = M.shape_margin[gb];
-#pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    // primitive contacts pass the gap check (d <= gap); a convex pair's contacts go
-                    // through the per-contact writer, which drops d > gap
-                    if (ncvx >= 0 ? i < ncvx : dist[i] < kMaxVal) {
-                        V3 xa, xb, nn;
-                        contact_points(pos[i], normal, dist[i], ra, rb, &xa, &xb, &nn);
-                        float total = ((ra + rb) + ma) + mbm;
-                        float dd = dot(xb - xa, nn) - total;
-                        valid[i] = ncvx >= 0 ? !(dd > gap_sum) : dd <= gap_sum;
-                        nv += valid[i] ? 1 : 0;
-                    }
-                }
+                narrow_pair<CVX, NC>(ShapeTable{M.shape_type, M.shape_scale, M.shape_margin, M.shape_gap}, X, AB, l1, l2,
+                                     g1, g2, cvx::Poly{poly + tid, NT}, P.unsupported, la, lb, ga, gb, ra, rb, dist, pos,
+                                     normal, ncvx, valid, nv);
             }
         }
         // block exclusive scan of nv in pair order
@@ -712,6 +732,139 @@ hipError_t launch_collide(const CollideParams& P, const PlaceParams& Q, hipStrea
     }
     const unsigned blocks = (unsigned)((Q.W + kPlaceWorlds - 1) / kPlaceWorlds);
     hipLaunchKernelGGL(sapx_place_kernel, dim3(blocks), dim3(kPlaceThreads), 0, stream, Q);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// One large world (a bin of a thousand bodies): no per-world LDS staging.  Shape transforms
+// and AABBs go to global scratch (one thread per shape); each candidate pair is one thread
+// running the same narrowphase and contact writer as the per-world kernel, staging its
+// contacts in its own slots; after an exclusive scan of the per-pair counts (pair order), the
+// write kernel applies the per-world capacity rule (a primitive pair's batch only if it fits;
+// a convex pair's contacts while their slots are inside) and places the contacts.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void sapx_big_shapes_kernel(BigCollideParams B) {
+    const int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= B.nshapes) {
+        if (s == B.nshapes) B.count[0] = 0;
+        return;
+    }
+    TF xs = load_tf(B.shape_transform + (size_t)s * 7);
+    const int b = B.shape_body[s];
+    if (b >= 0) xs = tf_mul(load_tf(B.body_q + (size_t)b * 7), xs);
+    float* o = B.X + (size_t)s * 8;
+    o[0] = xs.p.x; o[1] = xs.p.y; o[2] = xs.p.z; o[3] = xs.q.x; o[4] = xs.q.y; o[5] = xs.q.z; o[6] = xs.q.w;
+    V3 lo, hi;
+    const float* sc = B.shape_scale + (size_t)s * 3;
+    shape_aabb(B.shape_type[s], xs.p, xs.q, v3(sc[0], sc[1], sc[2]), B.shape_radius[s], B.shape_margin[s] + B.shape_gap[s],
+               &lo, &hi);
+    float* a = B.AB + (size_t)s * 6;
+    a[0] = lo.x; a[1] = lo.y; a[2] = lo.z; a[3] = hi.x; a[4] = hi.y; a[5] = hi.z;
+}
+
+constexpr int kBigPairThreads = 128;
+
+__global__ __launch_bounds__(kBigPairThreads) void sapx_big_pairs_kernel(BigCollideParams B) {
+    __shared__ float2 poly[kBigPairThreads * cvx::kBuf];
+    constexpr int NC = kBigSlots;
+    const int p = blockIdx.x * kBigPairThreads + threadIdx.x;
+    if (p >= B.P) return;
+    const int g1 = B.pairs[2 * p], g2 = B.pairs[2 * p + 1];
+    if (g1 < 0 || g2 < 0 || g1 == g2) {
+        B.cnt[p] = 0;
+        B.cvx[p] = 0;
+        return;
+    }
+    int nv = 0, ncvx = -1;
+    float dist[NC];
+    V3 pos[NC];
+    bool valid[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) { dist[i] = kMaxVal; pos[i] = v3(0, 0, 0); valid[i] = false; }
+    V3 normal = v3(0, 0, 0);
+    float ra = 0.0f, rb = 0.0f;
+    int la = -1, lb = -1, ga = -1, gb = -1;
+    const float* A = B.AB + (size_t)g1 * 6;
+    const float* Bb = B.AB + (size_t)g2 * 6;
+    const bool ov = A[0] <= Bb[3] && A[3] >= Bb[0] && A[1] <= Bb[4] && A[4] >= Bb[1] && A[2] <= Bb[5] && A[5] >= Bb[2];
+    if (ov) {
+        narrow_pair<true, NC>(ShapeTable{B.shape_type, B.shape_scale, B.shape_margin, B.shape_gap}, B.X, B.AB, g1, g2, g1,
+                              g2, cvx::Poly{poly + threadIdx.x, kBigPairThreads}, B.unsupported, la, lb, ga, gb, ra, rb,
+                              dist, pos, normal, ncvx, valid, nv);
+    }
+    B.cnt[p] = nv;
+    B.cvx[p] = ncvx >= 0 ? 1 : 0;
+    if (nv == 0) return;
+    const int bA = B.shape_body[ga], bB = B.shape_body[gb];
+    const float ma = B.shape_margin[ga], mbm = B.shape_margin[gb];
+    int k = 0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        if (!valid[i]) continue;
+        V3 xa, xb, nn;
+        contact_points(pos[i], normal, dist[i], ra, rb, &xa, &xb, &nn);
+        V3 pa = xa, pb = xb;
+        if (bA >= 0) {
+            TF t = load_tf(B.body_q + (size_t)bA * 7);
+            pa = qrot(qconj(t.q), xa - t.p);
+        }
+        if (bB >= 0) {
+            TF t = load_tf(B.body_q + (size_t)bB * 7);
+            pb = qrot(qconj(t.q), xb - t.p);
+        }
+        const size_t slot = (size_t)p * NC + k;
+        B.st_sh[2 * slot] = ga;
+        B.st_sh[2 * slot + 1] = gb;
+        float* f = B.st_f + slot * 11;
+        f[0] = pa.x; f[1] = pa.y; f[2] = pa.z;
+        f[3] = pb.x; f[4] = pb.y; f[5] = pb.z;
+        f[6] = nn.x; f[7] = nn.y; f[8] = nn.z;
+        f[9] = ra + ma;
+        f[10] = rb + mbm;
+        ++k;
+    }
+}
+
+__global__ __launch_bounds__(256) void sapx_big_write_kernel(BigCollideParams B) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= B.P) return;
+    const int nv = B.cnt[p];
+    const int prefix = B.off[p];
+    if (nv <= 0 || !(prefix + nv <= B.C || (B.cvx[p] && prefix < B.C))) return;
+    int idx = prefix;
+    for (int k = 0; k < nv && idx < B.C; ++k, ++idx) {
+        const size_t slot = (size_t)p * kBigSlots + k;
+        const float* f = B.st_f + slot * 11;
+        B.out_shape0[idx] = B.st_sh[2 * slot];
+        B.out_shape1[idx] = B.st_sh[2 * slot + 1];
+        B.out_p0[3 * idx] = f[0]; B.out_p0[3 * idx + 1] = f[1]; B.out_p0[3 * idx + 2] = f[2];
+        B.out_p1[3 * idx] = f[3]; B.out_p1[3 * idx + 1] = f[4]; B.out_p1[3 * idx + 2] = f[5];
+        B.out_n[3 * idx] = f[6]; B.out_n[3 * idx + 1] = f[7]; B.out_n[3 * idx + 2] = f[8];
+        B.out_m0[idx] = f[9];
+        B.out_m1[idx] = f[10];
+        if (B.out_pair) B.out_pair[idx] = p;
+    }
+    atomicMax(B.count, idx);
+}
+
+__global__ void sapx_big_finish_kernel(BigCollideParams B) {
+    if (threadIdx.x == 0) {
+        B.world_start[0] = 0;
+        B.world_start[1] = B.count[0];
+    }
+}
+
+hipError_t launch_big_pairs(const BigCollideParams& B, hipStream_t stream) {
+    hipLaunchKernelGGL(sapx_big_shapes_kernel, dim3((B.nshapes + 1 + 255) / 256), dim3(256), 0, stream, B);
+    if (B.P > 0)
+        hipLaunchKernelGGL(sapx_big_pairs_kernel, dim3((B.P + kBigPairThreads - 1) / kBigPairThreads),
+                           dim3(kBigPairThreads), 0, stream, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_big_write(const BigCollideParams& B, hipStream_t stream) {
+    if (B.P > 0) hipLaunchKernelGGL(sapx_big_write_kernel, dim3((B.P + 255) / 256), dim3(256), 0, stream, B);
+    hipLaunchKernelGGL(sapx_big_finish_kernel, dim3(1), dim3(64), 0, stream, B);
     return hipGetLastError();
 }
 

@@ -3,6 +3,8 @@
 
 #include "sapx_device.h"
 
+struct SapxModel;
+
 namespace sapx {
 
 // LDS layout of the one-wave-per-world step kernel (byte offsets); js / ls are
@@ -154,6 +156,43 @@ struct PlaceParams {
 };
 
 // grouping of an arbitrary flat contact list by world (stable radix sort)
+// One large world's collision (thread per candidate pair): shape tables, the pair list, scratch
+// and the flat contact output (sapx_bigworld.hip drives it)
+constexpr int kBigSlots = 5;  // contact slots staged per candidate pair
+struct BigCollideParams {
+    int nshapes, P, C;
+    const int* shape_type;
+    const int* shape_body;
+    const float* shape_transform;  // x7
+    const float* shape_scale;      // x3
+    const float* shape_radius;
+    const float* shape_margin;
+    const float* shape_gap;
+    const int* pairs;  // P x 2 shape indices, candidate order
+    const float* body_q;
+    float* X;        // nshapes x 8 world transforms
+    float* AB;       // nshapes x 6 AABBs
+    int* cnt;        // P: contacts passing the gap test
+    int* cvx;        // P: 1 for a convex (per-contact capacity) pair
+    int* off;        // P: exclusive prefix of cnt (written by the caller's scan)
+    int* st_sh;      // P x kBigSlots x 2 shape ids
+    float* st_f;     // P x kBigSlots x 11: point0, point1, normal, margin0, margin1
+    int* unsupported;
+    int* out_shape0;
+    int* out_shape1;
+    float* out_p0;
+    float* out_p1;
+    float* out_n;
+    float* out_m0;
+    float* out_m1;
+    int* out_pair;     // candidate index of each contact (may be null)
+    int* world_start;  // 2
+    int* count;        // 1
+};
+const DevModel& model_dev(const ::SapxModel* m);
+hipError_t launch_big_pairs(const BigCollideParams& B, hipStream_t stream);
+hipError_t launch_big_write(const BigCollideParams& B, hipStream_t stream);
+
 struct GroupParams {
     int W, NB, capacity;
     const int* count;

@@ -224,6 +224,60 @@ def build_pile_scene(num_worlds: int = 64, seed: int = 13, *, bodies: int = 8, c
     return _with_consistent_body_q(b.finalize(device))
 
 
+def build_bin_scene(bodies: int = 1024, seed: int = 17, *, columns: int = 6, contact_gap: float = 0.01,
+                    device: str = "cpu"):
+    """One world: ``bodies`` free boxes, spheres and capsules (by slot, sizes U(0.03, 0.15) m,
+    density U(500, 2000)) stacked in columns x columns piles 0.16 m apart inside a bin (the ground
+    plane and four static wall boxes, global shapes that do not collide with each other); each body
+    rests 2 mm above the one below it, so the columns touch from the first step and collapse into
+    a dense pile.  Boxes turn by a small yaw, capsules lie along x.  6 DOF per body."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = SceneBuilder(rigid_gap=contact_gap)
+    b.world_count = 1
+    b.current_world = 0
+    pitch = 0.16
+    top = np.zeros(columns * columns)
+    for k in range(bodies):
+        col = k % (columns * columns)
+        i, j = col % columns, col // columns
+        size = rng.uniform(0.03, 0.15)
+        kind = k % 3
+        cfg = ShapeCfg(density=rng.uniform(500.0, 2000.0), ke=2.0e5, tau=0.005, mu=0.7, gap=contact_gap)
+        if kind == 0:
+            half = (0.5 * size, 0.5 * size * rng.uniform(0.6, 1.0), 0.5 * size)
+            height = size
+            q = quat_from_axis_angle(np.array([0.0, 0.0, 1.0]), rng.uniform(-0.3, 0.3))
+        elif kind == 1:
+            height = size
+            q = np.array([0.0, 0.0, 0.0, 1.0])
+        else:
+            height = 0.5 * size
+            q = quat_from_axis_angle(np.array([0.0, 1.0, 0.0]), 0.5 * math.pi)
+        c = 0.5 * (columns - 1)
+        z = top[col] + 0.002 + 0.5 * height
+        top[col] = z + 0.5 * height
+        p = np.array([(i - c) * pitch, (j - c) * pitch, z]) + np.concatenate([rng.uniform(-0.01, 0.01, 2), [0.0]])
+        pose = tf(p, q)
+        body = b.add_body(pose)
+        if kind == 0:
+            b.add_shape(body, GEO_BOX, tf(), half, cfg)
+        elif kind == 1:
+            b.add_shape(body, GEO_SPHERE, tf(), (0.5 * size, 0.0, 0.0), cfg)
+        else:
+            b.add_shape(body, GEO_CAPSULE, tf(), (0.25 * size, 0.25 * size, 0.0), cfg)
+        j_ = b.add_free_joint(body)
+        b.add_articulation([j_])
+        _set_free_joint_state(b, j_, pose, np.zeros(6))
+    b.current_world = -1
+    wall = ShapeCfg(density=0.0, ke=2.0e5, tau=0.005, mu=0.7, gap=contact_gap, collision_group=-1)
+    b.add_ground_plane(wall)
+    half_h = 0.5 * (float(top.max()) + 0.1)
+    e = 0.5 * columns * pitch + 0.02
+    for x, y, hx, hy in ((e, 0.0, 0.01, e), (-e, 0.0, 0.01, e), (0.0, e, e, 0.01), (0.0, -e, e, 0.01)):
+        b.add_shape(-1, GEO_BOX, tf((x, y, half_h)), (hx, hy, half_h), wall)
+    return _with_consistent_body_q(b.finalize(device))
+
+
 _BUILDERS = {"arm_gripper": build_arm_gripper_scene, "quadruped": build_quadruped_scene,
              "humanoid": build_humanoid_scene, "pile": build_pile_scene}
 

@@ -20,6 +20,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _native as nat
+from .runtime import SapContacts, SapControl, SapState, allocate_contacts
+from .solver_sap import SolverSAP
 
 LINE_SEARCHES = {"armijo_decay": 0, "exact_root": 1}
 
@@ -241,3 +243,111 @@ def block_diagonal(blocks: np.ndarray) -> np.ndarray:
     for b in range(nb):
         A[6 * b:6 * b + 6, 6 * b:6 * b + 6] = blocks[b]
     return A
+
+
+class BigWorldSolver(SolverSAP):
+    """``SolverSAP`` plus ``SapCollisionPipeline.collide`` for ONE world of free rigid bodies
+    spread over the whole GPU (the native ``sapx_bigworld_*`` entry points): a bin of a thousand
+    boxes, spheres and capsules.
+
+    Same keyword arguments and presets as :class:`SolverSAP` (``solver_sap.py:749-968``); the
+    model must hold one world whose joints are all free joints rooted at the world with identity
+    joint frames, one per body, without drives or limits, and the line search must be
+    ``exact_root`` or ``armijo_decay``.  :meth:`collide` runs the model's explicit candidate pairs
+    (``pipeline.py``: AABB test, narrowphase, per-world capacity rule), :meth:`step` one
+    ``SolverSAP.step`` (``solver_sap.py:1388-1494``) with the Newton solve of
+    :class:`LargeWorldSolver`; the solve always starts from the free-motion velocity v0.
+    """
+
+    def _create_native(self, opts):
+        if int(self.model.world_count) != 1:
+            raise NotImplementedError("BigWorldSolver steps a model of exactly one world")
+        h = ctypes.c_void_p()
+        with nat.on_device(self._device):
+            nat.check(self._lib.sapx_bigworld_create(self._native_model.handle, ctypes.byref(opts),
+                                                     ctypes.byref(h)), "sapx_bigworld_create")
+        self._stats = None
+        return h
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None):
+            self._lib.sapx_bigworld_destroy(self._handle)
+            self._handle = None
+
+    def reset_runtime_state(self, state: SapState | None = None) -> None:
+        self.sim_time = 0.0
+        self.frame_id = 0
+        self.last_contact_count = 0
+        self.last_solve_iterations = 0
+        self.last_line_search_iterations = 0
+        self.last_converged = True
+        self.last_contacts = None
+        self.last_contact_solve_result = None
+
+    def allocate_contacts(self) -> SapContacts:
+        return allocate_contacts(self.max_rigid_contact, 1, self._device)
+
+    def collide(self, state: SapState, contacts: SapContacts) -> SapContacts:
+        """Contacts of the model's explicit pairs at ``state.body_q`` (asynchronous)."""
+        nat.check_contacts(contacts, self._device, 1, need_world_start=True)
+        if nat.contact_capacity(contacts) < self.max_rigid_contact:
+            raise ValueError("contact buffer smaller than max_rigid_contact")
+        self._check_state(state, "state")
+        cview = nat.contacts_view(contacts)
+        with nat.on_device(self._device):
+            nat.check(self._lib.sapx_bigworld_collide(self._handle, nat._ptr(state.body_q), ctypes.byref(cview),
+                                                      nat.stream_ptr()), "sapx_bigworld_collide")
+        return contacts
+
+    def step(self, state_in: SapState, state_out: SapState, control: SapControl | None,
+             contacts: SapContacts | None, dt, v_out=None, gamma_out=None) -> SapState:
+        """One step; ``v_out`` (dofs, float64) and ``gamma_out`` (max_rigid_contact x 3, float64)
+        optionally receive the SAP velocity and the contact impulses."""
+        import torch
+
+        if not isinstance(state_in, SapState) or not isinstance(state_out, SapState):
+            raise TypeError("BigWorldSolver.step requires SapState input and output.")
+        if isinstance(dt, torch.Tensor) or not float(dt) > 0.0:
+            raise ValueError("dt must be a positive scalar")
+        if contacts is None:
+            contacts = allocate_contacts(1, 1, self._device)
+        self._check_state(state_in, "state_in")
+        self._check_state(state_out, "state_out")
+        if control is not None:
+            self._check_control(control)
+        nat.check_contacts(contacts, self._device, 1)
+        for t, shape, name in ((v_out, (self.dof_count,), "v_out"),
+                               (gamma_out, (self.max_rigid_contact, 3), "gamma_out")):
+            if t is not None and (t.dtype != torch.float64 or tuple(t.shape) != shape or not t.is_contiguous()
+                                  or t.device != torch.device(self._device)):
+                raise ValueError(f"{name} must be a contiguous float64 tensor of shape {shape} on {self._device}")
+        sin = nat.SapxStateView(nat._ptr(state_in.joint_q), nat._ptr(state_in.joint_qd),
+                                nat._ptr(state_in.body_q), nat._ptr(state_in.body_qd), nat._ptr(state_in.body_f))
+        sout = nat.SapxStateView(nat._ptr(state_out.joint_q), nat._ptr(state_out.joint_qd),
+                                 nat._ptr(state_out.body_q), nat._ptr(state_out.body_qd),
+                                 nat._ptr(state_out.body_f))
+        ctl = nat.SapxControlView()
+        if control is not None:
+            ctl = nat.SapxControlView(nat._ptr(control.joint_f), nat._ptr(control.joint_target_pos),
+                                      nat._ptr(control.joint_target_vel), nat._ptr(control.joint_act))
+        cview = nat.contacts_view(contacts)
+        st = nat.SapxLargeStats()
+        with nat.on_device(self._device):
+            rc = self._lib.sapx_bigworld_step(self._handle, ctypes.byref(sin), ctypes.byref(sout),
+                                              ctypes.byref(ctl), ctypes.byref(cview), float(dt), nat._ptr(v_out),
+                                              nat._ptr(gamma_out), ctypes.byref(st), nat.stream_ptr())
+        nat.check(rc, "sapx_bigworld_step")
+        self._stats = LargeSolveStats(int(st.iterations), int(st.ls_iterations), bool(st.converged),
+                                      bool(st.optimality_reached), int(st.status), float(st.cost),
+                                      float(st.grad_norm), float(st.opt_tol))
+        self.last_solve_iterations = self._stats.iterations
+        self.last_line_search_iterations = self._stats.ls_iterations
+        self.last_converged = self._stats.converged
+        self.last_contacts = contacts
+        self.sim_time += float(dt)
+        self.frame_id += 1
+        return state_out
+
+    def solve_stats(self) -> LargeSolveStats | None:
+        """The Newton record of the last step."""
+        return self._stats

@@ -283,11 +283,7 @@ class SolverSAP:
                                      else nat.INTEGRATE_MIDPOINT)
         opts.capture_snapshots = int(self.capture_contact_jacobian_snapshots)
         self._opts = opts
-        h = ctypes.c_void_p()
-        with nat.on_device(self._device):
-            nat.check(self._lib.sapx_solver_create(self._native_model.handle, ctypes.byref(opts), ctypes.byref(h)),
-                      "sapx_solver_create")
-        self._handle = h
+        self._handle = self._create_native(opts)
         self.dof_count = int(model.joint_dof_count)
         self.num_envs = int(model.world_count)
         self.dof_per_env = self.dof_count // max(self.num_envs, 1)
@@ -298,6 +294,13 @@ class SolverSAP:
         self.last_contacts = None
         self.last_contact_solve_result = None
         self.reset_runtime_state()
+
+    def _create_native(self, opts):
+        h = ctypes.c_void_p()
+        with nat.on_device(self._device):
+            nat.check(self._lib.sapx_solver_create(self._native_model.handle, ctypes.byref(opts), ctypes.byref(h)),
+                      "sapx_solver_create")
+        return h
 
     # ------------------------------------------------------------------
     @property
