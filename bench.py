@@ -422,6 +422,59 @@ def hetero_run(worlds: int, steps: int, warmup: int, seed: int, dt: float, line_
     }
 
 
+def bin_world_run(bodies: int, warmup: int, steps: int, line_search: str) -> dict:
+    """One world of free bodies stepped over the whole GPU (BigWorldSolver): a bin of ``bodies``
+    boxes, spheres and capsules stacked in 6 x 6 collapsing columns (hetero.build_bin_scene,
+    synthetic, dt 1e-3), each step = collide (all explicit pairs, thread per pair) + free motion +
+    contact rows + Newton solve + integration; wall clock around ``steps`` steps after ``warmup``
+    untimed ones (the Newton loop is host-driven, one sync per iteration)."""
+    import time
+
+    import torch
+
+    from sap_warp_amd import hetero
+    from sap_warp_amd.large_world import BigWorldSolver
+
+    model = hetero.build_bin_scene(bodies, seed=17, device="cuda:0")
+    C = 16 * bodies
+    solver = BigWorldSolver(model, max_rigid_contact=C, line_search_variant=line_search, contact_tau_d=0.005)
+    contacts = solver.allocate_contacts()
+    s0, s1 = model.state(), model.state()
+    ctl = model.control()
+
+    def one():
+        nonlocal s0, s1
+        solver.collide(s0, contacts)
+        solver.step(s0, s1, ctl, contacts, 1.0e-3)
+        s0, s1 = s1, s0
+        return solver.solve_stats()
+
+    for _ in range(warmup):
+        one()
+    torch.cuda.synchronize()
+    its, ncs = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = one()
+        its.append(st.iterations)
+        ncs.append(int(contacts.rigid_contact_count.item()))
+    torch.cuda.synchronize()
+    dt_s = time.perf_counter() - t0
+    solver.close()
+    return {
+        "workload": f"one world: {bodies} free boxes / spheres / capsules in a bin (collapsing columns), "
+                    f"{6 * bodies} dofs, {line_search}, dt 1e-3, synthetic (hetero.build_bin_scene)",
+        "world_steps_per_s": round(steps / dt_s, 3),
+        "ms_per_step": round(1e3 * dt_s / steps, 3),
+        "steps": steps,
+        "warmup": warmup,
+        "contacts_mean": round(sum(ncs) / len(ncs), 1),
+        "newton_iterations_mean": round(sum(its) / len(its), 2),
+        "timing": "wall clock around the steps (collide + step, the host reads the contact count and each "
+                  "Newton iteration's record)",
+    }
+
+
 def large_world_run(nx: int, ny: int, nz: int, repeats: int, line_search: str) -> dict:
     """One large world on the whole GPU: the SAP Newton solve of a resting block of nx*ny*nz free
     cubes (sap_warp_amd.large_world.stacked_boxes_problem, synthetic; 16x8x8 = 6144 dofs and
@@ -529,6 +582,8 @@ def main() -> int:
     ap.add_argument("--large-world", type=int, default=1,
                     help="also time the single large world (1024 cubes, 6144 dofs) solve (1) or not (0)")
     ap.add_argument("--large-world-repeats", type=int, default=3)
+    ap.add_argument("--bin-bodies", type=int, default=1024, help="bodies of the bin world (0: skip it)")
+    ap.add_argument("--bin-steps", type=int, default=20)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three plain 512 MiB device copies (scales rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -642,6 +697,9 @@ def main() -> int:
         out["large_world"] = large_world_run(16, 8, 8, args.large_world_repeats, args.line_search)
         _log(f"large world done ({out['large_world']['solve_ms']:.1f} ms/solve, "
              f"factor {out['large_world']['factor_ms']:.2f} ms)")
+        if args.bin_bodies > 0:
+            out["large_world"]["bin_step"] = bin_world_run(args.bin_bodies, 5, args.bin_steps, args.line_search)
+            _log(f"bin world done ({out['large_world']['bin_step']['ms_per_step']:.1f} ms/step)")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_worlds, args.cpu_steps, args.seed, dt, args.line_search,
                                            args.cpu_threads)

@@ -236,6 +236,66 @@ __global__ __launch_bounds__(NT) void chol_load(const float* __restrict__ A, int
     if (blockIdx.x == 0 && threadIdx.x == 0) info[b] = 0;
 }
 
+// Block structure: bit (i, j) of the mask (row i: words of kMaskWords bits) says that block
+// (i, j), j <= i, of the work matrix may be nonzero.  chol_mask marks the loaded blocks with a
+// nonzero entry (one workgroup per lower block), chol_symbolic adds the fill of the block
+// elimination (block (i, j) fills when blocks (i, k) and (j, k) are nonzero for some k < j).  A
+// block outside the mask is exactly zero throughout the factorization, so the kernels skip its
+// panel product and every Schur update that would subtract a product with it (each such update
+// subtracts exact zeros); a dense matrix has every bit set and runs every tile.
+constexpr int kMaskWords = (kMaxResident + 31) / 32;
+
+__device__ __forceinline__ bool mask_bit(const unsigned* m, int i, int j) {
+    return (m[i * kMaskWords + (j >> 5)] >> (j & 31)) & 1u;
+}
+
+__global__ __launch_bounds__(NT) void chol_mask(const float* __restrict__ W, int npad, int T,
+                                                unsigned* __restrict__ mask) {
+    const int b = blockIdx.y;
+    int ib, jb;
+    {
+        const int t = (int)blockIdx.x;
+        int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+        while ((i + 1) * (i + 2) / 2 <= t) ++i;
+        while (i * (i + 1) / 2 > t) --i;
+        ib = i;
+        jb = t - i * (i + 1) / 2;
+    }
+    const float* Wb = W + (size_t)b * npad * npad;
+    bool nz = false;
+    for (int e = threadIdx.x; e < NB * NB; e += NT) {
+        const int r = e >> 6, c = e & 63;
+        nz = nz || Wb[(size_t)(ib * NB + r) * npad + jb * NB + c] != 0.0f;
+    }
+    nz = __syncthreads_or(nz);
+    unsigned* m = mask + (size_t)b * T * kMaskWords;
+    if (threadIdx.x == 0) {
+        const unsigned bit = 1u << (jb & 31);
+        if (nz || ib == jb) atomicOr(m + ib * kMaskWords + (jb >> 5), bit);
+    }
+}
+
+__global__ __launch_bounds__(1024) void chol_symbolic(int T, unsigned* __restrict__ mask) {
+    __shared__ unsigned sm[kMaxResident * kMaskWords];
+    unsigned* m = mask + (size_t)blockIdx.x * T * kMaskWords;
+    for (int e = threadIdx.x; e < T * kMaskWords; e += 1024) sm[e] = m[e];
+    __syncthreads();
+    for (int k = 0; k + 1 < T; ++k) {
+        const int r = T - k - 1;  // rows and columns after k
+        for (int t = threadIdx.x; t < r * (r + 1) / 2; t += 1024) {
+            int i = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+            while ((i + 1) * (i + 2) / 2 <= t) ++i;
+            while (i * (i + 1) / 2 > t) --i;
+            const int j = t - i * (i + 1) / 2;
+            const int ig = k + 1 + i, jg = k + 1 + j;
+            if (((sm[ig * kMaskWords + (k >> 5)] >> (k & 31)) & 1u) && ((sm[jg * kMaskWords + (k >> 5)] >> (k & 31)) & 1u))
+                atomicOr(&sm[ig * kMaskWords + (jg >> 5)], 1u << (jg & 31));
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < T * kMaskWords; e += 1024) m[e] = sm[e];
+}
+
 // 64 x 64 x 64 product on four waves: out = P Q^T with P, Q in LDS (row stride LS); wave w
 // owns the 32 x 32 quadrant (w >> 1, w & 1) as 2 x 2 MFMA tiles
 __device__ __forceinline__ void mfma_pqt(const float* P, const float* Q, f4 (&acc)[2][2]) {
@@ -356,36 +416,47 @@ __device__ __forceinline__ void tri_index(int t, int* ib, int* jb) {
 
 // the first panel (block column 0)
 __global__ __launch_bounds__(NT) void chol_panel(float* __restrict__ W, float* __restrict__ Xi, int npad, int T,
-                                                 int* __restrict__ info) {
+                                                 int* __restrict__ info, const unsigned* __restrict__ mask) {
     __shared__ float smem[kSmemFloats];
     __shared__ int first_bad;
     const int b = blockIdx.y;
+    if (blockIdx.x > 0 && !mask_bit(mask + (size_t)b * T * kMaskWords, blockIdx.x, 0)) return;
     panel_body(smem, &first_bad, W + (size_t)b * npad * npad, Xi, b, npad, T, 0, blockIdx.x, info);
 }
 
 // update of block column k + 1 by block column k: A_i,k+1 -= L_ik L_k+1,k^T, i > k
-__global__ __launch_bounds__(NT) void chol_schur_next(float* __restrict__ W, int npad, int k) {
+__global__ __launch_bounds__(NT) void chol_schur_next(float* __restrict__ W, int npad, int k, int T,
+                                                      const unsigned* __restrict__ mask) {
     __shared__ float smem[kSmemFloats];
     const int b = blockIdx.y;
-    schur_tile(smem, W + (size_t)b * npad * npad, npad, k, (k + 1 + (int)blockIdx.x) * NB, (k + 1) * NB);
+    const unsigned* m = mask + (size_t)b * T * kMaskWords;
+    const int ib = k + 1 + (int)blockIdx.x;
+    if (!mask_bit(m, ib, k) || !mask_bit(m, k + 1, k)) return;
+    schur_tile(smem, W + (size_t)b * npad * npad, npad, k, ib * NB, (k + 1) * NB);
 }
 
 // look-ahead: the panel of block column kp (T - kp workgroups) next to the rest of block
 // column kp - 1's Schur update (tiles kp < j <= i), which touches no block of column kp
 __global__ __launch_bounds__(NT) void chol_panel_schur(float* __restrict__ W, float* __restrict__ Xi, int npad, int T,
-                                                       int kp, int* __restrict__ info) {
+                                                       int kp, int* __restrict__ info,
+                                                       const unsigned* __restrict__ mask) {
     __shared__ float smem[kSmemFloats];
     __shared__ int first_bad;
     const int b = blockIdx.y;
     float* Wb = W + (size_t)b * npad * npad;
+    const unsigned* m = mask + (size_t)b * T * kMaskWords;
     const int npanel = T - kp;
     if ((int)blockIdx.x < npanel) {
+        if (blockIdx.x > 0 && !mask_bit(m, kp + blockIdx.x, kp)) return;
         panel_body(smem, &first_bad, Wb, Xi, b, npad, T, kp, blockIdx.x, info);
         return;
     }
     int ib, jb;
     tri_index((int)blockIdx.x - npanel, &ib, &jb);
-    schur_tile(smem, Wb, npad, kp - 1, (kp + 1 + ib) * NB, (kp + 1 + jb) * NB);
+    ib += kp + 1;
+    jb += kp + 1;
+    if (!mask_bit(m, ib, kp - 1) || !mask_bit(m, jb, kp - 1)) return;
+    schur_tile(smem, Wb, npad, kp - 1, ib * NB, jb * NB);
 }
 
 __global__ __launch_bounds__(NT) void chol_clear(int* __restrict__ flags, int count) {
@@ -590,6 +661,7 @@ struct SapxChol {
     float* dsc = nullptr;
     double* r = nullptr;
     int* flags = nullptr;  // 2 x batch x T: forward, backward
+    unsigned* mask = nullptr;  // batch x T x kMaskWords: block structure of the factor
     int* info = nullptr;
     int* timed_out = nullptr;
 };
@@ -619,6 +691,7 @@ int32_t sapx_chol_create(int32_t max_num_equations, int32_t batch_size, SapxChol
               hipMalloc(&s->dsc, (size_t)batch_size * npad * sizeof(float)) == hipSuccess &&
               hipMalloc(&s->r, (size_t)batch_size * npad * sizeof(double)) == hipSuccess &&
               hipMalloc(&s->flags, (size_t)2 * batch_size * T * sizeof(int)) == hipSuccess &&
+              hipMalloc(&s->mask, (size_t)batch_size * T * kMaskWords * sizeof(unsigned)) == hipSuccess &&
               hipMalloc(&s->info, (size_t)batch_size * sizeof(int)) == hipSuccess &&
               hipMalloc(&s->timed_out, sizeof(int)) == hipSuccess &&
               hipMemset(s->info, 0, (size_t)batch_size * sizeof(int)) == hipSuccess &&
@@ -641,6 +714,7 @@ void sapx_chol_destroy(SapxChol* s) {
     (void)hipFree(s->dsc);
     (void)hipFree(s->r);
     (void)hipFree(s->flags);
+    (void)hipFree(s->mask);
     (void)hipFree(s->info);
     (void)hipFree(s->timed_out);
     delete s;
@@ -661,14 +735,20 @@ int32_t sapx_chol_factorize(SapxChol* s, const float* A, int32_t n, int32_t lda,
     hipLaunchKernelGGL(chol_scale, dim3(T, B), dim3(NT), 0, st, A, lda, (long long)a_stride, n, s->dsc, npad);
     hipLaunchKernelGGL(chol_load, dim3(lgrid, B), dim3(NT), 0, st, A, lda, (long long)a_stride, n, s->dsc, s->W, s->A0,
                        npad, s->info);
+    // block structure of the factor (cleared by a kernel: the sequence is graph-capturable)
+    const int mwords = B * T * kMaskWords;
+    hipLaunchKernelGGL(chol_clear, dim3((mwords + NT - 1) / NT), dim3(NT), 0, st, reinterpret_cast<int*>(s->mask),
+                       mwords);
+    hipLaunchKernelGGL(chol_mask, dim3(T * (T + 1) / 2, B), dim3(NT), 0, st, s->W, npad, T, s->mask);
+    hipLaunchKernelGGL(chol_symbolic, dim3(B), dim3(1024), 0, st, T, s->mask);
     // panel 0; then per block column k: update column k + 1, then factor its panel while the
     // rest of column k's update runs in the same launch
-    hipLaunchKernelGGL(chol_panel, dim3(T, B), dim3(NT), 0, st, s->W, s->Xi, npad, T, s->info);
+    hipLaunchKernelGGL(chol_panel, dim3(T, B), dim3(NT), 0, st, s->W, s->Xi, npad, T, s->info, s->mask);
     for (int k = 0; k + 1 < T; ++k) {
-        hipLaunchKernelGGL(chol_schur_next, dim3(T - k - 1, B), dim3(NT), 0, st, s->W, npad, k);
+        hipLaunchKernelGGL(chol_schur_next, dim3(T - k - 1, B), dim3(NT), 0, st, s->W, npad, k, T, s->mask);
         const int m2 = T - k - 2;
         hipLaunchKernelGGL(chol_panel_schur, dim3((T - k - 1) + m2 * (m2 + 1) / 2, B), dim3(NT), 0, st, s->W, s->Xi,
-                           npad, T, k + 1, s->info);
+                           npad, T, k + 1, s->info, s->mask);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return cfail(SAPX_ERR_HIP, std::string("sapx_chol_factorize: ") + hipGetErrorString(e));

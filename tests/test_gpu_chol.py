@@ -11,6 +11,8 @@ matrix, and a matrix that is not positive definite must be reported.  Matrices:
   contact Hessian blocks, 10-3000 x the pair's effective mass, the range of SAP's
   regularised friction and normal compliances), up to 1024 bodies plus a 40-dof articulation
   (6184 dofs, a 1024-body bin plus a humanoid);
+* block-diagonal, block-banded and arrowhead SAP matrices (the factor skips the blocks outside
+  the block structure of the factor);
 * an indefinite matrix (info names the failing column);
 * graph capture: factorize + solve recorded once and replayed on new matrix contents.
 
@@ -124,6 +126,42 @@ def test_sap_newton_matrix(gpu_device, bodies, contacts, extra):
     ref = np.linalg.solve(_f32(H), b)
     x, info = _solve([H], [b], refine=3)
     print("n", H.shape[0], "rel err", _rel(x[0], ref))
+    assert info[0] == 0
+    assert _rel(x[0], ref) <= 1e-8
+
+
+def _banded_sap_matrix(bodies, reach, rng, arrow=False):
+    """A SAP matrix whose contacts couple bodies at most ``reach`` apart in index (block-banded
+    factor, the structure of a pile numbered by position); ``arrow`` adds contacts of body 0 with
+    every body (an arrowhead: the whole lower triangle fills)."""
+    H = _sap_matrix(bodies, 0, rng)
+    n = H.shape[0]
+    pairs = [(a, b) for a in range(bodies) for b in range(a + 1, min(bodies, a + reach + 1)) if rng.random() < 0.5]
+    if arrow:
+        pairs += [(0, b) for b in range(1, bodies)]
+    for a, b in pairs:
+        Jc = np.zeros((3, 12))  # the contact's columns: body a's six dofs, then body b's
+        for k, (body, sgn) in enumerate(((a, -1.0), (b, 1.0))):
+            r = rng.normal(scale=0.05, size=3)
+            rx = np.array([[0, -r[2], r[1]], [r[2], 0, -r[0]], [-r[1], r[0], 0]])
+            Jc[:, 6 * k:6 * k + 3] = -sgn * rx
+            Jc[:, 6 * k + 3:6 * k + 6] = sgn * np.eye(3)
+        idx = np.r_[6 * a:6 * a + 6, 6 * b:6 * b + 6]
+        H[np.ix_(idx, idx)] += Jc.T @ (_spd(3, rng, cond=1e2) * 10.0 ** rng.uniform(1.0, 3.0)) @ Jc
+    return 0.5 * (H + H.T)
+
+
+@pytest.mark.parametrize("bodies,reach,arrow", [(200, 0, False), (300, 12, False), (1024, 40, False),
+                                                (150, 3, True)])
+def test_block_sparse_matrix(gpu_device, bodies, reach, arrow):
+    """Block-diagonal, block-banded and arrowhead structures: the factor skips the tiles outside
+    the block structure (and its fill), and the solve is that of the whole matrix."""
+    rng = np.random.default_rng(bodies + reach)
+    H = _banded_sap_matrix(bodies, reach, rng, arrow)
+    b = rng.normal(size=H.shape[0])
+    ref = np.linalg.solve(_f32(H), b)
+    x, info = _solve([H], [b], refine=3)
+    print("n", H.shape[0], "reach", reach, "arrow", arrow, "rel err", _rel(x[0], ref))
     assert info[0] == 0
     assert _rel(x[0], ref) <= 1e-8
 
