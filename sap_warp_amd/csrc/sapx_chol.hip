@@ -249,6 +249,19 @@ __device__ __forceinline__ bool mask_bit(const unsigned* m, int i, int j) {
     return (m[i * kMaskWords + (j >> 5)] >> (j & 31)) & 1u;
 }
 
+// first block column >= from (and < limit) of mask row `row` with its bit set; limit if none
+__device__ __forceinline__ int mask_next(const unsigned* row, int from, int limit) {
+    for (int w = from >> 5; w * 32 < limit; ++w) {
+        unsigned bits = row[w];
+        if (w == (from >> 5)) bits &= ~0u << (from & 31);
+        if (bits) {
+            const int j = w * 32 + __builtin_ctz(bits);
+            return j < limit ? j : limit;
+        }
+    }
+    return limit;
+}
+
 __global__ __launch_bounds__(NT) void chol_mask(const float* __restrict__ W, int npad, int T,
                                                 unsigned* __restrict__ mask) {
     const int b = blockIdx.y;
@@ -493,7 +506,7 @@ __device__ __forceinline__ int ready_run(const int* fb, int j, int limit, int di
 __global__ __launch_bounds__(NT) void chol_fwd(const float* __restrict__ W, const float* __restrict__ Xi,
                                                const float* __restrict__ dsc, const double* __restrict__ rhs, int n,
                                                float* __restrict__ y, int* __restrict__ flags, int npad, int T,
-                                               int* __restrict__ timed_out) {
+                                               int* __restrict__ timed_out, const unsigned* __restrict__ mask) {
     __shared__ float Xs[NB * LS];
     __shared__ float part[4][NB];
     __shared__ float vbuf[kChunk * NB];
@@ -509,12 +522,15 @@ __global__ __launch_bounds__(NT) void chol_fwd(const float* __restrict__ W, cons
     float acc = 0.0f;
     // the L entries of the next block to wait for are loaded before the wait, so a block
     // published last costs one y load and the arithmetic
+    // only the blocks L_ij of the block structure are waited for (the others are zero)
+    const unsigned* mrow = mask + ((size_t)b * T + i) * kMaskWords;
+    const int j0 = mask_next(mrow, 0, i);
     float4 pre[4];
-    if (i > 0) {
+    if (j0 < i) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pre[u] = reinterpret_cast<const float4*>(Lrow)[u];
+        for (int u = 0; u < 4; ++u) pre[u] = reinterpret_cast<const float4*>(Lrow + j0 * NB)[u];
     }
-    for (int j = 0; j < i;) {
+    for (int j = j0; j < i;) {
         const int je = ready_run(fb, j, i, 1, timed_out, &s_end);
         for (int e = tid; e < (je - j) * NB; e += NT) vbuf[e] = coherent_load(yb + j * NB + e);
         __syncthreads();
@@ -530,7 +546,7 @@ __global__ __launch_bounds__(NT) void chol_fwd(const float* __restrict__ W, cons
             acc += sacc;
         }
         __syncthreads();
-        j = je;
+        j = mask_next(mrow, je, i);
         if (j < i) {
 #pragma unroll
             for (int u = 0; u < 4; ++u) pre[u] = reinterpret_cast<const float4*>(Lrow + j * NB)[u];
@@ -560,7 +576,8 @@ __global__ __launch_bounds__(NT) void chol_fwd(const float* __restrict__ W, cons
 __global__ __launch_bounds__(NT) void chol_bwd(const float* __restrict__ W, const float* __restrict__ Xi,
                                                const float* __restrict__ dsc, const float* __restrict__ y,
                                                float* __restrict__ xs, double* __restrict__ x, int n, int accumulate,
-                                               int* __restrict__ flags, int npad, int T, int* __restrict__ timed_out) {
+                                               int* __restrict__ flags, int npad, int T, int* __restrict__ timed_out,
+                                               const unsigned* __restrict__ mask) {
     __shared__ float part[4][NB];
     __shared__ float vbuf[kChunk * NB];
     __shared__ int s_end;
@@ -572,13 +589,21 @@ __global__ __launch_bounds__(NT) void chol_bwd(const float* __restrict__ W, cons
     int* fb = flags + (size_t)b * T;
     float acc = 0.0f;
     // the L entries of the next block to wait for are loaded before the wait
+    // only the blocks L_ji of the block structure are waited for (the others are zero)
+    const unsigned* mb = mask + (size_t)b * T * kMaskWords;
+    auto prev = [&](int from) {
+        for (int jj = from; jj > i; --jj)
+            if (mask_bit(mb, jj, i)) return jj;
+        return i;
+    };
+    const int j0 = prev(T - 1);
     float pre[16];
-    if (i < T - 1) {
-        const float* Lc = Wb + (size_t)((T - 1) * NB + q * 16) * npad + i * NB + c;
+    if (j0 > i) {
+        const float* Lc = Wb + (size_t)(j0 * NB + q * 16) * npad + i * NB + c;
 #pragma unroll
         for (int rr = 0; rr < 16; ++rr) pre[rr] = Lc[(size_t)rr * npad];
     }
-    for (int j = T - 1; j > i;) {
+    for (int j = j0; j > i;) {
         const int je = ready_run(fb, j, i, -1, timed_out, &s_end);  // blocks j, j-1, ..., je+1
         const int cnt = j - je;
         for (int e = tid; e < cnt * NB; e += NT) {
@@ -602,7 +627,7 @@ __global__ __launch_bounds__(NT) void chol_bwd(const float* __restrict__ W, cons
             acc += sacc;
         }
         __syncthreads();
-        j = je;
+        j = prev(je);
         if (j > i) {
             const float* Lc = Wb + (size_t)(j * NB + q * 16) * npad + i * NB + c;
 #pragma unroll
@@ -772,9 +797,9 @@ int32_t sapx_chol_solve(SapxChol* s, const double* rhs, double* result, int32_t 
         }
         hipLaunchKernelGGL(chol_clear, dim3((nflags + NT - 1) / NT), dim3(NT), 0, st, s->flags, nflags);
         hipLaunchKernelGGL(chol_fwd, dim3(T, B), dim3(NT), 0, st, s->W, s->Xi, s->dsc, b, n, s->y, ff, npad, T,
-                           s->timed_out);
+                           s->timed_out, s->mask);
         hipLaunchKernelGGL(chol_bwd, dim3(T, B), dim3(NT), 0, st, s->W, s->Xi, s->dsc, s->y, s->xs, result, n,
-                           pass > 0 ? 1 : 0, fbk, npad, T, s->timed_out);
+                           pass > 0 ? 1 : 0, fbk, npad, T, s->timed_out, s->mask);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return cfail(SAPX_ERR_HIP, std::string("sapx_chol_solve: ") + hipGetErrorString(e));
