@@ -566,6 +566,9 @@ typedef struct SapxBigWorld SapxBigWorld;
 
 int32_t sapx_bigworld_create(const SapxModel* model, const SapxSolverOpts* opts, SapxBigWorld** out);
 void sapx_bigworld_destroy(SapxBigWorld* big);
+/* Forget the warm start: the next step's solve starts from v0 (SolverSAP.reset_runtime_state,
+ * sim/solver_sap.py:1004-1017); after each step the solve starts from the last solution. */
+void sapx_bigworld_reset(SapxBigWorld* big);
 /*
  * Contacts of the model's explicit candidate pairs at body poses body_q (device,
  * body_count x 7) into the view (capacity >= max_rigid_contact), in candidate order with

@@ -169,6 +169,7 @@ EXPORTS = {
                               ctypes.POINTER(SapxLargeStats), _P]),
     "sapx_bigworld_create": (_I, [_P, ctypes.POINTER(SapxSolverOpts), ctypes.POINTER(_P)]),
     "sapx_bigworld_destroy": (None, [_P]),
+    "sapx_bigworld_reset": (None, [_P]),
     "sapx_bigworld_collide": (_I, [_P, _P, ctypes.POINTER(SapxContactsView), _P]),
     "sapx_bigworld_step": (_I, [_P, ctypes.POINTER(SapxStateView), ctypes.POINTER(SapxStateView),
                                 ctypes.POINTER(SapxControlView), ctypes.POINTER(SapxContactsView), ctypes.c_float,

@@ -16,7 +16,8 @@
 //   contacts      one thread per contact: witnesses, frame, signed distance, materials, the
 //                 twelve Jacobian columns of the two bodies, weights and the participating dofs
 //                 (contact_jacobian.py:404-562, sap_helpers.py:148-338, contact_solve.py:422-487)
-//   solve         sapx_large_solve (sapx_large.hip)
+//   solve         sapx_large_solve (sapx_large.hip), started from the previous step's solution
+//                 after the first step (warm start; sapx_bigworld_reset clears it)
 //   integration   one thread per free joint (solver_sap.py:288-519); the public body pose and
 //                 velocity of a root free body are its coordinates (sap_kinematics.py:84-229)
 //
@@ -468,6 +469,7 @@ struct SapxBigWorld {
     double *phi0 = nullptr, *weight = nullptr, *stiff = nullptr, *tau = nullptr, *mu = nullptr;
     int* hcount = nullptr;  // pinned
     SapxLarge* large = nullptr;
+    int has_guess = 0;      // the previous step's v (in v) starts the next solve
 };
 
 extern "C" {
@@ -602,6 +604,10 @@ int32_t sapx_bigworld_create(const SapxModel* model, const SapxSolverOpts* opts,
     return SAPX_OK;
 }
 
+void sapx_bigworld_reset(SapxBigWorld* b) {
+    if (b) b->has_guess = 0;
+}
+
 int32_t sapx_bigworld_collide(SapxBigWorld* b, const float* body_q, const SapxContactsView* c, void* stream) {
     if (!b || !body_q || !c) return bfail(SAPX_ERR_INVALID, "sapx_bigworld_collide: null argument");
     if (!c->count || !c->world_start || !c->shape0 || !c->shape1 || !c->point0 || !c->point1 || !c->normal ||
@@ -685,7 +691,9 @@ int32_t sapx_bigworld_step(SapxBigWorld* b, const SapxStateView* in, const SapxS
     p.kmax = KB;
     p.A = b->A;
     p.v_star = b->vstar;
-    p.v_guess = b->v0;
+    // warm start: the previous step's solution once there is one (solver_sap.py:1004-1017,
+    // 1388-1494); l_init reads it before the solve writes the new v into the same buffer
+    p.v_guess = b->has_guess ? b->v : b->v0;
     p.participating = b->part;
     p.J_cols = b->cols;
     p.J_vals = b->vals;
@@ -711,7 +719,11 @@ int32_t sapx_bigworld_step(SapxBigWorld* b, const SapxStateView* in, const SapxS
     lo.sigma = o.contact_sigma;
     lo.diag_shift = o.diag_shift;
     const int32_t rc = sapx_large_solve(b->large, &p, &lo, b->v, gamma_out, stats, stream);
-    if (rc != SAPX_OK) return rc;
+    if (rc != SAPX_OK) {
+        b->has_guess = 0;
+        return rc;
+    }
+    b->has_guess = 1;
     IntArgs ia{b->bm, in->joint_q, b->v0, b->v, (double)dt, o.position_integration, out->joint_q, out->joint_qd,
                out->body_q, out->body_qd};
     hipLaunchKernelGGL(bw_integrate, dim3((b->J + NT - 1) / NT), dim3(NT), 0, st, ia);

@@ -256,7 +256,8 @@ class BigWorldSolver(SolverSAP):
     ``exact_root`` or ``armijo_decay``.  :meth:`collide` runs the model's explicit candidate pairs
     (``pipeline.py``: AABB test, narrowphase, per-world capacity rule), :meth:`step` one
     ``SolverSAP.step`` (``solver_sap.py:1388-1494``) with the Newton solve of
-    :class:`LargeWorldSolver`; the solve always starts from the free-motion velocity v0.
+    :class:`LargeWorldSolver`, started from the previous step's solution after the first step
+    (the warm start of ``SolverSAP``; :meth:`reset_runtime_state` clears it).
     """
 
     def _create_native(self, opts):
@@ -275,6 +276,9 @@ class BigWorldSolver(SolverSAP):
             self._handle = None
 
     def reset_runtime_state(self, state: SapState | None = None) -> None:
+        """Clears the counters and the warm start (solver_sap.py:1004-1017)."""
+        if getattr(self, "_handle", None):
+            self._lib.sapx_bigworld_reset(self._handle)
         self.sim_time = 0.0
         self.frame_id = 0
         self.last_contact_count = 0

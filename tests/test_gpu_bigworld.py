@@ -8,8 +8,9 @@ contacts), then:
 * collide: the contact list of the model's explicit pairs is bit-identical to the oracle's
   (``oracle/narrowphase.c``: count, shapes, body-frame points, normals, margins), including a
   capacity that cuts the list (the per-world overflow rule);
-* step: one step from the same state and contacts against the oracle's float64 step
-  (``sap_oracle.step``): the SAP velocity, v_guess-free, within 1e-5 relative (max-norm), the
+* step: two steps (the second warm-started from the first's solution), each from the same
+  state, contacts and start as the oracle's float64 step (``sap_oracle.step``): the SAP
+  velocity within 1e-5 relative (max-norm), the
   public joint_q / joint_qd / body_q / body_qd within 1e-5, contact impulses within 1e-4 (the
   redundant pile contacts make gamma ill-conditioned; v is not), for both line searches and the
   drake preset (Delassus weights, contact midpoint, sap_euler);
@@ -86,6 +87,9 @@ def test_bigworld_collide_bit_exact(gpu_device, C):
 @pytest.mark.parametrize("preset,ls", [("approx32", "exact_root"), ("approx32", "armijo_decay"),
                                        ("drake", "exact_root")])
 def test_bigworld_step_matches_oracle(gpu_device, preset, ls):
+    """Two steps: the first solve starts from v0, the second from the first step's solution (the
+    solver's warm start), each compared with the oracle step from the same input state, contacts
+    and start."""
     import torch
 
     from oracle import sap_oracle as so
@@ -98,31 +102,37 @@ def test_bigworld_step_matches_oracle(gpu_device, preset, ls):
                             contact_tau_d=0.005)
     contacts = solver.allocate_contacts()
     s0, s1 = _gpu_state(model, st), model.state()
-    solver.collide(s0, contacts)
     n = model.joint_dof_count
     v = torch.zeros(n, dtype=torch.float64, device="cuda:0")
     gamma = torch.zeros((C, 3), dtype=torch.float64, device="cuda:0")
-    solver.step(s0, s1, model.control(), contacts, dt, v_out=v, gamma_out=gamma)
-    torch.cuda.synchronize()
-    nc = int(_np(contacts.rigid_contact_count)[0])
-    c_np = dict(count=nc, world_start=_np(contacts.rigid_contact_world_start))
-    for k in ("shape0", "shape1", "point0", "point1", "normal", "margin0", "margin1"):
-        c_np[k] = _np(getattr(contacts, "rigid_contact_" + k))
-    ref = so.step(model, st, ctl, c_np, dt, opts)
-    r = ref["per_world"][0]
-    stt = solver.solve_stats()
-    e = dict(v=world_rel_err(_np(v), r["v_sap"]), q=world_rel_err(_np(s1.joint_q), ref["joint_q"]),
-             qd=world_rel_err(_np(s1.joint_qd), ref["joint_qd"]), bq=world_rel_err(_np(s1.body_q), ref["body_q"]),
-             bqd=world_rel_err(_np(s1.body_qd), ref["body_qd"]),
-             gamma=world_rel_err(_np(gamma)[:nc], r["result"].gamma))
-    print(f"nc {nc} gpu it {stt.iterations} conv {stt.converged} | oracle it {r['result'].iterations} | "
-          + " ".join(f"{k} {x:.2e}" for k, x in e.items()), flush=True)
-    assert nc >= 60
-    assert r["contacts"].count == nc
-    assert stt.converged and r["result"].converged
-    for k in ("v", "q", "qd", "bq", "bqd"):
-        assert e[k] <= 1.0e-5, (k, e[k])
-    assert e["gamma"] <= 1.0e-4
+    guess = None
+    for k in range(2):
+        st_in = dict(joint_q=_np(s0.joint_q), joint_qd=_np(s0.joint_qd), body_q=_np(s0.body_q),
+                     body_qd=_np(s0.body_qd), body_f=np.zeros((model.body_count, 6), np.float32))
+        solver.collide(s0, contacts)
+        solver.step(s0, s1, model.control(), contacts, dt, v_out=v, gamma_out=gamma)
+        torch.cuda.synchronize()
+        nc = int(_np(contacts.rigid_contact_count)[0])
+        c_np = dict(count=nc, world_start=_np(contacts.rigid_contact_world_start))
+        for key in ("shape0", "shape1", "point0", "point1", "normal", "margin0", "margin1"):
+            c_np[key] = _np(getattr(contacts, "rigid_contact_" + key))
+        ref = so.step(model, st_in, ctl, c_np, dt, opts, v_guess=guess)
+        r = ref["per_world"][0]
+        stt = solver.solve_stats()
+        e = dict(v=world_rel_err(_np(v), r["v_sap"]), q=world_rel_err(_np(s1.joint_q), ref["joint_q"]),
+                 qd=world_rel_err(_np(s1.joint_qd), ref["joint_qd"]), bq=world_rel_err(_np(s1.body_q), ref["body_q"]),
+                 bqd=world_rel_err(_np(s1.body_qd), ref["body_qd"]),
+                 gamma=world_rel_err(_np(gamma)[:nc], r["result"].gamma))
+        print(f"step {k} nc {nc} gpu it {stt.iterations} conv {stt.converged} | oracle it {r['result'].iterations} | "
+              + " ".join(f"{name} {x:.2e}" for name, x in e.items()), flush=True)
+        assert nc >= 60
+        assert r["contacts"].count == nc
+        assert stt.converged and r["result"].converged
+        for name in ("v", "q", "qd", "bq", "bqd"):
+            assert e[name] <= 1.0e-5, (k, name, e[name])
+        assert e["gamma"] <= 1.0e-4
+        guess = _np(v).astype(np.float64)
+        s0, s1 = s1, s0
     solver.close()
 
 
