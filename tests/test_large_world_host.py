@@ -103,3 +103,28 @@ def test_native_create_rejects_bad_sizes_before_gpu_work():
         nat.check(lib.sapx_large_create(0, 10, 16, ctypes.byref(h)), "sapx_large_create")
     with pytest.raises(ValueError):
         nat.check(lib.sapx_large_create(100, 10, 300, ctypes.byref(h)), "sapx_large_create")
+
+
+def test_bin_scene_structure():
+    """The bin world: one world of free bodies (one free joint and articulation each), static
+    walls and ground as global shapes that never pair with each other, bodies stacked 2 mm apart
+    so the oracle's collision finds contacts at once, and every contact inside one world."""
+    from oracle import collide as oc
+    from sap_warp_amd import hetero
+
+    m = hetero.build_bin_scene(40, seed=3, columns=3)
+    assert m.world_count == 1 and m.body_count == 40 and m.joint_dof_count == 240
+    assert np.all(np.asarray(m.joint_type) == 4)  # free joints
+    assert np.all(np.asarray(m.joint_parent) == -1)
+    sw = np.asarray(m.shape_world)
+    sb = np.asarray(m.shape_body)
+    assert np.sum(sw == -1) == 5 and np.all(sb[sw == -1] == -1)  # ground plane + four walls
+    pairs = np.asarray(m.shape_pairs).reshape(-1, 2)
+    assert not np.any((sb[pairs[:, 0]] < 0) & (sb[pairs[:, 1]] < 0))
+    # every body pair and every body-static pair is a candidate
+    assert len(pairs) == 40 * 39 // 2 + 40 * 5
+    c = oc.collide(m, m.body_q, 400)
+    assert c["count"] > 40 and c["unsupported"] == 0
+    b0 = sb[c["shape0"][:c["count"]]]
+    b1 = sb[c["shape1"][:c["count"]]]
+    assert np.all((b0 >= 0) | (b1 >= 0))
