@@ -14,6 +14,7 @@ contacts), then:
   public joint_q / joint_qd / body_q / body_qd within 1e-5, contact impulses within 1e-4 (the
   redundant pile contacts make gamma ill-conditioned; v is not), for both line searches and the
   drake preset (Delassus weights, contact midpoint, sap_euler);
+* the full 1024-body bin of the bench: contacts bit-identical at the start and after 10 GPU steps;
 * models outside the path (articulated joints) are rejected.
 """
 import numpy as np
@@ -81,6 +82,39 @@ def test_bigworld_collide_bit_exact(gpu_device, C):
                        ("normal", "rigid_contact_normal"), ("margin0", "rigid_contact_margin0"),
                        ("margin1", "rigid_contact_margin1")):
         np.testing.assert_array_equal(_np(getattr(contacts, field))[:n], ref[key][:n], err_msg=key)
+    solver.close()
+
+
+def test_bigworld_collide_full_bin(gpu_device):
+    """The bench's 1024-body bin (528 896 candidate pairs): contacts bit-identical to the oracle's
+    at the initial stacking and after 10 GPU steps of the collapsing columns."""
+    import torch
+
+    from oracle import collide as oc
+    from sap_warp_amd import hetero
+    from sap_warp_amd.large_world import BigWorldSolver
+
+    print("bigworld collide full bin", flush=True)
+    model = hetero.build_bin_scene(1024, seed=17, device="cuda:0")
+    C = 16384
+    solver = BigWorldSolver(model, max_rigid_contact=C, line_search_variant="exact_root", contact_tau_d=0.005)
+    contacts = solver.allocate_contacts()
+    s0, s1 = model.state(), model.state()
+    ctl = model.control()
+    for k in range(11):
+        solver.collide(s0, contacts)
+        torch.cuda.synchronize()
+        if k in (0, 10):
+            ref = oc.collide(model, _np(s0.body_q), C, threads=8)
+            n = int(_np(contacts.rigid_contact_count)[0])
+            print(f"step {k}: contacts gpu {n} oracle {ref['count']}", flush=True)
+            assert n == ref["count"] and n > 1000
+            for key in ("shape0", "shape1", "point0", "point1", "normal", "margin0", "margin1"):
+                np.testing.assert_array_equal(_np(getattr(contacts, "rigid_contact_" + key))[:n], ref[key][:n],
+                                              err_msg=f"step {k} {key}")
+        if k < 10:
+            solver.step(s0, s1, ctl, contacts, 1.0e-3)
+            s0, s1 = s1, s0
     solver.close()
 
 
