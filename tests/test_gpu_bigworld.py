@@ -15,6 +15,7 @@ contacts), then:
   redundant pile contacts make gamma ill-conditioned; v is not), for both line searches and the
   drake preset (Delassus weights, contact midpoint, sap_euler);
 * the full 1024-body bin of the bench: contacts bit-identical at the start and after 10 GPU steps;
+* a contact list longer than max_rigid_contact: the step keeps the first C contacts, as the oracle;
 * models outside the path (articulated joints) are rejected.
 """
 import numpy as np
@@ -168,6 +169,44 @@ def test_bigworld_step_matches_oracle(gpu_device, preset, ls):
         guess = _np(v).astype(np.float64)
         s0, s1 = s1, s0
     solver.close()
+
+
+def test_bigworld_step_capacity_cut(gpu_device):
+    """A contact list longer than the solver's max_rigid_contact: the step keeps the first C
+    contacts (contact_jacobian.py:470-472), as the oracle step with the same capacity does."""
+    import torch
+
+    from oracle import sap_oracle as so
+    from sap_warp_amd.large_world import BigWorldSolver
+
+    print("bigworld step capacity cut", flush=True)
+    model, st, ctl, dt, _ = _bin(60, seed=11, steps=80, C=4000)
+    big = BigWorldSolver(model, max_rigid_contact=4000, line_search_variant="exact_root", contact_tau_d=0.005)
+    contacts = big.allocate_contacts()
+    s0, s1 = _gpu_state(model, st), model.state()
+    big.collide(s0, contacts)
+    torch.cuda.synchronize()
+    nc = int(_np(contacts.rigid_contact_count)[0])
+    C = nc - 25
+    assert C > 40
+    solver = BigWorldSolver(model, max_rigid_contact=C, line_search_variant="exact_root", contact_tau_d=0.005)
+    v = torch.zeros(model.joint_dof_count, dtype=torch.float64, device="cuda:0")
+    solver.step(s0, s1, model.control(), contacts, dt, v_out=v)
+    torch.cuda.synchronize()
+    c_np = dict(count=nc, world_start=_np(contacts.rigid_contact_world_start))
+    for key in ("shape0", "shape1", "point0", "point1", "normal", "margin0", "margin1"):
+        c_np[key] = _np(getattr(contacts, "rigid_contact_" + key))
+    opts = so.StepOptions(preset="approx32", line_search="exact_root", max_rigid_contact=C,
+                          solve=so.SolveOptions.for_variant("exact_root"))
+    ref = so.step(model, st, ctl, c_np, dt, opts)
+    r = ref["per_world"][0]
+    e = dict(v=world_rel_err(_np(v), r["v_sap"]), q=world_rel_err(_np(s1.joint_q), ref["joint_q"]))
+    print(f"nc {nc} C {C} oracle slots {r['contacts'].count} | v {e['v']:.2e} q {e['q']:.2e}", flush=True)
+    assert r["contacts"].count == C
+    assert solver.solve_stats().converged and r["result"].converged
+    assert e["v"] <= 1.0e-5 and e["q"] <= 1.0e-5
+    solver.close()
+    big.close()
 
 
 def test_bigworld_rejects_articulated_model(gpu_device):
